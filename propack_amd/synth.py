@@ -1,0 +1,166 @@
+"""Seeded synthetic sparse matrices for the truncated-SVD workloads.
+
+All generators use numpy's counter-based Philox bit generator, so a seed
+fixes the matrix bit-for-bit on any host; the CPU runs and the device runs
+are fed the very same CSR arrays.  Matrices are returned as ``Csr`` (int32
+row pointers and column indices, values in the requested dtype, columns
+sorted within each row), together with their explicit transposes when asked.
+
+Workloads (sizes as named by the benchmark configurations):
+
+* ``random_uniform`` -- m x n with ``nnz_draws`` uniform (row, col) draws,
+  duplicates summed, N(0,1) values (1850 x 712 with 8636 draws is the small
+  correctness case).
+* ``poisson_decay`` -- each row holds Poisson(lam) distinct uniform columns,
+  value N(0,1) * (1 + j)^-p for column j (2,000,000 x 1,000,000, lam = 16,
+  p = 0.6 is the large FP64 case).
+* ``powerlaw_bipartite`` -- row degrees P(d) ~ d^-gamma on [1, dmax], columns
+  from a Zipf(s) law over a seeded permutation, values 1.0.
+"""
+from __future__ import annotations
+
+from dataclasses import dataclass
+
+import numpy as np
+
+
+@dataclass
+class Csr:
+    m: int
+    n: int
+    rowptr: np.ndarray  # int32, length m + 1
+    col: np.ndarray     # int32, length nnz
+    val: np.ndarray     # float64 or float32, length nnz
+
+    @property
+    def nnz(self) -> int:
+        return int(self.rowptr[-1])
+
+    def astype(self, dtype) -> "Csr":
+        return Csr(self.m, self.n, self.rowptr, self.col,
+                   np.ascontiguousarray(self.val, dtype=dtype))
+
+    def transpose(self) -> "Csr":
+        """Explicit CSR of A^T (stable counting sort by column)."""
+        rows = np.repeat(np.arange(self.m, dtype=np.int32),
+                         np.diff(self.rowptr).astype(np.int64))
+        order = np.argsort(self.col, kind="stable")
+        counts = np.bincount(self.col, minlength=self.n)
+        rp = np.zeros(self.n + 1, dtype=np.int64)
+        np.cumsum(counts, out=rp[1:])
+        return Csr(self.n, self.m, rp.astype(np.int32),
+                   rows[order].astype(np.int32), self.val[order].copy())
+
+    def todense(self) -> np.ndarray:
+        a = np.zeros((self.m, self.n), dtype=np.float64)
+        rows = np.repeat(np.arange(self.m), np.diff(self.rowptr))
+        np.add.at(a, (rows, self.col), self.val.astype(np.float64))
+        return a
+
+    def matvec(self, x: np.ndarray) -> np.ndarray:
+        """y = A x in float64 (host helper for checks, not a product path)."""
+        rows = np.repeat(np.arange(self.m), np.diff(self.rowptr))
+        y = np.zeros(self.m, dtype=np.float64)
+        np.add.at(y, rows, self.val.astype(np.float64) * x[self.col])
+        return y
+
+
+def _rng(seed: int) -> np.random.Generator:
+    return np.random.Generator(np.random.Philox(seed))
+
+
+def from_coo(m: int, n: int, rows: np.ndarray, cols: np.ndarray,
+             vals: np.ndarray, sum_duplicates: bool = True) -> Csr:
+    """COO -> CSR with columns sorted inside each row, duplicates summed."""
+    rows = np.asarray(rows, dtype=np.int64)
+    cols = np.asarray(cols, dtype=np.int64)
+    vals = np.asarray(vals)
+    key = rows * n + cols
+    order = np.argsort(key, kind="stable")
+    key = key[order]
+    vals = vals[order]
+    if sum_duplicates and key.size:
+        first = np.ones(key.size, dtype=bool)
+        first[1:] = key[1:] != key[:-1]
+        seg = np.cumsum(first) - 1
+        summed = np.zeros(int(first.sum()), dtype=np.float64)
+        np.add.at(summed, seg, vals.astype(np.float64))
+        key = key[first]
+        vals = summed.astype(vals.dtype)
+    r = (key // n).astype(np.int64)
+    c = (key % n).astype(np.int32)
+    counts = np.bincount(r, minlength=m)
+    rp = np.zeros(m + 1, dtype=np.int64)
+    np.cumsum(counts, out=rp[1:])
+    if rp[-1] >= 2**31:
+        raise ValueError("nnz exceeds int32 row pointers")
+    return Csr(m, n, rp.astype(np.int32), c, np.ascontiguousarray(vals))
+
+
+def random_uniform(m: int = 1850, n: int = 712, nnz_draws: int = 8636,
+                   seed: int = 1850, dtype=np.float64) -> Csr:
+    g = _rng(seed)
+    rows = g.integers(0, m, nnz_draws)
+    cols = g.integers(0, n, nnz_draws)
+    vals = g.standard_normal(nnz_draws)
+    return from_coo(m, n, rows, cols, vals).astype(dtype)
+
+
+def poisson_decay(m: int = 2_000_000, n: int = 1_000_000, lam: float = 16.0,
+                  p: float = 0.6, seed: int = 2024, dtype=np.float64) -> Csr:
+    g = _rng(seed)
+    counts = np.minimum(g.poisson(lam, m), n).astype(np.int64)
+    rp = np.zeros(m + 1, dtype=np.int64)
+    np.cumsum(counts, out=rp[1:])
+    nnz = int(rp[-1])
+    if nnz >= 2**31:
+        raise ValueError("nnz exceeds int32 row pointers")
+    rows = np.repeat(np.arange(m, dtype=np.int64), counts)
+    cols = g.integers(0, n, nnz)
+    # Distinct columns per row: redraw the later copy of each duplicate.
+    for _ in range(64):
+        key = rows * n + cols
+        order = np.argsort(key, kind="stable")
+        ks = key[order]
+        dup = np.zeros(nnz, dtype=bool)
+        dup[1:] = ks[1:] == ks[:-1]
+        if not dup.any():
+            break
+        idx = order[dup]
+        cols[idx] = g.integers(0, n, idx.size)
+    else:
+        raise RuntimeError("could not make columns distinct")
+    key = rows * n + cols
+    order = np.argsort(key, kind="stable")
+    cols = cols[order].astype(np.int32)
+    vals = g.standard_normal(nnz) * np.power(1.0 + cols.astype(np.float64), -p)
+    return Csr(m, n, rp.astype(np.int32), cols, vals.astype(dtype))
+
+
+def powerlaw_bipartite(m: int = 8_000_000, n: int = 4_000_000,
+                       gamma: float = 2.1, dmax: int = 100_000,
+                       zipf_s: float = 1.8, seed: int = 8000,
+                       dtype=np.float32) -> Csr:
+    g = _rng(seed)
+    # Row degrees by inverse-CDF sampling of the discrete law d^-gamma.
+    d = np.arange(1, dmax + 1, dtype=np.float64)
+    cdf = np.cumsum(d ** -gamma)
+    cdf /= cdf[-1]
+    deg = np.searchsorted(cdf, g.random(m), side="right") + 1
+    deg = np.minimum(deg, n).astype(np.int64)
+    # Column ids: Zipf(s) rank over a seeded permutation of the columns.
+    perm = g.permutation(n).astype(np.int64)
+    rcdf = np.cumsum(np.arange(1, n + 1, dtype=np.float64) ** -zipf_s)
+    rcdf /= rcdf[-1]
+    nnz = int(deg.sum())
+    rows = np.repeat(np.arange(m, dtype=np.int64), deg)
+    ranks = np.minimum(np.searchsorted(rcdf, g.random(nnz), side="right"), n - 1)
+    cols = perm[ranks]
+    vals = np.ones(nnz, dtype=np.float64)
+    # Duplicate (row, col) pairs merge into one entry (value = multiplicity).
+    return from_coo(m, n, rows, cols, vals).astype(dtype)
+
+
+def start_vector(m: int, seed: int, dtype=np.float64) -> np.ndarray:
+    """Seeded N(0,1) start vector of length m."""
+    return _rng(seed).standard_normal(m).astype(dtype)
