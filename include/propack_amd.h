@@ -1,0 +1,168 @@
+/*
+ * libpropack_amd -- C ABI of the MI355X (gfx950) truncated-SVD engine.
+ *
+ * Two layers sit behind this header:
+ *
+ *  1. The drop-in drivers DLANSVD / DLANSVD_IRL / SLANSVD / SLANSVD_IRL with
+ *     the reference's Fortran calling sequences (declared in propack.h).
+ *     They are host Fortran and call the engine below through iso_c_binding.
+ *  2. The engine: a device-resident CSR operator that replaces the user
+ *     APROD callback, and the Lanczos bidiagonalization step (the
+ *     reference's DLANBPRO with its reorthogonalization and level-1 work)
+ *     running on the GPU, plus the basis update GEMM of restarts and Ritz
+ *     vectors.
+ *
+ * Conventions: plain C types only; vector/matrix arguments are column-major;
+ * basis column indices are 1-based as in the reference; every function
+ * returns PK_OK (0) or a negative PK_ERR_* code and never exits the process
+ * (the reference STOPs on workspace errors, double/dgemm_ovwr.F:17-18,68).
+ * Handles are not thread-safe; use one engine per host thread.
+ */
+#ifndef PROPACK_AMD_H
+#define PROPACK_AMD_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+  PK_OK = 0,
+  PK_ERR_ARG = -101,    /* invalid argument */
+  PK_ERR_HIP = -102,    /* HIP runtime error (launch, copy, allocation) */
+  PK_ERR_OOM = -103,    /* device memory exhausted */
+  PK_ERR_HANDLE = -104, /* not a live handle */
+  PK_ERR_NOTSUP = -105  /* combination not supported */
+};
+enum { PK_F64 = 0, PK_F32 = 1 };   /* element precision of arrays */
+enum { PK_HOST = 0, PK_DEVICE = 1 }; /* where caller arrays live */
+
+typedef struct pk_csr_opaque* pk_csr_t;
+typedef struct pk_lbd_opaque* pk_lbd_t;
+
+/* Last error message of the calling process (static storage). */
+const char* pk_last_error(void);
+
+/* ---------------------------------------------------------------------- */
+/* Device CSR operator.  Replaces the user-supplied APROD of
+ * double/dlansvd.F:20-33 and double/dlanbpro.F:28-41 (Y = A*X for 'n',
+ * Y = A^T*X for 't').  rowptr (m+1) and colind (nnz) are 0-based int32;
+ * values are double (PK_F64) or float (PK_F32).  A^T is built on the
+ * device at creation.                                                     */
+int pk_csr_create(int m, int n, long long nnz, const int* rowptr, const int* colind,
+                  const void* val, int val_prec, int where, pk_csr_t* op);
+int pk_csr_destroy(pk_csr_t op);
+int pk_csr_dims(pk_csr_t op, int* m, int* n, long long* nnz);
+/* y = op(A) x ; x, y in precision prec, on the host or the device. */
+int pk_csr_spmv(pk_csr_t op, char trans, const void* x, void* y, int prec, int where);
+/* Store the handle in IPARM(1:2) so that passing pk_dcsr_aprod_ (or
+ * pk_scsr_aprod_) as APROD selects the device operator in the drivers. */
+int pk_csr_to_iparm(pk_csr_t op, int* iparm);
+
+/* The library's own APROD with the reference callback signature
+ * (double/dlanbpro.F:28-41; hidden CHARACTER length last).  Passed to
+ * DLANSVD/DLANSVD_IRL together with IPARM from pk_csr_to_iparm, the
+ * drivers recognise it and keep the whole Lanczos process on the device;
+ * called directly it computes Y = op(A) X for host arrays. */
+void pk_dcsr_aprod_(const char* transa, const int* m, const int* n, const double* x, double* y,
+                    double* dparm, int* iparm, size_t len_transa);
+void pk_scsr_aprod_(const char* transa, const int* m, const int* n, const float* x, float* y,
+                    float* dparm, int* iparm, size_t len_transa);
+
+/* ---------------------------------------------------------------------- */
+/* Lanczos bidiagonalization engine.  prec = storage type of the bases
+ * U (m x (kmax+1)) and V (n x kmax) kept on the device; all reductions and
+ * the omega-recurrence run in double.  op may be NULL when a host callback
+ * is registered with pk_lbd_set_callback.                                   */
+int pk_lbd_create(int prec, int m, int n, int kmax, pk_csr_t op, pk_lbd_t* eng);
+int pk_lbd_destroy(pk_lbd_t eng);
+/* Host APROD callback (Fortran ABI, arrays in aprod_prec) used for
+ * user-supplied operators: X/Y are staged through pinned host memory. */
+int pk_lbd_set_callback(pk_lbd_t eng, void* aprod, int aprod_prec, void* dparm, int* iparm);
+/* Copy column col (1-based) of U ('u') or V ('v') from / to the host. */
+int pk_lbd_put(pk_lbd_t eng, char which, int col, const void* x, int prec);
+int pk_lbd_get(pk_lbd_t eng, char which, int ncols, void* X, int ldx, int prec);
+/* Device pointer and leading dimension of the current basis. */
+int pk_lbd_basis(pk_lbd_t eng, char which, void** dptr, long long* ld);
+/* ||column col|| of U or V (pdnrm2, double/dblasext.F:6-31). */
+int pk_lbd_nrm2(pk_lbd_t eng, char which, int col, double* nrm);
+
+/* DGETU0 (double/dgetu0.F:11-89): random vector (DLARNV IDIST=2, fixed
+ * seed (1,3,5,7)) mapped through op(A) into column j+1 of U ('n') or V
+ * ('t'), orthogonalized against columns 1..j.  ierr = -1 if every try
+ * gave a zero vector. */
+int pk_lbd_getu0(pk_lbd_t eng, char transa, int j, int ntry, double* u0norm, int icgs,
+                 double* anormest, int* ierr);
+
+/* DLANBPRO (double/dlanbpro.F:1-549): extend a k0-step Lanczos
+ * bidiagonalization to *k steps.  alpha/beta are B(:,1)/B(:,2) (length
+ * >= *k, host, in/out); rnorm in/out; doption(1:3) = delta, eta, anorm
+ * (anorm written back as the reference does at :547); ioption(1:2) =
+ * cgs, elr.  *ierr as the reference's IERR.  *k may shrink on an invariant
+ * subspace. */
+int pk_lbd_lanbpro(pk_lbd_t eng, int k0, int* k, double* alpha, double* beta, double* rnorm,
+                   double* doption, const int* ioption, int* ierr);
+
+/* Basis update of restarts and Ritz vectors, replacing DGEMM_OVWR_LEFT
+ * (double/dgemm_ovwr.F:56-89; calls at double/dlansvd_irl.F:387,394 and
+ * double/dritzvec.F:160,193):  X(:,1:ncols) <- X(:,1:kdim) * op(B), X the
+ * U ('u') or V ('v') basis, B host double column-major, op(B) kdim x ncols:
+ * transb 'n' -> B is kdim x ncols, 't' -> B is ncols x kdim. */
+int pk_lbd_update(pk_lbd_t eng, char which, char transb, int kdim, int ncols, const double* B,
+                  int ldb);
+
+/* Counters with the meaning of COMMON /timing/ (double/stat.h:12-15):
+ * out[0] nopx, [1] nreorth, [2] ndot (reference semantics, see
+ * double/dreorth.F:92,167), [3] basis columns actually streamed by
+ * reorthogonalization, [4] host synchronizations. */
+int pk_lbd_stats(pk_lbd_t eng, long long* out);
+int pk_lbd_reset_stats(pk_lbd_t eng);
+int pk_lbd_sync(pk_lbd_t eng);
+int pk_lbd_shape(pk_lbd_t eng, int* m, int* n, int* kmax, int* prec);
+
+/* ---------------------------------------------------------------------- */
+/* Device-resident drivers (no reference counterpart; for callers whose
+ * matrix and results live on the GPU).  They run the DLANSVD /
+ * DLANSVD_IRL algorithms (double/dlansvd.F:1-291, double/dlansvd_irl.F:1-419)
+ * on an engine created by the caller: the start vector is the engine's
+ * U(:,1) (zero => the reference's random start), sigma/bnd/doption are
+ * host double arrays with the reference's meaning, and the singular vectors
+ * are left in columns 1..k of the engine's bases (pk_lbd_basis).  Options
+ * are single characters by value.  Return INFO as the reference defines it
+ * (0, J > 0 invariant subspace, -1 not converged) or a PK_ERR_* code.     */
+int pk_lansvd_run(pk_lbd_t eng, char jobu, char jobv, int* k, int kmax, double* sigma,
+                  double* bnd, double tolin, double* doption, int* ioption);
+int pk_lansvd_irl_run(pk_lbd_t eng, char which, char jobu, char jobv, int* dim, int p, int* neig,
+                      int maxiter, double* sigma, double* bnd, double tolin, double* doption,
+                      int* ioption);
+
+/* ---------------------------------------------------------------------- */
+/* Host scalar routines, exported so their results can be compared with the
+ * reference's bit for bit.  Arrays are 1-based like the reference's
+ * (element 0 unused).                                                      */
+/* DLARNV(IDIST=2) from the fixed seed (1,3,5,7) of double/dgetu0.F:47-50,
+ * n values in double (prec 0) or float (prec 1). */
+int pk_larnv_pm1(int prec, long long n, void* x);
+/* DUPDATE_MU / DUPDATE_NU / DCOMPUTE_INT / DSET_MU (double/dlanbpro.F:555-723). */
+void pkh_update_mu(double* mumax, double* mu, const double* nu, int j, const double* alpha,
+                   const double* beta, double anorm, double eps1);
+void pkh_update_nu(double* numax, const double* mu, double* nu, int j, const double* alpha,
+                   const double* beta, double anorm, double eps1);
+void pkh_compute_int(const double* mu, int j, double delta, double eta, int* index);
+void pkh_set_mu(int k, double* mu, const int* index, double val);
+/* DLARTG (double/Lapack_Util/dlartg.f), DBDQR, DBSVDSTEP (jobu=jobv='y'),
+ * DREFINEBOUNDS (double/dbsvd.F); 0-based C arrays here. */
+void pkh_lartg(double f, double g, double* c, double* s, double* r);
+void pkh_bdqr(int ignorelast, char jobq, int n, double* d, double* e, double* c1, double* c2,
+              double* qt, int ldq);
+void pkh_bsvdstep(int m, int n, int k, double sigma, double* d, double* e, double* u, int ldu,
+                  double* v, int ldv);
+void pkh_refinebounds(int n, int k, const double* theta, double* bound, double tol,
+                      double eps34);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,342 @@
+"""Python host-side access to libpropack_amd (the C ABI of include/propack_amd.h).
+
+The product path is the HIP library; this module only marshals arguments.
+If the library is missing the import of ``lib()`` raises -- there is no CPU
+fallback.
+
+* ``CsrOperator`` -- a device-resident CSR operator (A and its explicit A^T).
+* ``lansvd`` / ``lansvd_irl`` -- call the Fortran drop-ins DLANSVD /
+  DLANSVD_IRL (or SLANSVD / SLANSVD_IRL) with the reference calling
+  sequence (double/dlansvd.F:1-3, double/dlansvd_irl.F:1-3), either with the
+  device operator (``aprod=op``) or with a Python callable as APROD
+  (host-callback path).
+* ``Engine`` + ``lansvd_irl_dev`` / ``lansvd_dev`` -- device-resident drivers
+  that leave the singular vectors on the GPU.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+from dataclasses import dataclass, field
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "_lib", "libpropack_amd.so")
+
+PK_F64, PK_F32 = 0, 1
+PK_HOST, PK_DEVICE = 0, 1
+
+EXPORTS = [
+    "pk_last_error", "pk_csr_create", "pk_csr_destroy", "pk_csr_dims", "pk_csr_spmv",
+    "pk_csr_to_iparm", "pk_dcsr_aprod_", "pk_scsr_aprod_", "pk_lbd_create",
+    "pk_lbd_destroy", "pk_lbd_set_callback", "pk_lbd_put", "pk_lbd_get", "pk_lbd_basis",
+    "pk_lbd_nrm2", "pk_lbd_getu0", "pk_lbd_lanbpro", "pk_lbd_update", "pk_lbd_stats",
+    "pk_lbd_reset_stats", "pk_lbd_sync", "pk_lbd_shape", "pk_lansvd_run",
+    "pk_lansvd_irl_run", "dlansvd_", "dlansvd_irl_", "slansvd_", "slansvd_irl_",
+    "clearstat_", "printstat_",
+]
+
+_lib = None
+
+
+class PropackError(RuntimeError):
+    pass
+
+
+def lib():
+    """Load libpropack_amd.so (raises if it has not been built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise PropackError(f"{LIB_PATH} is not built; run python -m propack_amd.build")
+        L = C.CDLL(LIB_PATH, mode=C.RTLD_GLOBAL)
+        L.pk_last_error.restype = C.c_char_p
+        _lib = L
+    return _lib
+
+
+def _check(rc: int, what: str) -> None:
+    if rc != 0:
+        msg = lib().pk_last_error().decode(errors="replace")
+        raise PropackError(f"{what} failed with code {rc}: {msg}")
+
+
+def _p(a):
+    return a.ctypes.data_as(C.c_void_p)
+
+
+def _dt(prec):
+    return np.float64 if prec == PK_F64 else np.float32
+
+
+class CsrOperator:
+    """Device CSR operator built from a host ``Csr`` (A^T built on device)."""
+
+    def __init__(self, A, prec: int = PK_F64):
+        L = lib()
+        self.m, self.n, self.prec = A.m, A.n, prec
+        rp = np.ascontiguousarray(A.rowptr, np.int32)
+        ci = np.ascontiguousarray(A.col, np.int32)
+        va = np.ascontiguousarray(A.val, _dt(prec))
+        h = C.c_void_p()
+        _check(L.pk_csr_create(C.c_int(A.m), C.c_int(A.n), C.c_longlong(int(rp[-1])), _p(rp),
+                               _p(ci), _p(va), C.c_int(prec), C.c_int(PK_HOST), C.byref(h)),
+               "pk_csr_create")
+        self.handle = h
+        self.iparm = np.zeros(4, np.int32)
+        _check(L.pk_csr_to_iparm(h, _p(self.iparm)), "pk_csr_to_iparm")
+
+    def spmv(self, x, trans="n"):
+        x = np.ascontiguousarray(x, _dt(self.prec))
+        y = np.zeros(self.n if trans == "t" else self.m, _dt(self.prec))
+        _check(lib().pk_csr_spmv(self.handle, C.c_char(trans.encode()), _p(x), _p(y),
+                                 C.c_int(self.prec), C.c_int(PK_HOST)), "pk_csr_spmv")
+        return y
+
+    def close(self):
+        if self.handle:
+            lib().pk_csr_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+@dataclass
+class SvdResult:
+    sigma: np.ndarray
+    bnd: np.ndarray
+    U: np.ndarray | None
+    V: np.ndarray | None
+    info: int
+    k: int
+    counters: dict = field(default_factory=dict)
+    dim: int = 0
+    anorm: float = 0.0
+
+
+_TIMING = ["nopx", "nreorth", "ndot", "nreorthu", "nreorthv", "nitref", "nrestart", "nbsvd"]
+
+
+class _Timing(C.Structure):
+    # COMMON /timing/ layout (8 ints, 14 reals, nlandim, 3 reals, nsing)
+    _fields_ = [(n, C.c_int) for n in _TIMING] + [("t1", C.c_float * 14),
+                                                   ("nlandim", C.c_int),
+                                                   ("t2", C.c_float * 3), ("nsing", C.c_int)]
+
+
+def timing() -> dict:
+    t = _Timing.in_dll(lib(), "timing_")
+    d = {n: getattr(t, n) for n in _TIMING}
+    d["nlandim"], d["nsing"] = t.nlandim, t.nsing
+    return d
+
+
+def clearstat() -> None:
+    lib().clearstat_()
+
+
+_APROD_D = C.CFUNCTYPE(None, C.c_char_p, C.POINTER(C.c_int), C.POINTER(C.c_int),
+                       C.POINTER(C.c_double), C.POINTER(C.c_double), C.c_void_p, C.c_void_p,
+                       C.c_size_t)
+_APROD_S = C.CFUNCTYPE(None, C.c_char_p, C.POINTER(C.c_int), C.POINTER(C.c_int),
+                       C.POINTER(C.c_float), C.POINTER(C.c_float), C.c_void_p, C.c_void_p,
+                       C.c_size_t)
+
+
+def _make_aprod(aprod, m, n, prec):
+    """(function pointer, iparm, keepalive) for an APROD argument."""
+    if isinstance(aprod, CsrOperator):
+        if aprod.prec != prec:
+            raise PropackError("operator precision differs from the driver's")
+        name = "pk_dcsr_aprod_" if prec == PK_F64 else "pk_scsr_aprod_"
+        fn = C.cast(getattr(lib(), name), C.c_void_p)
+        return fn, aprod.iparm, None
+    dt = _dt(prec)
+    ct = C.c_double if prec == PK_F64 else C.c_float
+
+    def cb(transa, mp, np_, x, y, dparm, iparm, ln):
+        t = transa[0:1].decode().lower()
+        nin, nout = (n, m) if t == "n" else (m, n)
+        xa = np.ctypeslib.as_array(C.cast(x, C.POINTER(ct)), shape=(nin,))
+        ya = np.ctypeslib.as_array(C.cast(y, C.POINTER(ct)), shape=(nout,))
+        ya[:] = np.asarray(aprod(t, xa.astype(np.float64)), dtype=dt)
+
+    f = (_APROD_D if prec == PK_F64 else _APROD_S)(cb)
+    return C.cast(f, C.c_void_p), np.zeros(4, np.int32), f
+
+
+def lansvd(aprod, m, n, k, kmax, u0=None, tol=1e-12, jobu="y", jobv="y", prec=PK_F64,
+           delta=None, eta=None, anorm=0.0, cgs=1, elr=1) -> SvdResult:
+    """DLANSVD / SLANSVD through the library's Fortran drop-in."""
+    L = lib()
+    dt = _dt(prec)
+    eps = float(np.finfo(dt).eps)
+    U = np.zeros((kmax + 1, m), dt)
+    V = np.zeros((kmax, n), dt)
+    if u0 is not None:
+        U[0] = u0
+    sigma = np.zeros(kmax, dt)
+    bnd = np.zeros(kmax, dt)
+    dopt = np.array([np.sqrt(eps) if delta is None else delta,
+                     eps ** 0.75 if eta is None else eta, anorm, 0.0], dt)
+    iopt = np.array([cgs, elr], np.int32)
+    lwork = m + n + 9 * kmax + 5 * kmax * kmax + 4 + max(3 * kmax * kmax + 4 * kmax + 4,
+                                                         16 * max(m, n))
+    work = np.zeros(1, dt)
+    iwork = np.zeros(1, np.int32)
+    fn, iparm, keep = _make_aprod(aprod, m, n, prec)
+    dparm = np.zeros(1, dt)
+    R = C.c_double if prec == PK_F64 else C.c_float
+    ii = lambda v: C.byref(C.c_int(v))
+    kk, info = C.c_int(k), C.c_int(0)
+    clearstat()
+    f = L.dlansvd_ if prec == PK_F64 else L.slansvd_
+    f(C.c_char_p(jobu.encode()), C.c_char_p(jobv.encode()), ii(m), ii(n), C.byref(kk),
+      ii(kmax), fn, _p(U), ii(m), _p(sigma), _p(bnd), _p(V), ii(n), C.byref(R(tol)), _p(work),
+      ii(lwork), _p(iwork), ii(8 * kmax), _p(dopt), _p(iopt), C.byref(info), _p(dparm),
+      _p(iparm), C.c_size_t(1), C.c_size_t(1))
+    del keep
+    kout = kk.value
+    return SvdResult(sigma[:kout].copy(), bnd[:kout].copy(),
+                     U[:kout].T.copy() if jobu == "y" else None,
+                     V[:kout].T.copy() if jobv == "y" else None,
+                     info.value, kout, timing(), anorm=float(dopt[2]))
+
+
+def lansvd_irl(aprod, m, n, neig, dim, p=None, u0=None, tol=1e-10, which="L", jobu="y",
+               jobv="y", maxiter=1000, prec=PK_F64, delta=None, eta=None, anorm=0.0,
+               min_relgap=0.002, cgs=1, elr=1) -> SvdResult:
+    """DLANSVD_IRL / SLANSVD_IRL through the library's Fortran drop-in."""
+    L = lib()
+    dt = _dt(prec)
+    eps = float(np.finfo(dt).eps)
+    if p is None:
+        p = dim - neig
+    U = np.zeros((dim + 1, m), dt)
+    V = np.zeros((dim, n), dt)
+    if u0 is not None:
+        U[0] = u0
+    sigma = np.zeros(dim, dt)
+    bnd = np.zeros(dim, dt)
+    dopt = np.array([np.sqrt(eps) if delta is None else delta,
+                     eps ** 0.75 if eta is None else eta, anorm, min_relgap], dt)
+    iopt = np.array([cgs, elr], np.int32)
+    work = np.zeros(1, dt)
+    iwork = np.zeros(1, np.int32)
+    fn, iparm, keep = _make_aprod(aprod, m, n, prec)
+    dparm = np.zeros(1, dt)
+    R = C.c_double if prec == PK_F64 else C.c_float
+    ii = lambda v: C.byref(C.c_int(v))
+    dd, ne, info = C.c_int(dim), C.c_int(neig), C.c_int(0)
+    clearstat()
+    f = L.dlansvd_irl_ if prec == PK_F64 else L.slansvd_irl_
+    f(C.c_char_p(which.encode()), C.c_char_p(jobu.encode()), C.c_char_p(jobv.encode()), ii(m),
+      ii(n), C.byref(dd), ii(p), C.byref(ne), ii(maxiter), fn, _p(U), ii(m), _p(sigma),
+      _p(bnd), _p(V), ii(n), C.byref(R(tol)), _p(work), ii(1), _p(iwork), ii(1), _p(dopt),
+      _p(iopt), C.byref(info), _p(dparm), _p(iparm), C.c_size_t(1), C.c_size_t(1),
+      C.c_size_t(1))
+    del keep
+    k = ne.value
+    return SvdResult(sigma[:k].copy(), bnd[:k].copy(),
+                     U[:k].T.copy() if jobu == "y" else None,
+                     V[:k].T.copy() if jobv == "y" else None,
+                     info.value, k, timing(), dim=dd.value, anorm=float(dopt[2]))
+
+
+class Engine:
+    """A Lanczos engine (device bases) bound to a device operator."""
+
+    def __init__(self, op: CsrOperator, kmax: int, prec: int | None = None):
+        self.op = op
+        self.prec = op.prec if prec is None else prec
+        self.m, self.n, self.kmax = op.m, op.n, kmax
+        h = C.c_void_p()
+        _check(lib().pk_lbd_create(C.c_int(self.prec), C.c_int(op.m), C.c_int(op.n),
+                                   C.c_int(kmax), op.handle, C.byref(h)), "pk_lbd_create")
+        self.handle = h
+
+    def put_u1(self, u0):
+        u0 = np.ascontiguousarray(u0, np.float64)
+        _check(lib().pk_lbd_put(self.handle, C.c_char(b"u"), C.c_int(1), _p(u0),
+                                C.c_int(PK_F64)), "pk_lbd_put")
+
+    def get(self, which: str, ncols: int, prec=PK_F64):
+        L = self.m if which == "u" else self.n
+        X = np.zeros((ncols, L), _dt(prec))
+        if ncols:
+            _check(lib().pk_lbd_get(self.handle, C.c_char(which.encode()), C.c_int(ncols), _p(X),
+                                    C.c_int(L), C.c_int(prec)), "pk_lbd_get")
+        return X.T
+
+    def stats(self) -> dict:
+        out = np.zeros(5, np.int64)
+        _check(lib().pk_lbd_stats(self.handle, _p(out)), "pk_lbd_stats")
+        return dict(zip(["nopx", "nreorth", "ndot", "ncols", "nsync"], out.tolist()))
+
+    def reset_stats(self):
+        lib().pk_lbd_reset_stats(self.handle)
+
+    def close(self):
+        if self.handle:
+            lib().pk_lbd_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def lansvd_irl_dev(eng: Engine, neig, dim, p=None, tol=1e-10, which="L", jobu="y", jobv="y",
+                   maxiter=1000, delta=None, eta=None, anorm=0.0, min_relgap=0.002, cgs=1,
+                   elr=1, fetch=True) -> SvdResult:
+    """Device-resident DLANSVD_IRL on ``eng`` (start vector = eng U(:,1))."""
+    eps = 2.0 ** -53 if eng.prec == PK_F64 else 2.0 ** -24
+    if p is None:
+        p = dim - neig
+    sigma = np.zeros(dim + 1)
+    bnd = np.zeros(dim + 1)
+    dopt = np.array([np.sqrt(2 * eps) if delta is None else delta,
+                     (2 * eps) ** 0.75 if eta is None else eta, anorm, min_relgap])
+    iopt = np.array([cgs, elr], np.int32)
+    dd, ne = C.c_int(dim), C.c_int(neig)
+    clearstat()
+    info = lib().pk_lansvd_irl_run(eng.handle, C.c_char(which.encode()), C.c_char(jobu.encode()),
+                                   C.c_char(jobv.encode()), C.byref(dd), C.c_int(p), C.byref(ne),
+                                   C.c_int(maxiter), _p(sigma), _p(bnd), C.c_double(tol),
+                                   _p(dopt), _p(iopt))
+    if info <= -100:
+        _check(info, "pk_lansvd_irl_run")
+    k = ne.value
+    U = eng.get("u", k) if (fetch and jobu == "y") else None
+    V = eng.get("v", k) if (fetch and jobv == "y") else None
+    return SvdResult(sigma[:k].copy(), bnd[:k].copy(), U, V, info, k, timing(), dim=dd.value,
+                     anorm=float(dopt[2]))
+
+
+def lansvd_dev(eng: Engine, k, kmax, tol=1e-12, jobu="y", jobv="y", delta=None, eta=None,
+               anorm=0.0, cgs=1, elr=1, fetch=True) -> SvdResult:
+    """Device-resident DLANSVD on ``eng`` (start vector = eng U(:,1))."""
+    eps = 2.0 ** -53 if eng.prec == PK_F64 else 2.0 ** -24
+    sigma = np.zeros(kmax + 1)
+    bnd = np.zeros(kmax + 1)
+    dopt = np.array([np.sqrt(2 * eps) if delta is None else delta,
+                     (2 * eps) ** 0.75 if eta is None else eta, anorm, 0.0])
+    iopt = np.array([cgs, elr], np.int32)
+    kk = C.c_int(k)
+    clearstat()
+    info = lib().pk_lansvd_run(eng.handle, C.c_char(jobu.encode()), C.c_char(jobv.encode()),
+                               C.byref(kk), C.c_int(kmax), _p(sigma), _p(bnd), C.c_double(tol),
+                               _p(dopt), _p(iopt))
+    if info <= -100:
+        _check(info, "pk_lansvd_run")
+    k = kk.value
+    U = eng.get("u", k) if (fetch and jobu == "y") else None
+    V = eng.get("v", k) if (fetch and jobv == "y") else None
+    return SvdResult(sigma[:k].copy(), bnd[:k].copy(), U, V, info, k, timing(), anorm=float(dopt[2]))

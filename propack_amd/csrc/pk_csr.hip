@@ -1,0 +1,362 @@
+// Device-resident CSR operator: A and its explicitly built transpose, both
+// in CSR with int32 row pointers / column indices, values in double or float.
+//
+// The transpose is built on the device by a stable radix sort of the
+// nonzero indices keyed by column (rocPRIM), so every row of A^T lists its
+// entries in increasing original-row order: the build is deterministic and
+// the A^T SpMV sums in a fixed order.
+#include <rocprim/device/device_radix_sort.hpp>
+
+#include <algorithm>
+#include <cstring>
+
+#include "pk_internal.h"
+
+namespace pk {
+
+namespace {
+
+__global__ void k_row_index(int m, const int* __restrict__ rp, int* __restrict__ rowidx) {
+  for (long long i = static_cast<long long>(blockIdx.x) * blockDim.x + threadIdx.x; i < m;
+       i += static_cast<long long>(gridDim.x) * blockDim.x) {
+    for (int p = rp[i]; p < rp[i + 1]; ++p) rowidx[p] = static_cast<int>(i);
+  }
+}
+
+__global__ void k_iota(long long n, int* __restrict__ v) {
+  for (long long i = static_cast<long long>(blockIdx.x) * blockDim.x + threadIdx.x; i < n;
+       i += static_cast<long long>(gridDim.x) * blockDim.x)
+    v[i] = static_cast<int>(i);
+}
+
+template <typename VT>
+__global__ void k_gather_t(long long nnz, const int* __restrict__ perm,
+                           const int* __restrict__ rowidx, const VT* __restrict__ va,
+                           int* __restrict__ cit, VT* __restrict__ vat) {
+  for (long long q = static_cast<long long>(blockIdx.x) * blockDim.x + threadIdx.x; q < nnz;
+       q += static_cast<long long>(gridDim.x) * blockDim.x) {
+    const int p = perm[q];
+    cit[q] = rowidx[p];
+    vat[q] = va[p];
+  }
+}
+
+// rpt[j] = first position q with key[q] >= j, for j in [0, n]; keys sorted.
+__global__ void k_rowptr_from_keys(long long nnz, int n, const int* __restrict__ keys,
+                                   int* __restrict__ rpt) {
+  for (long long q = static_cast<long long>(blockIdx.x) * blockDim.x + threadIdx.x; q <= nnz;
+       q += static_cast<long long>(gridDim.x) * blockDim.x) {
+    const int lo = q == 0 ? -1 : keys[q - 1];
+    const int hi = q == nnz ? n : keys[q];
+    for (int j = lo + 1; j <= hi; ++j) rpt[j] = static_cast<int>(q);
+  }
+}
+
+int bits_for(int n) {
+  int b = 1;
+  while ((1LL << b) < n) ++b;
+  return b;
+}
+
+// Lanes per row for the sub-wave SpMV: the power of two nearest the mean
+// row length, clamped to [2, 64].
+int choose_lpr(long long nnz, int rows) {
+  const double avg = rows > 0 ? static_cast<double>(nnz) / rows : 1.0;
+  int l = 2;
+  while (l < 64 && l * 1.5 < avg) l *= 2;
+  return l;
+}
+
+}  // namespace
+
+template <typename VT>
+static int build_transpose(CsrOp* op, hipStream_t s) {
+  const long long nnz = op->nnz;
+  const int m = op->m, n = op->n;
+  int *rowidx = nullptr, *keys_in = nullptr, *keys_out = nullptr, *perm_in = nullptr,
+      *perm_out = nullptr;
+  void* tmp = nullptr;
+  size_t tmp_bytes = 0;
+  int rc = PK_OK;
+  const size_t ib = sizeof(int) * static_cast<size_t>(std::max<long long>(nnz, 1));
+#define PKT(x)                 \
+  if ((x) != hipSuccess) {     \
+    rc = PK_ERR_HIP;           \
+    goto out;                  \
+  }
+  PKT(hipMalloc(&rowidx, ib));
+  PKT(hipMalloc(&keys_in, ib));
+  PKT(hipMalloc(&keys_out, ib));
+  PKT(hipMalloc(&perm_in, ib));
+  PKT(hipMalloc(&perm_out, ib));
+  PKT(hipMalloc(&op->rpt, sizeof(int) * (static_cast<size_t>(n) + 1)));
+  PKT(hipMalloc(&op->cit, ib));
+  PKT(hipMalloc(&op->vat, op->val_bytes * static_cast<size_t>(std::max<long long>(nnz, 1))));
+  if (nnz > 0) {
+    hipLaunchKernelGGL(k_row_index, dim3(2048), dim3(256), 0, s, m, op->rp, rowidx);
+    PKT(hipMemcpyAsync(keys_in, op->ci, sizeof(int) * nnz, hipMemcpyDeviceToDevice, s));
+    hipLaunchKernelGGL(k_iota, dim3(2048), dim3(256), 0, s, nnz, perm_in);
+    PKT(rocprim::radix_sort_pairs(nullptr, tmp_bytes, keys_in, keys_out, perm_in, perm_out,
+                                  static_cast<size_t>(nnz), 0, bits_for(n), s));
+    PKT(hipMalloc(&tmp, tmp_bytes));
+    PKT(rocprim::radix_sort_pairs(tmp, tmp_bytes, keys_in, keys_out, perm_in, perm_out,
+                                  static_cast<size_t>(nnz), 0, bits_for(n), s));
+    hipLaunchKernelGGL((k_gather_t<VT>), dim3(2048), dim3(256), 0, s, nnz, perm_out, rowidx,
+                       static_cast<const VT*>(op->va), op->cit, static_cast<VT*>(op->vat));
+  }
+  hipLaunchKernelGGL(k_rowptr_from_keys, dim3(2048), dim3(256), 0, s, nnz, n, keys_out,
+                     op->rpt);
+  PKT(hipGetLastError());
+  PKT(hipStreamSynchronize(s));
+out:
+#undef PKT
+  (void)hipFree(rowidx);
+  (void)hipFree(keys_in);
+  (void)hipFree(keys_out);
+  (void)hipFree(perm_in);
+  (void)hipFree(perm_out);
+  (void)hipFree(tmp);
+  return rc;
+}
+
+int csr_create(int m, int n, long long nnz, const int* rowptr, const int* colind,
+               const void* val, int val_prec, int where, CsrOp** out) {
+  if (!out || m < 0 || n < 0 || nnz < 0 || nnz >= (1LL << 31)) return PK_ERR_ARG;
+  if (val_prec != PK_F64 && val_prec != PK_F32) return PK_ERR_ARG;
+  if (where != PK_HOST && where != PK_DEVICE) return PK_ERR_ARG;
+  if (!rowptr || (nnz > 0 && (!colind || !val))) return PK_ERR_ARG;
+  if (where == PK_HOST) {
+    if (rowptr[0] != 0 || rowptr[m] != nnz) return PK_ERR_ARG;
+    for (int i = 0; i < m; ++i)
+      if (rowptr[i + 1] < rowptr[i]) return PK_ERR_ARG;
+    for (long long p = 0; p < nnz; ++p)
+      if (colind[p] < 0 || colind[p] >= n) return PK_ERR_ARG;
+  }
+  CsrOp* op = new CsrOp();
+  op->m = m;
+  op->n = n;
+  op->nnz = nnz;
+  op->val_prec = val_prec;
+  op->val_bytes = val_prec == PK_F64 ? 8 : 4;
+  op->magic = CsrOp::kMagic;
+  const hipMemcpyKind kind = where == PK_HOST ? hipMemcpyHostToDevice : hipMemcpyDeviceToDevice;
+  hipStream_t s = nullptr;
+  int rc = PK_OK;
+  const size_t nz = static_cast<size_t>(std::max<long long>(nnz, 1));
+  if (hipStreamCreate(&s) != hipSuccess) {
+    delete op;
+    return PK_ERR_HIP;
+  }
+  if (hipMalloc(&op->rp, sizeof(int) * (static_cast<size_t>(m) + 1)) != hipSuccess ||
+      hipMalloc(&op->ci, sizeof(int) * nz) != hipSuccess ||
+      hipMalloc(&op->va, op->val_bytes * nz) != hipSuccess ||
+      hipMemcpyAsync(op->rp, rowptr, sizeof(int) * (static_cast<size_t>(m) + 1), kind, s) !=
+          hipSuccess ||
+      (nnz > 0 && hipMemcpyAsync(op->ci, colind, sizeof(int) * nnz, kind, s) != hipSuccess) ||
+      (nnz > 0 && hipMemcpyAsync(op->va, val, op->val_bytes * nnz, kind, s) != hipSuccess)) {
+    rc = PK_ERR_HIP;
+  }
+  if (rc == PK_OK)
+    rc = val_prec == PK_F64 ? build_transpose<double>(op, s) : build_transpose<float>(op, s);
+  (void)hipStreamDestroy(s);
+  if (rc != PK_OK) {
+    csr_destroy(op);
+    return rc;
+  }
+  op->lpr_n = choose_lpr(nnz, m);
+  op->lpr_t = choose_lpr(nnz, n);
+  *out = op;
+  return PK_OK;
+}
+
+void csr_destroy(CsrOp* op) {
+  if (!op) return;
+  (void)hipFree(op->rp);
+  (void)hipFree(op->ci);
+  (void)hipFree(op->va);
+  (void)hipFree(op->rpt);
+  (void)hipFree(op->cit);
+  (void)hipFree(op->vat);
+  op->magic = 0;
+  delete op;
+}
+
+// y = op(A) x - coef * yprev with fused partial sums; x, y, yprev device
+// vectors of basis type T.  Returns nblk of the partials or < 0.
+template <typename T>
+int csr_apply(const CsrOp* op, hipStream_t s, char trans, const T* x, T* y, const T* yprev,
+              Scal coef, double* part, const int* gate) {
+  const bool tr = (trans == 't' || trans == 'T');
+  const int rows = tr ? op->n : op->m;
+  const int* rp = tr ? op->rpt : op->rp;
+  const int* ci = tr ? op->cit : op->ci;
+  const void* va = tr ? op->vat : op->va;
+  const int lpr = tr ? op->lpr_t : op->lpr_n;
+  if (op->val_prec == PK_F64)
+    return launch_spmv<double, T>(s, rows, rp, ci, static_cast<const double*>(va), x, y, yprev,
+                                  coef, part, lpr, gate);
+  return launch_spmv<float, T>(s, rows, rp, ci, static_cast<const float*>(va), x, y, yprev, coef,
+                               part, lpr, gate);
+}
+
+template int csr_apply<double>(const CsrOp*, hipStream_t, char, const double*, double*,
+                               const double*, Scal, double*, const int*);
+template int csr_apply<float>(const CsrOp*, hipStream_t, char, const float*, float*,
+                              const float*, Scal, double*, const int*);
+
+}  // namespace pk
+
+// ------------------------------------------------------------------ C ABI --
+namespace pk {
+void set_error(const char* fmt, ...);
+CsrOp* as_csr(pk_csr_t h);
+}  // namespace pk
+
+using namespace pk;
+
+namespace {
+
+// y = op(A) x for caller vectors on the host or the device, precision prec.
+template <typename T>
+int spmv_any(CsrOp* op, char trans, const T* x, T* y, int where) {
+  const bool tr = (trans == 't' || trans == 'T');
+  const long long rin = tr ? op->m : op->n, rout = tr ? op->n : op->m;
+  hipStream_t s;
+  if (hipStreamCreateWithFlags(&s, hipStreamNonBlocking) != hipSuccess) return PK_ERR_HIP;
+  T *dx = nullptr, *dy = nullptr;
+  double* part = nullptr;
+  int rc = PK_OK;
+  if (hipMalloc(&part, sizeof(double) * 2 * kMaxBlocks) != hipSuccess) rc = PK_ERR_OOM;
+  if (rc == PK_OK && where == PK_HOST) {
+    if (hipMalloc(&dx, sizeof(T) * std::max<long long>(rin, 1)) != hipSuccess ||
+        hipMalloc(&dy, sizeof(T) * std::max<long long>(rout, 1)) != hipSuccess ||
+        hipMemcpyAsync(dx, x, sizeof(T) * rin, hipMemcpyHostToDevice, s) != hipSuccess)
+      rc = PK_ERR_HIP;
+  } else if (rc == PK_OK) {
+    dx = const_cast<T*>(x);
+    dy = y;
+  }
+  if (rc == PK_OK) {
+    if (op->val_prec == (sizeof(T) == 8 ? PK_F64 : PK_F32)) {
+      if (csr_apply<T>(op, s, trans, dx, dy, nullptr, sval(0.0), part, nullptr) < 0)
+        rc = PK_ERR_HIP;
+    } else {
+      set_error("pk_csr_spmv: vector precision differs from the operator's");
+      rc = PK_ERR_NOTSUP;
+    }
+  }
+  if (rc == PK_OK && where == PK_HOST &&
+      hipMemcpyAsync(y, dy, sizeof(T) * rout, hipMemcpyDeviceToHost, s) != hipSuccess)
+    rc = PK_ERR_HIP;
+  if (hipStreamSynchronize(s) != hipSuccess && rc == PK_OK) rc = PK_ERR_HIP;
+  if (where == PK_HOST) {
+    (void)hipFree(dx);
+    (void)hipFree(dy);
+  }
+  (void)hipFree(part);
+  (void)hipStreamDestroy(s);
+  return rc;
+}
+
+CsrOp* csr_from_iparm(const int* iparm) {
+  if (!iparm) return nullptr;
+  const uint64_t lo = static_cast<uint32_t>(iparm[0]), hi = static_cast<uint32_t>(iparm[1]);
+  return as_csr(reinterpret_cast<pk_csr_t>(static_cast<uintptr_t>(lo | (hi << 32))));
+}
+
+}  // namespace
+
+extern "C" {
+
+int pk_csr_create(int m, int n, long long nnz, const int* rowptr, const int* colind,
+                  const void* val, int val_prec, int where, pk_csr_t* op) {
+  CsrOp* o = nullptr;
+  const int rc = csr_create(m, n, nnz, rowptr, colind, val, val_prec, where, &o);
+  if (rc != PK_OK) {
+    set_error("pk_csr_create failed (code %d)", rc);
+    return rc;
+  }
+  *op = reinterpret_cast<pk_csr_t>(o);
+  return PK_OK;
+}
+
+int pk_csr_destroy(pk_csr_t h) {
+  CsrOp* op = as_csr(h);
+  if (!op) return PK_ERR_HANDLE;
+  csr_destroy(op);
+  return PK_OK;
+}
+
+int pk_csr_dims(pk_csr_t h, int* m, int* n, long long* nnz) {
+  CsrOp* op = as_csr(h);
+  if (!op) return PK_ERR_HANDLE;
+  if (m) *m = op->m;
+  if (n) *n = op->n;
+  if (nnz) *nnz = op->nnz;
+  return PK_OK;
+}
+
+int pk_csr_spmv(pk_csr_t h, char trans, const void* x, void* y, int prec, int where) {
+  CsrOp* op = as_csr(h);
+  if (!op) return PK_ERR_HANDLE;
+  if (!x || !y || (where != PK_HOST && where != PK_DEVICE)) return PK_ERR_ARG;
+  if (prec == PK_F64)
+    return spmv_any<double>(op, trans, static_cast<const double*>(x), static_cast<double*>(y),
+                            where);
+  if (prec == PK_F32)
+    return spmv_any<float>(op, trans, static_cast<const float*>(x), static_cast<float*>(y), where);
+  return PK_ERR_ARG;
+}
+
+int pk_csr_to_iparm(pk_csr_t h, int* iparm) {
+  if (!as_csr(h) || !iparm) return PK_ERR_ARG;
+  const uint64_t v = static_cast<uint64_t>(reinterpret_cast<uintptr_t>(h));
+  iparm[0] = static_cast<int>(static_cast<uint32_t>(v & 0xffffffffu));
+  iparm[1] = static_cast<int>(static_cast<uint32_t>(v >> 32));
+  return PK_OK;
+}
+
+void pk_dcsr_aprod_(const char* transa, const int* m, const int* n, const double* x, double* y,
+                    double* dparm, int* iparm, size_t len_transa) {
+  (void)m; (void)n; (void)dparm; (void)len_transa;
+  CsrOp* op = csr_from_iparm(iparm);
+  if (op) spmv_any<double>(op, *transa, x, y, PK_HOST);
+}
+
+void pk_scsr_aprod_(const char* transa, const int* m, const int* n, const float* x, float* y,
+                    float* dparm, int* iparm, size_t len_transa) {
+  (void)m; (void)n; (void)dparm; (void)len_transa;
+  CsrOp* op = csr_from_iparm(iparm);
+  if (op) spmv_any<float>(op, *transa, x, y, PK_HOST);
+}
+
+// Fortran-side helper (implicit interface, all arguments by reference):
+// kind = 1 when APROD is the library's pk_dcsr_aprod_, 2 for pk_scsr_aprod_,
+// with a live operator handle in IPARM(1:2) returned in *handle; kind = 0
+// for any other (user) APROD.
+void pk_f_classify_(void* aprod, const int* iparm, int* kind, long long* handle) {
+  *kind = 0;
+  *handle = 0;
+  const bool dk = aprod == reinterpret_cast<void*>(&pk_dcsr_aprod_);
+  const bool sk = aprod == reinterpret_cast<void*>(&pk_scsr_aprod_);
+  if (!dk && !sk) return;
+  CsrOp* op = csr_from_iparm(iparm);
+  if (!op) return;
+  *kind = dk ? 1 : 2;
+  *handle = static_cast<long long>(reinterpret_cast<uintptr_t>(op));
+}
+
+// Fortran-side helpers: register a user APROD (double / real arrays) on an
+// engine given by its handle value.
+void pk_f_set_callback_d_(const long long* eng, void* aprod, double* dparm, int* iparm,
+                          int* info) {
+  *info = pk_lbd_set_callback(reinterpret_cast<pk_lbd_t>(static_cast<uintptr_t>(*eng)), aprod,
+                              PK_F64, dparm, iparm);
+}
+void pk_f_set_callback_s_(const long long* eng, void* aprod, float* sparm, int* iparm,
+                          int* info) {
+  *info = pk_lbd_set_callback(reinterpret_cast<pk_lbd_t>(static_cast<uintptr_t>(*eng)), aprod,
+                              PK_F32, sparm, iparm);
+}
+
+}  // extern "C"

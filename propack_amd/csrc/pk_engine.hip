@@ -1,0 +1,1022 @@
+// Lanczos bidiagonalization engine (the reference's L4 routine DLANBPRO with
+// its L3/L2 helpers DREORTH/DCGS, DGETU0, DSAFESCAL and the pd* level-1
+// wrappers) on the GPU.
+//
+// The bases U, V, the operator and every vector operation live on the
+// device.  The scalar control of the step (alpha/beta, the extended local
+// reorthogonalization test, the mu/nu omega-recurrence and the choice of
+// reorthogonalization intervals) is evaluated in double on the host from
+// deterministic device reductions, in exactly the order of the reference.
+// Each device reduction that feeds a decision costs one stream
+// synchronization; counters record them.
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <algorithm>
+
+#include "pk_internal.h"
+
+namespace pk {
+
+static thread_local char g_err[512] = "";
+
+void set_error(const char* fmt, ...) {
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(g_err, sizeof g_err, fmt, ap);
+  va_end(ap);
+}
+
+const char* last_error() { return g_err; }
+
+#define PK_HIP(expr)                                                                     \
+  do {                                                                                   \
+    hipError_t e_ = (expr);                                                              \
+    if (e_ != hipSuccess) {                                                              \
+      set_error("%s:%d %s: %s", __FILE__, __LINE__, #expr, hipGetErrorString(e_));       \
+      return e_ == hipErrorOutOfMemory ? PK_ERR_OOM : PK_ERR_HIP;                        \
+    }                                                                                    \
+  } while (0)
+#define PK_TRY(expr)        \
+  do {                      \
+    int r_ = (expr);        \
+    if (r_ < 0) return r_;  \
+  } while (0)
+#define PK_LAUNCH(expr)                                                     \
+  do {                                                                      \
+    if ((expr) < 0) {                                                       \
+      set_error("%s:%d launch failed: %s", __FILE__, __LINE__, #expr);      \
+      return PK_ERR_HIP;                                                    \
+    }                                                                       \
+  } while (0)
+
+// Unit roundoff and safe minimum as LAPACK xLAMCH('E') / xLAMCH('S') return
+// them for the storage precision (double/Lapack_Util/dlamch.f).
+template <typename T>
+struct Mach;
+template <>
+struct Mach<double> {
+  static constexpr double eps = 0x1p-53;
+  static constexpr double sfmin = 0x1p-1022;
+};
+template <>
+struct Mach<float> {
+  static constexpr double eps = 0x1p-24;
+  static constexpr double sfmin = 0x1p-126;
+};
+
+// DLARNV(IDIST=2) from the fixed seed ISEED=(1,3,5,7) used by DGETU0
+// (double/dgetu0.F:47-50): a 48-bit multiplicative congruential sequence
+// with multiplier 33952834046453; output g (0-based) is
+// 2*((s0*a^(g+1) mod 2^48)/2^48) - 1, the 48-bit value converted from four
+// 12-bit limbs in Horner form in the target precision, as xLARUV does.
+template <typename T>
+static void larnv_pm1(long long n, T* x) {
+  const uint64_t mask = (1ULL << 48) - 1, a = 33952834046453ULL;
+  uint64_t s = (1ULL << 36) + (3ULL << 24) + (5ULL << 12) + 7ULL;
+  const T r = T(1) / T(4096);
+  for (long long g = 0; g < n; ++g) {
+    s = static_cast<uint64_t>((static_cast<unsigned __int128>(s) * a) & mask);
+    const T i1 = static_cast<T>(s >> 36), i2 = static_cast<T>((s >> 24) & 4095);
+    const T i3 = static_cast<T>((s >> 12) & 4095), i4 = static_cast<T>(s & 4095);
+    const T u = r * (i1 + r * (i2 + r * (i3 + r * i4)));
+    x[g] = T(2) * u - T(1);
+  }
+}
+
+// DLAPY2 (double/Lapack_Util/dlapy2.f).
+static double lapy2(double x, double y) {
+  const double xa = std::fabs(x), ya = std::fabs(y);
+  const double w = std::max(xa, ya), z = std::min(xa, ya);
+  if (z == 0.0) return w;
+  return w * std::sqrt(1.0 + (z / w) * (z / w));
+}
+
+static double fsign(double a, double b) { return b >= 0 ? std::fabs(a) : -std::fabs(a); }
+
+// ---- omega recurrence (double/dlanbpro.F:555-723); 1-based arrays ----
+
+// dset_mu (dlanbpro.F:555-577)
+static void set_mu(int k, double* mu, const int* index, double val) {
+  int i = 1;
+  while (index[i] <= k && index[i] > 0) {
+    for (int j = index[i]; j <= index[i + 1]; ++j) mu[j] = val;
+    i += 2;
+  }
+}
+
+// dcompute_int (dlanbpro.F:581-624)
+static void compute_int(const double* mu, int j, double delta, double eta, int* index) {
+  if (delta < eta) return;
+  int ip = 0, i = 0, k, s;
+  index[1] = 0;
+  while (i < j) {
+    for (k = i + 1; k <= j; ++k)
+      if (std::fabs(mu[k]) > delta) break;
+    if (k > j) break;
+    const int lo = std::max(i, 1);
+    for (s = k; s >= lo; --s)
+      if (std::fabs(mu[s]) < eta) break;
+    index[++ip] = s + 1;
+    for (i = s + 1; i <= j; ++i)
+      if (std::fabs(mu[i]) < eta) break;
+    index[++ip] = i - 1;
+  }
+  index[++ip] = j + 1;
+}
+
+// dupdate_mu (dlanbpro.F:628-679); alpha/beta 1-based views of B.
+static void update_mu(double* mumax, double* mu, const double* nu, int j, const double* alpha,
+                      const double* beta, double anorm, double eps1) {
+  double d;
+  if (j == 1) {
+    d = eps1 * (lapy2(alpha[j], beta[j]) + alpha[1]) + eps1 * anorm;
+    mu[1] = eps1 / beta[1];
+    *mumax = std::fabs(mu[1]);
+  } else {
+    mu[1] = alpha[1] * nu[1] - alpha[j] * mu[1];
+    d = eps1 * (lapy2(alpha[j], beta[j]) + alpha[1]) + eps1 * anorm;
+    mu[1] = (mu[1] + fsign(d, mu[1])) / beta[j];
+    *mumax = std::fabs(mu[1]);
+    for (int k = 2; k <= j - 1; ++k) {
+      mu[k] = alpha[k] * nu[k] + beta[k - 1] * nu[k - 1] - alpha[j] * mu[k];
+      d = eps1 * (lapy2(alpha[j], beta[j]) + lapy2(alpha[k], beta[k - 1])) + eps1 * anorm;
+      mu[k] = (mu[k] + fsign(d, mu[k])) / beta[j];
+      *mumax = std::max(*mumax, std::fabs(mu[k]));
+    }
+    mu[j] = beta[j - 1] * nu[j - 1];
+    d = eps1 * (lapy2(alpha[j], beta[j]) + lapy2(alpha[j], beta[j - 1])) + eps1 * anorm;
+    mu[j] = (mu[j] + fsign(d, mu[j])) / beta[j];
+    *mumax = std::max(*mumax, std::fabs(mu[j]));
+  }
+  mu[j + 1] = 1.0;
+}
+
+// dupdate_nu (dlanbpro.F:684-723)
+static void update_nu(double* numax, const double* mu, double* nu, int j, const double* alpha,
+                      const double* beta, double anorm, double eps1) {
+  if (j > 1) {
+    *numax = 0.0;
+    for (int k = 1; k <= j - 1; ++k) {
+      nu[k] = beta[k] * mu[k + 1] + alpha[k] * mu[k] - beta[j - 1] * nu[k];
+      const double d =
+          eps1 * (lapy2(alpha[k], beta[k]) + lapy2(alpha[j], beta[j - 1])) + eps1 * anorm;
+      nu[k] = (nu[k] + fsign(d, nu[k])) / alpha[j];
+      *numax = std::max(*numax, std::fabs(nu[k]));
+    }
+    nu[j] = 1.0;
+  }
+}
+
+template <typename T>
+struct Impl {
+  Engine* e;
+  hipStream_t strm;
+  explicit Impl(Engine* eng) : e(eng), strm(eng->stream) {}
+
+  T* U() const { return static_cast<T*>(e->u()); }
+  T* V() const { return static_cast<T*>(e->v()); }
+  T* ucol(int j) const { return U() + static_cast<long long>(j - 1) * e->ldu; }
+  T* vcol(int j) const { return V() + static_cast<long long>(j - 1) * e->ldv; }
+  T* scratch(int i) const {
+    return static_cast<T*>(e->vec) + static_cast<long long>(i) * std::max(e->m, e->n);
+  }
+
+  int read(int offset, int nvals) {
+    PK_HIP(hipMemcpyAsync(e->hred + offset, e->red + offset, sizeof(double) * nvals,
+                          hipMemcpyDeviceToHost, strm));
+    PK_HIP(hipStreamSynchronize(strm));
+    e->cnt.nsync++;
+    return PK_OK;
+  }
+
+  // ||x|| of a device vector (pdnrm2).
+  int nrm2(const T* x, long long L, double* out) {
+    if (L <= 0) {
+      *out = 0.0;
+      return PK_OK;
+    }
+    const int nb = launch_norm<T>(strm, L, x, e->part);
+    PK_LAUNCH(nb);
+    PK_LAUNCH(launch_reduce(strm, 1, nb, e->part, e->red, nullptr));
+    PK_TRY(read(0, 1));
+    *out = std::sqrt(e->hred[0]);
+    return PK_OK;
+  }
+
+  // y = op(A) x - c * yprev; *s2 = ||y||^2, *sd = yprev.y (0 without yprev).
+  int apply(char trans, const T* x, T* y, const T* yprev, double c, double* s2, double* sd) {
+    const bool tr = (trans == 't' || trans == 'T');
+    const long long rows_out = tr ? e->n : e->m, rows_in = tr ? e->m : e->n;
+    int nb;
+    if (e->op) {
+      nb = csr_apply<T>(e->op, strm, trans, x, y, yprev, sval(c), e->part, nullptr);
+      PK_LAUNCH(nb);
+    } else if (e->aprod) {
+      PK_TRY(host_apply(trans, x, y, rows_in, rows_out));
+      if (yprev)
+        nb = launch_axpy_norm_dot<T>(strm, rows_out, y, yprev, sval(c), e->part, nullptr);
+      else
+        nb = launch_norm<T>(strm, rows_out, y, e->part);
+      PK_LAUNCH(nb);
+      if (!yprev) {
+        PK_LAUNCH(launch_reduce(strm, 1, nb, e->part, e->red, nullptr));
+        PK_TRY(read(0, 1));
+        *s2 = e->hred[0];
+        *sd = 0.0;
+        return PK_OK;
+      }
+    } else {
+      set_error("engine has no operator");
+      return PK_ERR_ARG;
+    }
+    PK_LAUNCH(launch_reduce(strm, 2, nb, e->part, e->red, nullptr));
+    PK_TRY(read(0, 2));
+    *s2 = e->hred[0];
+    *sd = yprev ? e->hred[1] : 0.0;
+    return PK_OK;
+  }
+
+  // Host callback: X staged device -> pinned host, Y back to the device.
+  int host_apply(char trans, const T* x, T* y, long long rows_in, long long rows_out) {
+    const char t = (trans == 't' || trans == 'T') ? 't' : 'n';
+    const int m = e->m, n = e->n;
+    const bool same = (e->aprod_prec == PK_F64) == (sizeof(T) == 8);
+    if (same) {
+      PK_HIP(hipMemcpyAsync(e->hx, x, sizeof(T) * rows_in, hipMemcpyDeviceToHost, strm));
+      PK_HIP(hipStreamSynchronize(strm));
+    } else {
+      PK_HIP(hipMemcpyAsync(e->hy, x, sizeof(T) * rows_in, hipMemcpyDeviceToHost, strm));
+      PK_HIP(hipStreamSynchronize(strm));
+      const T* src = static_cast<const T*>(e->hy);
+      if (e->aprod_prec == PK_F64)
+        for (long long i = 0; i < rows_in; ++i) static_cast<double*>(e->hx)[i] = src[i];
+      else
+        for (long long i = 0; i < rows_in; ++i) static_cast<float*>(e->hx)[i] = src[i];
+    }
+    e->cnt.nsync++;
+    if (e->aprod_prec == PK_F64)
+      reinterpret_cast<aprod_d_fn>(e->aprod)(&t, &m, &n, static_cast<const double*>(e->hx),
+                                             static_cast<double*>(e->hy),
+                                             static_cast<double*>(e->dparm), e->iparm, 1);
+    else
+      reinterpret_cast<aprod_s_fn>(e->aprod)(&t, &m, &n, static_cast<const float*>(e->hx),
+                                             static_cast<float*>(e->hy),
+                                             static_cast<float*>(e->dparm), e->iparm, 1);
+    if (!same) {
+      T* dst = static_cast<T*>(e->hx);
+      if (e->aprod_prec == PK_F64)
+        for (long long i = 0; i < rows_out; ++i)
+          dst[i] = static_cast<T>(static_cast<double*>(e->hy)[i]);
+      else
+        for (long long i = 0; i < rows_out; ++i)
+          dst[i] = static_cast<T>(static_cast<float*>(e->hy)[i]);
+      PK_HIP(hipMemcpyAsync(y, e->hx, sizeof(T) * rows_out, hipMemcpyHostToDevice, strm));
+    } else {
+      PK_HIP(hipMemcpyAsync(y, e->hy, sizeof(T) * rows_out, hipMemcpyHostToDevice, strm));
+    }
+    return PK_OK;
+  }
+
+  int scale(T* x, long long L, double a) {
+    PK_LAUNCH(launch_scale<T>(strm, L, x, sval(a), nullptr));
+    return PK_OK;
+  }
+
+  // dsafescal (double/dsafescal.F:4-55): x /= alpha; below the safe minimum
+  // the factor is applied in the stepwise way of dlascl.
+  int safescal(long long L, double alpha, T* x) {
+    if (std::fabs(alpha) >= Mach<T>::sfmin) return scale(x, L, 1.0 / alpha);
+    const double smlnum = Mach<T>::sfmin, bignum = 1.0 / smlnum;
+    double cfromc = alpha, ctoc = 1.0;
+    bool done = false;
+    while (!done) {
+      const double cfrom1 = cfromc * smlnum, cto1 = ctoc / bignum;
+      double mul;
+      if (std::fabs(cfrom1) > std::fabs(ctoc) && ctoc != 0.0) {
+        mul = smlnum;
+        cfromc = cfrom1;
+      } else if (std::fabs(cto1) > std::fabs(cfromc)) {
+        mul = bignum;
+        ctoc = cto1;
+      } else {
+        mul = ctoc / cfromc;
+        done = true;
+      }
+      PK_TRY(scale(x, L, mul));
+    }
+    return PK_OK;
+  }
+
+  // dreorth + dcgs (double/dreorth.F:5-210): iterated block classical
+  // Gram-Schmidt of vnew against basis(:, INDEX intervals), up to 5 passes
+  // until ||vnew'|| > alpha ||vnew||; otherwise vnew = 0.  INDEX is 1-based.
+  int reorth(long long L, int k, T* basis, long long ld, T* vnew, double* normvnew,
+             const int* index, double alpha) {
+    if (k <= 0 || L <= 0) return PK_OK;
+    int ncols = 0;
+    long long nd = 0;
+    for (int i = 1; index[i] <= k && index[i] > 0; i += 2) {
+      const int p = index[i], q = index[i + 1];
+      nd += q - p + 1;
+      for (int c = p; c <= q; ++c) {
+        if (ncols >= kMaxCols) {
+          set_error("reorthogonalization touches more than %d columns", kMaxCols);
+          return PK_ERR_NOTSUP;
+        }
+        e->hcols[ncols++] = c - 1;
+      }
+    }
+    if (ncols > 0)
+      PK_HIP(hipMemcpyAsync(e->dcols, e->hcols, sizeof(int) * ncols, hipMemcpyHostToDevice, strm));
+    bool ok = false;
+    for (int itry = 1; itry <= 5; ++itry) {
+      const double n0 = *normvnew;
+      e->cnt.ndot += nd + k;
+      e->cnt.ncols_touched += ncols;
+      if (ncols > 0) {
+        const int nb = launch_gemvt<T>(strm, L, basis, ld, e->dcols, ncols, vnew, e->part, nullptr);
+        PK_LAUNCH(nb);
+        PK_LAUNCH(launch_reduce(strm, ncols, nb, e->part, e->red, nullptr));
+        const int nb2 =
+            launch_gemv_update<T>(strm, L, basis, ld, e->dcols, ncols, e->red, vnew, e->part, nullptr);
+        PK_LAUNCH(nb2);
+        PK_LAUNCH(launch_reduce(strm, 1, nb2, e->part, e->red + kMaxCols, nullptr));
+        PK_TRY(read(kMaxCols, 1));
+        *normvnew = std::sqrt(e->hred[kMaxCols]);
+      } else {
+        PK_TRY(nrm2(vnew, L, normvnew));
+      }
+      if (*normvnew > alpha * n0) {
+        ok = true;
+        break;
+      }
+    }
+    if (!ok) {
+      *normvnew = 0.0;
+      PK_LAUNCH(launch_fill_zero_bytes(strm, vnew, sizeof(T) * L));
+    }
+    e->cnt.nreorth++;
+    return PK_OK;
+  }
+
+  // dgetu0 (double/dgetu0.F:11-89) writing op(A) x into u0 (device), then
+  // orthogonalizing against basis(:, 1:j).  Returns ierr (0 or -1).
+  int getu0(char transa, int j, int ntry, T* u0, double* u0norm, T* basis, long long ld,
+            double* anormest, int* ierr) {
+    const bool tn = (transa == 'n' || transa == 'N');
+    const long long rsize = tn ? e->n : e->m, usize = tn ? e->m : e->n;
+    T* x = scratch(0);
+    T* hx = reinterpret_cast<T*>(e->hvec);
+    // kappa is the single-precision literal 0.717 in dgetu0.F:28-29.
+    const double kap = static_cast<double>(0.717f);
+    *ierr = -1;
+    for (int itry = 1; itry <= ntry; ++itry) {
+      larnv_pm1<T>(rsize, hx);
+      PK_HIP(hipMemcpyAsync(x, hx, sizeof(T) * rsize, hipMemcpyHostToDevice, strm));
+      double nrm;
+      PK_TRY(nrm2(x, rsize, &nrm));
+      double s2, sd;
+      PK_TRY(apply(tn ? 'n' : 't', x, u0, nullptr, 0.0, &s2, &sd));
+      e->cnt.nopx++;
+      *u0norm = std::sqrt(s2);
+      *anormest = *u0norm / nrm;
+      if (j >= 1) {
+        int index[4] = {0, 1, j, j + 1};
+        PK_TRY(reorth(usize, j, basis, ld, u0, u0norm, index, kap));
+      }
+      if (*u0norm > 0) {
+        *ierr = 0;
+        break;
+      }
+    }
+    return PK_OK;
+  }
+
+  // dlanbpro (double/dlanbpro.F:1-549).  alpha_/beta_ are B(:,1), B(:,2).
+  int lanbpro(int k0, int* kp, double* alpha_, double* beta_, double* rnorm, double* doption,
+              const int* ioption, int* ierr_out) {
+    const double one = 1.0, zero = 0.0, FUDGE = 1.01, kappa = 0.717;
+    const int m = e->m, n = e->n;
+    int k = *kp, ierr = 0;
+    const double eps = Mach<T>::eps;
+    const double epsn = static_cast<double>(std::max(m, n)) * eps;
+    const double epsn2 = std::sqrt(static_cast<double>(std::max(m, n))) * eps;
+    double delta, eta, anorm, anormest = 0, s = 0, a1, b1, amax = 0, alpha, beta;
+    bool full_reorth, force_reorth;
+    int j0;
+    double* B1 = alpha_ - 1;  // B(j,1) = B1[j]
+    double* B2 = beta_ - 1;   // B(j,2) = B2[j]
+    const int elr = ioption[1];
+
+    if (k > e->kmax || k0 < 0 || k0 > k) {
+      set_error("lanbpro: k=%d k0=%d outside the engine's kmax=%d", k, k0, e->kmax);
+      return PK_ERR_ARG;
+    }
+    delta = doption[0] < zero ? std::sqrt(eps / k) : doption[0];
+    eta = doption[1] < zero ? std::pow(eps, 0.75) / std::sqrt(static_cast<double>(k)) : doption[1];
+    full_reorth = (delta <= eta || delta == zero);
+    if (doption[2] > zero) {
+      anorm = doption[2];
+    } else if (k0 > 0) {
+      anorm = lapy2(B1[1], B2[1]);
+      if (anorm <= zero) {
+        *ierr_out = -1;
+        doption[2] = anorm;
+        return PK_OK;
+      }
+    } else {
+      anorm = zero;
+    }
+    ierr = 0;
+
+    if (*rnorm == zero) {
+      int e2;
+      PK_TRY(getu0('n', k0, 3, ucol(k0 + 1), rnorm, U(), e->ldu, &anormest, &e2));
+      ierr = e2;
+      anorm = std::max(anorm, anormest);
+    }
+
+    std::vector<double> mu(k + 3, 0.0), nu(k + 3, 0.0);
+    std::vector<int> index(2 * k + 4, 0);
+
+    if (k0 == 0) {
+      amax = zero;
+      alpha = zero;
+      beta = *rnorm;
+      force_reorth = false;
+      int e2;
+      if (n > m)
+        PK_TRY(getu0('n', 0, 1, scratch(1), &s, U(), e->ldu, &anormest, &e2));
+      else
+        PK_TRY(getu0('t', 0, 1, scratch(1), &s, V(), e->ldv, &anormest, &e2));
+      ierr = e2;
+      anorm = std::max(anorm, FUDGE * anormest);
+      j0 = 1;
+      if (beta != zero) PK_TRY(safescal(m, beta, ucol(1)));
+      mu[1] = one;
+      nu[1] = one;
+    } else {
+      force_reorth = true;
+      alpha = B1[k0];
+      beta = *rnorm;
+      if (k0 < k && beta * delta < anorm * eps) {
+        full_reorth = true;
+        ierr = k0;
+      }
+      index[1] = 1;
+      index[2] = k0;
+      index[3] = k0 + 1;
+      PK_TRY(scale(ucol(k0 + 1), m, *rnorm));
+      PK_TRY(reorth(m, k0, U(), e->ldu, ucol(k0 + 1), rnorm, index.data(), kappa));
+      PK_TRY(safescal(m, *rnorm, ucol(k0 + 1)));
+      set_mu(k0, mu.data(), index.data(), epsn2);
+      set_mu(k0, nu.data(), index.data(), epsn2);
+      beta = *rnorm;
+      B2[k0] = beta;
+      amax = zero;
+      for (int j = 1; j <= k0; ++j) {
+        amax = std::max(amax, std::max(B1[j], B2[j]));
+        if (j == 1) {
+          anorm = std::max(anorm, FUDGE * alpha);
+        } else if (j == 2) {
+          a1 = B2[1] / amax;
+          a1 = FUDGE * amax * std::sqrt((B1[1] / amax) * (B1[1] / amax) + a1 * a1 + B1[2] / amax * a1);
+          anorm = std::max(anorm, a1);
+        } else {
+          a1 = B1[j - 1] / amax;
+          b1 = B2[j - 1] / amax;
+          a1 = FUDGE * amax * std::sqrt(a1 * a1 + b1 * b1 + a1 * B2[j - 2] / amax + B1[j] / amax * b1);
+          anorm = std::max(anorm, a1);
+        }
+      }
+      j0 = k0 + 1;
+    }
+    double numax = zero, mumax = zero;
+
+    for (int j = j0; j <= k; ++j) {
+      // alpha_j v_j = A^T u_j - beta_j v_{j-1}   (dlanbpro.F:288-316),
+      // fused: SpMV + recurrence + ||v_j||^2 + v_{j-1}.v_j in one pass.
+      double s2, sd;
+      PK_TRY(apply('t', ucol(j), vcol(j), j > 1 ? vcol(j - 1) : nullptr, j > 1 ? beta : 0.0, &s2,
+                   &sd));
+      e->cnt.nopx++;
+      if (j == 1) {
+        alpha = std::sqrt(s2);
+        anorm = std::max(anorm, FUDGE * alpha);
+      } else {
+        alpha = std::sqrt(s2);
+        if (elr > 0 && alpha < kappa * beta) {
+          double dt = sd;
+          for (int i = 1; i <= elr; ++i) {
+            s = dt;
+            double ns2, nsd;
+            const int nb = launch_axpy_norm_dot<T>(strm, n, vcol(j), vcol(j - 1), sval(s),
+                                                   e->part, nullptr);
+            PK_LAUNCH(nb);
+            PK_LAUNCH(launch_reduce(strm, 2, nb, e->part, e->red, nullptr));
+            PK_TRY(read(0, 2));
+            ns2 = e->hred[0];
+            nsd = e->hred[1];
+            if (beta != zero) {
+              beta = beta + s;
+              B2[j - 1] = beta;
+            }
+            s = std::sqrt(ns2);
+            dt = nsd;
+            if (s >= kappa * alpha) break;
+            alpha = s;
+          }
+          nu[j - 1] = eps;
+          alpha = s;
+        }
+        B1[j] = alpha;
+        amax = std::max(amax, alpha);
+        if (j == 2) {
+          a1 = B2[1] / amax;
+          a1 = FUDGE * amax * std::sqrt((B1[1] / amax) * (B1[1] / amax) + a1 * a1 + B1[2] / amax * a1);
+        } else {
+          a1 = B1[j - 1] / amax;
+          b1 = B2[j - 1] / amax;
+          a1 = FUDGE * amax * std::sqrt(a1 * a1 + b1 * b1 + a1 * B2[j - 2] / amax + B1[j] / amax * b1);
+        }
+        anorm = std::max(anorm, a1);
+      }
+      // nu recurrence and reorthogonalization of v_j (dlanbpro.F:340-367)
+      if (!full_reorth && alpha != zero)
+        update_nu(&numax, mu.data(), nu.data(), j, B1, B2, anorm, epsn2);
+      if ((full_reorth || numax > delta || force_reorth) && alpha != zero) {
+        if (full_reorth || eta == zero) {
+          index[1] = 1;
+          index[2] = j - 1;
+          index[3] = j;
+        } else if (!force_reorth) {
+          compute_int(nu.data(), j - 1, delta, eta, index.data());
+        }
+        PK_TRY(reorth(n, j - 1, V(), e->ldv, vcol(j), &alpha, index.data(), kappa));
+        set_mu(j - 1, nu.data(), index.data(), eps);
+        numax = eta;
+        force_reorth = !force_reorth;
+      }
+      // invariant subspace for alpha (dlanbpro.F:372-409)
+      if (alpha < anorm * epsn && j < k) {
+        *rnorm = alpha;
+        alpha = zero;
+        int e2;
+        PK_TRY(getu0('t', j - 1, 3, vcol(j), &alpha, V(), e->ldv, &anormest, &e2));
+        ierr = e2;
+        if (alpha == zero) {
+          k = j - 1;
+          ierr = -j;
+          goto done;
+        }
+        PK_TRY(safescal(n, alpha, vcol(j)));
+        alpha = zero;
+        force_reorth = true;
+        if (delta > zero) full_reorth = false;
+      } else if (j > 1 && !full_reorth && j < k && (delta * alpha < anorm * eps)) {
+        ierr = j;
+      }
+      B1[j] = alpha;
+      if (alpha != zero) PK_TRY(safescal(n, alpha, vcol(j)));
+
+      // beta_{j+1} u_{j+1} = A v_j - alpha_j u_j   (dlanbpro.F:420-443)
+      PK_TRY(apply('n', vcol(j), ucol(j + 1), ucol(j), alpha, &s2, &sd));
+      e->cnt.nopx++;
+      beta = std::sqrt(s2);
+      if (elr > 0 && beta < kappa * alpha) {
+        double dt = sd;
+        for (int i = 1; i <= elr; ++i) {
+          s = dt;
+          const int nb = launch_axpy_norm_dot<T>(strm, m, ucol(j + 1), ucol(j), sval(s),
+                                                 e->part, nullptr);
+          PK_LAUNCH(nb);
+          PK_LAUNCH(launch_reduce(strm, 2, nb, e->part, e->red, nullptr));
+          PK_TRY(read(0, 2));
+          const double ns2 = e->hred[0], nsd = e->hred[1];
+          if (alpha != zero) {
+            alpha = alpha + s;
+            B1[j] = alpha;
+          }
+          s = std::sqrt(ns2);
+          dt = nsd;
+          if (s >= kappa * beta) break;
+          beta = s;
+        }
+        mu[j] = eps;
+        beta = s;
+      }
+      B2[j] = beta;
+      amax = std::max(amax, beta);
+      if (j <= 1) {
+        a1 = lapy2(B1[1], B2[1]);
+      } else {
+        a1 = B1[j] / amax;
+        a1 = amax * std::sqrt(a1 * a1 + (B2[j] / amax) * (B2[j] / amax) + a1 * B2[j - 1] / amax);
+      }
+      anorm = std::max(anorm, a1);
+
+      // mu recurrence and reorthogonalization of u_{j+1} (dlanbpro.F:451-498)
+      if (!full_reorth && beta != zero)
+        update_mu(&mumax, mu.data(), nu.data(), j, B1, B2, anorm, epsn2);
+      if ((full_reorth || mumax > delta || force_reorth) && beta != zero) {
+        if (full_reorth || eta == zero) {
+          index[1] = 1;
+          index[2] = j;
+          index[3] = j + 1;
+        } else if (!force_reorth) {
+          compute_int(mu.data(), j, delta, eta, index.data());
+        } else {
+          for (int i = 1; i <= 2 * j + 1; ++i)
+            if (index[i] == j) {
+              index[i] = j + 1;
+              break;
+            }
+        }
+        PK_TRY(reorth(m, j, U(), e->ldu, ucol(j + 1), &beta, index.data(), kappa));
+        set_mu(j, mu.data(), index.data(), eps);
+        mumax = eta;
+        force_reorth = !force_reorth;
+      }
+      // invariant subspace for beta (dlanbpro.F:503-539)
+      if (beta < anorm * epsn && j < k) {
+        *rnorm = beta;
+        beta = zero;
+        int e2;
+        PK_TRY(getu0('n', j, 3, ucol(j + 1), &beta, U(), e->ldu, &anormest, &e2));
+        ierr = e2;
+        if (beta == zero) {
+          k = j;
+          ierr = -j;
+          goto done;
+        }
+        PK_TRY(safescal(m, beta, ucol(j + 1)));
+        beta = zero;
+        force_reorth = true;
+        if (delta > zero) full_reorth = false;
+      } else if (!full_reorth && j < k && (delta * beta < anorm * eps)) {
+        ierr = j;
+      }
+      B2[j] = beta;
+      if (beta != zero && beta != one) PK_TRY(safescal(m, beta, ucol(j + 1)));
+      *rnorm = beta;
+    }
+  done:
+    doption[2] = anorm;
+    *kp = k;
+    *ierr_out = ierr;
+    PK_HIP(hipStreamSynchronize(strm));
+    return PK_OK;
+  }
+
+  // X(:, 1:ncols) = X(:, 1:kdim) * op(B) into the spare buffer, then swap.
+  int update(char which, char transb, int kdim, int ncols, const double* B, int ldb) {
+    const bool isu = (which == 'u' || which == 'U');
+    const bool tb = (transb == 't' || transb == 'T');
+    const int maxk = isu ? e->kmax + 1 : e->kmax;
+    if (kdim < 1 || ncols < 1 || kdim > maxk || ncols > maxk) {
+      set_error("update: kdim=%d ncols=%d outside [1, %d]", kdim, ncols, maxk);
+      return PK_ERR_ARG;
+    }
+    if (ldb < (tb ? ncols : kdim)) {
+      set_error("update: ldb=%d too small", ldb);
+      return PK_ERR_ARG;
+    }
+    void** buf = isu ? e->U : e->V;
+    int& cur = isu ? e->ucur : e->vcur;
+    const long long ld = isu ? e->ldu : e->ldv;
+    const long long L = isu ? e->m : e->n;
+    if (!buf[1 - cur]) {
+      PK_HIP(hipMalloc(&buf[1 - cur], e->tbytes * ld * maxk));
+    }
+    for (int j = 0; j < ncols; ++j)
+      for (int l = 0; l < kdim; ++l)
+        e->hB[static_cast<size_t>(j) * kdim + l] =
+            tb ? B[static_cast<size_t>(l) * ldb + j] : B[static_cast<size_t>(j) * ldb + l];
+    PK_HIP(hipMemcpyAsync(e->dB, e->hB, sizeof(double) * kdim * ncols, hipMemcpyHostToDevice, strm));
+    PK_LAUNCH(launch_gemm_tall<T>(strm, L, static_cast<const T*>(buf[cur]), ld, kdim, e->dB, kdim,
+                                  ncols, static_cast<T*>(buf[1 - cur]), ld));
+    PK_HIP(hipStreamSynchronize(strm));
+    cur = 1 - cur;
+    return PK_OK;
+  }
+};
+
+static Engine* as_engine(pk_lbd_t h) {
+  Engine* e = reinterpret_cast<Engine*>(h);
+  if (!e || e->magic != Engine::kMagic) {
+    set_error("not a live engine handle");
+    return nullptr;
+  }
+  return e;
+}
+
+CsrOp* as_csr(pk_csr_t h) {
+  CsrOp* op = reinterpret_cast<CsrOp*>(h);
+  if (!op || op->magic != CsrOp::kMagic) {
+    set_error("not a live CSR operator handle");
+    return nullptr;
+  }
+  return op;
+}
+
+static void engine_free(Engine* e) {
+  for (int i = 0; i < 2; ++i) {
+    (void)hipFree(e->U[i]);
+    (void)hipFree(e->V[i]);
+  }
+  (void)hipFree(e->part);
+  (void)hipFree(e->red);
+  (void)hipFree(e->dcols);
+  (void)hipFree(e->dB);
+  (void)hipFree(e->vec);
+  (void)hipHostFree(e->hred);
+  (void)hipHostFree(e->hcols);
+  (void)hipHostFree(e->hB);
+  (void)hipHostFree(e->hvec);
+  (void)hipHostFree(e->hx);
+  (void)hipHostFree(e->hy);
+  if (e->stream) (void)hipStreamDestroy(e->stream);
+  e->magic = 0;
+  delete e;
+}
+
+}  // namespace pk
+
+using namespace pk;
+
+#define PK_ENGINE(h)                  \
+  Engine* e = as_engine(h);           \
+  if (!e) return PK_ERR_HANDLE;
+
+extern "C" {
+
+const char* pk_last_error(void) { return pk::last_error(); }
+
+int pk_lbd_create(int prec, int m, int n, int kmax, pk_csr_t op, pk_lbd_t* out) {
+  if (!out || m < 1 || n < 1 || kmax < 1 || (prec != PK_F64 && prec != PK_F32)) {
+    set_error("pk_lbd_create: bad arguments m=%d n=%d kmax=%d prec=%d", m, n, kmax, prec);
+    return PK_ERR_ARG;
+  }
+  CsrOp* cop = nullptr;
+  if (op) {
+    cop = as_csr(op);
+    if (!cop) return PK_ERR_HANDLE;
+    if (cop->m != m || cop->n != n) {
+      set_error("pk_lbd_create: operator is %dx%d, engine %dx%d", cop->m, cop->n, m, n);
+      return PK_ERR_ARG;
+    }
+    if (cop->val_prec != prec) {
+      set_error("pk_lbd_create: operator values and basis must share a precision");
+      return PK_ERR_NOTSUP;
+    }
+  }
+  Engine* e = new Engine();
+  e->magic = Engine::kMagic;
+  e->prec = prec;
+  e->m = m;
+  e->n = n;
+  e->kmax = kmax;
+  e->tbytes = prec == PK_F64 ? 8 : 4;
+  e->ldu = round_up(m, 64);
+  e->ldv = round_up(n, 64);
+  e->op = cop;
+  const long long mx = std::max(m, n);
+  const size_t nblk = static_cast<size_t>(std::max(stream_blocks(mx), kMaxBlocks));
+  const size_t nb = static_cast<size_t>(kmax + 1) * (kmax + 1);
+  hipError_t err = hipSuccess;
+  auto chk = [&](hipError_t r) {
+    if (r != hipSuccess && err == hipSuccess) err = r;
+  };
+  chk(hipStreamCreateWithFlags(&e->stream, hipStreamNonBlocking));
+  chk(hipMalloc(&e->U[0], e->tbytes * e->ldu * (kmax + 1)));
+  chk(hipMalloc(&e->V[0], e->tbytes * e->ldv * kmax));
+  chk(hipMalloc(&e->part, sizeof(double) * nblk * kMaxCols));
+  chk(hipMalloc(&e->red, sizeof(double) * (kMaxCols + 16)));
+  chk(hipMalloc(&e->dcols, sizeof(int) * kMaxCols));
+  chk(hipMalloc(&e->dB, sizeof(double) * nb));
+  chk(hipMalloc(&e->vec, e->tbytes * mx * 2));
+  chk(hipHostMalloc(&e->hred, sizeof(double) * (kMaxCols + 16), hipHostMallocDefault));
+  chk(hipHostMalloc(&e->hcols, sizeof(int) * kMaxCols, hipHostMallocDefault));
+  chk(hipHostMalloc(&e->hB, sizeof(double) * nb, hipHostMallocDefault));
+  chk(hipHostMalloc(&e->hvec, sizeof(double) * mx, hipHostMallocDefault));
+  if (err == hipSuccess) {
+    // The bases start zeroed so that unwritten columns never hold NaN garbage.
+    chk(hipMemsetAsync(e->U[0], 0, e->tbytes * e->ldu * (kmax + 1), e->stream));
+    chk(hipMemsetAsync(e->V[0], 0, e->tbytes * e->ldv * kmax, e->stream));
+    chk(hipStreamSynchronize(e->stream));
+  }
+  if (err != hipSuccess) {
+    set_error("pk_lbd_create: %s", hipGetErrorString(err));
+    engine_free(e);
+    return err == hipErrorOutOfMemory ? PK_ERR_OOM : PK_ERR_HIP;
+  }
+  *out = reinterpret_cast<pk_lbd_t>(e);
+  return PK_OK;
+}
+
+int pk_lbd_destroy(pk_lbd_t h) {
+  PK_ENGINE(h);
+  (void)hipStreamSynchronize(e->stream);
+  engine_free(e);
+  return PK_OK;
+}
+
+int pk_lbd_set_callback(pk_lbd_t h, void* aprod, int aprod_prec, void* dparm, int* iparm) {
+  PK_ENGINE(h);
+  if (!aprod || (aprod_prec != PK_F64 && aprod_prec != PK_F32)) return PK_ERR_ARG;
+  e->aprod = aprod;
+  e->aprod_prec = aprod_prec;
+  e->dparm = dparm;
+  e->iparm = iparm;
+  if (!e->hx) {
+    const size_t b = sizeof(double) * std::max(e->m, e->n);
+    if (hipHostMalloc(&e->hx, b, hipHostMallocDefault) != hipSuccess ||
+        hipHostMalloc(&e->hy, b, hipHostMallocDefault) != hipSuccess) {
+      set_error("pk_lbd_set_callback: pinned allocation failed");
+      return PK_ERR_OOM;
+    }
+  }
+  return PK_OK;
+}
+
+int pk_lbd_put(pk_lbd_t h, char which, int col, const void* x, int prec) {
+  PK_ENGINE(h);
+  const bool isu = (which == 'u' || which == 'U');
+  const int maxc = isu ? e->kmax + 1 : e->kmax;
+  if (col < 1 || col > maxc || !x) return PK_ERR_ARG;
+  const long long L = isu ? e->m : e->n;
+  char* dst = static_cast<char*>(isu ? e->u() : e->v()) +
+              e->tbytes * static_cast<size_t>(col - 1) * (isu ? e->ldu : e->ldv);
+  const size_t sb = prec == PK_F64 ? 8 : 4;
+  if (sb == e->tbytes) {
+    PK_HIP(hipMemcpyAsync(dst, x, sb * L, hipMemcpyHostToDevice, e->stream));
+  } else {
+    PK_HIP(hipMemcpyAsync(e->vec, x, sb * L, hipMemcpyHostToDevice, e->stream));
+    int r = e->prec == PK_F64
+                ? launch_convert<float, double>(e->stream, L, static_cast<const float*>(e->vec),
+                                                reinterpret_cast<double*>(dst))
+                : launch_convert<double, float>(e->stream, L, static_cast<const double*>(e->vec),
+                                                reinterpret_cast<float*>(dst));
+    if (r < 0) return PK_ERR_HIP;
+  }
+  PK_HIP(hipStreamSynchronize(e->stream));
+  return PK_OK;
+}
+
+int pk_lbd_get(pk_lbd_t h, char which, int ncols, void* X, int ldx, int prec) {
+  PK_ENGINE(h);
+  const bool isu = (which == 'u' || which == 'U');
+  const int maxc = isu ? e->kmax + 1 : e->kmax;
+  const long long L = isu ? e->m : e->n;
+  if (ncols < 0 || ncols > maxc || (ncols > 0 && !X) || ldx < L) return PK_ERR_ARG;
+  const size_t sb = prec == PK_F64 ? 8 : 4;
+  const char* src = static_cast<const char*>(isu ? e->u() : e->v());
+  const long long ld = isu ? e->ldu : e->ldv;
+  for (int j = 0; j < ncols; ++j) {
+    const char* cs = src + e->tbytes * static_cast<size_t>(j) * ld;
+    char* cd = static_cast<char*>(X) + sb * static_cast<size_t>(j) * ldx;
+    if (sb == e->tbytes) {
+      PK_HIP(hipMemcpyAsync(cd, cs, sb * L, hipMemcpyDeviceToHost, e->stream));
+    } else {
+      int r = e->prec == PK_F64
+                  ? launch_convert<double, float>(e->stream, L, reinterpret_cast<const double*>(cs),
+                                                  static_cast<float*>(e->vec))
+                  : launch_convert<float, double>(e->stream, L, reinterpret_cast<const float*>(cs),
+                                                  static_cast<double*>(e->vec));
+      if (r < 0) return PK_ERR_HIP;
+      PK_HIP(hipMemcpyAsync(cd, e->vec, sb * L, hipMemcpyDeviceToHost, e->stream));
+      PK_HIP(hipStreamSynchronize(e->stream));
+    }
+  }
+  PK_HIP(hipStreamSynchronize(e->stream));
+  return PK_OK;
+}
+
+int pk_lbd_basis(pk_lbd_t h, char which, void** dptr, long long* ld) {
+  PK_ENGINE(h);
+  const bool isu = (which == 'u' || which == 'U');
+  if (dptr) *dptr = isu ? e->u() : e->v();
+  if (ld) *ld = isu ? e->ldu : e->ldv;
+  return PK_OK;
+}
+
+int pk_lbd_nrm2(pk_lbd_t h, char which, int col, double* nrm) {
+  PK_ENGINE(h);
+  const bool isu = (which == 'u' || which == 'U');
+  const int maxc = isu ? e->kmax + 1 : e->kmax;
+  if (col < 1 || col > maxc || !nrm) return PK_ERR_ARG;
+  if (e->prec == PK_F64) {
+    Impl<double> im(e);
+    return im.nrm2(isu ? im.ucol(col) : im.vcol(col), isu ? e->m : e->n, nrm);
+  }
+  Impl<float> im(e);
+  return im.nrm2(isu ? im.ucol(col) : im.vcol(col), isu ? e->m : e->n, nrm);
+}
+
+int pk_lbd_getu0(pk_lbd_t h, char transa, int j, int ntry, double* u0norm, int icgs,
+                 double* anormest, int* ierr) {
+  PK_ENGINE(h);
+  (void)icgs;  // the device always uses block classical Gram-Schmidt
+  const bool tn = (transa == 'n' || transa == 'N');
+  const int maxc = tn ? e->kmax + 1 : e->kmax;
+  if (j < 0 || j + 1 > maxc || ntry < 1 || !u0norm || !anormest || !ierr) return PK_ERR_ARG;
+  if (e->prec == PK_F64) {
+    Impl<double> im(e);
+    return im.getu0(transa, j, ntry, tn ? im.ucol(j + 1) : im.vcol(j + 1), u0norm,
+                    tn ? im.U() : im.V(), tn ? e->ldu : e->ldv, anormest, ierr);
+  }
+  Impl<float> im(e);
+  return im.getu0(transa, j, ntry, tn ? im.ucol(j + 1) : im.vcol(j + 1), u0norm,
+                  tn ? im.U() : im.V(), tn ? e->ldu : e->ldv, anormest, ierr);
+}
+
+int pk_lbd_lanbpro(pk_lbd_t h, int k0, int* k, double* alpha, double* beta, double* rnorm,
+                   double* doption, const int* ioption, int* ierr) {
+  PK_ENGINE(h);
+  if (!k || !alpha || !beta || !rnorm || !doption || !ioption || !ierr) return PK_ERR_ARG;
+  if (e->prec == PK_F64) return Impl<double>(e).lanbpro(k0, k, alpha, beta, rnorm, doption, ioption, ierr);
+  return Impl<float>(e).lanbpro(k0, k, alpha, beta, rnorm, doption, ioption, ierr);
+}
+
+int pk_lbd_update(pk_lbd_t h, char which, char transb, int kdim, int ncols, const double* B,
+                  int ldb) {
+  PK_ENGINE(h);
+  if (!B) return PK_ERR_ARG;
+  if (e->prec == PK_F64) return Impl<double>(e).update(which, transb, kdim, ncols, B, ldb);
+  return Impl<float>(e).update(which, transb, kdim, ncols, B, ldb);
+}
+
+int pk_lbd_stats(pk_lbd_t h, long long* out) {
+  PK_ENGINE(h);
+  if (!out) return PK_ERR_ARG;
+  out[0] = e->cnt.nopx;
+  out[1] = e->cnt.nreorth;
+  out[2] = e->cnt.ndot;
+  out[3] = e->cnt.ncols_touched;
+  out[4] = e->cnt.nsync;
+  return PK_OK;
+}
+
+int pk_lbd_reset_stats(pk_lbd_t h) {
+  PK_ENGINE(h);
+  e->cnt = Counters();
+  return PK_OK;
+}
+
+int pk_lbd_shape(pk_lbd_t h, int* m, int* n, int* kmax, int* prec) {
+  PK_ENGINE(h);
+  if (m) *m = e->m;
+  if (n) *n = e->n;
+  if (kmax) *kmax = e->kmax;
+  if (prec) *prec = e->prec;
+  return PK_OK;
+}
+
+int pk_lbd_sync(pk_lbd_t h) {
+  PK_ENGINE(h);
+  PK_HIP(hipStreamSynchronize(e->stream));
+  return PK_OK;
+}
+
+}  // extern "C"
+
+// Host LAPACK (the small bidiagonal SVDs of the drivers) runs single-threaded:
+// select MKL's sequential layer before its first use, so that the library
+// never brings a second OpenMP runtime into the caller's process.
+extern "C" int MKL_Set_Threading_Layer(int);
+__attribute__((constructor)) static void pk_select_sequential_lapack() {
+  MKL_Set_Threading_Layer(1 /* MKL_THREADING_SEQUENTIAL */);
+}
+
+// ----------------------------------------------------------------------------
+// Host scalar routines of the engine, exported for verification against the
+// reference (arrays 1-based as in the reference: element 0 is ignored).
+extern "C" {
+
+int pk_larnv_pm1(int prec, long long n, void* x) {
+  if (n < 0 || !x) return PK_ERR_ARG;
+  if (prec == PK_F64) larnv_pm1<double>(n, static_cast<double*>(x));
+  else if (prec == PK_F32) larnv_pm1<float>(n, static_cast<float*>(x));
+  else return PK_ERR_ARG;
+  return PK_OK;
+}
+
+void pkh_update_mu(double* mumax, double* mu, const double* nu, int j, const double* alpha,
+                   const double* beta, double anorm, double eps1) {
+  update_mu(mumax, mu, nu, j, alpha, beta, anorm, eps1);
+}
+
+void pkh_update_nu(double* numax, const double* mu, double* nu, int j, const double* alpha,
+                   const double* beta, double anorm, double eps1) {
+  update_nu(numax, mu, nu, j, alpha, beta, anorm, eps1);
+}
+
+void pkh_compute_int(const double* mu, int j, double delta, double eta, int* index) {
+  compute_int(mu, j, delta, eta, index);
+}
+
+void pkh_set_mu(int k, double* mu, const int* index, double val) { set_mu(k, mu, index, val); }
+
+}  // extern "C"

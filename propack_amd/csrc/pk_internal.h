@@ -1,0 +1,79 @@
+// Internal structures of libpropack_amd: the device CSR operator and the
+// Lanczos bidiagonalization engine context.
+#pragma once
+
+#include <vector>
+
+#include "../../include/propack_amd.h"
+#include "pk_kernels.h"
+
+namespace pk {
+
+struct CsrOp {
+  static constexpr unsigned kMagic = 0x504b4353u;  // "PKCS"
+  unsigned magic = 0;
+  int m = 0, n = 0;
+  long long nnz = 0;
+  int val_prec = PK_F64;
+  size_t val_bytes = 8;
+  int *rp = nullptr, *ci = nullptr;   // A
+  void* va = nullptr;
+  int *rpt = nullptr, *cit = nullptr; // A^T (explicit)
+  void* vat = nullptr;
+  int lpr_n = 8, lpr_t = 8;           // lanes per row for A and A^T
+};
+
+int csr_create(int m, int n, long long nnz, const int* rowptr, const int* colind,
+               const void* val, int val_prec, int where, CsrOp** out);
+void csr_destroy(CsrOp* op);
+template <typename T>
+int csr_apply(const CsrOp* op, hipStream_t s, char trans, const T* x, T* y, const T* yprev,
+              Scal coef, double* part, const int* gate);
+
+// Fortran APROD callback: (TRANSA, M, N, X, Y, DPARM, IPARM, len(TRANSA)).
+typedef void (*aprod_d_fn)(const char*, const int*, const int*, const double*, double*,
+                           double*, int*, size_t);
+typedef void (*aprod_s_fn)(const char*, const int*, const int*, const float*, float*, float*,
+                           int*, size_t);
+
+struct Counters {
+  long long nopx = 0, nreorth = 0, ndot = 0, ncols_touched = 0, nsync = 0;
+};
+
+struct Engine {
+  static constexpr unsigned kMagic = 0x504b4c42u;  // "PKLB"
+  unsigned magic = 0;
+  int prec = PK_F64;                 // storage type of bases (and of the operator values)
+  int m = 0, n = 0, kmax = 0;
+  long long ldu = 0, ldv = 0;
+  void* U[2] = {nullptr, nullptr};   // current / spare left basis  (ldu x (kmax+1))
+  void* V[2] = {nullptr, nullptr};   // current / spare right basis (ldv x kmax)
+  int ucur = 0, vcur = 0;
+  size_t tbytes = 8;
+  // operator: device CSR, or a host APROD callback
+  CsrOp* op = nullptr;
+  void* aprod = nullptr;
+  int aprod_prec = PK_F64;           // precision of the callback's X/Y arrays
+  void* dparm = nullptr;
+  int* iparm = nullptr;
+  void* hx = nullptr;                // pinned staging for the callback path
+  void* hy = nullptr;
+  // device work
+  hipStream_t stream = nullptr;
+  double* part = nullptr;            // partial sums (max_blocks * kMaxCols)
+  double* red = nullptr;             // reduced sums (kMaxCols + 16)
+  double* hred = nullptr;            // pinned host mirror of red
+  int* dcols = nullptr;              // reorthogonalization column list
+  int* hcols = nullptr;              // pinned host staging of the list
+  double* dB = nullptr;              // small dense matrix for the tall GEMM
+  double* hB = nullptr;              // pinned host staging
+  void* vec = nullptr;               // scratch vector, max(m, n)
+  double* hvec = nullptr;            // pinned host scratch (random start vectors)
+  Counters cnt;
+  int last_error = 0;
+
+  void* u() const { return U[ucur]; }
+  void* v() const { return V[vcur]; }
+};
+
+}  // namespace pk

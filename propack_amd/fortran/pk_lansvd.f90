@@ -1,0 +1,473 @@
+! Host drivers of the truncated SVD: the outer convergence / restart loops
+! of DLANSVD (double/dlansvd.F:1-291), DLANSVD_IRL (double/dlansvd_irl.F:1-419)
+! and the Ritz-vector assembly of DRITZVEC (double/dritzvec.F:1-200), written
+! against the device engine of libpropack_amd.  The Lanczos steps, the
+! bases and the basis-update GEMMs run on the GPU; the small bidiagonal
+! problems (QR, LAPACK DBDSQR / DBDSDC, shifted sweeps, bound refinement)
+! stay on the host in double precision.
+!
+! Entry points:
+!   pk_lansvd_run / pk_lansvd_irl_run  (bind(C)) -- device-resident drivers on
+!       a caller-created engine: the start vector is the engine's U(:,1), the
+!       singular vectors are left in the engine's bases.
+!   dlansvd, dlansvd_irl, slansvd, slansvd_irl -- the reference's Fortran
+!       calling sequences (see module pk_dropin below).
+module pk_drivers
+  use iso_c_binding
+  use pk_api
+  implicit none
+  private
+  public :: lansvd_core, lansvd_irl_core, add_timing
+
+  ! External LAPACK (host): bidiagonal SVD.
+  interface
+    subroutine dbdsqr(uplo, n, ncvt, nru, ncc, d, e, vt, ldvt, u, ldu, c, ldc, work, info)
+      character(len=1) :: uplo
+      integer :: n, ncvt, nru, ncc, ldvt, ldu, ldc, info
+      double precision :: d(*), e(*), vt(ldvt, *), u(ldu, *), c(ldc, *), work(*)
+    end subroutine
+    subroutine dbdsdc(uplo, compq, n, d, e, u, ldu, vt, ldvt, q, iq, work, iwork, info)
+      character(len=1) :: uplo, compq
+      integer :: n, ldu, ldvt, iq(*), iwork(*), info
+      double precision :: d(*), e(*), u(ldu, *), vt(ldvt, *), q(*), work(*)
+    end subroutine
+  end interface
+
+contains
+
+  ! Accumulate the engine's counters into COMMON /timing/ (the reference's
+  ! double/stat.h:12-15 layout) so that PRINTSTAT-style callers see them.
+  subroutine add_timing(eng, nbsvd_add, nrestart_add, nlandim_set, nsing_set)
+    type(c_ptr), intent(in) :: eng
+    integer, intent(in) :: nbsvd_add, nrestart_add, nlandim_set, nsing_set
+    integer :: nopx, nreorth, ndot, nreorthu, nreorthv, nitref, nrestart, nbsvd, nlandim, nsing
+    real :: tmvopx, tgetu0, tupdmu, tupdnu, tintv, tlanbpro, treorth, treorthu, treorthv, &
+            telru, telrv, tbsvd, tnorm2, tlansvd, tritzvec, trestart, tdot
+    common /timing/ nopx, nreorth, ndot, nreorthu, nreorthv, nitref, nrestart, nbsvd, tmvopx, &
+        tgetu0, tupdmu, tupdnu, tintv, tlanbpro, treorth, treorthu, treorthv, telru, telrv, &
+        tbsvd, tnorm2, tlansvd, nlandim, tritzvec, trestart, tdot, nsing
+    integer(c_long_long) :: st(5)
+    integer(c_int) :: rc
+    rc = pk_lbd_stats(eng, st)
+    if (rc == PK_OK) then
+      nopx = nopx + int(st(1))
+      nreorth = nreorth + int(st(2))
+      ndot = ndot + int(st(3))
+    end if
+    nbsvd = nbsvd + nbsvd_add
+    nrestart = nrestart + nrestart_add
+    if (nlandim_set >= 0) nlandim = nlandim_set
+    if (nsing_set >= 0) nsing = nsing_set
+    rc = pk_lbd_reset_stats(eng)
+  end subroutine
+
+  ! Ritz vectors from A V_dim = U_{dim+1} B_dim (DRITZVEC): B = M [R; 0]
+  ! by Givens QR, R = P S Q^T by divide and conquer (DBDSDC), then
+  ! U <- U (P^T M^T)^T rows, V <- V Q on the device.
+  subroutine ritzvec(which, jobu, jobv, m, n, k, dim, d, e, eng, info)
+    character(len=1), intent(in) :: which, jobu, jobv
+    integer, intent(in) :: m, n, k, dim
+    double precision, intent(inout) :: d(*), e(*)
+    type(c_ptr), intent(in) :: eng
+    integer, intent(out) :: info
+    double precision, allocatable :: mt(:, :), qt(:, :), p(:, :), x(:, :), wrk(:)
+    integer, allocatable :: iwk(:)
+    double precision :: c1, c2, dd(1)
+    integer :: id(1), mstart, lapinfo
+    integer(c_int) :: rc
+    info = 0
+    if (k < 1) return
+    allocate(mt(dim + 1, dim + 1), qt(dim, dim), p(dim, dim), x(dim + 1, dim + 1))
+    allocate(wrk(3 * dim * dim + 4 * dim + 8), iwk(8 * dim + 8))
+    mt = 0d0; qt = 0d0; p = 0d0; x = 0d0
+    c1 = 0d0; c2 = 0d0
+    call pk_bdqr(dim == min(m, n), jobu, dim, d, e, c1, c2, mt, dim + 1)
+    call dbdsdc('U', 'I', dim, d, e, p, dim, qt, dim, dd, id, wrk, iwk, lapinfo)
+    ! X(1:dim, 1:dim+1) = P^T * M^T(1:dim, 1:dim+1)
+    x(1:dim, 1:dim + 1) = matmul(transpose(p), mt(1:dim, 1:dim + 1))
+    if (lower1(which) == 's') then
+      mstart = dim - k + 1
+    else
+      mstart = 1
+    end if
+    if (lower1(jobu) == 'y') then
+      ! U(:,1:k) <- U(:,1:dim+1) * X(mstart:mstart+k-1, :)^T
+      rc = pk_lbd_update(eng, 'u', 't', dim + 1, k, x(mstart, 1), dim + 1)
+      if (rc /= PK_OK) info = rc
+    end if
+    if (lower1(jobv) == 'y' .and. info == 0) then
+      ! V(:,1:k) <- V(:,1:dim) * Q^T(mstart:mstart+k-1, :)^T
+      rc = pk_lbd_update(eng, 'v', 't', dim, k, qt(mstart, 1), dim)
+      if (rc /= PK_OK) info = rc
+    end if
+  end subroutine
+
+  ! Non-restarted driver (DLANSVD control flow, double/dlansvd.F:130-291)
+  ! on an engine whose U(:,1) holds the start vector (zero => random).
+  ! eps is the unit roundoff of the engine's storage precision.
+  subroutine lansvd_core(jobu, jobv, m, n, k, kmax, eng, eps, sigma, bnd, tolin, doption, &
+                         ioption, info)
+    character(len=1), intent(in) :: jobu, jobv
+    integer, intent(in) :: m, n, kmax
+    integer, intent(inout) :: k
+    type(c_ptr), intent(in) :: eng
+    double precision, intent(in) :: eps, tolin
+    double precision, intent(inout) :: sigma(*), bnd(*), doption(*)
+    integer, intent(in) :: ioption(*)
+    integer, intent(out) :: info
+    double precision, allocatable :: alpha(:), beta(:), a1(:), b1(:), wb(:), wrk(:)
+    double precision :: eps34, epsn, tol, anorm, rnorm, dummy(1, 1)
+    integer :: lanmax, neig, jold, j, jj, dj, i, ierr, lapinfo, nbsvd
+    integer(c_int) :: rc, ierr_c, jj_c
+    logical :: vectors
+
+    eps34 = eps**(3d0 / 4d0)
+    epsn = dble(max(m, n)) * eps / 2d0
+    lanmax = min(n + 1, m + 1, kmax)
+    tol = min(1d0, max(16d0 * eps, tolin))
+    anorm = 0d0
+    info = 0
+    nbsvd = 0
+    allocate(alpha(lanmax), beta(lanmax), a1(lanmax), b1(lanmax), wb(lanmax + 1), &
+             wrk(4 * lanmax + 8))
+    alpha = 0d0; beta = 0d0
+
+    rc = pk_lbd_nrm2(eng, 'u', 1, rnorm)
+    if (rc /= PK_OK) then
+      info = rc
+      return
+    end if
+    if (rnorm == 0d0) then
+      rc = pk_lbd_getu0(eng, 'n', 0, 1, rnorm, ioption(1), anorm, ierr_c)
+      if (rc /= PK_OK) then
+        info = rc
+        return
+      end if
+    end if
+
+    neig = 0
+    jold = 0
+    j = min(k + max(8, k) + 1, lanmax)
+    ierr = 0
+    do while (neig < k)
+      jj_c = j
+      rc = pk_lbd_lanbpro(eng, jold, jj_c, alpha, beta, rnorm, doption, ioption, ierr_c)
+      if (rc /= PK_OK) then
+        info = rc
+        return
+      end if
+      j = jj_c
+      ierr = ierr_c
+      jold = j
+      ! SVD of B_j and error bounds (double/dlansvd.F:193-236)
+      a1 = alpha
+      b1 = beta
+      wb = 0d0
+      call pk_bdqr(j == min(m, n), 'N', j, a1, b1, wb(j), wb(j + 1), dummy, 1)
+      call dbdsqr('U', j, 0, 1, 0, a1, b1, dummy, 1, wb, 1, dummy, 1, wrk, lapinfo)
+      nbsvd = nbsvd + 1
+      if (j > 5) then
+        anorm = a1(1)
+      else
+        anorm = max(anorm, a1(1))
+      end if
+      wb(1:j) = abs(rnorm * wb(1:j))
+      call pk_refinebounds(min(m, n), j, a1, wb, epsn * anorm, eps34)
+      do i = 1, min(j, k)
+        bnd(i) = wb(i)
+      end do
+      neig = 0
+      i = 0
+      do while (i < min(j, k))
+        if (wb(i + 1) <= tol * a1(i + 1)) then
+          neig = neig + 1
+          sigma(neig) = a1(i + 1)
+          i = i + 1
+        else
+          exit
+        end if
+      end do
+      if (ierr < 0) then
+        if (j < k) info = j
+        exit
+      end if
+      if (j >= lanmax) then
+        if (neig < k) info = -1
+        exit
+      end if
+      ! grow the Krylov dimension (double/dlansvd.F:266-275)
+      if (neig > 1) then
+        dj = min(j / 2, ((k - neig) * (j - 6)) / (2 * neig + 1))
+        dj = min(100, max(2, dj))
+      else
+        dj = j / 2
+        dj = min(100, max(10, dj))
+      end if
+      j = min(j + dj, lanmax)
+    end do
+
+    vectors = lower1(jobu) == 'y' .or. lower1(jobv) == 'y'
+    if ((neig >= k .or. info > 0) .and. vectors) then
+      call ritzvec('L', jobu, jobv, m, n, neig, jold, alpha, beta, eng, jj)
+      if (jj /= 0) info = jj
+    end if
+    k = neig
+    call add_timing(eng, nbsvd, 0, j, -1)
+  end subroutine
+
+  ! Implicitly restarted driver (DLANSVD_IRL control flow,
+  ! double/dlansvd_irl.F:164-419).  dim and neig are in/out as in the
+  ! reference.  The engine must have room for dim V columns.
+  subroutine lansvd_irl_core(which, jobu, jobv, m, n, dim, p, neig, maxiter, eng, eps, &
+                             sigma, bnd, tolin, doption, ioption, info)
+    character(len=1), intent(in) :: which, jobu, jobv
+    integer, intent(in) :: m, n, p, maxiter
+    integer, intent(inout) :: dim, neig
+    type(c_ptr), intent(in) :: eng
+    double precision, intent(in) :: eps, tolin
+    double precision, intent(inout) :: sigma(*), bnd(*), doption(*)
+    integer, intent(in) :: ioption(*)
+    integer, intent(out) :: info
+    double precision, allocatable :: alpha(:), beta(:), a1(:), b1(:), wb(:), shift(:), &
+                                     pm(:, :), qm(:, :), wrk(:)
+    double precision :: eps34, epsn, tol, anorm, rnorm, relgap, dummy(1, 1)
+    integer :: k, i, iter, nconv, kold, nshft, ierr, lapinfo, nbsvd, nrestart, jj
+    integer(c_int) :: rc, ierr_c, kk_c
+    logical :: small, vectors
+
+    eps34 = eps**(3d0 / 4d0)
+    epsn = dble(max(m, n)) * eps / 2d0
+    dim = min(dim, n + 1, m + 1)
+    k = dim - p
+    tol = min(1d0, max(16d0 * eps, tolin))
+    anorm = 0d0
+    info = 0
+    nbsvd = 0
+    nrestart = 0
+    small = lower1(which) == 's'
+    allocate(alpha(dim), beta(dim), a1(dim), b1(dim), wb(dim + 1), shift(dim), &
+             pm(dim + 1, dim + 1), qm(dim, dim), wrk(4 * dim + 8))
+    alpha = 0d0; beta = 0d0
+
+    rc = pk_lbd_nrm2(eng, 'u', 1, rnorm)
+    if (rc /= PK_OK) then
+      info = rc
+      return
+    end if
+    if (rnorm == 0d0) then
+      rc = pk_lbd_getu0(eng, 'n', 0, 1, rnorm, ioption(1), anorm, ierr_c)
+      if (rc /= PK_OK) then
+        info = rc
+        return
+      end if
+    end if
+
+    iter = 0
+    nconv = 0
+    kold = 0
+    ierr = 0
+    do while (nconv < neig .and. iter < maxiter)
+      kk_c = dim
+      rc = pk_lbd_lanbpro(eng, kold, kk_c, alpha, beta, rnorm, doption, ioption, ierr_c)
+      if (rc /= PK_OK) then
+        info = rc
+        return
+      end if
+      ierr = ierr_c
+      kold = k
+      ! SVD of B_dim and error bounds (double/dlansvd_irl.F:220-251)
+      a1 = alpha
+      b1 = beta
+      wb = 0d0
+      call pk_bdqr(dim == min(m, n), 'N', dim, a1, b1, wb(dim), wb(dim + 1), dummy, 1)
+      call dbdsqr('U', dim, 0, 1, 0, a1, b1, dummy, 1, wb, 1, dummy, 1, wrk, lapinfo)
+      nbsvd = nbsvd + 1
+      if (dim > 5) then
+        anorm = a1(1)
+      else
+        anorm = max(anorm, a1(1))
+      end if
+      wb(1:dim) = abs(rnorm * wb(1:dim))
+      if (small) then
+        call pk_refinebounds(min(m, n), dim, a1, wb, epsn * anorm, eps34)
+      else
+        call pk_refinebounds(min(m, n), min(dim, neig), a1, wb, epsn * anorm, eps34)
+      end if
+      ! converged count (double/dlansvd_irl.F:262-290)
+      nconv = 0
+      if (small) then
+        do i = dim - neig + 1, dim
+          if (wb(i) <= tol * a1(1)) then
+            nconv = nconv + 1
+            sigma(nconv) = a1(i)
+            bnd(nconv) = wb(i)
+          end if
+        end do
+      else
+        i = 1
+        do while (i <= min(dim, neig))
+          if (wb(i) <= tol * a1(i)) then
+            nconv = nconv + 1
+            sigma(nconv) = a1(i)
+            bnd(nconv) = wb(i)
+            i = i + 1
+          else
+            exit
+          end if
+        end do
+      end if
+      if (ierr < 0) then
+        if (dim < k) info = dim
+        exit
+      end if
+      if (nconv < neig) then
+        ! exact shifts filtered by the relative gap (double/dlansvd_irl.F:318-344)
+        shift = 0d0
+        nshft = 0
+        if (small) then
+          do i = 1, k
+            relgap = a1(i) - wb(i) - a1(dim - neig)
+            nshft = nshft + 1
+            if (relgap > doption(4) * a1(dim - neig)) then
+              shift(nshft) = a1(i)
+            else
+              shift(nshft) = a1(1)
+            end if
+          end do
+        else
+          do i = dim, k + 1, -1
+            relgap = a1(k) - (a1(i) + wb(i))
+            nshft = nshft + 1
+            if (relgap > doption(4) * a1(k)) then
+              shift(nshft) = a1(i)
+            else
+              shift(nshft) = 0d0
+            end if
+          end do
+        end if
+        ! B+ = P B Q^T by dim-k shifted sweeps (double/dlansvd_irl.F:350-363)
+        pm = 0d0
+        qm = 0d0
+        do i = 1, dim + 1
+          pm(i, i) = 1d0
+        end do
+        do i = 1, dim
+          qm(i, i) = 1d0
+        end do
+        do i = dim, k + 1, -1
+          call pk_bsvdstep('y', 'y', dim + 1, dim, i, shift(dim - i + 1), alpha, beta, pm, &
+                           dim + 1, qm, dim)
+        end do
+        ! U(:,1:k+1) <- U(:,1:dim+1) P(:,1:k+1); V(:,1:k) <- V(:,1:dim) Q(:,1:k)
+        rc = pk_lbd_update(eng, 'u', 'n', dim + 1, k + 1, pm, dim + 1)
+        if (rc == PK_OK) rc = pk_lbd_update(eng, 'v', 'n', dim, k, qm, dim)
+        if (rc /= PK_OK) then
+          info = rc
+          return
+        end if
+        rnorm = beta(k)
+        nrestart = nrestart + 1
+      end if
+      iter = iter + 1
+    end do
+
+    vectors = lower1(jobu) == 'y' .or. lower1(jobv) == 'y'
+    if ((nconv >= neig .or. info > 0) .and. vectors) then
+      a1 = alpha
+      b1 = beta
+      call ritzvec(which, jobu, jobv, m, n, nconv, dim, a1, b1, eng, jj)
+      if (jj /= 0) info = jj
+    end if
+    neig = nconv
+    call add_timing(eng, nbsvd, nrestart, dim, 0)
+  end subroutine
+
+end module pk_drivers
+
+! ---------------------------------------------------------------------------
+! Device-resident entry points (no reference counterpart): run the drivers on
+! an engine created by the caller (pk_lbd_create with a device operator or a
+! registered callback).  Singular values and bounds return to host arrays;
+! the singular vectors stay in the engine's bases (pk_lbd_basis / pk_lbd_get).
+! Option characters are passed by value; returns INFO.
+integer(c_int) function pk_lansvd_run(eng, jobu, jobv, k, kmax, sigma, bnd, tolin, doption, &
+                                      ioption) bind(C, name='pk_lansvd_run')
+  use iso_c_binding
+  use pk_api
+  use pk_drivers
+  implicit none
+  type(c_ptr), value :: eng
+  character(kind=c_char), value :: jobu, jobv
+  integer(c_int) :: k
+  integer(c_int), value :: kmax
+  real(c_double) :: sigma(*), bnd(*), doption(*)
+  real(c_double), value :: tolin
+  integer(c_int) :: ioption(*)
+  integer :: m, n, info, kk
+  double precision :: eps
+  call engine_shape(eng, m, n, eps, info)
+  if (info /= 0) then
+    pk_lansvd_run = info
+    return
+  end if
+  kk = k
+  call lansvd_core(jobu, jobv, m, n, kk, kmax, eng, eps, sigma, bnd, tolin, doption, ioption, &
+                   info)
+  k = kk
+  pk_lansvd_run = info
+end function
+
+integer(c_int) function pk_lansvd_irl_run(eng, which, jobu, jobv, dim, p, neig, maxiter, sigma, &
+                                          bnd, tolin, doption, ioption) &
+    bind(C, name='pk_lansvd_irl_run')
+  use iso_c_binding
+  use pk_api
+  use pk_drivers
+  implicit none
+  type(c_ptr), value :: eng
+  character(kind=c_char), value :: which, jobu, jobv
+  integer(c_int) :: dim, neig
+  integer(c_int), value :: p, maxiter
+  real(c_double) :: sigma(*), bnd(*), doption(*)
+  real(c_double), value :: tolin
+  integer(c_int) :: ioption(*)
+  integer :: m, n, info, dd, ne
+  double precision :: eps
+  call engine_shape(eng, m, n, eps, info)
+  if (info /= 0) then
+    pk_lansvd_irl_run = info
+    return
+  end if
+  dd = dim
+  ne = neig
+  call lansvd_irl_core(which, jobu, jobv, m, n, dd, p, ne, maxiter, eng, eps, sigma, bnd, &
+                       tolin, doption, ioption, info)
+  dim = dd
+  neig = ne
+  pk_lansvd_irl_run = info
+end function
+
+! Shape and storage unit roundoff of an engine.
+subroutine engine_shape(eng, m, n, eps, info)
+  use iso_c_binding
+  implicit none
+  type(c_ptr), intent(in) :: eng
+  integer, intent(out) :: m, n, info
+  double precision, intent(out) :: eps
+  interface
+    integer(c_int) function pk_lbd_shape(e, m, n, kmax, prec) bind(C, name='pk_lbd_shape')
+      import :: c_int, c_ptr
+      type(c_ptr), value :: e
+      integer(c_int) :: m, n, kmax, prec
+    end function
+  end interface
+  integer(c_int) :: mm, nn, kk, pp
+  info = pk_lbd_shape(eng, mm, nn, kk, pp)
+  m = mm
+  n = nn
+  if (pp == 0) then
+    eps = 2d0**(-53)
+  else
+    eps = 2d0**(-24)
+  end if
+end subroutine

@@ -1,0 +1,112 @@
+"""GPU parity: the HIP engine behind the Fortran drop-ins against the
+reference PROPACK (compiled from its own sources, oracle/_ref) and the dense
+FP64 SVD, on the 1850 x 712 case and small restarted runs.
+
+Tolerances (floating point): singular values 1e-10 relative to the oracle,
+residual ||A v - s u|| <= tol * s_1, singular vectors within 1e-8 of the
+oracle's up to sign (cosine), counters reported next to the oracle's."""
+import numpy as np
+import pytest
+
+from oracle import refpy
+from propack_amd import api, synth
+
+pytestmark = pytest.mark.gpu
+
+
+def _cos(a, b):
+    return np.abs(np.sum(a * b, axis=0)) / (np.linalg.norm(a, axis=0) * np.linalg.norm(b, axis=0))
+
+
+def _check_triplets(r, D, tol, s1):
+    res = np.linalg.norm(D @ r.V - r.U * r.sigma, axis=0)
+    assert res.max() <= tol * s1, (res.max(), tol * s1)
+    # orthonormal singular vectors
+    k = r.k
+    assert np.abs(r.U.T @ r.U - np.eye(k)).max() < 1e-9
+    assert np.abs(r.V.T @ r.V - np.eye(k)).max() < 1e-9
+
+
+def test_spmv_against_host(small_case):
+    A, u0, D, s = small_case
+    op = api.CsrOperator(A)
+    x = synth.start_vector(A.n, 3)
+    y = synth.start_vector(A.m, 4)
+    np.testing.assert_allclose(op.spmv(x, "n"), D @ x, rtol=0, atol=1e-12)
+    np.testing.assert_allclose(op.spmv(y, "t"), D.T @ y, rtol=0, atol=1e-12)
+
+
+@pytest.mark.parametrize("cgs", [1, 0])
+def test_lansvd_device_operator(small_case, cgs):
+    A, u0, D, s = small_case
+    op = api.CsrOperator(A)
+    r = api.lansvd(op, A.m, A.n, 10, 300, u0=u0, tol=1e-12, cgs=cgs)
+    ref = refpy.lansvd(A, 10, 300, u0=u0, tol=1e-12, cgs=cgs)
+    print("gpu", r.info, r.k, r.counters)
+    print("ref", ref.info, ref.k, ref.counters)
+    assert r.info == 0 and r.k == 10
+    np.testing.assert_allclose(r.sigma, ref.sigma, rtol=1e-10)
+    np.testing.assert_allclose(r.sigma, s[:10], rtol=1e-12)
+    _check_triplets(r, D, 1e-12, s[0])
+    assert _cos(r.U, ref.U).min() > 1 - 1e-8
+    assert _cos(r.V, ref.V).min() > 1 - 1e-8
+    assert abs(r.counters["nopx"] - ref.counters["nopx"]) <= 0.1 * ref.counters["nopx"]
+
+
+def test_lansvd_irl_device_operator(small_case):
+    A, u0, D, s = small_case
+    op = api.CsrOperator(A)
+    r = api.lansvd_irl(op, A.m, A.n, 10, 30, u0=u0, tol=1e-12)
+    ref = refpy.lansvd_irl(A, 10, 30, u0=u0, tol=1e-12, cgs=1)
+    print("gpu", r.info, r.k, r.counters)
+    print("ref", ref.info, ref.k, ref.counters)
+    assert r.info == 0 and r.k == 10 and r.dim == 30
+    np.testing.assert_allclose(r.sigma, ref.sigma, rtol=1e-10)
+    _check_triplets(r, D, 1e-11, s[0])
+    assert _cos(r.U, ref.U).min() > 1 - 1e-8
+    assert abs(r.counters["nrestart"] - ref.counters["nrestart"]) <= 2
+
+
+def test_lansvd_host_callback(small_case):
+    A, u0, D, s = small_case
+    calls = []
+
+    def aprod(t, x):
+        calls.append(t)
+        return D @ x if t == "n" else D.T @ x
+
+    r = api.lansvd(aprod, A.m, A.n, 10, 300, u0=u0, tol=1e-12)
+    assert r.info == 0 and r.k == 10
+    np.testing.assert_allclose(r.sigma, s[:10], rtol=1e-12)
+    _check_triplets(r, D, 1e-12, s[0])
+    assert len(calls) == r.counters["nopx"]
+
+
+def test_random_start_vector(small_case):
+    A, u0, D, s = small_case
+    op = api.CsrOperator(A)
+    r = api.lansvd(op, A.m, A.n, 10, 300, u0=None, tol=1e-12)
+    ref = refpy.lansvd(A, 10, 300, u0=None, tol=1e-12, cgs=1)
+    assert r.info == 0 and r.k == 10
+    np.testing.assert_allclose(r.sigma, ref.sigma, rtol=1e-10)
+
+
+def test_single_precision_dropin(small_case):
+    A, u0, D, s = small_case
+    op = api.CsrOperator(A, prec=api.PK_F32)
+    r = api.lansvd(op, A.m, A.n, 10, 300, u0=u0, tol=1e-6, prec=api.PK_F32)
+    ref = refpy.lansvd(A, 10, 300, u0=u0, tol=1e-6, prec="s", cgs=1)
+    assert r.info == 0 and r.k == 10
+    np.testing.assert_allclose(r.sigma, s[:10], rtol=2e-6)
+    np.testing.assert_allclose(r.sigma, ref.sigma, rtol=5e-6)
+
+
+def test_device_resident_irl(small_case):
+    A, u0, D, s = small_case
+    op = api.CsrOperator(A)
+    eng = api.Engine(op, kmax=30)
+    eng.put_u1(u0)
+    r = api.lansvd_irl_dev(eng, 10, 30, tol=1e-12)
+    assert r.info == 0 and r.k == 10
+    np.testing.assert_allclose(r.sigma, s[:10], rtol=1e-12)
+    _check_triplets(r, D, 1e-11, s[0])
