@@ -121,6 +121,14 @@ int pk_lbd_stats(pk_lbd_t eng, long long* out);
 int pk_lbd_reset_stats(pk_lbd_t eng);
 int pk_lbd_sync(pk_lbd_t eng);
 int pk_lbd_shape(pk_lbd_t eng, int* m, int* n, int* kmax, int* prec);
+/* Live kernel timing (no reference counterpart): when enabled, every hot
+ * kernel launch is bracketed by HIP events on the engine's stream and its
+ * algorithmic bytes/flops are accumulated per class.  out holds 6 classes x
+ * {launches, total ms, bytes, flops}: 0 SpMV A*v, 1 SpMV A^T*u,
+ * 2 Gram-Schmidt V^T w, 3 Gram-Schmidt w -= V h, 4 basis GEMM, 5 level-1. */
+int pk_lbd_profile(pk_lbd_t eng, int enable);
+int pk_lbd_profile_get(pk_lbd_t eng, double* out);
+int pk_lbd_profile_reset(pk_lbd_t eng);
 
 /* ---------------------------------------------------------------------- */
 /* Device-resident drivers (no reference counterpart; for callers whose

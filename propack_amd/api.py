@@ -33,7 +33,9 @@ EXPORTS = [
     "pk_lbd_destroy", "pk_lbd_set_callback", "pk_lbd_put", "pk_lbd_get", "pk_lbd_basis",
     "pk_lbd_nrm2", "pk_lbd_getu0", "pk_lbd_lanbpro", "pk_lbd_update", "pk_lbd_stats",
     "pk_lbd_reset_stats", "pk_lbd_sync", "pk_lbd_shape", "pk_lansvd_run",
-    "pk_lansvd_irl_run", "dlansvd_", "dlansvd_irl_", "slansvd_", "slansvd_irl_",
+    "pk_lansvd_irl_run", "pk_lbd_profile", "pk_lbd_profile_get", "pk_lbd_profile_reset",
+    "pk_larnv_pm1", "pkh_update_mu", "pkh_update_nu", "pkh_compute_int", "pkh_set_mu",
+    "pkh_lartg", "pkh_bdqr", "pkh_bsvdstep", "pkh_refinebounds", "dlansvd_", "dlansvd_irl_", "slansvd_", "slansvd_irl_",
     "clearstat_", "printstat_",
 ]
 
@@ -280,6 +282,19 @@ class Engine:
 
     def reset_stats(self):
         lib().pk_lbd_reset_stats(self.handle)
+
+    KCLASSES = ["spmv_n", "spmv_t", "gemvt", "gemv", "gemm", "vec"]
+
+    def profile(self, enable: bool = True):
+        _check(lib().pk_lbd_profile(self.handle, C.c_int(int(enable))), "pk_lbd_profile")
+        _check(lib().pk_lbd_profile_reset(self.handle), "pk_lbd_profile_reset")
+
+    def profile_get(self) -> dict:
+        out = np.zeros(4 * len(self.KCLASSES))
+        _check(lib().pk_lbd_profile_get(self.handle, _p(out)), "pk_lbd_profile_get")
+        return {c: dict(launches=int(out[4 * i]), ms=float(out[4 * i + 1]),
+                        bytes=float(out[4 * i + 2]), flops=float(out[4 * i + 3]))
+                for i, c in enumerate(self.KCLASSES)}
 
     def close(self):
         if self.handle:

@@ -10,6 +10,7 @@
 // Each device reduction that feeds a decision costs one stream
 // synchronization; counters record them.
 #include <cmath>
+#include <cstdlib>
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
@@ -188,7 +189,58 @@ struct Impl {
                           hipMemcpyDeviceToHost, strm));
     PK_HIP(hipStreamSynchronize(strm));
     e->cnt.nsync++;
+    return resolve_timers();
+  }
+
+  // ---- live kernel timing (HIP events on the engine stream) ----
+  hipEvent_t ev_get() {
+    hipEvent_t ev = nullptr;
+    if (!e->prof.pool.empty()) {
+      ev = e->prof.pool.back();
+      e->prof.pool.pop_back();
+    } else if (hipEventCreate(&ev) != hipSuccess) {
+      ev = nullptr;
+    }
+    return ev;
+  }
+  hipEvent_t tstart() {
+    if (!e->prof.on) return nullptr;
+    hipEvent_t a = ev_get();
+    if (a) (void)hipEventRecord(a, strm);
+    return a;
+  }
+  void tstop(hipEvent_t a, int cls, double bytes, double flops = 0.0) {
+    if (!a) return;
+    hipEvent_t b = ev_get();
+    if (!b) return;
+    (void)hipEventRecord(b, strm);
+    e->prof.pending.push_back(KTimer{cls, a, b, bytes, flops});
+  }
+  // Called after a stream synchronization: every pending pair has completed.
+  int resolve_timers() {
+    for (const KTimer& t : e->prof.pending) {
+      float ms = 0.f;
+      if (hipEventElapsedTime(&ms, t.a, t.b) == hipSuccess) {
+        e->prof.ms[t.cls] += ms;
+        e->prof.bytes[t.cls] += t.bytes;
+        e->prof.flops[t.cls] += t.flops;
+        e->prof.count[t.cls] += 1;
+      }
+      e->prof.pool.push_back(t.a);
+      e->prof.pool.push_back(t.b);
+    }
+    e->prof.pending.clear();
     return PK_OK;
+  }
+  // Algorithmic bytes of y = op(A) x [- c yprev] (compulsory traffic: every
+  // nonzero and index once, x and y once, yprev once).
+  double spmv_bytes(bool tr, bool with_prev) const {
+    const double sv = e->op ? static_cast<double>(e->op->val_bytes) : 0.0;
+    const double sb = static_cast<double>(e->tbytes);
+    const double nnz = e->op ? static_cast<double>(e->op->nnz) : 0.0;
+    const double rows = tr ? e->n : e->m, cols = tr ? e->m : e->n;
+    return nnz * (sv + 4.0) + (rows + 1) * 4.0 + cols * sb + rows * sb +
+           (with_prev ? rows * sb : 0.0);
   }
 
   // ||x|| of a device vector (pdnrm2).
@@ -211,8 +263,11 @@ struct Impl {
     const long long rows_out = tr ? e->n : e->m, rows_in = tr ? e->m : e->n;
     int nb;
     if (e->op) {
+      hipEvent_t t0 = tstart();
       nb = csr_apply<T>(e->op, strm, trans, x, y, yprev, sval(c), e->part, nullptr);
       PK_LAUNCH(nb);
+      tstop(t0, tr ? KC_SPMV_T : KC_SPMV_N, spmv_bytes(tr, yprev != nullptr),
+            2.0 * static_cast<double>(e->op->nnz));
     } else if (e->aprod) {
       PK_TRY(host_apply(trans, x, y, rows_in, rows_out));
       if (yprev)
@@ -280,7 +335,9 @@ struct Impl {
   }
 
   int scale(T* x, long long L, double a) {
+    hipEvent_t t0 = tstart();
     PK_LAUNCH(launch_scale<T>(strm, L, x, sval(a), nullptr));
+    tstop(t0, KC_VEC, 2.0 * sizeof(T) * L);
     return PK_OK;
   }
 
@@ -336,12 +393,17 @@ struct Impl {
       e->cnt.ndot += nd + k;
       e->cnt.ncols_touched += ncols;
       if (ncols > 0) {
+        const double sb = static_cast<double>(sizeof(T));
+        hipEvent_t t0 = tstart();
         const int nb = launch_gemvt<T>(strm, L, basis, ld, e->dcols, ncols, vnew, e->part, nullptr);
         PK_LAUNCH(nb);
+        tstop(t0, KC_GEMVT, (ncols + 1.0) * L * sb, 2.0 * ncols * L);
         PK_LAUNCH(launch_reduce(strm, ncols, nb, e->part, e->red, nullptr));
+        hipEvent_t t1 = tstart();
         const int nb2 =
             launch_gemv_update<T>(strm, L, basis, ld, e->dcols, ncols, e->red, vnew, e->part, nullptr);
         PK_LAUNCH(nb2);
+        tstop(t1, KC_GEMV, (ncols + 2.0) * L * sb, 2.0 * ncols * L);
         PK_LAUNCH(launch_reduce(strm, 1, nb2, e->part, e->red + kMaxCols, nullptr));
         PK_TRY(read(kMaxCols, 1));
         *normvnew = std::sqrt(e->hred[kMaxCols]);
@@ -667,7 +729,7 @@ struct Impl {
     *kp = k;
     *ierr_out = ierr;
     PK_HIP(hipStreamSynchronize(strm));
-    return PK_OK;
+    return resolve_timers();
   }
 
   // X(:, 1:ncols) = X(:, 1:kdim) * op(B) into the spare buffer, then swap.
@@ -695,9 +757,13 @@ struct Impl {
         e->hB[static_cast<size_t>(j) * kdim + l] =
             tb ? B[static_cast<size_t>(l) * ldb + j] : B[static_cast<size_t>(j) * ldb + l];
     PK_HIP(hipMemcpyAsync(e->dB, e->hB, sizeof(double) * kdim * ncols, hipMemcpyHostToDevice, strm));
+    hipEvent_t t0 = tstart();
     PK_LAUNCH(launch_gemm_tall<T>(strm, L, static_cast<const T*>(buf[cur]), ld, kdim, e->dB, kdim,
                                   ncols, static_cast<T*>(buf[1 - cur]), ld));
+    tstop(t0, KC_GEMM, static_cast<double>(sizeof(T)) * L * (kdim + ncols),
+          2.0 * static_cast<double>(L) * kdim * ncols);
     PK_HIP(hipStreamSynchronize(strm));
+    resolve_timers();
     cur = 1 - cur;
     return PK_OK;
   }
@@ -722,6 +788,11 @@ CsrOp* as_csr(pk_csr_t h) {
 }
 
 static void engine_free(Engine* e) {
+  for (const KTimer& t : e->prof.pending) {
+    (void)hipEventDestroy(t.a);
+    (void)hipEventDestroy(t.b);
+  }
+  for (hipEvent_t ev : e->prof.pool) (void)hipEventDestroy(ev);
   for (int i = 0; i < 2; ++i) {
     (void)hipFree(e->U[i]);
     (void)hipFree(e->V[i]);
@@ -965,6 +1036,33 @@ int pk_lbd_reset_stats(pk_lbd_t h) {
   return PK_OK;
 }
 
+int pk_lbd_profile(pk_lbd_t h, int enable) {
+  PK_ENGINE(h);
+  e->prof.on = enable != 0;
+  return PK_OK;
+}
+
+int pk_lbd_profile_get(pk_lbd_t h, double* out) {
+  PK_ENGINE(h);
+  if (!out) return PK_ERR_ARG;
+  for (int c = 0; c < KC_COUNT; ++c) {
+    out[4 * c + 0] = static_cast<double>(e->prof.count[c]);
+    out[4 * c + 1] = e->prof.ms[c];
+    out[4 * c + 2] = e->prof.bytes[c];
+    out[4 * c + 3] = e->prof.flops[c];
+  }
+  return PK_OK;
+}
+
+int pk_lbd_profile_reset(pk_lbd_t h) {
+  PK_ENGINE(h);
+  for (int c = 0; c < KC_COUNT; ++c) {
+    e->prof.count[c] = 0;
+    e->prof.ms[c] = e->prof.bytes[c] = e->prof.flops[c] = 0.0;
+  }
+  return PK_OK;
+}
+
 int pk_lbd_shape(pk_lbd_t h, int* m, int* n, int* kmax, int* prec) {
   PK_ENGINE(h);
   if (m) *m = e->m;
@@ -1020,3 +1118,18 @@ void pkh_compute_int(const double* mu, int j, double delta, double eta, int* ind
 void pkh_set_mu(int k, double* mu, const int* index, double val) { set_mu(k, mu, index, val); }
 
 }  // extern "C"
+
+// Progress line per outer iteration of the drivers when PK_PROGRESS is set
+// in the environment (Fortran-side helper, arguments by reference).
+#include <chrono>
+extern "C" void pk_f_progress_(const int* iter, const int* nconv, const int* neig,
+                               const double* s1, const double* rnorm) {
+  static const bool on = std::getenv("PK_PROGRESS") != nullptr;
+  if (!on) return;
+  static const auto t0 = std::chrono::steady_clock::now();
+  const double t =
+      std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+  std::fprintf(stderr, "[propack_amd] t=%.3fs iter=%d converged=%d/%d sigma1=%.6e rnorm=%.3e\n",
+               t, *iter, *nconv, *neig, *s1, *rnorm);
+  std::fflush(stderr);
+}

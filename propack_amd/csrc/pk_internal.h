@@ -36,6 +36,21 @@ typedef void (*aprod_d_fn)(const char*, const int*, const int*, const double*, d
 typedef void (*aprod_s_fn)(const char*, const int*, const int*, const float*, float*, float*,
                            int*, size_t);
 
+// Live per-kernel-class timing with HIP events on the engine's stream.
+enum KClass { KC_SPMV_N = 0, KC_SPMV_T, KC_GEMVT, KC_GEMV, KC_GEMM, KC_VEC, KC_COUNT };
+struct KTimer {
+  int cls;
+  hipEvent_t a, b;
+  double bytes, flops;
+};
+struct Profile {
+  bool on = false;
+  std::vector<hipEvent_t> pool;
+  std::vector<KTimer> pending;
+  double ms[KC_COUNT] = {}, bytes[KC_COUNT] = {}, flops[KC_COUNT] = {};
+  long long count[KC_COUNT] = {};
+};
+
 struct Counters {
   long long nopx = 0, nreorth = 0, ndot = 0, ncols_touched = 0, nsync = 0;
 };
@@ -70,6 +85,7 @@ struct Engine {
   void* vec = nullptr;               // scratch vector, max(m, n)
   double* hvec = nullptr;            // pinned host scratch (random start vectors)
   Counters cnt;
+  Profile prof;
   int last_error = 0;
 
   void* u() const { return U[ucur]; }

@@ -187,6 +187,7 @@ contains
           exit
         end if
       end do
+      call pk_f_progress(j, neig, k, a1(1), rnorm)
       if (ierr < 0) then
         if (j < k) info = j
         exit
@@ -211,8 +212,8 @@ contains
       call ritzvec('L', jobu, jobv, m, n, neig, jold, alpha, beta, eng, jj)
       if (jj /= 0) info = jj
     end if
+    call add_timing(eng, nbsvd, 0, j, k)
     k = neig
-    call add_timing(eng, nbsvd, 0, j, -1)
   end subroutine
 
   ! Implicitly restarted driver (DLANSVD_IRL control flow,
@@ -316,6 +317,7 @@ contains
           end if
         end do
       end if
+      call pk_f_progress(iter, nconv, neig, a1(1), rnorm)
       if (ierr < 0) then
         if (dim < k) info = dim
         exit
@@ -379,7 +381,7 @@ contains
       if (jj /= 0) info = jj
     end if
     neig = nconv
-    call add_timing(eng, nbsvd, nrestart, dim, 0)
+    call add_timing(eng, nbsvd, nrestart, dim, -1)
   end subroutine
 
 end module pk_drivers

