@@ -116,11 +116,21 @@ int pk_lbd_update(pk_lbd_t eng, char which, char transb, int kdim, int ncols, co
 /* Counters with the meaning of COMMON /timing/ (double/stat.h:12-15):
  * out[0] nopx, [1] nreorth, [2] ndot (reference semantics, see
  * double/dreorth.F:92,167), [3] basis columns actually streamed by
- * reorthogonalization, [4] host synchronizations. */
+ * reorthogonalization, [4] host synchronizations, [5] CGS passes beyond the
+ * first of a reorthogonalization. */
 int pk_lbd_stats(pk_lbd_t eng, long long* out);
 int pk_lbd_reset_stats(pk_lbd_t eng);
 int pk_lbd_sync(pk_lbd_t eng);
 int pk_lbd_shape(pk_lbd_t eng, int* m, int* n, int* kmax, int* prec);
+/* Step execution (no reference counterpart): mode 0 runs the Lanczos steps
+ * as device-resident kernel chains whose scalar control (the omega
+ * recurrence, ELR and reorthogonalization decisions) runs in control
+ * kernels on the GPU, holding `tries` CGS passes per side; rare paths and
+ * further passes continue on the host.  mode 1 runs every decision on the
+ * host after a synchronization (same control code, same results).  The
+ * host-callback operator always uses mode 1.  Environment PK_LBD_MODE=host
+ * selects mode 1 at creation. */
+int pk_lbd_set_mode(pk_lbd_t eng, int mode, int tries);
 /* Live kernel timing (no reference counterpart): when enabled, every hot
  * kernel launch is bracketed by HIP events on the engine's stream and its
  * algorithmic bytes/flops are accumulated per class.  out holds 6 classes x

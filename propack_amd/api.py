@@ -33,7 +33,7 @@ EXPORTS = [
     "pk_lbd_destroy", "pk_lbd_set_callback", "pk_lbd_put", "pk_lbd_get", "pk_lbd_basis",
     "pk_lbd_nrm2", "pk_lbd_getu0", "pk_lbd_lanbpro", "pk_lbd_update", "pk_lbd_stats",
     "pk_lbd_reset_stats", "pk_lbd_sync", "pk_lbd_shape", "pk_lansvd_run",
-    "pk_lansvd_irl_run", "pk_lbd_profile", "pk_lbd_profile_get", "pk_lbd_profile_reset",
+    "pk_lansvd_irl_run", "pk_lbd_set_mode", "pk_lbd_profile", "pk_lbd_profile_get", "pk_lbd_profile_reset",
     "pk_larnv_pm1", "pkh_update_mu", "pkh_update_nu", "pkh_compute_int", "pkh_set_mu",
     "pkh_lartg", "pkh_bdqr", "pkh_bsvdstep", "pkh_refinebounds", "dlansvd_", "dlansvd_irl_", "slansvd_", "slansvd_irl_",
     "clearstat_", "printstat_",
@@ -276,9 +276,14 @@ class Engine:
         return X.T
 
     def stats(self) -> dict:
-        out = np.zeros(5, np.int64)
+        out = np.zeros(6, np.int64)
         _check(lib().pk_lbd_stats(self.handle, _p(out)), "pk_lbd_stats")
-        return dict(zip(["nopx", "nreorth", "ndot", "ncols", "nsync"], out.tolist()))
+        return dict(zip(["nopx", "nreorth", "ndot", "ncols", "nsync", "nretry"], out.tolist()))
+
+    def set_mode(self, mode: str = "device", tries: int = 2):
+        """'device': device-resident step chains; 'host': host executor."""
+        _check(lib().pk_lbd_set_mode(self.handle, C.c_int(0 if mode == "device" else 1),
+                                     C.c_int(tries)), "pk_lbd_set_mode")
 
     def reset_stats(self):
         lib().pk_lbd_reset_stats(self.handle)

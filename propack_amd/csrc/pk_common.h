@@ -34,8 +34,6 @@ struct Scal {
 inline Scal sval(double v) { return Scal{nullptr, v}; }
 inline Scal sptr(const double* p) { return Scal{p, 0.0}; }
 
-// A kernel with a non-null gate runs only when *gate != 0.
-__device__ __forceinline__ bool gated_off(const int* gate) { return gate && *gate == 0; }
 
 __device__ __forceinline__ double wave_sum(double v) {
 #pragma unroll
@@ -73,6 +71,16 @@ __device__ __forceinline__ void block_sum(double (&v)[NV], double* lds) {
     }
   }
   __syncthreads();
+}
+
+// Sum of n partials by one 256-thread workgroup in a fixed order (thread t
+// sums p[t], p[t+256], ...; then the block reduction); every caller that
+// reduces the same partials gets the same bits.
+__device__ __forceinline__ double reduce_partials(const double* p, int n, double* lds) {
+  double v[1] = {0.0};
+  for (int i = threadIdx.x; i < n; i += kBlock) v[0] += p[i];
+  block_sum<1>(v, lds);
+  return v[0];  // valid in thread 0
 }
 
 inline int round_up(int x, int a) { return (x + a - 1) / a * a; }

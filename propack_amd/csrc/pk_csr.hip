@@ -9,6 +9,7 @@
 
 #include <algorithm>
 #include <cstring>
+#include <vector>
 
 #include "pk_internal.h"
 
@@ -65,6 +66,28 @@ int choose_lpr(long long nnz, int rows) {
   int l = 2;
   while (l < 64 && l * 1.5 < avg) l *= 2;
   return l;
+}
+
+// Greedy partition of rows into blocks of at most kSpmvRows rows and
+// kSpmvCap nonzeros; a row longer than the cap gets a block of its own.
+std::vector<int> row_blocks(const std::vector<int>& rp, int rows) {
+  std::vector<int> rb;
+  rb.reserve(static_cast<size_t>(rows) / 64 + 2);
+  rb.push_back(0);
+  int r = 0;
+  while (r < rows) {
+    const int start = r;
+    long long nz = 0;
+    while (r < rows && r - start < kSpmvRows) {
+      const long long len = static_cast<long long>(rp[r + 1]) - rp[r];
+      if (nz + len > kSpmvCap) break;
+      nz += len;
+      ++r;
+    }
+    if (r == start) ++r;
+    rb.push_back(r);
+  }
+  return rb;
 }
 
 }  // namespace
@@ -165,6 +188,27 @@ int csr_create(int m, int n, long long nnz, const int* rowptr, const int* colind
   }
   op->lpr_n = choose_lpr(nnz, m);
   op->lpr_t = choose_lpr(nnz, n);
+  {
+    // row-block partitions for the LDS-staged SpMV (both directions)
+    std::vector<int> hrp(static_cast<size_t>(m) + 1), hrpt(static_cast<size_t>(n) + 1);
+    bool ok = hipMemcpy(hrp.data(), op->rp, sizeof(int) * (m + 1), hipMemcpyDeviceToHost) ==
+                  hipSuccess &&
+              hipMemcpy(hrpt.data(), op->rpt, sizeof(int) * (n + 1), hipMemcpyDeviceToHost) ==
+                  hipSuccess;
+    std::vector<int> rb = row_blocks(hrp, m), rbt = row_blocks(hrpt, n);
+    op->nrb = static_cast<int>(rb.size()) - 1;
+    op->nrbt = static_cast<int>(rbt.size()) - 1;
+    ok = ok && hipMalloc(&op->rb, sizeof(int) * rb.size()) == hipSuccess &&
+         hipMalloc(&op->rbt, sizeof(int) * rbt.size()) == hipSuccess &&
+         hipMemcpy(op->rb, rb.data(), sizeof(int) * rb.size(), hipMemcpyHostToDevice) ==
+             hipSuccess &&
+         hipMemcpy(op->rbt, rbt.data(), sizeof(int) * rbt.size(), hipMemcpyHostToDevice) ==
+             hipSuccess;
+    if (!ok) {
+      csr_destroy(op);
+      return PK_ERR_HIP;
+    }
+  }
   *out = op;
   return PK_OK;
 }
@@ -177,6 +221,8 @@ void csr_destroy(CsrOp* op) {
   (void)hipFree(op->rpt);
   (void)hipFree(op->cit);
   (void)hipFree(op->vat);
+  (void)hipFree(op->rb);
+  (void)hipFree(op->rbt);
   op->magic = 0;
   delete op;
 }
@@ -185,24 +231,27 @@ void csr_destroy(CsrOp* op) {
 // vectors of basis type T.  Returns nblk of the partials or < 0.
 template <typename T>
 int csr_apply(const CsrOp* op, hipStream_t s, char trans, const T* x, T* y, const T* yprev,
-              Scal coef, double* part, const int* gate) {
+              Scal coef, double* part, Gate gate) {
   const bool tr = (trans == 't' || trans == 'T');
   const int rows = tr ? op->n : op->m;
   const int* rp = tr ? op->rpt : op->rp;
   const int* ci = tr ? op->cit : op->ci;
   const void* va = tr ? op->vat : op->va;
   const int lpr = tr ? op->lpr_t : op->lpr_n;
+  const int* rb = tr ? op->rbt : op->rb;
+  const int nrb = tr ? op->nrbt : op->nrb;
+  (void)rows;
   if (op->val_prec == PK_F64)
-    return launch_spmv<double, T>(s, rows, rp, ci, static_cast<const double*>(va), x, y, yprev,
+    return launch_spmv_rb<double, T>(s, nrb, rb, rp, ci, static_cast<const double*>(va), x, y,
+                                     yprev, coef, part, lpr, gate);
+  return launch_spmv_rb<float, T>(s, nrb, rb, rp, ci, static_cast<const float*>(va), x, y, yprev,
                                   coef, part, lpr, gate);
-  return launch_spmv<float, T>(s, rows, rp, ci, static_cast<const float*>(va), x, y, yprev, coef,
-                               part, lpr, gate);
 }
 
 template int csr_apply<double>(const CsrOp*, hipStream_t, char, const double*, double*,
-                               const double*, Scal, double*, const int*);
+                               const double*, Scal, double*, Gate);
 template int csr_apply<float>(const CsrOp*, hipStream_t, char, const float*, float*,
-                              const float*, Scal, double*, const int*);
+                              const float*, Scal, double*, Gate);
 
 }  // namespace pk
 
@@ -226,7 +275,8 @@ int spmv_any(CsrOp* op, char trans, const T* x, T* y, int where) {
   T *dx = nullptr, *dy = nullptr;
   double* part = nullptr;
   int rc = PK_OK;
-  if (hipMalloc(&part, sizeof(double) * 2 * kMaxBlocks) != hipSuccess) rc = PK_ERR_OOM;
+  const size_t npart = 2 * static_cast<size_t>(std::max(std::max(op->nrb, op->nrbt), kMaxBlocks));
+  if (hipMalloc(&part, sizeof(double) * npart) != hipSuccess) rc = PK_ERR_OOM;
   if (rc == PK_OK && where == PK_HOST) {
     if (hipMalloc(&dx, sizeof(T) * std::max<long long>(rin, 1)) != hipSuccess ||
         hipMalloc(&dy, sizeof(T) * std::max<long long>(rout, 1)) != hipSuccess ||
@@ -238,7 +288,7 @@ int spmv_any(CsrOp* op, char trans, const T* x, T* y, int where) {
   }
   if (rc == PK_OK) {
     if (op->val_prec == (sizeof(T) == 8 ? PK_F64 : PK_F32)) {
-      if (csr_apply<T>(op, s, trans, dx, dy, nullptr, sval(0.0), part, nullptr) < 0)
+      if (csr_apply<T>(op, s, trans, dx, dy, nullptr, sval(0.0), part, Gate{}) < 0)
         rc = PK_ERR_HIP;
     } else {
       set_error("pk_csr_spmv: vector precision differs from the operator's");

@@ -86,89 +86,10 @@ static void larnv_pm1(long long n, T* x) {
   }
 }
 
-// DLAPY2 (double/Lapack_Util/dlapy2.f).
-static double lapy2(double x, double y) {
-  const double xa = std::fabs(x), ya = std::fabs(y);
-  const double w = std::max(xa, ya), z = std::min(xa, ya);
-  if (z == 0.0) return w;
-  return w * std::sqrt(1.0 + (z / w) * (z / w));
-}
-
-static double fsign(double a, double b) { return b >= 0 ? std::fabs(a) : -std::fabs(a); }
-
-// ---- omega recurrence (double/dlanbpro.F:555-723); 1-based arrays ----
-
-// dset_mu (dlanbpro.F:555-577)
-static void set_mu(int k, double* mu, const int* index, double val) {
-  int i = 1;
-  while (index[i] <= k && index[i] > 0) {
-    for (int j = index[i]; j <= index[i + 1]; ++j) mu[j] = val;
-    i += 2;
-  }
-}
-
-// dcompute_int (dlanbpro.F:581-624)
-static void compute_int(const double* mu, int j, double delta, double eta, int* index) {
-  if (delta < eta) return;
-  int ip = 0, i = 0, k, s;
-  index[1] = 0;
-  while (i < j) {
-    for (k = i + 1; k <= j; ++k)
-      if (std::fabs(mu[k]) > delta) break;
-    if (k > j) break;
-    const int lo = std::max(i, 1);
-    for (s = k; s >= lo; --s)
-      if (std::fabs(mu[s]) < eta) break;
-    index[++ip] = s + 1;
-    for (i = s + 1; i <= j; ++i)
-      if (std::fabs(mu[i]) < eta) break;
-    index[++ip] = i - 1;
-  }
-  index[++ip] = j + 1;
-}
-
-// dupdate_mu (dlanbpro.F:628-679); alpha/beta 1-based views of B.
-static void update_mu(double* mumax, double* mu, const double* nu, int j, const double* alpha,
-                      const double* beta, double anorm, double eps1) {
-  double d;
-  if (j == 1) {
-    d = eps1 * (lapy2(alpha[j], beta[j]) + alpha[1]) + eps1 * anorm;
-    mu[1] = eps1 / beta[1];
-    *mumax = std::fabs(mu[1]);
-  } else {
-    mu[1] = alpha[1] * nu[1] - alpha[j] * mu[1];
-    d = eps1 * (lapy2(alpha[j], beta[j]) + alpha[1]) + eps1 * anorm;
-    mu[1] = (mu[1] + fsign(d, mu[1])) / beta[j];
-    *mumax = std::fabs(mu[1]);
-    for (int k = 2; k <= j - 1; ++k) {
-      mu[k] = alpha[k] * nu[k] + beta[k - 1] * nu[k - 1] - alpha[j] * mu[k];
-      d = eps1 * (lapy2(alpha[j], beta[j]) + lapy2(alpha[k], beta[k - 1])) + eps1 * anorm;
-      mu[k] = (mu[k] + fsign(d, mu[k])) / beta[j];
-      *mumax = std::max(*mumax, std::fabs(mu[k]));
-    }
-    mu[j] = beta[j - 1] * nu[j - 1];
-    d = eps1 * (lapy2(alpha[j], beta[j]) + lapy2(alpha[j], beta[j - 1])) + eps1 * anorm;
-    mu[j] = (mu[j] + fsign(d, mu[j])) / beta[j];
-    *mumax = std::max(*mumax, std::fabs(mu[j]));
-  }
-  mu[j + 1] = 1.0;
-}
-
-// dupdate_nu (dlanbpro.F:684-723)
-static void update_nu(double* numax, const double* mu, double* nu, int j, const double* alpha,
-                      const double* beta, double anorm, double eps1) {
-  if (j > 1) {
-    *numax = 0.0;
-    for (int k = 1; k <= j - 1; ++k) {
-      nu[k] = beta[k] * mu[k + 1] + alpha[k] * mu[k] - beta[j - 1] * nu[k];
-      const double d =
-          eps1 * (lapy2(alpha[k], beta[k]) + lapy2(alpha[j], beta[j - 1])) + eps1 * anorm;
-      nu[k] = (nu[k] + fsign(d, nu[k])) / alpha[j];
-      *numax = std::max(*numax, std::fabs(nu[k]));
-    }
-    nu[j] = 1.0;
-  }
-}
+// The scalar routines of the step are the __host__ __device__ ones of
+// pk_lbd.h, shared with the device control kernels.
+static double lapy2(double x, double y) { return hd_lapy2(x, y); }
+static void set_mu(int k, double* mu, const int* index, double val) { hd_set_mu(k, mu, index, val); }
 
 template <typename T>
 struct Impl {
@@ -251,7 +172,7 @@ struct Impl {
     }
     const int nb = launch_norm<T>(strm, L, x, e->part);
     PK_LAUNCH(nb);
-    PK_LAUNCH(launch_reduce(strm, 1, nb, e->part, e->red, nullptr));
+    PK_LAUNCH(launch_reduce(strm, 1, nb, e->part, e->red, Gate{}));
     PK_TRY(read(0, 1));
     *out = std::sqrt(e->hred[0]);
     return PK_OK;
@@ -264,19 +185,19 @@ struct Impl {
     int nb;
     if (e->op) {
       hipEvent_t t0 = tstart();
-      nb = csr_apply<T>(e->op, strm, trans, x, y, yprev, sval(c), e->part, nullptr);
+      nb = csr_apply<T>(e->op, strm, trans, x, y, yprev, sval(c), e->part, Gate{});
       PK_LAUNCH(nb);
       tstop(t0, tr ? KC_SPMV_T : KC_SPMV_N, spmv_bytes(tr, yprev != nullptr),
             2.0 * static_cast<double>(e->op->nnz));
     } else if (e->aprod) {
       PK_TRY(host_apply(trans, x, y, rows_in, rows_out));
       if (yprev)
-        nb = launch_axpy_norm_dot<T>(strm, rows_out, y, yprev, sval(c), e->part, nullptr);
+        nb = launch_axpy_norm_dot<T>(strm, rows_out, y, yprev, sval(c), e->part, Gate{});
       else
         nb = launch_norm<T>(strm, rows_out, y, e->part);
       PK_LAUNCH(nb);
       if (!yprev) {
-        PK_LAUNCH(launch_reduce(strm, 1, nb, e->part, e->red, nullptr));
+        PK_LAUNCH(launch_reduce(strm, 1, nb, e->part, e->red, Gate{}));
         PK_TRY(read(0, 1));
         *s2 = e->hred[0];
         *sd = 0.0;
@@ -286,7 +207,7 @@ struct Impl {
       set_error("engine has no operator");
       return PK_ERR_ARG;
     }
-    PK_LAUNCH(launch_reduce(strm, 2, nb, e->part, e->red, nullptr));
+    PK_LAUNCH(launch_reduce(strm, 2, nb, e->part, e->red, Gate{}));
     PK_TRY(read(0, 2));
     *s2 = e->hred[0];
     *sd = yprev ? e->hred[1] : 0.0;
@@ -336,7 +257,7 @@ struct Impl {
 
   int scale(T* x, long long L, double a) {
     hipEvent_t t0 = tstart();
-    PK_LAUNCH(launch_scale<T>(strm, L, x, sval(a), nullptr));
+    PK_LAUNCH(launch_scale<T>(strm, L, x, sval(a), Gate{}));
     tstop(t0, KC_VEC, 2.0 * sizeof(T) * L);
     return PK_OK;
   }
@@ -395,16 +316,18 @@ struct Impl {
       if (ncols > 0) {
         const double sb = static_cast<double>(sizeof(T));
         hipEvent_t t0 = tstart();
-        const int nb = launch_gemvt<T>(strm, L, basis, ld, e->dcols, ncols, vnew, e->part, nullptr);
+        const int nb = launch_gemvt<T>(strm, L, basis, ld, e->dcols, ncols, nullptr, vnew, e->part,
+                                       Gate{});
         PK_LAUNCH(nb);
         tstop(t0, KC_GEMVT, (ncols + 1.0) * L * sb, 2.0 * ncols * L);
-        PK_LAUNCH(launch_reduce(strm, ncols, nb, e->part, e->red, nullptr));
+        PK_LAUNCH(launch_reduce(strm, ncols, nb, e->part, e->red, Gate{}));
         hipEvent_t t1 = tstart();
         const int nb2 =
-            launch_gemv_update<T>(strm, L, basis, ld, e->dcols, ncols, e->red, vnew, e->part, nullptr);
+            launch_gemv_update<T>(strm, L, basis, ld, e->dcols, ncols, nullptr, e->red, vnew,
+                                  e->part, Gate{});
         PK_LAUNCH(nb2);
         tstop(t1, KC_GEMV, (ncols + 2.0) * L * sb, 2.0 * ncols * L);
-        PK_LAUNCH(launch_reduce(strm, 1, nb2, e->part, e->red + kMaxCols, nullptr));
+        PK_LAUNCH(launch_reduce(strm, 1, nb2, e->part, e->red + kMaxCols, Gate{}));
         PK_TRY(read(kMaxCols, 1));
         *normvnew = std::sqrt(e->hred[kMaxCols]);
       } else {
@@ -456,6 +379,250 @@ struct Impl {
     return PK_OK;
   }
 
+  // One CGS pass of vnew against basis(:, c.cols) on the host executor:
+  // returns ||vnew'||^2.
+  int reo_pass(long long L, const T* basis, long long ld, T* vnew, const LbdCtl& c, double* s2) {
+    const int ncols = c.reo_ncols;
+    if (ncols > 0) {
+      std::memcpy(e->hcols, c.cols, sizeof(int) * ncols);
+      PK_HIP(hipMemcpyAsync(e->dcols, e->hcols, sizeof(int) * ncols, hipMemcpyHostToDevice, strm));
+      const double sb = static_cast<double>(sizeof(T));
+      hipEvent_t t0 = tstart();
+      const int nb = launch_gemvt<T>(strm, L, basis, ld, e->dcols, ncols, nullptr, vnew, e->part,
+                                     Gate{});
+      PK_LAUNCH(nb);
+      tstop(t0, KC_GEMVT, (ncols + 1.0) * L * sb, 2.0 * ncols * L);
+      PK_LAUNCH(launch_reduce(strm, ncols, nb, e->part, e->red, Gate{}));
+      hipEvent_t t1 = tstart();
+      const int nb2 = launch_gemv_update<T>(strm, L, basis, ld, e->dcols, ncols, nullptr, e->red,
+                                            vnew, e->part, Gate{});
+      PK_LAUNCH(nb2);
+      tstop(t1, KC_GEMV, (ncols + 2.0) * L * sb, 2.0 * ncols * L);
+      PK_LAUNCH(launch_reduce(strm, 1, nb2, e->part, e->red + kMaxCols, Gate{}));
+      PK_TRY(read(kMaxCols, 1));
+      *s2 = e->hred[kMaxCols];
+    } else {
+      double nrm;
+      PK_TRY(nrm2(vnew, L, &nrm));
+      *s2 = nrm * nrm;
+    }
+    return PK_OK;
+  }
+
+  int axpy_read(long long L, T* w, const T* v, double coef, double* ns2, double* nsd) {
+    const int nb = launch_axpy_norm_dot<T>(strm, L, w, v, sval(coef), e->part, Gate{});
+    PK_LAUNCH(nb);
+    PK_LAUNCH(launch_reduce(strm, 2, nb, e->part, e->red, Gate{}));
+    PK_TRY(read(0, 2));
+    *ns2 = e->hred[0];
+    *nsd = e->hred[1];
+    return PK_OK;
+  }
+
+  // Execute the current phase of the step on the host (same control
+  // functions as the device chain, plus the rare paths).
+  int host_phase(LbdCtl& c) {
+    const int j = c.j, m = e->m, n = e->n;
+    double s2 = 0, sd = 0, anormest = 0;
+    int e2 = 0;
+    switch (c.phase) {
+      case PH_V_SPMV:
+        if (c.uscale) {
+          PK_TRY(scale(ucol(j), m, c.uscale_f));
+          c.uscale = 0;
+        }
+        PK_TRY(apply('t', ucol(j), vcol(j), j > 1 ? vcol(j - 1) : nullptr, j > 1 ? c.beta : 0.0,
+                     &s2, &sd));
+        ctl_v_spmv(c, s2, sd);
+        break;
+      case PH_V_ELR:
+        PK_TRY(axpy_read(n, vcol(j), vcol(j - 1), c.elr_s, &s2, &sd));
+        ctl_v_elr(c, s2, sd);
+        break;
+      case PH_V_REO:
+        PK_TRY(reo_pass(n, V(), e->ldv, vcol(j), c, &s2));
+        ctl_v_reo(c, s2);
+        break;
+      case PH_V_INVARIANT:  // dlanbpro.F:372-409
+        c.rnorm = c.alpha;
+        c.alpha = 0.0;
+        PK_TRY(getu0('t', j - 1, 3, vcol(j), &c.alpha, V(), e->ldv, &anormest, &e2));
+        c.ierr = e2;
+        if (c.alpha == 0.0) {
+          c.k = j - 1;
+          c.ierr = -j;
+          c.phase = PH_DONE;
+        } else {
+          PK_TRY(safescal(n, c.alpha, vcol(j)));
+          c.alpha = 0.0;
+          c.force_reorth = 1;
+          if (c.delta > 0.0) c.full_reorth = 0;
+          c.B1[j] = 0.0;
+          c.vscale = 0;
+          c.phase = PH_U_SPMV;
+        }
+        break;
+      case PH_V_TINY:
+        PK_TRY(safescal(n, c.alpha, vcol(j)));
+        c.vscale = 0;
+        c.phase = PH_U_SPMV;
+        break;
+      case PH_V_ZERO:  // dreorth.F:96-99
+        PK_LAUNCH(launch_fill_zero_bytes(strm, vcol(j), sizeof(T) * n));
+        c.alpha = 0.0;
+        c.nreorth++;
+        ctl_v_after_reorth(c);
+        break;
+      case PH_U_SPMV:
+        if (c.vscale) {
+          PK_TRY(scale(vcol(j), n, c.vscale_f));
+          c.vscale = 0;
+        }
+        PK_TRY(apply('n', vcol(j), ucol(j + 1), ucol(j), c.alpha, &s2, &sd));
+        ctl_u_spmv(c, s2, sd);
+        break;
+      case PH_U_ELR:
+        PK_TRY(axpy_read(m, ucol(j + 1), ucol(j), c.elr_s, &s2, &sd));
+        ctl_u_elr(c, s2, sd);
+        break;
+      case PH_U_REO:
+        PK_TRY(reo_pass(m, U(), e->ldu, ucol(j + 1), c, &s2));
+        ctl_u_reo(c, s2);
+        break;
+      case PH_U_INVARIANT:  // dlanbpro.F:503-539
+        c.rnorm = c.beta;
+        c.beta = 0.0;
+        PK_TRY(getu0('n', j, 3, ucol(j + 1), &c.beta, U(), e->ldu, &anormest, &e2));
+        c.ierr = e2;
+        if (c.beta == 0.0) {
+          c.k = j;
+          c.ierr = -j;
+          c.phase = PH_DONE;
+        } else {
+          PK_TRY(safescal(m, c.beta, ucol(j + 1)));
+          c.beta = 0.0;
+          c.force_reorth = 1;
+          if (c.delta > 0.0) c.full_reorth = 0;
+          c.B2[j] = 0.0;
+          c.uscale = 0;
+          c.rnorm = 0.0;
+          c.j = j + 1;
+          c.phase = c.j > c.k ? PH_DONE : PH_V_SPMV;
+        }
+        break;
+      case PH_U_TINY:
+        PK_TRY(safescal(m, c.beta, ucol(j + 1)));
+        c.uscale = 0;
+        c.rnorm = c.beta;
+        c.j = j + 1;
+        c.phase = c.j > c.k ? PH_DONE : PH_V_SPMV;
+        break;
+      case PH_U_ZERO:
+        PK_LAUNCH(launch_fill_zero_bytes(strm, ucol(j + 1), sizeof(T) * m));
+        c.beta = 0.0;
+        c.nreorth++;
+        ctl_u_after_reorth(c);
+        break;
+      default:
+        set_error("lanbpro: unknown phase %d", c.phase);
+        return PK_ERR_ARG;
+    }
+    return PK_OK;
+  }
+
+  // Enqueue the fixed kernel chain of step J.  Every kernel is gated on the
+  // control state being in its phase of step J; control kernels advance the
+  // state.  Nothing here waits for the device.
+  int enqueue_step(int J, int elr, int tries) {
+    LbdCtl* dc = e->dctl;
+    const int m = e->m, n = e->n;
+    const double sb = static_cast<double>(sizeof(T));
+    // ---- v_J = A^T u_J - beta v_{J-1} ----
+    PK_LAUNCH(launch_scale<T>(strm, m, ucol(J), sptr(&dc->uscale_f), Gate{dc, PH_V_SPMV, J, 2}));
+    hipEvent_t t0 = tstart();
+    int nb = csr_apply<T>(e->op, strm, 't', ucol(J), vcol(J), J > 1 ? vcol(J - 1) : nullptr,
+                          sptr(&dc->beta), e->part, Gate{dc, PH_V_SPMV, J, 0});
+    PK_LAUNCH(nb);
+    tstop(t0, KC_SPMV_T, spmv_bytes(true, J > 1), 2.0 * static_cast<double>(e->op->nnz));
+    PK_LAUNCH(launch_ctl(strm, dc, CTL_V_SPMV, PH_V_SPMV, J, e->part, nb));
+    if (J > 1) {
+      for (int i = 0; i < elr; ++i) {
+        nb = launch_axpy_norm_dot<T>(strm, n, vcol(J), vcol(J - 1), sptr(&dc->elr_s), e->part,
+                                     Gate{dc, PH_V_ELR, J, 0});
+        PK_LAUNCH(nb);
+        PK_LAUNCH(launch_ctl(strm, dc, CTL_V_ELR, PH_V_ELR, J, e->part, nb));
+      }
+      for (int t = 0; t < tries; ++t) {
+        const Gate g{dc, PH_V_REO, J, 0};
+        hipEvent_t t1 = tstart();
+        nb = launch_gemvt<T>(strm, n, V(), e->ldv, dc->cols, J - 1, &dc->reo_ncols, vcol(J),
+                             e->part, g);
+        PK_LAUNCH(nb);
+        tstop(t1, KC_GEMVT, 0.0);
+        PK_LAUNCH(launch_reduce(strm, J - 1, nb, e->part, e->red, g, &dc->reo_ncols));
+        hipEvent_t t2 = tstart();
+        const int nb2 = launch_gemv_update<T>(strm, n, V(), e->ldv, dc->cols, J - 1,
+                                              &dc->reo_ncols, e->red, vcol(J), e->part, g);
+        PK_LAUNCH(nb2);
+        tstop(t2, KC_GEMV, 0.0);
+        PK_LAUNCH(launch_ctl(strm, dc, CTL_V_REO, PH_V_REO, J, e->part, nb2));
+      }
+    }
+    // ---- u_{J+1} = A v_J - alpha u_J ----
+    PK_LAUNCH(launch_scale<T>(strm, n, vcol(J), sptr(&dc->vscale_f), Gate{dc, PH_U_SPMV, J, 1}));
+    hipEvent_t t3 = tstart();
+    nb = csr_apply<T>(e->op, strm, 'n', vcol(J), ucol(J + 1), ucol(J), sptr(&dc->alpha), e->part,
+                      Gate{dc, PH_U_SPMV, J, 0});
+    PK_LAUNCH(nb);
+    tstop(t3, KC_SPMV_N, spmv_bytes(false, true), 2.0 * static_cast<double>(e->op->nnz));
+    PK_LAUNCH(launch_ctl(strm, dc, CTL_U_SPMV, PH_U_SPMV, J, e->part, nb));
+    for (int i = 0; i < elr; ++i) {
+      nb = launch_axpy_norm_dot<T>(strm, m, ucol(J + 1), ucol(J), sptr(&dc->elr_s), e->part,
+                                   Gate{dc, PH_U_ELR, J, 0});
+      PK_LAUNCH(nb);
+      PK_LAUNCH(launch_ctl(strm, dc, CTL_U_ELR, PH_U_ELR, J, e->part, nb));
+    }
+    for (int t = 0; t < tries; ++t) {
+      const Gate g{dc, PH_U_REO, J, 0};
+      hipEvent_t t1 = tstart();
+      nb = launch_gemvt<T>(strm, m, U(), e->ldu, dc->cols, J, &dc->reo_ncols, ucol(J + 1),
+                           e->part, g);
+      PK_LAUNCH(nb);
+      tstop(t1, KC_GEMVT, 0.0);
+      PK_LAUNCH(launch_reduce(strm, J, nb, e->part, e->red, g, &dc->reo_ncols));
+      hipEvent_t t2 = tstart();
+      const int nb2 = launch_gemv_update<T>(strm, m, U(), e->ldu, dc->cols, J, &dc->reo_ncols,
+                                            e->red, ucol(J + 1), e->part, g);
+      PK_LAUNCH(nb2);
+      tstop(t2, KC_GEMV, 0.0);
+      PK_LAUNCH(launch_ctl(strm, dc, CTL_U_REO, PH_U_REO, J, e->part, nb2));
+    }
+    (void)sb;
+    return PK_OK;
+  }
+
+  // Run steps c.j..c.k on the device until done or until the state needs
+  // the host (a rare path, or more CGS passes than the chain holds).
+  int run_device(LbdCtl& c, int elr) {
+    const long long pv0 = c.npass_v, pu0 = c.npass_u, cv0 = c.ncols_v, cu0 = c.ncols_u;
+    PK_HIP(hipMemcpyAsync(e->dctl, &c, sizeof(LbdCtl), hipMemcpyHostToDevice, strm));
+    for (int J = c.j; J <= c.k; ++J) PK_TRY(enqueue_step(J, elr, e->dev_tries));
+    PK_HIP(hipMemcpyAsync(&c, e->dctl, sizeof(LbdCtl), hipMemcpyDeviceToHost, strm));
+    PK_HIP(hipStreamSynchronize(strm));
+    e->cnt.nsync++;
+    // algorithmic bytes of the CGS passes the chain actually ran
+    if (e->prof.on) {
+      const double sb = static_cast<double>(sizeof(T));
+      const double pv = static_cast<double>(c.npass_v - pv0), pu = static_cast<double>(c.npass_u - pu0);
+      const double cv = static_cast<double>(c.ncols_v - cv0), cu = static_cast<double>(c.ncols_u - cu0);
+      e->prof.bytes[KC_GEMVT] += sb * (e->n * (cv + pv) + e->m * (cu + pu));
+      e->prof.bytes[KC_GEMV] += sb * (e->n * (cv + 2 * pv) + e->m * (cu + 2 * pu));
+      e->prof.flops[KC_GEMVT] += 2.0 * (e->n * cv + e->m * cu);
+      e->prof.flops[KC_GEMV] += 2.0 * (e->n * cv + e->m * cu);
+    }
+    return resolve_timers();
+  }
+
   // dlanbpro (double/dlanbpro.F:1-549).  alpha_/beta_ are B(:,1), B(:,2).
   int lanbpro(int k0, int* kp, double* alpha_, double* beta_, double* rnorm, double* doption,
               const int* ioption, int* ierr_out) {
@@ -468,14 +635,22 @@ struct Impl {
     double delta, eta, anorm, anormest = 0, s = 0, a1, b1, amax = 0, alpha, beta;
     bool full_reorth, force_reorth;
     int j0;
-    double* B1 = alpha_ - 1;  // B(j,1) = B1[j]
-    double* B2 = beta_ - 1;   // B(j,2) = B2[j]
-    const int elr = ioption[1];
-
     if (k > e->kmax || k0 < 0 || k0 > k) {
       set_error("lanbpro: k=%d k0=%d outside the engine's kmax=%d", k, k0, e->kmax);
       return PK_ERR_ARG;
     }
+    LbdCtl& c = *e->hctl;
+    std::memset(&c, 0, sizeof(LbdCtl));
+    double* mu = c.mu;
+    double* nu = c.nu;
+    int* index = c.index;
+    double* B1 = c.B1;  // B(j,1) = B1[j]
+    double* B2 = c.B2;  // B(j,2) = B2[j]
+    for (int i = 1; i <= k; ++i) {
+      B1[i] = alpha_[i - 1];
+      B2[i] = beta_[i - 1];
+    }
+
     delta = doption[0] < zero ? std::sqrt(eps / k) : doption[0];
     eta = doption[1] < zero ? std::pow(eps, 0.75) / std::sqrt(static_cast<double>(k)) : doption[1];
     full_reorth = (delta <= eta || delta == zero);
@@ -499,9 +674,6 @@ struct Impl {
       ierr = e2;
       anorm = std::max(anorm, anormest);
     }
-
-    std::vector<double> mu(k + 3, 0.0), nu(k + 3, 0.0);
-    std::vector<int> index(2 * k + 4, 0);
 
     if (k0 == 0) {
       amax = zero;
@@ -531,10 +703,10 @@ struct Impl {
       index[2] = k0;
       index[3] = k0 + 1;
       PK_TRY(scale(ucol(k0 + 1), m, *rnorm));
-      PK_TRY(reorth(m, k0, U(), e->ldu, ucol(k0 + 1), rnorm, index.data(), kappa));
+      PK_TRY(reorth(m, k0, U(), e->ldu, ucol(k0 + 1), rnorm, index, kappa));
       PK_TRY(safescal(m, *rnorm, ucol(k0 + 1)));
-      set_mu(k0, mu.data(), index.data(), epsn2);
-      set_mu(k0, nu.data(), index.data(), epsn2);
+      set_mu(k0, mu, index, epsn2);
+      set_mu(k0, nu, index, epsn2);
       beta = *rnorm;
       B2[k0] = beta;
       amax = zero;
@@ -555,179 +727,58 @@ struct Impl {
       }
       j0 = k0 + 1;
     }
-    double numax = zero, mumax = zero;
 
-    for (int j = j0; j <= k; ++j) {
-      // alpha_j v_j = A^T u_j - beta_j v_{j-1}   (dlanbpro.F:288-316),
-      // fused: SpMV + recurrence + ||v_j||^2 + v_{j-1}.v_j in one pass.
-      double s2, sd;
-      PK_TRY(apply('t', ucol(j), vcol(j), j > 1 ? vcol(j - 1) : nullptr, j > 1 ? beta : 0.0, &s2,
-                   &sd));
-      e->cnt.nopx++;
-      if (j == 1) {
-        alpha = std::sqrt(s2);
-        anorm = std::max(anorm, FUDGE * alpha);
-      } else {
-        alpha = std::sqrt(s2);
-        if (elr > 0 && alpha < kappa * beta) {
-          double dt = sd;
-          for (int i = 1; i <= elr; ++i) {
-            s = dt;
-            double ns2, nsd;
-            const int nb = launch_axpy_norm_dot<T>(strm, n, vcol(j), vcol(j - 1), sval(s),
-                                                   e->part, nullptr);
-            PK_LAUNCH(nb);
-            PK_LAUNCH(launch_reduce(strm, 2, nb, e->part, e->red, nullptr));
-            PK_TRY(read(0, 2));
-            ns2 = e->hred[0];
-            nsd = e->hred[1];
-            if (beta != zero) {
-              beta = beta + s;
-              B2[j - 1] = beta;
-            }
-            s = std::sqrt(ns2);
-            dt = nsd;
-            if (s >= kappa * alpha) break;
-            alpha = s;
-          }
-          nu[j - 1] = eps;
-          alpha = s;
-        }
-        B1[j] = alpha;
-        amax = std::max(amax, alpha);
-        if (j == 2) {
-          a1 = B2[1] / amax;
-          a1 = FUDGE * amax * std::sqrt((B1[1] / amax) * (B1[1] / amax) + a1 * a1 + B1[2] / amax * a1);
-        } else {
-          a1 = B1[j - 1] / amax;
-          b1 = B2[j - 1] / amax;
-          a1 = FUDGE * amax * std::sqrt(a1 * a1 + b1 * b1 + a1 * B2[j - 2] / amax + B1[j] / amax * b1);
-        }
-        anorm = std::max(anorm, a1);
-      }
-      // nu recurrence and reorthogonalization of v_j (dlanbpro.F:340-367)
-      if (!full_reorth && alpha != zero)
-        update_nu(&numax, mu.data(), nu.data(), j, B1, B2, anorm, epsn2);
-      if ((full_reorth || numax > delta || force_reorth) && alpha != zero) {
-        if (full_reorth || eta == zero) {
-          index[1] = 1;
-          index[2] = j - 1;
-          index[3] = j;
-        } else if (!force_reorth) {
-          compute_int(nu.data(), j - 1, delta, eta, index.data());
-        }
-        PK_TRY(reorth(n, j - 1, V(), e->ldv, vcol(j), &alpha, index.data(), kappa));
-        set_mu(j - 1, nu.data(), index.data(), eps);
-        numax = eta;
-        force_reorth = !force_reorth;
-      }
-      // invariant subspace for alpha (dlanbpro.F:372-409)
-      if (alpha < anorm * epsn && j < k) {
-        *rnorm = alpha;
-        alpha = zero;
-        int e2;
-        PK_TRY(getu0('t', j - 1, 3, vcol(j), &alpha, V(), e->ldv, &anormest, &e2));
-        ierr = e2;
-        if (alpha == zero) {
-          k = j - 1;
-          ierr = -j;
-          goto done;
-        }
-        PK_TRY(safescal(n, alpha, vcol(j)));
-        alpha = zero;
-        force_reorth = true;
-        if (delta > zero) full_reorth = false;
-      } else if (j > 1 && !full_reorth && j < k && (delta * alpha < anorm * eps)) {
-        ierr = j;
-      }
-      B1[j] = alpha;
-      if (alpha != zero) PK_TRY(safescal(n, alpha, vcol(j)));
+    c.m = m;
+    c.n = n;
+    c.k = k;
+    c.elr = ioption[1];
+    c.delta = delta;
+    c.eta = eta;
+    c.eps = eps;
+    c.epsn = epsn;
+    c.epsn2 = epsn2;
+    c.sfmin = Mach<T>::sfmin;
+    c.kappa = kappa;
+    c.fudge = FUDGE;
+    c.j = j0;
+    c.phase = j0 <= k ? PH_V_SPMV : PH_DONE;
+    c.full_reorth = full_reorth;
+    c.force_reorth = force_reorth;
+    c.ierr = ierr;
+    c.alpha = alpha;
+    c.beta = beta;
+    c.anorm = anorm;
+    c.amax = amax;
+    c.numax = zero;
+    c.mumax = zero;
+    c.rnorm = *rnorm;
 
-      // beta_{j+1} u_{j+1} = A v_j - alpha_j u_j   (dlanbpro.F:420-443)
-      PK_TRY(apply('n', vcol(j), ucol(j + 1), ucol(j), alpha, &s2, &sd));
-      e->cnt.nopx++;
-      beta = std::sqrt(s2);
-      if (elr > 0 && beta < kappa * alpha) {
-        double dt = sd;
-        for (int i = 1; i <= elr; ++i) {
-          s = dt;
-          const int nb = launch_axpy_norm_dot<T>(strm, m, ucol(j + 1), ucol(j), sval(s),
-                                                 e->part, nullptr);
-          PK_LAUNCH(nb);
-          PK_LAUNCH(launch_reduce(strm, 2, nb, e->part, e->red, nullptr));
-          PK_TRY(read(0, 2));
-          const double ns2 = e->hred[0], nsd = e->hred[1];
-          if (alpha != zero) {
-            alpha = alpha + s;
-            B1[j] = alpha;
-          }
-          s = std::sqrt(ns2);
-          dt = nsd;
-          if (s >= kappa * beta) break;
-          beta = s;
-        }
-        mu[j] = eps;
-        beta = s;
+    const bool device = e->mode == 0 && e->op != nullptr;
+    while (c.phase != PH_DONE) {
+      if (device && c.phase == PH_V_SPMV) {
+        const int j_before = c.j;
+        PK_TRY(run_device(c, c.elr));
+        // A chain always advances at least one phase; if it did not, finish
+        // the step on the host rather than enqueue the same chain again.
+        if (c.phase != PH_V_SPMV || c.j != j_before) continue;
       }
-      B2[j] = beta;
-      amax = std::max(amax, beta);
-      if (j <= 1) {
-        a1 = lapy2(B1[1], B2[1]);
-      } else {
-        a1 = B1[j] / amax;
-        a1 = amax * std::sqrt(a1 * a1 + (B2[j] / amax) * (B2[j] / amax) + a1 * B2[j - 1] / amax);
-      }
-      anorm = std::max(anorm, a1);
-
-      // mu recurrence and reorthogonalization of u_{j+1} (dlanbpro.F:451-498)
-      if (!full_reorth && beta != zero)
-        update_mu(&mumax, mu.data(), nu.data(), j, B1, B2, anorm, epsn2);
-      if ((full_reorth || mumax > delta || force_reorth) && beta != zero) {
-        if (full_reorth || eta == zero) {
-          index[1] = 1;
-          index[2] = j;
-          index[3] = j + 1;
-        } else if (!force_reorth) {
-          compute_int(mu.data(), j, delta, eta, index.data());
-        } else {
-          for (int i = 1; i <= 2 * j + 1; ++i)
-            if (index[i] == j) {
-              index[i] = j + 1;
-              break;
-            }
-        }
-        PK_TRY(reorth(m, j, U(), e->ldu, ucol(j + 1), &beta, index.data(), kappa));
-        set_mu(j, mu.data(), index.data(), eps);
-        mumax = eta;
-        force_reorth = !force_reorth;
-      }
-      // invariant subspace for beta (dlanbpro.F:503-539)
-      if (beta < anorm * epsn && j < k) {
-        *rnorm = beta;
-        beta = zero;
-        int e2;
-        PK_TRY(getu0('n', j, 3, ucol(j + 1), &beta, U(), e->ldu, &anormest, &e2));
-        ierr = e2;
-        if (beta == zero) {
-          k = j;
-          ierr = -j;
-          goto done;
-        }
-        PK_TRY(safescal(m, beta, ucol(j + 1)));
-        beta = zero;
-        force_reorth = true;
-        if (delta > zero) full_reorth = false;
-      } else if (!full_reorth && j < k && (delta * beta < anorm * eps)) {
-        ierr = j;
-      }
-      B2[j] = beta;
-      if (beta != zero && beta != one) PK_TRY(safescal(m, beta, ucol(j + 1)));
-      *rnorm = beta;
+      PK_TRY(host_phase(c));
     }
-  done:
-    doption[2] = anorm;
-    *kp = k;
-    *ierr_out = ierr;
+
+    for (int i = 1; i <= c.k; ++i) {
+      alpha_[i - 1] = B1[i];
+      beta_[i - 1] = B2[i];
+    }
+    if (k0 > 0 && k0 <= c.k) beta_[k0 - 1] = B2[k0];
+    doption[2] = c.anorm;
+    *kp = c.k;
+    *ierr_out = c.ierr;
+    *rnorm = c.rnorm;
+    e->cnt.nopx += c.nopx;
+    e->cnt.nreorth += c.nreorth;
+    e->cnt.ndot += c.ndot;
+    e->cnt.ncols_touched += c.ncols_touched;
+    e->cnt.nretry += c.nretry;
     PK_HIP(hipStreamSynchronize(strm));
     return resolve_timers();
   }
@@ -758,7 +809,7 @@ struct Impl {
             tb ? B[static_cast<size_t>(l) * ldb + j] : B[static_cast<size_t>(j) * ldb + l];
     PK_HIP(hipMemcpyAsync(e->dB, e->hB, sizeof(double) * kdim * ncols, hipMemcpyHostToDevice, strm));
     hipEvent_t t0 = tstart();
-    PK_LAUNCH(launch_gemm_tall<T>(strm, L, static_cast<const T*>(buf[cur]), ld, kdim, e->dB, kdim,
+    PK_LAUNCH(launch_gemm_mfma<T>(strm, L, static_cast<const T*>(buf[cur]), ld, kdim, e->dB, kdim,
                                   ncols, static_cast<T*>(buf[1 - cur]), ld));
     tstop(t0, KC_GEMM, static_cast<double>(sizeof(T)) * L * (kdim + ncols),
           2.0 * static_cast<double>(L) * kdim * ncols);
@@ -802,6 +853,8 @@ static void engine_free(Engine* e) {
   (void)hipFree(e->dcols);
   (void)hipFree(e->dB);
   (void)hipFree(e->vec);
+  (void)hipFree(e->dctl);
+  (void)hipHostFree(e->hctl);
   (void)hipHostFree(e->hred);
   (void)hipHostFree(e->hcols);
   (void)hipHostFree(e->hB);
@@ -826,7 +879,8 @@ extern "C" {
 const char* pk_last_error(void) { return pk::last_error(); }
 
 int pk_lbd_create(int prec, int m, int n, int kmax, pk_csr_t op, pk_lbd_t* out) {
-  if (!out || m < 1 || n < 1 || kmax < 1 || (prec != PK_F64 && prec != PK_F32)) {
+  if (!out || m < 1 || n < 1 || kmax < 1 || kmax > kMaxK - 2 ||
+      (prec != PK_F64 && prec != PK_F32)) {
     set_error("pk_lbd_create: bad arguments m=%d n=%d kmax=%d prec=%d", m, n, kmax, prec);
     return PK_ERR_ARG;
   }
@@ -854,7 +908,12 @@ int pk_lbd_create(int prec, int m, int n, int kmax, pk_csr_t op, pk_lbd_t* out) 
   e->ldv = round_up(n, 64);
   e->op = cop;
   const long long mx = std::max(m, n);
-  const size_t nblk = static_cast<size_t>(std::max(stream_blocks(mx), kMaxBlocks));
+  size_t nblk = static_cast<size_t>(std::max(stream_blocks(mx), kMaxBlocks));
+  if (cop) {
+    // the SpMV writes two partials per row block
+    const size_t need = 2 * static_cast<size_t>(std::max(cop->nrb, cop->nrbt));
+    nblk = std::max(nblk, (need + kMaxCols - 1) / kMaxCols);
+  }
   const size_t nb = static_cast<size_t>(kmax + 1) * (kmax + 1);
   hipError_t err = hipSuccess;
   auto chk = [&](hipError_t r) {
@@ -872,6 +931,9 @@ int pk_lbd_create(int prec, int m, int n, int kmax, pk_csr_t op, pk_lbd_t* out) 
   chk(hipHostMalloc(&e->hcols, sizeof(int) * kMaxCols, hipHostMallocDefault));
   chk(hipHostMalloc(&e->hB, sizeof(double) * nb, hipHostMallocDefault));
   chk(hipHostMalloc(&e->hvec, sizeof(double) * mx, hipHostMallocDefault));
+  chk(hipMalloc(&e->dctl, sizeof(LbdCtl)));
+  chk(hipHostMalloc(&e->hctl, sizeof(LbdCtl), hipHostMallocDefault));
+  if (const char* md = std::getenv("PK_LBD_MODE")) e->mode = (md[0] == 'h' || md[0] == 'H') ? 1 : 0;
   if (err == hipSuccess) {
     // The bases start zeroed so that unwritten columns never hold NaN garbage.
     chk(hipMemsetAsync(e->U[0], 0, e->tbytes * e->ldu * (kmax + 1), e->stream));
@@ -1027,12 +1089,21 @@ int pk_lbd_stats(pk_lbd_t h, long long* out) {
   out[2] = e->cnt.ndot;
   out[3] = e->cnt.ncols_touched;
   out[4] = e->cnt.nsync;
+  out[5] = e->cnt.nretry;
   return PK_OK;
 }
 
 int pk_lbd_reset_stats(pk_lbd_t h) {
   PK_ENGINE(h);
   e->cnt = Counters();
+  return PK_OK;
+}
+
+int pk_lbd_set_mode(pk_lbd_t h, int mode, int tries) {
+  PK_ENGINE(h);
+  if ((mode != 0 && mode != 1) || tries < 1 || tries > 5) return PK_ERR_ARG;
+  e->mode = mode;
+  e->dev_tries = tries;
   return PK_OK;
 }
 
@@ -1103,19 +1174,19 @@ int pk_larnv_pm1(int prec, long long n, void* x) {
 
 void pkh_update_mu(double* mumax, double* mu, const double* nu, int j, const double* alpha,
                    const double* beta, double anorm, double eps1) {
-  update_mu(mumax, mu, nu, j, alpha, beta, anorm, eps1);
+  hd_update_mu(mumax, mu, nu, j, alpha, beta, anorm, eps1);
 }
 
 void pkh_update_nu(double* numax, const double* mu, double* nu, int j, const double* alpha,
                    const double* beta, double anorm, double eps1) {
-  update_nu(numax, mu, nu, j, alpha, beta, anorm, eps1);
+  hd_update_nu(numax, mu, nu, j, alpha, beta, anorm, eps1);
 }
 
 void pkh_compute_int(const double* mu, int j, double delta, double eta, int* index) {
-  compute_int(mu, j, delta, eta, index);
+  hd_compute_int(mu, j, delta, eta, index);
 }
 
-void pkh_set_mu(int k, double* mu, const int* index, double val) { set_mu(k, mu, index, val); }
+void pkh_set_mu(int k, double* mu, const int* index, double val) { hd_set_mu(k, mu, index, val); }
 
 }  // extern "C"
 

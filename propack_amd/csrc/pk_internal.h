@@ -21,6 +21,8 @@ struct CsrOp {
   int *rpt = nullptr, *cit = nullptr; // A^T (explicit)
   void* vat = nullptr;
   int lpr_n = 8, lpr_t = 8;           // lanes per row for A and A^T
+  int *rb = nullptr, *rbt = nullptr;  // row-block partitions of A and A^T
+  int nrb = 0, nrbt = 0;              // number of row blocks
 };
 
 int csr_create(int m, int n, long long nnz, const int* rowptr, const int* colind,
@@ -28,7 +30,7 @@ int csr_create(int m, int n, long long nnz, const int* rowptr, const int* colind
 void csr_destroy(CsrOp* op);
 template <typename T>
 int csr_apply(const CsrOp* op, hipStream_t s, char trans, const T* x, T* y, const T* yprev,
-              Scal coef, double* part, const int* gate);
+              Scal coef, double* part, Gate gate);
 
 // Fortran APROD callback: (TRANSA, M, N, X, Y, DPARM, IPARM, len(TRANSA)).
 typedef void (*aprod_d_fn)(const char*, const int*, const int*, const double*, double*,
@@ -52,7 +54,7 @@ struct Profile {
 };
 
 struct Counters {
-  long long nopx = 0, nreorth = 0, ndot = 0, ncols_touched = 0, nsync = 0;
+  long long nopx = 0, nreorth = 0, ndot = 0, ncols_touched = 0, nsync = 0, nretry = 0;
 };
 
 struct Engine {
@@ -84,6 +86,10 @@ struct Engine {
   double* hB = nullptr;              // pinned host staging
   void* vec = nullptr;               // scratch vector, max(m, n)
   double* hvec = nullptr;            // pinned host scratch (random start vectors)
+  LbdCtl* dctl = nullptr;            // step control state (device)
+  LbdCtl* hctl = nullptr;            // its pinned host copy
+  int mode = 0;                      // 0: device-resident steps, 1: host executor
+  int dev_tries = 2;                 // CGS passes per side held by a device chain
   Counters cnt;
   Profile prof;
   int last_error = 0;

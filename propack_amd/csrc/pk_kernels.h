@@ -3,6 +3,7 @@
 #pragma once
 
 #include "pk_common.h"
+#include "pk_lbd.h"
 
 namespace pk {
 
@@ -16,31 +17,42 @@ int stream_blocks(long long L);
 template <typename VT, typename T>
 int launch_spmv(hipStream_t s, int rows, const int* rp, const int* ci, const VT* va,
                 const T* x, T* y, const T* yprev, Scal coef, double* part, int lpr,
-                const int* gate);
+                Gate gate);
+
+// Row-block SpMV: rb (nblk+1) partitions the rows into blocks of at most
+// kSpmvRows rows and kSpmvCap nonzeros (or a single longer row); g lanes
+// per row in the LDS reduction.  Same outputs and partials as launch_spmv,
+// one partial pair per row block.
+constexpr int kSpmvCap = 4096;
+constexpr int kSpmvRows = 256;
+template <typename VT, typename T>
+int launch_spmv_rb(hipStream_t s, int nblk, const int* rb, const int* rp, const int* ci,
+                   const VT* va, const T* x, T* y, const T* yprev, Scal coef, double* part,
+                   int g, Gate gate);
 
 // part (nval x nblk) -> out[v] = sum_b part[v*nblk+b] in index order.
 int launch_reduce(hipStream_t s, int nval, int nblk, const double* part, double* out,
-                  const int* gate);
+                  Gate gate, const int* nval_p = nullptr);
 
 // h_part(ncols x nblk) = partial V(:,cols)^T w over row blocks.
 template <typename T>
 int launch_gemvt(hipStream_t s, long long L, const T* V, long long ldv, const int* cols,
-                 int ncols, const T* w, double* part, const int* gate);
+                 int ncols, const int* ncols_p, const T* w, double* part, Gate gate);
 
 // w -= V(:,cols) h ; partials sum w^2 (1 value per block).
 template <typename T>
 int launch_gemv_update(hipStream_t s, long long L, const T* V, long long ldv,
-                       const int* cols, int ncols, const double* h, T* w, double* part,
-                       const int* gate);
+                       const int* cols, int ncols, const int* ncols_p, const double* h, T* w,
+                       double* part, Gate gate);
 
 // x *= a
 template <typename T>
-int launch_scale(hipStream_t s, long long L, T* x, Scal a, const int* gate);
+int launch_scale(hipStream_t s, long long L, T* x, Scal a, Gate gate);
 
 // w -= c * v ; partials: sum w^2, sum v.w  (2 values per block)
 template <typename T>
 int launch_axpy_norm_dot(hipStream_t s, long long L, T* w, const T* v, Scal c, double* part,
-                         const int* gate);
+                         Gate gate);
 
 // partials: sum x^2 (1 value per block)
 template <typename T>
@@ -54,6 +66,16 @@ int launch_convert(hipStream_t s, long long L, const S* src, D* dst);
 template <typename T>
 int launch_gemm_tall(hipStream_t s, long long L, const T* A, long long lda, int kdim,
                      const double* B, int ldb, int ncols, T* C, long long ldc);
+
+// Same product on FP64 matrix cores (v_mfma_f64_16x16x4_f64).
+template <typename T>
+int launch_gemm_mfma(hipStream_t s, long long L, const T* A, long long lda, int kdim,
+                     const double* B, int ldb, int ncols, T* C, long long ldc);
+
+// Control kernel of the device-resident step (see pk_lbd.h).
+enum CtlKind { CTL_V_SPMV = 0, CTL_V_ELR, CTL_V_REO, CTL_U_SPMV, CTL_U_ELR, CTL_U_REO };
+int launch_ctl(hipStream_t s, LbdCtl* c, int kind, int want_phase, int want_j,
+               const double* part, int nb);
 
 int launch_fill_zero_bytes(hipStream_t s, void* p, size_t bytes);
 

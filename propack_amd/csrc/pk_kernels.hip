@@ -30,8 +30,8 @@ __global__ __launch_bounds__(kBlock) void k_spmv(int rows, const int* __restrict
                                                  const VT* __restrict__ va,
                                                  const T* __restrict__ x, T* __restrict__ y,
                                                  const T* __restrict__ yprev, Scal coef,
-                                                 double* __restrict__ part, const int* gate) {
-  if (gated_off(gate)) return;
+                                                 double* __restrict__ part, Gate gate) {
+  if (gate.off()) return;
   __shared__ double lds[(kBlock / kWave) * 2];
   const double c = coef.get();
   const int lane = threadIdx.x & (LPR - 1);
@@ -70,7 +70,7 @@ __global__ __launch_bounds__(kBlock) void k_spmv(int rows, const int* __restrict
 template <typename VT, typename T>
 int launch_spmv(hipStream_t s, int rows, const int* rp, const int* ci, const VT* va,
                 const T* x, T* y, const T* yprev, Scal coef, double* part, int lpr,
-                const int* gate) {
+                Gate gate) {
   const int nblk = kMaxBlocks;
 #define PK_SPMV_CASE(L)                                                                    \
   case L:                                                                                  \
@@ -92,22 +92,135 @@ int launch_spmv(hipStream_t s, int rows, const int* rp, const int* ci, const VT*
   return hipGetLastError() == hipSuccess ? nblk : -1;
 }
 
+// ------------------------------------------------------ row-block SpMV ----
+// Workgroup b owns the rows [rb[b], rb[b+1]) whose nonzeros (at most
+// kSpmvCap, at most kSpmvRows rows) are first turned into products
+// val * x[col] in LDS by all 256 threads -- index and value streams read
+// fully coalesced, kUnroll independent gathers in flight per thread -- and
+// then summed row by row in LDS by groups of G lanes (conflict-free
+// consecutive reads, shuffle reduction), in a fixed order.  A block that
+// holds a single row longer than kSpmvCap sums it with the whole workgroup.
+template <typename VT, typename T, int G>
+__global__ __launch_bounds__(kBlock) void k_spmv_rb(const int* __restrict__ rb,
+                                                    const int* __restrict__ rp,
+                                                    const int* __restrict__ ci,
+                                                    const VT* __restrict__ va,
+                                                    const T* __restrict__ x, T* __restrict__ y,
+                                                    const T* __restrict__ yprev, Scal coef,
+                                                    double* __restrict__ part, Gate gate) {
+  if (gate.off()) return;
+  __shared__ double prod[kSpmvCap];
+  __shared__ double lds[(kBlock / kWave) * 2];
+  const double c = coef.get();
+  const int r0 = rb[blockIdx.x], r1 = rb[blockIdx.x + 1];
+  const int p0 = rp[r0];
+  const int nz = rp[r1] - p0;
+  double s2 = 0.0, sd = 0.0;
+  if (nz <= kSpmvCap) {
+    constexpr int kUnroll = 8;
+    for (int base = 0; base < nz; base += kBlock * kUnroll) {
+      int idx[kUnroll];
+      double v[kUnroll], xv[kUnroll];
+#pragma unroll
+      for (int u = 0; u < kUnroll; ++u) {
+        const int t = base + u * kBlock + static_cast<int>(threadIdx.x);
+        idx[u] = t < nz ? ci[p0 + t] : -1;
+        v[u] = t < nz ? static_cast<double>(va[p0 + t]) : 0.0;
+      }
+#pragma unroll
+      for (int u = 0; u < kUnroll; ++u) xv[u] = idx[u] >= 0 ? static_cast<double>(x[idx[u]]) : 0.0;
+#pragma unroll
+      for (int u = 0; u < kUnroll; ++u) {
+        const int t = base + u * kBlock + static_cast<int>(threadIdx.x);
+        if (t < nz) prod[t] = v[u] * xv[u];
+      }
+    }
+    __syncthreads();
+    const int lane = threadIdx.x & (G - 1);
+    const int group = threadIdx.x / G;
+    constexpr int ngroups = kBlock / G;
+    const int nrows = r1 - r0;
+    const int iters = (nrows + ngroups - 1) / ngroups;  // uniform across the block
+    for (int it = 0; it < iters; ++it) {
+      const int r = r0 + group + it * ngroups;
+      const bool valid = r < r1;
+      int a = 0, b = 0;
+      if (valid) {
+        a = rp[r] - p0;
+        b = rp[r + 1] - p0;
+      }
+      double acc = 0.0;
+      for (int q = a + lane; q < b; q += G) acc += prod[q];
+      acc = group_sum<G>(acc);
+      if (valid && lane == 0) {
+        const double yp = yprev ? static_cast<double>(yprev[r]) : 0.0;
+        const T yv = static_cast<T>(acc - c * yp);
+        y[r] = yv;
+        s2 += static_cast<double>(yv) * static_cast<double>(yv);
+        sd += yp * static_cast<double>(yv);
+      }
+    }
+  } else {
+    // a single long row (r1 == r0 + 1)
+    double acc[1] = {0.0};
+    for (int p = p0 + static_cast<int>(threadIdx.x); p < p0 + nz; p += kBlock)
+      acc[0] += static_cast<double>(va[p]) * static_cast<double>(x[ci[p]]);
+    block_sum<1>(acc, lds);
+    if (threadIdx.x == 0) {
+      const double yp = yprev ? static_cast<double>(yprev[r0]) : 0.0;
+      const T yv = static_cast<T>(acc[0] - c * yp);
+      y[r0] = yv;
+      s2 = static_cast<double>(yv) * static_cast<double>(yv);
+      sd = yp * static_cast<double>(yv);
+    }
+  }
+  double v2[2] = {s2, sd};
+  block_sum<2>(v2, lds);
+  if (threadIdx.x == 0) {
+    part[blockIdx.x] = v2[0];
+    part[gridDim.x + blockIdx.x] = v2[1];
+  }
+}
+
+template <typename VT, typename T>
+int launch_spmv_rb(hipStream_t s, int nblk, const int* rb, const int* rp, const int* ci,
+                   const VT* va, const T* x, T* y, const T* yprev, Scal coef, double* part,
+                   int g, Gate gate) {
+  if (nblk <= 0) return 0;
+#define PK_RB_CASE(G)                                                                        \
+  case G:                                                                                    \
+    hipLaunchKernelGGL((k_spmv_rb<VT, T, G>), dim3(nblk), dim3(kBlock), 0, s, rb, rp, ci, va, \
+                       x, y, yprev, coef, part, gate);                                       \
+    break;
+  switch (g) {
+    PK_RB_CASE(2)
+    PK_RB_CASE(4)
+    PK_RB_CASE(8)
+    PK_RB_CASE(16)
+    PK_RB_CASE(32)
+    PK_RB_CASE(64)
+    default:
+      return -1;
+  }
+#undef PK_RB_CASE
+  return hipGetLastError() == hipSuccess ? nblk : -1;
+}
+
 // -------------------------------------------------------------- reduce ----
 __global__ __launch_bounds__(kBlock) void k_reduce(int nblk, const double* __restrict__ part,
-                                                   double* __restrict__ out, const int* gate) {
-  if (gated_off(gate)) return;
+                                                   double* __restrict__ out,
+                                                   const int* nval_p, Gate gate) {
+  if (gate.off()) return;
+  if (nval_p && static_cast<int>(blockIdx.x) >= *nval_p) return;
   __shared__ double lds[kBlock / kWave];
-  const double* p = part + static_cast<size_t>(blockIdx.x) * nblk;
-  double v[1] = {0.0};
-  for (int i = threadIdx.x; i < nblk; i += kBlock) v[0] += p[i];
-  block_sum<1>(v, lds);
-  if (threadIdx.x == 0) out[blockIdx.x] = v[0];
+  const double v = reduce_partials(part + static_cast<size_t>(blockIdx.x) * nblk, nblk, lds);
+  if (threadIdx.x == 0) out[blockIdx.x] = v;
 }
 
 int launch_reduce(hipStream_t s, int nval, int nblk, const double* part, double* out,
-                  const int* gate) {
+                  Gate gate, const int* nval_p) {
   if (nval <= 0) return 0;
-  hipLaunchKernelGGL(k_reduce, dim3(nval), dim3(kBlock), 0, s, nblk, part, out, gate);
+  hipLaunchKernelGGL(k_reduce, dim3(nval), dim3(kBlock), 0, s, nblk, part, out, nval_p, gate);
   return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 
@@ -119,9 +232,11 @@ constexpr int kCT = 8;
 template <typename T>
 __global__ __launch_bounds__(kBlock) void k_gemvt(long long L, const T* __restrict__ V,
                                                   long long ldv, const int* __restrict__ cols,
-                                                  int ncols, const T* __restrict__ w,
-                                                  double* __restrict__ part, const int* gate) {
-  if (gated_off(gate)) return;
+                                                  int ncols_v, const int* ncols_p,
+                                                  const T* __restrict__ w,
+                                                  double* __restrict__ part, Gate gate) {
+  if (gate.off()) return;
+  const int ncols = ncols_p ? *ncols_p : ncols_v;
   __shared__ int scol[kMaxCols];
   __shared__ double lds[2][(kBlock / kWave) * kCT];
   for (int i = threadIdx.x; i < ncols; i += kBlock) scol[i] = cols[i];
@@ -169,11 +284,11 @@ __global__ __launch_bounds__(kBlock) void k_gemvt(long long L, const T* __restri
 
 template <typename T>
 int launch_gemvt(hipStream_t s, long long L, const T* V, long long ldv, const int* cols,
-                 int ncols, const T* w, double* part, const int* gate) {
+                 int ncols, const int* ncols_p, const T* w, double* part, Gate gate) {
   if (ncols > kMaxCols) return -1;
   const int nblk = stream_blocks(L);
-  hipLaunchKernelGGL(k_gemvt<T>, dim3(nblk), dim3(kBlock), 0, s, L, V, ldv, cols, ncols, w,
-                     part, gate);
+  hipLaunchKernelGGL(k_gemvt<T>, dim3(nblk), dim3(kBlock), 0, s, L, V, ldv, cols, ncols,
+                     ncols_p, w, part, gate);
   return hipGetLastError() == hipSuccess ? nblk : -1;
 }
 
@@ -181,12 +296,14 @@ int launch_gemvt(hipStream_t s, long long L, const T* V, long long ldv, const in
 template <typename T>
 __global__ __launch_bounds__(kBlock) void k_gemv_update(long long L, const T* __restrict__ V,
                                                         long long ldv,
-                                                        const int* __restrict__ cols, int ncols,
+                                                        const int* __restrict__ cols,
+                                                        int ncols_v, const int* ncols_p,
                                                         const double* __restrict__ h,
                                                         T* __restrict__ w,
                                                         double* __restrict__ part,
-                                                        const int* gate) {
-  if (gated_off(gate)) return;
+                                                        Gate gate) {
+  if (gate.off()) return;
+  const int ncols = ncols_p ? *ncols_p : ncols_v;
   __shared__ int scol[kMaxCols];
   __shared__ double sh[kMaxCols];
   __shared__ double lds[kBlock / kWave];
@@ -227,19 +344,20 @@ __global__ __launch_bounds__(kBlock) void k_gemv_update(long long L, const T* __
 
 template <typename T>
 int launch_gemv_update(hipStream_t s, long long L, const T* V, long long ldv, const int* cols,
-                       int ncols, const double* h, T* w, double* part, const int* gate) {
+                       int ncols, const int* ncols_p, const double* h, T* w, double* part,
+                       Gate gate) {
   if (ncols > kMaxCols) return -1;
   const int nblk = stream_blocks(L);
   hipLaunchKernelGGL(k_gemv_update<T>, dim3(nblk), dim3(kBlock), 0, s, L, V, ldv, cols, ncols,
-                     h, w, part, gate);
+                     ncols_p, h, w, part, gate);
   return hipGetLastError() == hipSuccess ? nblk : -1;
 }
 
 // ------------------------------------------------------------- level 1 ----
 template <typename T>
 __global__ __launch_bounds__(kBlock) void k_scale(long long L, T* __restrict__ x, Scal a,
-                                                  const int* gate) {
-  if (gated_off(gate)) return;
+                                                  Gate gate) {
+  if (gate.off()) return;
   const double av = a.get();
   const long long base = static_cast<long long>(blockIdx.x) * kRowsPerBlock + threadIdx.x;
 #pragma unroll
@@ -250,7 +368,7 @@ __global__ __launch_bounds__(kBlock) void k_scale(long long L, T* __restrict__ x
 }
 
 template <typename T>
-int launch_scale(hipStream_t s, long long L, T* x, Scal a, const int* gate) {
+int launch_scale(hipStream_t s, long long L, T* x, Scal a, Gate gate) {
   hipLaunchKernelGGL(k_scale<T>, dim3(stream_blocks(L)), dim3(kBlock), 0, s, L, x, a, gate);
   return hipGetLastError() == hipSuccess ? 0 : -1;
 }
@@ -259,8 +377,8 @@ template <typename T>
 __global__ __launch_bounds__(kBlock) void k_axpy_norm_dot(long long L, T* __restrict__ w,
                                                           const T* __restrict__ v, Scal c,
                                                           double* __restrict__ part,
-                                                          const int* gate) {
-  if (gated_off(gate)) return;
+                                                          Gate gate) {
+  if (gate.off()) return;
   __shared__ double lds[(kBlock / kWave) * 2];
   const double cv = c.get();
   const long long base = static_cast<long long>(blockIdx.x) * kRowsPerBlock + threadIdx.x;
@@ -285,7 +403,7 @@ __global__ __launch_bounds__(kBlock) void k_axpy_norm_dot(long long L, T* __rest
 
 template <typename T>
 int launch_axpy_norm_dot(hipStream_t s, long long L, T* w, const T* v, Scal c, double* part,
-                         const int* gate) {
+                         Gate gate) {
   const int nblk = stream_blocks(L);
   hipLaunchKernelGGL(k_axpy_norm_dot<T>, dim3(nblk), dim3(kBlock), 0, s, L, w, v, c, part,
                      gate);
@@ -400,34 +518,161 @@ int launch_gemm_tall(hipStream_t s, long long L, const T* A, long long lda, int 
   return 0;
 }
 
+// ------------------------------------------------------------- control ----
+// One workgroup: reduce the partials of the preceding vector kernel (same
+// order as k_reduce), stage the control state in LDS, let thread 0 run the
+// step's scalar logic, write the state back.  Runs only while the state is
+// in (want_phase, want_j); otherwise the whole chain for this step is idle.
+__global__ __launch_bounds__(kBlock) void k_ctl(LbdCtl* __restrict__ gc, int kind, int want_phase,
+                                                int want_j, const double* __restrict__ part,
+                                                int nb) {
+  __shared__ LbdCtl c;
+  __shared__ double lds[kBlock / kWave];
+  if (gc->phase != want_phase || gc->j != want_j) return;
+  const bool two = (kind == CTL_V_SPMV || kind == CTL_V_ELR || kind == CTL_U_SPMV ||
+                    kind == CTL_U_ELR);
+  const double a = reduce_partials(part, nb, lds);
+  const double b = two ? reduce_partials(part + nb, nb, lds) : 0.0;
+  constexpr int nw = sizeof(LbdCtl) / sizeof(int);
+  static_assert(sizeof(LbdCtl) % sizeof(int) == 0, "state must be whole words");
+  const int* src = reinterpret_cast<const int*>(gc);
+  int* dst = reinterpret_cast<int*>(&c);
+  for (int i = threadIdx.x; i < nw; i += kBlock) dst[i] = src[i];
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    switch (kind) {
+      case CTL_V_SPMV: ctl_v_spmv(c, a, b); break;
+      case CTL_V_ELR: ctl_v_elr(c, a, b); break;
+      case CTL_V_REO: ctl_v_reo(c, a); break;
+      case CTL_U_SPMV: ctl_u_spmv(c, a, b); break;
+      case CTL_U_ELR: ctl_u_elr(c, a, b); break;
+      case CTL_U_REO: ctl_u_reo(c, a); break;
+      default: break;
+    }
+  }
+  __syncthreads();
+  int* back = reinterpret_cast<int*>(gc);
+  for (int i = threadIdx.x; i < nw; i += kBlock) back[i] = dst[i];
+}
+
+int launch_ctl(hipStream_t s, LbdCtl* c, int kind, int want_phase, int want_j,
+               const double* part, int nb) {
+  hipLaunchKernelGGL(k_ctl, dim3(1), dim3(kBlock), 0, s, c, kind, want_phase, want_j, part, nb);
+  return hipGetLastError() == hipSuccess ? 0 : -1;
+}
+
+// ------------------------------------------------------------ MFMA GEMM ----
+// C(:,0:ncols) = A(:,0:kdim) * B on FP64 matrix cores
+// (v_mfma_f64_16x16x4_f64).  Each wave owns 16 rows of C and up to 64
+// columns (four 16-wide column tiles).  The product is formed transposed,
+// C^T tile = B^T(16 x 4) * A^T(4 x 16), so that:
+//   * the A^T operand of lane l is A[i0 + l%16, k + l/16]: four 128-byte
+//     row segments of column-major A per instruction (coalesced);
+//   * the B^T operand of lane l is B[k + l/16, j0 + l%16] (small, cached);
+//   * accumulator r of lane l holds row l/16 + 4r of the 16x16 result, i.e.
+//     C[i0 + l%16, j0 + l/16 + 4r], so each group of 16 lanes stores 128
+//     contiguous bytes of one C column.
+// Out-of-range rows and the k padding to a multiple of 4 read as zero.
+typedef double v4d __attribute__((ext_vector_type(4)));
+constexpr int kMfmaRowsPerWave = 16;
+constexpr int kMfmaWaves = 4;
+
+template <typename T>
+__global__ __launch_bounds__(kWave * kMfmaWaves) void k_gemm_mfma(
+    long long L, const T* __restrict__ A, long long lda, int kdim, const double* __restrict__ B,
+    int ldb, int ncols, T* __restrict__ C, long long ldc) {
+  const int lane = threadIdx.x % kWave, wid = threadIdx.x / kWave;
+  const long long i0 =
+      (static_cast<long long>(blockIdx.x) * kMfmaWaves + wid) * kMfmaRowsPerWave;
+  if (i0 >= L) return;  // whole wave out of range (wave-uniform)
+  const int li = lane % 16, lk = lane / 16;
+  const long long row = i0 + li;
+  const bool row_ok = row < L;
+  const int ntiles = (ncols + 15) / 16;
+  v4d acc[4];
+#pragma unroll
+  for (int t = 0; t < 4; ++t) acc[t] = v4d{0.0, 0.0, 0.0, 0.0};
+  for (int k0 = 0; k0 < kdim; k0 += 4) {
+    const int k = k0 + lk;
+    const double a = (row_ok && k < kdim) ? static_cast<double>(A[static_cast<long long>(k) * lda + row]) : 0.0;
+#pragma unroll
+    for (int t = 0; t < 4; ++t) {
+      if (t < ntiles) {
+        const int j = t * 16 + li;
+        const double b = (k < kdim && j < ncols) ? B[static_cast<long long>(j) * ldb + k] : 0.0;
+        acc[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(b, a, acc[t], 0, 0, 0);
+      }
+    }
+  }
+  if (!row_ok) return;
+#pragma unroll
+  for (int t = 0; t < 4; ++t) {
+    if (t < ntiles) {
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int j = t * 16 + lk + 4 * r;
+        if (j < ncols) C[static_cast<long long>(j) * ldc + row] = static_cast<T>(acc[t][r]);
+      }
+    }
+  }
+}
+
+template <typename T>
+int launch_gemm_mfma(hipStream_t s, long long L, const T* A, long long lda, int kdim,
+                     const double* B, int ldb, int ncols, T* C, long long ldc) {
+  if (L <= 0 || ncols <= 0 || kdim <= 0) return 0;
+  const int rows_per_block = kMfmaRowsPerWave * kMfmaWaves;
+  const int nblk = cdiv(L, rows_per_block);
+  for (int j0 = 0; j0 < ncols; j0 += 64) {
+    const int nc = std::min(64, ncols - j0);
+    hipLaunchKernelGGL(k_gemm_mfma<T>, dim3(nblk), dim3(kWave * kMfmaWaves), 0, s, L, A, lda,
+                       kdim, B + static_cast<long long>(j0) * ldb, ldb, nc,
+                       C + static_cast<long long>(j0) * ldc, ldc);
+    if (hipGetLastError() != hipSuccess) return -1;
+  }
+  return 0;
+}
+
 // ------------------------------------------------------ instantiations ----
 #define PK_INST_T(T)                                                                          \
   template int launch_gemvt<T>(hipStream_t, long long, const T*, long long, const int*, int,  \
-                               const T*, double*, const int*);                                \
+                               const int*, const T*, double*, Gate);                          \
   template int launch_gemv_update<T>(hipStream_t, long long, const T*, long long, const int*, \
-                                     int, const double*, T*, double*, const int*);            \
-  template int launch_scale<T>(hipStream_t, long long, T*, Scal, const int*);                 \
+                                     int, const int*, const double*, T*, double*, Gate);      \
+  template int launch_scale<T>(hipStream_t, long long, T*, Scal, Gate);                       \
   template int launch_axpy_norm_dot<T>(hipStream_t, long long, T*, const T*, Scal, double*,   \
-                                       const int*);                                           \
+                                       Gate);                                                 \
   template int launch_norm<T>(hipStream_t, long long, const T*, double*);                     \
   template int launch_gemm_tall<T>(hipStream_t, long long, const T*, long long, int,         \
+                                   const double*, int, int, T*, long long);                  \
+  template int launch_gemm_mfma<T>(hipStream_t, long long, const T*, long long, int,         \
                                    const double*, int, int, T*, long long);
 PK_INST_T(double)
 PK_INST_T(float)
 #undef PK_INST_T
 
+#define PK_INST_RB(VT, T)                                                                    \
+  template int launch_spmv_rb<VT, T>(hipStream_t, int, const int*, const int*, const int*,   \
+                                     const VT*, const T*, T*, const T*, Scal, double*, int,  \
+                                     Gate);
+PK_INST_RB(double, double)
+PK_INST_RB(float, float)
+PK_INST_RB(double, float)
+PK_INST_RB(float, double)
+#undef PK_INST_RB
+
 template int launch_spmv<double, double>(hipStream_t, int, const int*, const int*,
                                          const double*, const double*, double*, const double*,
-                                         Scal, double*, int, const int*);
+                                         Scal, double*, int, Gate);
 template int launch_spmv<float, float>(hipStream_t, int, const int*, const int*, const float*,
                                        const float*, float*, const float*, Scal, double*, int,
-                                       const int*);
+                                       Gate);
 template int launch_spmv<double, float>(hipStream_t, int, const int*, const int*,
                                         const double*, const float*, float*, const float*,
-                                        Scal, double*, int, const int*);
+                                        Scal, double*, int, Gate);
 template int launch_spmv<float, double>(hipStream_t, int, const int*, const int*,
                                         const float*, const double*, double*, const double*,
-                                        Scal, double*, int, const int*);
+                                        Scal, double*, int, Gate);
 template int launch_convert<double, float>(hipStream_t, long long, const double*, float*);
 template int launch_convert<float, double>(hipStream_t, long long, const float*, double*);
 template int launch_convert<double, double>(hipStream_t, long long, const double*, double*);

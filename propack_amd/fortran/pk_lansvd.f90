@@ -35,10 +35,12 @@ module pk_drivers
 
 contains
 
-  ! Accumulate the engine's counters into COMMON /timing/ (the reference's
-  ! double/stat.h:12-15 layout) so that PRINTSTAT-style callers see them.
-  subroutine add_timing(eng, nbsvd_add, nrestart_add, nlandim_set, nsing_set)
+  ! Accumulate the engine's counters since st0 (taken at driver entry) into
+  ! COMMON /timing/ (the reference's double/stat.h:12-15 layout) so that
+  ! PRINTSTAT-style callers see them.
+  subroutine add_timing(eng, st0, nbsvd_add, nrestart_add, nlandim_set, nsing_set)
     type(c_ptr), intent(in) :: eng
+    integer(c_long_long), intent(in) :: st0(5)
     integer, intent(in) :: nbsvd_add, nrestart_add, nlandim_set, nsing_set
     integer :: nopx, nreorth, ndot, nreorthu, nreorthv, nitref, nrestart, nbsvd, nlandim, nsing
     real :: tmvopx, tgetu0, tupdmu, tupdnu, tintv, tlanbpro, treorth, treorthu, treorthv, &
@@ -50,15 +52,14 @@ contains
     integer(c_int) :: rc
     rc = pk_lbd_stats(eng, st)
     if (rc == PK_OK) then
-      nopx = nopx + int(st(1))
-      nreorth = nreorth + int(st(2))
-      ndot = ndot + int(st(3))
+      nopx = nopx + int(st(1) - st0(1))
+      nreorth = nreorth + int(st(2) - st0(2))
+      ndot = ndot + int(st(3) - st0(3))
     end if
     nbsvd = nbsvd + nbsvd_add
     nrestart = nrestart + nrestart_add
     if (nlandim_set >= 0) nlandim = nlandim_set
     if (nsing_set >= 0) nsing = nsing_set
-    rc = pk_lbd_reset_stats(eng)
   end subroutine
 
   ! Ritz vectors from A V_dim = U_{dim+1} B_dim (DRITZVEC): B = M [R; 0]
@@ -119,8 +120,10 @@ contains
     double precision :: eps34, epsn, tol, anorm, rnorm, dummy(1, 1)
     integer :: lanmax, neig, jold, j, jj, dj, i, ierr, lapinfo, nbsvd
     integer(c_int) :: rc, ierr_c, jj_c
+    integer(c_long_long) :: st0(5)
     logical :: vectors
 
+    rc = pk_lbd_stats(eng, st0)
     eps34 = eps**(3d0 / 4d0)
     epsn = dble(max(m, n)) * eps / 2d0
     lanmax = min(n + 1, m + 1, kmax)
@@ -212,7 +215,7 @@ contains
       call ritzvec('L', jobu, jobv, m, n, neig, jold, alpha, beta, eng, jj)
       if (jj /= 0) info = jj
     end if
-    call add_timing(eng, nbsvd, 0, j, k)
+    call add_timing(eng, st0, nbsvd, 0, j, k)
     k = neig
   end subroutine
 
@@ -234,8 +237,10 @@ contains
     double precision :: eps34, epsn, tol, anorm, rnorm, relgap, dummy(1, 1)
     integer :: k, i, iter, nconv, kold, nshft, ierr, lapinfo, nbsvd, nrestart, jj
     integer(c_int) :: rc, ierr_c, kk_c
+    integer(c_long_long) :: st0(5)
     logical :: small, vectors
 
+    rc = pk_lbd_stats(eng, st0)
     eps34 = eps**(3d0 / 4d0)
     epsn = dble(max(m, n)) * eps / 2d0
     dim = min(dim, n + 1, m + 1)
@@ -381,7 +386,7 @@ contains
       if (jj /= 0) info = jj
     end if
     neig = nconv
-    call add_timing(eng, nbsvd, nrestart, dim, -1)
+    call add_timing(eng, st0, nbsvd, nrestart, dim, -1)
   end subroutine
 
 end module pk_drivers

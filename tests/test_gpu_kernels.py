@@ -1,0 +1,103 @@
+"""GPU kernel checks through the C ABI against float64 host references:
+the basis-update GEMM (FP64 MFMA), the row-block SpMV on edge-case shapes
+(empty rows, rows longer than one LDS block, single row / column), and the
+column put/get round trip.  Tolerances: FP64 results within 1e-13 relative
+to the norm of the host product."""
+import ctypes as C
+
+import numpy as np
+import pytest
+
+from propack_amd import api, synth
+
+pytestmark = pytest.mark.gpu
+
+
+def _p(a):
+    return a.ctypes.data_as(C.c_void_p)
+
+
+def _engine(m, n, kmax, prec=api.PK_F64):
+    # a tiny operator just to own an engine of the right shape
+    A = synth.random_uniform(m, n, 3 * m, seed=5)
+    op = api.CsrOperator(A, prec=prec)
+    return op, api.Engine(op, kmax=kmax)
+
+
+@pytest.mark.parametrize("m,kmax,ncols,transb", [(1000, 37, 20, "n"), (2049, 80, 70, "n"),
+                                                  (777, 30, 12, "t"), (64, 5, 3, "n")])
+def test_basis_update_gemm(m, kmax, ncols, transb):
+    op, eng = _engine(m, 50, kmax)
+    rng = np.random.default_rng(m)
+    U = rng.standard_normal((m, kmax + 1))
+    for j in range(kmax + 1):
+        assert api.lib().pk_lbd_put(eng.handle, C.c_char(b"u"), C.c_int(j + 1),
+                                    _p(np.ascontiguousarray(U[:, j])), C.c_int(0)) == 0
+    kdim = kmax + 1
+    ncols = min(ncols, kdim)
+    B = rng.standard_normal((kdim, ncols))
+    Bh = np.asfortranarray(B if transb == "n" else B.T)
+    ldb = Bh.shape[0]
+    rc = api.lib().pk_lbd_update(eng.handle, C.c_char(b"u"), C.c_char(transb.encode()),
+                                 C.c_int(kdim), C.c_int(ncols), _p(Bh), C.c_int(ldb))
+    assert rc == 0, api.lib().pk_last_error()
+    got = eng.get("u", ncols)
+    want = U @ B
+    assert np.abs(got - want).max() <= 1e-13 * np.abs(want).max() * np.sqrt(kdim)
+
+
+def test_put_get_roundtrip_f32():
+    op, eng = _engine(300, 200, 10, prec=api.PK_F32)
+    x = np.random.default_rng(1).standard_normal(300)
+    eng.put_u1(x)
+    got = eng.get("u", 1, prec=api.PK_F64)[:, 0]
+    assert np.array_equal(got, x.astype(np.float32).astype(np.float64))
+
+
+def _csr_from_dense(D):
+    r, c = np.nonzero(D)
+    return synth.from_coo(D.shape[0], D.shape[1], r, c, D[r, c])
+
+
+def test_spmv_edge_shapes():
+    rng = np.random.default_rng(9)
+    # empty rows, a row of 9000 nonzeros (longer than one LDS row block),
+    # a column hit 5000 times (long row of A^T)
+    m, n = 300, 9500
+    D = np.zeros((m, n))
+    for i in range(0, m, 3):
+        cols = rng.choice(n, rng.integers(1, 20), replace=False)
+        D[i, cols] = rng.standard_normal(cols.size)
+    D[7, :9000] = rng.standard_normal(9000)
+    D[:, 42] = rng.standard_normal(m)
+    A = _csr_from_dense(D)
+    op = api.CsrOperator(A)
+    x = rng.standard_normal(n)
+    y = rng.standard_normal(m)
+    np.testing.assert_allclose(op.spmv(x, "n"), D @ x, rtol=0, atol=1e-11)
+    np.testing.assert_allclose(op.spmv(y, "t"), D.T @ y, rtol=0, atol=1e-11)
+    AT = A.transpose()
+    opt = api.CsrOperator(AT)  # 9500 x 300 with a 300-long row and a 9000-long column
+    np.testing.assert_allclose(opt.spmv(y, "n"), D.T @ y, rtol=0, atol=1e-11)
+
+
+@pytest.mark.parametrize("m,n", [(1, 1), (1, 50), (50, 1), (5000, 3)])
+def test_spmv_tiny(m, n):
+    rng = np.random.default_rng(m * 31 + n)
+    D = rng.standard_normal((m, n)) * (rng.random((m, n)) < 0.5)
+    if not D.any():
+        D[0, 0] = 1.0
+    A = _csr_from_dense(D)
+    op = api.CsrOperator(A)
+    x, y = rng.standard_normal(n), rng.standard_normal(m)
+    np.testing.assert_allclose(op.spmv(x, "n"), D @ x, rtol=0, atol=1e-12)
+    np.testing.assert_allclose(op.spmv(y, "t"), D.T @ y, rtol=0, atol=1e-12)
+
+
+def test_spmv_deterministic():
+    A = synth.poisson_decay(200_000, 100_000, 16.0, 0.6, seed=4)
+    op = api.CsrOperator(A)
+    x = synth.start_vector(A.n, 2)
+    y1, y2 = op.spmv(x, "n"), op.spmv(x, "n")
+    assert np.array_equal(y1, y2)
+    np.testing.assert_allclose(y1, A.matvec(x), rtol=0, atol=1e-12 * np.abs(y1).max())

@@ -112,3 +112,57 @@ def test_device_resident_irl(small_case):
     assert r.info == 0 and r.k == 10
     np.testing.assert_allclose(r.sigma, s[:10], rtol=1e-12)
     _check_triplets(r, D, 1e-11, s[0])
+
+
+@pytest.mark.parametrize("tries", [1, 2])
+def test_device_chain_matches_host_executor(small_case, tries):
+    """The device-resident steps (control kernels on the GPU) and the host
+    executor run the same control code on the same deterministic
+    reductions: identical singular values, bounds and counters."""
+    A, u0, D, s = small_case
+    op = api.CsrOperator(A)
+    res = {}
+    for mode in ("host", "device"):
+        eng = api.Engine(op, kmax=300)
+        eng.set_mode(mode, tries)
+        eng.put_u1(u0)
+        r = api.lansvd_dev(eng, 10, 300, tol=1e-12)
+        res[mode] = (r, eng.stats(), eng.get("u", r.k))
+    (rh, sh, Uh), (rd, sd, Ud) = res["host"], res["device"]
+    assert rh.info == rd.info == 0 and rh.k == rd.k == 10
+    assert np.array_equal(rh.sigma, rd.sigma)
+    assert np.array_equal(rh.bnd, rd.bnd)
+    assert np.array_equal(Uh, Ud)
+    for key in ("nopx", "nreorth", "ndot"):
+        assert sh[key] == sd[key], key
+    assert sd["nsync"] < sh["nsync"] / 5
+
+
+def test_device_chain_irl_large(small_case):
+    """Restarted runs on a 200k x 100k matrix: device-resident and host
+    executor agree bit for bit, and the triplets satisfy the residual."""
+    A = synth.poisson_decay(200_000, 100_000, 16.0, 0.6, seed=7)
+    u0 = synth.start_vector(A.m, 8)
+    op = api.CsrOperator(A)
+    res = {}
+    for mode in ("host", "device"):
+        eng = api.Engine(op, kmax=40)
+        eng.set_mode(mode, 2)
+        eng.put_u1(u0)
+        r = api.lansvd_irl_dev(eng, 20, 40, tol=1e-10)
+        res[mode] = (r, eng.stats())
+    (rh, sh), (rd, sd) = res["host"], res["device"]
+    print("host", rh.counters, sh)
+    print("device", rd.counters, sd)
+    assert rh.info == rd.info == 0 and rh.k == rd.k == 20
+    assert rd.counters["nrestart"] >= 1
+    assert np.array_equal(rh.sigma, rd.sigma)
+    for key in ("nopx", "nreorth", "ndot"):
+        assert sh[key] == sd[key], key
+    # residuals of the device run
+    eng_r = rd
+    worst = 0.0
+    for i in range(eng_r.k):
+        av = op.spmv(np.ascontiguousarray(eng_r.V[:, i]), "n")
+        worst = max(worst, np.linalg.norm(av - eng_r.sigma[i] * eng_r.U[:, i]))
+    assert worst <= 1e-9 * eng_r.sigma[0]
