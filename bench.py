@@ -100,6 +100,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-check", action="store_true")
+    ap.add_argument("--no-mixed", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported "
@@ -160,6 +161,38 @@ def main():
         resid = residuals(op, rr)
         log(f"[bench] max ||A v - s u|| / s_1 = {resid:.3e}")
 
+    # FP32-basis form with FP64 polish on the same matrix and start vector
+    mixed = None
+    if not args.no_mixed:
+        op32 = api.CsrOperator(A.astype(np.float32), prec=api.PK_F32)
+        eng32 = api.Engine(op32, kmax=DIM)
+        u32 = u0.astype(np.float32).astype(np.float64)
+
+        def solve_mixed():
+            eng32.put_u1(u32)
+            return api.lansvd_irl_mixed(eng, eng32, NEIG, DIM, tol=TOL, fetch=False)
+
+        rm, f32 = solve_mixed()  # warm-up
+        tm = []
+        for _ in range(args.steps):
+            t0 = time.perf_counter()
+            rm, f32 = solve_mixed()
+            api.lib().pk_lbd_sync(eng.handle)
+            tm.append(time.perf_counter() - t0)
+        k32 = min(f32["k"], r.k)
+        err32 = np.abs(f32["sigma"][:k32] - r.sigma[:k32]) / r.sigma[:k32]
+        errp = np.abs(rm.sigma[:r.k] - r.sigma[:r.k]) / r.sigma[:r.k] if rm.k >= r.k else None
+        mixed = {"s": float(np.mean(tm)), "speedup_vs_fp64": float(ms_per_step / 1000.0 / np.mean(tm)),
+                 "fp32_phase_converged": f32["k"], "fp32_phase_info": f32["info"],
+                 "fp32_phase_max_rel_sigma_err_vs_fp64": float(err32.max()) if k32 else None,
+                 "fp32_phase_rel_sigma_err_per_triplet": [float(x) for x in err32],
+                 "polished_converged": rm.k,
+                 "polished_max_rel_sigma_err_vs_fp64": float(errp.max()) if errp is not None else None,
+                 "nopx_total": rm.counters["nopx"]}
+        log(f"[bench] FP32-basis + FP64 polish: {mixed['s']:.3f}s "
+            f"(x{mixed['speedup_vs_fp64']:.2f} vs FP64), fp32 phase k={f32['k']} "
+            f"max sigma err {mixed['fp32_phase_max_rel_sigma_err_vs_fp64']}")
+
     cpu = None
     if not args.no_cpu:
         log("[bench] cpu baseline (reference PROPACK, bounded sample) ...")
@@ -191,6 +224,7 @@ def main():
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
                      "avg_launch_ms": per_launch_ms, "bytes_per_launch": per_launch_bytes},
         "kernels": kernels,
+        "fp32_basis_form": mixed,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out))

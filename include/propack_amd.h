@@ -131,6 +131,10 @@ int pk_lbd_shape(pk_lbd_t eng, int* m, int* n, int* kmax, int* prec);
  * host-callback operator always uses mode 1.  Environment PK_LBD_MODE=host
  * selects mode 1 at creation. */
 int pk_lbd_set_mode(pk_lbd_t eng, int mode, int tries);
+/* U(:,1) of the double engine dst <- sum of the first ncols columns of
+ * src's U (any precision): the start vector of the FP64 polish that follows
+ * an FP32-basis solve (no reference counterpart). */
+int pk_lbd_seed(pk_lbd_t dst, pk_lbd_t src, int ncols);
 /* Live kernel timing (no reference counterpart): when enabled, every hot
  * kernel launch is bracketed by HIP events on the engine's stream and its
  * algorithmic bytes/flops are accumulated per class.  out holds 6 classes x
@@ -155,6 +159,16 @@ int pk_lansvd_run(pk_lbd_t eng, char jobu, char jobv, int* k, int kmax, double* 
 int pk_lansvd_irl_run(pk_lbd_t eng, char which, char jobu, char jobv, int* dim, int p, int* neig,
                       int maxiter, double* sigma, double* bnd, double tolin, double* doption,
                       int* ioption);
+/* FP32-basis form with FP64 polish: the restarted solve runs first on
+ * eng32 (FP32 operator values and bases, FP64 reductions, omega recurrence,
+ * B and its SVD) to tolerance max(tolin, 16*eps32); the sum of its left Ritz
+ * vectors seeds the FP64 restarted solve on eng64, which meets tolin.
+ * sigma32/bnd32 receive the FP32-phase values, info32 = {INFO, converged}
+ * of that phase; the polished vectors are left in eng64's bases. */
+int pk_lansvd_irl_mixed(pk_lbd_t eng64, pk_lbd_t eng32, char which, char jobu, char jobv,
+                        int* dim, int p, int* neig, int maxiter, double* sigma, double* bnd,
+                        double* sigma32, double* bnd32, double tolin, double* doption,
+                        int* ioption, int* info32);
 
 /* ---------------------------------------------------------------------- */
 /* Host scalar routines, exported so their results can be compared with the

@@ -33,7 +33,7 @@ EXPORTS = [
     "pk_lbd_destroy", "pk_lbd_set_callback", "pk_lbd_put", "pk_lbd_get", "pk_lbd_basis",
     "pk_lbd_nrm2", "pk_lbd_getu0", "pk_lbd_lanbpro", "pk_lbd_update", "pk_lbd_stats",
     "pk_lbd_reset_stats", "pk_lbd_sync", "pk_lbd_shape", "pk_lansvd_run",
-    "pk_lansvd_irl_run", "pk_lbd_set_mode", "pk_lbd_profile", "pk_lbd_profile_get", "pk_lbd_profile_reset",
+    "pk_lansvd_irl_run", "pk_lansvd_irl_mixed", "pk_lbd_seed", "pk_lbd_set_mode", "pk_lbd_profile", "pk_lbd_profile_get", "pk_lbd_profile_reset",
     "pk_larnv_pm1", "pkh_update_mu", "pkh_update_nu", "pkh_compute_int", "pkh_set_mu",
     "pkh_lartg", "pkh_bdqr", "pkh_bsvdstep", "pkh_refinebounds", "dlansvd_", "dlansvd_irl_", "slansvd_", "slansvd_irl_",
     "clearstat_", "printstat_",
@@ -360,3 +360,37 @@ def lansvd_dev(eng: Engine, k, kmax, tol=1e-12, jobu="y", jobv="y", delta=None, 
     U = eng.get("u", k) if (fetch and jobu == "y") else None
     V = eng.get("v", k) if (fetch and jobv == "y") else None
     return SvdResult(sigma[:k].copy(), bnd[:k].copy(), U, V, info, k, timing(), anorm=float(dopt[2]))
+
+
+def lansvd_irl_mixed(eng64: Engine, eng32: Engine, neig, dim, p=None, tol=1e-10, which="L",
+                     jobu="y", jobv="y", maxiter=1000, cgs=1, elr=1, min_relgap=0.002,
+                     fetch=True):
+    """FP32-basis solve on ``eng32`` then the seeded FP64 polish on ``eng64``
+    (both engines start from their own U(:,1); eng32's is the start vector).
+    Returns (polished SvdResult, dict with the FP32-phase sigma/bnd/info)."""
+    if p is None:
+        p = dim - neig
+    sigma = np.zeros(dim + 1)
+    bnd = np.zeros(dim + 1)
+    s32 = np.zeros(dim + 1)
+    b32 = np.zeros(dim + 1)
+    eps = 2.0 ** -52
+    dopt = np.array([np.sqrt(eps), eps ** 0.75, 0.0, min_relgap])
+    iopt = np.array([cgs, elr], np.int32)
+    info32 = np.zeros(2, np.int32)
+    dd, ne = C.c_int(dim), C.c_int(neig)
+    clearstat()
+    info = lib().pk_lansvd_irl_mixed(eng64.handle, eng32.handle, C.c_char(which.encode()),
+                                     C.c_char(jobu.encode()), C.c_char(jobv.encode()),
+                                     C.byref(dd), C.c_int(p), C.byref(ne), C.c_int(maxiter),
+                                     _p(sigma), _p(bnd), _p(s32), _p(b32), C.c_double(tol),
+                                     _p(dopt), _p(iopt), _p(info32))
+    if info <= -100:
+        _check(info, "pk_lansvd_irl_mixed")
+    k = ne.value
+    k32 = int(info32[1])
+    U = eng64.get("u", k) if (fetch and jobu == "y") else None
+    V = eng64.get("v", k) if (fetch and jobv == "y") else None
+    res = SvdResult(sigma[:k].copy(), bnd[:k].copy(), U, V, info, k, timing(), dim=dd.value)
+    return res, {"sigma": s32[:k32].copy(), "bnd": b32[:k32].copy(), "info": int(info32[0]),
+                 "k": k32}

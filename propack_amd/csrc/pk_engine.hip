@@ -764,6 +764,12 @@ struct Impl {
       }
       PK_TRY(host_phase(c));
     }
+    // The 1/beta scaling of u_{k+1} is deferred to the start of the next
+    // step (see the chains); the last step of the call applies it here.
+    if (c.uscale) {
+      PK_TRY(scale(ucol(c.j), m, c.uscale_f));
+      c.uscale = 0;
+    }
 
     for (int i = 1; i <= c.k; ++i) {
       alpha_[i - 1] = B1[i];
@@ -1096,6 +1102,26 @@ int pk_lbd_stats(pk_lbd_t h, long long* out) {
 int pk_lbd_reset_stats(pk_lbd_t h) {
   PK_ENGINE(h);
   e->cnt = Counters();
+  return PK_OK;
+}
+
+int pk_lbd_seed(pk_lbd_t dst_h, pk_lbd_t src_h, int ncols) {
+  Engine* d = as_engine(dst_h);
+  Engine* sr = as_engine(src_h);
+  if (!d || !sr) return PK_ERR_HANDLE;
+  if (d->prec != PK_F64 || d->m != sr->m || ncols < 1 || ncols > sr->kmax + 1) {
+    set_error("pk_lbd_seed: needs a double engine of the same shape and 1 <= ncols <= kmax+1");
+    return PK_ERR_ARG;
+  }
+  PK_HIP(hipStreamSynchronize(sr->stream));
+  int r = sr->prec == PK_F32
+              ? launch_colsum<float, double>(d->stream, d->m, static_cast<const float*>(sr->u()),
+                                             sr->ldu, ncols, static_cast<double*>(d->u()))
+              : launch_colsum<double, double>(d->stream, d->m,
+                                              static_cast<const double*>(sr->u()), sr->ldu,
+                                              ncols, static_cast<double*>(d->u()));
+  if (r < 0) return PK_ERR_HIP;
+  PK_HIP(hipStreamSynchronize(d->stream));
   return PK_OK;
 }
 

@@ -77,6 +77,10 @@ enum CtlKind { CTL_V_SPMV = 0, CTL_V_ELR, CTL_V_REO, CTL_U_SPMV, CTL_U_ELR, CTL_
 int launch_ctl(hipStream_t s, LbdCtl* c, int kind, int want_phase, int want_j,
                const double* part, int nb);
 
+// dst = sum of the first ncols columns of src (double accumulation).
+template <typename S, typename D>
+int launch_colsum(hipStream_t s, long long L, const S* src, long long ld, int ncols, D* dst);
+
 int launch_fill_zero_bytes(hipStream_t s, void* p, size_t bytes);
 
 }  // namespace pk

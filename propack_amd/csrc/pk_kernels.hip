@@ -633,6 +633,35 @@ int launch_gemm_mfma(hipStream_t s, long long L, const T* A, long long lda, int 
   return 0;
 }
 
+// ------------------------------------------------------------ seeding ----
+// dst = sum_{c < ncols} src(:, c), accumulated in double (start vector of an
+// FP64 polish from the Ritz vectors of an FP32-basis solve).
+template <typename S, typename D>
+__global__ __launch_bounds__(kBlock) void k_colsum(long long L, const S* __restrict__ src,
+                                                   long long ld, int ncols, D* __restrict__ dst) {
+  const long long base = static_cast<long long>(blockIdx.x) * kRowsPerBlock + threadIdx.x;
+#pragma unroll
+  for (int r = 0; r < kRPT; ++r) {
+    const long long row = base + static_cast<long long>(r) * kBlock;
+    if (row < L) {
+      double acc = 0.0;
+      for (int c = 0; c < ncols; ++c) acc += static_cast<double>(src[c * ld + row]);
+      dst[row] = static_cast<D>(acc);
+    }
+  }
+}
+
+template <typename S, typename D>
+int launch_colsum(hipStream_t s, long long L, const S* src, long long ld, int ncols, D* dst) {
+  hipLaunchKernelGGL((k_colsum<S, D>), dim3(stream_blocks(L)), dim3(kBlock), 0, s, L, src, ld,
+                     ncols, dst);
+  return hipGetLastError() == hipSuccess ? 0 : -1;
+}
+template int launch_colsum<float, double>(hipStream_t, long long, const float*, long long, int,
+                                          double*);
+template int launch_colsum<double, double>(hipStream_t, long long, const double*, long long, int,
+                                           double*);
+
 // ------------------------------------------------------ instantiations ----
 #define PK_INST_T(T)                                                                          \
   template int launch_gemvt<T>(hipStream_t, long long, const T*, long long, const int*, int,  \

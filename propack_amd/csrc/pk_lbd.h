@@ -10,7 +10,17 @@
 
 #include <cmath>
 
+#ifdef __HIPCC__
 #include "pk_common.h"
+#else
+// Host-only compilation (the control logic is plain C++).
+#define __host__
+#define __device__
+#define __forceinline__ inline
+namespace pk {
+constexpr int kMaxCols = 512;
+}
+#endif
 
 namespace pk {
 

@@ -478,3 +478,74 @@ subroutine engine_shape(eng, m, n, eps, info)
     eps = 2d0**(-24)
   end if
 end subroutine
+
+! FP32-basis form with FP64 polish (no reference counterpart).  eng32 holds
+! FP32 values and bases (all reductions and scalars FP64); it runs the
+! restarted driver to the FP32-attainable tolerance max(tolin, 16*eps32).
+! Its left Ritz vectors seed an FP64 restarted solve on eng64 (same shape),
+! which delivers the triplets to tolin.  sigma32/bnd32 return the FP32-phase
+! values (for the per-triplet error report), sigma/bnd the polished ones.
+! info32 returns the FP32 phase's INFO and converged count in info32(1:2).
+integer(c_int) function pk_lansvd_irl_mixed(eng64, eng32, which, jobu, jobv, dim, p, neig, &
+                                            maxiter, sigma, bnd, sigma32, bnd32, tolin, &
+                                            doption, ioption, info32) &
+    bind(C, name='pk_lansvd_irl_mixed')
+  use iso_c_binding
+  use pk_api
+  use pk_drivers
+  implicit none
+  type(c_ptr), value :: eng64, eng32
+  character(kind=c_char), value :: which, jobu, jobv
+  integer(c_int) :: dim, neig, info32(2)
+  integer(c_int), value :: p, maxiter
+  real(c_double) :: sigma(*), bnd(*), sigma32(*), bnd32(*), doption(*)
+  real(c_double), value :: tolin
+  integer(c_int) :: ioption(*)
+  interface
+    integer(c_int) function pk_lbd_seed(dst, src, ncols) bind(C, name='pk_lbd_seed')
+      import :: c_int, c_ptr
+      type(c_ptr), value :: dst, src
+      integer(c_int), value :: ncols
+    end function
+  end interface
+  integer :: m, n, info, dd, ne, ne32, m32, n32
+  double precision :: eps, eps32, dopt32(4)
+  integer(c_int) :: rc
+  call engine_shape(eng64, m, n, eps, info)
+  if (info /= 0) then
+    pk_lansvd_irl_mixed = info
+    return
+  end if
+  call engine_shape(eng32, m32, n32, eps32, info)
+  if (info /= 0 .or. m32 /= m .or. n32 /= n) then
+    pk_lansvd_irl_mixed = -101
+    return
+  end if
+  ! FP32-basis phase, left vectors kept on the device for the seed
+  dd = dim
+  ne32 = neig
+  dopt32 = doption(1:4)
+  call lansvd_irl_core(which, 'y', 'n', m, n, dd, p, ne32, maxiter, eng32, eps32, sigma32, &
+                       bnd32, max(tolin, 16d0 * eps32), dopt32, ioption, info)
+  info32(1) = info
+  info32(2) = ne32
+  if (info <= -100) then
+    pk_lansvd_irl_mixed = info
+    return
+  end if
+  ! FP64 polish seeded with the sum of the FP32 left Ritz vectors
+  if (ne32 > 0) then
+    rc = pk_lbd_seed(eng64, eng32, ne32)
+    if (rc /= PK_OK) then
+      pk_lansvd_irl_mixed = rc
+      return
+    end if
+  end if
+  dd = dim
+  ne = neig
+  call lansvd_irl_core(which, jobu, jobv, m, n, dd, p, ne, maxiter, eng64, eps, sigma, bnd, &
+                       tolin, doption, ioption, info)
+  dim = dd
+  neig = ne
+  pk_lansvd_irl_mixed = info
+end function

@@ -166,3 +166,22 @@ def test_device_chain_irl_large(small_case):
         av = op.spmv(np.ascontiguousarray(eng_r.V[:, i]), "n")
         worst = max(worst, np.linalg.norm(av - eng_r.sigma[i] * eng_r.U[:, i]))
     assert worst <= 1e-9 * eng_r.sigma[0]
+
+
+def test_fp32_basis_with_fp64_polish(small_case):
+    """FP32 operator values and bases (FP64 reductions and scalars), then the
+    seeded FP64 polish: polished sigma to 1e-10 of the FP64 oracle and the
+    residual bound; the FP32 phase alone is accurate to ~1e-6."""
+    A, u0, D, s = small_case
+    op64 = api.CsrOperator(A)
+    op32 = api.CsrOperator(A, prec=api.PK_F32)
+    e64 = api.Engine(op64, kmax=30)
+    e32 = api.Engine(op32, kmax=30)
+    e32.put_u1(u0)
+    r, f32 = api.lansvd_irl_mixed(e64, e32, 10, 30, tol=1e-12)
+    ref = refpy.lansvd_irl(A, 10, 30, u0=u0, tol=1e-12, cgs=1)
+    assert f32["info"] == 0 and f32["k"] == 10
+    np.testing.assert_allclose(f32["sigma"], s[:10], rtol=1e-5)
+    assert r.info == 0 and r.k == 10
+    np.testing.assert_allclose(r.sigma, ref.sigma, rtol=1e-10)
+    _check_triplets(r, D, 1e-11, s[0])
