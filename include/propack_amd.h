@@ -113,8 +113,8 @@ int pk_lbd_lanbpro(pk_lbd_t eng, int k0, int* k, double* alpha, double* beta, do
 int pk_lbd_update(pk_lbd_t eng, char which, char transb, int kdim, int ncols, const double* B,
                   int ldb);
 
-/* Counters with the meaning of COMMON /timing/ (double/stat.h:12-15):
- * out[0] nopx, [1] nreorth, [2] ndot (reference semantics, see
+/* Counters with the meaning of COMMON /timing/ (double/stat.h:12-15);
+ * out must hold 6 values: out[0] nopx, [1] nreorth, [2] ndot (reference semantics, see
  * double/dreorth.F:92,167), [3] basis columns actually streamed by
  * reorthogonalization, [4] host synchronizations, [5] CGS passes beyond the
  * first of a reorthogonalization. */

@@ -68,26 +68,57 @@ int choose_lpr(long long nnz, int rows) {
   return l;
 }
 
-// Greedy partition of rows into blocks of at most kSpmvRows rows and
-// kSpmvCap nonzeros; a row longer than the cap gets a block of its own.
-std::vector<int> row_blocks(const std::vector<int>& rp, int rows) {
-  std::vector<int> rb;
-  rb.reserve(static_cast<size_t>(rows) / 64 + 2);
-  rb.push_back(0);
+// Workgroup descriptors of the row-block SpMV: greedy blocks of at most
+// kSpmvRows rows and kSpmvCap nonzeros; a row longer than kSpmvCap becomes
+// ceil(len / kLongChunk) chunk descriptors plus an entry of the long-row
+// table {row, first chunk, chunks, 0}.
+struct Partition {
+  std::vector<int4> desc, ldesc;
+  int nchunks = 0;
+};
+
+Partition partition(const std::vector<int>& rp, int rows) {
+  Partition P;
+  P.desc.reserve(static_cast<size_t>(rows) / 64 + 2);
   int r = 0;
   while (r < rows) {
+    const long long len0 = static_cast<long long>(rp[r + 1]) - rp[r];
+    if (len0 > kSpmvCap) {
+      const int nch = static_cast<int>((len0 + kLongChunk - 1) / kLongChunk);
+      for (int c = 0; c < nch; ++c) {
+        const int p0 = rp[r] + c * kLongChunk;
+        const int p1 = static_cast<int>(std::min<long long>(p0 + static_cast<long long>(kLongChunk), rp[r + 1]));
+        P.desc.push_back(make_int4(r, -(P.nchunks + c + 1), p0, p1));
+      }
+      P.ldesc.push_back(make_int4(r, P.nchunks, nch, 0));
+      P.nchunks += nch;
+      ++r;
+      continue;
+    }
     const int start = r;
     long long nz = 0;
     while (r < rows && r - start < kSpmvRows) {
       const long long len = static_cast<long long>(rp[r + 1]) - rp[r];
-      if (nz + len > kSpmvCap) break;
+      if (len > kSpmvCap || nz + len > kSpmvCap) break;
       nz += len;
       ++r;
     }
-    if (r == start) ++r;
-    rb.push_back(r);
+    P.desc.push_back(make_int4(start, r, -1, 0));
   }
-  return rb;
+  return P;
+}
+
+bool upload(const Partition& P, CsrOp::Part& d) {
+  d.ndesc = static_cast<int>(P.desc.size());
+  d.nlong = static_cast<int>(P.ldesc.size());
+  const size_t nd = std::max<size_t>(P.desc.size(), 1), nl = std::max<size_t>(P.ldesc.size(), 1);
+  return hipMalloc(&d.desc, sizeof(int4) * nd) == hipSuccess &&
+         hipMalloc(&d.ldesc, sizeof(int4) * nl) == hipSuccess &&
+         hipMalloc(&d.lsum, sizeof(double) * std::max(P.nchunks, 1)) == hipSuccess &&
+         (P.desc.empty() || hipMemcpy(d.desc, P.desc.data(), sizeof(int4) * P.desc.size(),
+                                      hipMemcpyHostToDevice) == hipSuccess) &&
+         (P.ldesc.empty() || hipMemcpy(d.ldesc, P.ldesc.data(), sizeof(int4) * P.ldesc.size(),
+                                       hipMemcpyHostToDevice) == hipSuccess);
 }
 
 }  // namespace
@@ -195,15 +226,9 @@ int csr_create(int m, int n, long long nnz, const int* rowptr, const int* colind
                   hipSuccess &&
               hipMemcpy(hrpt.data(), op->rpt, sizeof(int) * (n + 1), hipMemcpyDeviceToHost) ==
                   hipSuccess;
-    std::vector<int> rb = row_blocks(hrp, m), rbt = row_blocks(hrpt, n);
-    op->nrb = static_cast<int>(rb.size()) - 1;
-    op->nrbt = static_cast<int>(rbt.size()) - 1;
-    ok = ok && hipMalloc(&op->rb, sizeof(int) * rb.size()) == hipSuccess &&
-         hipMalloc(&op->rbt, sizeof(int) * rbt.size()) == hipSuccess &&
-         hipMemcpy(op->rb, rb.data(), sizeof(int) * rb.size(), hipMemcpyHostToDevice) ==
-             hipSuccess &&
-         hipMemcpy(op->rbt, rbt.data(), sizeof(int) * rbt.size(), hipMemcpyHostToDevice) ==
-             hipSuccess;
+    ok = ok && upload(partition(hrp, m), op->pa) && upload(partition(hrpt, n), op->pt);
+    op->nrb = op->pa.slots();
+    op->nrbt = op->pt.slots();
     if (!ok) {
       csr_destroy(op);
       return PK_ERR_HIP;
@@ -221,8 +246,11 @@ void csr_destroy(CsrOp* op) {
   (void)hipFree(op->rpt);
   (void)hipFree(op->cit);
   (void)hipFree(op->vat);
-  (void)hipFree(op->rb);
-  (void)hipFree(op->rbt);
+  for (CsrOp::Part* d : {&op->pa, &op->pt}) {
+    (void)hipFree(d->desc);
+    (void)hipFree(d->ldesc);
+    (void)hipFree(d->lsum);
+  }
   op->magic = 0;
   delete op;
 }
@@ -238,14 +266,15 @@ int csr_apply(const CsrOp* op, hipStream_t s, char trans, const T* x, T* y, cons
   const int* ci = tr ? op->cit : op->ci;
   const void* va = tr ? op->vat : op->va;
   const int lpr = tr ? op->lpr_t : op->lpr_n;
-  const int* rb = tr ? op->rbt : op->rb;
-  const int nrb = tr ? op->nrbt : op->nrb;
+  const CsrOp::Part& P = tr ? op->pt : op->pa;
   (void)rows;
   if (op->val_prec == PK_F64)
-    return launch_spmv_rb<double, T>(s, nrb, rb, rp, ci, static_cast<const double*>(va), x, y,
-                                     yprev, coef, part, lpr, gate);
-  return launch_spmv_rb<float, T>(s, nrb, rb, rp, ci, static_cast<const float*>(va), x, y, yprev,
-                                  coef, part, lpr, gate);
+    return launch_spmv_rb<double, T>(s, P.ndesc, P.desc, P.nlong, P.ldesc, P.lsum, rp, ci,
+                                     static_cast<const double*>(va), x, y, yprev, coef, part, lpr,
+                                     gate);
+  return launch_spmv_rb<float, T>(s, P.ndesc, P.desc, P.nlong, P.ldesc, P.lsum, rp, ci,
+                                  static_cast<const float*>(va), x, y, yprev, coef, part, lpr,
+                                  gate);
 }
 
 template int csr_apply<double>(const CsrOp*, hipStream_t, char, const double*, double*,

@@ -21,8 +21,17 @@ struct CsrOp {
   int *rpt = nullptr, *cit = nullptr; // A^T (explicit)
   void* vat = nullptr;
   int lpr_n = 8, lpr_t = 8;           // lanes per row for A and A^T
-  int *rb = nullptr, *rbt = nullptr;  // row-block partitions of A and A^T
-  int nrb = 0, nrbt = 0;              // number of row blocks
+  // SpMV workgroup descriptors of A and A^T (row blocks and long-row
+  // chunks), long-row tables and their chunk partial sums
+  struct Part {
+    int4* desc = nullptr;
+    int ndesc = 0;
+    int4* ldesc = nullptr;
+    int nlong = 0;
+    double* lsum = nullptr;
+    int slots() const { return ndesc + nlong; }
+  } pa, pt;
+  int nrb = 0, nrbt = 0;              // partial slots per direction
 };
 
 int csr_create(int m, int n, long long nnz, const int* rowptr, const int* colind,

@@ -19,16 +19,18 @@ int launch_spmv(hipStream_t s, int rows, const int* rp, const int* ci, const VT*
                 const T* x, T* y, const T* yprev, Scal coef, double* part, int lpr,
                 Gate gate);
 
-// Row-block SpMV: rb (nblk+1) partitions the rows into blocks of at most
-// kSpmvRows rows and kSpmvCap nonzeros (or a single longer row); g lanes
-// per row in the LDS reduction.  Same outputs and partials as launch_spmv,
-// one partial pair per row block.
+// Row-block SpMV over nblk workgroup descriptors (row blocks of at most
+// kSpmvRows rows / kSpmvCap nonzeros, or chunks of kLongChunk nonzeros of a
+// long row) plus nlong long rows finished by a fix-up kernel; g lanes per row
+// in the LDS reduction.  y = op(A) x - coef * yprev with fused partials
+// (2 per slot, nblk + nlong slots).  Returns the slot count.
 constexpr int kSpmvCap = 4096;
 constexpr int kSpmvRows = 256;
+constexpr int kLongChunk = 8192;
 template <typename VT, typename T>
-int launch_spmv_rb(hipStream_t s, int nblk, const int* rb, const int* rp, const int* ci,
-                   const VT* va, const T* x, T* y, const T* yprev, Scal coef, double* part,
-                   int g, Gate gate);
+int launch_spmv_rb(hipStream_t s, int nblk, const int4* desc, int nlong, const int4* ldesc,
+                   double* lsum, const int* rp, const int* ci, const VT* va, const T* x, T* y,
+                   const T* yprev, Scal coef, double* part, int g, Gate gate);
 
 // part (nval x nblk) -> out[v] = sum_b part[v*nblk+b] in index order.
 int launch_reduce(hipStream_t s, int nval, int nblk, const double* part, double* out,

@@ -93,117 +93,148 @@ int launch_spmv(hipStream_t s, int rows, const int* rp, const int* ci, const VT*
 }
 
 // ------------------------------------------------------ row-block SpMV ----
-// Workgroup b owns the rows [rb[b], rb[b+1]) whose nonzeros (at most
-// kSpmvCap, at most kSpmvRows rows) are first turned into products
-// val * x[col] in LDS by all 256 threads -- index and value streams read
-// fully coalesced, kUnroll independent gathers in flight per thread -- and
-// then summed row by row in LDS by groups of G lanes (conflict-free
-// consecutive reads, shuffle reduction), in a fixed order.  A block that
-// holds a single row longer than kSpmvCap sums it with the whole workgroup.
+// The rows are partitioned into workgroup descriptors {x, y, z, w}:
+//   * y >= 0: a row block, rows [x, y) holding at most kSpmvCap nonzeros
+//     and kSpmvRows rows.  All 256 threads turn its nonzeros into products
+//     val * x[col] in LDS -- index and value streams read fully coalesced,
+//     kUnroll independent gathers in flight per thread -- then groups of G
+//     lanes sum each row from LDS (consecutive, conflict-free reads;
+//     shuffle reduction) in a fixed order and apply the fused epilogue.
+//   * y < 0: chunk -(y+1) of the long row x, nonzeros [z, w): its partial
+//     dot product goes to lsum[-(y+1)]; k_spmv_longfix finishes the row.
+// Partials: part[b] = sum y^2, part[nbt + b] = sum yprev.y (zero for
+// chunks); the long rows' contributions follow the blocks' slots.
 template <typename VT, typename T, int G>
-__global__ __launch_bounds__(kBlock) void k_spmv_rb(const int* __restrict__ rb,
+__global__ __launch_bounds__(kBlock) void k_spmv_rb(const int4* __restrict__ desc,
                                                     const int* __restrict__ rp,
                                                     const int* __restrict__ ci,
                                                     const VT* __restrict__ va,
                                                     const T* __restrict__ x, T* __restrict__ y,
                                                     const T* __restrict__ yprev, Scal coef,
-                                                    double* __restrict__ part, Gate gate) {
+                                                    double* __restrict__ part, int nbt,
+                                                    double* __restrict__ lsum, Gate gate) {
   if (gate.off()) return;
   __shared__ double prod[kSpmvCap];
   __shared__ double lds[(kBlock / kWave) * 2];
-  const double c = coef.get();
-  const int r0 = rb[blockIdx.x], r1 = rb[blockIdx.x + 1];
-  const int p0 = rp[r0];
-  const int nz = rp[r1] - p0;
-  double s2 = 0.0, sd = 0.0;
-  if (nz <= kSpmvCap) {
-    constexpr int kUnroll = 8;
-    for (int base = 0; base < nz; base += kBlock * kUnroll) {
-      int idx[kUnroll];
-      double v[kUnroll], xv[kUnroll];
-#pragma unroll
-      for (int u = 0; u < kUnroll; ++u) {
-        const int t = base + u * kBlock + static_cast<int>(threadIdx.x);
-        idx[u] = t < nz ? ci[p0 + t] : -1;
-        v[u] = t < nz ? static_cast<double>(va[p0 + t]) : 0.0;
-      }
-#pragma unroll
-      for (int u = 0; u < kUnroll; ++u) xv[u] = idx[u] >= 0 ? static_cast<double>(x[idx[u]]) : 0.0;
-#pragma unroll
-      for (int u = 0; u < kUnroll; ++u) {
-        const int t = base + u * kBlock + static_cast<int>(threadIdx.x);
-        if (t < nz) prod[t] = v[u] * xv[u];
-      }
-    }
-    __syncthreads();
-    const int lane = threadIdx.x & (G - 1);
-    const int group = threadIdx.x / G;
-    constexpr int ngroups = kBlock / G;
-    const int nrows = r1 - r0;
-    const int iters = (nrows + ngroups - 1) / ngroups;  // uniform across the block
-    for (int it = 0; it < iters; ++it) {
-      const int r = r0 + group + it * ngroups;
-      const bool valid = r < r1;
-      int a = 0, b = 0;
-      if (valid) {
-        a = rp[r] - p0;
-        b = rp[r + 1] - p0;
-      }
-      double acc = 0.0;
-      for (int q = a + lane; q < b; q += G) acc += prod[q];
-      acc = group_sum<G>(acc);
-      if (valid && lane == 0) {
-        const double yp = yprev ? static_cast<double>(yprev[r]) : 0.0;
-        const T yv = static_cast<T>(acc - c * yp);
-        y[r] = yv;
-        s2 += static_cast<double>(yv) * static_cast<double>(yv);
-        sd += yp * static_cast<double>(yv);
-      }
-    }
-  } else {
-    // a single long row (r1 == r0 + 1)
+  const int4 d = desc[blockIdx.x];
+  if (d.y < 0) {
     double acc[1] = {0.0};
-    for (int p = p0 + static_cast<int>(threadIdx.x); p < p0 + nz; p += kBlock)
+    for (int p = d.z + static_cast<int>(threadIdx.x); p < d.w; p += kBlock)
       acc[0] += static_cast<double>(va[p]) * static_cast<double>(x[ci[p]]);
     block_sum<1>(acc, lds);
     if (threadIdx.x == 0) {
-      const double yp = yprev ? static_cast<double>(yprev[r0]) : 0.0;
-      const T yv = static_cast<T>(acc[0] - c * yp);
-      y[r0] = yv;
-      s2 = static_cast<double>(yv) * static_cast<double>(yv);
-      sd = yp * static_cast<double>(yv);
+      lsum[-(d.y + 1)] = acc[0];
+      part[blockIdx.x] = 0.0;
+      part[nbt + blockIdx.x] = 0.0;
+    }
+    return;
+  }
+  const double c = coef.get();
+  const int r0 = d.x, r1 = d.y;
+  const int p0 = rp[r0];
+  const int nz = rp[r1] - p0;
+  double s2 = 0.0, sd = 0.0;
+  constexpr int kUnroll = 8;
+  for (int base = 0; base < nz; base += kBlock * kUnroll) {
+    int idx[kUnroll];
+    double v[kUnroll], xv[kUnroll];
+#pragma unroll
+    for (int u = 0; u < kUnroll; ++u) {
+      const int t = base + u * kBlock + static_cast<int>(threadIdx.x);
+      idx[u] = t < nz ? ci[p0 + t] : -1;
+      v[u] = t < nz ? static_cast<double>(va[p0 + t]) : 0.0;
+    }
+#pragma unroll
+    for (int u = 0; u < kUnroll; ++u) xv[u] = idx[u] >= 0 ? static_cast<double>(x[idx[u]]) : 0.0;
+#pragma unroll
+    for (int u = 0; u < kUnroll; ++u) {
+      const int t = base + u * kBlock + static_cast<int>(threadIdx.x);
+      if (t < nz) prod[t] = v[u] * xv[u];
+    }
+  }
+  __syncthreads();
+  const int lane = threadIdx.x & (G - 1);
+  const int group = threadIdx.x / G;
+  constexpr int ngroups = kBlock / G;
+  const int nrows = r1 - r0;
+  const int iters = (nrows + ngroups - 1) / ngroups;  // uniform across the block
+  for (int it = 0; it < iters; ++it) {
+    const int r = r0 + group + it * ngroups;
+    const bool valid = r < r1;
+    int a = 0, b = 0;
+    if (valid) {
+      a = rp[r] - p0;
+      b = rp[r + 1] - p0;
+    }
+    double acc = 0.0;
+    for (int q = a + lane; q < b; q += G) acc += prod[q];
+    acc = group_sum<G>(acc);
+    if (valid && lane == 0) {
+      const double yp = yprev ? static_cast<double>(yprev[r]) : 0.0;
+      const T yv = static_cast<T>(acc - c * yp);
+      y[r] = yv;
+      s2 += static_cast<double>(yv) * static_cast<double>(yv);
+      sd += yp * static_cast<double>(yv);
     }
   }
   double v2[2] = {s2, sd};
   block_sum<2>(v2, lds);
   if (threadIdx.x == 0) {
     part[blockIdx.x] = v2[0];
-    part[gridDim.x + blockIdx.x] = v2[1];
+    part[nbt + blockIdx.x] = v2[1];
   }
 }
 
+// Long rows: ld[i] = {row, first chunk, number of chunks, -}; one thread per
+// row sums its chunks in order and applies the epilogue; its two partials
+// go to slots nbmain + i.
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_spmv_longfix(int nlong, const int4* __restrict__ ld,
+                                                         const double* __restrict__ lsum,
+                                                         T* __restrict__ y,
+                                                         const T* __restrict__ yprev, Scal coef,
+                                                         double* __restrict__ part, int nbt,
+                                                         int nbmain, Gate gate) {
+  if (gate.off()) return;
+  const int i = blockIdx.x * kBlock + threadIdx.x;
+  if (i >= nlong) return;
+  const int4 d = ld[i];
+  double acc = 0.0;
+  for (int q = 0; q < d.z; ++q) acc += lsum[d.y + q];
+  const double yp = yprev ? static_cast<double>(yprev[d.x]) : 0.0;
+  const T yv = static_cast<T>(acc - coef.get() * yp);
+  y[d.x] = yv;
+  part[nbmain + i] = static_cast<double>(yv) * static_cast<double>(yv);
+  part[nbt + nbmain + i] = yp * static_cast<double>(yv);
+}
+
 template <typename VT, typename T>
-int launch_spmv_rb(hipStream_t s, int nblk, const int* rb, const int* rp, const int* ci,
-                   const VT* va, const T* x, T* y, const T* yprev, Scal coef, double* part,
-                   int g, Gate gate) {
-  if (nblk <= 0) return 0;
-#define PK_RB_CASE(G)                                                                        \
-  case G:                                                                                    \
-    hipLaunchKernelGGL((k_spmv_rb<VT, T, G>), dim3(nblk), dim3(kBlock), 0, s, rb, rp, ci, va, \
-                       x, y, yprev, coef, part, gate);                                       \
+int launch_spmv_rb(hipStream_t s, int nblk, const int4* desc, int nlong, const int4* ldesc,
+                   double* lsum, const int* rp, const int* ci, const VT* va, const T* x, T* y,
+                   const T* yprev, Scal coef, double* part, int g, Gate gate) {
+  const int nbt = nblk + nlong;
+  if (nblk > 0) {
+#define PK_RB_CASE(G)                                                                          \
+  case G:                                                                                      \
+    hipLaunchKernelGGL((k_spmv_rb<VT, T, G>), dim3(nblk), dim3(kBlock), 0, s, desc, rp, ci, va, \
+                       x, y, yprev, coef, part, nbt, lsum, gate);                              \
     break;
-  switch (g) {
-    PK_RB_CASE(2)
-    PK_RB_CASE(4)
-    PK_RB_CASE(8)
-    PK_RB_CASE(16)
-    PK_RB_CASE(32)
-    PK_RB_CASE(64)
-    default:
-      return -1;
-  }
+    switch (g) {
+      PK_RB_CASE(2)
+      PK_RB_CASE(4)
+      PK_RB_CASE(8)
+      PK_RB_CASE(16)
+      PK_RB_CASE(32)
+      PK_RB_CASE(64)
+      default:
+        return -1;
+    }
 #undef PK_RB_CASE
-  return hipGetLastError() == hipSuccess ? nblk : -1;
+  }
+  if (nlong > 0)
+    hipLaunchKernelGGL(k_spmv_longfix<T>, dim3(cdiv(nlong, kBlock)), dim3(kBlock), 0, s, nlong,
+                       ldesc, lsum, y, yprev, coef, part, nbt, nblk, gate);
+  return hipGetLastError() == hipSuccess ? nbt : -1;
 }
 
 // -------------------------------------------------------------- reduce ----
@@ -680,10 +711,10 @@ PK_INST_T(double)
 PK_INST_T(float)
 #undef PK_INST_T
 
-#define PK_INST_RB(VT, T)                                                                    \
-  template int launch_spmv_rb<VT, T>(hipStream_t, int, const int*, const int*, const int*,   \
-                                     const VT*, const T*, T*, const T*, Scal, double*, int,  \
-                                     Gate);
+#define PK_INST_RB(VT, T)                                                                   \
+  template int launch_spmv_rb<VT, T>(hipStream_t, int, const int4*, int, const int4*, double*, \
+                                     const int*, const int*, const VT*, const T*, T*,         \
+                                     const T*, Scal, double*, int, Gate);
 PK_INST_RB(double, double)
 PK_INST_RB(float, float)
 PK_INST_RB(double, float)

@@ -40,7 +40,7 @@ contains
   ! PRINTSTAT-style callers see them.
   subroutine add_timing(eng, st0, nbsvd_add, nrestart_add, nlandim_set, nsing_set)
     type(c_ptr), intent(in) :: eng
-    integer(c_long_long), intent(in) :: st0(5)
+    integer(c_long_long), intent(in) :: st0(8)
     integer, intent(in) :: nbsvd_add, nrestart_add, nlandim_set, nsing_set
     integer :: nopx, nreorth, ndot, nreorthu, nreorthv, nitref, nrestart, nbsvd, nlandim, nsing
     real :: tmvopx, tgetu0, tupdmu, tupdnu, tintv, tlanbpro, treorth, treorthu, treorthv, &
@@ -48,7 +48,7 @@ contains
     common /timing/ nopx, nreorth, ndot, nreorthu, nreorthv, nitref, nrestart, nbsvd, tmvopx, &
         tgetu0, tupdmu, tupdnu, tintv, tlanbpro, treorth, treorthu, treorthv, telru, telrv, &
         tbsvd, tnorm2, tlansvd, nlandim, tritzvec, trestart, tdot, nsing
-    integer(c_long_long) :: st(5)
+    integer(c_long_long) :: st(8)
     integer(c_int) :: rc
     rc = pk_lbd_stats(eng, st)
     if (rc == PK_OK) then
@@ -120,7 +120,7 @@ contains
     double precision :: eps34, epsn, tol, anorm, rnorm, dummy(1, 1)
     integer :: lanmax, neig, jold, j, jj, dj, i, ierr, lapinfo, nbsvd
     integer(c_int) :: rc, ierr_c, jj_c
-    integer(c_long_long) :: st0(5)
+    integer(c_long_long) :: st0(8)
     logical :: vectors
 
     rc = pk_lbd_stats(eng, st0)
@@ -237,7 +237,7 @@ contains
     double precision :: eps34, epsn, tol, anorm, rnorm, relgap, dummy(1, 1)
     integer :: k, i, iter, nconv, kold, nshft, ierr, lapinfo, nbsvd, nrestart, jj
     integer(c_int) :: rc, ierr_c, kk_c
-    integer(c_long_long) :: st0(5)
+    integer(c_long_long) :: st0(8)
     logical :: small, vectors
 
     rc = pk_lbd_stats(eng, st0)

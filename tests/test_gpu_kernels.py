@@ -101,3 +101,18 @@ def test_spmv_deterministic():
     y1, y2 = op.spmv(x, "n"), op.spmv(x, "n")
     assert np.array_equal(y1, y2)
     np.testing.assert_allclose(y1, A.matvec(x), rtol=0, atol=1e-12 * np.abs(y1).max())
+
+
+def test_spmv_powerlaw_long_rows():
+    """Zipf-distributed columns give rows of A^T with tens of thousands of
+    nonzeros, split into chunks and finished by the fix-up kernel."""
+    A = synth.powerlaw_bipartite(400_000, 200_000, seed=3)
+    AT = A.transpose()
+    assert np.diff(AT.rowptr).max() > 4 * 8192
+    op = api.CsrOperator(A)
+    rng = np.random.default_rng(5)
+    x, y = rng.standard_normal(A.n), rng.standard_normal(A.m)
+    want_n, want_t = A.matvec(x), AT.matvec(y)
+    np.testing.assert_allclose(op.spmv(x, "n"), want_n, rtol=0, atol=1e-10 * np.abs(want_n).max())
+    np.testing.assert_allclose(op.spmv(y, "t"), want_t, rtol=0, atol=1e-10 * np.abs(want_t).max())
+    assert np.array_equal(op.spmv(y, "t"), op.spmv(y, "t"))

@@ -185,3 +185,33 @@ def test_fp32_basis_with_fp64_polish(small_case):
     assert r.info == 0 and r.k == 10
     np.testing.assert_allclose(r.sigma, ref.sigma, rtol=1e-10)
     _check_triplets(r, D, 1e-11, s[0])
+
+
+def test_single_precision_irl(small_case):
+    A, u0, D, s = small_case
+    op = api.CsrOperator(A, prec=api.PK_F32)
+    r = api.lansvd_irl(op, A.m, A.n, 10, 30, u0=u0, tol=1e-6, prec=api.PK_F32)
+    ref = refpy.lansvd_irl(A, 10, 30, u0=u0, tol=1e-6, prec="s", cgs=1)
+    assert r.info == 0 and r.k == 10 and ref.k == 10
+    np.testing.assert_allclose(r.sigma, s[:10], rtol=2e-6)
+    np.testing.assert_allclose(r.sigma, ref.sigma, rtol=1e-5)
+
+
+def test_irl_smallest_triplets():
+    """WHICH='S' (double/dlansvd_irl.F:245-344) on a small well-conditioned
+    matrix: the k smallest singular values against the dense SVD and the
+    reference run.  p <= dim/2: the reference documents WHICH='S' as unreliable
+    for p > dim/2 (Changelog:56)."""
+    A = synth.random_uniform(400, 120, 6000, seed=11)
+    D = A.todense()
+    s = np.linalg.svd(D, compute_uv=False)
+    u0 = synth.start_vector(A.m, 12)
+    op = api.CsrOperator(A)
+    r = api.lansvd_irl(op, A.m, A.n, 5, 60, p=30, u0=u0, tol=1e-10, which="S", maxiter=2000)
+    ref = refpy.lansvd_irl(A, 5, 60, p=30, u0=u0, tol=1e-10, which="S", cgs=1, maxiter=2000)
+    print("gpu", r.info, r.k, r.sigma, r.counters)
+    print("ref", ref.info, ref.k, ref.sigma, ref.counters)
+    assert r.k == ref.k and r.k > 0
+    np.testing.assert_allclose(np.sort(r.sigma), np.sort(ref.sigma), rtol=1e-9)
+    smallest = np.sort(s)[: r.k]
+    np.testing.assert_allclose(np.sort(r.sigma), smallest, rtol=1e-8)
