@@ -1,0 +1,119 @@
+"""scipy-compatible front end: ``svdp`` with the signature and return
+convention of scipy's PROPACK wrapper (scipy/sparse/linalg/_svdp.py,
+``_svdp``), running on the MI355X library.
+
+Dense arrays and scipy sparse matrices become a device CSR operator (A and
+A^T resident in HBM; the drivers recognise the library's own APROD and keep
+the whole solve on the GPU).  A ``scipy.sparse.linalg.LinearOperator`` is
+called from the host as the APROD callback (matvec / rmatvec), the rest of
+the Lanczos process still runs on the GPU.  float32 inputs use SLANSVD /
+SLANSVD_IRL, everything else DLANSVD / DLANSVD_IRL.  Complex inputs are
+not supported.
+"""
+from __future__ import annotations
+
+import numpy as np
+from numpy.linalg import LinAlgError
+
+from . import api, synth
+
+_which = {"LM": "L", "SM": "S"}
+
+
+def _as_operator(A, prec):
+    """(aprod, m, n, keepalive): a CsrOperator for explicit matrices, a
+    Python callable for LinearOperators."""
+    try:
+        import scipy.sparse as sp
+        from scipy.sparse.linalg import LinearOperator
+    except Exception:  # pragma: no cover - scipy is optional for explicit matrices
+        sp, LinearOperator = None, None
+    dt = np.float64 if prec == api.PK_F64 else np.float32
+    if LinearOperator is not None and isinstance(A, LinearOperator):
+        m, n = A.shape
+
+        def aprod(t, x):
+            return A.matvec(x) if t == "n" else A.rmatvec(x)
+        return aprod, m, n, None
+    if sp is not None and sp.issparse(A):
+        C = sp.csr_matrix(A)
+        C.sum_duplicates()
+        C.sort_indices()
+        csr = synth.Csr(C.shape[0], C.shape[1], C.indptr.astype(np.int32),
+                        C.indices.astype(np.int32), np.asarray(C.data, dtype=dt))
+    else:
+        D = np.asarray(A)
+        if D.ndim != 2:
+            raise ValueError("A must be two-dimensional")
+        r, c = np.nonzero(D)
+        csr = synth.from_coo(D.shape[0], D.shape[1], r, c, D[r, c].astype(dt))
+    op = api.CsrOperator(csr, prec=prec)
+    return op, csr.m, csr.n, op
+
+
+def svdp(A, k, which="LM", irl_mode=True, kmax=None, compute_u=True, compute_v=True,
+         v0=None, full_output=False, tol=0, delta=None, eta=None, anorm=0, cgs=False,
+         elr=True, min_relgap=0.002, shifts=None, maxiter=None, rng=None):
+    """Truncated SVD of A through the library's DLANSVD(_IRL) drop-ins.
+
+    Returns ``(u, sigma, vt)`` or ``(u, sigma, vt, bnd)`` with
+    ``full_output`` -- the layout of scipy's ``_svdp``.
+    """
+    which = which.upper()
+    if which not in _which:
+        raise ValueError("`which` must be either 'LM' or 'SM'")
+    if not irl_mode and which == "SM":
+        raise ValueError("`which`='SM' requires irl_mode=True")
+    dtype = getattr(A, "dtype", np.float64)
+    if np.issubdtype(np.dtype(dtype), np.complexfloating):
+        raise NotImplementedError("complex matrices are not supported")
+    prec = api.PK_F32 if np.dtype(dtype) == np.float32 else api.PK_F64
+    aprod, m, n, keep = _as_operator(A, prec)
+    if k < 1 or k > min(m, n):
+        raise ValueError("k must be positive and not greater than m or n")
+    if kmax is None:
+        kmax = 10 * k
+    if maxiter is None:
+        maxiter = 1000
+    kmax = min(m + 1, n + 1, kmax)
+    if kmax < k:
+        raise ValueError(f"kmax must be greater than or equal to k, but kmax ({kmax}) < k ({k})")
+    if v0 is None:
+        if rng is None:
+            rng = np.random.default_rng()
+        u0 = rng.uniform(size=m)
+    else:
+        u0 = np.asarray(v0, dtype=np.float64)
+        if u0.shape != (m,):
+            raise ValueError(f"v0 must be of length {m}")
+    eps = float(np.finfo(np.float64 if prec == api.PK_F64 else np.float32).eps)
+    delta = np.sqrt(eps) if delta is None else delta
+    eta = eps ** 0.75 if eta is None else eta
+    jobu = "y" if compute_u else "n"
+    jobv = "y" if compute_v else "n"
+    if irl_mode:
+        if shifts is None:
+            shifts = kmax - k
+        if k > min(kmax - shifts, m, n):
+            raise ValueError("shifts must satisfy k <= min(kmax-shifts, m, n)!")
+        if shifts < 0:
+            raise ValueError("shifts must be >= 0!")
+        r = api.lansvd_irl(aprod, m, n, k, kmax, p=shifts, u0=u0, tol=tol, which=_which[which],
+                           jobu=jobu, jobv=jobv, maxiter=maxiter, prec=prec, delta=delta,
+                           eta=eta, anorm=anorm, min_relgap=min_relgap, cgs=int(bool(cgs)),
+                           elr=int(bool(elr)))
+    else:
+        r = api.lansvd(aprod, m, n, k, kmax, u0=u0, tol=tol, jobu=jobu, jobv=jobv, prec=prec,
+                       delta=delta, eta=eta, anorm=anorm, cgs=int(bool(cgs)),
+                       elr=int(bool(elr)))
+    del keep
+    if r.info > 0:
+        raise LinAlgError(f"An invariant subspace of dimension {r.info} was found.")
+    if r.info < 0:
+        raise LinAlgError(f"k={k} singular triplets did not converge within kmax={kmax} "
+                          "iterations")
+    u = r.U if compute_u else None
+    vt = r.V.T.copy() if compute_v else None
+    if full_output:
+        return u, r.sigma, vt, r.bnd
+    return u, r.sigma, vt

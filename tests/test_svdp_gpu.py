@@ -1,0 +1,73 @@
+"""The scipy-compatible front end on the GPU, on the cases and tolerances of
+scipy's own PROPACK tests (scipy/sparse/linalg/tests/test_propack.py:12-89):
+random 10 x 20 matrices as dense / CSR / CSC, float32 and float64, IRL and
+plain, largest and smallest triplets; singular values against the dense
+SVD, singular vectors orthogonal to the dense ones up to sign."""
+import numpy as np
+import pytest
+from numpy.testing import assert_allclose
+from scipy.sparse import csc_array, csr_array
+from scipy.sparse.linalg import aslinearoperator
+
+from propack_amd.svdp import svdp
+
+pytestmark = pytest.mark.gpu
+
+TOLS = {np.float32: 1e-4, np.float64: 1e-8}
+
+
+def generate_matrix(n, m, f, dtype, rseed=0):
+    rng = np.random.RandomState(rseed)
+    M = (-5 + 10 * rng.rand(n, m)).astype(dtype)
+    M[M > 10 * f - 5] = 0
+    return M
+
+
+def assert_orthogonal(u1, u2, rtol, atol):
+    A = abs(np.dot(u1.T, u2))
+    assert_allclose(A, np.eye(u1.shape[1], u2.shape[1]), rtol=rtol, atol=atol)
+
+
+@pytest.mark.parametrize("ctor", (np.array, csr_array, csc_array))
+@pytest.mark.parametrize("dtype", (np.float32, np.float64))
+@pytest.mark.parametrize("irl", (True, False))
+@pytest.mark.parametrize("which", ("LM", "SM"))
+def test_svdp(ctor, dtype, irl, which):
+    n, m, k = 10, 20, 3
+    tol = TOLS[dtype]
+    M = generate_matrix(n, m, 0.8, dtype)
+    if which == "SM" and not irl:
+        with pytest.raises(ValueError, match="requires irl_mode=True"):
+            svdp(ctor(M), k=k, which=which, irl_mode=irl, tol=tol)
+        return
+    u1, s1, vt1 = np.linalg.svd(M.astype(np.float64), full_matrices=False)
+    u2, s2, vt2 = svdp(ctor(M), k=k, which=which, irl_mode=irl, tol=tol,
+                       rng=np.random.default_rng(0))
+    if which == "SM":
+        u1 = np.roll(u1, k, 1)
+        vt1 = np.roll(vt1, k, 0)
+        s1 = np.roll(s1, k)
+    assert_allclose(s1[:k], s2, rtol=tol, atol=tol)
+    assert_orthogonal(u1, u2.astype(np.float64), rtol=tol, atol=tol)
+    assert_orthogonal(vt1.T, vt2.T.astype(np.float64), rtol=tol, atol=tol)
+
+
+def test_linear_operator_callback():
+    """A LinearOperator runs as a host APROD callback."""
+    M = generate_matrix(30, 50, 0.5, np.float64, rseed=3)
+    u, s, vt, bnd = svdp(aslinearoperator(M), k=4, full_output=True, tol=1e-10,
+                         rng=np.random.default_rng(1))
+    assert_allclose(s, np.linalg.svd(M, compute_uv=False)[:4], rtol=1e-10)
+    assert np.all(bnd >= 0)
+
+
+def test_argument_errors():
+    M = generate_matrix(10, 20, 0.8, np.float64)
+    with pytest.raises(ValueError):
+        svdp(M, k=0)
+    with pytest.raises(ValueError):
+        svdp(M, k=3, which="XX")
+    with pytest.raises(ValueError):
+        svdp(M, k=3, v0=np.ones(5))
+    with pytest.raises(NotImplementedError):
+        svdp(M.astype(complex), k=3)
