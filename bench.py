@@ -93,6 +93,19 @@ def cpu_baseline(A, u0, gpu_nopx):
             "sample_s": t, "sample_nopx": nopx}
 
 
+def copy_calibration():
+    """A plain device-to-device copy of exactly 2^30 bytes, five times: the
+    HBM counters of this kernel give bytes-counted / bytes-moved."""
+    import torch
+    src = torch.ones(1 << 27, dtype=torch.float64, device="cuda")
+    dst = torch.empty_like(src)
+    torch.cuda.synchronize()
+    for _ in range(5):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    log(f"[bench] copy calibration: 5 x {src.numel() * 8} bytes")
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -101,7 +114,13 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-check", action="store_true")
     ap.add_argument("--no-mixed", action="store_true")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="only run a plain 1 GiB device copy 5 times (the reference "
+                         "kernel for scaling FETCH_SIZE/WRITE_SIZE counters) and exit")
     args = ap.parse_args()
+    if args.copy_calibration:
+        copy_calibration()
+        return
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported "
               "(the multi-GPU layout is a design note only)", file=sys.stderr)

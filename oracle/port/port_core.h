@@ -893,6 +893,7 @@ int P(lansvd_irl)(char which, char jobu, char jobv, int m, int n, int *dimp,
         int kk = dim;
         ierr = P(lanbpro)(m, n, kold, &kk, aprod, ctx, U, ldu, V, ldv, al, be, &rnorm,
                           doption, ioption);
+        dim = kk; /* DLANBPRO lowers K (= DIM) at an invariant subspace (dlansvd_irl.F:213) */
         kold = k;
         memcpy(al1, al, sizeof(R) * dim);
         memcpy(be1, be, sizeof(R) * dim);
@@ -965,6 +966,7 @@ int P(lansvd_irl)(char which, char jobu, char jobv, int m, int n, int *dimp,
         ritzvec(which, jobu, jobv, m, n, nconv, dim, al1, be1, U, ldu, V, ldv);
     }
     *neigp = nconv;
+    *dimp = dim;
     st.nlandim = dim;
     free(al); free(be); free(al1); free(be1); free(wb); free(shift);
     free(Pm); free(Qm); free(work);

@@ -280,6 +280,9 @@ contains
         return
       end if
       ierr = ierr_c
+      ! DLANBPRO lowers its K argument (DIM here) when it stops at an
+      ! invariant subspace (double/dlanbpro.F:389,519; double/dlansvd_irl.F:213)
+      dim = kk_c
       kold = k
       ! SVD of B_dim and error bounds (double/dlansvd_irl.F:220-251)
       a1 = alpha

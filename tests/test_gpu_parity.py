@@ -219,3 +219,30 @@ def test_irl_smallest_triplets():
     np.testing.assert_allclose(np.sort(r.sigma), np.sort(ref.sigma), rtol=1e-9)
     smallest = np.sort(s)[: r.k]
     np.testing.assert_allclose(np.sort(r.sigma), smallest, rtol=1e-8)
+
+
+@pytest.mark.parametrize("prec", ["d", "s"])
+def test_invariant_subspace_lowers_dim(prec):
+    """DIM = m + 1: the Lanczos process meets an invariant subspace at step
+    m, DLANBPRO lowers DIM, and the restarted driver continues with it
+    (double/dlanbpro.F:519, double/dlansvd_irl.F:213).  Smallest triplets of
+    scipy's 10 x 20 test matrix; DIM is written back as 10."""
+    rng = np.random.RandomState(0)
+    M = (-5 + 10 * rng.rand(10, 20)).astype(np.float32)
+    M[M > 3] = 0
+    r_, c_ = np.nonzero(M)
+    A = synth.from_coo(10, 20, r_, c_, M[r_, c_].astype(np.float64))
+    pk = api.PK_F64
+    if prec == "s":
+        A, pk = A.astype(np.float32), api.PK_F32
+    u0 = np.random.default_rng(0).uniform(size=10)
+    tol = 1e-4 if prec == "s" else 1e-8
+    op = api.CsrOperator(A, prec=pk)
+    r = api.lansvd_irl(op, 10, 20, 3, 11, p=8, u0=u0, tol=tol, which="S", prec=pk)
+    ref = refpy.lansvd_irl(A, 3, 11, p=8, u0=u0, tol=tol, which="S", prec=prec)
+    s = np.linalg.svd(M.astype(np.float64), compute_uv=False)
+    assert r.info == ref.info == 0 and r.k == ref.k == 3
+    assert r.dim == ref.dim == 10 and r.counters["nlandim"] == 10
+    rt = 1e-5 if prec == "s" else 1e-12
+    np.testing.assert_allclose(np.sort(r.sigma), np.sort(s)[:3], rtol=rt)
+    np.testing.assert_allclose(r.sigma, ref.sigma, rtol=rt)

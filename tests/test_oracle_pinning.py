@@ -18,7 +18,7 @@ import numpy as np
 import pytest
 
 from oracle import portpy, refpy
-from propack_amd import api
+from propack_amd import api, synth
 
 pytestmark = pytest.mark.skipif(not (refpy.available("d") and portpy.available()),
                                 reason="oracle not built (make -C oracle)")
@@ -311,3 +311,27 @@ def test_single_precision_reference(small_case):
     np.testing.assert_allclose(r.sigma, s[:10], rtol=2e-6)
     np.testing.assert_allclose(q.sigma, r.sigma, rtol=2e-6)
     assert r.counters["nopx"] == q.counters["nopx"]
+
+
+@pytest.mark.parametrize("prec", ["d", "s"])
+def test_port_invariant_subspace_lowers_dim(prec):
+    """DIM > min(m, n): at step m the Lanczos process reaches an invariant
+    subspace, DLANBPRO lowers its K argument, and the restarted driver goes
+    on with the lowered DIM (double/dlanbpro.F:519, double/dlansvd_irl.F:213).
+    Smallest triplets of a 10 x 20 matrix (scipy's test_svdp case)."""
+    rng = np.random.RandomState(0)
+    M = (-5 + 10 * rng.rand(10, 20)).astype(np.float32)
+    M[M > 3] = 0
+    r_, c_ = np.nonzero(M)
+    A = synth.from_coo(10, 20, r_, c_, M[r_, c_].astype(np.float64))
+    if prec == "s":
+        A = A.astype(np.float32)
+    u0 = np.random.default_rng(0).uniform(size=10)
+    tol = 1e-4 if prec == "s" else 1e-8
+    r = refpy.lansvd_irl(A, 3, 11, p=8, u0=u0, tol=tol, which="S", prec=prec)
+    q = portpy.lansvd_irl(A, 3, 11, p=8, u0=u0, tol=tol, which="S", prec=prec)
+    s = np.linalg.svd(M.astype(np.float64), compute_uv=False)
+    assert r.info == q.info == 0 and r.k == q.k == 3
+    assert r.dim == q.dim == 10 and r.counters["nlandim"] == q.counters["nlandim"] == 10
+    np.testing.assert_allclose(np.sort(q.sigma), np.sort(s)[:3], rtol=1e-5 if prec == "s" else 1e-12)
+    np.testing.assert_allclose(q.sigma, r.sigma, rtol=1e-5 if prec == "s" else 1e-12)
