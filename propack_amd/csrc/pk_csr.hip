@@ -8,6 +8,8 @@
 #include <rocprim/device/device_radix_sort.hpp>
 
 #include <algorithm>
+#include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <vector>
 
@@ -123,13 +125,15 @@ bool upload(const Partition& P, CsrOp::Part& d) {
 
 }  // namespace
 
+// A^T of the CSR (m x n, nnz) as CSR (n x m): stable radix sort of the
+// nonzero positions keyed by column.
 template <typename VT>
-static int build_transpose(CsrOp* op, hipStream_t s) {
-  const long long nnz = op->nnz;
-  const int m = op->m, n = op->n;
+static int build_transpose(int m, int n, long long nnz, const int* rp, const int* ci,
+                           const void* va, int** rpt_out, int** cit_out, void** vat_out,
+                           hipStream_t s) {
   int *rowidx = nullptr, *keys_in = nullptr, *keys_out = nullptr, *perm_in = nullptr,
-      *perm_out = nullptr;
-  void* tmp = nullptr;
+      *perm_out = nullptr, *rpt = nullptr, *cit = nullptr;
+  void *vat = nullptr, *tmp = nullptr;
   size_t tmp_bytes = 0;
   int rc = PK_OK;
   const size_t ib = sizeof(int) * static_cast<size_t>(std::max<long long>(nnz, 1));
@@ -143,12 +147,12 @@ static int build_transpose(CsrOp* op, hipStream_t s) {
   PKT(hipMalloc(&keys_out, ib));
   PKT(hipMalloc(&perm_in, ib));
   PKT(hipMalloc(&perm_out, ib));
-  PKT(hipMalloc(&op->rpt, sizeof(int) * (static_cast<size_t>(n) + 1)));
-  PKT(hipMalloc(&op->cit, ib));
-  PKT(hipMalloc(&op->vat, op->val_bytes * static_cast<size_t>(std::max<long long>(nnz, 1))));
+  PKT(hipMalloc(&rpt, sizeof(int) * (static_cast<size_t>(n) + 1)));
+  PKT(hipMalloc(&cit, ib));
+  PKT(hipMalloc(&vat, sizeof(VT) * static_cast<size_t>(std::max<long long>(nnz, 1))));
   if (nnz > 0) {
-    hipLaunchKernelGGL(k_row_index, dim3(2048), dim3(256), 0, s, m, op->rp, rowidx);
-    PKT(hipMemcpyAsync(keys_in, op->ci, sizeof(int) * nnz, hipMemcpyDeviceToDevice, s));
+    hipLaunchKernelGGL(k_row_index, dim3(2048), dim3(256), 0, s, m, rp, rowidx);
+    PKT(hipMemcpyAsync(keys_in, ci, sizeof(int) * nnz, hipMemcpyDeviceToDevice, s));
     hipLaunchKernelGGL(k_iota, dim3(2048), dim3(256), 0, s, nnz, perm_in);
     PKT(rocprim::radix_sort_pairs(nullptr, tmp_bytes, keys_in, keys_out, perm_in, perm_out,
                                   static_cast<size_t>(nnz), 0, bits_for(n), s));
@@ -156,10 +160,9 @@ static int build_transpose(CsrOp* op, hipStream_t s) {
     PKT(rocprim::radix_sort_pairs(tmp, tmp_bytes, keys_in, keys_out, perm_in, perm_out,
                                   static_cast<size_t>(nnz), 0, bits_for(n), s));
     hipLaunchKernelGGL((k_gather_t<VT>), dim3(2048), dim3(256), 0, s, nnz, perm_out, rowidx,
-                       static_cast<const VT*>(op->va), op->cit, static_cast<VT*>(op->vat));
+                       static_cast<const VT*>(va), cit, static_cast<VT*>(vat));
   }
-  hipLaunchKernelGGL(k_rowptr_from_keys, dim3(2048), dim3(256), 0, s, nnz, n, keys_out,
-                     op->rpt);
+  hipLaunchKernelGGL(k_rowptr_from_keys, dim3(2048), dim3(256), 0, s, nnz, n, keys_out, rpt);
   PKT(hipGetLastError());
   PKT(hipStreamSynchronize(s));
 out:
@@ -170,7 +173,174 @@ out:
   (void)hipFree(perm_in);
   (void)hipFree(perm_out);
   (void)hipFree(tmp);
+  if (rc != PK_OK) {
+    (void)hipFree(rpt);
+    (void)hipFree(cit);
+    (void)hipFree(vat);
+    rpt = cit = nullptr;
+    vat = nullptr;
+  }
+  *rpt_out = rpt;
+  *cit_out = cit;
+  *vat_out = vat;
   return rc;
+}
+
+namespace {
+
+__global__ void k_panel_key(long long nnz, const int* __restrict__ ci, int width,
+                            int* __restrict__ key) {
+  for (long long q = static_cast<long long>(blockIdx.x) * blockDim.x + threadIdx.x; q < nnz;
+       q += static_cast<long long>(gridDim.x) * blockDim.x)
+    key[q] = ci[q] / width;
+}
+
+template <typename VT>
+__global__ void k_gather_panel(long long cnt, const int* __restrict__ perm,
+                               const int* __restrict__ rowidx, const int* __restrict__ ci,
+                               const VT* __restrict__ va, int* __restrict__ ridx,
+                               int* __restrict__ cip, VT* __restrict__ vap) {
+  for (long long q = static_cast<long long>(blockIdx.x) * blockDim.x + threadIdx.x; q < cnt;
+       q += static_cast<long long>(gridDim.x) * blockDim.x) {
+    const int p = perm[q];
+    ridx[q] = rowidx[p];
+    cip[q] = ci[p];
+    vap[q] = va[p];
+  }
+}
+
+// Column panels of an operator direction: PK_SPMV_PANELS ("p" for both
+// directions or "pA,pAt"), else one panel.  Powers of two up to
+// CsrOp::kMaxPanels.
+int panels_for(bool transposed) {
+  const char* e = std::getenv("PK_SPMV_PANELS");
+  if (!e || !*e) return 1;
+  int a = 1, b = 1;
+  const int got = std::sscanf(e, "%d,%d", &a, &b);
+  int p = got == 2 && transposed ? b : a;
+  if (got < 1) return 1;
+  int q = 1;
+  while (q * 2 <= p && q * 2 <= CsrOp::kMaxPanels) q *= 2;
+  return q;
+}
+
+bool finish_panel(CsrOp::Panel& P, int rows) {
+  std::vector<int> hrp(static_cast<size_t>(rows) + 1);
+  if (hipMemcpy(hrp.data(), P.rp, sizeof(int) * (static_cast<size_t>(rows) + 1),
+                hipMemcpyDeviceToHost) != hipSuccess)
+    return false;
+  P.g = choose_lpr(P.nnz, rows);
+  return upload(partition(hrp, rows), P.part);
+}
+
+}  // namespace
+
+// Lays one direction out as np column panels.  Takes ownership of the
+// plain CSR arrays (rp, ci, va): kept as the single panel when np = 1,
+// freed after the panels are built otherwise.
+template <typename VT>
+static int build_dir(CsrOp::Dir& D, int rows, int cols, long long nnz, int np, int* rp,
+                     int* ci, void* va, hipStream_t s) {
+  D.rows = rows;
+  D.cols = cols;
+  if (np <= 1 || cols < np) {
+    D.np = 1;
+    D.width = cols;
+    D.pan[0].rp = rp;
+    D.pan[0].ci = ci;
+    D.pan[0].va = va;
+    D.pan[0].nnz = nnz;
+    if (!finish_panel(D.pan[0], rows)) return PK_ERR_HIP;
+    D.slots = D.pan[0].part.slots();
+    return PK_OK;
+  }
+  D.np = np;
+  D.width = (cols + np - 1) / np;
+  int *rowidx = nullptr, *keys_in = nullptr, *keys_out = nullptr, *perm_in = nullptr,
+      *perm_out = nullptr, *poff_d = nullptr;
+  void* tmp = nullptr;
+  size_t tmp_bytes = 0;
+  int rc = PK_OK;
+  std::vector<int> poff(static_cast<size_t>(np) + 1, 0);
+  const size_t ib = sizeof(int) * static_cast<size_t>(std::max<long long>(nnz, 1));
+#define PKT(x)                 \
+  if ((x) != hipSuccess) {     \
+    rc = PK_ERR_HIP;           \
+    goto out;                  \
+  }
+  PKT(hipMalloc(&rowidx, ib));
+  PKT(hipMalloc(&keys_in, ib));
+  PKT(hipMalloc(&keys_out, ib));
+  PKT(hipMalloc(&perm_in, ib));
+  PKT(hipMalloc(&perm_out, ib));
+  PKT(hipMalloc(&poff_d, sizeof(int) * (static_cast<size_t>(np) + 1)));
+  PKT(hipMalloc(&D.yacc, sizeof(double) * std::max(rows, 1)));
+  if (nnz > 0) {
+    hipLaunchKernelGGL(k_row_index, dim3(2048), dim3(256), 0, s, rows, rp, rowidx);
+    hipLaunchKernelGGL(k_panel_key, dim3(2048), dim3(256), 0, s, nnz, ci, D.width, keys_in);
+    hipLaunchKernelGGL(k_iota, dim3(2048), dim3(256), 0, s, nnz, perm_in);
+    PKT(rocprim::radix_sort_pairs(nullptr, tmp_bytes, keys_in, keys_out, perm_in, perm_out,
+                                  static_cast<size_t>(nnz), 0, bits_for(np), s));
+    PKT(hipMalloc(&tmp, tmp_bytes));
+    PKT(rocprim::radix_sort_pairs(tmp, tmp_bytes, keys_in, keys_out, perm_in, perm_out,
+                                  static_cast<size_t>(nnz), 0, bits_for(np), s));
+  }
+  hipLaunchKernelGGL(k_rowptr_from_keys, dim3(64), dim3(256), 0, s, nnz, np, keys_out, poff_d);
+  PKT(hipGetLastError());
+  PKT(hipMemcpyAsync(poff.data(), poff_d, sizeof(int) * (np + 1), hipMemcpyDeviceToHost, s));
+  PKT(hipStreamSynchronize(s));
+  for (int p = 0; p < np; ++p) {
+    CsrOp::Panel& P = D.pan[p];
+    const long long cnt = static_cast<long long>(poff[p + 1]) - poff[p];
+    const size_t cb = static_cast<size_t>(std::max<long long>(cnt, 1));
+    P.nnz = cnt;
+    PKT(hipMalloc(&P.rp, sizeof(int) * (static_cast<size_t>(rows) + 1)));
+    PKT(hipMalloc(&P.ci, sizeof(int) * cb));
+    PKT(hipMalloc(&P.va, sizeof(VT) * cb));
+    if (cnt > 0)
+      hipLaunchKernelGGL((k_gather_panel<VT>), dim3(2048), dim3(256), 0, s, cnt,
+                         perm_out + poff[p], rowidx, ci, static_cast<const VT*>(va), keys_in,
+                         P.ci, static_cast<VT*>(P.va));
+    // entries of a panel stay in row-major order (stable sort): row
+    // pointers from the gathered row indices
+    hipLaunchKernelGGL(k_rowptr_from_keys, dim3(2048), dim3(256), 0, s, cnt, rows, keys_in,
+                       P.rp);
+    PKT(hipGetLastError());
+    PKT(hipStreamSynchronize(s));
+    if (!finish_panel(P, rows)) {
+      rc = PK_ERR_HIP;
+      goto out;
+    }
+    D.slots = std::max(D.slots, P.part.slots());
+  }
+out:
+#undef PKT
+  (void)hipFree(rowidx);
+  (void)hipFree(keys_in);
+  (void)hipFree(keys_out);
+  (void)hipFree(perm_in);
+  (void)hipFree(perm_out);
+  (void)hipFree(poff_d);
+  (void)hipFree(tmp);
+  (void)hipFree(rp);
+  (void)hipFree(ci);
+  (void)hipFree(va);
+  return rc;
+}
+
+static void free_dir(CsrOp::Dir& D) {
+  for (int p = 0; p < CsrOp::kMaxPanels; ++p) {
+    CsrOp::Panel& P = D.pan[p];
+    (void)hipFree(P.rp);
+    (void)hipFree(P.ci);
+    (void)hipFree(P.va);
+    (void)hipFree(P.part.desc);
+    (void)hipFree(P.part.ldesc);
+    (void)hipFree(P.part.lsum);
+    P = CsrOp::Panel{};
+  }
+  (void)hipFree(D.yacc);
+  D.yacc = nullptr;
 }
 
 int csr_create(int m, int n, long long nnz, const int* rowptr, const int* colind,
@@ -197,84 +367,91 @@ int csr_create(int m, int n, long long nnz, const int* rowptr, const int* colind
   hipStream_t s = nullptr;
   int rc = PK_OK;
   const size_t nz = static_cast<size_t>(std::max<long long>(nnz, 1));
+  int *rp = nullptr, *ci = nullptr, *rpt = nullptr, *cit = nullptr;
+  void *va = nullptr, *vat = nullptr;
   if (hipStreamCreate(&s) != hipSuccess) {
     delete op;
     return PK_ERR_HIP;
   }
-  if (hipMalloc(&op->rp, sizeof(int) * (static_cast<size_t>(m) + 1)) != hipSuccess ||
-      hipMalloc(&op->ci, sizeof(int) * nz) != hipSuccess ||
-      hipMalloc(&op->va, op->val_bytes * nz) != hipSuccess ||
-      hipMemcpyAsync(op->rp, rowptr, sizeof(int) * (static_cast<size_t>(m) + 1), kind, s) !=
+  if (hipMalloc(&rp, sizeof(int) * (static_cast<size_t>(m) + 1)) != hipSuccess ||
+      hipMalloc(&ci, sizeof(int) * nz) != hipSuccess ||
+      hipMalloc(&va, op->val_bytes * nz) != hipSuccess ||
+      hipMemcpyAsync(rp, rowptr, sizeof(int) * (static_cast<size_t>(m) + 1), kind, s) !=
           hipSuccess ||
-      (nnz > 0 && hipMemcpyAsync(op->ci, colind, sizeof(int) * nnz, kind, s) != hipSuccess) ||
-      (nnz > 0 && hipMemcpyAsync(op->va, val, op->val_bytes * nnz, kind, s) != hipSuccess)) {
+      (nnz > 0 && hipMemcpyAsync(ci, colind, sizeof(int) * nnz, kind, s) != hipSuccess) ||
+      (nnz > 0 && hipMemcpyAsync(va, val, op->val_bytes * nnz, kind, s) != hipSuccess) ||
+      hipStreamSynchronize(s) != hipSuccess) {
     rc = PK_ERR_HIP;
   }
   if (rc == PK_OK)
-    rc = val_prec == PK_F64 ? build_transpose<double>(op, s) : build_transpose<float>(op, s);
+    rc = val_prec == PK_F64
+             ? build_transpose<double>(m, n, nnz, rp, ci, va, &rpt, &cit, &vat, s)
+             : build_transpose<float>(m, n, nnz, rp, ci, va, &rpt, &cit, &vat, s);
+  if (rc == PK_OK) {
+    // build_dir owns the arrays from here on
+    rc = val_prec == PK_F64
+             ? build_dir<double>(op->da, m, n, nnz, panels_for(false), rp, ci, va, s)
+             : build_dir<float>(op->da, m, n, nnz, panels_for(false), rp, ci, va, s);
+    rp = ci = nullptr;
+    va = nullptr;
+    const int rc2 = val_prec == PK_F64
+                        ? build_dir<double>(op->dt, n, m, nnz, panels_for(true), rpt, cit, vat, s)
+                        : build_dir<float>(op->dt, n, m, nnz, panels_for(true), rpt, cit, vat, s);
+    rpt = cit = nullptr;
+    vat = nullptr;
+    if (rc == PK_OK) rc = rc2;
+  }
+  (void)hipFree(rp);
+  (void)hipFree(ci);
+  (void)hipFree(va);
+  (void)hipFree(rpt);
+  (void)hipFree(cit);
+  (void)hipFree(vat);
   (void)hipStreamDestroy(s);
   if (rc != PK_OK) {
     csr_destroy(op);
     return rc;
   }
-  op->lpr_n = choose_lpr(nnz, m);
-  op->lpr_t = choose_lpr(nnz, n);
-  {
-    // row-block partitions for the LDS-staged SpMV (both directions)
-    std::vector<int> hrp(static_cast<size_t>(m) + 1), hrpt(static_cast<size_t>(n) + 1);
-    bool ok = hipMemcpy(hrp.data(), op->rp, sizeof(int) * (m + 1), hipMemcpyDeviceToHost) ==
-                  hipSuccess &&
-              hipMemcpy(hrpt.data(), op->rpt, sizeof(int) * (n + 1), hipMemcpyDeviceToHost) ==
-                  hipSuccess;
-    ok = ok && upload(partition(hrp, m), op->pa) && upload(partition(hrpt, n), op->pt);
-    op->nrb = op->pa.slots();
-    op->nrbt = op->pt.slots();
-    if (!ok) {
-      csr_destroy(op);
-      return PK_ERR_HIP;
-    }
-  }
+  op->nrb = op->da.slots;
+  op->nrbt = op->dt.slots;
   *out = op;
   return PK_OK;
 }
 
 void csr_destroy(CsrOp* op) {
   if (!op) return;
-  (void)hipFree(op->rp);
-  (void)hipFree(op->ci);
-  (void)hipFree(op->va);
-  (void)hipFree(op->rpt);
-  (void)hipFree(op->cit);
-  (void)hipFree(op->vat);
-  for (CsrOp::Part* d : {&op->pa, &op->pt}) {
-    (void)hipFree(d->desc);
-    (void)hipFree(d->ldesc);
-    (void)hipFree(d->lsum);
-  }
+  free_dir(op->da);
+  free_dir(op->dt);
   op->magic = 0;
   delete op;
 }
 
 // y = op(A) x - coef * yprev with fused partial sums; x, y, yprev device
-// vectors of basis type T.  Returns nblk of the partials or < 0.
+// vectors of basis type T.  One launch per column panel (the running row
+// sums stay in FP64 between them).  Returns the partial count or < 0.
 template <typename T>
 int csr_apply(const CsrOp* op, hipStream_t s, char trans, const T* x, T* y, const T* yprev,
               Scal coef, double* part, Gate gate) {
   const bool tr = (trans == 't' || trans == 'T');
-  const int rows = tr ? op->n : op->m;
-  const int* rp = tr ? op->rpt : op->rp;
-  const int* ci = tr ? op->cit : op->ci;
-  const void* va = tr ? op->vat : op->va;
-  const int lpr = tr ? op->lpr_t : op->lpr_n;
-  const CsrOp::Part& P = tr ? op->pt : op->pa;
-  (void)rows;
-  if (op->val_prec == PK_F64)
-    return launch_spmv_rb<double, T>(s, P.ndesc, P.desc, P.nlong, P.ldesc, P.lsum, rp, ci,
-                                     static_cast<const double*>(va), x, y, yprev, coef, part, lpr,
-                                     gate);
-  return launch_spmv_rb<float, T>(s, P.ndesc, P.desc, P.nlong, P.ldesc, P.lsum, rp, ci,
-                                  static_cast<const float*>(va), x, y, yprev, coef, part, lpr,
-                                  gate);
+  const CsrOp::Dir& D = tr ? op->dt : op->da;
+  int nb = -1;
+  for (int p = 0; p < D.np; ++p) {
+    const CsrOp::Panel& P = D.pan[p];
+    const CsrOp::Part& Q = P.part;
+    const double* yin = p == 0 ? nullptr : D.yacc;
+    const int last = p == D.np - 1;
+    double* yout = last ? nullptr : D.yacc;
+    if (op->val_prec == PK_F64)
+      nb = launch_spmv_rb<double, T>(s, Q.ndesc, Q.desc, Q.nlong, Q.ldesc, Q.lsum, P.rp, P.ci,
+                                     static_cast<const double*>(P.va), x, y, yprev, coef, part,
+                                     P.g, yin, yout, last, gate);
+    else
+      nb = launch_spmv_rb<float, T>(s, Q.ndesc, Q.desc, Q.nlong, Q.ldesc, Q.lsum, P.rp, P.ci,
+                                    static_cast<const float*>(P.va), x, y, yprev, coef, part,
+                                    P.g, yin, yout, last, gate);
+    if (nb < 0) return nb;
+  }
+  return nb;
 }
 
 template int csr_apply<double>(const CsrOp*, hipStream_t, char, const double*, double*,

@@ -11,18 +11,14 @@ namespace pk {
 
 struct CsrOp {
   static constexpr unsigned kMagic = 0x504b4353u;  // "PKCS"
+  static constexpr int kMaxPanels = 16;
   unsigned magic = 0;
   int m = 0, n = 0;
   long long nnz = 0;
   int val_prec = PK_F64;
   size_t val_bytes = 8;
-  int *rp = nullptr, *ci = nullptr;   // A
-  void* va = nullptr;
-  int *rpt = nullptr, *cit = nullptr; // A^T (explicit)
-  void* vat = nullptr;
-  int lpr_n = 8, lpr_t = 8;           // lanes per row for A and A^T
-  // SpMV workgroup descriptors of A and A^T (row blocks and long-row
-  // chunks), long-row tables and their chunk partial sums
+  // SpMV workgroup descriptors (row blocks and long-row chunks), long-row
+  // table and its chunk partial sums, for one CSR array set
   struct Part {
     int4* desc = nullptr;
     int ndesc = 0;
@@ -30,8 +26,27 @@ struct CsrOp {
     int nlong = 0;
     double* lsum = nullptr;
     int slots() const { return ndesc + nlong; }
-  } pa, pt;
-  int nrb = 0, nrbt = 0;              // partial slots per direction
+  };
+  // One direction (A or the explicit A^T), stored as np column panels:
+  // panel p holds the entries with column in [p * width, (p + 1) * width)
+  // as its own CSR over all rows (global column indices, entries of a row in
+  // their original order).  np = 1 is the plain CSR.
+  struct Panel {
+    int* rp = nullptr;
+    int* ci = nullptr;
+    void* va = nullptr;
+    long long nnz = 0;
+    int g = 8;  // lanes per row in the LDS row reduction
+    Part part;
+  };
+  struct Dir {
+    int rows = 0, cols = 0;
+    int np = 1, width = 0;
+    Panel pan[kMaxPanels];
+    double* yacc = nullptr;  // FP64 running row sums between panels (np > 1)
+    int slots = 0;           // partial slots of the widest panel
+  } da, dt;
+  int nrb = 0, nrbt = 0;  // partial slots per direction
 };
 
 int csr_create(int m, int n, long long nnz, const int* rowptr, const int* colind,

@@ -11,26 +11,22 @@ namespace pk {
 // stream_blocks(L) is the fixed grid every streaming kernel over L rows uses.
 int stream_blocks(long long L);
 
-// y = op(A) x  - coef * yprev   (yprev may be null), CSR rows [0, rows).
-// Writes 2 partials per block: sum y^2, sum yprev.y  (of the stored values).
-// lpr = lanes per row (1..64, power of two).  Returns nblk used.
-template <typename VT, typename T>
-int launch_spmv(hipStream_t s, int rows, const int* rp, const int* ci, const VT* va,
-                const T* x, T* y, const T* yprev, Scal coef, double* part, int lpr,
-                Gate gate);
-
 // Row-block SpMV over nblk workgroup descriptors (row blocks of at most
 // kSpmvRows rows / kSpmvCap nonzeros, or chunks of kLongChunk nonzeros of a
 // long row) plus nlong long rows finished by a fix-up kernel; g lanes per row
 // in the LDS reduction.  y = op(A) x - coef * yprev with fused partials
-// (2 per slot, nblk + nlong slots).  Returns the slot count.
+// (2 per slot, nblk + nlong slots).  Column panels: yacc_in (FP64 row sums
+// of the earlier panels, null for the first), yacc_out (where a panel that
+// is not the last stores its running sums), last (1: apply the epilogue and
+// write the partials).  Returns the slot count.
 constexpr int kSpmvCap = 4096;
 constexpr int kSpmvRows = 256;
 constexpr int kLongChunk = 8192;
 template <typename VT, typename T>
 int launch_spmv_rb(hipStream_t s, int nblk, const int4* desc, int nlong, const int4* ldesc,
                    double* lsum, const int* rp, const int* ci, const VT* va, const T* x, T* y,
-                   const T* yprev, Scal coef, double* part, int g, Gate gate);
+                   const T* yprev, Scal coef, double* part, int g, const double* yacc_in,
+                   double* yacc_out, int last, Gate gate);
 
 // part (nval x nblk) -> out[v] = sum_b part[v*nblk+b] in index order.
 int launch_reduce(hipStream_t s, int nval, int nblk, const double* part, double* out,

@@ -1,10 +1,11 @@
 // Hot-path kernels of the Lanczos bidiagonalization engine, written for
 // gfx950 (CDNA4, wave64).
 //
-//   k_spmv          y = op(A) x - c * yprev over a CSR operator (A or its
-//                   explicit transpose), lanes-per-row sub-waves, fused with
-//                   the Lanczos recurrence update and the partial sums for
-//                   ||y||^2 and yprev.y (the norm and the ELR dot product).
+//   k_spmv_rb       y = op(A) x - c * yprev over a CSR operator (A or its
+//                   explicit transpose) in row blocks staged through LDS,
+//                   fused with the Lanczos recurrence update and the partial
+//                   sums for ||y||^2 and yprev.y (the norm and the ELR dot
+//                   product); column panels accumulate in FP64 across launches.
 //   k_gemvt         h = V(:,I)^T w for the iterated classical Gram-Schmidt
 //                   step; each workgroup owns a contiguous row slice, keeps
 //                   its slice of w in registers and streams column tiles of V.
@@ -23,75 +24,6 @@ constexpr int kRowsPerBlock = kBlock * kRPT;
 
 int stream_blocks(long long L) { return L > 0 ? cdiv(L, kRowsPerBlock) : 1; }
 
-// ---------------------------------------------------------------- SpMV ----
-template <typename VT, typename T, int LPR>
-__global__ __launch_bounds__(kBlock) void k_spmv(int rows, const int* __restrict__ rp,
-                                                 const int* __restrict__ ci,
-                                                 const VT* __restrict__ va,
-                                                 const T* __restrict__ x, T* __restrict__ y,
-                                                 const T* __restrict__ yprev, Scal coef,
-                                                 double* __restrict__ part, Gate gate) {
-  if (gate.off()) return;
-  __shared__ double lds[(kBlock / kWave) * 2];
-  const double c = coef.get();
-  const int lane = threadIdx.x & (LPR - 1);
-  const long long group = (static_cast<long long>(blockIdx.x) * kBlock + threadIdx.x) / LPR;
-  const long long ngroups = static_cast<long long>(gridDim.x) * kBlock / LPR;
-  const long long iters = (rows + ngroups - 1) / ngroups;  // uniform trip count
-  double s2 = 0.0, sd = 0.0;
-  for (long long it = 0; it < iters; ++it) {
-    const long long row = group + it * ngroups;
-    const bool valid = row < rows;
-    int p0 = 0, p1 = 0;
-    if (valid) {
-      p0 = rp[row];
-      p1 = rp[row + 1];
-    }
-    double acc = 0.0;
-    for (int p = p0 + lane; p < p1; p += LPR)
-      acc += static_cast<double>(va[p]) * static_cast<double>(x[ci[p]]);
-    acc = group_sum<LPR>(acc);
-    if (valid && lane == 0) {
-      const double yp = yprev ? static_cast<double>(yprev[row]) : 0.0;
-      const T yv = static_cast<T>(acc - c * yp);
-      y[row] = yv;
-      s2 += static_cast<double>(yv) * static_cast<double>(yv);
-      sd += yp * static_cast<double>(yv);
-    }
-  }
-  double v[2] = {s2, sd};
-  block_sum<2>(v, lds);
-  if (threadIdx.x == 0) {
-    part[blockIdx.x] = v[0];
-    part[gridDim.x + blockIdx.x] = v[1];
-  }
-}
-
-template <typename VT, typename T>
-int launch_spmv(hipStream_t s, int rows, const int* rp, const int* ci, const VT* va,
-                const T* x, T* y, const T* yprev, Scal coef, double* part, int lpr,
-                Gate gate) {
-  const int nblk = kMaxBlocks;
-#define PK_SPMV_CASE(L)                                                                    \
-  case L:                                                                                  \
-    hipLaunchKernelGGL((k_spmv<VT, T, L>), dim3(nblk), dim3(kBlock), 0, s, rows, rp, ci, va, \
-                       x, y, yprev, coef, part, gate);                                     \
-    break;
-  switch (lpr) {
-    PK_SPMV_CASE(1)
-    PK_SPMV_CASE(2)
-    PK_SPMV_CASE(4)
-    PK_SPMV_CASE(8)
-    PK_SPMV_CASE(16)
-    PK_SPMV_CASE(32)
-    PK_SPMV_CASE(64)
-    default:
-      return -1;
-  }
-#undef PK_SPMV_CASE
-  return hipGetLastError() == hipSuccess ? nblk : -1;
-}
-
 // ------------------------------------------------------ row-block SpMV ----
 // The rows are partitioned into workgroup descriptors {x, y, z, w}:
 //   * y >= 0: a row block, rows [x, y) holding at most kSpmvCap nonzeros
@@ -104,6 +36,10 @@ int launch_spmv(hipStream_t s, int rows, const int* rp, const int* ci, const VT*
 //     dot product goes to lsum[-(y+1)]; k_spmv_longfix finishes the row.
 // Partials: part[b] = sum y^2, part[nbt + b] = sum yprev.y (zero for
 // chunks); the long rows' contributions follow the blocks' slots.
+// Column panels: an operator split into column panels runs one launch per
+// panel; the row sums of the panels before the last one go to the FP64
+// accumulator yacc (yacc_in = null for the first panel), and only the last
+// launch (last = 1) adds its own sum and applies the epilogue and partials.
 template <typename VT, typename T, int G>
 __global__ __launch_bounds__(kBlock) void k_spmv_rb(const int4* __restrict__ desc,
                                                     const int* __restrict__ rp,
@@ -112,7 +48,10 @@ __global__ __launch_bounds__(kBlock) void k_spmv_rb(const int4* __restrict__ des
                                                     const T* __restrict__ x, T* __restrict__ y,
                                                     const T* __restrict__ yprev, Scal coef,
                                                     double* __restrict__ part, int nbt,
-                                                    double* __restrict__ lsum, Gate gate) {
+                                                    double* __restrict__ lsum,
+                                                    const double* __restrict__ yacc_in,
+                                                    double* __restrict__ yacc_out, int last,
+                                                    Gate gate) {
   if (gate.off()) return;
   __shared__ double prod[kSpmvCap];
   __shared__ double lds[(kBlock / kWave) * 2];
@@ -124,8 +63,10 @@ __global__ __launch_bounds__(kBlock) void k_spmv_rb(const int4* __restrict__ des
     block_sum<1>(acc, lds);
     if (threadIdx.x == 0) {
       lsum[-(d.y + 1)] = acc[0];
-      part[blockIdx.x] = 0.0;
-      part[nbt + blockIdx.x] = 0.0;
+      if (last) {
+        part[blockIdx.x] = 0.0;
+        part[nbt + blockIdx.x] = 0.0;
+      }
     }
     return;
   }
@@ -170,13 +111,19 @@ __global__ __launch_bounds__(kBlock) void k_spmv_rb(const int4* __restrict__ des
     for (int q = a + lane; q < b; q += G) acc += prod[q];
     acc = group_sum<G>(acc);
     if (valid && lane == 0) {
-      const double yp = yprev ? static_cast<double>(yprev[r]) : 0.0;
-      const T yv = static_cast<T>(acc - c * yp);
-      y[r] = yv;
-      s2 += static_cast<double>(yv) * static_cast<double>(yv);
-      sd += yp * static_cast<double>(yv);
+      if (yacc_in) acc = yacc_in[r] + acc;
+      if (!last) {
+        yacc_out[r] = acc;
+      } else {
+        const double yp = yprev ? static_cast<double>(yprev[r]) : 0.0;
+        const T yv = static_cast<T>(acc - c * yp);
+        y[r] = yv;
+        s2 += static_cast<double>(yv) * static_cast<double>(yv);
+        sd += yp * static_cast<double>(yv);
+      }
     }
   }
+  if (!last) return;
   double v2[2] = {s2, sd};
   block_sum<2>(v2, lds);
   if (threadIdx.x == 0) {
@@ -194,13 +141,21 @@ __global__ __launch_bounds__(kBlock) void k_spmv_longfix(int nlong, const int4* 
                                                          T* __restrict__ y,
                                                          const T* __restrict__ yprev, Scal coef,
                                                          double* __restrict__ part, int nbt,
-                                                         int nbmain, Gate gate) {
+                                                         int nbmain,
+                                                         const double* __restrict__ yacc_in,
+                                                         double* __restrict__ yacc_out, int last,
+                                                         Gate gate) {
   if (gate.off()) return;
   const int i = blockIdx.x * kBlock + threadIdx.x;
   if (i >= nlong) return;
   const int4 d = ld[i];
   double acc = 0.0;
   for (int q = 0; q < d.z; ++q) acc += lsum[d.y + q];
+  if (yacc_in) acc = yacc_in[d.x] + acc;
+  if (!last) {
+    yacc_out[d.x] = acc;
+    return;
+  }
   const double yp = yprev ? static_cast<double>(yprev[d.x]) : 0.0;
   const T yv = static_cast<T>(acc - coef.get() * yp);
   y[d.x] = yv;
@@ -211,13 +166,14 @@ __global__ __launch_bounds__(kBlock) void k_spmv_longfix(int nlong, const int4* 
 template <typename VT, typename T>
 int launch_spmv_rb(hipStream_t s, int nblk, const int4* desc, int nlong, const int4* ldesc,
                    double* lsum, const int* rp, const int* ci, const VT* va, const T* x, T* y,
-                   const T* yprev, Scal coef, double* part, int g, Gate gate) {
+                   const T* yprev, Scal coef, double* part, int g, const double* yacc_in,
+                   double* yacc_out, int last, Gate gate) {
   const int nbt = nblk + nlong;
   if (nblk > 0) {
 #define PK_RB_CASE(G)                                                                          \
   case G:                                                                                      \
     hipLaunchKernelGGL((k_spmv_rb<VT, T, G>), dim3(nblk), dim3(kBlock), 0, s, desc, rp, ci, va, \
-                       x, y, yprev, coef, part, nbt, lsum, gate);                              \
+                       x, y, yprev, coef, part, nbt, lsum, yacc_in, yacc_out, last, gate);     \
     break;
     switch (g) {
       PK_RB_CASE(2)
@@ -233,7 +189,8 @@ int launch_spmv_rb(hipStream_t s, int nblk, const int4* desc, int nlong, const i
   }
   if (nlong > 0)
     hipLaunchKernelGGL(k_spmv_longfix<T>, dim3(cdiv(nlong, kBlock)), dim3(kBlock), 0, s, nlong,
-                       ldesc, lsum, y, yprev, coef, part, nbt, nblk, gate);
+                       ldesc, lsum, y, yprev, coef, part, nbt, nblk, yacc_in, yacc_out, last,
+                       gate);
   return hipGetLastError() == hipSuccess ? nbt : -1;
 }
 
@@ -714,25 +671,14 @@ PK_INST_T(float)
 #define PK_INST_RB(VT, T)                                                                   \
   template int launch_spmv_rb<VT, T>(hipStream_t, int, const int4*, int, const int4*, double*, \
                                      const int*, const int*, const VT*, const T*, T*,         \
-                                     const T*, Scal, double*, int, Gate);
+                                     const T*, Scal, double*, int, const double*, double*,   \
+                                     int, Gate);
 PK_INST_RB(double, double)
 PK_INST_RB(float, float)
 PK_INST_RB(double, float)
 PK_INST_RB(float, double)
 #undef PK_INST_RB
 
-template int launch_spmv<double, double>(hipStream_t, int, const int*, const int*,
-                                         const double*, const double*, double*, const double*,
-                                         Scal, double*, int, Gate);
-template int launch_spmv<float, float>(hipStream_t, int, const int*, const int*, const float*,
-                                       const float*, float*, const float*, Scal, double*, int,
-                                       Gate);
-template int launch_spmv<double, float>(hipStream_t, int, const int*, const int*,
-                                        const double*, const float*, float*, const float*,
-                                        Scal, double*, int, Gate);
-template int launch_spmv<float, double>(hipStream_t, int, const int*, const int*,
-                                        const float*, const double*, double*, const double*,
-                                        Scal, double*, int, Gate);
 template int launch_convert<double, float>(hipStream_t, long long, const double*, float*);
 template int launch_convert<float, double>(hipStream_t, long long, const float*, double*);
 template int launch_convert<double, double>(hipStream_t, long long, const double*, double*);

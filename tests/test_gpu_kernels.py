@@ -116,3 +116,56 @@ def test_spmv_powerlaw_long_rows():
     np.testing.assert_allclose(op.spmv(x, "n"), want_n, rtol=0, atol=1e-10 * np.abs(want_n).max())
     np.testing.assert_allclose(op.spmv(y, "t"), want_t, rtol=0, atol=1e-10 * np.abs(want_t).max())
     assert np.array_equal(op.spmv(y, "t"), op.spmv(y, "t"))
+
+
+@pytest.fixture
+def panels(monkeypatch):
+    """Operators created inside the test use the given column panels."""
+    def set_(spec):
+        monkeypatch.setenv("PK_SPMV_PANELS", spec)
+    yield set_
+
+
+@pytest.mark.parametrize("spec", ["2", "4,8", "16"])
+def test_spmv_column_panels(panels, spec):
+    """Column-panel layout: one launch per panel, FP64 running row sums
+    between launches; same products as the plain CSR on edge shapes (a
+    long row split into chunks, empty rows, a dense column) and on the
+    power-law matrix, and bit-identical run to run."""
+    panels(spec)
+    rng = np.random.default_rng(11)
+    m, n = 300, 9500
+    D = np.zeros((m, n))
+    for i in range(0, m, 3):
+        cols = rng.choice(n, rng.integers(1, 20), replace=False)
+        D[i, cols] = rng.standard_normal(cols.size)
+    D[7, :9000] = rng.standard_normal(9000)
+    D[:, 42] = rng.standard_normal(m)
+    op = api.CsrOperator(_csr_from_dense(D))
+    x, y = rng.standard_normal(n), rng.standard_normal(m)
+    np.testing.assert_allclose(op.spmv(x, "n"), D @ x, rtol=0, atol=1e-11)
+    np.testing.assert_allclose(op.spmv(y, "t"), D.T @ y, rtol=0, atol=1e-11)
+    A = synth.powerlaw_bipartite(400_000, 200_000, seed=3)
+    op = api.CsrOperator(A)
+    x, y = rng.standard_normal(A.n), rng.standard_normal(A.m)
+    want_n, want_t = A.matvec(x), A.transpose().matvec(y)
+    got_t = op.spmv(y, "t")
+    np.testing.assert_allclose(op.spmv(x, "n"), want_n, rtol=0, atol=1e-10 * np.abs(want_n).max())
+    np.testing.assert_allclose(got_t, want_t, rtol=0, atol=1e-10 * np.abs(want_t).max())
+    assert np.array_equal(got_t, op.spmv(y, "t"))
+    # FP32 operator (FP64 row sums between panels, FP32 output)
+    op32 = api.CsrOperator(A.astype(np.float32), prec=api.PK_F32)
+    got32 = op32.spmv(x.astype(np.float32), "n")
+    np.testing.assert_allclose(got32, want_n, rtol=0, atol=1e-5 * np.abs(want_n).max())
+
+
+def test_panels_same_triplets(panels, small_case):
+    """DLANSVD through a panelled operator gives the plain operator's
+    singular values (summation grouping differs: not bit-identical)."""
+    A, u0, Dm, s = small_case
+    r0 = api.lansvd(api.CsrOperator(A), A.m, A.n, 10, 300, u0=u0, tol=1e-12)
+    panels("4")
+    r1 = api.lansvd(api.CsrOperator(A), A.m, A.n, 10, 300, u0=u0, tol=1e-12)
+    assert r0.k == r1.k == 10
+    np.testing.assert_allclose(r1.sigma, r0.sigma, rtol=1e-12)
+    np.testing.assert_allclose(r1.sigma, s[:10], rtol=1e-12)
