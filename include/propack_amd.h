@@ -122,6 +122,14 @@ int pk_lbd_stats(pk_lbd_t eng, long long* out);
 int pk_lbd_reset_stats(pk_lbd_t eng);
 int pk_lbd_sync(pk_lbd_t eng);
 int pk_lbd_shape(pk_lbd_t eng, int* m, int* n, int* kmax, int* prec);
+/* Rounding model of the Lanczos thresholds (no reference counterpart; the
+ * reference's are double/dlanbpro.F:148-149): epsn = leff * eps and
+ * epsn2 = sqrt(leff) * eps with leff = max(m, n) (model 0, the reference's,
+ * default) or leff = 4 (model 1: FP32 storage with FP64 accumulation, where
+ * only the storage rounding enters; the FP32-basis form uses it).
+ * pk_lbd_eps returns eps of the engine's precision and leff. */
+int pk_lbd_set_rounding(pk_lbd_t eng, int model);
+int pk_lbd_eps(pk_lbd_t eng, double* eps, double* leff);
 /* Step execution (no reference counterpart): mode 0 runs the Lanczos steps
  * as device-resident kernel chains whose scalar control (the omega
  * recurrence, ELR and reorthogonalization decisions) runs in control

@@ -1,0 +1,266 @@
+// Per-workgroup bodies of the streaming kernels.  The single-problem
+// kernels (pk_kernels.hip) and the batched kernels (pk_batch.hip) both call
+// these, so one problem gives bit-identical results whether it runs alone
+// or inside a batch: the same rows per workgroup, the same summation order,
+// the same partial slots.
+#pragma once
+
+#include "pk_kernels.h"
+
+namespace pk {
+
+// Rows per thread of the dense streaming kernels; workgroup b of a vector of
+// length L owns the contiguous slice [b * kRowsPerBlock, (b + 1) * kRowsPerBlock).
+constexpr int kRPT = 4;
+constexpr int kRowsPerBlock = kBlock * kRPT;
+// Column tile width of the Gram-Schmidt projection: kCT accumulators per
+// thread, kCT column loads in flight per row held.
+constexpr int kCT = 8;
+
+// ------------------------------------------------------ row-block SpMV ----
+// Descriptor d = {x, y, z, w}:
+//   * y >= 0: a row block, rows [x, y) holding at most kSpmvCap nonzeros
+//     and kSpmvRows rows.  All 256 threads turn its nonzeros into products
+//     val * x[col] in LDS (prod) -- index and value streams read fully
+//     coalesced, kUnroll independent gathers in flight per thread -- then
+//     groups of G lanes sum each row from LDS (consecutive, conflict-free
+//     reads; shuffle reduction) in a fixed order and apply the epilogue.
+//   * y < 0: chunk -(y+1) of the long row x, nonzeros [z, w): its partial
+//     dot product goes to lsum[-(y+1)]; the long-row fix-up finishes the row.
+// Epilogue (last = 1): y = T(sum - c * yprev), *q_s2 = sum y^2,
+// *q_sd = sum yprev.y over the block (zero for chunks).  Column panels
+// (last = 0) store the FP64 running row sums in yacc_out instead; a panel
+// after the first adds yacc_in.
+template <typename VT, typename T, int G>
+__device__ __forceinline__ void spmv_rb_body(const int4 d, const int* __restrict__ rp,
+                                             const int* __restrict__ ci,
+                                             const VT* __restrict__ va, const T* __restrict__ x,
+                                             T* __restrict__ y, const T* __restrict__ yprev,
+                                             double c, double* q_s2, double* q_sd,
+                                             double* __restrict__ lsum,
+                                             const double* __restrict__ yacc_in,
+                                             double* __restrict__ yacc_out, int last,
+                                             double* prod, double* lds) {
+  if (d.y < 0) {
+    double acc[1] = {0.0};
+    for (int p = d.z + static_cast<int>(threadIdx.x); p < d.w; p += kBlock)
+      acc[0] += static_cast<double>(va[p]) * static_cast<double>(x[ci[p]]);
+    block_sum<1>(acc, lds);
+    if (threadIdx.x == 0) {
+      lsum[-(d.y + 1)] = acc[0];
+      if (last) {
+        *q_s2 = 0.0;
+        *q_sd = 0.0;
+      }
+    }
+    return;
+  }
+  const int r0 = d.x, r1 = d.y;
+  const int p0 = rp[r0];
+  const int nz = rp[r1] - p0;
+  double s2 = 0.0, sd = 0.0;
+  constexpr int kUnroll = 8;
+  for (int base = 0; base < nz; base += kBlock * kUnroll) {
+    int idx[kUnroll];
+    double v[kUnroll], xv[kUnroll];
+#pragma unroll
+    for (int u = 0; u < kUnroll; ++u) {
+      const int t = base + u * kBlock + static_cast<int>(threadIdx.x);
+      idx[u] = t < nz ? ci[p0 + t] : -1;
+      v[u] = t < nz ? static_cast<double>(va[p0 + t]) : 0.0;
+    }
+#pragma unroll
+    for (int u = 0; u < kUnroll; ++u) xv[u] = idx[u] >= 0 ? static_cast<double>(x[idx[u]]) : 0.0;
+#pragma unroll
+    for (int u = 0; u < kUnroll; ++u) {
+      const int t = base + u * kBlock + static_cast<int>(threadIdx.x);
+      if (t < nz) prod[t] = v[u] * xv[u];
+    }
+  }
+  __syncthreads();
+  const int lane = threadIdx.x & (G - 1);
+  const int group = threadIdx.x / G;
+  constexpr int ngroups = kBlock / G;
+  const int nrows = r1 - r0;
+  const int iters = (nrows + ngroups - 1) / ngroups;  // uniform across the block
+  for (int it = 0; it < iters; ++it) {
+    const int r = r0 + group + it * ngroups;
+    const bool valid = r < r1;
+    int a = 0, b = 0;
+    if (valid) {
+      a = rp[r] - p0;
+      b = rp[r + 1] - p0;
+    }
+    double acc = 0.0;
+    for (int q = a + lane; q < b; q += G) acc += prod[q];
+    acc = group_sum<G>(acc);
+    if (valid && lane == 0) {
+      if (yacc_in) acc = yacc_in[r] + acc;
+      if (!last) {
+        yacc_out[r] = acc;
+      } else {
+        const double yp = yprev ? static_cast<double>(yprev[r]) : 0.0;
+        const T yv = static_cast<T>(acc - c * yp);
+        y[r] = yv;
+        s2 += static_cast<double>(yv) * static_cast<double>(yv);
+        sd += yp * static_cast<double>(yv);
+      }
+    }
+  }
+  if (!last) return;
+  double v2[2] = {s2, sd};
+  block_sum<2>(v2, lds);
+  if (threadIdx.x == 0) {
+    *q_s2 = v2[0];
+    *q_sd = v2[1];
+  }
+}
+
+// Long row ld = {row, first chunk, number of chunks, -}: one thread sums the
+// chunks in order and applies the epilogue (or stores the running sum).
+template <typename T>
+__device__ __forceinline__ void spmv_longfix_body(const int4 d, const double* __restrict__ lsum,
+                                                  T* __restrict__ y,
+                                                  const T* __restrict__ yprev, double c,
+                                                  double* q_s2, double* q_sd,
+                                                  const double* __restrict__ yacc_in,
+                                                  double* __restrict__ yacc_out, int last) {
+  double acc = 0.0;
+  for (int q = 0; q < d.z; ++q) acc += lsum[d.y + q];
+  if (yacc_in) acc = yacc_in[d.x] + acc;
+  if (!last) {
+    yacc_out[d.x] = acc;
+    return;
+  }
+  const double yp = yprev ? static_cast<double>(yprev[d.x]) : 0.0;
+  const T yv = static_cast<T>(acc - c * yp);
+  y[d.x] = yv;
+  *q_s2 = static_cast<double>(yv) * static_cast<double>(yv);
+  *q_sd = yp * static_cast<double>(yv);
+}
+
+// ----------------------------------------------- Gram-Schmidt projection ----
+// Slice b of h = V(:, scol[0:ncols])^T w: the slice of w stays in registers,
+// kCT-column tiles of V stream through (wave shuffles + LDS per tile).
+// Partial of column c goes to part[c * stride + slot].  scol in LDS.
+template <typename T>
+__device__ __forceinline__ void gemvt_body(long long b, long long L, const T* __restrict__ V,
+                                           long long ldv, const int* scol, int ncols,
+                                           const T* __restrict__ w, double* __restrict__ part,
+                                           long long stride, long long slot,
+                                           double (*lds)[(kBlock / kWave) * kCT]) {
+  const long long base = b * kRowsPerBlock + threadIdx.x;
+  double wr[kRPT];
+#pragma unroll
+  for (int r = 0; r < kRPT; ++r) {
+    const long long row = base + static_cast<long long>(r) * kBlock;
+    wr[r] = row < L ? static_cast<double>(w[row]) : 0.0;
+  }
+  const int lane = threadIdx.x % kWave, wid = threadIdx.x / kWave;
+  int buf = 0;
+  for (int c0 = 0; c0 < ncols; c0 += kCT) {
+    double acc[kCT];
+#pragma unroll
+    for (int c = 0; c < kCT; ++c) acc[c] = 0.0;
+#pragma unroll
+    for (int r = 0; r < kRPT; ++r) {
+      const long long row = base + static_cast<long long>(r) * kBlock;
+      if (row < L) {
+#pragma unroll
+        for (int c = 0; c < kCT; ++c) {
+          if (c0 + c < ncols)
+            acc[c] += static_cast<double>(V[static_cast<long long>(scol[c0 + c]) * ldv + row]) *
+                      wr[r];
+        }
+      }
+    }
+#pragma unroll
+    for (int c = 0; c < kCT; ++c) acc[c] = wave_sum(acc[c]);
+    if (lane == 0) {
+#pragma unroll
+      for (int c = 0; c < kCT; ++c) lds[buf][wid * kCT + c] = acc[c];
+    }
+    __syncthreads();
+    if (threadIdx.x < kCT && c0 + threadIdx.x < ncols) {
+      double sum = 0.0;
+      for (int ww = 0; ww < kBlock / kWave; ++ww) sum += lds[buf][ww * kCT + threadIdx.x];
+      part[static_cast<long long>(c0 + threadIdx.x) * stride + slot] = sum;
+    }
+    buf ^= 1;  // the other buffer is free: its readers passed the barrier above
+  }
+}
+
+// Slice b of w -= V(:, scol[0:ncols]) sh (scol, sh in LDS); *q = ||w slice||^2.
+template <typename T>
+__device__ __forceinline__ void gemv_update_body(long long b, long long L,
+                                                 const T* __restrict__ V, long long ldv,
+                                                 const int* scol, const double* sh, int ncols,
+                                                 T* __restrict__ w, double* q, double* lds) {
+  const long long base = b * kRowsPerBlock + threadIdx.x;
+  double acc[kRPT];
+#pragma unroll
+  for (int r = 0; r < kRPT; ++r) {
+    const long long row = base + static_cast<long long>(r) * kBlock;
+    acc[r] = row < L ? static_cast<double>(w[row]) : 0.0;
+  }
+  for (int c = 0; c < ncols; ++c) {
+    const T* vc = V + static_cast<long long>(scol[c]) * ldv;
+    const double hc = sh[c];
+#pragma unroll
+    for (int r = 0; r < kRPT; ++r) {
+      const long long row = base + static_cast<long long>(r) * kBlock;
+      if (row < L) acc[r] -= static_cast<double>(vc[row]) * hc;
+    }
+  }
+  double v[1] = {0.0};
+#pragma unroll
+  for (int r = 0; r < kRPT; ++r) {
+    const long long row = base + static_cast<long long>(r) * kBlock;
+    if (row < L) {
+      const T o = static_cast<T>(acc[r]);
+      w[row] = o;
+      v[0] += static_cast<double>(o) * static_cast<double>(o);
+    }
+  }
+  block_sum<1>(v, lds);
+  if (threadIdx.x == 0) *q = v[0];
+}
+
+// ------------------------------------------------------------- level 1 ----
+template <typename T>
+__device__ __forceinline__ void scale_body(long long b, long long L, T* __restrict__ x,
+                                           double av) {
+  const long long base = b * kRowsPerBlock + threadIdx.x;
+#pragma unroll
+  for (int r = 0; r < kRPT; ++r) {
+    const long long row = base + static_cast<long long>(r) * kBlock;
+    if (row < L) x[row] = static_cast<T>(av * static_cast<double>(x[row]));
+  }
+}
+
+// Slice b of w = w - cv * v; *q0 = ||w||^2, *q1 = v.w over the slice.
+template <typename T>
+__device__ __forceinline__ void axpy_norm_dot_body(long long b, long long L, T* __restrict__ w,
+                                                   const T* __restrict__ v, double cv,
+                                                   double* q0, double* q1, double* lds) {
+  const long long base = b * kRowsPerBlock + threadIdx.x;
+  double acc[2] = {0.0, 0.0};
+#pragma unroll
+  for (int r = 0; r < kRPT; ++r) {
+    const long long row = base + static_cast<long long>(r) * kBlock;
+    if (row < L) {
+      const double vv = static_cast<double>(v[row]);
+      const T o = static_cast<T>(static_cast<double>(w[row]) - cv * vv);
+      w[row] = o;
+      acc[0] += static_cast<double>(o) * static_cast<double>(o);
+      acc[1] += vv * static_cast<double>(o);
+    }
+  }
+  block_sum<2>(acc, lds);
+  if (threadIdx.x == 0) {
+    *q0 = acc[0];
+    *q1 = acc[1];
+  }
+}
+
+}  // namespace pk

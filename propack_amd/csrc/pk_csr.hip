@@ -209,18 +209,27 @@ __global__ void k_gather_panel(long long cnt, const int* __restrict__ perm,
   }
 }
 
-// Column panels of an operator direction: PK_SPMV_PANELS ("p" for both
-// directions or "pA,pAt"), else one panel.  Powers of two up to
-// CsrOp::kMaxPanels.
-int panels_for(bool transposed) {
-  const char* e = std::getenv("PK_SPMV_PANELS");
-  if (!e || !*e) return 1;
-  int a = 1, b = 1;
-  const int got = std::sscanf(e, "%d,%d", &a, &b);
-  int p = got == 2 && transposed ? b : a;
-  if (got < 1) return 1;
+// Column panels of an operator direction (rows x cols, nnz): the smallest
+// power of two that cuts the gathered vector into slices of at most
+// kPanelSliceBytes (2 MiB: a slice plus the streamed matrix data fits one
+// XCD's 4 MiB L2, so the gathers hit L2 instead of the Infinity Cache), up
+// to 8; one panel when rows are short (mean below 8 nonzeros: the FP64
+// running row sums would cost more traffic than the gathers save).
+// PK_SPMV_PANELS ("p" for both directions or "pA,pAt") overrides.
+constexpr double kPanelSliceBytes = 2.0 * 1024 * 1024;
+
+int panels_for(bool transposed, int rows, int cols, long long nnz, size_t vbytes) {
+  int want = 1;
+  if (const char* e = std::getenv("PK_SPMV_PANELS"); e && *e) {
+    int a = 1, b = 1;
+    const int got = std::sscanf(e, "%d,%d", &a, &b);
+    if (got >= 1) want = got == 2 && transposed ? b : a;
+  } else if (rows > 0 && static_cast<double>(nnz) / rows >= 8.0) {
+    const double xbytes = static_cast<double>(cols) * static_cast<double>(vbytes);
+    while (want < 8 && xbytes / want > kPanelSliceBytes) want *= 2;
+  }
   int q = 1;
-  while (q * 2 <= p && q * 2 <= CsrOp::kMaxPanels) q *= 2;
+  while (q * 2 <= want && q * 2 <= CsrOp::kMaxPanels) q *= 2;
   return q;
 }
 
@@ -389,14 +398,15 @@ int csr_create(int m, int n, long long nnz, const int* rowptr, const int* colind
              : build_transpose<float>(m, n, nnz, rp, ci, va, &rpt, &cit, &vat, s);
   if (rc == PK_OK) {
     // build_dir owns the arrays from here on
-    rc = val_prec == PK_F64
-             ? build_dir<double>(op->da, m, n, nnz, panels_for(false), rp, ci, va, s)
-             : build_dir<float>(op->da, m, n, nnz, panels_for(false), rp, ci, va, s);
+    const int pa = panels_for(false, m, n, nnz, op->val_bytes);
+    const int pt = panels_for(true, n, m, nnz, op->val_bytes);
+    rc = val_prec == PK_F64 ? build_dir<double>(op->da, m, n, nnz, pa, rp, ci, va, s)
+                            : build_dir<float>(op->da, m, n, nnz, pa, rp, ci, va, s);
     rp = ci = nullptr;
     va = nullptr;
     const int rc2 = val_prec == PK_F64
-                        ? build_dir<double>(op->dt, n, m, nnz, panels_for(true), rpt, cit, vat, s)
-                        : build_dir<float>(op->dt, n, m, nnz, panels_for(true), rpt, cit, vat, s);
+                        ? build_dir<double>(op->dt, n, m, nnz, pt, rpt, cit, vat, s)
+                        : build_dir<float>(op->dt, n, m, nnz, pt, rpt, cit, vat, s);
     rpt = cit = nullptr;
     vat = nullptr;
     if (rc == PK_OK) rc = rc2;

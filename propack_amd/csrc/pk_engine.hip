@@ -164,6 +164,11 @@ struct Impl {
            (with_prev ? rows * sb : 0.0);
   }
 
+  // Effective vector length of the rounding model (see Engine::rounding).
+  double eff_length() const {
+    return e->rounding ? 4.0 : static_cast<double>(std::max(e->m, e->n));
+  }
+
   // ||x|| of a device vector (pdnrm2).
   int nrm2(const T* x, long long L, double* out) {
     if (L <= 0) {
@@ -630,8 +635,9 @@ struct Impl {
     const int m = e->m, n = e->n;
     int k = *kp, ierr = 0;
     const double eps = Mach<T>::eps;
-    const double epsn = static_cast<double>(std::max(m, n)) * eps;
-    const double epsn2 = std::sqrt(static_cast<double>(std::max(m, n))) * eps;
+    const double leff = eff_length();
+    const double epsn = leff * eps;
+    const double epsn2 = std::sqrt(leff) * eps;
     double delta, eta, anorm, anormest = 0, s = 0, a1, b1, amax = 0, alpha, beta;
     bool full_reorth, force_reorth;
     int j0;
@@ -1166,6 +1172,20 @@ int pk_lbd_shape(pk_lbd_t h, int* m, int* n, int* kmax, int* prec) {
   if (n) *n = e->n;
   if (kmax) *kmax = e->kmax;
   if (prec) *prec = e->prec;
+  return PK_OK;
+}
+
+int pk_lbd_set_rounding(pk_lbd_t h, int model) {
+  PK_ENGINE(h);
+  if (model != 0 && model != 1) return PK_ERR_ARG;
+  e->rounding = model;
+  return PK_OK;
+}
+
+int pk_lbd_eps(pk_lbd_t h, double* eps, double* leff) {
+  PK_ENGINE(h);
+  if (eps) *eps = e->prec == PK_F64 ? 0x1p-53 : 0x1p-24;
+  if (leff) *leff = e->rounding ? 4.0 : static_cast<double>(std::max(e->m, e->n));
   return PK_OK;
 }
 

@@ -114,6 +114,12 @@ struct Engine {
   LbdCtl* hctl = nullptr;            // its pinned host copy
   int mode = 0;                      // 0: device-resident steps, 1: host executor
   int dev_tries = 2;                 // CGS passes per side held by a device chain
+  // Rounding model of the thresholds epsn = leff * eps, epsn2 = sqrt(leff) * eps:
+  // 0 = the reference's (leff = max(m, n): accumulation in the storage
+  // precision); 1 = storage rounding only (leff = 4: FP32 storage with FP64
+  // accumulation, where max(m, n) * 2^-24 would exceed the Lanczos residuals
+  // of large problems and flag false invariant subspaces).
+  int rounding = 0;
   Counters cnt;
   Profile prof;
   int last_error = 0;

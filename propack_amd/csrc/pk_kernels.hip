@@ -13,33 +13,16 @@
 //   k_axpy_norm_dot, k_scale, k_norm   level-1 updates with fused reductions.
 //   k_reduce        second reduction stage (fixed order => deterministic).
 //   k_gemm_tall     C(:,1:n) = A(:,1:k) B for restarts / Ritz vectors (spare buffer).
-#include "pk_kernels.h"
+#include "pk_bodies.h"
 
 namespace pk {
-
-// Rows per thread of the dense streaming kernels; each workgroup owns the
-// contiguous slice [b * kRowsPerBlock, (b + 1) * kRowsPerBlock).
-constexpr int kRPT = 4;
-constexpr int kRowsPerBlock = kBlock * kRPT;
 
 int stream_blocks(long long L) { return L > 0 ? cdiv(L, kRowsPerBlock) : 1; }
 
 // ------------------------------------------------------ row-block SpMV ----
-// The rows are partitioned into workgroup descriptors {x, y, z, w}:
-//   * y >= 0: a row block, rows [x, y) holding at most kSpmvCap nonzeros
-//     and kSpmvRows rows.  All 256 threads turn its nonzeros into products
-//     val * x[col] in LDS -- index and value streams read fully coalesced,
-//     kUnroll independent gathers in flight per thread -- then groups of G
-//     lanes sum each row from LDS (consecutive, conflict-free reads;
-//     shuffle reduction) in a fixed order and apply the fused epilogue.
-//   * y < 0: chunk -(y+1) of the long row x, nonzeros [z, w): its partial
-//     dot product goes to lsum[-(y+1)]; k_spmv_longfix finishes the row.
-// Partials: part[b] = sum y^2, part[nbt + b] = sum yprev.y (zero for
-// chunks); the long rows' contributions follow the blocks' slots.
-// Column panels: an operator split into column panels runs one launch per
-// panel; the row sums of the panels before the last one go to the FP64
-// accumulator yacc (yacc_in = null for the first panel), and only the last
-// launch (last = 1) adds its own sum and applies the epilogue and partials.
+// One workgroup per descriptor (row block or long-row chunk, see
+// spmv_rb_body); partial slots b and nbt + b.  The long rows' two partials
+// follow the blocks' slots (nbmain + i).
 template <typename VT, typename T, int G>
 __global__ __launch_bounds__(kBlock) void k_spmv_rb(const int4* __restrict__ desc,
                                                     const int* __restrict__ rp,
@@ -55,86 +38,11 @@ __global__ __launch_bounds__(kBlock) void k_spmv_rb(const int4* __restrict__ des
   if (gate.off()) return;
   __shared__ double prod[kSpmvCap];
   __shared__ double lds[(kBlock / kWave) * 2];
-  const int4 d = desc[blockIdx.x];
-  if (d.y < 0) {
-    double acc[1] = {0.0};
-    for (int p = d.z + static_cast<int>(threadIdx.x); p < d.w; p += kBlock)
-      acc[0] += static_cast<double>(va[p]) * static_cast<double>(x[ci[p]]);
-    block_sum<1>(acc, lds);
-    if (threadIdx.x == 0) {
-      lsum[-(d.y + 1)] = acc[0];
-      if (last) {
-        part[blockIdx.x] = 0.0;
-        part[nbt + blockIdx.x] = 0.0;
-      }
-    }
-    return;
-  }
-  const double c = coef.get();
-  const int r0 = d.x, r1 = d.y;
-  const int p0 = rp[r0];
-  const int nz = rp[r1] - p0;
-  double s2 = 0.0, sd = 0.0;
-  constexpr int kUnroll = 8;
-  for (int base = 0; base < nz; base += kBlock * kUnroll) {
-    int idx[kUnroll];
-    double v[kUnroll], xv[kUnroll];
-#pragma unroll
-    for (int u = 0; u < kUnroll; ++u) {
-      const int t = base + u * kBlock + static_cast<int>(threadIdx.x);
-      idx[u] = t < nz ? ci[p0 + t] : -1;
-      v[u] = t < nz ? static_cast<double>(va[p0 + t]) : 0.0;
-    }
-#pragma unroll
-    for (int u = 0; u < kUnroll; ++u) xv[u] = idx[u] >= 0 ? static_cast<double>(x[idx[u]]) : 0.0;
-#pragma unroll
-    for (int u = 0; u < kUnroll; ++u) {
-      const int t = base + u * kBlock + static_cast<int>(threadIdx.x);
-      if (t < nz) prod[t] = v[u] * xv[u];
-    }
-  }
-  __syncthreads();
-  const int lane = threadIdx.x & (G - 1);
-  const int group = threadIdx.x / G;
-  constexpr int ngroups = kBlock / G;
-  const int nrows = r1 - r0;
-  const int iters = (nrows + ngroups - 1) / ngroups;  // uniform across the block
-  for (int it = 0; it < iters; ++it) {
-    const int r = r0 + group + it * ngroups;
-    const bool valid = r < r1;
-    int a = 0, b = 0;
-    if (valid) {
-      a = rp[r] - p0;
-      b = rp[r + 1] - p0;
-    }
-    double acc = 0.0;
-    for (int q = a + lane; q < b; q += G) acc += prod[q];
-    acc = group_sum<G>(acc);
-    if (valid && lane == 0) {
-      if (yacc_in) acc = yacc_in[r] + acc;
-      if (!last) {
-        yacc_out[r] = acc;
-      } else {
-        const double yp = yprev ? static_cast<double>(yprev[r]) : 0.0;
-        const T yv = static_cast<T>(acc - c * yp);
-        y[r] = yv;
-        s2 += static_cast<double>(yv) * static_cast<double>(yv);
-        sd += yp * static_cast<double>(yv);
-      }
-    }
-  }
-  if (!last) return;
-  double v2[2] = {s2, sd};
-  block_sum<2>(v2, lds);
-  if (threadIdx.x == 0) {
-    part[blockIdx.x] = v2[0];
-    part[nbt + blockIdx.x] = v2[1];
-  }
+  spmv_rb_body<VT, T, G>(desc[blockIdx.x], rp, ci, va, x, y, yprev, coef.get(),
+                         part + blockIdx.x, part + nbt + blockIdx.x, lsum, yacc_in, yacc_out,
+                         last, prod, lds);
 }
 
-// Long rows: ld[i] = {row, first chunk, number of chunks, -}; one thread per
-// row sums its chunks in order and applies the epilogue; its two partials
-// go to slots nbmain + i.
 template <typename T>
 __global__ __launch_bounds__(kBlock) void k_spmv_longfix(int nlong, const int4* __restrict__ ld,
                                                          const double* __restrict__ lsum,
@@ -148,19 +56,8 @@ __global__ __launch_bounds__(kBlock) void k_spmv_longfix(int nlong, const int4* 
   if (gate.off()) return;
   const int i = blockIdx.x * kBlock + threadIdx.x;
   if (i >= nlong) return;
-  const int4 d = ld[i];
-  double acc = 0.0;
-  for (int q = 0; q < d.z; ++q) acc += lsum[d.y + q];
-  if (yacc_in) acc = yacc_in[d.x] + acc;
-  if (!last) {
-    yacc_out[d.x] = acc;
-    return;
-  }
-  const double yp = yprev ? static_cast<double>(yprev[d.x]) : 0.0;
-  const T yv = static_cast<T>(acc - coef.get() * yp);
-  y[d.x] = yv;
-  part[nbmain + i] = static_cast<double>(yv) * static_cast<double>(yv);
-  part[nbt + nbmain + i] = yp * static_cast<double>(yv);
+  spmv_longfix_body<T>(ld[i], lsum, y, yprev, coef.get(), part + nbmain + i,
+                       part + nbt + nbmain + i, yacc_in, yacc_out, last);
 }
 
 template <typename VT, typename T>
@@ -213,10 +110,6 @@ int launch_reduce(hipStream_t s, int nval, int nblk, const double* part, double*
 }
 
 // --------------------------------------------------------------- gemvt ----
-// Column tile width: CT accumulators per thread, CT column loads in flight
-// per row held.
-constexpr int kCT = 8;
-
 template <typename T>
 __global__ __launch_bounds__(kBlock) void k_gemvt(long long L, const T* __restrict__ V,
                                                   long long ldv, const int* __restrict__ cols,
@@ -229,45 +122,7 @@ __global__ __launch_bounds__(kBlock) void k_gemvt(long long L, const T* __restri
   __shared__ double lds[2][(kBlock / kWave) * kCT];
   for (int i = threadIdx.x; i < ncols; i += kBlock) scol[i] = cols[i];
   __syncthreads();
-  const long long base = static_cast<long long>(blockIdx.x) * kRowsPerBlock + threadIdx.x;
-  double wr[kRPT];
-#pragma unroll
-  for (int r = 0; r < kRPT; ++r) {
-    const long long row = base + static_cast<long long>(r) * kBlock;
-    wr[r] = row < L ? static_cast<double>(w[row]) : 0.0;
-  }
-  const int lane = threadIdx.x % kWave, wid = threadIdx.x / kWave;
-  int buf = 0;
-  for (int c0 = 0; c0 < ncols; c0 += kCT) {
-    double acc[kCT];
-#pragma unroll
-    for (int c = 0; c < kCT; ++c) acc[c] = 0.0;
-#pragma unroll
-    for (int r = 0; r < kRPT; ++r) {
-      const long long row = base + static_cast<long long>(r) * kBlock;
-      if (row < L) {
-#pragma unroll
-        for (int c = 0; c < kCT; ++c) {
-          if (c0 + c < ncols)
-            acc[c] += static_cast<double>(V[static_cast<long long>(scol[c0 + c]) * ldv + row]) *
-                      wr[r];
-        }
-      }
-    }
-#pragma unroll
-    for (int c = 0; c < kCT; ++c) acc[c] = wave_sum(acc[c]);
-    if (lane == 0) {
-#pragma unroll
-      for (int c = 0; c < kCT; ++c) lds[buf][wid * kCT + c] = acc[c];
-    }
-    __syncthreads();
-    if (threadIdx.x < kCT && c0 + threadIdx.x < ncols) {
-      double sum = 0.0;
-      for (int ww = 0; ww < kBlock / kWave; ++ww) sum += lds[buf][ww * kCT + threadIdx.x];
-      part[static_cast<size_t>(c0 + threadIdx.x) * gridDim.x + blockIdx.x] = sum;
-    }
-    buf ^= 1;  // the other buffer is free: its readers passed the barrier above
-  }
+  gemvt_body<T>(blockIdx.x, L, V, ldv, scol, ncols, w, part, gridDim.x, blockIdx.x, lds);
 }
 
 template <typename T>
@@ -300,34 +155,7 @@ __global__ __launch_bounds__(kBlock) void k_gemv_update(long long L, const T* __
     sh[i] = h[i];
   }
   __syncthreads();
-  const long long base = static_cast<long long>(blockIdx.x) * kRowsPerBlock + threadIdx.x;
-  double acc[kRPT];
-#pragma unroll
-  for (int r = 0; r < kRPT; ++r) {
-    const long long row = base + static_cast<long long>(r) * kBlock;
-    acc[r] = row < L ? static_cast<double>(w[row]) : 0.0;
-  }
-  for (int c = 0; c < ncols; ++c) {
-    const T* vc = V + static_cast<long long>(scol[c]) * ldv;
-    const double hc = sh[c];
-#pragma unroll
-    for (int r = 0; r < kRPT; ++r) {
-      const long long row = base + static_cast<long long>(r) * kBlock;
-      if (row < L) acc[r] -= static_cast<double>(vc[row]) * hc;
-    }
-  }
-  double v[1] = {0.0};
-#pragma unroll
-  for (int r = 0; r < kRPT; ++r) {
-    const long long row = base + static_cast<long long>(r) * kBlock;
-    if (row < L) {
-      const T o = static_cast<T>(acc[r]);
-      w[row] = o;
-      v[0] += static_cast<double>(o) * static_cast<double>(o);
-    }
-  }
-  block_sum<1>(v, lds);
-  if (threadIdx.x == 0) part[blockIdx.x] = v[0];
+  gemv_update_body<T>(blockIdx.x, L, V, ldv, scol, sh, ncols, w, part + blockIdx.x, lds);
 }
 
 template <typename T>
@@ -346,13 +174,7 @@ template <typename T>
 __global__ __launch_bounds__(kBlock) void k_scale(long long L, T* __restrict__ x, Scal a,
                                                   Gate gate) {
   if (gate.off()) return;
-  const double av = a.get();
-  const long long base = static_cast<long long>(blockIdx.x) * kRowsPerBlock + threadIdx.x;
-#pragma unroll
-  for (int r = 0; r < kRPT; ++r) {
-    const long long row = base + static_cast<long long>(r) * kBlock;
-    if (row < L) x[row] = static_cast<T>(av * static_cast<double>(x[row]));
-  }
+  scale_body<T>(blockIdx.x, L, x, a.get());
 }
 
 template <typename T>
@@ -368,25 +190,8 @@ __global__ __launch_bounds__(kBlock) void k_axpy_norm_dot(long long L, T* __rest
                                                           Gate gate) {
   if (gate.off()) return;
   __shared__ double lds[(kBlock / kWave) * 2];
-  const double cv = c.get();
-  const long long base = static_cast<long long>(blockIdx.x) * kRowsPerBlock + threadIdx.x;
-  double acc[2] = {0.0, 0.0};
-#pragma unroll
-  for (int r = 0; r < kRPT; ++r) {
-    const long long row = base + static_cast<long long>(r) * kBlock;
-    if (row < L) {
-      const double vv = static_cast<double>(v[row]);
-      const T o = static_cast<T>(static_cast<double>(w[row]) - cv * vv);
-      w[row] = o;
-      acc[0] += static_cast<double>(o) * static_cast<double>(o);
-      acc[1] += vv * static_cast<double>(o);
-    }
-  }
-  block_sum<2>(acc, lds);
-  if (threadIdx.x == 0) {
-    part[blockIdx.x] = acc[0];
-    part[gridDim.x + blockIdx.x] = acc[1];
-  }
+  axpy_norm_dot_body<T>(blockIdx.x, L, w, v, c.get(), part + blockIdx.x,
+                        part + gridDim.x + blockIdx.x, lds);
 }
 
 template <typename T>
@@ -519,13 +324,26 @@ __global__ __launch_bounds__(kBlock) void k_ctl(LbdCtl* __restrict__ gc, int kin
   if (gc->phase != want_phase || gc->j != want_j) return;
   const bool two = (kind == CTL_V_SPMV || kind == CTL_V_ELR || kind == CTL_U_SPMV ||
                     kind == CTL_U_ELR);
+  // stage the state: every thread issues all its 16-byte loads before the
+  // first LDS store, so the copy costs about one memory latency
+  constexpr int nv = sizeof(LbdCtl) / sizeof(int4);
+  constexpr int per = (nv + kBlock - 1) / kBlock;
+  static_assert(sizeof(LbdCtl) % sizeof(int4) == 0, "state must be whole 16-byte words");
+  const int4* src = reinterpret_cast<const int4*>(gc);
+  int4* dst = reinterpret_cast<int4*>(&c);
+  int4 stage[per];
+#pragma unroll
+  for (int u = 0; u < per; ++u) {
+    const int i = threadIdx.x + u * kBlock;
+    if (i < nv) stage[u] = src[i];
+  }
   const double a = reduce_partials(part, nb, lds);
   const double b = two ? reduce_partials(part + nb, nb, lds) : 0.0;
-  constexpr int nw = sizeof(LbdCtl) / sizeof(int);
-  static_assert(sizeof(LbdCtl) % sizeof(int) == 0, "state must be whole words");
-  const int* src = reinterpret_cast<const int*>(gc);
-  int* dst = reinterpret_cast<int*>(&c);
-  for (int i = threadIdx.x; i < nw; i += kBlock) dst[i] = src[i];
+#pragma unroll
+  for (int u = 0; u < per; ++u) {
+    const int i = threadIdx.x + u * kBlock;
+    if (i < nv) dst[i] = stage[u];
+  }
   __syncthreads();
   if (threadIdx.x == 0) {
     switch (kind) {
@@ -539,8 +357,12 @@ __global__ __launch_bounds__(kBlock) void k_ctl(LbdCtl* __restrict__ gc, int kin
     }
   }
   __syncthreads();
-  int* back = reinterpret_cast<int*>(gc);
-  for (int i = threadIdx.x; i < nw; i += kBlock) back[i] = dst[i];
+  int4* back = reinterpret_cast<int4*>(gc);
+#pragma unroll
+  for (int u = 0; u < per; ++u) {
+    const int i = threadIdx.x + u * kBlock;
+    if (i < nv) back[i] = dst[i];
+  }
 }
 
 int launch_ctl(hipStream_t s, LbdCtl* c, int kind, int want_phase, int want_j,

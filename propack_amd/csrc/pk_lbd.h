@@ -44,7 +44,9 @@ enum Phase : int {
   PH_DONE = 100
 };
 
-struct LbdCtl {
+// 16-byte aligned, size a multiple of 16: control kernels stage it in LDS
+// with 16-byte loads.
+struct alignas(16) LbdCtl {
   // parameters of the call
   int m, n, k, elr;
   double delta, eta, eps, epsn, epsn2, sfmin, kappa, fudge;

@@ -35,6 +35,27 @@ module pk_drivers
 
 contains
 
+  ! Effective vector length of the engine's rounding model: max(m, n) as in
+  ! the reference (epsn = max(m,n)*eps/2, double/dlansvd.F:138), or the
+  ! storage-rounding length of the FP32-basis form.
+  double precision function eps_leff(eng, m, n)
+    type(c_ptr), intent(in) :: eng
+    integer, intent(in) :: m, n
+    interface
+      integer(c_int) function pk_lbd_eps(e, eps, leff) bind(C, name='pk_lbd_eps')
+        import :: c_int, c_ptr, c_double
+        type(c_ptr), value :: e
+        real(c_double) :: eps, leff
+      end function
+    end interface
+    real(c_double) :: ee, ll
+    if (pk_lbd_eps(eng, ee, ll) == PK_OK) then
+      eps_leff = ll
+    else
+      eps_leff = dble(max(m, n))
+    end if
+  end function
+
   ! Accumulate the engine's counters since st0 (taken at driver entry) into
   ! COMMON /timing/ (the reference's double/stat.h:12-15 layout) so that
   ! PRINTSTAT-style callers see them.
@@ -125,7 +146,7 @@ contains
 
     rc = pk_lbd_stats(eng, st0)
     eps34 = eps**(3d0 / 4d0)
-    epsn = dble(max(m, n)) * eps / 2d0
+    epsn = eps_leff(eng, m, n) * eps / 2d0
     lanmax = min(n + 1, m + 1, kmax)
     tol = min(1d0, max(16d0 * eps, tolin))
     anorm = 0d0
@@ -242,7 +263,7 @@ contains
 
     rc = pk_lbd_stats(eng, st0)
     eps34 = eps**(3d0 / 4d0)
-    epsn = dble(max(m, n)) * eps / 2d0
+    epsn = eps_leff(eng, m, n) * eps / 2d0
     dim = min(dim, n + 1, m + 1)
     k = dim - p
     tol = min(1d0, max(16d0 * eps, tolin))
@@ -470,16 +491,19 @@ subroutine engine_shape(eng, m, n, eps, info)
       type(c_ptr), value :: e
       integer(c_int) :: m, n, kmax, prec
     end function
+    integer(c_int) function pk_lbd_eps(e, eps, leff) bind(C, name='pk_lbd_eps')
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: e
+      real(c_double) :: eps, leff
+    end function
   end interface
   integer(c_int) :: mm, nn, kk, pp
+  real(c_double) :: ee, ll
   info = pk_lbd_shape(eng, mm, nn, kk, pp)
   m = mm
   n = nn
-  if (pp == 0) then
-    eps = 2d0**(-53)
-  else
-    eps = 2d0**(-24)
-  end if
+  if (info == 0) info = pk_lbd_eps(eng, ee, ll)
+  eps = ee
 end subroutine
 
 ! FP32-basis form with FP64 polish (no reference counterpart).  eng32 holds
@@ -510,6 +534,11 @@ integer(c_int) function pk_lansvd_irl_mixed(eng64, eng32, which, jobu, jobv, dim
       type(c_ptr), value :: dst, src
       integer(c_int), value :: ncols
     end function
+    integer(c_int) function pk_lbd_set_rounding(e, model) bind(C, name='pk_lbd_set_rounding')
+      import :: c_int, c_ptr
+      type(c_ptr), value :: e
+      integer(c_int), value :: model
+    end function
   end interface
   integer :: m, n, info, dd, ne, ne32, m32, n32
   double precision :: eps, eps32, dopt32(4)
@@ -524,7 +553,9 @@ integer(c_int) function pk_lansvd_irl_mixed(eng64, eng32, which, jobu, jobv, dim
     pk_lansvd_irl_mixed = -101
     return
   end if
-  ! FP32-basis phase, left vectors kept on the device for the seed
+  ! FP32-basis phase, left vectors kept on the device for the seed.  Its
+  ! thresholds follow the storage-rounding model (FP64 accumulation).
+  rc = pk_lbd_set_rounding(eng32, 1)
   dd = dim
   ne32 = neig
   dopt32 = doption(1:4)
