@@ -66,8 +66,10 @@ __device__ __forceinline__ void spmv_rb_body(const int4 d, const int* __restrict
 #pragma unroll
     for (int u = 0; u < kUnroll; ++u) {
       const int t = base + u * kBlock + static_cast<int>(threadIdx.x);
-      idx[u] = t < nz ? ci[p0 + t] : -1;
-      v[u] = t < nz ? static_cast<double>(va[p0 + t]) : 0.0;
+      // index and value streams are read once: non-temporal, so that they
+      // do not evict the gathered vector slice from L2
+      idx[u] = t < nz ? __builtin_nontemporal_load(ci + p0 + t) : -1;
+      v[u] = t < nz ? static_cast<double>(__builtin_nontemporal_load(va + p0 + t)) : 0.0;
     }
 #pragma unroll
     for (int u = 0; u < kUnroll; ++u) xv[u] = idx[u] >= 0 ? static_cast<double>(x[idx[u]]) : 0.0;

@@ -26,6 +26,10 @@ namespace pk {
 
 constexpr int kMaxK = kMaxCols;  // largest Lanczos dimension of the engine
 
+// An omega-recurrence update a control stage left to run before its
+// remainder (the control kernels spread it over the workgroup).
+enum Pending : int { PEND_NONE = 0, PEND_NU = 1, PEND_MU = 2 };
+
 enum Phase : int {
   // phases a device chain executes
   PH_V_SPMV = 1,  // alpha_j v_j = A^T u_j - beta_j v_{j-1}   (dlanbpro.F:288-296)
@@ -61,6 +65,7 @@ struct alignas(16) LbdCtl {
   double vscale_f, uscale_f;
   long long nopx, nreorth, ndot, ncols_touched, nretry;
   long long npass_v, npass_u, ncols_v, ncols_u;  // CGS passes and columns streamed
+  int pending;        // an omega update the step still has to run (Pending)
   // 1-based arrays
   double mu[kMaxK + 3], nu[kMaxK + 3], B1[kMaxK + 2], B2[kMaxK + 2];
   int index[2 * kMaxK + 4];
@@ -206,10 +211,9 @@ __host__ __device__ inline void ctl_v_after_reorth(LbdCtl& c) {
   ctl_v_post(c);
 }
 
-__host__ __device__ inline void ctl_v_nu(LbdCtl& c) {
+// Remainder of the nu step after the omega update (dlanbpro.F:321-367).
+__host__ __device__ inline void ctl_v_nu_post(LbdCtl& c) {
   const int j = c.j;
-  if (!c.full_reorth && c.alpha != 0.0)
-    hd_update_nu(&c.numax, c.mu, c.nu, j, c.B1, c.B2, c.anorm, c.epsn2);
   if ((c.full_reorth || c.numax > c.delta || c.force_reorth) && c.alpha != 0.0) {
     if (c.full_reorth || c.eta == 0.0) {
       c.index[1] = 1;
@@ -228,6 +232,15 @@ __host__ __device__ inline void ctl_v_nu(LbdCtl& c) {
     return;
   }
   ctl_v_post(c);
+}
+
+// nu step (dlanbpro.F:318-320): the omega update is left pending.
+__host__ __device__ inline void ctl_v_nu(LbdCtl& c) {
+  if (!c.full_reorth && c.alpha != 0.0) {
+    c.pending = PEND_NU;
+    return;
+  }
+  ctl_v_nu_post(c);
 }
 
 __host__ __device__ inline void ctl_v_fin(LbdCtl& c) {
@@ -251,7 +264,7 @@ __host__ __device__ inline void ctl_v_fin(LbdCtl& c) {
 }
 
 // After the fused SpMV A^T u_j - beta v_{j-1}: s2 = ||v_j||^2, sd = v_{j-1}.v_j.
-__host__ __device__ inline void ctl_v_spmv(LbdCtl& c, double s2, double sd) {
+__host__ __device__ inline void ctl_v_spmv_s(LbdCtl& c, double s2, double sd) {
 #pragma clang fp contract(off)
   c.nopx++;
   c.uscale = 0;
@@ -271,7 +284,7 @@ __host__ __device__ inline void ctl_v_spmv(LbdCtl& c, double s2, double sd) {
 }
 
 // After v_j -= elr_s v_{j-1}: ns2 = ||v_j||^2, nsd = v_{j-1}.v_j.
-__host__ __device__ inline void ctl_v_elr(LbdCtl& c, double ns2, double nsd) {
+__host__ __device__ inline void ctl_v_elr_s(LbdCtl& c, double ns2, double nsd) {
 #pragma clang fp contract(off)
   const double s = c.elr_s;
   if (c.beta != 0.0) {
@@ -291,7 +304,7 @@ __host__ __device__ inline void ctl_v_elr(LbdCtl& c, double ns2, double nsd) {
 }
 
 // After one CGS pass of v_j: s2 = ||v_j||^2 (dreorth.F:85-99).
-__host__ __device__ inline void ctl_v_reo(LbdCtl& c, double s2) {
+__host__ __device__ inline void ctl_v_reo_s(LbdCtl& c, double s2) {
 #pragma clang fp contract(off)
   c.ndot += c.reo_nd + c.reo_k;
   c.ncols_touched += c.reo_ncols;
@@ -344,10 +357,9 @@ __host__ __device__ inline void ctl_u_after_reorth(LbdCtl& c) {
   ctl_u_post(c);
 }
 
-__host__ __device__ inline void ctl_u_mu(LbdCtl& c) {
+// Remainder of the mu step after the omega update (dlanbpro.F:449-498).
+__host__ __device__ inline void ctl_u_mu_post(LbdCtl& c) {
   const int j = c.j;
-  if (!c.full_reorth && c.beta != 0.0)
-    hd_update_mu(&c.mumax, c.mu, c.nu, j, c.B1, c.B2, c.anorm, c.epsn2);
   if ((c.full_reorth || c.mumax > c.delta || c.force_reorth) && c.beta != 0.0) {
     if (c.full_reorth || c.eta == 0.0) {
       c.index[1] = 1;
@@ -370,6 +382,15 @@ __host__ __device__ inline void ctl_u_mu(LbdCtl& c) {
   ctl_u_post(c);
 }
 
+// mu step (dlanbpro.F:446-448): the omega update is left pending.
+__host__ __device__ inline void ctl_u_mu(LbdCtl& c) {
+  if (!c.full_reorth && c.beta != 0.0) {
+    c.pending = PEND_MU;
+    return;
+  }
+  ctl_u_mu_post(c);
+}
+
 __host__ __device__ inline void ctl_u_fin(LbdCtl& c) {
 #pragma clang fp contract(off)
   const int j = c.j;
@@ -386,7 +407,7 @@ __host__ __device__ inline void ctl_u_fin(LbdCtl& c) {
   ctl_u_mu(c);
 }
 
-__host__ __device__ inline void ctl_u_spmv(LbdCtl& c, double s2, double sd) {
+__host__ __device__ inline void ctl_u_spmv_s(LbdCtl& c, double s2, double sd) {
 #pragma clang fp contract(off)
   c.nopx++;
   c.vscale = 0;
@@ -400,7 +421,7 @@ __host__ __device__ inline void ctl_u_spmv(LbdCtl& c, double s2, double sd) {
   ctl_u_fin(c);
 }
 
-__host__ __device__ inline void ctl_u_elr(LbdCtl& c, double ns2, double nsd) {
+__host__ __device__ inline void ctl_u_elr_s(LbdCtl& c, double ns2, double nsd) {
 #pragma clang fp contract(off)
   const double s = c.elr_s;
   if (c.alpha != 0.0) {
@@ -419,7 +440,7 @@ __host__ __device__ inline void ctl_u_elr(LbdCtl& c, double ns2, double nsd) {
   }
 }
 
-__host__ __device__ inline void ctl_u_reo(LbdCtl& c, double s2) {
+__host__ __device__ inline void ctl_u_reo_s(LbdCtl& c, double s2) {
 #pragma clang fp contract(off)
   c.ndot += c.reo_nd + c.reo_k;
   c.ncols_touched += c.reo_ncols;
@@ -437,6 +458,49 @@ __host__ __device__ inline void ctl_u_reo(LbdCtl& c, double s2) {
   } else {
     c.reo_norm0 = nrm;
   }
+}
+
+// Runs a pending omega update serially, then the rest of its step.
+__host__ __device__ inline void ctl_resume(LbdCtl& c) {
+  if (c.pending == PEND_NU) {
+    c.pending = PEND_NONE;
+    hd_update_nu(&c.numax, c.mu, c.nu, c.j, c.B1, c.B2, c.anorm, c.epsn2);
+    ctl_v_nu_post(c);
+  } else if (c.pending == PEND_MU) {
+    c.pending = PEND_NONE;
+    hd_update_mu(&c.mumax, c.mu, c.nu, c.j, c.B1, c.B2, c.anorm, c.epsn2);
+    ctl_u_mu_post(c);
+  }
+}
+
+// Control stages of one step.  The *_s forms stop at a pending omega update
+// (the control kernels run it in parallel); these complete the stage.
+// After the fused SpMV A^T u_j - beta v_{j-1}: s2 = ||v_j||^2, sd = v_{j-1}.v_j.
+__host__ __device__ inline void ctl_v_spmv(LbdCtl& c, double s2, double sd) {
+  ctl_v_spmv_s(c, s2, sd);
+  ctl_resume(c);
+}
+// After v_j -= elr_s v_{j-1}: ns2 = ||v_j||^2, nsd = v_{j-1}.v_j.
+__host__ __device__ inline void ctl_v_elr(LbdCtl& c, double ns2, double nsd) {
+  ctl_v_elr_s(c, ns2, nsd);
+  ctl_resume(c);
+}
+// After one CGS pass of v_j: s2 = ||v_j||^2.
+__host__ __device__ inline void ctl_v_reo(LbdCtl& c, double s2) {
+  ctl_v_reo_s(c, s2);
+  ctl_resume(c);
+}
+__host__ __device__ inline void ctl_u_spmv(LbdCtl& c, double s2, double sd) {
+  ctl_u_spmv_s(c, s2, sd);
+  ctl_resume(c);
+}
+__host__ __device__ inline void ctl_u_elr(LbdCtl& c, double ns2, double nsd) {
+  ctl_u_elr_s(c, ns2, nsd);
+  ctl_resume(c);
+}
+__host__ __device__ inline void ctl_u_reo(LbdCtl& c, double s2) {
+  ctl_u_reo_s(c, s2);
+  ctl_resume(c);
 }
 
 // Gate of a device-chain kernel: runs only while the control state is in
