@@ -19,8 +19,8 @@ int stream_blocks(long long L);
 // of the earlier panels, null for the first), yacc_out (where a panel that
 // is not the last stores its running sums), last (1: apply the epilogue and
 // write the partials).  Returns the slot count.
-constexpr int kSpmvCap = 4096;
-constexpr int kSpmvRows = 256;
+constexpr int kSpmvCap = 2048;
+constexpr int kSpmvRows = 1024;
 constexpr int kLongChunk = 8192;
 template <typename VT, typename T>
 int launch_spmv_rb(hipStream_t s, int nblk, const int4* desc, int nlong, const int4* ldesc,
