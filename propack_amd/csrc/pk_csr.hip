@@ -211,12 +211,13 @@ __global__ void k_gather_panel(long long cnt, const int* __restrict__ perm,
 
 // Column panels of an operator direction (rows x cols, nnz): the smallest
 // power of two that cuts the gathered vector into slices of at most
-// kPanelSliceBytes (2 MiB: a slice plus the streamed matrix data fits one
-// XCD's 4 MiB L2, so the gathers hit L2 instead of the Infinity Cache), up
-// to 8; one panel when rows are short (mean below 8 nonzeros: the FP64
-// running row sums would cost more traffic than the gathers save).
-// PK_SPMV_PANELS ("p" for both directions or "pA,pAt") overrides.
-constexpr double kPanelSliceBytes = 2.0 * 1024 * 1024;
+// kPanelSliceBytes (4 MiB, one XCD's L2: with the index and value streams
+// read non-temporally the slice stays in L2, so the gathers hit L2 instead
+// of the Infinity Cache), up to 8; one panel when rows are short (mean
+// below 8 nonzeros: the FP64 running row sums would cost more traffic than
+// the gathers save).  PK_SPMV_PANELS ("p" for both directions or "pA,pAt")
+// overrides.
+constexpr double kPanelSliceBytes = 4.0 * 1024 * 1024;
 
 int panels_for(bool transposed, int rows, int cols, long long nnz, size_t vbytes) {
   int want = 1;

@@ -19,7 +19,7 @@ int stream_blocks(long long L);
 // of the earlier panels, null for the first), yacc_out (where a panel that
 // is not the last stores its running sums), last (1: apply the epilogue and
 // write the partials).  Returns the slot count.
-constexpr int kSpmvCap = 2048;
+constexpr int kSpmvCap = 1024;
 constexpr int kSpmvRows = 1024;
 constexpr int kLongChunk = 8192;
 template <typename VT, typename T>
