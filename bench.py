@@ -49,6 +49,24 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
+TRAFFIC_PROFILE = os.path.join(ROOT, "profiles", "hbm_traffic_spmv_poisson_decay_2Mx1M.md")
+
+
+def measured_traffic(kclass):
+    """HBM bytes per operator application of kernel class kclass, as
+    measured by the rocprofv3 FETCH_SIZE / WRITE_SIZE runs summarised in
+    profiles/ (calibrated against a plain copy); None if absent."""
+    try:
+        with open(TRAFFIC_PROFILE) as f:
+            for line in f:
+                w = line.split()
+                if len(w) == 3 and w[0] == "traffic_bytes_per_application" and w[1] == kclass:
+                    return float(w[2])
+    except OSError:
+        pass
+    return None
+
+
 def residuals(op, r):
     """max_i ||A v_i - s_i u_i|| / s_1 over the returned triplets."""
     worst = 0.0
@@ -240,7 +258,7 @@ def main():
                    "max_rel_residual": resid},
         "roofline": {"kernel": dom, "bound": "hbm", "achieved": round(achieved, 1),
                      "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": measured_traffic(dom),
                      "avg_launch_ms": per_launch_ms, "bytes_per_launch": per_launch_bytes},
         "kernels": kernels,
         "fp32_basis_form": mixed,

@@ -211,13 +211,14 @@ __global__ void k_gather_panel(long long cnt, const int* __restrict__ perm,
 
 // Column panels of an operator direction (rows x cols, nnz): the smallest
 // power of two that cuts the gathered vector into slices of at most
-// kPanelSliceBytes (4 MiB, one XCD's L2: with the index and value streams
-// read non-temporally the slice stays in L2, so the gathers hit L2 instead
-// of the Infinity Cache), up to 8; one panel when rows are short (mean
-// below 8 nonzeros: the FP64 running row sums would cost more traffic than
-// the gathers save).  PK_SPMV_PANELS ("p" for both directions or "pA,pAt")
-// overrides.
-constexpr double kPanelSliceBytes = 4.0 * 1024 * 1024;
+// kPanelSliceBytes (2 MiB, half of one XCD's L2: with the index and value
+// streams read non-temporally the slice stays in L2, so the random gathers
+// hit L2 instead of going to the Infinity Cache; a 4 MiB slice measured 2x
+// the algorithmic fabric traffic), up to 8; one panel when rows are short
+// (mean below 8 nonzeros: the FP64 running row sums would cost more traffic
+// than the gathers save).  PK_SPMV_PANELS ("p" for both directions or
+// "pA,pAt") overrides.
+constexpr double kPanelSliceBytes = 2.0 * 1024 * 1024;
 
 int panels_for(bool transposed, int rows, int cols, long long nnz, size_t vbytes) {
   int want = 1;
