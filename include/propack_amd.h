@@ -130,6 +130,22 @@ int pk_lbd_shape(pk_lbd_t eng, int* m, int* n, int* kmax, int* prec);
  * pk_lbd_eps returns eps of the engine's precision and leff. */
 int pk_lbd_set_rounding(pk_lbd_t eng, int model);
 int pk_lbd_eps(pk_lbd_t eng, double* eps, double* leff);
+
+/* Batched Lanczos steps (no reference counterpart; replaces a per-problem
+ * loop of DLANBPRO calls, double/dlanbpro.F:1-2, over independent
+ * problems): the engines of a batch (same precision, each with its own
+ * device CSR operator) run their Lanczos steps together, one launch per
+ * kernel per step for the whole batch, each problem gated by its own
+ * control state; a problem gives the same results as alone. */
+typedef struct pk_batch_opaque* pk_batch_t;
+int pk_batch_create(int nprob, const pk_lbd_t* engines, pk_batch_t* batch);
+int pk_batch_destroy(pk_batch_t batch);
+/* pk_lbd_lanbpro for every problem i with active[i] != 0: k0[i] -> k[i]
+ * steps; alpha[i], beta[i] its B(:,1), B(:,2); rnorm[i]; doption(4*i+1:
+ * 4*i+3) its delta, eta, anorm; ioption shared; ierr[i]. */
+int pk_batch_lanbpro(pk_batch_t batch, const int* active, const int* k0, int* k,
+                     double* const* alpha, double* const* beta, double* rnorm, double* doption,
+                     const int* ioption, int* ierr);
 /* Step execution (no reference counterpart): mode 0 runs the Lanczos steps
  * as device-resident kernel chains whose scalar control (the omega
  * recurrence, ELR and reorthogonalization decisions) runs in control

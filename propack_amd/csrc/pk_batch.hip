@@ -1,0 +1,634 @@
+// Batched Lanczos steps: many independent engines (one truncated SVD each,
+// of uneven sizes) advance together, one launch per kernel per step for
+// the whole batch.  Every workgroup looks up its problem in a table and runs
+// the same per-workgroup body as the single-problem kernels (pk_bodies.h),
+// on the same rows, slots and summation order, so a problem gives
+// bit-identical results in a batch and alone.  Each problem's control state
+// gates its own workgroups (a problem that needs no reorthogonalization, or
+// has finished, idles through that part of the step).
+//
+// A problem whose state leaves the device-resident phases (invariant
+// subspace, tiny scale, failed Gram-Schmidt) drops out of the batched chain
+// and finishes its call on its own engine afterwards.
+#include <algorithm>
+#include <cstring>
+#include <vector>
+
+#include "pk_bodies.h"
+#include "pk_internal.h"
+
+namespace pk {
+void set_error(const char* fmt, ...);
+
+namespace {
+
+constexpr int kPanelsMax = CsrOp::kMaxPanels;
+
+// Per-problem view, rebuilt for every batched call (restarts swap bases).
+struct BProb {
+  LbdCtl* ctl;
+  void* U;
+  void* V;
+  long long ldu, ldv;
+  int m, n;
+  int sm0, nsm;  // workgroup slices over m rows: first global slice, count
+  int sn0, nsn;  // over n rows
+  int qa0, nqa;  // partial slots of A v (last panel): first global slot, count
+  int qt0, nqt;  // of A^T u
+  int npa, npt;  // column panels of A, A^T
+  double* yacc_a;
+  double* yacc_t;
+};
+
+struct BPanel {
+  const int* rp;
+  const int* ci;
+  const void* va;
+  double* lsum;
+};
+
+// Work item of a batched SpMV launch: descriptor of problem pid, local
+// partial slot loc (meaningful in the problem's last panel).
+struct BWork {
+  int pid, loc;
+};
+
+template <typename T>
+__device__ __forceinline__ T* bcol(const BProb& P, bool vside, int j) {
+  return vside ? static_cast<T*>(P.V) + static_cast<long long>(j - 1) * P.ldv
+               : static_cast<T*>(P.U) + static_cast<long long>(j - 1) * P.ldu;
+}
+
+// ---------------------------------------------------------------- kernels --
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_bscale(const BProb* __restrict__ probs,
+                                                   const int* __restrict__ slice_pid, int vside,
+                                                   int J, int phase, int flag) {
+  const BProb P = probs[slice_pid[blockIdx.x]];
+  if (Gate{P.ctl, phase, J, flag}.off()) return;
+  const long long b = static_cast<long long>(blockIdx.x) - (vside ? P.sn0 : P.sm0);
+  const double f = flag == 2 ? P.ctl->uscale_f : P.ctl->vscale_f;
+  scale_body<T>(b, vside ? P.n : P.m, bcol<T>(P, vside, J), f);
+}
+
+// tr = 1: v_J = A^T u_J - beta v_{J-1}; tr = 0: u_{J+1} = A v_J - alpha u_J.
+template <typename VT, typename T, int G>
+__global__ __launch_bounds__(kBlock) void k_bspmv(const BProb* __restrict__ probs,
+                                                  const BPanel* __restrict__ panels,
+                                                  const int4* __restrict__ desc,
+                                                  const BWork* __restrict__ work, int tr, int p,
+                                                  int J, int phase, double* __restrict__ part,
+                                                  int Q) {
+  const BWork w = work[blockIdx.x];
+  const BProb P = probs[w.pid];
+  if (Gate{P.ctl, phase, J, 0}.off()) return;
+  __shared__ double prod[kSpmvCap];
+  __shared__ double lds[(kBlock / kWave) * 2];
+  const BPanel pan = panels[(w.pid * 2 + tr) * kPanelsMax + p];
+  const int np = tr ? P.npt : P.npa;
+  const int last = p == np - 1;
+  double* yacc = tr ? P.yacc_t : P.yacc_a;
+  const T* x = tr ? bcol<T>(P, false, J) : bcol<T>(P, true, J);
+  T* y = tr ? bcol<T>(P, true, J) : bcol<T>(P, false, J + 1);
+  const T* yprev = tr ? (J > 1 ? bcol<T>(P, true, J - 1) : nullptr) : bcol<T>(P, false, J);
+  const double c = tr ? P.ctl->beta : P.ctl->alpha;
+  const int q = (tr ? P.qt0 : P.qa0) + w.loc;
+  spmv_rb_body<VT, T, G>(desc[blockIdx.x], pan.rp, pan.ci, static_cast<const VT*>(pan.va), x, y,
+                         yprev, c, part + q, part + Q + q, pan.lsum, p == 0 ? nullptr : yacc,
+                         last ? nullptr : yacc, last, prod, lds);
+}
+
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_blongfix(const BProb* __restrict__ probs,
+                                                     const BPanel* __restrict__ panels,
+                                                     const int4* __restrict__ ldesc,
+                                                     const BWork* __restrict__ work, int nlong,
+                                                     int tr, int p, int J, int phase,
+                                                     double* __restrict__ part, int Q) {
+  const int i = blockIdx.x * kBlock + threadIdx.x;
+  if (i >= nlong) return;
+  const BWork w = work[i];
+  const BProb P = probs[w.pid];
+  if (Gate{P.ctl, phase, J, 0}.off()) return;
+  const BPanel pan = panels[(w.pid * 2 + tr) * kPanelsMax + p];
+  const int np = tr ? P.npt : P.npa;
+  const int last = p == np - 1;
+  double* yacc = tr ? P.yacc_t : P.yacc_a;
+  T* y = tr ? bcol<T>(P, true, J) : bcol<T>(P, false, J + 1);
+  const T* yprev = tr ? (J > 1 ? bcol<T>(P, true, J - 1) : nullptr) : bcol<T>(P, false, J);
+  const double c = tr ? P.ctl->beta : P.ctl->alpha;
+  const int q = (tr ? P.qt0 : P.qa0) + w.loc;
+  spmv_longfix_body<T>(ldesc[i], pan.lsum, y, yprev, c, part + q, part + Q + q,
+                       p == 0 ? nullptr : yacc, last ? nullptr : yacc, last);
+}
+
+// One workgroup per problem: the control stage of `kind` over the problem's
+// own partial range.
+__global__ __launch_bounds__(kBlock) void k_bctl(const BProb* __restrict__ probs, int kind,
+                                                 int phase, int J, const double* __restrict__ part,
+                                                 int Qa, int Qt, int Sm, int Sn) {
+  const BProb P = probs[blockIdx.x];
+  __shared__ LbdCtl c;
+  __shared__ double lds[kBlock / kWave];
+  if (P.ctl->phase != phase || P.ctl->j != J) return;
+  const double* pa;
+  const double* pb = nullptr;
+  int na;
+  switch (kind) {
+    case CTL_V_SPMV: pa = part + P.qt0; pb = part + Qt + P.qt0; na = P.nqt; break;
+    case CTL_U_SPMV: pa = part + P.qa0; pb = part + Qa + P.qa0; na = P.nqa; break;
+    case CTL_V_ELR: pa = part + P.sn0; pb = part + Sn + P.sn0; na = P.nsn; break;
+    case CTL_U_ELR: pa = part + P.sm0; pb = part + Sm + P.sm0; na = P.nsm; break;
+    case CTL_V_REO: pa = part + P.sn0; na = P.nsn; break;
+    default: pa = part + P.sm0; na = P.nsm; break;  // CTL_U_REO
+  }
+  ctl_stage_body(P.ctl, kind, pa, na, pb, na, c, lds);
+}
+
+// ELR update: v side w = v_J, v = v_{J-1}; u side w = u_{J+1}, v = u_J.
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_baxpy(const BProb* __restrict__ probs,
+                                                  const int* __restrict__ slice_pid, int vside,
+                                                  int J, int phase, double* __restrict__ part,
+                                                  int S) {
+  const BProb P = probs[slice_pid[blockIdx.x]];
+  if (Gate{P.ctl, phase, J, 0}.off()) return;
+  __shared__ double lds[(kBlock / kWave) * 2];
+  const int s0 = vside ? P.sn0 : P.sm0;
+  const long long b = static_cast<long long>(blockIdx.x) - s0;
+  T* w = vside ? bcol<T>(P, true, J) : bcol<T>(P, false, J + 1);
+  const T* v = vside ? bcol<T>(P, true, J - 1) : bcol<T>(P, false, J);
+  axpy_norm_dot_body<T>(b, vside ? P.n : P.m, w, v, P.ctl->elr_s, part + blockIdx.x,
+                        part + S + blockIdx.x, lds);
+}
+
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_bgemvt(const BProb* __restrict__ probs,
+                                                   const int* __restrict__ slice_pid, int vside,
+                                                   int J, int phase, double* __restrict__ part,
+                                                   int S) {
+  const BProb P = probs[slice_pid[blockIdx.x]];
+  if (Gate{P.ctl, phase, J, 0}.off()) return;
+  __shared__ int scol[kMaxCols];
+  __shared__ double lds[2][(kBlock / kWave) * kCT];
+  const int ncols = P.ctl->reo_ncols;
+  for (int i = threadIdx.x; i < ncols; i += kBlock) scol[i] = P.ctl->cols[i];
+  __syncthreads();
+  const long long b = static_cast<long long>(blockIdx.x) - (vside ? P.sn0 : P.sm0);
+  const T* basis = static_cast<const T*>(vside ? P.V : P.U);
+  const T* w = vside ? bcol<T>(P, true, J) : bcol<T>(P, false, J + 1);
+  gemvt_body<T>(b, vside ? P.n : P.m, basis, vside ? P.ldv : P.ldu, scol, ncols, w, part, S,
+                blockIdx.x, lds);
+}
+
+// red[pid * kMaxCols + c] = sum over the problem's slices of the projection
+// partials of column c (grid: problems x columns).
+__global__ __launch_bounds__(kBlock) void k_breduce(const BProb* __restrict__ probs, int vside,
+                                                    int J, int phase,
+                                                    const double* __restrict__ part, int S,
+                                                    double* __restrict__ red) {
+  const BProb P = probs[blockIdx.x];
+  if (Gate{P.ctl, phase, J, 0}.off()) return;
+  const int c = blockIdx.y;
+  if (c >= P.ctl->reo_ncols) return;
+  __shared__ double lds[kBlock / kWave];
+  const int s0 = vside ? P.sn0 : P.sm0, ns = vside ? P.nsn : P.nsm;
+  const double v = reduce_partials(part + static_cast<long long>(c) * S + s0, ns, lds);
+  if (threadIdx.x == 0) red[static_cast<long long>(blockIdx.x) * kMaxCols + c] = v;
+}
+
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_bgemv(const BProb* __restrict__ probs,
+                                                  const int* __restrict__ slice_pid, int vside,
+                                                  int J, int phase,
+                                                  const double* __restrict__ red,
+                                                  double* __restrict__ part) {
+  const int pid = slice_pid[blockIdx.x];
+  const BProb P = probs[pid];
+  if (Gate{P.ctl, phase, J, 0}.off()) return;
+  __shared__ int scol[kMaxCols];
+  __shared__ double sh[kMaxCols];
+  __shared__ double lds[kBlock / kWave];
+  const int ncols = P.ctl->reo_ncols;
+  for (int i = threadIdx.x; i < ncols; i += kBlock) {
+    scol[i] = P.ctl->cols[i];
+    sh[i] = red[static_cast<long long>(pid) * kMaxCols + i];
+  }
+  __syncthreads();
+  const long long b = static_cast<long long>(blockIdx.x) - (vside ? P.sn0 : P.sm0);
+  const T* basis = static_cast<const T*>(vside ? P.V : P.U);
+  T* w = vside ? bcol<T>(P, true, J) : bcol<T>(P, false, J + 1);
+  gemv_update_body<T>(b, vside ? P.n : P.m, basis, vside ? P.ldv : P.ldu, scol, sh, ncols, w,
+                      part + blockIdx.x, lds);
+}
+
+}  // namespace
+
+// ------------------------------------------------------------------ batch --
+struct Batch {
+  static constexpr unsigned kMagic = 0x504b4254u;  // "PKBT"
+  unsigned magic = 0;
+  int nprob = 0, prec = PK_F64, val_prec = PK_F64;
+  std::vector<Engine*> eng;
+  hipStream_t stream = nullptr;
+  // static tables (operators do not change)
+  int Sm = 0, Sn = 0, Qa = 0, Qt = 0, maxk = 0;
+  int* slice_pid_m = nullptr;  // global slice over m -> problem
+  int* slice_pid_n = nullptr;
+  BPanel* panels = nullptr;    // (pid * 2 + tr) * kPanelsMax + p
+  struct Launch {              // one batched SpMV launch
+    int tr, p, g, count;
+    int4* desc;
+    BWork* work;
+  };
+  struct Long {
+    int tr, p, count;
+    int4* ldesc;
+    BWork* work;
+  };
+  std::vector<Launch> spmv;
+  std::vector<Long> lng;
+  std::vector<BProb> hprob;    // host template (pointers filled per call)
+  BProb* dprob = nullptr;
+  double* part = nullptr;
+  double* red = nullptr;
+  size_t part_len = 0;
+};
+
+namespace {
+
+template <typename T>
+void enqueue_batch_step(Batch& B, int J, int elr, int tries) {
+  hipStream_t s = B.stream;
+  const int np = B.nprob;
+  // ---- v_J = A^T u_J - beta v_{J-1} ----
+  hipLaunchKernelGGL(k_bscale<T>, dim3(B.Sm), dim3(kBlock), 0, s, B.dprob, B.slice_pid_m, 0, J,
+                     static_cast<int>(PH_V_SPMV), 2);
+  auto spmv = [&](int tr, int phase) {
+    for (int p = 0; p < kPanelsMax; ++p) {
+      bool any = false;
+      for (const auto& L : B.spmv) {
+        if (L.tr != tr || L.p != p || L.count == 0) continue;
+        any = true;
+        const int Q = tr ? B.Qt : B.Qa;
+#define PK_BS_CASE(G)                                                                           \
+  case G:                                                                                       \
+    if (B.val_prec == PK_F64)                                                                   \
+      hipLaunchKernelGGL((k_bspmv<double, T, G>), dim3(L.count), dim3(kBlock), 0, s, B.dprob,   \
+                         B.panels, L.desc, L.work, tr, p, J, phase, B.part, Q);                 \
+    else                                                                                        \
+      hipLaunchKernelGGL((k_bspmv<float, T, G>), dim3(L.count), dim3(kBlock), 0, s, B.dprob,    \
+                         B.panels, L.desc, L.work, tr, p, J, phase, B.part, Q);                 \
+    break;
+        switch (L.g) {
+          PK_BS_CASE(2)
+          PK_BS_CASE(4)
+          PK_BS_CASE(8)
+          PK_BS_CASE(16)
+          PK_BS_CASE(32)
+          PK_BS_CASE(64)
+          default: break;
+        }
+#undef PK_BS_CASE
+      }
+      for (const auto& L : B.lng) {
+        if (L.tr != tr || L.p != p || L.count == 0) continue;
+        any = true;
+        hipLaunchKernelGGL(k_blongfix<T>, dim3(cdiv(L.count, kBlock)), dim3(kBlock), 0, s,
+                           B.dprob, B.panels, L.ldesc, L.work, L.count, tr, p, J, phase, B.part,
+                           tr ? B.Qt : B.Qa);
+      }
+      if (!any) break;
+    }
+  };
+  spmv(1, PH_V_SPMV);
+  hipLaunchKernelGGL(k_bctl, dim3(np), dim3(kBlock), 0, s, B.dprob, static_cast<int>(CTL_V_SPMV),
+                     static_cast<int>(PH_V_SPMV), J, B.part, B.Qa, B.Qt, B.Sm, B.Sn);
+  if (J > 1) {
+    for (int i = 0; i < elr; ++i) {
+      hipLaunchKernelGGL(k_baxpy<T>, dim3(B.Sn), dim3(kBlock), 0, s, B.dprob, B.slice_pid_n, 1,
+                         J, static_cast<int>(PH_V_ELR), B.part, B.Sn);
+      hipLaunchKernelGGL(k_bctl, dim3(np), dim3(kBlock), 0, s, B.dprob,
+                         static_cast<int>(CTL_V_ELR), static_cast<int>(PH_V_ELR), J, B.part, B.Qa,
+                         B.Qt, B.Sm, B.Sn);
+    }
+    for (int t = 0; t < tries; ++t) {
+      hipLaunchKernelGGL(k_bgemvt<T>, dim3(B.Sn), dim3(kBlock), 0, s, B.dprob, B.slice_pid_n, 1,
+                         J, static_cast<int>(PH_V_REO), B.part, B.Sn);
+      hipLaunchKernelGGL(k_breduce, dim3(np, J - 1), dim3(kBlock), 0, s, B.dprob, 1, J,
+                         static_cast<int>(PH_V_REO), B.part, B.Sn, B.red);
+      hipLaunchKernelGGL(k_bgemv<T>, dim3(B.Sn), dim3(kBlock), 0, s, B.dprob, B.slice_pid_n, 1,
+                         J, static_cast<int>(PH_V_REO), B.red, B.part);
+      hipLaunchKernelGGL(k_bctl, dim3(np), dim3(kBlock), 0, s, B.dprob,
+                         static_cast<int>(CTL_V_REO), static_cast<int>(PH_V_REO), J, B.part, B.Qa,
+                         B.Qt, B.Sm, B.Sn);
+    }
+  }
+  // ---- u_{J+1} = A v_J - alpha u_J ----
+  hipLaunchKernelGGL(k_bscale<T>, dim3(B.Sn), dim3(kBlock), 0, s, B.dprob, B.slice_pid_n, 1, J,
+                     static_cast<int>(PH_U_SPMV), 1);
+  spmv(0, PH_U_SPMV);
+  hipLaunchKernelGGL(k_bctl, dim3(np), dim3(kBlock), 0, s, B.dprob, static_cast<int>(CTL_U_SPMV),
+                     static_cast<int>(PH_U_SPMV), J, B.part, B.Qa, B.Qt, B.Sm, B.Sn);
+  for (int i = 0; i < elr; ++i) {
+    hipLaunchKernelGGL(k_baxpy<T>, dim3(B.Sm), dim3(kBlock), 0, s, B.dprob, B.slice_pid_m, 0, J,
+                       static_cast<int>(PH_U_ELR), B.part, B.Sm);
+    hipLaunchKernelGGL(k_bctl, dim3(np), dim3(kBlock), 0, s, B.dprob, static_cast<int>(CTL_U_ELR),
+                       static_cast<int>(PH_U_ELR), J, B.part, B.Qa, B.Qt, B.Sm, B.Sn);
+  }
+  for (int t = 0; t < tries; ++t) {
+    hipLaunchKernelGGL(k_bgemvt<T>, dim3(B.Sm), dim3(kBlock), 0, s, B.dprob, B.slice_pid_m, 0, J,
+                       static_cast<int>(PH_U_REO), B.part, B.Sm);
+    hipLaunchKernelGGL(k_breduce, dim3(np, J), dim3(kBlock), 0, s, B.dprob, 0, J,
+                       static_cast<int>(PH_U_REO), B.part, B.Sm, B.red);
+    hipLaunchKernelGGL(k_bgemv<T>, dim3(B.Sm), dim3(kBlock), 0, s, B.dprob, B.slice_pid_m, 0, J,
+                       static_cast<int>(PH_U_REO), B.red, B.part);
+    hipLaunchKernelGGL(k_bctl, dim3(np), dim3(kBlock), 0, s, B.dprob, static_cast<int>(CTL_U_REO),
+                       static_cast<int>(PH_U_REO), J, B.part, B.Qa, B.Qt, B.Sm, B.Sn);
+  }
+}
+
+void batch_free(Batch* b) {
+  if (!b) return;
+  for (auto& L : b->spmv) {
+    (void)hipFree(L.desc);
+    (void)hipFree(L.work);
+  }
+  for (auto& L : b->lng) {
+    (void)hipFree(L.ldesc);
+    (void)hipFree(L.work);
+  }
+  (void)hipFree(b->slice_pid_m);
+  (void)hipFree(b->slice_pid_n);
+  (void)hipFree(b->panels);
+  (void)hipFree(b->dprob);
+  (void)hipFree(b->part);
+  (void)hipFree(b->red);
+  if (b->stream) (void)hipStreamDestroy(b->stream);
+  b->magic = 0;
+  delete b;
+}
+
+template <typename X>
+bool upload_vec(const std::vector<X>& h, X** d) {
+  *d = nullptr;
+  if (h.empty()) return true;
+  return hipMalloc(d, sizeof(X) * h.size()) == hipSuccess &&
+         hipMemcpy(*d, h.data(), sizeof(X) * h.size(), hipMemcpyHostToDevice) == hipSuccess;
+}
+
+int batch_create(int nprob, const pk_lbd_t* handles, Batch** out) {
+  if (nprob < 1 || !handles || !out) return PK_ERR_ARG;
+  Batch* B = new Batch();
+  B->magic = Batch::kMagic;
+  B->nprob = nprob;
+  int rc = PK_OK;
+  std::vector<int> spm, spn;
+  std::vector<BPanel> hpan(static_cast<size_t>(nprob) * 2 * kPanelsMax, BPanel{});
+  // SpMV work lists keyed by (tr, panel, lanes per row)
+  struct Key {
+    int tr, p, g;
+    std::vector<int4> desc;
+    std::vector<BWork> work;
+  };
+  std::vector<Key> keys;
+  struct LKey {
+    int tr, p;
+    std::vector<int4> ld;
+    std::vector<BWork> work;
+  };
+  std::vector<LKey> lkeys;
+  B->hprob.resize(nprob);
+  for (int i = 0; i < nprob && rc == PK_OK; ++i) {
+    Engine* e = engine_from_handle(handles[i]);
+    if (!e || !e->op) {
+      set_error("pk_batch_create: problem %d has no device operator", i);
+      rc = PK_ERR_ARG;
+      break;
+    }
+    if (i == 0) {
+      B->prec = e->prec;
+      B->val_prec = e->op->val_prec;
+    } else if (e->prec != B->prec || e->op->val_prec != B->val_prec) {
+      set_error("pk_batch_create: problem %d has another precision", i);
+      rc = PK_ERR_ARG;
+      break;
+    }
+    B->eng.push_back(e);
+    BProb& P = B->hprob[i];
+    std::memset(&P, 0, sizeof(BProb));
+    P.m = e->m;
+    P.n = e->n;
+    P.sm0 = B->Sm;
+    P.nsm = stream_blocks(e->m);
+    P.sn0 = B->Sn;
+    P.nsn = stream_blocks(e->n);
+    B->Sm += P.nsm;
+    B->Sn += P.nsn;
+    for (int s = 0; s < P.nsm; ++s) spm.push_back(i);
+    for (int s = 0; s < P.nsn; ++s) spn.push_back(i);
+    B->maxk = std::max(B->maxk, e->kmax);
+    for (int tr = 0; tr < 2 && rc == PK_OK; ++tr) {
+      const CsrOp::Dir& D = tr ? e->op->dt : e->op->da;
+      (tr ? P.npt : P.npa) = D.np;
+      (tr ? P.yacc_t : P.yacc_a) = D.yacc;
+      for (int p = 0; p < D.np; ++p) {
+        const CsrOp::Panel& pan = D.pan[p];
+        hpan[(static_cast<size_t>(i) * 2 + tr) * kPanelsMax + p] =
+            BPanel{pan.rp, pan.ci, pan.va, pan.part.lsum};
+        std::vector<int4> hd(pan.part.ndesc), hl(pan.part.nlong);
+        if ((!hd.empty() && hipMemcpy(hd.data(), pan.part.desc, sizeof(int4) * hd.size(),
+                                      hipMemcpyDeviceToHost) != hipSuccess) ||
+            (!hl.empty() && hipMemcpy(hl.data(), pan.part.ldesc, sizeof(int4) * hl.size(),
+                                      hipMemcpyDeviceToHost) != hipSuccess)) {
+          rc = PK_ERR_HIP;
+          break;
+        }
+        auto kit = std::find_if(keys.begin(), keys.end(), [&](const Key& k) {
+          return k.tr == tr && k.p == p && k.g == pan.g;
+        });
+        if (kit == keys.end()) {
+          keys.push_back(Key{tr, p, pan.g, {}, {}});
+          kit = keys.end() - 1;
+        }
+        for (int b = 0; b < static_cast<int>(hd.size()); ++b) {
+          kit->desc.push_back(hd[b]);
+          kit->work.push_back(BWork{i, b});
+        }
+        auto lit = std::find_if(lkeys.begin(), lkeys.end(),
+                                [&](const LKey& k) { return k.tr == tr && k.p == p; });
+        if (lit == lkeys.end()) {
+          lkeys.push_back(LKey{tr, p, {}, {}});
+          lit = lkeys.end() - 1;
+        }
+        for (int l = 0; l < static_cast<int>(hl.size()); ++l) {
+          lit->ld.push_back(hl[l]);
+          lit->work.push_back(BWork{i, pan.part.ndesc + l});
+        }
+        if (p == D.np - 1) {
+          const int slots = pan.part.slots();
+          if (tr) {
+            P.qt0 = B->Qt;
+            P.nqt = slots;
+            B->Qt += slots;
+          } else {
+            P.qa0 = B->Qa;
+            P.nqa = slots;
+            B->Qa += slots;
+          }
+        }
+      }
+    }
+  }
+  if (rc == PK_OK && hipStreamCreateWithFlags(&B->stream, hipStreamNonBlocking) != hipSuccess)
+    rc = PK_ERR_HIP;
+  if (rc == PK_OK) {
+    bool ok = upload_vec(spm, &B->slice_pid_m) && upload_vec(spn, &B->slice_pid_n) &&
+              upload_vec(hpan, &B->panels);
+    for (auto& k : keys) {
+      Batch::Launch L{k.tr, k.p, k.g, static_cast<int>(k.desc.size()), nullptr, nullptr};
+      ok = ok && upload_vec(k.desc, &L.desc) && upload_vec(k.work, &L.work);
+      B->spmv.push_back(L);
+    }
+    for (auto& k : lkeys) {
+      Batch::Long L{k.tr, k.p, static_cast<int>(k.ld.size()), nullptr, nullptr};
+      ok = ok && upload_vec(k.ld, &L.ldesc) && upload_vec(k.work, &L.work);
+      B->lng.push_back(L);
+    }
+    const size_t S = static_cast<size_t>(std::max(B->Sm, B->Sn));
+    const size_t Q = static_cast<size_t>(std::max(B->Qa, B->Qt));
+    B->part_len = std::max(2 * Q, static_cast<size_t>(std::max(B->maxk + 1, 2)) * S);
+    ok = ok && hipMalloc(&B->part, sizeof(double) * B->part_len) == hipSuccess &&
+         hipMalloc(&B->red, sizeof(double) * static_cast<size_t>(nprob) * kMaxCols) ==
+             hipSuccess &&
+         hipMalloc(&B->dprob, sizeof(BProb) * nprob) == hipSuccess;
+    if (!ok) rc = PK_ERR_HIP;
+  }
+  if (rc != PK_OK) {
+    batch_free(B);
+    return rc;
+  }
+  *out = B;
+  return PK_OK;
+}
+
+// One batched DLANBPRO call: problem i (active[i] != 0) goes from k0[i] to
+// k[i] steps.  alpha[i] / beta[i]: its B(:,1), B(:,2); doption 4 per problem.
+int batch_lanbpro(Batch& B, const int* active, const int* k0, int* k, double* const* alpha,
+                  double* const* beta, double* rnorm, double* doption, const int* ioption,
+                  int* ierr) {
+  const int np = B.nprob;
+  std::vector<char> run(np, 0), fin(np, 0);
+  int Jmin = 1 << 30, Jmax = 0, elr = 0, tries = 2;
+  for (int i = 0; i < np; ++i) {
+    if (!active[i]) continue;
+    Engine* e = B.eng[i];
+    bool finished = false;
+    const int rc = engine_lanbpro_begin(e, k0[i], k[i], alpha[i], beta[i], &rnorm[i],
+                                        doption + 4 * i, ioption, &ierr[i], &finished);
+    if (rc != PK_OK) return rc;
+    if (finished) {
+      fin[i] = 1;
+      continue;
+    }
+    if (hipStreamSynchronize(e->stream) != hipSuccess) return PK_ERR_HIP;
+    const LbdCtl& c = *e->hctl;
+    if (e->mode == 0 && c.phase == PH_V_SPMV) {
+      run[i] = 1;
+      Jmin = std::min(Jmin, c.j);
+      Jmax = std::max(Jmax, c.k);
+      elr = std::max(elr, c.elr);
+      tries = e->dev_tries;
+    }
+  }
+  bool any = false;
+  for (int i = 0; i < np; ++i) {
+    BProb& P = B.hprob[i];
+    Engine* e = B.eng[i];
+    P.ctl = e->dctl;
+    P.U = e->u();
+    P.V = e->v();
+    P.ldu = e->ldu;
+    P.ldv = e->ldv;
+    if (run[i]) {
+      any = true;
+      if (hipMemcpyAsync(e->dctl, e->hctl, sizeof(LbdCtl), hipMemcpyHostToDevice, B.stream) !=
+          hipSuccess)
+        return PK_ERR_HIP;
+    } else if (hipMemsetAsync(&e->dctl->phase, 0, sizeof(int), B.stream) != hipSuccess) {
+      return PK_ERR_HIP;  // phase 0 gates the problem off
+    }
+  }
+  if (any) {
+    if (hipMemcpyAsync(B.dprob, B.hprob.data(), sizeof(BProb) * np, hipMemcpyHostToDevice,
+                       B.stream) != hipSuccess)
+      return PK_ERR_HIP;
+    for (int J = Jmin; J <= Jmax; ++J) {
+      if (B.prec == PK_F64)
+        enqueue_batch_step<double>(B, J, elr, tries);
+      else
+        enqueue_batch_step<float>(B, J, elr, tries);
+    }
+    if (hipGetLastError() != hipSuccess) return PK_ERR_HIP;
+    for (int i = 0; i < np; ++i)
+      if (run[i] && hipMemcpyAsync(B.eng[i]->hctl, B.eng[i]->dctl, sizeof(LbdCtl),
+                                   hipMemcpyDeviceToHost, B.stream) != hipSuccess)
+        return PK_ERR_HIP;
+  }
+  if (hipStreamSynchronize(B.stream) != hipSuccess) return PK_ERR_HIP;
+  for (int i = 0; i < np; ++i) {
+    if (!active[i] || fin[i]) continue;
+    Engine* e = B.eng[i];
+    if (run[i]) e->cnt.nsync++;
+    // whatever the chain left (rare phases, further CGS passes) runs on the
+    // problem's own engine, then its outputs
+    int rc = engine_lanbpro_loop(e);
+    if (rc == PK_OK) rc = engine_lanbpro_end(e, k0[i], &k[i], alpha[i], beta[i], &rnorm[i],
+                                             doption + 4 * i, &ierr[i]);
+    if (rc != PK_OK) return rc;
+  }
+  return PK_OK;
+}
+
+Batch* as_batch(pk_batch_t h) {
+  Batch* b = reinterpret_cast<Batch*>(h);
+  if (!b || b->magic != Batch::kMagic) {
+    set_error("not a live batch handle");
+    return nullptr;
+  }
+  return b;
+}
+
+}  // namespace
+}  // namespace pk
+
+using namespace pk;
+
+extern "C" {
+
+int pk_batch_create(int nprob, const pk_lbd_t* engines, pk_batch_t* out) {
+  Batch* b = nullptr;
+  const int rc = batch_create(nprob, engines, &b);
+  if (rc == PK_OK) *out = reinterpret_cast<pk_batch_t>(b);
+  return rc;
+}
+
+int pk_batch_destroy(pk_batch_t h) {
+  Batch* b = as_batch(h);
+  if (!b) return PK_ERR_HANDLE;
+  (void)hipStreamSynchronize(b->stream);
+  batch_free(b);
+  return PK_OK;
+}
+
+int pk_batch_lanbpro(pk_batch_t h, const int* active, const int* k0, int* k,
+                     double* const* alpha, double* const* beta, double* rnorm, double* doption,
+                     const int* ioption, int* ierr) {
+  Batch* b = as_batch(h);
+  if (!b) return PK_ERR_HANDLE;
+  if (!active || !k0 || !k || !alpha || !beta || !rnorm || !doption || !ioption || !ierr)
+    return PK_ERR_ARG;
+  return batch_lanbpro(*b, active, k0, k, alpha, beta, rnorm, doption, ioption, ierr);
+}
+
+}  // extern "C"

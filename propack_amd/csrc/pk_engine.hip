@@ -631,9 +631,22 @@ struct Impl {
   // dlanbpro (double/dlanbpro.F:1-549).  alpha_/beta_ are B(:,1), B(:,2).
   int lanbpro(int k0, int* kp, double* alpha_, double* beta_, double* rnorm, double* doption,
               const int* ioption, int* ierr_out) {
+    bool finished = false;
+    PK_TRY(lanbpro_begin(k0, *kp, alpha_, beta_, rnorm, doption, ioption, ierr_out, &finished));
+    if (finished) return PK_OK;
+    PK_TRY(lanbpro_loop());
+    return lanbpro_end(k0, kp, alpha_, beta_, rnorm, doption, ierr_out);
+  }
+
+  // Setup of a DLANBPRO call (double/dlanbpro.F:134-282): the state of the
+  // first step goes to e->hctl.  *finished: the call already returned (as
+  // the reference does for a zero B at k0 > 0).
+  int lanbpro_begin(int k0, int k_in, double* alpha_, double* beta_, double* rnorm,
+                    double* doption, const int* ioption, int* ierr_out, bool* finished) {
+    *finished = false;
     const double one = 1.0, zero = 0.0, FUDGE = 1.01, kappa = 0.717;
     const int m = e->m, n = e->n;
-    int k = *kp, ierr = 0;
+    int k = k_in, ierr = 0;
     const double eps = Mach<T>::eps;
     const double leff = eff_length();
     const double epsn = leff * eps;
@@ -667,6 +680,7 @@ struct Impl {
       if (anorm <= zero) {
         *ierr_out = -1;
         doption[2] = anorm;
+        *finished = true;
         return PK_OK;
       }
     } else {
@@ -758,7 +772,13 @@ struct Impl {
     c.numax = zero;
     c.mumax = zero;
     c.rnorm = *rnorm;
+    return PK_OK;
+  }
 
+  // Steps of the call from the state in e->hctl until it is done: device
+  // chains from PH_V_SPMV, host execution of every other phase.
+  int lanbpro_loop() {
+    LbdCtl& c = *e->hctl;
     const bool device = e->mode == 0 && e->op != nullptr;
     while (c.phase != PH_DONE) {
       if (device && c.phase == PH_V_SPMV) {
@@ -770,10 +790,19 @@ struct Impl {
       }
       PK_TRY(host_phase(c));
     }
+    return PK_OK;
+  }
+
+  // End of the call: outputs in the reference's places (:540-548).
+  int lanbpro_end(int k0, int* kp, double* alpha_, double* beta_, double* rnorm, double* doption,
+                  int* ierr_out) {
+    LbdCtl& c = *e->hctl;
+    const double* B1 = c.B1;
+    const double* B2 = c.B2;
     // The 1/beta scaling of u_{k+1} is deferred to the start of the next
     // step (see the chains); the last step of the call applies it here.
     if (c.uscale) {
-      PK_TRY(scale(ucol(c.j), m, c.uscale_f));
+      PK_TRY(scale(ucol(c.j), e->m, c.uscale_f));
       c.uscale = 0;
     }
 
@@ -832,6 +861,32 @@ struct Impl {
   }
 };
 
+// Pieces of a DLANBPRO call for the batched step engine (pk_batch.hip).
+int engine_lanbpro_begin(Engine* e, int k0, int k, double* alpha, double* beta, double* rnorm,
+                         double* doption, const int* ioption, int* ierr, bool* finished) {
+  if (k > e->kmax || k0 < 0 || k0 > k) {
+    set_error("lanbpro: k=%d k0=%d outside the engine's kmax=%d", k, k0, e->kmax);
+    return PK_ERR_ARG;
+  }
+  if (e->prec == PK_F64)
+    return Impl<double>(e).lanbpro_begin(k0, k, alpha, beta, rnorm, doption, ioption, ierr,
+                                         finished);
+  return Impl<float>(e).lanbpro_begin(k0, k, alpha, beta, rnorm, doption, ioption, ierr,
+                                      finished);
+}
+
+int engine_lanbpro_loop(Engine* e) {
+  if (e->prec == PK_F64) return Impl<double>(e).lanbpro_loop();
+  return Impl<float>(e).lanbpro_loop();
+}
+
+int engine_lanbpro_end(Engine* e, int k0, int* kp, double* alpha, double* beta, double* rnorm,
+                       double* doption, int* ierr) {
+  if (e->prec == PK_F64)
+    return Impl<double>(e).lanbpro_end(k0, kp, alpha, beta, rnorm, doption, ierr);
+  return Impl<float>(e).lanbpro_end(k0, kp, alpha, beta, rnorm, doption, ierr);
+}
+
 static Engine* as_engine(pk_lbd_t h) {
   Engine* e = reinterpret_cast<Engine*>(h);
   if (!e || e->magic != Engine::kMagic) {
@@ -877,6 +932,8 @@ static void engine_free(Engine* e) {
   e->magic = 0;
   delete e;
 }
+
+Engine* engine_from_handle(pk_lbd_t h) { return as_engine(h); }
 
 }  // namespace pk
 

@@ -128,4 +128,13 @@ struct Engine {
   void* v() const { return V[vcur]; }
 };
 
+// A DLANBPRO call in three pieces (setup into e->hctl, the steps, the
+// outputs), so that several engines can run their steps together.
+int engine_lanbpro_begin(Engine* e, int k0, int k, double* alpha, double* beta, double* rnorm,
+                         double* doption, const int* ioption, int* ierr, bool* finished);
+int engine_lanbpro_loop(Engine* e);
+int engine_lanbpro_end(Engine* e, int k0, int* kp, double* alpha, double* beta, double* rnorm,
+                       double* doption, int* ierr);
+Engine* engine_from_handle(pk_lbd_t h);
+
 }  // namespace pk

@@ -311,88 +311,11 @@ int launch_gemm_tall(hipStream_t s, long long L, const T* A, long long lda, int 
   return 0;
 }
 
-// Maximum over the workgroup with the control code's max (exact, so the
-// result does not depend on the order); valid in thread 0.
-__device__ inline double block_max(double v, double* red) {
-  for (int o = kWave / 2; o > 0; o >>= 1) v = hd_max(v, __shfl_xor(v, o));
-  if (threadIdx.x % kWave == 0) red[threadIdx.x / kWave] = v;
-  __syncthreads();
-  if (threadIdx.x == 0)
-    for (int w = 1; w < kBlock / kWave; ++w) v = hd_max(v, red[w]);
-  __syncthreads();
-  return v;
-}
-
-// hd_update_nu (dlanbpro.F:684-723) with its elements spread over the
-// workgroup: every element is computed by the serial form's expression and
-// the maximum is exact, so the results are bit-identical to it.
-__device__ void par_update_nu(LbdCtl& c, double* red) {
-#pragma clang fp contract(off)
-  const int j = c.j;
-  if (j <= 1) return;
-  const double* alpha = c.B1;
-  const double* beta = c.B2;
-  const double* mu = c.mu;
-  double* nu = c.nu;
-  const double eps1 = c.epsn2, anorm = c.anorm;
-  double mx = 0.0;
-  for (int k = 1 + static_cast<int>(threadIdx.x); k <= j - 1; k += kBlock) {
-    double v = beta[k] * mu[k + 1] + alpha[k] * mu[k] - beta[j - 1] * nu[k];
-    const double d =
-        eps1 * (hd_lapy2(alpha[k], beta[k]) + hd_lapy2(alpha[j], beta[j - 1])) + eps1 * anorm;
-    v = (v + hd_sign(d, v)) / alpha[j];
-    nu[k] = v;
-    mx = hd_max(mx, fabs(v));
-  }
-  mx = block_max(mx, red);
-  if (threadIdx.x == 0) {
-    c.numax = mx;
-    nu[j] = 1.0;
-  }
-}
-
-// hd_update_mu (dlanbpro.F:628-679), spread over the workgroup likewise.
-__device__ void par_update_mu(LbdCtl& c, double* red) {
-#pragma clang fp contract(off)
-  const int j = c.j;
-  if (j == 1) {
-    if (threadIdx.x == 0) hd_update_mu(&c.mumax, c.mu, c.nu, j, c.B1, c.B2, c.anorm, c.epsn2);
-    return;
-  }
-  const double* alpha = c.B1;
-  const double* beta = c.B2;
-  const double* nu = c.nu;
-  double* mu = c.mu;
-  const double eps1 = c.epsn2, anorm = c.anorm;
-  double mx = 0.0;
-  for (int k = 1 + static_cast<int>(threadIdx.x); k <= j; k += kBlock) {
-    double v, d;
-    if (k == 1) {
-      v = alpha[1] * nu[1] - alpha[j] * mu[1];
-      d = eps1 * (hd_lapy2(alpha[j], beta[j]) + alpha[1]) + eps1 * anorm;
-    } else if (k < j) {
-      v = alpha[k] * nu[k] + beta[k - 1] * nu[k - 1] - alpha[j] * mu[k];
-      d = eps1 * (hd_lapy2(alpha[j], beta[j]) + hd_lapy2(alpha[k], beta[k - 1])) + eps1 * anorm;
-    } else {
-      v = beta[j - 1] * nu[j - 1];
-      d = eps1 * (hd_lapy2(alpha[j], beta[j]) + hd_lapy2(alpha[j], beta[j - 1])) + eps1 * anorm;
-    }
-    v = (v + hd_sign(d, v)) / beta[j];
-    mu[k] = v;
-    mx = hd_max(mx, fabs(v));
-  }
-  mx = block_max(mx, red);
-  if (threadIdx.x == 0) {
-    c.mumax = mx;
-    mu[j + 1] = 1.0;
-  }
-}
-
 // ------------------------------------------------------------- control ----
 // One workgroup: reduce the partials of the preceding vector kernel (same
-// order as k_reduce), stage the control state in LDS, let thread 0 run the
-// step's scalar logic, write the state back.  Runs only while the state is
-// in (want_phase, want_j); otherwise the whole chain for this step is idle.
+// order as k_reduce), then run the step's control stage (ctl_stage_body).
+// Runs only while the state is in (want_phase, want_j); otherwise the whole
+// chain for this step is idle.
 __global__ __launch_bounds__(kBlock) void k_ctl(LbdCtl* __restrict__ gc, int kind, int want_phase,
                                                 int want_j, const double* __restrict__ part,
                                                 int nb) {
@@ -401,61 +324,7 @@ __global__ __launch_bounds__(kBlock) void k_ctl(LbdCtl* __restrict__ gc, int kin
   if (gc->phase != want_phase || gc->j != want_j) return;
   const bool two = (kind == CTL_V_SPMV || kind == CTL_V_ELR || kind == CTL_U_SPMV ||
                     kind == CTL_U_ELR);
-  // stage the state: every thread issues all its 16-byte loads before the
-  // first LDS store, so the copy costs about one memory latency
-  constexpr int nv = sizeof(LbdCtl) / sizeof(int4);
-  constexpr int per = (nv + kBlock - 1) / kBlock;
-  static_assert(sizeof(LbdCtl) % sizeof(int4) == 0, "state must be whole 16-byte words");
-  const int4* src = reinterpret_cast<const int4*>(gc);
-  int4* dst = reinterpret_cast<int4*>(&c);
-  int4 stage[per];
-#pragma unroll
-  for (int u = 0; u < per; ++u) {
-    const int i = threadIdx.x + u * kBlock;
-    if (i < nv) stage[u] = src[i];
-  }
-  const double a = reduce_partials(part, nb, lds);
-  const double b = two ? reduce_partials(part + nb, nb, lds) : 0.0;
-#pragma unroll
-  for (int u = 0; u < per; ++u) {
-    const int i = threadIdx.x + u * kBlock;
-    if (i < nv) dst[i] = stage[u];
-  }
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    switch (kind) {
-      case CTL_V_SPMV: ctl_v_spmv_s(c, a, b); break;
-      case CTL_V_ELR: ctl_v_elr_s(c, a, b); break;
-      case CTL_V_REO: ctl_v_reo_s(c, a); break;
-      case CTL_U_SPMV: ctl_u_spmv_s(c, a, b); break;
-      case CTL_U_ELR: ctl_u_elr_s(c, a, b); break;
-      case CTL_U_REO: ctl_u_reo_s(c, a); break;
-      default: break;
-    }
-  }
-  __syncthreads();
-  const int pend = c.pending;
-  if (pend != PEND_NONE) {  // uniform: every thread read the same LDS word
-    if (pend == PEND_NU)
-      par_update_nu(c, lds);
-    else
-      par_update_mu(c, lds);
-    __syncthreads();
-    if (threadIdx.x == 0) {
-      c.pending = PEND_NONE;
-      if (pend == PEND_NU)
-        ctl_v_nu_post(c);
-      else
-        ctl_u_mu_post(c);
-    }
-  }
-  __syncthreads();
-  int4* back = reinterpret_cast<int4*>(gc);
-#pragma unroll
-  for (int u = 0; u < per; ++u) {
-    const int i = threadIdx.x + u * kBlock;
-    if (i < nv) back[i] = dst[i];
-  }
+  ctl_stage_body(gc, kind, part, nb, two ? part + nb : nullptr, nb, c, lds);
 }
 
 int launch_ctl(hipStream_t s, LbdCtl* c, int kind, int want_phase, int want_j,

@@ -17,7 +17,7 @@ module pk_drivers
   use pk_api
   implicit none
   private
-  public :: lansvd_core, lansvd_irl_core, add_timing
+  public :: lansvd_core, lansvd_irl_core, add_timing, irl_begin, irl_after, irl_end
 
   ! External LAPACK (host): bidiagonal SVD.
   interface
@@ -32,6 +32,21 @@ module pk_drivers
       double precision :: d(*), e(*), u(ldu, *), vt(ldvt, *), q(*), work(*)
     end subroutine
   end interface
+
+  ! Per-problem state of the restarted driver between DLANBPRO calls.
+  type, public :: irl_state
+    type(c_ptr) :: eng = c_null_ptr
+    character(len=1) :: which = 'L', jobu = 'n', jobv = 'n'
+    integer :: m = 0, n = 0, dim = 0, neig = 0, maxiter = 0, k = 0
+    integer :: iter = 0, nconv = 0, nmax = 0, kold = 0, ierr = 0, nbsvd = 0, nrestart = 0
+    integer :: info = 0
+    double precision :: eps = 0d0, eps34 = 0d0, epsn = 0d0, tol = 0d0, anorm = 0d0
+    double precision :: rnorm = 0d0, dopt(4) = 0d0
+    logical :: small = .false., done = .true., failed = .false.
+    integer(c_long_long) :: st0(8) = 0
+    double precision, allocatable :: alpha(:), beta(:), a1(:), b1(:), wb(:), shift(:), &
+                                     pm(:, :), qm(:, :), wrk(:), sig(:), bd(:)
+  end type
 
 contains
 
@@ -242,7 +257,9 @@ contains
 
   ! Implicitly restarted driver (DLANSVD_IRL control flow,
   ! double/dlansvd_irl.F:164-419).  dim and neig are in/out as in the
-  ! reference.  The engine must have room for dim V columns.
+  ! reference.  The engine must have room for dim V columns.  The body is
+  ! split into irl_begin / irl_after / irl_end around the DLANBPRO call so
+  ! that the batched driver runs the same code per problem.
   subroutine lansvd_irl_core(which, jobu, jobv, m, n, dim, p, neig, maxiter, eng, eps, &
                              sigma, bnd, tolin, doption, ioption, info)
     character(len=1), intent(in) :: which, jobu, jobv
@@ -253,113 +270,165 @@ contains
     double precision, intent(inout) :: sigma(*), bnd(*), doption(*)
     integer, intent(in) :: ioption(*)
     integer, intent(out) :: info
-    double precision, allocatable :: alpha(:), beta(:), a1(:), b1(:), wb(:), shift(:), &
-                                     pm(:, :), qm(:, :), wrk(:)
-    double precision :: eps34, epsn, tol, anorm, rnorm, relgap, dummy(1, 1)
-    integer :: k, i, iter, nconv, kold, nshft, ierr, lapinfo, nbsvd, nrestart, jj
+    type(irl_state), target :: st
     integer(c_int) :: rc, ierr_c, kk_c
-    integer(c_long_long) :: st0(8)
-    logical :: small, vectors
 
-    rc = pk_lbd_stats(eng, st0)
-    eps34 = eps**(3d0 / 4d0)
-    epsn = eps_leff(eng, m, n) * eps / 2d0
-    dim = min(dim, n + 1, m + 1)
-    k = dim - p
-    tol = min(1d0, max(16d0 * eps, tolin))
-    anorm = 0d0
-    info = 0
-    nbsvd = 0
-    nrestart = 0
-    small = lower1(which) == 's'
-    allocate(alpha(dim), beta(dim), a1(dim), b1(dim), wb(dim + 1), shift(dim), &
-             pm(dim + 1, dim + 1), qm(dim, dim), wrk(4 * dim + 8))
-    alpha = 0d0; beta = 0d0
+    call irl_begin(st, which, jobu, jobv, m, n, dim, p, neig, maxiter, eng, eps, tolin, &
+                   doption, ioption)
+    do while (.not. st%done)
+      kk_c = st%dim
+      rc = pk_lbd_lanbpro(eng, st%kold, kk_c, st%alpha, st%beta, st%rnorm, st%dopt, ioption, &
+                          ierr_c)
+      if (rc /= PK_OK) then
+        st%info = rc
+        st%done = .true.
+        st%failed = .true.
+        exit
+      end if
+      call irl_after(st, kk_c, ierr_c)
+    end do
+    call irl_end(st, sigma, bnd, dim, neig, info, doption)
+  end subroutine
 
-    rc = pk_lbd_nrm2(eng, 'u', 1, rnorm)
+  ! Driver setup up to the first DLANBPRO call (double/dlansvd_irl.F:160-206).
+  subroutine irl_begin(st, which, jobu, jobv, m, n, dim, p, neig, maxiter, eng, eps, tolin, &
+                       doption, ioption)
+    type(irl_state), intent(inout) :: st
+    character(len=1), intent(in) :: which, jobu, jobv
+    integer, intent(in) :: m, n, dim, p, neig, maxiter
+    type(c_ptr), intent(in) :: eng
+    double precision, intent(in) :: eps, tolin
+    double precision, intent(in) :: doption(*)
+    integer, intent(in) :: ioption(*)
+    integer(c_int) :: rc, ierr_c
+
+    st%eng = eng
+    st%which = which
+    st%jobu = jobu
+    st%jobv = jobv
+    st%m = m
+    st%n = n
+    st%neig = neig
+    st%maxiter = maxiter
+    st%eps = eps
+    rc = pk_lbd_stats(eng, st%st0)
+    st%eps34 = eps**(3d0 / 4d0)
+    st%epsn = eps_leff(eng, m, n) * eps / 2d0
+    st%dim = min(dim, n + 1, m + 1)
+    st%k = st%dim - p
+    st%tol = min(1d0, max(16d0 * eps, tolin))
+    st%anorm = 0d0
+    st%info = 0
+    st%nbsvd = 0
+    st%nrestart = 0
+    st%small = lower1(which) == 's'
+    st%dopt = doption(1:4)
+    allocate(st%alpha(st%dim), st%beta(st%dim), st%a1(st%dim), st%b1(st%dim), &
+             st%wb(st%dim + 1), st%shift(st%dim), st%pm(st%dim + 1, st%dim + 1), &
+             st%qm(st%dim, st%dim), st%wrk(4 * st%dim + 8), st%sig(max(neig, 1)), &
+             st%bd(max(neig, 1)))
+    st%alpha = 0d0
+    st%beta = 0d0
+    st%sig = 0d0
+    st%bd = 0d0
+    st%iter = 0
+    st%nconv = 0
+    st%nmax = 0
+    st%kold = 0
+    st%ierr = 0
+    st%done = .false.
+    st%failed = .false.
+
+    rc = pk_lbd_nrm2(eng, 'u', 1, st%rnorm)
+    if (rc == PK_OK .and. st%rnorm == 0d0) &
+      rc = pk_lbd_getu0(eng, 'n', 0, 1, st%rnorm, ioption(1), st%anorm, ierr_c)
     if (rc /= PK_OK) then
-      info = rc
+      st%info = rc
+      st%done = .true.
+      st%failed = .true.
       return
     end if
-    if (rnorm == 0d0) then
-      rc = pk_lbd_getu0(eng, 'n', 0, 1, rnorm, ioption(1), anorm, ierr_c)
-      if (rc /= PK_OK) then
-        info = rc
-        return
-      end if
-    end if
+    st%done = .not. (st%nconv < st%neig .and. st%iter < st%maxiter)
+  end subroutine
 
-    iter = 0
-    nconv = 0
-    kold = 0
-    ierr = 0
-    do while (nconv < neig .and. iter < maxiter)
-      kk_c = dim
-      rc = pk_lbd_lanbpro(eng, kold, kk_c, alpha, beta, rnorm, doption, ioption, ierr_c)
-      if (rc /= PK_OK) then
-        info = rc
-        return
-      end if
-      ierr = ierr_c
-      ! DLANBPRO lowers its K argument (DIM here) when it stops at an
-      ! invariant subspace (double/dlanbpro.F:389,519; double/dlansvd_irl.F:213)
-      dim = kk_c
-      kold = k
+  ! One pass of the restart loop after its DLANBPRO call returned dim (kk)
+  ! and ierr (double/dlansvd_irl.F:213-414).
+  subroutine irl_after(st, kk, ierr_c)
+    type(irl_state), intent(inout) :: st
+    integer(c_int), intent(in) :: kk, ierr_c
+    double precision :: relgap, dummy(1, 1)
+    integer :: i, nshft, lapinfo, dim, k, neig, m, n
+    integer(c_int) :: rc
+
+    st%ierr = ierr_c
+    ! DLANBPRO lowers its K argument (DIM here) when it stops at an
+    ! invariant subspace (double/dlanbpro.F:389,519; double/dlansvd_irl.F:213)
+    st%dim = kk
+    st%kold = st%k
+    dim = st%dim
+    k = st%k
+    neig = st%neig
+    m = st%m
+    n = st%n
+    associate (a1 => st%a1, b1 => st%b1, wb => st%wb, alpha => st%alpha, beta => st%beta, &
+               shift => st%shift, pm => st%pm, qm => st%qm)
       ! SVD of B_dim and error bounds (double/dlansvd_irl.F:220-251)
       a1 = alpha
       b1 = beta
       wb = 0d0
       call pk_bdqr(dim == min(m, n), 'N', dim, a1, b1, wb(dim), wb(dim + 1), dummy, 1)
-      call dbdsqr('U', dim, 0, 1, 0, a1, b1, dummy, 1, wb, 1, dummy, 1, wrk, lapinfo)
-      nbsvd = nbsvd + 1
+      call dbdsqr('U', dim, 0, 1, 0, a1, b1, dummy, 1, wb, 1, dummy, 1, st%wrk, lapinfo)
+      st%nbsvd = st%nbsvd + 1
       if (dim > 5) then
-        anorm = a1(1)
+        st%anorm = a1(1)
       else
-        anorm = max(anorm, a1(1))
+        st%anorm = max(st%anorm, a1(1))
       end if
-      wb(1:dim) = abs(rnorm * wb(1:dim))
-      if (small) then
-        call pk_refinebounds(min(m, n), dim, a1, wb, epsn * anorm, eps34)
+      wb(1:dim) = abs(st%rnorm * wb(1:dim))
+      if (st%small) then
+        call pk_refinebounds(min(m, n), dim, a1, wb, st%epsn * st%anorm, st%eps34)
       else
-        call pk_refinebounds(min(m, n), min(dim, neig), a1, wb, epsn * anorm, eps34)
+        call pk_refinebounds(min(m, n), min(dim, neig), a1, wb, st%epsn * st%anorm, st%eps34)
       end if
       ! converged count (double/dlansvd_irl.F:262-290)
-      nconv = 0
-      if (small) then
+      st%nconv = 0
+      if (st%small) then
         do i = dim - neig + 1, dim
-          if (wb(i) <= tol * a1(1)) then
-            nconv = nconv + 1
-            sigma(nconv) = a1(i)
-            bnd(nconv) = wb(i)
+          if (wb(i) <= st%tol * a1(1)) then
+            st%nconv = st%nconv + 1
+            st%sig(st%nconv) = a1(i)
+            st%bd(st%nconv) = wb(i)
           end if
         end do
       else
         i = 1
         do while (i <= min(dim, neig))
-          if (wb(i) <= tol * a1(i)) then
-            nconv = nconv + 1
-            sigma(nconv) = a1(i)
-            bnd(nconv) = wb(i)
+          if (wb(i) <= st%tol * a1(i)) then
+            st%nconv = st%nconv + 1
+            st%sig(st%nconv) = a1(i)
+            st%bd(st%nconv) = wb(i)
             i = i + 1
           else
             exit
           end if
         end do
       end if
-      call pk_f_progress(iter, nconv, neig, a1(1), rnorm)
-      if (ierr < 0) then
-        if (dim < k) info = dim
-        exit
+      st%nmax = max(st%nmax, st%nconv)
+      call pk_f_progress(st%iter, st%nconv, neig, a1(1), st%rnorm)
+      if (st%ierr < 0) then
+        if (dim < k) st%info = dim
+        st%done = .true.
+        return
       end if
-      if (nconv < neig) then
+      if (st%nconv < neig) then
         ! exact shifts filtered by the relative gap (double/dlansvd_irl.F:318-344)
         shift = 0d0
         nshft = 0
-        if (small) then
+        if (st%small) then
           do i = 1, k
             relgap = a1(i) - wb(i) - a1(dim - neig)
             nshft = nshft + 1
-            if (relgap > doption(4) * a1(dim - neig)) then
+            if (relgap > st%dopt(4) * a1(dim - neig)) then
               shift(nshft) = a1(i)
             else
               shift(nshft) = a1(1)
@@ -369,7 +438,7 @@ contains
           do i = dim, k + 1, -1
             relgap = a1(k) - (a1(i) + wb(i))
             nshft = nshft + 1
-            if (relgap > doption(4) * a1(k)) then
+            if (relgap > st%dopt(4) * a1(k)) then
               shift(nshft) = a1(i)
             else
               shift(nshft) = 0d0
@@ -387,30 +456,55 @@ contains
         end do
         do i = dim, k + 1, -1
           call pk_bsvdstep('y', 'y', dim + 1, dim, i, shift(dim - i + 1), alpha, beta, pm, &
-                           dim + 1, qm, dim)
+                           size(pm, 1), qm, size(qm, 1))
         end do
         ! U(:,1:k+1) <- U(:,1:dim+1) P(:,1:k+1); V(:,1:k) <- V(:,1:dim) Q(:,1:k)
-        rc = pk_lbd_update(eng, 'u', 'n', dim + 1, k + 1, pm, dim + 1)
-        if (rc == PK_OK) rc = pk_lbd_update(eng, 'v', 'n', dim, k, qm, dim)
+        rc = pk_lbd_update(st%eng, 'u', 'n', dim + 1, k + 1, pm, size(pm, 1))
+        if (rc == PK_OK) rc = pk_lbd_update(st%eng, 'v', 'n', dim, k, qm, size(qm, 1))
         if (rc /= PK_OK) then
-          info = rc
+          st%info = rc
+          st%done = .true.
+          st%failed = .true.
           return
         end if
-        rnorm = beta(k)
-        nrestart = nrestart + 1
+        st%rnorm = beta(k)
+        st%nrestart = st%nrestart + 1
       end if
-      iter = iter + 1
-    end do
+    end associate
+    st%iter = st%iter + 1
+    st%done = .not. (st%nconv < st%neig .and. st%iter < st%maxiter)
+  end subroutine
 
-    vectors = lower1(jobu) == 'y' .or. lower1(jobv) == 'y'
-    if ((nconv >= neig .or. info > 0) .and. vectors) then
-      a1 = alpha
-      b1 = beta
-      call ritzvec(which, jobu, jobv, m, n, nconv, dim, a1, b1, eng, jj)
-      if (jj /= 0) info = jj
+  ! Ritz vectors and outputs (double/dlansvd_irl.F:396-418).
+  subroutine irl_end(st, sigma, bnd, dim, neig, info, doption)
+    type(irl_state), intent(inout) :: st
+    double precision, intent(inout) :: sigma(*), bnd(*), doption(*)
+    integer, intent(inout) :: dim, neig
+    integer, intent(out) :: info
+    integer :: jj
+    logical :: vectors
+
+    doption(1:4) = st%dopt
+    info = st%info
+    if (st%failed) return  ! an engine error: outputs as they were
+    if (.true.) then
+      vectors = lower1(st%jobu) == 'y' .or. lower1(st%jobv) == 'y'
+      if ((st%nconv >= st%neig .or. st%info > 0) .and. vectors) then
+        st%a1 = st%alpha
+        st%b1 = st%beta
+        call ritzvec(st%which, st%jobu, st%jobv, st%m, st%n, st%nconv, st%dim, st%a1, st%b1, &
+                     st%eng, jj)
+        if (jj /= 0) st%info = jj
+      end if
+      call add_timing(st%eng, st%st0, st%nbsvd, st%nrestart, st%dim, -1)
     end if
-    neig = nconv
-    call add_timing(eng, st0, nbsvd, nrestart, dim, -1)
+    if (st%nmax > 0) then
+      sigma(1:st%nmax) = st%sig(1:st%nmax)
+      bnd(1:st%nmax) = st%bd(1:st%nmax)
+    end if
+    dim = st%dim
+    neig = st%nconv
+    info = st%info
   end subroutine
 
 end module pk_drivers
@@ -479,6 +573,94 @@ integer(c_int) function pk_lansvd_irl_run(eng, which, jobu, jobv, dim, p, neig, 
 end function
 
 ! Shape and storage unit roundoff of an engine.
+! Batched restarted driver (no reference counterpart; replaces a loop of
+! DLANSVD_IRL calls over independent problems, double/dlansvd_irl.F:1-3):
+! nb engines of one batch (pk_batch_create) run DLANSVD_IRL's control flow
+! each, their Lanczos steps advancing together through pk_batch_lanbpro.
+! sigma, bnd: ldsig x nb; per problem nconv(i) and info(i).  doption(1:4)
+! and ioption as for one problem (each problem gets its own doption copy).
+integer(c_int) function pk_lansvd_irl_batch(nb, engs, batch, which, jobu, jobv, dim, p, neig, &
+                                            maxiter, tolin, sigma, bnd, ldsig, nconv, info, &
+                                            doption, ioption) &
+    bind(C, name='pk_lansvd_irl_batch')
+  use iso_c_binding
+  use pk_api
+  use pk_drivers
+  implicit none
+  integer(c_int), value :: nb, dim, p, neig, maxiter, ldsig
+  type(c_ptr) :: engs(nb)
+  type(c_ptr), value :: batch
+  character(kind=c_char), value :: which, jobu, jobv
+  real(c_double), value :: tolin
+  real(c_double) :: sigma(ldsig, nb), bnd(ldsig, nb), doption(4)
+  integer(c_int) :: nconv(nb), info(nb), ioption(*)
+  interface
+    integer(c_int) function pk_batch_lanbpro(b, active, k0, k, alpha, beta, rnorm, dopt, &
+                                             iopt, ierr) bind(C, name='pk_batch_lanbpro')
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: b
+      integer(c_int) :: active(*), k0(*), k(*), iopt(*), ierr(*)
+      type(c_ptr) :: alpha(*), beta(*)
+      real(c_double) :: rnorm(*), dopt(4, *)
+    end function
+  end interface
+  type(irl_state), allocatable, target :: st(:)
+  integer(c_int), allocatable :: active(:), k0(:), kk(:), ierr(:)
+  type(c_ptr), allocatable :: ap(:), bp(:)
+  real(c_double), allocatable :: rn(:), dop(:, :)
+  integer :: i, m, n, rc, dd, ne
+  double precision :: eps, dummy(4)
+
+  pk_lansvd_irl_batch = PK_OK
+  allocate(st(nb), active(nb), k0(nb), kk(nb), ierr(nb), ap(nb), bp(nb), rn(nb), dop(4, nb))
+  do i = 1, nb
+    call engine_shape(engs(i), m, n, eps, rc)
+    if (rc /= 0) then
+      pk_lansvd_irl_batch = rc
+      return
+    end if
+    call irl_begin(st(i), which, jobu, jobv, m, n, dim, p, neig, maxiter, engs(i), eps, tolin, &
+                   doption, ioption)
+  end do
+  do while (any(.not. st(:)%done))
+    do i = 1, nb
+      active(i) = merge(1, 0, .not. st(i)%done)
+      k0(i) = st(i)%kold
+      kk(i) = st(i)%dim
+      ap(i) = c_loc(st(i)%alpha)
+      bp(i) = c_loc(st(i)%beta)
+      rn(i) = st(i)%rnorm
+      dop(:, i) = st(i)%dopt
+      ierr(i) = 0
+    end do
+    rc = pk_batch_lanbpro(batch, active, k0, kk, ap, bp, rn, dop, ioption, ierr)
+    if (rc /= PK_OK) then
+      pk_lansvd_irl_batch = rc
+      do i = 1, nb
+        if (active(i) /= 0) then
+          st(i)%info = rc
+          st(i)%failed = .true.
+          st(i)%done = .true.
+        end if
+      end do
+      exit
+    end if
+    do i = 1, nb
+      if (active(i) == 0) cycle
+      st(i)%rnorm = rn(i)
+      st(i)%dopt = dop(:, i)
+      call irl_after(st(i), kk(i), ierr(i))
+    end do
+  end do
+  do i = 1, nb
+    dd = dim
+    ne = neig
+    dummy = doption
+    call irl_end(st(i), sigma(:, i), bnd(:, i), dd, ne, info(i), dummy)
+    nconv(i) = ne
+  end do
+end function
+
 subroutine engine_shape(eng, m, n, eps, info)
   use iso_c_binding
   implicit none
