@@ -193,6 +193,17 @@ int pk_lansvd_irl_mixed(pk_lbd_t eng64, pk_lbd_t eng32, char which, char jobu, c
                         int* dim, int p, int* neig, int maxiter, double* sigma, double* bnd,
                         double* sigma32, double* bnd32, double tolin, double* doption,
                         int* ioption, int* info32);
+/* DLANSVD_IRL on every engine of a batch (engs[i] in pk_batch_create's
+ * order; start vectors their U(:,1)); no reference counterpart, it replaces
+ * a loop of DLANSVD_IRL calls (double/dlansvd_irl.F:1-3) over independent
+ * problems.  Every problem runs the reference's control flow on its own;
+ * their Lanczos steps advance together.  sigma/bnd are ldsig x nb
+ * (column-major, column i for problem i); nconv[i], info[i] per problem.
+ * Returns PK_OK or the first PK_ERR_* code. */
+int pk_lansvd_irl_batch(int nb, pk_lbd_t* engs, pk_batch_t batch, char which, char jobu,
+                        char jobv, int dim, int p, int neig, int maxiter, double tolin,
+                        double* sigma, double* bnd, int ldsig, int* nconv, int* info,
+                        double* doption, int* ioption);
 
 /* ---------------------------------------------------------------------- */
 /* Host scalar routines, exported so their results can be compared with the

@@ -36,7 +36,8 @@ EXPORTS = [
     "pk_lansvd_irl_run", "pk_lansvd_irl_mixed", "pk_lbd_seed", "pk_lbd_set_mode", "pk_lbd_profile", "pk_lbd_profile_get", "pk_lbd_profile_reset",
     "pk_larnv_pm1", "pkh_update_mu", "pkh_update_nu", "pkh_compute_int", "pkh_set_mu",
     "pkh_lartg", "pkh_bdqr", "pkh_bsvdstep", "pkh_refinebounds", "dlansvd_", "dlansvd_irl_", "slansvd_", "slansvd_irl_",
-    "clearstat_", "printstat_",
+    "clearstat_", "printstat_", "pk_lbd_set_rounding", "pk_lbd_eps", "pk_batch_create",
+    "pk_batch_destroy", "pk_batch_lanbpro", "pk_lansvd_irl_batch",
 ]
 
 _lib = None
@@ -288,6 +289,11 @@ class Engine:
     def reset_stats(self):
         lib().pk_lbd_reset_stats(self.handle)
 
+    def set_rounding(self, model: int):
+        """0: the reference's thresholds (max(m, n) * eps); 1: storage
+        rounding only (FP32 storage with FP64 accumulation)."""
+        _check(lib().pk_lbd_set_rounding(self.handle, C.c_int(model)), "pk_lbd_set_rounding")
+
     KCLASSES = ["spmv_n", "spmv_t", "gemvt", "gemv", "gemm", "vec"]
 
     def profile(self, enable: bool = True):
@@ -338,6 +344,66 @@ def lansvd_irl_dev(eng: Engine, neig, dim, p=None, tol=1e-10, which="L", jobu="y
     V = eng.get("v", k) if (fetch and jobv == "y") else None
     return SvdResult(sigma[:k].copy(), bnd[:k].copy(), U, V, info, k, timing(), dim=dd.value,
                      anorm=float(dopt[2]))
+
+
+class Batch:
+    """Engines whose Lanczos steps run together (pk_batch_create)."""
+
+    def __init__(self, engines):
+        self.engines = list(engines)
+        arr = (C.c_void_p * len(self.engines))(*[e.handle for e in self.engines])
+        h = C.c_void_p()
+        _check(lib().pk_batch_create(C.c_int(len(self.engines)), arr, C.byref(h)),
+               "pk_batch_create")
+        self.handle = h
+
+    def close(self):
+        if self.handle:
+            lib().pk_batch_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def lansvd_irl_batch(batch: Batch, neig, dim, p=None, tol=1e-10, which="L", jobu="n", jobv="n",
+                     maxiter=1000, delta=None, eta=None, anorm=0.0, min_relgap=0.002, cgs=1,
+                     elr=1, fetch=False):
+    """DLANSVD_IRL on every engine of ``batch`` (start vectors = each
+    engine's U(:,1)); their Lanczos steps advance together.  Returns one
+    SvdResult per problem (vectors fetched when ``fetch``)."""
+    engs = batch.engines
+    nb = len(engs)
+    eps = 2.0 ** -53 if engs[0].prec == PK_F64 else 2.0 ** -24
+    if p is None:
+        p = dim - neig
+    ld = dim + 1
+    sigma = np.zeros((nb, ld))
+    bnd = np.zeros((nb, ld))
+    nconv = np.zeros(nb, np.int32)
+    info = np.zeros(nb, np.int32)
+    dopt = np.array([np.sqrt(2 * eps) if delta is None else delta,
+                     (2 * eps) ** 0.75 if eta is None else eta, anorm, min_relgap])
+    iopt = np.array([cgs, elr], np.int32)
+    arr = (C.c_void_p * nb)(*[e.handle for e in engs])
+    clearstat()
+    rc = lib().pk_lansvd_irl_batch(C.c_int(nb), arr, batch.handle, C.c_char(which.encode()),
+                                   C.c_char(jobu.encode()), C.c_char(jobv.encode()),
+                                   C.c_int(dim), C.c_int(p), C.c_int(neig), C.c_int(maxiter),
+                                   C.c_double(tol), _p(sigma), _p(bnd), C.c_int(ld), _p(nconv),
+                                   _p(info), _p(dopt), _p(iopt))
+    _check(rc, "pk_lansvd_irl_batch")
+    out = []
+    for i, e in enumerate(engs):
+        k = int(nconv[i])
+        U = e.get("u", k) if (fetch and jobu == "y") else None
+        V = e.get("v", k) if (fetch and jobv == "y") else None
+        out.append(SvdResult(sigma[i, :k].copy(), bnd[i, :k].copy(), U, V, int(info[i]), k,
+                             {"nopx": e.stats()["nopx"]}))
+    return out
 
 
 def lansvd_dev(eng: Engine, k, kmax, tol=1e-12, jobu="y", jobv="y", delta=None, eta=None,

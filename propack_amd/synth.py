@@ -161,6 +161,27 @@ def powerlaw_bipartite(m: int = 8_000_000, n: int = 4_000_000,
     return from_coo(m, n, rows, cols, vals).astype(dtype)
 
 
+def batch_params(count: int = 256, m_lo: int = 5_000, m_hi: int = 500_000,
+                 seed: int = 4096) -> list:
+    """Shapes of the batched workload, one dict per problem: m log-uniform
+    in [m_lo, m_hi], n = round(m * r) with r ~ U[0.25, 1], Poisson(lam)
+    nonzeros per row with lam ~ U[4, 48], column decay p ~ U[0.3, 1.0], and
+    the seed of its poisson_decay matrix."""
+    g = _rng(seed)
+    m = np.exp(g.uniform(np.log(m_lo), np.log(m_hi), count)).round().astype(np.int64)
+    r = g.uniform(0.25, 1.0, count)
+    n = np.maximum(1, np.round(m * r)).astype(np.int64)
+    lam = g.uniform(4.0, 48.0, count)
+    p = g.uniform(0.3, 1.0, count)
+    return [dict(m=int(m[i]), n=int(n[i]), lam=float(lam[i]), p=float(p[i]),
+                 seed=int(seed + 1 + i)) for i in range(count)]
+
+
+def batch_problem(prm: dict, dtype=np.float64) -> Csr:
+    """The matrix of one batched problem (see batch_params)."""
+    return poisson_decay(prm["m"], prm["n"], prm["lam"], prm["p"], seed=prm["seed"], dtype=dtype)
+
+
 def start_vector(m: int, seed: int, dtype=np.float64) -> np.ndarray:
     """Seeded N(0,1) start vector of length m."""
     return _rng(seed).standard_normal(m).astype(dtype)

@@ -1,0 +1,95 @@
+"""Batched restarted solves (pk_lansvd_irl_batch): independent problems of
+uneven sizes whose Lanczos steps run together, one launch per kernel per
+step.  Every problem runs the same kernels bodies on the same rows and
+slots as alone, so the batch must reproduce the per-problem engine's
+results bit for bit (same singular values, same operator count); the
+per-problem engine in turn matches the reference DLANSVD_IRL on the same
+matrix and start vector (to 1e-10 relative, the restarted solve's
+tolerance on the FP64 path), and the dense SVD."""
+import numpy as np
+import pytest
+
+from oracle import refpy
+from propack_amd import api, synth
+
+pytestmark = pytest.mark.gpu
+
+NEIG, DIM, TOL = 16, 48, 1e-8
+
+
+def _problems(count, m_lo, m_hi, seed, prec=api.PK_F64):
+    out = []
+    for i, prm in enumerate(synth.batch_params(count, m_lo, m_hi, seed=seed)):
+        A = synth.batch_problem(prm, dtype=np.float64 if prec == api.PK_F64 else np.float32)
+        op = api.CsrOperator(A, prec=prec)
+        eng = api.Engine(op, kmax=DIM, prec=prec)
+        out.append((prm, A, op, eng, synth.start_vector(A.m, prm["seed"] + 7)))
+    return out
+
+
+def _sequential(probs):
+    res = []
+    for prm, A, op, eng, u0 in probs:
+        eng.put_u1(u0)
+        eng.reset_stats()
+        r = api.lansvd_irl_dev(eng, NEIG, DIM, tol=TOL, jobu="n", jobv="n", fetch=False)
+        res.append((r, eng.stats()["nopx"]))
+    return res
+
+
+def _batched(probs):
+    for prm, A, op, eng, u0 in probs:
+        eng.put_u1(u0)
+        eng.reset_stats()
+    b = api.Batch([p[3] for p in probs])
+    out = api.lansvd_irl_batch(b, NEIG, DIM, tol=TOL)
+    b.close()
+    return out
+
+
+def test_batch_matches_sequential_fp64():
+    probs = _problems(6, 2_000, 60_000, seed=7)
+    seq = _sequential(probs)
+    bat = _batched(probs)
+    for (r, nopx), rb, p in zip(seq, bat, probs):
+        print(p[0], r.k, rb.k, nopx, rb.counters["nopx"])
+        assert r.info == rb.info == 0 and r.k == rb.k == NEIG
+        assert np.array_equal(r.sigma, rb.sigma)
+        assert nopx == rb.counters["nopx"]
+    # the per-problem engine against the reference on two of them
+    for (prm, A, op, eng, u0), (r, _) in list(zip(probs, seq))[:2]:
+        ref = refpy.lansvd_irl(A, NEIG, DIM, u0=u0, tol=TOL, cgs=1, jobu="n", jobv="n")
+        assert ref.k == r.k
+        np.testing.assert_allclose(r.sigma, ref.sigma, rtol=1e-10)
+
+
+def test_batch_fp32_basis():
+    """FP32 values and bases (FP64 accumulation, storage-rounding
+    thresholds): bit-identical to the per-problem FP32 engine; singular
+    values within FP32-storage accuracy of the FP64 dense SVD."""
+    probs = _problems(4, 2_000, 20_000, seed=11, prec=api.PK_F32)
+    for p in probs:
+        p[3].set_rounding(1)
+    seq = _sequential(probs)
+    bat = _batched(probs)
+    for (r, nopx), rb, (prm, A, op, eng, u0) in zip(seq, bat, probs):
+        assert r.k == rb.k == NEIG
+        assert np.array_equal(r.sigma, rb.sigma) and nopx == rb.counters["nopx"]
+        s = np.linalg.svd(A.astype(np.float64).todense(), compute_uv=False)[:NEIG]
+        np.testing.assert_allclose(rb.sigma, s, rtol=2e-5)
+
+
+def test_batch_small_and_uneven():
+    """A problem smaller than the Lanczos dimension (invariant subspace at
+    step m, host-side fallback of that problem) among larger ones."""
+    probs = _problems(3, 3_000, 30_000, seed=21)
+    A = synth.random_uniform(40, 30, 600, seed=3)
+    op = api.CsrOperator(A)
+    probs.append(({"m": 40}, A, op, api.Engine(op, kmax=DIM), synth.start_vector(40, 5)))
+    seq = _sequential(probs)
+    bat = _batched(probs)
+    for (r, nopx), rb in zip(seq, bat):
+        assert r.info == rb.info and r.k == rb.k
+        assert np.array_equal(r.sigma, rb.sigma) and nopx == rb.counters["nopx"]
+    s = np.linalg.svd(A.todense(), compute_uv=False)
+    np.testing.assert_allclose(bat[-1].sigma, s[: bat[-1].k], rtol=1e-10)
