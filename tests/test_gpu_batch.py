@@ -67,7 +67,7 @@ def test_batch_fp32_basis():
     """FP32 values and bases (FP64 accumulation, storage-rounding
     thresholds): bit-identical to the per-problem FP32 engine; singular
     values within FP32-storage accuracy of the FP64 dense SVD."""
-    probs = _problems(4, 2_000, 20_000, seed=11, prec=api.PK_F32)
+    probs = _problems(4, 1_000, 3_000, seed=11, prec=api.PK_F32)
     for p in probs:
         p[3].set_rounding(1)
     seq = _sequential(probs)
