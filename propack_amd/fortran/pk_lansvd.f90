@@ -740,7 +740,13 @@ integer(c_int) function pk_lansvd_irl_mixed(eng64, eng32, which, jobu, jobv, dim
   rc = pk_lbd_set_rounding(eng32, 1)
   dd = dim
   ne32 = neig
-  dopt32 = doption(1:4)
+  ! the FP32 phase keeps its basis semi-orthogonal at its own precision:
+  ! delta = sqrt(eps32), eta = eps32**(3/4) (the reference's defaults for
+  ! SLANSVD_IRL callers), the caller's relative-gap filter
+  dopt32(1) = sqrt(eps32)
+  dopt32(2) = eps32**0.75d0
+  dopt32(3) = 0d0
+  dopt32(4) = doption(4)
   call lansvd_irl_core(which, 'y', 'n', m, n, dd, p, ne32, maxiter, eng32, eps32, sigma32, &
                        bnd32, max(tolin, 16d0 * eps32), dopt32, ioption, info)
   info32(1) = info

@@ -246,3 +246,29 @@ def test_invariant_subspace_lowers_dim(prec):
     rt = 1e-5 if prec == "s" else 1e-12
     np.testing.assert_allclose(np.sort(r.sigma), np.sort(s)[:3], rtol=rt)
     np.testing.assert_allclose(r.sigma, ref.sigma, rtol=rt)
+
+
+def test_powerlaw_fp32_basis_polish():
+    """The power-law bipartite shape (row degrees ~ d^-2.1, Zipf(1.8)
+    columns: rows of A^T with tens of thousands of nonzeros) at 400k x 200k,
+    k=16, dim=48, tol=1e-8, in the FP32-basis form with FP64 polish: the
+    FP32 phase within 1e-5 sigma_1 of the reference's FP64 values, the polished
+    values within 1e-10, residuals within the tolerance."""
+    A = synth.powerlaw_bipartite(400_000, 200_000, seed=3, dtype=np.float64)
+    u0 = synth.start_vector(A.m, 9)
+    e64 = api.Engine(api.CsrOperator(A), kmax=48)
+    op32 = api.CsrOperator(A.astype(np.float32), prec=api.PK_F32)
+    e32 = api.Engine(op32, kmax=48)
+    e32.put_u1(u0)
+    r, f32 = api.lansvd_irl_mixed(e64, e32, 16, 48, tol=1e-8)
+    refpy.set_threads(8)
+    ref = refpy.lansvd_irl(A, 16, 48, u0=u0, tol=1e-8, cgs=1, jobu="n", jobv="n")
+    print("fp32", f32["k"], f32["info"], "polished", r.k, r.info, "ref", ref.k, ref.counters)
+    assert ref.k == 16 and r.k == 16 and f32["k"] == 16
+    # FP32 storage: errors of order eps32 * ||A|| (sigma_1 = 2.9e4 dominates here)
+    np.testing.assert_allclose(f32["sigma"], ref.sigma, rtol=0, atol=1e-5 * ref.sigma[0])
+    np.testing.assert_allclose(r.sigma, ref.sigma, rtol=1e-10)
+    op = api.CsrOperator(A)
+    for i in range(r.k):
+        res = np.linalg.norm(op.spmv(np.ascontiguousarray(r.V[:, i]), "n") - r.sigma[i] * r.U[:, i])
+        assert res <= 1e-7 * r.sigma[0]
