@@ -153,7 +153,8 @@ int pk_batch_lanbpro(pk_batch_t batch, const int* active, const int* k0, int* k,
  * further passes continue on the host.  mode 1 runs every decision on the
  * host after a synchronization (same control code, same results).  The
  * host-callback operator always uses mode 1.  Environment PK_LBD_MODE=host
- * selects mode 1 at creation. */
+ * selects mode 1 at creation.  Without a call, mode 0 holds one CGS pass
+ * per side and moves to two once a step needed a second pass. */
 int pk_lbd_set_mode(pk_lbd_t eng, int mode, int tries);
 /* U(:,1) of the double engine dst <- sum of the first ncols columns of
  * src's U (any precision): the start vector of the FP64 polish that follows

@@ -519,7 +519,7 @@ int batch_lanbpro(Batch& B, const int* active, const int* k0, int* k, double* co
                   int* ierr) {
   const int np = B.nprob;
   std::vector<char> run(np, 0), fin(np, 0);
-  int Jmin = 1 << 30, Jmax = 0, elr = 0, tries = 2;
+  int Jmin = 1 << 30, Jmax = 0, elr = 0, tries = 1;
   for (int i = 0; i < np; ++i) {
     if (!active[i]) continue;
     Engine* e = B.eng[i];
@@ -538,7 +538,7 @@ int batch_lanbpro(Batch& B, const int* active, const int* k0, int* k, double* co
       Jmin = std::min(Jmin, c.j);
       Jmax = std::max(Jmax, c.k);
       elr = std::max(elr, c.elr);
-      tries = e->dev_tries;
+      tries = std::max(tries, e->dev_tries);
     }
   }
   bool any = false;

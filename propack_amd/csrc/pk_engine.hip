@@ -788,6 +788,11 @@ struct Impl {
         // the step on the host rather than enqueue the same chain again.
         if (c.phase != PH_V_SPMV || c.j != j_before) continue;
       }
+      // a chain that stopped in a CGS phase ran out of passes: hold two
+      // passes in the engine's later chains
+      if (device && e->tries_auto && e->dev_tries < 2 &&
+          (c.phase == PH_V_REO || c.phase == PH_U_REO))
+        e->dev_tries = 2;
       PK_TRY(host_phase(c));
     }
     return PK_OK;
@@ -1193,6 +1198,7 @@ int pk_lbd_set_mode(pk_lbd_t h, int mode, int tries) {
   if ((mode != 0 && mode != 1) || tries < 1 || tries > 5) return PK_ERR_ARG;
   e->mode = mode;
   e->dev_tries = tries;
+  e->tries_auto = false;
   return PK_OK;
 }
 

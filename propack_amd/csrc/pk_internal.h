@@ -113,7 +113,11 @@ struct Engine {
   LbdCtl* dctl = nullptr;            // step control state (device)
   LbdCtl* hctl = nullptr;            // its pinned host copy
   int mode = 0;                      // 0: device-resident steps, 1: host executor
-  int dev_tries = 2;                 // CGS passes per side held by a device chain
+  int dev_tries = 1;                 // CGS passes per side held by a device chain
+  // dev_tries adapts: a chain that ran out of passes (a second CGS pass was
+  // needed, finished on the host) raises it to 2 for the engine's later
+  // calls.  pk_lbd_set_mode fixes it.
+  bool tries_auto = true;
   // Rounding model of the thresholds epsn = leff * eps, epsn2 = sqrt(leff) * eps:
   // 0 = the reference's (leff = max(m, n): accumulation in the storage
   // precision); 1 = storage rounding only (leff = 4: FP32 storage with FP64
