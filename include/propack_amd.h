@@ -57,6 +57,32 @@ int pk_csr_destroy(pk_csr_t op);
 int pk_csr_dims(pk_csr_t op, int* m, int* n, long long* nnz);
 /* y = op(A) x ; x, y in precision prec, on the host or the device. */
 int pk_csr_spmv(pk_csr_t op, char trans, const void* x, void* y, int prec, int where);
+
+/* Matrix ingest (no reference counterpart in the library; the reference's
+ * example drivers read Harwell-Boeing, coordinate and dense files,
+ * README:103-118).
+ * pk_matrix_read: a Matrix Market coordinate file (real / integer / pattern;
+ * general / symmetric / skew-symmetric), a Harwell-Boeing file of an
+ * assembled real or pattern matrix (RUA, RRA, RSA, RZA, P?A), or plain
+ * "i j value" triples (1-based, optional "m n nnz" first line), as 0-based
+ * coordinate arrays with symmetric storage expanded; free each array with
+ * pk_matrix_free.
+ * pk_coo_to_csr: coordinate entries (any order, duplicates allowed, indices
+ * 0- or 1-based by `base`) -> CSR sorted by (row, column) with duplicates
+ * summed in input order, built on the device; inputs and outputs in host
+ * or device memory (`where`); colind/val need nnz entries of room,
+ * *nnz_out receives the distinct count.
+ * pk_csr_create_coo: the same straight into a device operator.
+ * pk_csr_read: a file (formats above) straight into a device operator. */
+int pk_matrix_read(const char* path, int* m, int* n, long long* nnz, int** rows, int** cols,
+                   double** vals);
+void pk_matrix_free(void* p);
+int pk_coo_to_csr(int m, int n, long long nnz, const int* rows, const int* cols, const void* vals,
+                  int val_prec, int where, int base, int* rowptr, int* colind, void* val,
+                  long long* nnz_out);
+int pk_csr_create_coo(int m, int n, long long nnz, const int* rows, const int* cols,
+                      const void* vals, int val_prec, int where, int base, pk_csr_t* op);
+int pk_csr_read(const char* path, int val_prec, pk_csr_t* op);
 /* Store the handle in IPARM(1:2) so that passing pk_dcsr_aprod_ (or
  * pk_scsr_aprod_) as APROD selects the device operator in the drivers. */
 int pk_csr_to_iparm(pk_csr_t op, int* iparm);

@@ -37,7 +37,8 @@ EXPORTS = [
     "pk_larnv_pm1", "pkh_update_mu", "pkh_update_nu", "pkh_compute_int", "pkh_set_mu",
     "pkh_lartg", "pkh_bdqr", "pkh_bsvdstep", "pkh_refinebounds", "dlansvd_", "dlansvd_irl_", "slansvd_", "slansvd_irl_",
     "clearstat_", "printstat_", "pk_lbd_set_rounding", "pk_lbd_eps", "pk_batch_create",
-    "pk_batch_destroy", "pk_batch_lanbpro", "pk_lansvd_irl_batch",
+    "pk_batch_destroy", "pk_batch_lanbpro", "pk_lansvd_irl_batch", "pk_matrix_read",
+    "pk_matrix_free", "pk_coo_to_csr", "pk_csr_create_coo", "pk_csr_read",
 ]
 
 _lib = None
@@ -89,6 +90,21 @@ class CsrOperator:
         self.handle = h
         self.iparm = np.zeros(4, np.int32)
         _check(L.pk_csr_to_iparm(h, _p(self.iparm)), "pk_csr_to_iparm")
+
+    @classmethod
+    def from_file(cls, path, prec=PK_F64):
+        """Device operator straight from a Matrix Market / Harwell-Boeing /
+        coordinate file (pk_csr_read)."""
+        self = cls.__new__(cls)
+        h = C.c_void_p()
+        _check(lib().pk_csr_read(str(path).encode(), C.c_int(prec), C.byref(h)), "pk_csr_read")
+        m, n, nnz = C.c_int(), C.c_int(), C.c_longlong()
+        _check(lib().pk_csr_dims(h, C.byref(m), C.byref(n), C.byref(nnz)), "pk_csr_dims")
+        self.handle, self.m, self.n, self.prec = h, m.value, n.value, prec
+        self.nnz = nnz.value
+        self.iparm = np.zeros(4, np.int32)
+        _check(lib().pk_csr_to_iparm(h, _p(self.iparm)), "pk_csr_to_iparm")
+        return self
 
     def spmv(self, x, trans="n"):
         x = np.ascontiguousarray(x, _dt(self.prec))
@@ -344,6 +360,46 @@ def lansvd_irl_dev(eng: Engine, neig, dim, p=None, tol=1e-10, which="L", jobu="y
     V = eng.get("v", k) if (fetch and jobv == "y") else None
     return SvdResult(sigma[:k].copy(), bnd[:k].copy(), U, V, info, k, timing(), dim=dd.value,
                      anorm=float(dopt[2]))
+
+
+def read_matrix(path):
+    """Coordinate form of a Matrix Market / Harwell-Boeing / coordinate file
+    (host parse): (m, n, rows, cols, vals), 0-based, symmetric storage
+    expanded, duplicates kept."""
+    m, n, nnz = C.c_int(), C.c_int(), C.c_longlong()
+    r, c, v = C.POINTER(C.c_int)(), C.POINTER(C.c_int)(), C.POINTER(C.c_double)()
+    L = lib()
+    _check(L.pk_matrix_read(str(path).encode(), C.byref(m), C.byref(n), C.byref(nnz), C.byref(r),
+                            C.byref(c), C.byref(v)), "pk_matrix_read")
+    k = nnz.value
+    try:
+        rows = np.ctypeslib.as_array(r, (max(k, 1),))[:k].copy()
+        cols = np.ctypeslib.as_array(c, (max(k, 1),))[:k].copy()
+        vals = np.ctypeslib.as_array(v, (max(k, 1),))[:k].copy()
+    finally:
+        for ptr in (r, c, v):
+            L.pk_matrix_free(C.cast(ptr, C.c_void_p))
+    return m.value, n.value, rows, cols, vals
+
+
+def coo_to_csr(m, n, rows, cols, vals, base=0):
+    """CSR (synth.Csr) of coordinate entries, built on the device: sorted by
+    (row, column), duplicates summed in input order."""
+    vals = np.ascontiguousarray(vals)
+    prec = PK_F32 if vals.dtype == np.float32 else PK_F64
+    vals = vals.astype(_dt(prec))
+    rows = np.ascontiguousarray(rows, np.int32)
+    cols = np.ascontiguousarray(cols, np.int32)
+    nnz = rows.size
+    rp = np.zeros(m + 1, np.int32)
+    ci = np.zeros(max(nnz, 1), np.int32)
+    va = np.zeros(max(nnz, 1), _dt(prec))
+    nz = C.c_longlong()
+    _check(lib().pk_coo_to_csr(C.c_int(m), C.c_int(n), C.c_longlong(nnz), _p(rows), _p(cols),
+                               _p(vals), C.c_int(prec), C.c_int(PK_HOST), C.c_int(base), _p(rp),
+                               _p(ci), _p(va), C.byref(nz)), "pk_coo_to_csr")
+    from .synth import Csr
+    return Csr(m, n, rp, ci[: nz.value].copy(), va[: nz.value].copy())
 
 
 class Batch:
