@@ -145,8 +145,9 @@ __device__ __forceinline__ void spmv_longfix_body(const int4 d, const double* __
 
 // ----------------------------------------------- Gram-Schmidt projection ----
 // Slice b of h = V(:, scol[0:ncols])^T w: the slice of w stays in registers,
-// kCT-column tiles of V stream through (wave shuffles + LDS per tile).
-// Partial of column c goes to part[c * stride + slot].  scol in LDS.
+// kCT-column tiles of V stream through (wave shuffles + LDS per tile), the
+// next tile's loads issued before the current tile is reduced.  Partial of
+// column c goes to part[c * stride + slot].  scol in LDS.
 template <typename T>
 __device__ __forceinline__ void gemvt_body(long long b, long long L, const T* __restrict__ V,
                                            long long ldv, const int* scol, int ncols,
@@ -155,28 +156,44 @@ __device__ __forceinline__ void gemvt_body(long long b, long long L, const T* __
                                            double (*lds)[(kBlock / kWave) * kCT]) {
   const long long base = b * kRowsPerBlock + threadIdx.x;
   double wr[kRPT];
+  bool ok[kRPT];
 #pragma unroll
   for (int r = 0; r < kRPT; ++r) {
     const long long row = base + static_cast<long long>(r) * kBlock;
-    wr[r] = row < L ? static_cast<double>(w[row]) : 0.0;
+    ok[r] = row < L;
+    wr[r] = ok[r] ? static_cast<double>(w[row]) : 0.0;
   }
   const int lane = threadIdx.x % kWave, wid = threadIdx.x / kWave;
   int buf = 0;
+  // software pipeline: the loads of the next column tile are in flight
+  // while the current tile is reduced
+  T cur[kCT][kRPT], nxt[kCT][kRPT];
+  if (ncols <= 0) return;
+#pragma unroll
+  for (int c = 0; c < kCT; ++c) {
+    const T* vc = V + static_cast<long long>(scol[c < ncols ? c : ncols - 1]) * ldv;
+#pragma unroll
+    for (int r = 0; r < kRPT; ++r)
+      cur[c][r] = (ok[r] && c < ncols) ? vc[base + static_cast<long long>(r) * kBlock] : T(0);
+  }
   for (int c0 = 0; c0 < ncols; c0 += kCT) {
+    const int n0 = c0 + kCT;
+    if (n0 < ncols) {
+#pragma unroll
+      for (int c = 0; c < kCT; ++c) {
+        const T* vc = V + static_cast<long long>(scol[n0 + c < ncols ? n0 + c : ncols - 1]) * ldv;
+#pragma unroll
+        for (int r = 0; r < kRPT; ++r)
+          nxt[c][r] = (ok[r] && n0 + c < ncols) ? vc[base + static_cast<long long>(r) * kBlock]
+                                                : T(0);
+      }
+    }
     double acc[kCT];
 #pragma unroll
-    for (int c = 0; c < kCT; ++c) acc[c] = 0.0;
+    for (int c = 0; c < kCT; ++c) {
+      acc[c] = 0.0;
 #pragma unroll
-    for (int r = 0; r < kRPT; ++r) {
-      const long long row = base + static_cast<long long>(r) * kBlock;
-      if (row < L) {
-#pragma unroll
-        for (int c = 0; c < kCT; ++c) {
-          if (c0 + c < ncols)
-            acc[c] += static_cast<double>(V[static_cast<long long>(scol[c0 + c]) * ldv + row]) *
-                      wr[r];
-        }
-      }
+      for (int r = 0; r < kRPT; ++r) acc[c] += static_cast<double>(cur[c][r]) * wr[r];
     }
 #pragma unroll
     for (int c = 0; c < kCT; ++c) acc[c] = wave_sum(acc[c]);
@@ -190,7 +207,11 @@ __device__ __forceinline__ void gemvt_body(long long b, long long L, const T* __
       for (int ww = 0; ww < kBlock / kWave; ++ww) sum += lds[buf][ww * kCT + threadIdx.x];
       part[static_cast<long long>(c0 + threadIdx.x) * stride + slot] = sum;
     }
-    buf ^= 1;  // the other buffer is free: its readers passed the barrier above
+    buf ^= 1;
+#pragma unroll
+    for (int c = 0; c < kCT; ++c)
+#pragma unroll
+      for (int r = 0; r < kRPT; ++r) cur[c][r] = nxt[c][r];
   }
 }
 

@@ -10,7 +10,7 @@
 #include "propack.h"
 #include "propack_amd.h"
 
-enum { M = 250, N = 180, PER = 5, KMAX = 50, NSV = 6 };
+enum { M = 250, N = 180, PER = 5, KMAX = 180, NSV = 6 };
 static int rowptr[M + 1], colind[M * PER];
 static double val[M * PER];
 

@@ -70,7 +70,7 @@ program dropin_example
       integer(c_int) :: iparm(*)
     end function
   end interface
-  integer, parameter :: kmax = 60, nsv = 8
+  integer, parameter :: kmax = 200, nsv = 8
   external :: my_aprod, pk_dcsr_aprod
   double precision :: U(m, kmax + 1), V(n, kmax), sigma(kmax), bnd(kmax), doption(4), dparm(1)
   double precision :: work(1), tol

@@ -1,13 +1,17 @@
 """Compiled callers of the drop-in drivers (tests/fortran/dropin_example.f90,
 tests/c/dropin_example.c, built by tests/build_programs.py): the
 reference's calling sequences linked against libpropack_amd, with a host
-APROD and with the device operator.  Their singular values against the dense
-SVD of the same matrix built here."""
+APROD and with the device operator.  Their INFO and converged counts equal
+the compiled reference's on the same matrix and start vector, and their
+singular values the dense SVD of the matrix built here."""
 import os
 import subprocess
 
 import numpy as np
 import pytest
+
+from oracle import refpy
+from propack_amd import synth
 
 pytestmark = pytest.mark.gpu
 
@@ -37,7 +41,11 @@ def test_fortran_caller():
             D[i - 1, (i * 7 + t * 31 + t * t) % n] += np.sin(0.37 * i + 1.1 * t) + 0.1 * t
     s = np.linalg.svd(D, compute_uv=False)
     out = _run("dropin_example_f")
-    assert out["HOST_INFO_K"] == [0, 8]
+    r, c = np.nonzero(D)
+    A = synth.from_coo(m, n, r, c, D[r, c])
+    u0 = np.cos(0.11 * np.arange(1, m + 1))
+    ref = refpy.lansvd(A, 8, 200, u0=u0, tol=1e-12, cgs=1)
+    assert out["HOST_INFO_K"] == [ref.info, ref.k] == [0, 8]
     np.testing.assert_allclose(out["HOST_SIGMA"], s[:8], rtol=1e-11)
     assert out["DEV_INFO_K"][:2] == [0, 8]
     np.testing.assert_allclose(out["DEV_SIGMA"], s[:8], rtol=1e-11)
@@ -52,7 +60,11 @@ def test_c_caller():
             D[i, (i * 11 + t * 29 + t * t) % n] += np.cos(0.23 * i + 0.7 * t) + 0.05 * t
     s = np.linalg.svd(D, compute_uv=False)
     out = _run("dropin_example_c")
-    assert out["HOST_INFO_K"] == [0, 6]
+    r, c = np.nonzero(D)
+    A = synth.from_coo(m, n, r, c, D[r, c])
+    u0 = np.sin(0.3 * np.arange(m) + 0.1)
+    ref = refpy.lansvd(A, 6, 180, u0=u0, tol=1e-12, cgs=1)
+    assert out["HOST_INFO_K"] == [ref.info, ref.k] == [0, 6]
     np.testing.assert_allclose(out["HOST_SIGMA"], s[:6], rtol=1e-11)
     assert out["DEV_INFO_K"][:2] == [0, 6]
     np.testing.assert_allclose(out["DEV_SIGMA"], s[:6], rtol=2e-5)
