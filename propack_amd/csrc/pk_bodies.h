@@ -216,6 +216,7 @@ __device__ __forceinline__ void gemvt_body(long long b, long long L, const T* __
 }
 
 // Slice b of w -= V(:, scol[0:ncols]) sh (scol, sh in LDS); *q = ||w slice||^2.
+// kCT columns' loads are issued together; the updates keep column order.
 template <typename T>
 __device__ __forceinline__ void gemv_update_body(long long b, long long L,
                                                  const T* __restrict__ V, long long ldv,
@@ -228,14 +229,32 @@ __device__ __forceinline__ void gemv_update_body(long long b, long long L,
     const long long row = base + static_cast<long long>(r) * kBlock;
     acc[r] = row < L ? static_cast<double>(w[row]) : 0.0;
   }
-  for (int c = 0; c < ncols; ++c) {
+  bool ok[kRPT];
+#pragma unroll
+  for (int r = 0; r < kRPT; ++r) ok[r] = base + static_cast<long long>(r) * kBlock < L;
+  int c = 0;
+  for (; c + kCT <= ncols; c += kCT) {  // kCT columns' loads in flight together
+    T x[kCT][kRPT];
+#pragma unroll
+    for (int u = 0; u < kCT; ++u) {
+      const T* vc = V + static_cast<long long>(scol[c + u]) * ldv;
+#pragma unroll
+      for (int r = 0; r < kRPT; ++r)
+        x[u][r] = ok[r] ? vc[base + static_cast<long long>(r) * kBlock] : T(0);
+    }
+#pragma unroll
+    for (int u = 0; u < kCT; ++u) {
+      const double hc = sh[c + u];
+#pragma unroll
+      for (int r = 0; r < kRPT; ++r) acc[r] -= static_cast<double>(x[u][r]) * hc;
+    }
+  }
+  for (; c < ncols; ++c) {
     const T* vc = V + static_cast<long long>(scol[c]) * ldv;
     const double hc = sh[c];
 #pragma unroll
-    for (int r = 0; r < kRPT; ++r) {
-      const long long row = base + static_cast<long long>(r) * kBlock;
-      if (row < L) acc[r] -= static_cast<double>(vc[row]) * hc;
-    }
+    for (int r = 0; r < kRPT; ++r)
+      if (ok[r]) acc[r] -= static_cast<double>(vc[base + static_cast<long long>(r) * kBlock]) * hc;
   }
   double v[1] = {0.0};
 #pragma unroll
