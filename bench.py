@@ -251,6 +251,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-check", action="store_true")
     ap.add_argument("--no-mixed", action="store_true")
+    ap.add_argument("--no-tune", action="store_true",
+                    help="built-in SpMV column panels instead of the autotuner's")
     ap.add_argument("--workload", choices=["c2", "powerlaw", "batch"], default="c2",
                     help="c2 (default): the headline solve; powerlaw: 8M x 4M Zipf "
                          "bipartite, k=64, dim=160, tol=1e-8, FP32 basis + FP64 polish vs "
@@ -279,7 +281,14 @@ def main():
     u0 = synth.start_vector(M, SEED + 1)
     log(f"[bench] matrix {A.m}x{A.n} nnz={A.nnz} generated in {time.perf_counter() - t0:.1f}s")
     t0 = time.perf_counter()
-    op = api.CsrOperator(A)
+    panels, tuned = None, None
+    if not args.no_tune:
+        panels, tuned = api.autotune(A)
+        log(f"[bench] SpMV autotune: panels A={panels[0]} A^T={panels[1]} "
+            f"({tuned[0]:.1f} / {tuned[1]:.1f} us per application) in "
+            f"{time.perf_counter() - t0:.1f}s")
+    t0 = time.perf_counter()
+    op = api.CsrOperator(A, panels=panels)
     eng = api.Engine(op, kmax=DIM)
     log(f"[bench] device operator + A^T + engine in {time.perf_counter() - t0:.1f}s")
 
@@ -331,7 +340,9 @@ def main():
     # FP32-basis form with FP64 polish on the same matrix and start vector
     mixed = None
     if not args.no_mixed:
-        op32 = api.CsrOperator(A.astype(np.float32), prec=api.PK_F32)
+        A32 = A.astype(np.float32)
+        p32 = None if args.no_tune else api.autotune(A32, prec=api.PK_F32)[0]
+        op32 = api.CsrOperator(A32, prec=api.PK_F32, panels=p32)
         eng32 = api.Engine(op32, kmax=DIM)
         u32 = u0.astype(np.float32).astype(np.float64)
 
@@ -404,6 +415,7 @@ def main():
                    "basis": "fp64", "converged": r.k, "info": r.info,
                    "nopx": r.counters["nopx"], "nreorth": r.counters["nreorth"],
                    "nrestart": r.counters["nrestart"], "host_syncs": stats["nsync"],
+                   "spmv_panels": list(op.panels()), "autotuned": panels is not None,
                    "max_rel_residual": resid},
         "roofline": {"kernel": dom, "bound": "hbm", "achieved": round(achieved, 1),
                      "peak": HBM_PEAK_GBS, "unit": "GB/s",

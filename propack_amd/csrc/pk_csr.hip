@@ -235,6 +235,12 @@ int panels_for(bool transposed, int rows, int cols, long long nnz, size_t vbytes
   return q;
 }
 
+int pow2_floor(int p) {
+  int q = 1;
+  while (q * 2 <= p && q * 2 <= CsrOp::kMaxPanels) q *= 2;
+  return q;
+}
+
 bool finish_panel(CsrOp::Panel& P, int rows) {
   std::vector<int> hrp(static_cast<size_t>(rows) + 1);
   if (hipMemcpy(hrp.data(), P.rp, sizeof(int) * (static_cast<size_t>(rows) + 1),
@@ -355,7 +361,8 @@ static void free_dir(CsrOp::Dir& D) {
 }
 
 int csr_create(int m, int n, long long nnz, const int* rowptr, const int* colind,
-               const void* val, int val_prec, int where, CsrOp** out) {
+               const void* val, int val_prec, int where, CsrOp** out, int panels_a,
+               int panels_t) {
   if (!out || m < 0 || n < 0 || nnz < 0 || nnz >= (1LL << 31)) return PK_ERR_ARG;
   if (val_prec != PK_F64 && val_prec != PK_F32) return PK_ERR_ARG;
   if (where != PK_HOST && where != PK_DEVICE) return PK_ERR_ARG;
@@ -400,8 +407,8 @@ int csr_create(int m, int n, long long nnz, const int* rowptr, const int* colind
              : build_transpose<float>(m, n, nnz, rp, ci, va, &rpt, &cit, &vat, s);
   if (rc == PK_OK) {
     // build_dir owns the arrays from here on
-    const int pa = panels_for(false, m, n, nnz, op->val_bytes);
-    const int pt = panels_for(true, n, m, nnz, op->val_bytes);
+    const int pa = panels_a > 0 ? pow2_floor(panels_a) : panels_for(false, m, n, nnz, op->val_bytes);
+    const int pt = panels_t > 0 ? pow2_floor(panels_t) : panels_for(true, n, m, nnz, op->val_bytes);
     rc = val_prec == PK_F64 ? build_dir<double>(op->da, m, n, nnz, pa, rp, ci, va, s)
                             : build_dir<float>(op->da, m, n, nnz, pa, rp, ci, va, s);
     rp = ci = nullptr;
@@ -534,6 +541,50 @@ CsrOp* csr_from_iparm(const int* iparm) {
 
 }  // namespace
 
+namespace {
+
+// Times `reps` applications of op(A) (after one warm-up) on a device vector
+// of ones; microseconds per application.
+template <typename T>
+int time_apply(const CsrOp* op, char trans, int reps, double* us) {
+  const bool tr = trans == 't';
+  const long long rin = tr ? op->m : op->n, rout = tr ? op->n : op->m;
+  hipStream_t s;
+  if (hipStreamCreateWithFlags(&s, hipStreamNonBlocking) != hipSuccess) return PK_ERR_HIP;
+  T *x = nullptr, *y = nullptr;
+  double* part = nullptr;
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  int rc = PK_OK;
+  const size_t npart = 2 * static_cast<size_t>(std::max(std::max(op->nrb, op->nrbt), kMaxBlocks));
+  std::vector<T> ones(static_cast<size_t>(std::max<long long>(rin, 1)), T(1));
+  if (hipMalloc(&x, sizeof(T) * ones.size()) != hipSuccess ||
+      hipMalloc(&y, sizeof(T) * std::max<long long>(rout, 1)) != hipSuccess ||
+      hipMalloc(&part, sizeof(double) * npart) != hipSuccess ||
+      hipMemcpy(x, ones.data(), sizeof(T) * ones.size(), hipMemcpyHostToDevice) != hipSuccess ||
+      hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess) {
+    rc = PK_ERR_HIP;
+  } else {
+    float ms = 0;
+    for (int i = 0; i <= reps && rc == PK_OK; ++i) {
+      if (i == 1 && hipEventRecord(e0, s) != hipSuccess) rc = PK_ERR_HIP;
+      if (csr_apply<T>(op, s, trans, x, y, nullptr, sval(0.0), part, Gate{}) < 0) rc = PK_ERR_HIP;
+    }
+    if (rc == PK_OK && (hipEventRecord(e1, s) != hipSuccess || hipEventSynchronize(e1) != hipSuccess ||
+                        hipEventElapsedTime(&ms, e0, e1) != hipSuccess))
+      rc = PK_ERR_HIP;
+    *us = 1000.0 * ms / std::max(reps, 1);
+  }
+  (void)hipFree(x);
+  (void)hipFree(y);
+  (void)hipFree(part);
+  if (e0) (void)hipEventDestroy(e0);
+  if (e1) (void)hipEventDestroy(e1);
+  (void)hipStreamDestroy(s);
+  return rc;
+}
+
+}  // namespace
+
 extern "C" {
 
 int pk_csr_create(int m, int n, long long nnz, const int* rowptr, const int* colind,
@@ -545,6 +596,70 @@ int pk_csr_create(int m, int n, long long nnz, const int* rowptr, const int* col
     return rc;
   }
   *op = reinterpret_cast<pk_csr_t>(o);
+  return PK_OK;
+}
+
+int pk_csr_create_panels(int m, int n, long long nnz, const int* rowptr, const int* colind,
+                         const void* val, int val_prec, int where, int panels_a, int panels_t,
+                         pk_csr_t* op) {
+  CsrOp* o = nullptr;
+  const int rc = csr_create(m, n, nnz, rowptr, colind, val, val_prec, where, &o,
+                            panels_a > 0 ? panels_a : 1, panels_t > 0 ? panels_t : 1);
+  if (rc != PK_OK) {
+    set_error("pk_csr_create_panels failed (code %d)", rc);
+    return rc;
+  }
+  *op = reinterpret_cast<pk_csr_t>(o);
+  return PK_OK;
+}
+
+int pk_csr_panels(pk_csr_t h, int* panels_a, int* panels_t) {
+  CsrOp* op = as_csr(h);
+  if (!op) return PK_ERR_HANDLE;
+  if (panels_a) *panels_a = op->da.np;
+  if (panels_t) *panels_t = op->dt.np;
+  return PK_OK;
+}
+
+// SpMV autotuner: builds the operator with 1, 2, 4 and 8 column panels per
+// direction, times each direction's application, returns the fastest
+// panel count of each direction and its time.  The choice changes how the
+// row sums are grouped (not what is summed), so an operator built with it
+// gives results equal to the built-in layout's up to rounding.
+int pk_csr_autotune(int m, int n, long long nnz, const int* rowptr, const int* colind,
+                    const void* val, int val_prec, int where, int* panels_a, int* panels_t,
+                    double* us_a, double* us_t) {
+  if (!panels_a || !panels_t) return PK_ERR_ARG;
+  double best_a = 1e300, best_t = 1e300;
+  int pa = 1, pt = 1;
+  for (int P = 1; P <= 8; P *= 2) {
+    CsrOp* o = nullptr;
+    int rc = csr_create(m, n, nnz, rowptr, colind, val, val_prec, where, &o, P, P);
+    if (rc != PK_OK) return rc;
+    double ta = 0, tt = 0;
+    if (val_prec == PK_F64) {
+      rc = time_apply<double>(o, 'n', 5, &ta);
+      if (rc == PK_OK) rc = time_apply<double>(o, 't', 5, &tt);
+    } else {
+      rc = time_apply<float>(o, 'n', 5, &ta);
+      if (rc == PK_OK) rc = time_apply<float>(o, 't', 5, &tt);
+    }
+    const int npa = o->da.np, npt = o->dt.np;
+    csr_destroy(o);
+    if (rc != PK_OK) return rc;
+    if (npa == P && ta < best_a) {
+      best_a = ta;
+      pa = P;
+    }
+    if (npt == P && tt < best_t) {
+      best_t = tt;
+      pt = P;
+    }
+  }
+  *panels_a = pa;
+  *panels_t = pt;
+  if (us_a) *us_a = best_a;
+  if (us_t) *us_t = best_t;
   return PK_OK;
 }
 

@@ -49,8 +49,11 @@ struct CsrOp {
   int nrb = 0, nrbt = 0;  // partial slots per direction
 };
 
+// panels_a / panels_t: column panels of A and A^T (powers of two up to
+// CsrOp::kMaxPanels), or <= 0 for the built-in choice.
 int csr_create(int m, int n, long long nnz, const int* rowptr, const int* colind,
-               const void* val, int val_prec, int where, CsrOp** out);
+               const void* val, int val_prec, int where, CsrOp** out, int panels_a = 0,
+               int panels_t = 0);
 void csr_destroy(CsrOp* op);
 template <typename T>
 int csr_apply(const CsrOp* op, hipStream_t s, char trans, const T* x, T* y, const T* yprev,

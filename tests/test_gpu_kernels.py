@@ -169,3 +169,22 @@ def test_panels_same_triplets(panels, small_case):
     assert r0.k == r1.k == 10
     np.testing.assert_allclose(r1.sigma, r0.sigma, rtol=1e-12)
     np.testing.assert_allclose(r1.sigma, s[:10], rtol=1e-12)
+
+
+def test_autotune_and_explicit_panels():
+    """The autotuner returns panel counts it measured; an operator built
+    with them (and with any explicit layout) gives the plain layout's
+    products up to rounding."""
+    A = synth.poisson_decay(200_000, 100_000, 16.0, 0.6, seed=4)
+    (pa, pt), (ua, ut) = api.autotune(A)
+    assert pa in (1, 2, 4, 8) and pt in (1, 2, 4, 8) and ua > 0 and ut > 0
+    x = synth.start_vector(A.n, 2)
+    y = synth.start_vector(A.m, 3)
+    want_n, want_t = A.matvec(x), A.transpose().matvec(y)
+    for panels in ((pa, pt), (1, 1), (8, 2), (16, 16)):
+        op = api.CsrOperator(A, panels=panels)
+        assert op.panels() == tuple(min(p, 16) for p in panels)
+        np.testing.assert_allclose(op.spmv(x, "n"), want_n, rtol=0,
+                                   atol=1e-12 * np.abs(want_n).max())
+        np.testing.assert_allclose(op.spmv(y, "t"), want_t, rtol=0,
+                                   atol=1e-12 * np.abs(want_t).max())
