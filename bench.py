@@ -52,19 +52,23 @@ def log(*a):
 TRAFFIC_PROFILE = os.path.join(ROOT, "profiles", "hbm_traffic_spmv_poisson_decay_2Mx1M.md")
 
 
-def measured_traffic(kclass):
+def measured_traffic(kclass, panels):
     """HBM bytes per operator application of kernel class kclass, as
     measured by the rocprofv3 FETCH_SIZE / WRITE_SIZE runs summarised in
-    profiles/ (calibrated against a plain copy); None if absent."""
+    profiles/ (calibrated against a plain copy); None if absent or if the
+    summary was measured on another column-panel layout."""
+    traffic, layout = None, None
     try:
         with open(TRAFFIC_PROFILE) as f:
             for line in f:
                 w = line.split()
                 if len(w) == 3 and w[0] == "traffic_bytes_per_application" and w[1] == kclass:
-                    return float(w[2])
+                    traffic = float(w[2])
+                if len(w) == 3 and w[0] == "measured_layout":
+                    layout = (int(w[1]), int(w[2]))
     except OSError:
         pass
-    return None
+    return traffic if layout == tuple(panels) else None
 
 
 def residuals(op, r):
@@ -253,6 +257,8 @@ def main():
     ap.add_argument("--no-mixed", action="store_true")
     ap.add_argument("--no-tune", action="store_true",
                     help="built-in SpMV column panels instead of the autotuner's")
+    ap.add_argument("--panels", default=None,
+                    help="explicit SpMV column panels 'A,At' (no autotuning)")
     ap.add_argument("--workload", choices=["c2", "powerlaw", "batch"], default="c2",
                     help="c2 (default): the headline solve; powerlaw: 8M x 4M Zipf "
                          "bipartite, k=64, dim=160, tol=1e-8, FP32 basis + FP64 polish vs "
@@ -282,7 +288,9 @@ def main():
     log(f"[bench] matrix {A.m}x{A.n} nnz={A.nnz} generated in {time.perf_counter() - t0:.1f}s")
     t0 = time.perf_counter()
     panels, tuned = None, None
-    if not args.no_tune:
+    if args.panels:
+        panels = tuple(int(x) for x in args.panels.split(","))
+    elif not args.no_tune:
         panels, tuned = api.autotune(A)
         log(f"[bench] SpMV autotune: panels A={panels[0]} A^T={panels[1]} "
             f"({tuned[0]:.1f} / {tuned[1]:.1f} us per application) in "
@@ -415,11 +423,11 @@ def main():
                    "basis": "fp64", "converged": r.k, "info": r.info,
                    "nopx": r.counters["nopx"], "nreorth": r.counters["nreorth"],
                    "nrestart": r.counters["nrestart"], "host_syncs": stats["nsync"],
-                   "spmv_panels": list(op.panels()), "autotuned": panels is not None,
+                   "spmv_panels": list(op.panels()), "autotuned": tuned is not None,
                    "max_rel_residual": resid},
         "roofline": {"kernel": dom, "bound": "hbm", "achieved": round(achieved, 1),
                      "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": measured_traffic(dom),
+                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": measured_traffic(dom, op.panels()),
                      "avg_launch_ms": per_launch_ms, "bytes_per_launch": per_launch_bytes},
         "kernels": kernels,
         "fp32_basis_form": mixed,
