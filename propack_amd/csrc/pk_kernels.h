@@ -60,12 +60,8 @@ int launch_norm(hipStream_t s, long long L, const T* x, double* part);
 template <typename S, typename D>
 int launch_convert(hipStream_t s, long long L, const S* src, D* dst);
 
-// C(:, 0:ncols) = A(:, 0:kdim) * B   with B (kdim x ncols, ldb) in double.
-template <typename T>
-int launch_gemm_tall(hipStream_t s, long long L, const T* A, long long lda, int kdim,
-                     const double* B, int ldb, int ncols, T* C, long long ldc);
-
-// Same product on FP64 matrix cores (v_mfma_f64_16x16x4_f64).
+// C(:, 0:ncols) = A(:, 0:kdim) * B with B (kdim x ncols, ldb) in double, on
+// FP64 matrix cores (v_mfma_f64_16x16x4_f64).
 template <typename T>
 int launch_gemm_mfma(hipStream_t s, long long L, const T* A, long long lda, int kdim,
                      const double* B, int ldb, int ncols, T* C, long long ldc);

@@ -12,7 +12,8 @@
 //   k_gemv_update   w -= V(:,I) h with h broadcast from LDS, fused with ||w||^2.
 //   k_axpy_norm_dot, k_scale, k_norm   level-1 updates with fused reductions.
 //   k_reduce        second reduction stage (fixed order => deterministic).
-//   k_gemm_tall     C(:,1:n) = A(:,1:k) B for restarts / Ritz vectors (spare buffer).
+//   k_gemm_mfma     C(:,1:n) = A(:,1:k) B on FP64 matrix cores for restarts /
+//                   Ritz vectors (out of place into the spare basis buffer).
 #include "pk_bodies.h"
 
 namespace pk {
@@ -257,60 +258,6 @@ int launch_fill_zero_bytes(hipStream_t s, void* p, size_t bytes) {
   return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 
-// ---------------------------------------------------------------- GEMM ----
-// C(:,0:ncols) = A(:,0:kdim) * B (B kdim x ncols, double, column-major).
-// Out of place: the engine writes into its spare basis buffer and swaps, so
-// restarts and Ritz vectors need no in-place hazard handling.  One thread
-// owns one row and up to NC output columns (register accumulators); B is
-// read with wave-uniform addresses (scalar loads).  FP64 accumulation.
-constexpr int kGemmNC = 64;
-
-template <typename T, int NC>
-__global__ __launch_bounds__(kBlock) void k_gemm_tall(long long L, const T* __restrict__ A,
-                                                      long long lda, int kdim,
-                                                      const double* __restrict__ B, int ldb,
-                                                      int ncols, T* __restrict__ C,
-                                                      long long ldc) {
-  const long long row = static_cast<long long>(blockIdx.x) * kBlock + threadIdx.x;
-  if (row >= L) return;
-  double acc[NC];
-#pragma unroll
-  for (int j = 0; j < NC; ++j) acc[j] = 0.0;
-  for (int l = 0; l < kdim; ++l) {
-    const double a = static_cast<double>(A[static_cast<long long>(l) * lda + row]);
-    const double* bl = B + l;
-#pragma unroll
-    for (int j = 0; j < NC; ++j)
-      if (j < ncols) acc[j] += a * bl[static_cast<long long>(j) * ldb];
-  }
-#pragma unroll
-  for (int j = 0; j < NC; ++j)
-    if (j < ncols) C[static_cast<long long>(j) * ldc + row] = static_cast<T>(acc[j]);
-}
-
-template <typename T>
-int launch_gemm_tall(hipStream_t s, long long L, const T* A, long long lda, int kdim,
-                     const double* B, int ldb, int ncols, T* C, long long ldc) {
-  if (L <= 0 || ncols <= 0 || kdim <= 0) return 0;
-  const int nblk = cdiv(L, kBlock);
-  for (int j0 = 0; j0 < ncols; j0 += kGemmNC) {
-    const int nc = std::min(kGemmNC, ncols - j0);
-    const double* Bj = B + static_cast<long long>(j0) * ldb;
-    T* Cj = C + static_cast<long long>(j0) * ldc;
-    if (nc <= 16)
-      hipLaunchKernelGGL((k_gemm_tall<T, 16>), dim3(nblk), dim3(kBlock), 0, s, L, A, lda, kdim,
-                         Bj, ldb, nc, Cj, ldc);
-    else if (nc <= 32)
-      hipLaunchKernelGGL((k_gemm_tall<T, 32>), dim3(nblk), dim3(kBlock), 0, s, L, A, lda, kdim,
-                         Bj, ldb, nc, Cj, ldc);
-    else
-      hipLaunchKernelGGL((k_gemm_tall<T, 64>), dim3(nblk), dim3(kBlock), 0, s, L, A, lda, kdim,
-                         Bj, ldb, nc, Cj, ldc);
-    if (hipGetLastError() != hipSuccess) return -1;
-  }
-  return 0;
-}
-
 // ------------------------------------------------------------- control ----
 // One workgroup: reduce the partials of the preceding vector kernel (same
 // order as k_reduce), then run the step's control stage (ctl_stage_body).
@@ -444,8 +391,6 @@ template int launch_colsum<double, double>(hipStream_t, long long, const double*
   template int launch_axpy_norm_dot<T>(hipStream_t, long long, T*, const T*, Scal, double*,   \
                                        Gate);                                                 \
   template int launch_norm<T>(hipStream_t, long long, const T*, double*);                     \
-  template int launch_gemm_tall<T>(hipStream_t, long long, const T*, long long, int,         \
-                                   const double*, int, int, T*, long long);                  \
   template int launch_gemm_mfma<T>(hipStream_t, long long, const T*, long long, int,         \
                                    const double*, int, int, T*, long long);
 PK_INST_T(double)
