@@ -201,12 +201,16 @@ int pk_lbd_set_mode(pk_lbd_t eng, int mode, int tries);
  * src's U (any precision): the start vector of the FP64 polish that follows
  * an FP32-basis solve (no reference counterpart). */
 int pk_lbd_seed(pk_lbd_t dst, pk_lbd_t src, int ncols);
-/* Live kernel timing (no reference counterpart): when enabled, every hot
- * kernel launch is bracketed by HIP events on the engine's stream and its
- * algorithmic bytes/flops are accumulated per class.  out holds 6 classes x
+/* Live kernel timing (no reference counterpart).  classes: bit mask of the
+ * kernel classes to time (0 turns timing off, PK_PROF_ALL times all); each
+ * launch of a selected class is bracketed by HIP events on the engine's
+ * stream (two marker packets, a few microseconds of idle stream each) and
+ * its algorithmic bytes/flops are accumulated.  out holds 6 classes x
  * {launches, total ms, bytes, flops}: 0 SpMV A*v, 1 SpMV A^T*u,
  * 2 Gram-Schmidt V^T w, 3 Gram-Schmidt w -= V h, 4 basis GEMM, 5 level-1. */
-int pk_lbd_profile(pk_lbd_t eng, int enable);
+#define PK_PROF_SPMV 0x03
+#define PK_PROF_ALL 0x3f
+int pk_lbd_profile(pk_lbd_t eng, int classes);
 int pk_lbd_profile_get(pk_lbd_t eng, double* out);
 int pk_lbd_profile_reset(pk_lbd_t eng);
 

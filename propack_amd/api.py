@@ -327,8 +327,13 @@ class Engine:
 
     KCLASSES = ["spmv_n", "spmv_t", "gemvt", "gemv", "gemm", "vec"]
 
-    def profile(self, enable: bool = True):
-        _check(lib().pk_lbd_profile(self.handle, C.c_int(int(enable))), "pk_lbd_profile")
+    PROF_SPMV, PROF_ALL = 0x03, 0x3F
+
+    def profile(self, enable: bool = True, classes: int = None):
+        """Live kernel timing of the classes in the bit mask (bit i: KCLASSES[i];
+        all classes by default); enable=False turns it off."""
+        mask = (self.PROF_ALL if classes is None else int(classes)) if enable else 0
+        _check(lib().pk_lbd_profile(self.handle, C.c_int(mask)), "pk_lbd_profile")
         _check(lib().pk_lbd_profile_reset(self.handle), "pk_lbd_profile_reset")
 
     def profile_get(self) -> dict:

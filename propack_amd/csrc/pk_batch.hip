@@ -129,7 +129,7 @@ __global__ __launch_bounds__(kBlock) void k_bctl(const BProb* __restrict__ probs
                                                  int Qa, int Qt, int Sm, int Sn) {
   const BProb P = probs[blockIdx.x];
   __shared__ LbdCtl c;
-  __shared__ double lds[kBlock / kWave];
+  __shared__ double lds[(kBlock / kWave) * 2];
   if (P.ctl->phase != phase || P.ctl->j != J) return;
   const double* pa;
   const double* pb = nullptr;

@@ -389,7 +389,8 @@ __device__ inline void par_update_mu(LbdCtl& c, double* red) {
 // with 16-byte loads issued before the partial reductions, reduce
 // a = sum pa[0:na) (and b = sum pb[0:nb) when pb is set) in index order, let
 // thread 0 run the stage, spread a pending omega update over the workgroup,
-// finish the stage, write the state back.
+// finish the stage, write the state back.  lds holds (kBlock / kWave) * 2
+// doubles.
 __device__ __forceinline__ void ctl_stage_body(LbdCtl* __restrict__ gc, int kind,
                                                const double* pa, int na, const double* pb,
                                                int nb, LbdCtl& c, double* lds) {
@@ -398,19 +399,26 @@ __device__ __forceinline__ void ctl_stage_body(LbdCtl* __restrict__ gc, int kind
   static_assert(sizeof(LbdCtl) % sizeof(int4) == 0, "state must be whole 16-byte words");
   const int4* src = reinterpret_cast<const int4*>(gc);
   int4* dst = reinterpret_cast<int4*>(&c);
-  int4 stage[per];
+  // Word indices past the end are clamped to the last word, so every load is
+  // unconditional and all of them are in flight at once; the clamped lanes
+  // store that word's own value again.
 #pragma unroll
   for (int u = 0; u < per; ++u) {
-    const int i = threadIdx.x + u * kBlock;
-    if (i < nv) stage[u] = src[i];
+    const int i = min(static_cast<int>(threadIdx.x) + u * kBlock, nv - 1);
+    dst[i] = src[i];
   }
-  const double a = reduce_partials(pa, na, lds);
-  const double b = pb ? reduce_partials(pb, nb, lds) : 0.0;
-#pragma unroll
-  for (int u = 0; u < per; ++u) {
-    const int i = threadIdx.x + u * kBlock;
-    if (i < nv) dst[i] = stage[u];
+  double a, b = 0.0;
+  if (pb && nb == na) {
+    const double* const q[2] = {pa, pb};
+    double v[2];
+    reduce_partials_n<2>(q, na, v, lds);
+    a = v[0];
+    b = v[1];
+  } else {
+    a = reduce_partials(pa, na, lds);
+    if (pb) b = reduce_partials(pb, nb, lds);
   }
+
   __syncthreads();
   if (threadIdx.x == 0) {
     switch (kind) {
@@ -443,8 +451,8 @@ __device__ __forceinline__ void ctl_stage_body(LbdCtl* __restrict__ gc, int kind
   int4* back = reinterpret_cast<int4*>(gc);
 #pragma unroll
   for (int u = 0; u < per; ++u) {
-    const int i = threadIdx.x + u * kBlock;
-    if (i < nv) back[i] = dst[i];
+    const int i = min(static_cast<int>(threadIdx.x) + u * kBlock, nv - 1);
+    back[i] = dst[i];
   }
 }
 

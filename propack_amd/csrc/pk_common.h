@@ -73,13 +73,39 @@ __device__ __forceinline__ void block_sum(double (&v)[NV], double* lds) {
   __syncthreads();
 }
 
-// Sum of n partials by one 256-thread workgroup in a fixed order (thread t
-// sums p[t], p[t+256], ...; then the block reduction); every caller that
-// reduces the same partials gets the same bits.
+// Sums of NV arrays of n partials each by one 256-thread workgroup in a
+// fixed order (thread t adds p[t], p[t+256], ... in that order; then the
+// block reduction); every caller that reduces the same partials gets the same
+// bits.  Eight loads per array are issued ahead of their additions, so a
+// thread waits for one memory latency per eight partials, not per partial.
+// lds must hold (kBlock / kWave) * NV doubles; the sums land in thread 0.
+template <int NV>
+__device__ __forceinline__ void reduce_partials_n(const double* const (&p)[NV], int n,
+                                                  double (&v)[NV], double* lds) {
+  constexpr int U = 8;
+#pragma unroll
+  for (int k = 0; k < NV; ++k) v[k] = 0.0;
+  for (int i = threadIdx.x; i < n; i += U * kBlock) {
+    // indices past the end load the last partial (unconditional loads, all
+    // in flight together) and are not added
+    double t[NV][U];
+#pragma unroll
+    for (int k = 0; k < NV; ++k)
+#pragma unroll
+      for (int u = 0; u < U; ++u) t[k][u] = p[k][min(i + u * kBlock, n - 1)];
+#pragma unroll
+    for (int k = 0; k < NV; ++k)
+#pragma unroll
+      for (int u = 0; u < U; ++u)
+        if (i + u * kBlock < n) v[k] += t[k][u];
+  }
+  block_sum<NV>(v, lds);
+}
+
 __device__ __forceinline__ double reduce_partials(const double* p, int n, double* lds) {
-  double v[1] = {0.0};
-  for (int i = threadIdx.x; i < n; i += kBlock) v[0] += p[i];
-  block_sum<1>(v, lds);
+  const double* const q[1] = {p};
+  double v[1];
+  reduce_partials_n<1>(q, n, v, lds);
   return v[0];  // valid in thread 0
 }
 

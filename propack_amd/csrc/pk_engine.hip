@@ -124,8 +124,10 @@ struct Impl {
     }
     return ev;
   }
-  hipEvent_t tstart() {
-    if (!e->prof.on) return nullptr;
+  // Every timed region costs two marker packets (an idle gap of a few
+  // microseconds each) on the stream, so only the selected classes are timed.
+  hipEvent_t tstart(int cls) {
+    if (!e->prof.on || !((e->prof.mask >> cls) & 1u)) return nullptr;
     hipEvent_t a = ev_get();
     if (a) (void)hipEventRecord(a, strm);
     return a;
@@ -189,7 +191,7 @@ struct Impl {
     const long long rows_out = tr ? e->n : e->m, rows_in = tr ? e->m : e->n;
     int nb;
     if (e->op) {
-      hipEvent_t t0 = tstart();
+      hipEvent_t t0 = tstart(tr ? KC_SPMV_T : KC_SPMV_N);
       nb = csr_apply<T>(e->op, strm, trans, x, y, yprev, sval(c), e->part, Gate{});
       PK_LAUNCH(nb);
       tstop(t0, tr ? KC_SPMV_T : KC_SPMV_N, spmv_bytes(tr, yprev != nullptr),
@@ -261,7 +263,7 @@ struct Impl {
   }
 
   int scale(T* x, long long L, double a) {
-    hipEvent_t t0 = tstart();
+    hipEvent_t t0 = tstart(KC_VEC);
     PK_LAUNCH(launch_scale<T>(strm, L, x, sval(a), Gate{}));
     tstop(t0, KC_VEC, 2.0 * sizeof(T) * L);
     return PK_OK;
@@ -320,13 +322,13 @@ struct Impl {
       e->cnt.ncols_touched += ncols;
       if (ncols > 0) {
         const double sb = static_cast<double>(sizeof(T));
-        hipEvent_t t0 = tstart();
+        hipEvent_t t0 = tstart(KC_GEMVT);
         const int nb = launch_gemvt<T>(strm, L, basis, ld, e->dcols, ncols, nullptr, vnew, e->part,
                                        Gate{});
         PK_LAUNCH(nb);
         tstop(t0, KC_GEMVT, (ncols + 1.0) * L * sb, 2.0 * ncols * L);
         PK_LAUNCH(launch_reduce(strm, ncols, nb, e->part, e->red, Gate{}));
-        hipEvent_t t1 = tstart();
+        hipEvent_t t1 = tstart(KC_GEMV);
         const int nb2 =
             launch_gemv_update<T>(strm, L, basis, ld, e->dcols, ncols, nullptr, e->red, vnew,
                                   e->part, Gate{});
@@ -392,13 +394,13 @@ struct Impl {
       std::memcpy(e->hcols, c.cols, sizeof(int) * ncols);
       PK_HIP(hipMemcpyAsync(e->dcols, e->hcols, sizeof(int) * ncols, hipMemcpyHostToDevice, strm));
       const double sb = static_cast<double>(sizeof(T));
-      hipEvent_t t0 = tstart();
+      hipEvent_t t0 = tstart(KC_GEMVT);
       const int nb = launch_gemvt<T>(strm, L, basis, ld, e->dcols, ncols, nullptr, vnew, e->part,
                                      Gate{});
       PK_LAUNCH(nb);
       tstop(t0, KC_GEMVT, (ncols + 1.0) * L * sb, 2.0 * ncols * L);
       PK_LAUNCH(launch_reduce(strm, ncols, nb, e->part, e->red, Gate{}));
-      hipEvent_t t1 = tstart();
+      hipEvent_t t1 = tstart(KC_GEMV);
       const int nb2 = launch_gemv_update<T>(strm, L, basis, ld, e->dcols, ncols, nullptr, e->red,
                                             vnew, e->part, Gate{});
       PK_LAUNCH(nb2);
@@ -544,7 +546,7 @@ struct Impl {
     const double sb = static_cast<double>(sizeof(T));
     // ---- v_J = A^T u_J - beta v_{J-1} ----
     PK_LAUNCH(launch_scale<T>(strm, m, ucol(J), sptr(&dc->uscale_f), Gate{dc, PH_V_SPMV, J, 2}));
-    hipEvent_t t0 = tstart();
+    hipEvent_t t0 = tstart(KC_SPMV_T);
     int nb = csr_apply<T>(e->op, strm, 't', ucol(J), vcol(J), J > 1 ? vcol(J - 1) : nullptr,
                           sptr(&dc->beta), e->part, Gate{dc, PH_V_SPMV, J, 0});
     PK_LAUNCH(nb);
@@ -559,13 +561,13 @@ struct Impl {
       }
       for (int t = 0; t < tries; ++t) {
         const Gate g{dc, PH_V_REO, J, 0};
-        hipEvent_t t1 = tstart();
+        hipEvent_t t1 = tstart(KC_GEMVT);
         nb = launch_gemvt<T>(strm, n, V(), e->ldv, dc->cols, J - 1, &dc->reo_ncols, vcol(J),
                              e->part, g);
         PK_LAUNCH(nb);
         tstop(t1, KC_GEMVT, 0.0);
         PK_LAUNCH(launch_reduce(strm, J - 1, nb, e->part, e->red, g, &dc->reo_ncols));
-        hipEvent_t t2 = tstart();
+        hipEvent_t t2 = tstart(KC_GEMV);
         const int nb2 = launch_gemv_update<T>(strm, n, V(), e->ldv, dc->cols, J - 1,
                                               &dc->reo_ncols, e->red, vcol(J), e->part, g);
         PK_LAUNCH(nb2);
@@ -575,7 +577,7 @@ struct Impl {
     }
     // ---- u_{J+1} = A v_J - alpha u_J ----
     PK_LAUNCH(launch_scale<T>(strm, n, vcol(J), sptr(&dc->vscale_f), Gate{dc, PH_U_SPMV, J, 1}));
-    hipEvent_t t3 = tstart();
+    hipEvent_t t3 = tstart(KC_SPMV_N);
     nb = csr_apply<T>(e->op, strm, 'n', vcol(J), ucol(J + 1), ucol(J), sptr(&dc->alpha), e->part,
                       Gate{dc, PH_U_SPMV, J, 0});
     PK_LAUNCH(nb);
@@ -589,13 +591,13 @@ struct Impl {
     }
     for (int t = 0; t < tries; ++t) {
       const Gate g{dc, PH_U_REO, J, 0};
-      hipEvent_t t1 = tstart();
+      hipEvent_t t1 = tstart(KC_GEMVT);
       nb = launch_gemvt<T>(strm, m, U(), e->ldu, dc->cols, J, &dc->reo_ncols, ucol(J + 1),
                            e->part, g);
       PK_LAUNCH(nb);
       tstop(t1, KC_GEMVT, 0.0);
       PK_LAUNCH(launch_reduce(strm, J, nb, e->part, e->red, g, &dc->reo_ncols));
-      hipEvent_t t2 = tstart();
+      hipEvent_t t2 = tstart(KC_GEMV);
       const int nb2 = launch_gemv_update<T>(strm, m, U(), e->ldu, dc->cols, J, &dc->reo_ncols,
                                             e->red, ucol(J + 1), e->part, g);
       PK_LAUNCH(nb2);
@@ -620,10 +622,14 @@ struct Impl {
       const double sb = static_cast<double>(sizeof(T));
       const double pv = static_cast<double>(c.npass_v - pv0), pu = static_cast<double>(c.npass_u - pu0);
       const double cv = static_cast<double>(c.ncols_v - cv0), cu = static_cast<double>(c.ncols_u - cu0);
-      e->prof.bytes[KC_GEMVT] += sb * (e->n * (cv + pv) + e->m * (cu + pu));
-      e->prof.bytes[KC_GEMV] += sb * (e->n * (cv + 2 * pv) + e->m * (cu + 2 * pu));
-      e->prof.flops[KC_GEMVT] += 2.0 * (e->n * cv + e->m * cu);
-      e->prof.flops[KC_GEMV] += 2.0 * (e->n * cv + e->m * cu);
+      if ((e->prof.mask >> KC_GEMVT) & 1u) {
+        e->prof.bytes[KC_GEMVT] += sb * (e->n * (cv + pv) + e->m * (cu + pu));
+        e->prof.flops[KC_GEMVT] += 2.0 * (e->n * cv + e->m * cu);
+      }
+      if ((e->prof.mask >> KC_GEMV) & 1u) {
+        e->prof.bytes[KC_GEMV] += sb * (e->n * (cv + 2 * pv) + e->m * (cu + 2 * pu));
+        e->prof.flops[KC_GEMV] += 2.0 * (e->n * cv + e->m * cu);
+      }
     }
     return resolve_timers();
   }
@@ -854,7 +860,7 @@ struct Impl {
         e->hB[static_cast<size_t>(j) * kdim + l] =
             tb ? B[static_cast<size_t>(l) * ldb + j] : B[static_cast<size_t>(j) * ldb + l];
     PK_HIP(hipMemcpyAsync(e->dB, e->hB, sizeof(double) * kdim * ncols, hipMemcpyHostToDevice, strm));
-    hipEvent_t t0 = tstart();
+    hipEvent_t t0 = tstart(KC_GEMM);
     PK_LAUNCH(launch_gemm_mfma<T>(strm, L, static_cast<const T*>(buf[cur]), ld, kdim, e->dB, kdim,
                                   ncols, static_cast<T*>(buf[1 - cur]), ld));
     tstop(t0, KC_GEMM, static_cast<double>(sizeof(T)) * L * (kdim + ncols),
@@ -1202,9 +1208,10 @@ int pk_lbd_set_mode(pk_lbd_t h, int mode, int tries) {
   return PK_OK;
 }
 
-int pk_lbd_profile(pk_lbd_t h, int enable) {
+int pk_lbd_profile(pk_lbd_t h, int classes) {
   PK_ENGINE(h);
-  e->prof.on = enable != 0;
+  e->prof.mask = static_cast<unsigned>(classes) & ((1u << KC_COUNT) - 1u);
+  e->prof.on = e->prof.mask != 0;
   return PK_OK;
 }
 

@@ -74,6 +74,7 @@ struct KTimer {
 };
 struct Profile {
   bool on = false;
+  unsigned mask = 0;  // kernel classes timed (bit c: class c)
   std::vector<hipEvent_t> pool;
   std::vector<KTimer> pending;
   double ms[KC_COUNT] = {}, bytes[KC_COUNT] = {}, flops[KC_COUNT] = {};

@@ -267,7 +267,7 @@ __global__ __launch_bounds__(kBlock) void k_ctl(LbdCtl* __restrict__ gc, int kin
                                                 int want_j, const double* __restrict__ part,
                                                 int nb) {
   __shared__ LbdCtl c;
-  __shared__ double lds[kBlock / kWave];
+  __shared__ double lds[(kBlock / kWave) * 2];
   if (gc->phase != want_phase || gc->j != want_j) return;
   const bool two = (kind == CTL_V_SPMV || kind == CTL_V_ELR || kind == CTL_U_SPMV ||
                     kind == CTL_U_ELR);
