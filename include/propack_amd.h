@@ -122,7 +122,8 @@ int pk_lbd_destroy(pk_lbd_t eng);
 /* Host APROD callback (Fortran ABI, arrays in aprod_prec) used for
  * user-supplied operators: X/Y are staged through pinned host memory. */
 int pk_lbd_set_callback(pk_lbd_t eng, void* aprod, int aprod_prec, void* dparm, int* iparm);
-/* Copy column col (1-based) of U ('u') or V ('v') from / to the host. */
+/* Copy column col (1-based) of U ('u') or V ('v') from x (host or device
+ * memory) / to the host. */
 int pk_lbd_put(pk_lbd_t eng, char which, int col, const void* x, int prec);
 int pk_lbd_get(pk_lbd_t eng, char which, int ncols, void* X, int ldx, int prec);
 /* Device pointer and leading dimension of the current basis. */
@@ -258,6 +259,9 @@ int pk_lansvd_irl_batch(int nb, pk_lbd_t* engs, pk_batch_t batch, char which, ch
 /* DLARNV(IDIST=2) from the fixed seed (1,3,5,7) of double/dgetu0.F:47-50,
  * n values in double (prec 0) or float (prec 1). */
 int pk_larnv_pm1(int prec, long long n, void* x);
+/* The same n values generated on the device into x (device memory), as the
+ * engine's DGETU0 does; bit-identical to pk_larnv_pm1. */
+int pk_larnv_pm1_device(int prec, long long n, void* x);
 /* DUPDATE_MU / DUPDATE_NU / DCOMPUTE_INT / DSET_MU (double/dlanbpro.F:555-723). */
 void pkh_update_mu(double* mumax, double* mu, const double* nu, int j, const double* alpha,
                    const double* beta, double anorm, double eps1);

@@ -34,7 +34,7 @@ EXPORTS = [
     "pk_lbd_nrm2", "pk_lbd_getu0", "pk_lbd_lanbpro", "pk_lbd_update", "pk_lbd_stats",
     "pk_lbd_reset_stats", "pk_lbd_sync", "pk_lbd_shape", "pk_lansvd_run",
     "pk_lansvd_irl_run", "pk_lansvd_irl_mixed", "pk_lbd_seed", "pk_lbd_set_mode", "pk_lbd_profile", "pk_lbd_profile_get", "pk_lbd_profile_reset",
-    "pk_larnv_pm1", "pkh_update_mu", "pkh_update_nu", "pkh_compute_int", "pkh_set_mu",
+    "pk_larnv_pm1", "pk_larnv_pm1_device", "pkh_update_mu", "pkh_update_nu", "pkh_compute_int", "pkh_set_mu",
     "pkh_lartg", "pkh_bdqr", "pkh_bsvdstep", "pkh_refinebounds", "dlansvd_", "dlansvd_irl_", "slansvd_", "slansvd_irl_",
     "clearstat_", "printstat_", "pk_lbd_set_rounding", "pk_lbd_eps", "pk_batch_create",
     "pk_batch_destroy", "pk_batch_lanbpro", "pk_lansvd_irl_batch", "pk_matrix_read",
@@ -295,6 +295,15 @@ class Engine:
         self.handle = h
 
     def put_u1(self, u0):
+        """U(:,1) <- u0: a host array, or a contiguous float64 torch tensor on
+        the GPU (then a device-to-device copy)."""
+        if hasattr(u0, "is_cuda") and u0.is_cuda:
+            if str(u0.dtype) != "torch.float64" or not u0.is_contiguous() or u0.numel() != self.m:
+                raise ValueError("put_u1: device start vector must be contiguous float64 of length m")
+            ptr = C.c_void_p(u0.data_ptr())
+            _check(lib().pk_lbd_put(self.handle, C.c_char(b"u"), C.c_int(1), ptr,
+                                    C.c_int(PK_F64)), "pk_lbd_put")
+            return
         u0 = np.ascontiguousarray(u0, np.float64)
         _check(lib().pk_lbd_put(self.handle, C.c_char(b"u"), C.c_int(1), _p(u0),
                                 C.c_int(PK_F64)), "pk_lbd_put")

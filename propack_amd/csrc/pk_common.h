@@ -109,6 +109,39 @@ __device__ __forceinline__ double reduce_partials(const double* p, int n, double
   return v[0];  // valid in thread 0
 }
 
+// xLARUV's 48-bit multiplicative congruential generator (LAPACK
+// double/Lapack_Util/dlaruv.f:42-46 tabulates powers of this multiplier):
+// state s' = s * a mod 2^48, seed (1,3,5,7) as 12-bit limbs.  The low 64 bits
+// of a 64-bit product, masked to 48 bits, are the product mod 2^48.
+constexpr unsigned long long kLcgMask = (1ULL << 48) - 1;
+constexpr unsigned long long kLcgA = 33952834046453ULL;
+constexpr unsigned long long kLcgSeed = (1ULL << 36) + (3ULL << 24) + (5ULL << 12) + 7ULL;
+__host__ __device__ inline unsigned long long lcg_mulmod(unsigned long long x,
+                                                         unsigned long long y) {
+  return (x * y) & kLcgMask;
+}
+// a^e mod 2^48
+__host__ __device__ inline unsigned long long lcg_pow(unsigned long long e) {
+  unsigned long long r = 1, b = kLcgA;
+  for (; e; e >>= 1) {
+    if (e & 1) r = lcg_mulmod(r, b);
+    b = lcg_mulmod(b, b);
+  }
+  return r;
+}
+// 2 u - 1 for the state s, u converted from the four 12-bit limbs of s in
+// Horner form in T (xLARUV, then xLARNV IDIST=2); no contraction, so float
+// rounds as the reference's REAL arithmetic.
+template <typename T>
+__host__ __device__ inline T lcg_pm1(unsigned long long s) {
+#pragma clang fp contract(off)
+  const T r = T(1) / T(4096);
+  const T i1 = static_cast<T>(s >> 36), i2 = static_cast<T>((s >> 24) & 4095);
+  const T i3 = static_cast<T>((s >> 12) & 4095), i4 = static_cast<T>(s & 4095);
+  const T u = r * (i1 + r * (i2 + r * (i3 + r * i4)));
+  return T(2) * u - T(1);
+}
+
 inline int round_up(int x, int a) { return (x + a - 1) / a * a; }
 inline int cdiv(long long a, long long b) { return static_cast<int>((a + b - 1) / b); }
 

@@ -71,18 +71,15 @@ struct Mach<float> {
 // (double/dgetu0.F:47-50): a 48-bit multiplicative congruential sequence
 // with multiplier 33952834046453; output g (0-based) is
 // 2*((s0*a^(g+1) mod 2^48)/2^48) - 1, the 48-bit value converted from four
-// 12-bit limbs in Horner form in the target precision, as xLARUV does.
+// 12-bit limbs in Horner form in the target precision, as xLARUV does.  The
+// engine generates it on the device (launch_larnv_pm1); this host loop over
+// the same helpers is the C-ABI's pk_larnv_pm1.
 template <typename T>
 static void larnv_pm1(long long n, T* x) {
-  const uint64_t mask = (1ULL << 48) - 1, a = 33952834046453ULL;
-  uint64_t s = (1ULL << 36) + (3ULL << 24) + (5ULL << 12) + 7ULL;
-  const T r = T(1) / T(4096);
+  unsigned long long s = kLcgSeed;
   for (long long g = 0; g < n; ++g) {
-    s = static_cast<uint64_t>((static_cast<unsigned __int128>(s) * a) & mask);
-    const T i1 = static_cast<T>(s >> 36), i2 = static_cast<T>((s >> 24) & 4095);
-    const T i3 = static_cast<T>((s >> 12) & 4095), i4 = static_cast<T>(s & 4095);
-    const T u = r * (i1 + r * (i2 + r * (i3 + r * i4)));
-    x[g] = T(2) * u - T(1);
+    s = lcg_mulmod(s, kLcgA);
+    x[g] = lcg_pm1<T>(s);
   }
 }
 
@@ -90,6 +87,22 @@ static void larnv_pm1(long long n, T* x) {
 // pk_lbd.h, shared with the device control kernels.
 static double lapy2(double x, double y) { return hd_lapy2(x, y); }
 static void set_mu(int k, double* mu, const int* index, double val) { hd_set_mu(k, mu, index, val); }
+
+// Accumulates the pending timed regions of e (the stream has passed them).
+void profile_resolve(Engine* e) {
+  for (const KTimer& t : e->prof.pending) {
+    float ms = 0.f;
+    if (hipEventElapsedTime(&ms, t.a, t.b) == hipSuccess) {
+      e->prof.ms[t.cls] += ms;
+      e->prof.bytes[t.cls] += t.bytes;
+      e->prof.flops[t.cls] += t.flops;
+      e->prof.count[t.cls] += 1;
+    }
+    e->prof.pool.push_back(t.a);
+    e->prof.pool.push_back(t.b);
+  }
+  e->prof.pending.clear();
+}
 
 template <typename T>
 struct Impl {
@@ -141,18 +154,7 @@ struct Impl {
   }
   // Called after a stream synchronization: every pending pair has completed.
   int resolve_timers() {
-    for (const KTimer& t : e->prof.pending) {
-      float ms = 0.f;
-      if (hipEventElapsedTime(&ms, t.a, t.b) == hipSuccess) {
-        e->prof.ms[t.cls] += ms;
-        e->prof.bytes[t.cls] += t.bytes;
-        e->prof.flops[t.cls] += t.flops;
-        e->prof.count[t.cls] += 1;
-      }
-      e->prof.pool.push_back(t.a);
-      e->prof.pool.push_back(t.b);
-    }
-    e->prof.pending.clear();
+    profile_resolve(e);
     return PK_OK;
   }
   // Algorithmic bytes of y = op(A) x [- c yprev] (compulsory traffic: every
@@ -360,13 +362,11 @@ struct Impl {
     const bool tn = (transa == 'n' || transa == 'N');
     const long long rsize = tn ? e->n : e->m, usize = tn ? e->m : e->n;
     T* x = scratch(0);
-    T* hx = reinterpret_cast<T*>(e->hvec);
     // kappa is the single-precision literal 0.717 in dgetu0.F:28-29.
     const double kap = static_cast<double>(0.717f);
     *ierr = -1;
     for (int itry = 1; itry <= ntry; ++itry) {
-      larnv_pm1<T>(rsize, hx);
-      PK_HIP(hipMemcpyAsync(x, hx, sizeof(T) * rsize, hipMemcpyHostToDevice, strm));
+      PK_LAUNCH(launch_larnv_pm1<T>(strm, rsize, x));
       double nrm;
       PK_TRY(nrm2(x, rsize, &nrm));
       double s2, sd;
@@ -855,18 +855,22 @@ struct Impl {
     if (!buf[1 - cur]) {
       PK_HIP(hipMalloc(&buf[1 - cur], e->tbytes * ld * maxk));
     }
+    // The call returns with the GEMM in flight: the host refills hB only
+    // after the stream has read its previous contents, and dB is rewritten by
+    // a copy that follows the previous GEMM on the same stream.
+    if (e->evB) PK_HIP(hipEventSynchronize(e->evB));
     for (int j = 0; j < ncols; ++j)
       for (int l = 0; l < kdim; ++l)
         e->hB[static_cast<size_t>(j) * kdim + l] =
             tb ? B[static_cast<size_t>(l) * ldb + j] : B[static_cast<size_t>(j) * ldb + l];
     PK_HIP(hipMemcpyAsync(e->dB, e->hB, sizeof(double) * kdim * ncols, hipMemcpyHostToDevice, strm));
+    if (!e->evB) PK_HIP(hipEventCreateWithFlags(&e->evB, hipEventDisableTiming));
+    PK_HIP(hipEventRecord(e->evB, strm));
     hipEvent_t t0 = tstart(KC_GEMM);
     PK_LAUNCH(launch_gemm_mfma<T>(strm, L, static_cast<const T*>(buf[cur]), ld, kdim, e->dB, kdim,
                                   ncols, static_cast<T*>(buf[1 - cur]), ld));
     tstop(t0, KC_GEMM, static_cast<double>(sizeof(T)) * L * (kdim + ncols),
           2.0 * static_cast<double>(L) * kdim * ncols);
-    PK_HIP(hipStreamSynchronize(strm));
-    resolve_timers();
     cur = 1 - cur;
     return PK_OK;
   }
@@ -936,7 +940,7 @@ static void engine_free(Engine* e) {
   (void)hipHostFree(e->hred);
   (void)hipHostFree(e->hcols);
   (void)hipHostFree(e->hB);
-  (void)hipHostFree(e->hvec);
+  if (e->evB) (void)hipEventDestroy(e->evB);
   (void)hipHostFree(e->hx);
   (void)hipHostFree(e->hy);
   if (e->stream) (void)hipStreamDestroy(e->stream);
@@ -1010,7 +1014,6 @@ int pk_lbd_create(int prec, int m, int n, int kmax, pk_csr_t op, pk_lbd_t* out) 
   chk(hipHostMalloc(&e->hred, sizeof(double) * (kMaxCols + 16), hipHostMallocDefault));
   chk(hipHostMalloc(&e->hcols, sizeof(int) * kMaxCols, hipHostMallocDefault));
   chk(hipHostMalloc(&e->hB, sizeof(double) * nb, hipHostMallocDefault));
-  chk(hipHostMalloc(&e->hvec, sizeof(double) * mx, hipHostMallocDefault));
   chk(hipMalloc(&e->dctl, sizeof(LbdCtl)));
   chk(hipHostMalloc(&e->hctl, sizeof(LbdCtl), hipHostMallocDefault));
   if (const char* md = std::getenv("PK_LBD_MODE")) e->mode = (md[0] == 'h' || md[0] == 'H') ? 1 : 0;
@@ -1063,10 +1066,12 @@ int pk_lbd_put(pk_lbd_t h, char which, int col, const void* x, int prec) {
   char* dst = static_cast<char*>(isu ? e->u() : e->v()) +
               e->tbytes * static_cast<size_t>(col - 1) * (isu ? e->ldu : e->ldv);
   const size_t sb = prec == PK_F64 ? 8 : 4;
+  // hipMemcpyDefault: x may be host or device memory (the runtime reads the
+  // direction from the pointer); a device-resident x costs a device copy.
   if (sb == e->tbytes) {
-    PK_HIP(hipMemcpyAsync(dst, x, sb * L, hipMemcpyHostToDevice, e->stream));
+    PK_HIP(hipMemcpyAsync(dst, x, sb * L, hipMemcpyDefault, e->stream));
   } else {
-    PK_HIP(hipMemcpyAsync(e->vec, x, sb * L, hipMemcpyHostToDevice, e->stream));
+    PK_HIP(hipMemcpyAsync(e->vec, x, sb * L, hipMemcpyDefault, e->stream));
     int r = e->prec == PK_F64
                 ? launch_convert<float, double>(e->stream, L, static_cast<const float*>(e->vec),
                                                 reinterpret_cast<double*>(dst))
@@ -1218,6 +1223,8 @@ int pk_lbd_profile(pk_lbd_t h, int classes) {
 int pk_lbd_profile_get(pk_lbd_t h, double* out) {
   PK_ENGINE(h);
   if (!out) return PK_ERR_ARG;
+  PK_HIP(hipStreamSynchronize(e->stream));
+  profile_resolve(e);
   for (int c = 0; c < KC_COUNT; ++c) {
     out[4 * c + 0] = static_cast<double>(e->prof.count[c]);
     out[4 * c + 1] = e->prof.ms[c];
@@ -1262,6 +1269,7 @@ int pk_lbd_eps(pk_lbd_t h, double* eps, double* leff) {
 int pk_lbd_sync(pk_lbd_t h) {
   PK_ENGINE(h);
   PK_HIP(hipStreamSynchronize(e->stream));
+  profile_resolve(e);
   return PK_OK;
 }
 
@@ -1285,6 +1293,15 @@ int pk_larnv_pm1(int prec, long long n, void* x) {
   if (prec == PK_F64) larnv_pm1<double>(n, static_cast<double*>(x));
   else if (prec == PK_F32) larnv_pm1<float>(n, static_cast<float*>(x));
   else return PK_ERR_ARG;
+  return PK_OK;
+}
+
+int pk_larnv_pm1_device(int prec, long long n, void* x) {
+  if (n < 0 || (n > 0 && !x) || (prec != PK_F64 && prec != PK_F32)) return PK_ERR_ARG;
+  const int r = prec == PK_F64 ? launch_larnv_pm1<double>(nullptr, n, static_cast<double*>(x))
+                               : launch_larnv_pm1<float>(nullptr, n, static_cast<float*>(x));
+  if (r < 0) return PK_ERR_HIP;
+  PK_HIP(hipStreamSynchronize(nullptr));
   return PK_OK;
 }
 

@@ -112,8 +112,8 @@ struct Engine {
   int* hcols = nullptr;              // pinned host staging of the list
   double* dB = nullptr;              // small dense matrix for the tall GEMM
   double* hB = nullptr;              // pinned host staging
+  hipEvent_t evB = nullptr;          // the last copy out of hB (reuse guard)
   void* vec = nullptr;               // scratch vector, max(m, n)
-  double* hvec = nullptr;            // pinned host scratch (random start vectors)
   LbdCtl* dctl = nullptr;            // step control state (device)
   LbdCtl* hctl = nullptr;            // its pinned host copy
   int mode = 0;                      // 0: device-resident steps, 1: host executor

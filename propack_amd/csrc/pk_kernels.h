@@ -56,6 +56,11 @@ int launch_axpy_norm_dot(hipStream_t s, long long L, T* w, const T* v, Scal c, d
 template <typename T>
 int launch_norm(hipStream_t s, long long L, const T* x, double* part);
 
+// x[0:n) = xLARNV(IDIST=2) from the fixed seed (1,3,5,7), generated on the
+// device; bit-identical to the host restatement for T = double and float.
+template <typename T>
+int launch_larnv_pm1(hipStream_t s, long long n, T* x);
+
 // dst = src (typed copy with conversion), L elements
 template <typename S, typename D>
 int launch_convert(hipStream_t s, long long L, const S* src, D* dst);

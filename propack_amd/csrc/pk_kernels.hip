@@ -352,6 +352,34 @@ int launch_gemm_mfma(hipStream_t s, long long L, const T* A, long long lda, int 
   return 0;
 }
 
+// ------------------------------------------------------- random start ----
+// Thread t writes elements t, t + S, t + 2S, ... (S threads in the grid, so
+// every store instruction is coalesced); element g uses the state
+// s0 a^(g+1), and stepping by S multiplies the state by a^S.
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_larnv_pm1(long long n, unsigned long long step,
+                                                      T* __restrict__ x) {
+  const long long t = static_cast<long long>(blockIdx.x) * kBlock + threadIdx.x;
+  const long long S = static_cast<long long>(gridDim.x) * kBlock;
+  if (t >= n) return;
+  unsigned long long s = lcg_mulmod(kLcgSeed, lcg_pow(static_cast<unsigned long long>(t) + 1));
+  for (long long g = t; g < n; g += S) {
+    x[g] = lcg_pm1<T>(s);
+    s = lcg_mulmod(s, step);
+  }
+}
+
+template <typename T>
+int launch_larnv_pm1(hipStream_t s, long long n, T* x) {
+  if (n <= 0) return 0;
+  const int nblk = static_cast<int>(std::min<long long>(cdiv(n, kBlock), 2048));
+  const unsigned long long step = lcg_pow(static_cast<unsigned long long>(nblk) * kBlock);
+  hipLaunchKernelGGL(k_larnv_pm1<T>, dim3(nblk), dim3(kBlock), 0, s, n, step, x);
+  return hipGetLastError() == hipSuccess ? 0 : -1;
+}
+template int launch_larnv_pm1<double>(hipStream_t, long long, double*);
+template int launch_larnv_pm1<float>(hipStream_t, long long, float*);
+
 // ------------------------------------------------------------ seeding ----
 // dst = sum_{c < ncols} src(:, c), accumulated in double (start vector of an
 // FP64 polish from the Ritz vectors of an FP32-basis solve).

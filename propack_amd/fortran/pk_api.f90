@@ -9,7 +9,8 @@ module pk_api
   integer(c_int), parameter, public :: PK_OK = 0
 
   public :: pk_lbd_create, pk_lbd_destroy, pk_lbd_put, pk_lbd_get, pk_lbd_nrm2, &
-            pk_lbd_getu0, pk_lbd_lanbpro, pk_lbd_update, pk_lbd_stats, pk_lbd_reset_stats
+            pk_lbd_getu0, pk_lbd_lanbpro, pk_lbd_update, pk_lbd_sync, pk_lbd_stats, &
+            pk_lbd_reset_stats
   public :: pk_lapy2, pk_lartg, pk_bdqr, pk_bsvdstep, pk_refinebounds, lower1
 
   interface
@@ -71,6 +72,10 @@ module pk_api
       character(kind=c_char), value :: which, transb
       integer(c_int), value :: kdim, ncols, ldb
       real(c_double), intent(in) :: b(*)
+    end function
+    integer(c_int) function pk_lbd_sync(eng) bind(C, name='pk_lbd_sync')
+      import :: c_int, c_ptr
+      type(c_ptr), value :: eng
     end function
     integer(c_int) function pk_lbd_stats(eng, out) bind(C, name='pk_lbd_stats')
       import :: c_int, c_ptr, c_long_long

@@ -137,6 +137,11 @@ contains
       rc = pk_lbd_update(eng, 'v', 't', dim, k, qt(mstart, 1), dim)
       if (rc /= PK_OK) info = rc
     end if
+    ! the basis GEMMs return in flight: the Ritz vectors are complete on return
+    if (info == 0) then
+      rc = pk_lbd_sync(eng)
+      if (rc /= PK_OK) info = rc
+    end if
   end subroutine
 
   ! Non-restarted driver (DLANSVD control flow, double/dlansvd.F:130-291)

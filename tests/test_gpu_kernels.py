@@ -46,6 +46,53 @@ def test_basis_update_gemm(m, kmax, ncols, transb):
     assert np.abs(got - want).max() <= 1e-13 * np.abs(want).max() * np.sqrt(kdim)
 
 
+def test_basis_update_chained():
+    """Two updates in a row return with their GEMMs in flight: the second
+    call's B staging must wait for the first call's copy, and its GEMM reads
+    the first one's output."""
+    m, kmax = 3001, 24
+    op, eng = _engine(m, 50, kmax)
+    rng = np.random.default_rng(7)
+    U = rng.standard_normal((m, kmax + 1))
+    for j in range(kmax + 1):
+        assert api.lib().pk_lbd_put(eng.handle, C.c_char(b"u"), C.c_int(j + 1),
+                                    _p(np.ascontiguousarray(U[:, j])), C.c_int(0)) == 0
+    B1 = np.asfortranarray(rng.standard_normal((kmax + 1, 20)))
+    B2 = np.asfortranarray(rng.standard_normal((20, 9)))
+    for B in (B1, B2):
+        rc = api.lib().pk_lbd_update(eng.handle, C.c_char(b"u"), C.c_char(b"n"),
+                                     C.c_int(B.shape[0]), C.c_int(B.shape[1]), _p(B),
+                                     C.c_int(B.shape[0]))
+        assert rc == 0, api.lib().pk_last_error()
+    want = U @ B1 @ B2
+    got = eng.get("u", 9)
+    assert np.abs(got - want).max() <= 1e-12 * np.abs(want).max()
+
+
+@pytest.mark.parametrize("prec", [api.PK_F64, api.PK_F32])
+@pytest.mark.parametrize("n", [5, 1_000_003])
+def test_larnv_device_matches_host(prec, n):
+    """DGETU0's random vector generated on the device equals the host
+    restatement (pinned against the reference's xLARNV) bit for bit; n beyond
+    the generator's grid exercises the per-thread stepping."""
+    import torch
+    dt = np.float64 if prec == api.PK_F64 else np.float32
+    host = np.zeros(n, dt)
+    assert api.lib().pk_larnv_pm1(prec, C.c_longlong(n), _p(host)) == 0
+    dev = torch.zeros(n, dtype=torch.float64 if prec == api.PK_F64 else torch.float32,
+                      device="cuda")
+    assert api.lib().pk_larnv_pm1_device(prec, C.c_longlong(n), C.c_void_p(dev.data_ptr())) == 0
+    assert np.array_equal(dev.cpu().numpy().view(np.uint8), host.view(np.uint8))
+
+
+def test_put_u1_from_device():
+    import torch
+    op, eng = _engine(4097, 300, 6)
+    x = np.random.default_rng(3).standard_normal(4097)
+    eng.put_u1(torch.from_numpy(x).to("cuda"))
+    assert np.array_equal(eng.get("u", 1)[:, 0], x)
+
+
 def test_put_get_roundtrip_f32():
     op, eng = _engine(300, 200, 10, prec=api.PK_F32)
     x = np.random.default_rng(1).standard_normal(300)
