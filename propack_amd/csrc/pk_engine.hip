@@ -88,6 +88,15 @@ static void larnv_pm1(long long n, T* x) {
 static double lapy2(double x, double y) { return hd_lapy2(x, y); }
 static void set_mu(int k, double* mu, const int* index, double val) { hd_set_mu(k, mu, index, val); }
 
+// Waits for stream s by polling.  The engine's waits sit between dependent
+// host steps of a solve (control state back, small reductions read); a
+// blocking wait adds the host thread's wake-up latency to each of them.
+static hipError_t stream_wait(hipStream_t s) {
+  hipError_t r;
+  while ((r = hipStreamQuery(s)) == hipErrorNotReady) __builtin_ia32_pause();
+  return r;
+}
+
 // Accumulates the pending timed regions of e (the stream has passed them).
 void profile_resolve(Engine* e) {
   for (const KTimer& t : e->prof.pending) {
@@ -121,7 +130,7 @@ struct Impl {
   int read(int offset, int nvals) {
     PK_HIP(hipMemcpyAsync(e->hred + offset, e->red + offset, sizeof(double) * nvals,
                           hipMemcpyDeviceToHost, strm));
-    PK_HIP(hipStreamSynchronize(strm));
+    PK_HIP(stream_wait(strm));
     e->cnt.nsync++;
     return resolve_timers();
   }
@@ -615,7 +624,7 @@ struct Impl {
     PK_HIP(hipMemcpyAsync(e->dctl, &c, sizeof(LbdCtl), hipMemcpyHostToDevice, strm));
     for (int J = c.j; J <= c.k; ++J) PK_TRY(enqueue_step(J, elr, e->dev_tries));
     PK_HIP(hipMemcpyAsync(&c, e->dctl, sizeof(LbdCtl), hipMemcpyDeviceToHost, strm));
-    PK_HIP(hipStreamSynchronize(strm));
+    PK_HIP(stream_wait(strm));
     e->cnt.nsync++;
     // algorithmic bytes of the CGS passes the chain actually ran
     if (e->prof.on) {
@@ -831,7 +840,7 @@ struct Impl {
     e->cnt.ndot += c.ndot;
     e->cnt.ncols_touched += c.ncols_touched;
     e->cnt.nretry += c.nretry;
-    PK_HIP(hipStreamSynchronize(strm));
+    PK_HIP(stream_wait(strm));
     return resolve_timers();
   }
 
@@ -1268,7 +1277,7 @@ int pk_lbd_eps(pk_lbd_t h, double* eps, double* leff) {
 
 int pk_lbd_sync(pk_lbd_t h) {
   PK_ENGINE(h);
-  PK_HIP(hipStreamSynchronize(e->stream));
+  PK_HIP(stream_wait(e->stream));
   profile_resolve(e);
   return PK_OK;
 }

@@ -121,7 +121,7 @@ contains
     call pk_bdqr(dim == min(m, n), jobu, dim, d, e, c1, c2, mt, dim + 1)
     call dbdsdc('U', 'I', dim, d, e, p, dim, qt, dim, dd, id, wrk, iwk, lapinfo)
     ! X(1:dim, 1:dim+1) = P^T * M^T(1:dim, 1:dim+1)
-    x(1:dim, 1:dim + 1) = matmul(transpose(p), mt(1:dim, 1:dim + 1))
+    call dgemm('T', 'N', dim, dim + 1, dim, 1d0, p, dim, mt, dim + 1, 0d0, x, dim + 1)
     if (lower1(which) == 's') then
       mstart = dim - k + 1
     else
