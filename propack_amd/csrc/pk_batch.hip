@@ -33,6 +33,8 @@ struct BProb {
   int m, n;
   int sm0, nsm;  // workgroup slices over m rows: first global slice, count
   int sn0, nsn;  // over n rows
+  int cm0, ncm;  // Gram-Schmidt slices (cgs_blocks) over m rows
+  int cn0, ncn;  // over n rows
   int qa0, nqa;  // partial slots of A v (last panel): first global slot, count
   int qt0, nqt;  // of A^T u
   int npa, npt;  // column panels of A, A^T
@@ -139,8 +141,8 @@ __global__ __launch_bounds__(kBlock) void k_bctl(const BProb* __restrict__ probs
     case CTL_U_SPMV: pa = part + P.qa0; pb = part + Qa + P.qa0; na = P.nqa; break;
     case CTL_V_ELR: pa = part + P.sn0; pb = part + Sn + P.sn0; na = P.nsn; break;
     case CTL_U_ELR: pa = part + P.sm0; pb = part + Sm + P.sm0; na = P.nsm; break;
-    case CTL_V_REO: pa = part + P.sn0; na = P.nsn; break;
-    default: pa = part + P.sm0; na = P.nsm; break;  // CTL_U_REO
+    case CTL_V_REO: pa = part + P.cn0; na = P.ncn; break;
+    default: pa = part + P.cm0; na = P.ncm; break;  // CTL_U_REO
   }
   ctl_stage_body(P.ctl, kind, pa, na, pb, na, c, lds);
 }
@@ -169,12 +171,10 @@ __global__ __launch_bounds__(kBlock) void k_bgemvt(const BProb* __restrict__ pro
                                                    int S) {
   const BProb P = probs[slice_pid[blockIdx.x]];
   if (Gate{P.ctl, phase, J, 0}.off()) return;
-  __shared__ int scol[kMaxCols];
   __shared__ double lds[2][(kBlock / kWave) * kCT];
   const int ncols = P.ctl->reo_ncols;
-  for (int i = threadIdx.x; i < ncols; i += kBlock) scol[i] = P.ctl->cols[i];
-  __syncthreads();
-  const long long b = static_cast<long long>(blockIdx.x) - (vside ? P.sn0 : P.sm0);
+  const int* scol = P.ctl->cols;
+  const long long b = static_cast<long long>(blockIdx.x) - (vside ? P.cn0 : P.cm0);
   const T* basis = static_cast<const T*>(vside ? P.V : P.U);
   const T* w = vside ? bcol<T>(P, true, J) : bcol<T>(P, false, J + 1);
   gemvt_body<T>(b, vside ? P.n : P.m, basis, vside ? P.ldv : P.ldu, scol, ncols, w, part, S,
@@ -192,7 +192,7 @@ __global__ __launch_bounds__(kBlock) void k_breduce(const BProb* __restrict__ pr
   const int c = blockIdx.y;
   if (c >= P.ctl->reo_ncols) return;
   __shared__ double lds[kBlock / kWave];
-  const int s0 = vside ? P.sn0 : P.sm0, ns = vside ? P.nsn : P.nsm;
+  const int s0 = vside ? P.cn0 : P.cm0, ns = vside ? P.ncn : P.ncm;
   const double v = reduce_partials(part + static_cast<long long>(c) * S + s0, ns, lds);
   if (threadIdx.x == 0) red[static_cast<long long>(blockIdx.x) * kMaxCols + c] = v;
 }
@@ -206,16 +206,11 @@ __global__ __launch_bounds__(kBlock) void k_bgemv(const BProb* __restrict__ prob
   const int pid = slice_pid[blockIdx.x];
   const BProb P = probs[pid];
   if (Gate{P.ctl, phase, J, 0}.off()) return;
-  __shared__ int scol[kMaxCols];
-  __shared__ double sh[kMaxCols];
   __shared__ double lds[kBlock / kWave];
   const int ncols = P.ctl->reo_ncols;
-  for (int i = threadIdx.x; i < ncols; i += kBlock) {
-    scol[i] = P.ctl->cols[i];
-    sh[i] = red[static_cast<long long>(pid) * kMaxCols + i];
-  }
-  __syncthreads();
-  const long long b = static_cast<long long>(blockIdx.x) - (vside ? P.sn0 : P.sm0);
+  const int* scol = P.ctl->cols;
+  const double* sh = red + static_cast<long long>(pid) * kMaxCols;
+  const long long b = static_cast<long long>(blockIdx.x) - (vside ? P.cn0 : P.cm0);
   const T* basis = static_cast<const T*>(vside ? P.V : P.U);
   T* w = vside ? bcol<T>(P, true, J) : bcol<T>(P, false, J + 1);
   gemv_update_body<T>(b, vside ? P.n : P.m, basis, vside ? P.ldv : P.ldu, scol, sh, ncols, w,
@@ -233,8 +228,11 @@ struct Batch {
   hipStream_t stream = nullptr;
   // static tables (operators do not change)
   int Sm = 0, Sn = 0, Qa = 0, Qt = 0, maxk = 0;
+  int Cm = 0, Cn = 0;          // Gram-Schmidt slices over m, n rows
   int* slice_pid_m = nullptr;  // global slice over m -> problem
   int* slice_pid_n = nullptr;
+  int* cslice_pid_m = nullptr;  // Gram-Schmidt slice over m -> problem
+  int* cslice_pid_n = nullptr;
   BPanel* panels = nullptr;    // (pid * 2 + tr) * kPanelsMax + p
   struct Launch {              // one batched SpMV launch
     int tr, p, g, count;
@@ -313,11 +311,11 @@ void enqueue_batch_step(Batch& B, int J, int elr, int tries) {
                          B.Qt, B.Sm, B.Sn);
     }
     for (int t = 0; t < tries; ++t) {
-      hipLaunchKernelGGL(k_bgemvt<T>, dim3(B.Sn), dim3(kBlock), 0, s, B.dprob, B.slice_pid_n, 1,
-                         J, static_cast<int>(PH_V_REO), B.part, B.Sn);
+      hipLaunchKernelGGL(k_bgemvt<T>, dim3(B.Cn), dim3(kBlock), 0, s, B.dprob, B.cslice_pid_n, 1,
+                         J, static_cast<int>(PH_V_REO), B.part, B.Cn);
       hipLaunchKernelGGL(k_breduce, dim3(np, J - 1), dim3(kBlock), 0, s, B.dprob, 1, J,
-                         static_cast<int>(PH_V_REO), B.part, B.Sn, B.red);
-      hipLaunchKernelGGL(k_bgemv<T>, dim3(B.Sn), dim3(kBlock), 0, s, B.dprob, B.slice_pid_n, 1,
+                         static_cast<int>(PH_V_REO), B.part, B.Cn, B.red);
+      hipLaunchKernelGGL(k_bgemv<T>, dim3(B.Cn), dim3(kBlock), 0, s, B.dprob, B.cslice_pid_n, 1,
                          J, static_cast<int>(PH_V_REO), B.red, B.part);
       hipLaunchKernelGGL(k_bctl, dim3(np), dim3(kBlock), 0, s, B.dprob,
                          static_cast<int>(CTL_V_REO), static_cast<int>(PH_V_REO), J, B.part, B.Qa,
@@ -337,11 +335,11 @@ void enqueue_batch_step(Batch& B, int J, int elr, int tries) {
                        static_cast<int>(PH_U_ELR), J, B.part, B.Qa, B.Qt, B.Sm, B.Sn);
   }
   for (int t = 0; t < tries; ++t) {
-    hipLaunchKernelGGL(k_bgemvt<T>, dim3(B.Sm), dim3(kBlock), 0, s, B.dprob, B.slice_pid_m, 0, J,
-                       static_cast<int>(PH_U_REO), B.part, B.Sm);
+    hipLaunchKernelGGL(k_bgemvt<T>, dim3(B.Cm), dim3(kBlock), 0, s, B.dprob, B.cslice_pid_m, 0, J,
+                       static_cast<int>(PH_U_REO), B.part, B.Cm);
     hipLaunchKernelGGL(k_breduce, dim3(np, J), dim3(kBlock), 0, s, B.dprob, 0, J,
-                       static_cast<int>(PH_U_REO), B.part, B.Sm, B.red);
-    hipLaunchKernelGGL(k_bgemv<T>, dim3(B.Sm), dim3(kBlock), 0, s, B.dprob, B.slice_pid_m, 0, J,
+                       static_cast<int>(PH_U_REO), B.part, B.Cm, B.red);
+    hipLaunchKernelGGL(k_bgemv<T>, dim3(B.Cm), dim3(kBlock), 0, s, B.dprob, B.cslice_pid_m, 0, J,
                        static_cast<int>(PH_U_REO), B.red, B.part);
     hipLaunchKernelGGL(k_bctl, dim3(np), dim3(kBlock), 0, s, B.dprob, static_cast<int>(CTL_U_REO),
                        static_cast<int>(PH_U_REO), J, B.part, B.Qa, B.Qt, B.Sm, B.Sn);
@@ -360,6 +358,8 @@ void batch_free(Batch* b) {
   }
   (void)hipFree(b->slice_pid_m);
   (void)hipFree(b->slice_pid_n);
+  (void)hipFree(b->cslice_pid_m);
+  (void)hipFree(b->cslice_pid_n);
   (void)hipFree(b->panels);
   (void)hipFree(b->dprob);
   (void)hipFree(b->part);
@@ -383,7 +383,7 @@ int batch_create(int nprob, const pk_lbd_t* handles, Batch** out) {
   B->magic = Batch::kMagic;
   B->nprob = nprob;
   int rc = PK_OK;
-  std::vector<int> spm, spn;
+  std::vector<int> spm, spn, cpm, cpn;
   std::vector<BPanel> hpan(static_cast<size_t>(nprob) * 2 * kPanelsMax, BPanel{});
   // SpMV work lists keyed by (tr, panel, lanes per row)
   struct Key {
@@ -427,6 +427,14 @@ int batch_create(int nprob, const pk_lbd_t* handles, Batch** out) {
     B->Sn += P.nsn;
     for (int s = 0; s < P.nsm; ++s) spm.push_back(i);
     for (int s = 0; s < P.nsn; ++s) spn.push_back(i);
+    P.cm0 = B->Cm;
+    P.ncm = cgs_blocks(e->m);
+    P.cn0 = B->Cn;
+    P.ncn = cgs_blocks(e->n);
+    B->Cm += P.ncm;
+    B->Cn += P.ncn;
+    for (int s = 0; s < P.ncm; ++s) cpm.push_back(i);
+    for (int s = 0; s < P.ncn; ++s) cpn.push_back(i);
     B->maxk = std::max(B->maxk, e->kmax);
     for (int tr = 0; tr < 2 && rc == PK_OK; ++tr) {
       const CsrOp::Dir& D = tr ? e->op->dt : e->op->da;
@@ -484,6 +492,7 @@ int batch_create(int nprob, const pk_lbd_t* handles, Batch** out) {
     rc = PK_ERR_HIP;
   if (rc == PK_OK) {
     bool ok = upload_vec(spm, &B->slice_pid_m) && upload_vec(spn, &B->slice_pid_n) &&
+              upload_vec(cpm, &B->cslice_pid_m) && upload_vec(cpn, &B->cslice_pid_n) &&
               upload_vec(hpan, &B->panels);
     for (auto& k : keys) {
       Batch::Launch L{k.tr, k.p, k.g, static_cast<int>(k.desc.size()), nullptr, nullptr};

@@ -13,9 +13,11 @@ namespace pk {
 // length L owns the contiguous slice [b * kRowsPerBlock, (b + 1) * kRowsPerBlock).
 constexpr int kRPT = 4;
 constexpr int kRowsPerBlock = kBlock * kRPT;
-// Column tile width of the Gram-Schmidt projection: kCT accumulators per
-// thread, kCT column loads in flight per row held.
-constexpr int kCT = 8;
+// Gram-Schmidt kernels (projection and update) own slices of kCgsRows rows,
+// kCgsRPT rows per thread, and stream kCT-column tiles of the basis.
+constexpr int kCgsRPT = 8;
+constexpr int kCgsRows = kBlock * kCgsRPT;
+constexpr int kCT = 4;
 
 // ------------------------------------------------------ row-block SpMV ----
 // Descriptor d = {x, y, z, w}:
@@ -144,37 +146,66 @@ __device__ __forceinline__ void spmv_longfix_body(const int4 d, const double* __
 }
 
 // ----------------------------------------------- Gram-Schmidt projection ----
-// Slice b of h = V(:, scol[0:ncols])^T w: the slice of w stays in registers,
-// kCT-column tiles of V stream through (wave shuffles + LDS per tile), the
-// next tile's loads issued before the current tile is reduced.  Partial of
-// column c goes to part[c * stride + slot].  scol in LDS.
+// Sums of kCT = 4 per-lane partials over the wave by a butterfly: the
+// exchanges at lane distances 32 and 16 halve the values a lane holds (lane
+// keeps column 2*bit5 + bit4 of its index), distances 8..1 finish the sum.
+// Lanes with (lane & 15) == 0 return the sum of column lane >> 4.
+__device__ __forceinline__ double wave_sum4(double (&v)[4], int lane) {
+  static_assert(kCT == 4, "the butterfly is written for four columns");
+  const bool h5 = lane & 32, h4 = lane & 16;
+#pragma unroll
+  for (int k = 0; k < 2; ++k) {
+    const double send = h5 ? v[k] : v[k + 2];
+    const double keep = h5 ? v[k + 2] : v[k];
+    v[k] = keep + __shfl_xor(send, 32, kWave);
+  }
+  {
+    const double send = h4 ? v[0] : v[1];
+    const double keep = h4 ? v[1] : v[0];
+    v[0] = keep + __shfl_xor(send, 16, kWave);
+  }
+  double t = v[0];
+#pragma unroll
+  for (int off = 8; off > 0; off >>= 1) t += __shfl_xor(t, off, kWave);
+  return t;
+}
+
+// Slice b (kCgsRows rows) of h = V(:, scol[0:ncols])^T w: the slice of w
+// stays in registers, kCT-column tiles of V stream through (a butterfly over
+// the wave, then LDS across the waves, per tile), the next tile's loads
+// issued before the current tile is reduced.  Partial of column c goes to
+// part[c * stride + slot].  scol: global column list, read with uniform
+// indices (scalar loads, no LDS staging).  Rows past L load row L - 1 (every
+// load unconditional, so a tile's loads issue back to back) and are weighted
+// by 0; columns past ncols load the last column and are not written.
 template <typename T>
 __device__ __forceinline__ void gemvt_body(long long b, long long L, const T* __restrict__ V,
-                                           long long ldv, const int* scol, int ncols,
-                                           const T* __restrict__ w, double* __restrict__ part,
+                                           long long ldv, const int* __restrict__ scol,
+                                           int ncols, const T* __restrict__ w,
+                                           double* __restrict__ part,
                                            long long stride, long long slot,
                                            double (*lds)[(kBlock / kWave) * kCT]) {
-  const long long base = b * kRowsPerBlock + threadIdx.x;
-  double wr[kRPT];
-  bool ok[kRPT];
+  if (ncols <= 0 || L <= 0) return;
+  const long long base = b * kCgsRows + threadIdx.x;
+  double wr[kCgsRPT];
+  long long rowc[kCgsRPT];
 #pragma unroll
-  for (int r = 0; r < kRPT; ++r) {
+  for (int r = 0; r < kCgsRPT; ++r) {
     const long long row = base + static_cast<long long>(r) * kBlock;
-    ok[r] = row < L;
-    wr[r] = ok[r] ? static_cast<double>(w[row]) : 0.0;
+    rowc[r] = row < L ? row : L - 1;
+    const double wv = static_cast<double>(w[rowc[r]]);
+    wr[r] = row < L ? wv : 0.0;
   }
   const int lane = threadIdx.x % kWave, wid = threadIdx.x / kWave;
   int buf = 0;
   // software pipeline: the loads of the next column tile are in flight
   // while the current tile is reduced
-  T cur[kCT][kRPT], nxt[kCT][kRPT];
-  if (ncols <= 0) return;
+  T cur[kCT][kCgsRPT], nxt[kCT][kCgsRPT];
 #pragma unroll
   for (int c = 0; c < kCT; ++c) {
     const T* vc = V + static_cast<long long>(scol[c < ncols ? c : ncols - 1]) * ldv;
 #pragma unroll
-    for (int r = 0; r < kRPT; ++r)
-      cur[c][r] = (ok[r] && c < ncols) ? vc[base + static_cast<long long>(r) * kBlock] : T(0);
+    for (int r = 0; r < kCgsRPT; ++r) cur[c][r] = vc[rowc[r]];
   }
   for (int c0 = 0; c0 < ncols; c0 += kCT) {
     const int n0 = c0 + kCT;
@@ -183,9 +214,7 @@ __device__ __forceinline__ void gemvt_body(long long b, long long L, const T* __
       for (int c = 0; c < kCT; ++c) {
         const T* vc = V + static_cast<long long>(scol[n0 + c < ncols ? n0 + c : ncols - 1]) * ldv;
 #pragma unroll
-        for (int r = 0; r < kRPT; ++r)
-          nxt[c][r] = (ok[r] && n0 + c < ncols) ? vc[base + static_cast<long long>(r) * kBlock]
-                                                : T(0);
+        for (int r = 0; r < kCgsRPT; ++r) nxt[c][r] = vc[rowc[r]];
       }
     }
     double acc[kCT];
@@ -193,14 +222,10 @@ __device__ __forceinline__ void gemvt_body(long long b, long long L, const T* __
     for (int c = 0; c < kCT; ++c) {
       acc[c] = 0.0;
 #pragma unroll
-      for (int r = 0; r < kRPT; ++r) acc[c] += static_cast<double>(cur[c][r]) * wr[r];
+      for (int r = 0; r < kCgsRPT; ++r) acc[c] += static_cast<double>(cur[c][r]) * wr[r];
     }
-#pragma unroll
-    for (int c = 0; c < kCT; ++c) acc[c] = wave_sum(acc[c]);
-    if (lane == 0) {
-#pragma unroll
-      for (int c = 0; c < kCT; ++c) lds[buf][wid * kCT + c] = acc[c];
-    }
+    const double colsum = wave_sum4(acc, lane);
+    if ((lane & 15) == 0) lds[buf][wid * kCT + (lane >> 4)] = colsum;
     __syncthreads();
     if (threadIdx.x < kCT && c0 + threadIdx.x < ncols) {
       double sum = 0.0;
@@ -211,54 +236,51 @@ __device__ __forceinline__ void gemvt_body(long long b, long long L, const T* __
 #pragma unroll
     for (int c = 0; c < kCT; ++c)
 #pragma unroll
-      for (int r = 0; r < kRPT; ++r) cur[c][r] = nxt[c][r];
+      for (int r = 0; r < kCgsRPT; ++r) cur[c][r] = nxt[c][r];
   }
 }
 
-// Slice b of w -= V(:, scol[0:ncols]) sh (scol, sh in LDS); *q = ||w slice||^2.
-// kCT columns' loads are issued together; the updates keep column order.
+// Slice b (kCgsRows rows) of w -= V(:, scol[0:ncols]) sh (scol, sh global,
+// read with uniform indices); *q = ||w slice||^2.  kCT columns' loads are
+// issued together (unconditional: rows past L load
+// row L - 1 and are never stored, columns past ncols load the last column
+// and are not applied); the updates keep column order.
 template <typename T>
 __device__ __forceinline__ void gemv_update_body(long long b, long long L,
                                                  const T* __restrict__ V, long long ldv,
-                                                 const int* scol, const double* sh, int ncols,
+                                                 const int* __restrict__ scol,
+                                                 const double* __restrict__ sh, int ncols,
                                                  T* __restrict__ w, double* q, double* lds) {
-  const long long base = b * kRowsPerBlock + threadIdx.x;
-  double acc[kRPT];
+  if (L <= 0) return;
+  const long long base = b * kCgsRows + threadIdx.x;
+  double acc[kCgsRPT];
+  long long rowc[kCgsRPT];
 #pragma unroll
-  for (int r = 0; r < kRPT; ++r) {
+  for (int r = 0; r < kCgsRPT; ++r) {
     const long long row = base + static_cast<long long>(r) * kBlock;
-    acc[r] = row < L ? static_cast<double>(w[row]) : 0.0;
+    rowc[r] = row < L ? row : L - 1;
+    acc[r] = static_cast<double>(w[rowc[r]]);
   }
-  bool ok[kRPT];
-#pragma unroll
-  for (int r = 0; r < kRPT; ++r) ok[r] = base + static_cast<long long>(r) * kBlock < L;
-  int c = 0;
-  for (; c + kCT <= ncols; c += kCT) {  // kCT columns' loads in flight together
-    T x[kCT][kRPT];
+  for (int c = 0; c < ncols; c += kCT) {  // kCT columns' loads in flight together
+    T x[kCT][kCgsRPT];
 #pragma unroll
     for (int u = 0; u < kCT; ++u) {
-      const T* vc = V + static_cast<long long>(scol[c + u]) * ldv;
+      const T* vc = V + static_cast<long long>(scol[c + u < ncols ? c + u : ncols - 1]) * ldv;
 #pragma unroll
-      for (int r = 0; r < kRPT; ++r)
-        x[u][r] = ok[r] ? vc[base + static_cast<long long>(r) * kBlock] : T(0);
+      for (int r = 0; r < kCgsRPT; ++r) x[u][r] = vc[rowc[r]];
     }
 #pragma unroll
     for (int u = 0; u < kCT; ++u) {
-      const double hc = sh[c + u];
+      if (c + u < ncols) {
+        const double hc = sh[c + u];
 #pragma unroll
-      for (int r = 0; r < kRPT; ++r) acc[r] -= static_cast<double>(x[u][r]) * hc;
+        for (int r = 0; r < kCgsRPT; ++r) acc[r] -= static_cast<double>(x[u][r]) * hc;
+      }
     }
-  }
-  for (; c < ncols; ++c) {
-    const T* vc = V + static_cast<long long>(scol[c]) * ldv;
-    const double hc = sh[c];
-#pragma unroll
-    for (int r = 0; r < kRPT; ++r)
-      if (ok[r]) acc[r] -= static_cast<double>(vc[base + static_cast<long long>(r) * kBlock]) * hc;
   }
   double v[1] = {0.0};
 #pragma unroll
-  for (int r = 0; r < kRPT; ++r) {
+  for (int r = 0; r < kCgsRPT; ++r) {
     const long long row = base + static_cast<long long>(r) * kBlock;
     if (row < L) {
       const T o = static_cast<T>(acc[r]);

@@ -8,8 +8,10 @@
 namespace pk {
 
 // Partial-sum buffer layout: part[v * nblk + b] for value v of block b.
-// stream_blocks(L) is the fixed grid every streaming kernel over L rows uses.
+// stream_blocks(L) is the fixed grid every streaming kernel over L rows uses;
+// cgs_blocks(L) the grid of the Gram-Schmidt kernels (larger row slices).
 int stream_blocks(long long L);
+int cgs_blocks(long long L);
 
 // Row-block SpMV over nblk workgroup descriptors (row blocks of at most
 // kSpmvRows rows / kSpmvCap nonzeros, or chunks of kLongChunk nonzeros of a
@@ -32,7 +34,8 @@ int launch_spmv_rb(hipStream_t s, int nblk, const int4* desc, int nlong, const i
 int launch_reduce(hipStream_t s, int nval, int nblk, const double* part, double* out,
                   Gate gate, const int* nval_p = nullptr);
 
-// h_part(ncols x nblk) = partial V(:,cols)^T w over row blocks.
+// h_part(ncols x nblk) = partial V(:,cols)^T w over row blocks
+// (nblk = cgs_blocks(L), returned).
 template <typename T>
 int launch_gemvt(hipStream_t s, long long L, const T* V, long long ldv, const int* cols,
                  int ncols, const int* ncols_p, const T* w, double* part, Gate gate);

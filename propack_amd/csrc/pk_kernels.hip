@@ -19,6 +19,7 @@
 namespace pk {
 
 int stream_blocks(long long L) { return L > 0 ? cdiv(L, kRowsPerBlock) : 1; }
+int cgs_blocks(long long L) { return L > 0 ? cdiv(L, kCgsRows) : 1; }
 
 // ------------------------------------------------------ row-block SpMV ----
 // One workgroup per descriptor (row block or long-row chunk, see
@@ -119,18 +120,15 @@ __global__ __launch_bounds__(kBlock) void k_gemvt(long long L, const T* __restri
                                                   double* __restrict__ part, Gate gate) {
   if (gate.off()) return;
   const int ncols = ncols_p ? *ncols_p : ncols_v;
-  __shared__ int scol[kMaxCols];
   __shared__ double lds[2][(kBlock / kWave) * kCT];
-  for (int i = threadIdx.x; i < ncols; i += kBlock) scol[i] = cols[i];
-  __syncthreads();
-  gemvt_body<T>(blockIdx.x, L, V, ldv, scol, ncols, w, part, gridDim.x, blockIdx.x, lds);
+  gemvt_body<T>(blockIdx.x, L, V, ldv, cols, ncols, w, part, gridDim.x, blockIdx.x, lds);
 }
 
 template <typename T>
 int launch_gemvt(hipStream_t s, long long L, const T* V, long long ldv, const int* cols,
                  int ncols, const int* ncols_p, const T* w, double* part, Gate gate) {
   if (ncols > kMaxCols) return -1;
-  const int nblk = stream_blocks(L);
+  const int nblk = cgs_blocks(L);
   hipLaunchKernelGGL(k_gemvt<T>, dim3(nblk), dim3(kBlock), 0, s, L, V, ldv, cols, ncols,
                      ncols_p, w, part, gate);
   return hipGetLastError() == hipSuccess ? nblk : -1;
@@ -148,15 +146,8 @@ __global__ __launch_bounds__(kBlock) void k_gemv_update(long long L, const T* __
                                                         Gate gate) {
   if (gate.off()) return;
   const int ncols = ncols_p ? *ncols_p : ncols_v;
-  __shared__ int scol[kMaxCols];
-  __shared__ double sh[kMaxCols];
   __shared__ double lds[kBlock / kWave];
-  for (int i = threadIdx.x; i < ncols; i += kBlock) {
-    scol[i] = cols[i];
-    sh[i] = h[i];
-  }
-  __syncthreads();
-  gemv_update_body<T>(blockIdx.x, L, V, ldv, scol, sh, ncols, w, part + blockIdx.x, lds);
+  gemv_update_body<T>(blockIdx.x, L, V, ldv, cols, h, ncols, w, part + blockIdx.x, lds);
 }
 
 template <typename T>
@@ -164,7 +155,7 @@ int launch_gemv_update(hipStream_t s, long long L, const T* V, long long ldv, co
                        int ncols, const int* ncols_p, const double* h, T* w, double* part,
                        Gate gate) {
   if (ncols > kMaxCols) return -1;
-  const int nblk = stream_blocks(L);
+  const int nblk = cgs_blocks(L);
   hipLaunchKernelGGL(k_gemv_update<T>, dim3(nblk), dim3(kBlock), 0, s, L, V, ldv, cols, ncols,
                      ncols_p, h, w, part, gate);
   return hipGetLastError() == hipSuccess ? nblk : -1;
