@@ -52,12 +52,13 @@ def log(*a):
 TRAFFIC_PROFILE = os.path.join(ROOT, "profiles", "hbm_traffic_spmv_poisson_decay_2Mx1M.md")
 
 
-def measured_traffic(kclass, panels):
+def measured_traffic(kclass, panels, lanes):
     """HBM bytes per operator application of kernel class kclass, as
     measured by the rocprofv3 FETCH_SIZE / WRITE_SIZE runs summarised in
     profiles/ (calibrated against a plain copy); None if absent or if the
-    summary was measured on another column-panel layout."""
-    traffic, layout = None, None
+    summary was measured on another SpMV layout (column panels and
+    lanes-per-row shifts)."""
+    traffic, layout, lane_shifts = None, None, (0, 0)
     try:
         with open(TRAFFIC_PROFILE) as f:
             for line in f:
@@ -66,9 +67,11 @@ def measured_traffic(kclass, panels):
                     traffic = float(w[2])
                 if len(w) == 3 and w[0] == "measured_layout":
                     layout = (int(w[1]), int(w[2]))
+                if len(w) == 3 and w[0] == "measured_lanes":
+                    lane_shifts = (int(w[1]), int(w[2]))
     except OSError:
         pass
-    return traffic if layout == tuple(panels) else None
+    return traffic if layout == tuple(panels) and lane_shifts == tuple(lanes) else None
 
 
 def residuals(op, r):
@@ -209,9 +212,10 @@ def workload_batch(args):
     against a per-problem loop on the same engines."""
     t0 = time.perf_counter()
     probs = []
+    lanes = tuple(int(x) for x in args.lanes.split(",")) if args.lanes else None
     for prm in synth.batch_params(256)[: args.count]:
         A = synth.batch_problem(prm, dtype=np.float32)
-        op = api.CsrOperator(A, prec=api.PK_F32)
+        op = api.CsrOperator(A, prec=api.PK_F32, lanes=lanes)
         e = api.Engine(op, kmax=48, prec=api.PK_F32)
         e.set_rounding(1)
         probs.append((op, e, synth.start_vector(A.m, prm["seed"] + 7), A.nnz))
@@ -259,6 +263,9 @@ def main():
                     help="built-in SpMV column panels instead of the autotuner's")
     ap.add_argument("--panels", default=None,
                     help="explicit SpMV column panels 'A,At' (no autotuning)")
+    ap.add_argument("--lanes", default=None,
+                    help="lanes-per-row shifts 'A,At' with --panels, or of every problem "
+                         "of --workload batch (default 0,0)")
     ap.add_argument("--workload", choices=["c2", "powerlaw", "batch"], default="c2",
                     help="c2 (default): the headline solve; powerlaw: 8M x 4M Zipf "
                          "bipartite, k=64, dim=160, tol=1e-8, FP32 basis + FP64 polish vs "
@@ -287,16 +294,18 @@ def main():
     u0 = synth.start_vector(M, SEED + 1)
     log(f"[bench] matrix {A.m}x{A.n} nnz={A.nnz} generated in {time.perf_counter() - t0:.1f}s")
     t0 = time.perf_counter()
-    panels, tuned = None, None
+    panels, lanes, tuned = None, None, None
     if args.panels:
         panels = tuple(int(x) for x in args.panels.split(","))
+        lanes = tuple(int(x) for x in args.lanes.split(",")) if args.lanes else None
     elif not args.no_tune:
-        panels, tuned = api.autotune(A)
-        log(f"[bench] SpMV autotune: panels A={panels[0]} A^T={panels[1]} "
-            f"({tuned[0]:.1f} / {tuned[1]:.1f} us per application) in "
-            f"{time.perf_counter() - t0:.1f}s")
+        tuned = api.autotune(A)
+        panels, lanes = tuned.panels, tuned.lanes
+        log(f"[bench] SpMV autotune: panels A={panels[0]} A^T={panels[1]}, lane shifts "
+            f"A={lanes[0]} A^T={lanes[1]} ({tuned.us[0]:.1f} / {tuned.us[1]:.1f} us per "
+            f"application) in {time.perf_counter() - t0:.1f}s")
     t0 = time.perf_counter()
-    op = api.CsrOperator(A, panels=panels)
+    op = api.CsrOperator(A, panels=panels, lanes=lanes)
     eng = api.Engine(op, kmax=DIM)
     log(f"[bench] device operator + A^T + engine in {time.perf_counter() - t0:.1f}s")
 
@@ -363,8 +372,9 @@ def main():
     mixed = None
     if not args.no_mixed:
         A32 = A.astype(np.float32)
-        p32 = None if args.no_tune else api.autotune(A32, prec=api.PK_F32)[0]
-        op32 = api.CsrOperator(A32, prec=api.PK_F32, panels=p32)
+        t32 = None if args.no_tune else api.autotune(A32, prec=api.PK_F32)
+        op32 = api.CsrOperator(A32, prec=api.PK_F32, panels=t32.panels if t32 else None,
+                               lanes=t32.lanes if t32 else None)
         eng32 = api.Engine(op32, kmax=DIM)
         u32 = u0.astype(np.float32).astype(np.float64)
 
@@ -437,11 +447,12 @@ def main():
                    "basis": "fp64", "converged": r.k, "info": r.info,
                    "nopx": r.counters["nopx"], "nreorth": r.counters["nreorth"],
                    "nrestart": r.counters["nrestart"], "host_syncs": stats["nsync"],
-                   "spmv_panels": list(op.panels()), "autotuned": tuned is not None,
+                   "spmv_panels": list(op.panels()), "spmv_lane_shifts": list(op.lanes()),
+                   "autotuned": tuned is not None,
                    "max_rel_residual": resid},
         "roofline": {"kernel": dom, "bound": "hbm", "achieved": round(achieved, 1),
                      "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": measured_traffic(dom, op.panels()),
+                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": measured_traffic(dom, op.panels(), op.lanes()),
                      "avg_launch_ms": per_launch_ms, "bytes_per_launch": per_launch_bytes,
                      "timed_in": "the timed solves" if prof_timed[dom]["launches"] else
                                  "an extra solve with every class timed"},

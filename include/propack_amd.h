@@ -57,21 +57,29 @@ int pk_csr_destroy(pk_csr_t op);
 int pk_csr_dims(pk_csr_t op, int* m, int* n, long long* nnz);
 /* y = op(A) x ; x, y in precision prec, on the host or the device. */
 int pk_csr_spmv(pk_csr_t op, char trans, const void* x, void* y, int prec, int where);
-/* Column-panel layout of the SpMV (no reference counterpart).
+/* Layout of the SpMV (no reference counterpart).
  * pk_csr_create_panels: pk_csr_create with the column panels of A and A^T
  * given (powers of two up to 16; a direction keeps one panel when its
  * columns are fewer).  pk_csr_panels reports an operator's layout.
+ * pk_csr_set_lanes / pk_csr_lanes: lanes of a wavefront per row in the
+ * SpMV's row reduction, as the built-in choice (the power of two nearest a
+ * panel's mean row length) times 2^shift, shift in [-2, 2], clamped to
+ * [2, 64].
  * pk_csr_autotune: builds the operator with 1, 2, 4 and 8 panels per
- * direction, times each direction on the device and returns the fastest
- * counts and their microseconds per application (inputs as pk_csr_create;
- * nothing is kept).  Layouts differ only in how row sums are grouped. */
+ * direction and, when lanes_a and lanes_t are not NULL, lane shifts -1, 0
+ * and 1 on each; times each direction on the device and returns the
+ * fastest panel counts (and lane shifts) and their microseconds per
+ * application (inputs as pk_csr_create; nothing is kept).  Settings differ
+ * only in how row sums are grouped. */
 int pk_csr_create_panels(int m, int n, long long nnz, const int* rowptr, const int* colind,
                          const void* val, int val_prec, int where, int panels_a, int panels_t,
                          pk_csr_t* op);
 int pk_csr_panels(pk_csr_t op, int* panels_a, int* panels_t);
+int pk_csr_set_lanes(pk_csr_t op, int shift_a, int shift_t);
+int pk_csr_lanes(pk_csr_t op, int* shift_a, int* shift_t);
 int pk_csr_autotune(int m, int n, long long nnz, const int* rowptr, const int* colind,
                     const void* val, int val_prec, int where, int* panels_a, int* panels_t,
-                    double* us_a, double* us_t);
+                    double* us_a, double* us_t, int* lanes_a, int* lanes_t);
 
 /* Matrix ingest (no reference counterpart in the library; the reference's
  * example drivers read Harwell-Boeing, coordinate and dense files,

@@ -39,7 +39,7 @@ EXPORTS = [
     "clearstat_", "printstat_", "pk_lbd_set_rounding", "pk_lbd_eps", "pk_batch_create",
     "pk_batch_destroy", "pk_batch_lanbpro", "pk_lansvd_irl_batch", "pk_matrix_read",
     "pk_matrix_free", "pk_coo_to_csr", "pk_csr_create_coo", "pk_csr_read",
-    "pk_csr_create_panels", "pk_csr_panels", "pk_csr_autotune",
+    "pk_csr_create_panels", "pk_csr_panels", "pk_csr_set_lanes", "pk_csr_lanes", "pk_csr_autotune",
 ]
 
 _lib = None
@@ -78,9 +78,9 @@ def _dt(prec):
 class CsrOperator:
     """Device CSR operator built from a host ``Csr`` (A^T built on device)."""
 
-    def __init__(self, A, prec: int = PK_F64, panels=None):
-        """``panels``: (A, A^T) column-panel counts, default the built-in
-        choice (see autotune)."""
+    def __init__(self, A, prec: int = PK_F64, panels=None, lanes=None):
+        """``panels``: (A, A^T) column-panel counts, ``lanes``: (A, A^T)
+        lanes-per-row shifts; default the built-in choices (see autotune)."""
         L = lib()
         self.m, self.n, self.prec = A.m, A.n, prec
         rp = np.ascontiguousarray(A.rowptr, np.int32)
@@ -98,6 +98,8 @@ class CsrOperator:
                                           C.c_int(panels[1]), C.byref(h)),
                    "pk_csr_create_panels")
         self.handle = h
+        if lanes is not None:
+            _check(L.pk_csr_set_lanes(h, C.c_int(lanes[0]), C.c_int(lanes[1])), "pk_csr_set_lanes")
         self.iparm = np.zeros(4, np.int32)
         _check(L.pk_csr_to_iparm(h, _p(self.iparm)), "pk_csr_to_iparm")
 
@@ -120,6 +122,11 @@ class CsrOperator:
         pa, pt = C.c_int(), C.c_int()
         _check(lib().pk_csr_panels(self.handle, C.byref(pa), C.byref(pt)), "pk_csr_panels")
         return pa.value, pt.value
+
+    def lanes(self):
+        sa, st = C.c_int(), C.c_int()
+        _check(lib().pk_csr_lanes(self.handle, C.byref(sa), C.byref(st)), "pk_csr_lanes")
+        return sa.value, st.value
 
     def spmv(self, x, trans="n"):
         x = np.ascontiguousarray(x, _dt(self.prec))
@@ -391,17 +398,29 @@ def lansvd_irl_dev(eng: Engine, neig, dim, p=None, tol=1e-10, which="L", jobu="y
                      anorm=float(dopt[2]))
 
 
-def autotune(A, prec: int = PK_F64):
-    """Fastest (A, A^T) column-panel counts for ``A`` on this GPU and their
-    microseconds per application (pk_csr_autotune)."""
+@dataclass
+class Tuning:
+    """pk_csr_autotune's choice: column panels and lane shifts of (A, A^T)
+    and their microseconds per application."""
+    panels: tuple
+    lanes: tuple
+    us: tuple
+
+
+def autotune(A, prec: int = PK_F64, lanes: bool = True) -> Tuning:
+    """Fastest (A, A^T) column-panel counts (and, with ``lanes``, lanes-per-row
+    shifts) for ``A`` on this GPU (pk_csr_autotune)."""
     rp = np.ascontiguousarray(A.rowptr, np.int32)
     ci = np.ascontiguousarray(A.col, np.int32)
     va = np.ascontiguousarray(A.val, _dt(prec))
     pa, pt, ua, ut = C.c_int(), C.c_int(), C.c_double(), C.c_double()
+    sa, st = C.c_int(), C.c_int()
     _check(lib().pk_csr_autotune(C.c_int(A.m), C.c_int(A.n), C.c_longlong(int(rp[-1])), _p(rp),
                                  _p(ci), _p(va), C.c_int(prec), C.c_int(PK_HOST), C.byref(pa),
-                                 C.byref(pt), C.byref(ua), C.byref(ut)), "pk_csr_autotune")
-    return (pa.value, pt.value), (ua.value, ut.value)
+                                 C.byref(pt), C.byref(ua), C.byref(ut),
+                                 C.byref(sa) if lanes else None, C.byref(st) if lanes else None),
+           "pk_csr_autotune")
+    return Tuning((pa.value, pt.value), (sa.value, st.value), (ua.value, ut.value))
 
 
 def read_matrix(path):

@@ -62,11 +62,13 @@ int bits_for(int n) {
 }
 
 // Lanes per row for the sub-wave SpMV: the power of two nearest the mean
-// row length, clamped to [2, 64].
-int choose_lpr(long long nnz, int rows) {
+// row length, clamped to [2, 64], times 2^shift (clamped again).
+int choose_lpr(long long nnz, int rows, int shift = 0) {
   const double avg = rows > 0 ? static_cast<double>(nnz) / rows : 1.0;
   int l = 2;
   while (l < 64 && l * 1.5 < avg) l *= 2;
+  for (; shift > 0 && l < 64; --shift) l *= 2;
+  for (; shift < 0 && l > 2; ++shift) l /= 2;
   return l;
 }
 
@@ -234,6 +236,15 @@ int panels_for(bool transposed, int rows, int cols, long long nnz, size_t vbytes
   while (q * 2 <= want && q * 2 <= CsrOp::kMaxPanels) q *= 2;
   return q;
 }
+
+}  // namespace
+
+void csr_set_lanes(CsrOp::Dir& D, int shift) {
+  D.lane_shift = shift;
+  for (int p = 0; p < D.np; ++p) D.pan[p].g = choose_lpr(D.pan[p].nnz, D.rows, shift);
+}
+
+namespace {
 
 int pow2_floor(int p) {
   int q = 1;
@@ -622,44 +633,78 @@ int pk_csr_panels(pk_csr_t h, int* panels_a, int* panels_t) {
 }
 
 // SpMV autotuner: builds the operator with 1, 2, 4 and 8 column panels per
-// direction, times each direction's application, returns the fastest
-// panel count of each direction and its time.  The choice changes how the
-// row sums are grouped (not what is summed), so an operator built with it
-// gives results equal to the built-in layout's up to rounding.
+// direction and, when lanes_a / lanes_t are given, tries lanes per row of
+// half, equal to and twice the built-in choice on each; times each
+// direction's application and returns the fastest setting of each direction
+// and its time.  The choice changes how the row sums are grouped (not what
+// is summed), so an operator built with it gives results equal to the
+// built-in layout's up to rounding.
 int pk_csr_autotune(int m, int n, long long nnz, const int* rowptr, const int* colind,
                     const void* val, int val_prec, int where, int* panels_a, int* panels_t,
-                    double* us_a, double* us_t) {
+                    double* us_a, double* us_t, int* lanes_a, int* lanes_t) {
   if (!panels_a || !panels_t) return PK_ERR_ARG;
+  const bool lanes = lanes_a && lanes_t;
   double best_a = 1e300, best_t = 1e300;
-  int pa = 1, pt = 1;
+  int pa = 1, pt = 1, sa = 0, st = 0;
   for (int P = 1; P <= 8; P *= 2) {
     CsrOp* o = nullptr;
     int rc = csr_create(m, n, nnz, rowptr, colind, val, val_prec, where, &o, P, P);
     if (rc != PK_OK) return rc;
-    double ta = 0, tt = 0;
-    if (val_prec == PK_F64) {
-      rc = time_apply<double>(o, 'n', 5, &ta);
-      if (rc == PK_OK) rc = time_apply<double>(o, 't', 5, &tt);
-    } else {
-      rc = time_apply<float>(o, 'n', 5, &ta);
-      if (rc == PK_OK) rc = time_apply<float>(o, 't', 5, &tt);
-    }
     const int npa = o->da.np, npt = o->dt.np;
+    for (int sh = lanes ? -1 : 0; sh <= (lanes ? 1 : 0) && rc == PK_OK; ++sh) {
+      csr_set_lanes(o->da, sh);
+      csr_set_lanes(o->dt, sh);
+      double ta = 0, tt = 0;
+      if (val_prec == PK_F64) {
+        rc = time_apply<double>(o, 'n', 5, &ta);
+        if (rc == PK_OK) rc = time_apply<double>(o, 't', 5, &tt);
+      } else {
+        rc = time_apply<float>(o, 'n', 5, &ta);
+        if (rc == PK_OK) rc = time_apply<float>(o, 't', 5, &tt);
+      }
+      if (rc != PK_OK) break;
+      if (npa == P && ta < best_a) {
+        best_a = ta;
+        pa = P;
+        sa = sh;
+      }
+      if (npt == P && tt < best_t) {
+        best_t = tt;
+        pt = P;
+        st = sh;
+      }
+    }
     csr_destroy(o);
     if (rc != PK_OK) return rc;
-    if (npa == P && ta < best_a) {
-      best_a = ta;
-      pa = P;
-    }
-    if (npt == P && tt < best_t) {
-      best_t = tt;
-      pt = P;
-    }
   }
   *panels_a = pa;
   *panels_t = pt;
+  if (lanes) {
+    *lanes_a = sa;
+    *lanes_t = st;
+  }
   if (us_a) *us_a = best_a;
   if (us_t) *us_t = best_t;
+  return PK_OK;
+}
+
+int pk_csr_set_lanes(pk_csr_t h, int shift_a, int shift_t) {
+  CsrOp* op = as_csr(h);
+  if (!op) return PK_ERR_HANDLE;
+  if (shift_a < -2 || shift_a > 2 || shift_t < -2 || shift_t > 2) {
+    set_error("pk_csr_set_lanes: shifts must lie in [-2, 2]");
+    return PK_ERR_ARG;
+  }
+  csr_set_lanes(op->da, shift_a);
+  csr_set_lanes(op->dt, shift_t);
+  return PK_OK;
+}
+
+int pk_csr_lanes(pk_csr_t h, int* shift_a, int* shift_t) {
+  CsrOp* op = as_csr(h);
+  if (!op) return PK_ERR_HANDLE;
+  if (shift_a) *shift_a = op->da.lane_shift;
+  if (shift_t) *shift_t = op->dt.lane_shift;
   return PK_OK;
 }
 

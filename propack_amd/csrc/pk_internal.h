@@ -45,6 +45,7 @@ struct CsrOp {
     Panel pan[kMaxPanels];
     double* yacc = nullptr;  // FP64 running row sums between panels (np > 1)
     int slots = 0;           // partial slots of the widest panel
+    int lane_shift = 0;      // lanes per row = built-in choice * 2^lane_shift
   } da, dt;
   int nrb = 0, nrbt = 0;  // partial slots per direction
 };
@@ -55,6 +56,9 @@ int csr_create(int m, int n, long long nnz, const int* rowptr, const int* colind
                const void* val, int val_prec, int where, CsrOp** out, int panels_a = 0,
                int panels_t = 0);
 void csr_destroy(CsrOp* op);
+// Lanes per row of every panel of D: the built-in choice times 2^shift,
+// clamped to [2, 64]; shift in [-2, 2].
+void csr_set_lanes(CsrOp::Dir& D, int shift);
 template <typename T>
 int csr_apply(const CsrOp* op, hipStream_t s, char trans, const T* x, T* y, const T* yprev,
               Scal coef, double* part, Gate gate);

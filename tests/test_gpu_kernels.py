@@ -219,18 +219,22 @@ def test_panels_same_triplets(panels, small_case):
 
 
 def test_autotune_and_explicit_panels():
-    """The autotuner returns panel counts it measured; an operator built
-    with them (and with any explicit layout) gives the plain layout's
-    products up to rounding."""
+    """The autotuner returns panel counts and lanes-per-row shifts it
+    measured; an operator built with them (and with any explicit layout)
+    gives the plain layout's products up to rounding."""
     A = synth.poisson_decay(200_000, 100_000, 16.0, 0.6, seed=4)
-    (pa, pt), (ua, ut) = api.autotune(A)
+    t = api.autotune(A)
+    (pa, pt), (sa, st), (ua, ut) = t.panels, t.lanes, t.us
     assert pa in (1, 2, 4, 8) and pt in (1, 2, 4, 8) and ua > 0 and ut > 0
+    assert sa in (-1, 0, 1) and st in (-1, 0, 1)
     x = synth.start_vector(A.n, 2)
     y = synth.start_vector(A.m, 3)
     want_n, want_t = A.matvec(x), A.transpose().matvec(y)
-    for panels in ((pa, pt), (1, 1), (8, 2), (16, 16)):
-        op = api.CsrOperator(A, panels=panels)
+    for panels, lanes in (((pa, pt), (sa, st)), ((1, 1), None), ((8, 2), (2, -2)),
+                          ((16, 16), (-1, 1))):
+        op = api.CsrOperator(A, panels=panels, lanes=lanes)
         assert op.panels() == tuple(min(p, 16) for p in panels)
+        assert op.lanes() == (lanes or (0, 0))
         np.testing.assert_allclose(op.spmv(x, "n"), want_n, rtol=0,
                                    atol=1e-12 * np.abs(want_n).max())
         np.testing.assert_allclose(op.spmv(y, "t"), want_t, rtol=0,
