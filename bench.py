@@ -55,23 +55,20 @@ TRAFFIC_PROFILE = os.path.join(ROOT, "profiles", "hbm_traffic_spmv_poisson_decay
 def measured_traffic(kclass, panels, lanes):
     """HBM bytes per operator application of kernel class kclass, as
     measured by the rocprofv3 FETCH_SIZE / WRITE_SIZE runs summarised in
-    profiles/ (calibrated against a plain copy); None if absent or if the
-    summary was measured on another SpMV layout (column panels and
-    lanes-per-row shifts)."""
-    traffic, layout, lane_shifts = None, None, (0, 0)
+    profiles/ (calibrated against a plain copy) for the SpMV layout in use
+    (column panels and lanes-per-row shifts); None if that layout was not
+    measured."""
+    want = (*tuple(panels), *tuple(lanes))
     try:
         with open(TRAFFIC_PROFILE) as f:
             for line in f:
                 w = line.split()
-                if len(w) == 3 and w[0] == "traffic_bytes_per_application" and w[1] == kclass:
-                    traffic = float(w[2])
-                if len(w) == 3 and w[0] == "measured_layout":
-                    layout = (int(w[1]), int(w[2]))
-                if len(w) == 3 and w[0] == "measured_lanes":
-                    lane_shifts = (int(w[1]), int(w[2]))
+                if len(w) == 7 and w[0] == "traffic_bytes_per_application" and w[1] == kclass \
+                        and tuple(int(x) for x in w[3:]) == want:
+                    return float(w[2])
     except OSError:
         pass
-    return traffic if layout == tuple(panels) and lane_shifts == tuple(lanes) else None
+    return None
 
 
 def residuals(op, r):
