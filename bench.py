@@ -149,9 +149,16 @@ def workload_powerlaw(args):
     A = synth.powerlaw_bipartite(8_000_000, 4_000_000, seed=8000, dtype=np.float64)
     u0 = synth.start_vector(A.m, 8001)
     log(f"[bench] power-law matrix {A.m}x{A.n} nnz={A.nnz} in {time.perf_counter() - t0:.1f}s")
-    op = api.CsrOperator(A)
+    A32 = A.astype(np.float32)
+    tu64 = None if args.no_tune else api.autotune(A)
+    tu32 = None if args.no_tune else api.autotune(A32, prec=api.PK_F32)
+    if tu64:
+        log(f"[bench] SpMV autotune: FP64 panels {tu64.panels} lanes {tu64.lanes}, FP32 panels "
+            f"{tu32.panels} lanes {tu32.lanes}")
+    op = api.CsrOperator(A, panels=tu64.panels if tu64 else None, lanes=tu64.lanes if tu64 else None)
     eng = api.Engine(op, kmax=dim)
-    op32 = api.CsrOperator(A.astype(np.float32), prec=api.PK_F32)
+    op32 = api.CsrOperator(A32, prec=api.PK_F32, panels=tu32.panels if tu32 else None,
+                           lanes=tu32.lanes if tu32 else None)
     eng32 = api.Engine(op32, kmax=dim)
     sync = lambda: api.lib().pk_lbd_sync(eng.handle)  # noqa: E731
 
