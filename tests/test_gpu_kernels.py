@@ -93,6 +93,43 @@ def test_put_u1_from_device():
     assert np.array_equal(eng.get("u", 1)[:, 0], x)
 
 
+@pytest.mark.parametrize("prec", [api.PK_F64, api.PK_F32])
+@pytest.mark.parametrize("m,j", [(100, 1), (2048, 4), (4097, 5), (6000, 11), (257, 9)])
+def test_cgs_projection_edge_shapes(prec, m, j):
+    """DGETU0 through the device Gram-Schmidt kernels: U(:, j+1) = op(A) x
+    made orthogonal to U(:, 1:j), on row counts around the kernels'
+    2048-row slices (a last slice of one row, fewer rows than a workgroup)
+    and column counts around their 4-column tiles.  Reference: the
+    projection (I - Q Q^T) A x in float64 on the host, with Q and A as the
+    engine stores them.  Tolerances: 1e-13 (FP64 storage), 2e-5 (FP32
+    storage) relative to ||A x||."""
+    n = 300
+    A = synth.random_uniform(m, n, 6 * m, seed=m + j)
+    dt = np.float64 if prec == api.PK_F64 else np.float32
+    A = A.astype(dt)
+    op = api.CsrOperator(A, prec=prec)
+    eng = api.Engine(op, kmax=j + 2, prec=prec)
+    Q, _ = np.linalg.qr(np.random.default_rng(m * 7 + j).standard_normal((m, j)))
+    Qs = Q.astype(dt).astype(np.float64)
+    for c in range(j):
+        assert api.lib().pk_lbd_put(eng.handle, C.c_char(b"u"), C.c_int(c + 1),
+                                    _p(np.ascontiguousarray(Q[:, c])), C.c_int(api.PK_F64)) == 0
+    u0norm, anormest, ierr = C.c_double(), C.c_double(), C.c_int()
+    rc = api.lib().pk_lbd_getu0(eng.handle, C.c_char(b"n"), C.c_int(j), C.c_int(1),
+                                C.byref(u0norm), C.c_int(1), C.byref(anormest), C.byref(ierr))
+    assert rc == 0 and ierr.value == 0, api.lib().pk_last_error()
+    x = np.zeros(n, dt)
+    assert api.lib().pk_larnv_pm1(prec, C.c_longlong(n), _p(x)) == 0
+    y = A.astype(np.float64).matvec(x.astype(np.float64))
+    want = y - Qs @ (Qs.T @ y)
+    got = eng.get("u", j + 1)[:, j]
+    tol = 1e-13 if prec == api.PK_F64 else 2e-5
+    ny = np.linalg.norm(y)
+    assert np.linalg.norm(got - want) <= tol * ny
+    assert np.abs(Qs.T @ got).max() <= tol * ny
+    assert abs(u0norm.value - np.linalg.norm(got)) <= tol * ny
+
+
 def test_put_get_roundtrip_f32():
     op, eng = _engine(300, 200, 10, prec=api.PK_F32)
     x = np.random.default_rng(1).standard_normal(300)
