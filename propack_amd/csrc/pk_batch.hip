@@ -11,7 +11,10 @@
 // subspace, tiny scale, failed Gram-Schmidt) drops out of the batched chain
 // and finishes its call on its own engine afterwards.
 #include <algorithm>
+#include <atomic>
+#include <cstdlib>
 #include <cstring>
+#include <thread>
 #include <vector>
 
 #include "pk_bodies.h"
@@ -528,19 +531,26 @@ int batch_lanbpro(Batch& B, const int* active, const int* k0, int* k, double* co
                   int* ierr) {
   const int np = B.nprob;
   std::vector<char> run(np, 0), fin(np, 0);
+  std::vector<int> rcs(np, PK_OK);
   int Jmin = 1 << 30, Jmax = 0, elr = 0, tries = 1;
-  for (int i = 0; i < np; ++i) {
-    if (!active[i]) continue;
+  // each problem's setup (its own engine and stream) on the host threads
+  host_parallel_for(np, [&](int i) {
+    if (!active[i]) return;
     Engine* e = B.eng[i];
     bool finished = false;
-    const int rc = engine_lanbpro_begin(e, k0[i], k[i], alpha[i], beta[i], &rnorm[i],
-                                        doption + 4 * i, ioption, &ierr[i], &finished);
-    if (rc != PK_OK) return rc;
-    if (finished) {
+    int rc = engine_lanbpro_begin(e, k0[i], k[i], alpha[i], beta[i], &rnorm[i],
+                                  doption + 4 * i, ioption, &ierr[i], &finished);
+    if (rc == PK_OK && finished)
       fin[i] = 1;
-      continue;
-    }
-    if (hipStreamSynchronize(e->stream) != hipSuccess) return PK_ERR_HIP;
+    else if (rc == PK_OK && hipStreamSynchronize(e->stream) != hipSuccess)
+      rc = PK_ERR_HIP;
+    rcs[i] = rc;
+  });
+  for (int i = 0; i < np; ++i)
+    if (rcs[i] != PK_OK) return rcs[i];
+  for (int i = 0; i < np; ++i) {
+    if (!active[i] || fin[i]) continue;
+    Engine* e = B.eng[i];
     const LbdCtl& c = *e->hctl;
     if (e->mode == 0 && c.phase == PH_V_SPMV) {
       run[i] = 1;
@@ -585,17 +595,19 @@ int batch_lanbpro(Batch& B, const int* active, const int* k0, int* k, double* co
         return PK_ERR_HIP;
   }
   if (hipStreamSynchronize(B.stream) != hipSuccess) return PK_ERR_HIP;
-  for (int i = 0; i < np; ++i) {
-    if (!active[i] || fin[i]) continue;
+  // whatever the chain left (rare phases, further CGS passes) runs on each
+  // problem's own engine, then its outputs
+  host_parallel_for(np, [&](int i) {
+    if (!active[i] || fin[i]) return;
     Engine* e = B.eng[i];
     if (run[i]) e->cnt.nsync++;
-    // whatever the chain left (rare phases, further CGS passes) runs on the
-    // problem's own engine, then its outputs
     int rc = engine_lanbpro_loop(e);
     if (rc == PK_OK) rc = engine_lanbpro_end(e, k0[i], &k[i], alpha[i], beta[i], &rnorm[i],
                                              doption + 4 * i, &ierr[i]);
-    if (rc != PK_OK) return rc;
-  }
+    rcs[i] = rc;
+  });
+  for (int i = 0; i < np; ++i)
+    if (rcs[i] != PK_OK) return rcs[i];
   return PK_OK;
 }
 
@@ -609,11 +621,44 @@ Batch* as_batch(pk_batch_t h) {
 }
 
 }  // namespace
+
+int host_threads() {
+  if (const char* v = std::getenv("PK_HOST_THREADS")) {
+    const int t = std::atoi(v);
+    if (t >= 1) return t;
+  }
+  const int hw = static_cast<int>(std::thread::hardware_concurrency());
+  return std::max(1, std::min(hw, 16));
+}
+
+void host_parallel_for(int n, const std::function<void(int)>& f) {
+  const int nt = std::min(n, host_threads());
+  if (nt <= 1) {
+    for (int i = 0; i < n; ++i) f(i);
+    return;
+  }
+  std::atomic<int> next{0};
+  auto work = [&] {
+    for (int i = next.fetch_add(1); i < n; i = next.fetch_add(1)) f(i);
+  };
+  std::vector<std::thread> pool;
+  pool.reserve(nt - 1);
+  for (int t = 1; t < nt; ++t) pool.emplace_back(work);
+  work();
+  for (auto& th : pool) th.join();
+}
+
 }  // namespace pk
 
 using namespace pk;
 
 extern "C" {
+
+// fn(ctx, i) for i in [0, n) on the host threads (the batched Fortran driver's
+// per-problem restart work).
+void pkx_parallel_for(int n, void (*fn)(void*, int), void* ctx) {
+  host_parallel_for(n, [&](int i) { fn(ctx, i); });
+}
 
 int pk_batch_create(int nprob, const pk_lbd_t* engines, pk_batch_t* out) {
   Batch* b = nullptr;

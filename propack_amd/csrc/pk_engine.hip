@@ -12,6 +12,7 @@
 #include <cmath>
 #include <cstdlib>
 #include <cstdarg>
+#include <mutex>
 #include <cstdio>
 #include <cstring>
 #include <algorithm>
@@ -22,7 +23,12 @@ namespace pk {
 
 static thread_local char g_err[512] = "";
 
+// Batched drivers run per-problem host work on several threads: the message
+// buffer is written under a lock.
+static std::mutex g_err_lock;
+
 void set_error(const char* fmt, ...) {
+  std::lock_guard<std::mutex> lk(g_err_lock);
   va_list ap;
   va_start(ap, fmt);
   vsnprintf(g_err, sizeof g_err, fmt, ap);

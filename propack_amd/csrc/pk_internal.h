@@ -2,6 +2,7 @@
 // Lanczos bidiagonalization engine context.
 #pragma once
 
+#include <functional>
 #include <vector>
 
 #include "../../include/propack_amd.h"
@@ -49,6 +50,13 @@ struct CsrOp {
   } da, dt;
   int nrb = 0, nrbt = 0;  // partial slots per direction
 };
+
+// Runs f(i) for i in [0, n) on up to host_threads() threads (the calling
+// thread among them), indices handed out one at a time; returns when all
+// are done.  host_threads(): PK_HOST_THREADS when set, else the hardware
+// threads, at most 16.
+int host_threads();
+void host_parallel_for(int n, const std::function<void(int)>& f);
 
 // panels_a / panels_t: column panels of A and A^T (powers of two up to
 // CsrOp::kMaxPanels), or <= 0 for the built-in choice.

@@ -492,17 +492,15 @@ contains
     doption(1:4) = st%dopt
     info = st%info
     if (st%failed) return  ! an engine error: outputs as they were
-    if (.true.) then
-      vectors = lower1(st%jobu) == 'y' .or. lower1(st%jobv) == 'y'
-      if ((st%nconv >= st%neig .or. st%info > 0) .and. vectors) then
-        st%a1 = st%alpha
-        st%b1 = st%beta
-        call ritzvec(st%which, st%jobu, st%jobv, st%m, st%n, st%nconv, st%dim, st%a1, st%b1, &
-                     st%eng, jj)
-        if (jj /= 0) st%info = jj
-      end if
-      call add_timing(st%eng, st%st0, st%nbsvd, st%nrestart, st%dim, -1)
+    vectors = lower1(st%jobu) == 'y' .or. lower1(st%jobv) == 'y'
+    if ((st%nconv >= st%neig .or. st%info > 0) .and. vectors) then
+      st%a1 = st%alpha
+      st%b1 = st%beta
+      call ritzvec(st%which, st%jobu, st%jobv, st%m, st%n, st%nconv, st%dim, st%a1, st%b1, &
+                   st%eng, jj)
+      if (jj /= 0) st%info = jj
     end if
+    call add_timing(st%eng, st%st0, st%nbsvd, st%nrestart, st%dim, -1)
     if (st%nmax > 0) then
       sigma(1:st%nmax) = st%sig(1:st%nmax)
       bnd(1:st%nmax) = st%bd(1:st%nmax)
@@ -513,6 +511,47 @@ contains
   end subroutine
 
 end module pk_drivers
+
+! Per-problem restart work of the batched driver on the library's host
+! threads: each index touches only its own problem's state and engine.
+module pk_batch_par
+  use iso_c_binding
+  use pk_drivers
+  implicit none
+  private
+  public :: after_ctx, after_one, pkx_parallel_for
+
+  type :: after_ctx
+    type(irl_state), pointer :: st(:) => null()
+    integer(c_int), pointer :: active(:) => null(), kk(:) => null(), ierr(:) => null()
+    real(c_double), pointer :: rn(:) => null(), dop(:, :) => null()
+  end type
+
+  interface
+    subroutine pkx_parallel_for(n, fn, ctx) bind(C, name='pkx_parallel_for')
+      import :: c_int, c_funptr, c_ptr
+      integer(c_int), value :: n
+      type(c_funptr), value :: fn
+      type(c_ptr), value :: ctx
+    end subroutine
+  end interface
+
+contains
+
+  subroutine after_one(ctx, i0) bind(C)
+    type(c_ptr), value :: ctx
+    integer(c_int), value :: i0
+    type(after_ctx), pointer :: c
+    integer :: i
+    call c_f_pointer(ctx, c)
+    i = i0 + 1
+    if (c%active(i) == 0) return
+    c%st(i)%rnorm = c%rn(i)
+    c%st(i)%dopt = c%dop(:, i)
+    call irl_after(c%st(i), c%kk(i), c%ierr(i))
+  end subroutine
+
+end module pk_batch_par
 
 ! ---------------------------------------------------------------------------
 ! Device-resident entry points (no reference counterpart): run the drivers on
@@ -591,6 +630,7 @@ integer(c_int) function pk_lansvd_irl_batch(nb, engs, batch, which, jobu, jobv, 
   use iso_c_binding
   use pk_api
   use pk_drivers
+  use pk_batch_par
   implicit none
   integer(c_int), value :: nb, dim, p, neig, maxiter, ldsig
   type(c_ptr) :: engs(nb)
@@ -610,9 +650,10 @@ integer(c_int) function pk_lansvd_irl_batch(nb, engs, batch, which, jobu, jobv, 
     end function
   end interface
   type(irl_state), allocatable, target :: st(:)
-  integer(c_int), allocatable :: active(:), k0(:), kk(:), ierr(:)
+  integer(c_int), allocatable, target :: active(:), k0(:), kk(:), ierr(:)
   type(c_ptr), allocatable :: ap(:), bp(:)
-  real(c_double), allocatable :: rn(:), dop(:, :)
+  real(c_double), allocatable, target :: rn(:), dop(:, :)
+  type(after_ctx), target :: ctx
   integer :: i, m, n, rc, dd, ne
   double precision :: eps, dummy(4)
 
@@ -650,12 +691,13 @@ integer(c_int) function pk_lansvd_irl_batch(nb, engs, batch, which, jobu, jobv, 
       end do
       exit
     end if
-    do i = 1, nb
-      if (active(i) == 0) cycle
-      st(i)%rnorm = rn(i)
-      st(i)%dopt = dop(:, i)
-      call irl_after(st(i), kk(i), ierr(i))
-    end do
+    ctx%st => st
+    ctx%active => active
+    ctx%kk => kk
+    ctx%ierr => ierr
+    ctx%rn => rn
+    ctx%dop => dop
+    call pkx_parallel_for(nb, c_funloc(after_one), c_loc(ctx))
   end do
   do i = 1, nb
     dd = dim
