@@ -211,6 +211,39 @@ def workload_powerlaw(args):
     print(json.dumps(out))
 
 
+def workload_spmv_probe(args):
+    """Where the SpMV time goes: A x on the bench matrix's rows (same row
+    pointers, values and nonzero count) with three column patterns: every
+    row's k-th nonzero in column k (all gathers hit a few cache lines: L1),
+    the original columns folded into a 64K-column window (512 KiB of x:
+    L2-resident), and the original uniform columns over 1M (the bench
+    matrix).  Each pattern's best time per application over the autotuner's
+    layouts; the first two bound what the gathers' misses cost."""
+    A = synth.poisson_decay(M, N, LAM, DECAY, seed=SEED)
+    rp = A.rowptr.astype(np.int64)
+    lens = np.diff(rp)
+    pos = np.arange(A.nnz, dtype=np.int64) - np.repeat(rp[:-1], lens)
+    pats = {"l1_same_lines": pos.astype(np.int32),
+            "l2_window_64k": (A.col % 65536).astype(np.int32),
+            "uniform_1m": A.col}
+    res = {}
+    alg = A.nnz * 12.0 + (A.m + 1) * 4.0 + A.n * 8.0 + 2 * A.m * 8.0
+    for name, cols in pats.items():
+        B = synth.Csr(A.m, A.n, A.rowptr, np.ascontiguousarray(cols), A.val)
+        t = api.autotune(B)
+        res[name] = {"us_per_application": round(t.us[0], 1), "panels": t.panels[0],
+                     "lane_shift": t.lanes[0],
+                     "algorithmic_GBps": round(alg / (t.us[0] * 1e-6) / 1e9, 1)}
+        log(f"[bench] spmv probe {name}: {res[name]}")
+    out = {"metric": "SpMV A x time per application by column pattern (2Mx1M, 32M nnz, FP64)",
+           "value": res["uniform_1m"]["us_per_application"], "unit": "us", "n_gpus": 1,
+           "steps": 5, "warmup": 1, "higher_is_better": False, "dtype": "f64",
+           "data": "synthetic (rows, values of the bench matrix; columns per pattern)",
+           "config": {"workload": "SpMV column-pattern probe", "nnz": A.nnz,
+                      "algorithmic_bytes": alg}, "patterns": res}
+    print(json.dumps(out))
+
+
 def workload_batch(args):
     """The batched workload (synth.batch_params: m log-uniform in [5e3, 5e5],
     n = r m, Poisson(lam ~ U[4, 48]) nonzeros per row, decay p ~ U[0.3, 1]):
@@ -275,7 +308,7 @@ def main():
     ap.add_argument("--lanes", default=None,
                     help="lanes-per-row shifts 'A,At' with --panels, or of every problem "
                          "of --workload batch (default 0,0)")
-    ap.add_argument("--workload", choices=["c2", "powerlaw", "batch"], default="c2",
+    ap.add_argument("--workload", choices=["c2", "powerlaw", "batch", "spmv-probe"], default="c2",
                     help="c2 (default): the headline solve; powerlaw: 8M x 4M Zipf "
                          "bipartite, k=64, dim=160, tol=1e-8, FP32 basis + FP64 polish vs "
                          "FP64; batch: the 256-problem batched workload vs a per-problem loop")
@@ -292,6 +325,9 @@ def main():
         return
     if args.workload == "batch":
         workload_batch(args)
+        return
+    if args.workload == "spmv-probe":
+        workload_spmv_probe(args)
         return
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported "
