@@ -14,6 +14,7 @@
 #include <atomic>
 #include <cstdlib>
 #include <cstring>
+#include <system_error>
 #include <thread>
 #include <vector>
 
@@ -643,7 +644,12 @@ void host_parallel_for(int n, const std::function<void(int)>& f) {
   };
   std::vector<std::thread> pool;
   pool.reserve(nt - 1);
-  for (int t = 1; t < nt; ++t) pool.emplace_back(work);
+  try {
+    for (int t = 1; t < nt; ++t) pool.emplace_back(work);
+  } catch (const std::system_error&) {
+    // fewer threads than asked for: the ones started and this thread share
+    // the remaining indices
+  }
   work();
   for (auto& th : pool) th.join();
 }
