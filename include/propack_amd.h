@@ -186,7 +186,10 @@ int pk_lbd_eps(pk_lbd_t eng, double* eps, double* leff);
  * problems): the engines of a batch (same precision, each with its own
  * device CSR operator) run their Lanczos steps together, one launch per
  * kernel per step for the whole batch, each problem gated by its own
- * control state; a problem gives the same results as alone. */
+ * control state; a problem gives the same results as alone.  Engines and
+ * operators may not be shared between problems of one batch (PK_ERR_ARG):
+ * their SpMVs run at the same time.  Per-problem host work runs on up to
+ * 16 host threads (PK_HOST_THREADS overrides). */
 typedef struct pk_batch_opaque* pk_batch_t;
 int pk_batch_create(int nprob, const pk_lbd_t* engines, pk_batch_t* batch);
 int pk_batch_destroy(pk_batch_t batch);

@@ -418,6 +418,16 @@ int batch_create(int nprob, const pk_lbd_t* handles, Batch** out) {
       rc = PK_ERR_ARG;
       break;
     }
+    // An operator's panel row sums and partial slots are its own scratch:
+    // two problems of one batch run their SpMVs at the same time, so they
+    // may not share an operator (or an engine).
+    bool shared = false;
+    for (const Engine* o : B->eng) shared = shared || o == e || o->op == e->op;
+    if (shared) {
+      set_error("pk_batch_create: problem %d shares its engine or operator with another", i);
+      rc = PK_ERR_ARG;
+      break;
+    }
     B->eng.push_back(e);
     BProb& P = B->hprob[i];
     std::memset(&P, 0, sizeof(BProb));

@@ -19,10 +19,22 @@ namespace pk {
 
 namespace {
 
-__global__ void k_row_index(int m, const int* __restrict__ rp, int* __restrict__ rowidx) {
-  for (long long i = static_cast<long long>(blockIdx.x) * blockDim.x + threadIdx.x; i < m;
-       i += static_cast<long long>(gridDim.x) * blockDim.x) {
-    for (int p = rp[i]; p < rp[i + 1]; ++p) rowidx[p] = static_cast<int>(i);
+// rowidx[p] = the row holding nonzero p: one thread per nonzero, a binary
+// search for the last row i with rp[i] <= p (empty rows are passed over), so
+// a row of 10^5 nonzeros costs no more than any other.
+__global__ void k_row_index(int m, long long nnz, const int* __restrict__ rp,
+                            int* __restrict__ rowidx) {
+  for (long long p = static_cast<long long>(blockIdx.x) * blockDim.x + threadIdx.x; p < nnz;
+       p += static_cast<long long>(gridDim.x) * blockDim.x) {
+    int lo = 0, hi = m;  // rp[lo] <= p < rp[hi]
+    while (hi - lo > 1) {
+      const int mid = lo + (hi - lo) / 2;
+      if (rp[mid] <= p)
+        lo = mid;
+      else
+        hi = mid;
+    }
+    rowidx[p] = lo;
   }
 }
 
@@ -155,7 +167,7 @@ static int build_transpose(int m, int n, long long nnz, const int* rp, const int
   PKT(hipMalloc(&cit, ib));
   PKT(hipMalloc(&vat, sizeof(VT) * static_cast<size_t>(std::max<long long>(nnz, 1))));
   if (nnz > 0) {
-    hipLaunchKernelGGL(k_row_index, dim3(2048), dim3(256), 0, s, m, rp, rowidx);
+    hipLaunchKernelGGL(k_row_index, dim3(2048), dim3(256), 0, s, m, nnz, rp, rowidx);
     PKT(hipMemcpyAsync(keys_in, ci, sizeof(int) * nnz, hipMemcpyDeviceToDevice, s));
     hipLaunchKernelGGL(k_iota, dim3(2048), dim3(256), 0, s, nnz, perm_in);
     PKT(rocprim::radix_sort_pairs(nullptr, tmp_bytes, keys_in, keys_out, perm_in, perm_out,
@@ -306,7 +318,7 @@ static int build_dir(CsrOp::Dir& D, int rows, int cols, long long nnz, int np, i
   PKT(hipMalloc(&poff_d, sizeof(int) * (static_cast<size_t>(np) + 1)));
   PKT(hipMalloc(&D.yacc, sizeof(double) * std::max(rows, 1)));
   if (nnz > 0) {
-    hipLaunchKernelGGL(k_row_index, dim3(2048), dim3(256), 0, s, rows, rp, rowidx);
+    hipLaunchKernelGGL(k_row_index, dim3(2048), dim3(256), 0, s, rows, nnz, rp, rowidx);
     hipLaunchKernelGGL(k_panel_key, dim3(2048), dim3(256), 0, s, nnz, ci, D.width, keys_in);
     hipLaunchKernelGGL(k_iota, dim3(2048), dim3(256), 0, s, nnz, perm_in);
     PKT(rocprim::radix_sort_pairs(nullptr, tmp_bytes, keys_in, keys_out, perm_in, perm_out,

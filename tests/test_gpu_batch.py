@@ -93,3 +93,31 @@ def test_batch_small_and_uneven():
         assert np.array_equal(r.sigma, rb.sigma) and nopx == rb.counters["nopx"]
     s = np.linalg.svd(A.todense(), compute_uv=False)
     np.testing.assert_allclose(bat[-1].sigma, s[: bat[-1].k], rtol=1e-10)
+
+
+def test_batch_rejects_shared_operator():
+    """Problems of one batch run their SpMVs at the same time, so two
+    engines on one operator (or one engine twice) cannot form a batch."""
+    A = synth.random_uniform(3000, 1000, 20000, seed=5)
+    op = api.CsrOperator(A)
+    e1, e2 = api.Engine(op, kmax=DIM), api.Engine(op, kmax=DIM)
+    with pytest.raises(api.PropackError, match="shares"):
+        api.Batch([e1, e2])
+    op2 = api.CsrOperator(A)
+    e3 = api.Engine(op2, kmax=DIM)
+    with pytest.raises(api.PropackError, match="shares"):
+        api.Batch([e1, e3, e1])
+    api.Batch([e1, e3]).close()
+
+
+def test_batch_host_threads_same_results(monkeypatch):
+    """The per-problem host work on one thread or on several gives the same
+    bits (each problem touches only its own engine)."""
+    probs = _problems(5, 2_000, 20_000, seed=31)
+    monkeypatch.setenv("PK_HOST_THREADS", "1")
+    one = _batched(probs)
+    monkeypatch.setenv("PK_HOST_THREADS", "8")
+    many = _batched(probs)
+    for a, b in zip(one, many):
+        assert a.info == b.info and a.k == b.k
+        assert np.array_equal(a.sigma, b.sigma) and a.counters["nopx"] == b.counters["nopx"]
