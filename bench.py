@@ -317,6 +317,10 @@ def main():
                     help="only run a plain 1 GiB device copy 5 times (the reference "
                          "kernel for scaling FETCH_SIZE/WRITE_SIZE counters) and exit")
     args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported "
+              "(the multi-GPU layout is a design note only)", file=sys.stderr)
+        sys.exit(2)
     if args.copy_calibration:
         copy_calibration()
         return
@@ -329,10 +333,6 @@ def main():
     if args.workload == "spmv-probe":
         workload_spmv_probe(args)
         return
-    if args.gpus != 1:
-        print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported "
-              "(the multi-GPU layout is a design note only)", file=sys.stderr)
-        sys.exit(2)
 
     t0 = time.perf_counter()
     A = synth.poisson_decay(M, N, LAM, DECAY, seed=SEED)

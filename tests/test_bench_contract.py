@@ -1,0 +1,40 @@
+"""bench.py pieces that run without a GPU: the traffic figure is read from
+the committed rocprof summary only for the SpMV layout it was measured on,
+and a multi-GPU request exits with a message (one GPU is measured)."""
+import os
+import subprocess
+import sys
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+sys.path.insert(0, ROOT)
+
+import bench  # noqa: E402
+
+
+def _profile_lines():
+    out = {}
+    with open(bench.TRAFFIC_PROFILE) as f:
+        for line in f:
+            w = line.split()
+            if len(w) == 7 and w[0] == "traffic_bytes_per_application":
+                out[(w[1], tuple(int(x) for x in w[3:]))] = float(w[2])
+    return out
+
+
+def test_traffic_lookup_matches_profile():
+    lines = _profile_lines()
+    assert lines, "the traffic summary holds machine-readable lines"
+    for (kclass, layout), value in lines.items():
+        assert bench.measured_traffic(kclass, layout[:2], layout[2:]) == value
+        # more than algorithmic bytes (about 0.42 GB per application), less than 3x
+        assert 4.0e8 < value < 1.3e9
+    assert bench.measured_traffic("spmv_t", (3, 5), (0, 0)) is None
+    assert bench.measured_traffic("gemvt", (4, 4), (0, -1)) is None
+
+
+def test_multi_gpu_request_exits():
+    r = subprocess.run([sys.executable, os.path.join(ROOT, "bench.py"), "--gpus", "2"],
+                       capture_output=True, text=True, timeout=120)
+    assert r.returncode == 2
+    assert "one GPU" in r.stderr
+    assert r.stdout == ""
