@@ -111,8 +111,8 @@ __global__ __launch_bounds__(kBlock) void k_blongfix(const BProb* __restrict__ p
                                                      const BWork* __restrict__ work, int nlong,
                                                      int tr, int p, int J, int phase,
                                                      double* __restrict__ part, int Q) {
-  const int i = blockIdx.x * kBlock + threadIdx.x;
-  if (i >= nlong) return;
+  const int i = blockIdx.x * (kBlock / kWave) + static_cast<int>(threadIdx.x) / kWave;
+  if (i >= nlong) return;  // one wave per long row
   const BWork w = work[i];
   const BProb P = probs[w.pid];
   if (Gate{P.ctl, phase, J, 0}.off()) return;
@@ -125,7 +125,8 @@ __global__ __launch_bounds__(kBlock) void k_blongfix(const BProb* __restrict__ p
   const double c = tr ? P.ctl->beta : P.ctl->alpha;
   const int q = (tr ? P.qt0 : P.qa0) + w.loc;
   spmv_longfix_body<T>(ldesc[i], pan.lsum, y, yprev, c, part + q, part + Q + q,
-                       p == 0 ? nullptr : yacc, last ? nullptr : yacc, last);
+                       p == 0 ? nullptr : yacc, last ? nullptr : yacc, last,
+                       static_cast<int>(threadIdx.x) % kWave);
 }
 
 // One workgroup per problem: the control stage of `kind` over the problem's
@@ -296,9 +297,9 @@ void enqueue_batch_step(Batch& B, int J, int elr, int tries) {
       for (const auto& L : B.lng) {
         if (L.tr != tr || L.p != p || L.count == 0) continue;
         any = true;
-        hipLaunchKernelGGL(k_blongfix<T>, dim3(cdiv(L.count, kBlock)), dim3(kBlock), 0, s,
-                           B.dprob, B.panels, L.ldesc, L.work, L.count, tr, p, J, phase, B.part,
-                           tr ? B.Qt : B.Qa);
+        hipLaunchKernelGGL(k_blongfix<T>, dim3(cdiv(L.count, kBlock / kWave)), dim3(kBlock), 0,
+                           s, B.dprob, B.panels, L.ldesc, L.work, L.count, tr, p, J, phase,
+                           B.part, tr ? B.Qt : B.Qa);
       }
       if (!any) break;
     }

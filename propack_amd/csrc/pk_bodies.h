@@ -122,17 +122,21 @@ __device__ __forceinline__ void spmv_rb_body(const int4 d, const int* __restrict
   }
 }
 
-// Long row ld = {row, first chunk, number of chunks, -}: one thread sums the
-// chunks in order and applies the epilogue (or stores the running sum).
+// Long row ld = {row, first chunk, number of chunks, -}: one wave sums the
+// chunks (lane l takes chunks l, l + 64, ... in order, then a fixed shuffle
+// tree) and lane 0 applies the epilogue (or stores the running sum).
 template <typename T>
 __device__ __forceinline__ void spmv_longfix_body(const int4 d, const double* __restrict__ lsum,
                                                   T* __restrict__ y,
                                                   const T* __restrict__ yprev, double c,
                                                   double* q_s2, double* q_sd,
                                                   const double* __restrict__ yacc_in,
-                                                  double* __restrict__ yacc_out, int last) {
+                                                  double* __restrict__ yacc_out, int last,
+                                                  int lane) {
   double acc = 0.0;
-  for (int q = 0; q < d.z; ++q) acc += lsum[d.y + q];
+  for (int q = lane; q < d.z; q += kWave) acc += lsum[d.y + q];
+  acc = wave_sum(acc);
+  if (lane != 0) return;
   if (yacc_in) acc = yacc_in[d.x] + acc;
   if (!last) {
     yacc_out[d.x] = acc;

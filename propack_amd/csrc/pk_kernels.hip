@@ -56,10 +56,11 @@ __global__ __launch_bounds__(kBlock) void k_spmv_longfix(int nlong, const int4* 
                                                          double* __restrict__ yacc_out, int last,
                                                          Gate gate) {
   if (gate.off()) return;
-  const int i = blockIdx.x * kBlock + threadIdx.x;
-  if (i >= nlong) return;
+  const int i = blockIdx.x * (kBlock / kWave) + static_cast<int>(threadIdx.x) / kWave;
+  if (i >= nlong) return;  // wave-uniform
   spmv_longfix_body<T>(ld[i], lsum, y, yprev, coef.get(), part + nbmain + i,
-                       part + nbt + nbmain + i, yacc_in, yacc_out, last);
+                       part + nbt + nbmain + i, yacc_in, yacc_out, last,
+                       static_cast<int>(threadIdx.x) % kWave);
 }
 
 template <typename VT, typename T>
@@ -87,9 +88,9 @@ int launch_spmv_rb(hipStream_t s, int nblk, const int4* desc, int nlong, const i
 #undef PK_RB_CASE
   }
   if (nlong > 0)
-    hipLaunchKernelGGL(k_spmv_longfix<T>, dim3(cdiv(nlong, kBlock)), dim3(kBlock), 0, s, nlong,
-                       ldesc, lsum, y, yprev, coef, part, nbt, nblk, yacc_in, yacc_out, last,
-                       gate);
+    hipLaunchKernelGGL(k_spmv_longfix<T>, dim3(cdiv(nlong, kBlock / kWave)), dim3(kBlock), 0,
+                       s, nlong, ldesc, lsum, y, yprev, coef, part, nbt, nblk, yacc_in, yacc_out,
+                       last, gate);
   return hipGetLastError() == hipSuccess ? nbt : -1;
 }
 
