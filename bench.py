@@ -404,6 +404,14 @@ def main():
                    "bytes_per_launch": v["bytes"] / v["launches"] if v["launches"] else 0}
                for c, v in prof_all.items()}
 
+    # SpMV and reorthogonalization together (the two HBM-bound classes):
+    # algorithmic bytes over device time
+    cls = ("spmv_n", "spmv_t", "gemvt", "gemv")
+    cb = sum(prof_all[c]["bytes"] for c in cls)
+    cms = sum(prof_all[c]["ms"] for c in cls)
+    combined = ({"GBps": round(cb / (cms * 1e-3) / 1e9, 1),
+                 "frac": round(cb / (cms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+                 "ms": round(cms, 3), "bytes": cb} if cms > 0 else None)
     log(f"[bench] kernels: {json.dumps(kernels)}")
     log(f"[bench] engine stats: {stats}")
 
@@ -503,6 +511,7 @@ def main():
                                  "an extra solve with every class timed"},
         "kernels": kernels,
         "kernels_timed_in": "one extra solve with every kernel class timed (after the timed solves)",
+        "spmv_plus_reorth": combined,
         "fp32_basis_form": mixed,
         "cpu_baseline": cpu,
     }
