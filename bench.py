@@ -9,12 +9,19 @@ Workload (one "step" = one full restarted solve):
   DLANSVD_IRL: k = 50 largest triplets, Lanczos dimension 120, p = 70 shifts,
   tol = 1e-10, delta = sqrt(eps), eta = eps^(3/4), ICGS, ELR, seeded N(0,1)
   start vector; singular vectors computed (left on the device).
-The matrix is resident in HBM (device CSR operator) before timing starts.
+The matrix (device CSR operator, SpMV layout from the autotuner) and the start
+vector are resident in HBM before timing starts.
 
-Also reported: the same solve's per-kernel live timings (HIP events on the
-engine stream), the max relative residual of the 50 triplets, and the
-reference PROPACK (OpenMP build, oracle/_ref) timed on the host cores on a
+Also reported: live HIP-event timing of the SpMV (the roofline kernel) inside
+the timed solves, every kernel class timed in one more solve after them, the
+SpMV and reorthogonalization bandwidth together, the max relative residual of
+the 50 triplets, the FP32-basis form with FP64 polish on the same matrix, and
+the reference PROPACK (OpenMP build, oracle/_ref) timed on the host cores on a
 bounded sample of the same solve.
+
+Other workloads: --workload powerlaw (8M x 4M Zipf bipartite, FP32 basis +
+FP64 polish), --workload batch (256 uneven problems, batched vs per-problem
+loop), --workload spmv-probe (SpMV time by column pattern).
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu]
 """
