@@ -59,19 +59,21 @@ def log(*a):
 TRAFFIC_PROFILE = os.path.join(ROOT, "profiles", "hbm_traffic_spmv_poisson_decay_2Mx1M.md")
 
 
-def measured_traffic(kclass, panels, lanes):
+def measured_traffic(kclass, panels, lanes, sort=(0, 0)):
     """HBM bytes per operator application of kernel class kclass, as
     measured by the rocprofv3 FETCH_SIZE / WRITE_SIZE runs summarised in
     profiles/ (calibrated against a plain copy) for the SpMV layout in use
-    (column panels and lanes-per-row shifts); None if that layout was not
+    (column panels, lanes-per-row shifts, sorted-layout block caps; a line
+    without caps is the row-block layout); None if that layout was not
     measured."""
-    want = (*tuple(panels), *tuple(lanes))
+    want = (*tuple(panels), *tuple(lanes), *tuple(sort))
     try:
         with open(TRAFFIC_PROFILE) as f:
             for line in f:
                 w = line.split()
-                if len(w) == 7 and w[0] == "traffic_bytes_per_application" and w[1] == kclass \
-                        and tuple(int(x) for x in w[3:]) == want:
+                if len(w) in (7, 9) and w[0] == "traffic_bytes_per_application" \
+                        and w[1] == kclass \
+                        and tuple(int(x) for x in w[3:] + ["0", "0"][: 9 - len(w)]) == want:
                     return float(w[2])
     except OSError:
         pass
@@ -162,12 +164,13 @@ def workload_powerlaw(args):
     tu64 = None if args.no_tune else api.autotune(A)
     tu32 = None if args.no_tune else api.autotune(A32, prec=api.PK_F32)
     if tu64:
-        log(f"[bench] SpMV autotune: FP64 panels {tu64.panels} lanes {tu64.lanes}, FP32 panels "
-            f"{tu32.panels} lanes {tu32.lanes}")
-    op = api.CsrOperator(A, panels=tu64.panels if tu64 else None, lanes=tu64.lanes if tu64 else None)
+        log(f"[bench] SpMV autotune: FP64 panels {tu64.panels} lanes {tu64.lanes} sort "
+            f"{tu64.sort}, FP32 panels {tu32.panels} lanes {tu32.lanes} sort {tu32.sort}")
+    op = api.CsrOperator(A, panels=tu64.panels if tu64 else None, lanes=tu64.lanes if tu64 else None,
+                         sort=tu64.sort if tu64 else None)
     eng = api.Engine(op, kmax=dim)
     op32 = api.CsrOperator(A32, prec=api.PK_F32, panels=tu32.panels if tu32 else None,
-                           lanes=tu32.lanes if tu32 else None)
+                           lanes=tu32.lanes if tu32 else None, sort=tu32.sort if tu32 else None)
     eng32 = api.Engine(op32, kmax=dim)
     sync = lambda: api.lib().pk_lbd_sync(eng.handle)  # noqa: E731
 
@@ -239,7 +242,7 @@ def workload_spmv_probe(args):
         B = synth.Csr(A.m, A.n, A.rowptr, np.ascontiguousarray(cols), A.val)
         t = api.autotune(B)
         res[name] = {"us_per_application": round(t.us[0], 1), "panels": t.panels[0],
-                     "lane_shift": t.lanes[0],
+                     "lane_shift": t.lanes[0], "sort_cap": t.sort[0],
                      "algorithmic_GBps": round(alg / (t.us[0] * 1e-6) / 1e9, 1)}
         log(f"[bench] spmv probe {name}: {res[name]}")
     out = {"metric": "SpMV A x time per application by column pattern (2Mx1M, 32M nnz, FP64)",
@@ -260,9 +263,13 @@ def workload_batch(args):
     t0 = time.perf_counter()
     probs = []
     lanes = tuple(int(x) for x in args.lanes.split(",")) if args.lanes else None
+    # column-sorted SpMV blocks of 8192 nonzeros for every problem: the
+    # fastest cap on this workload (measured: 417, 462, 532 and 435
+    # problems/s for the row-block layout and caps 4096, 8192, 16384)
+    sort = tuple(int(x) for x in args.sort.split(",")) if args.sort else (8192, 8192)
     for prm in synth.batch_params(256)[: args.count]:
         A = synth.batch_problem(prm, dtype=np.float32)
-        op = api.CsrOperator(A, prec=api.PK_F32, lanes=lanes)
+        op = api.CsrOperator(A, prec=api.PK_F32, lanes=lanes, sort=sort)
         e = api.Engine(op, kmax=48, prec=api.PK_F32)
         e.set_rounding(1)
         # start vectors are inputs: resident in HBM before the timed region
@@ -293,7 +300,8 @@ def workload_batch(args):
            "warmup": 1, "ms_per_step": tb * 1e3, "higher_is_better": True, "scaling": "strong",
            "vs_baseline": None, "dtype": "f32 storage / f64 accumulation",
            "data": "synthetic (seeded generator, matrices resident in HBM)",
-           "config": {"workload": f"{len(probs)} problems of batch_params(256)", "nnz": nnz},
+           "config": {"workload": f"{len(probs)} problems of batch_params(256)", "nnz": nnz,
+                      "spmv_sort_caps": list(sort)},
            "sequential_problems_per_s": len(probs) / ts, "speedup_vs_sequential": ts / tb,
            "converged": sum(r.k for r in rb), "requested": 16 * len(probs),
            "bit_identical_to_sequential": same}
@@ -315,6 +323,9 @@ def main():
     ap.add_argument("--lanes", default=None,
                     help="lanes-per-row shifts 'A,At' with --panels, or of every problem "
                          "of --workload batch (default 0,0)")
+    ap.add_argument("--sort", default=None,
+                    help="column-sorted SpMV layout block caps 'A,At' with --panels, or of "
+                         "every problem of --workload batch (0: row-block layout; default 0,0)")
     ap.add_argument("--workload", choices=["c2", "powerlaw", "batch", "spmv-probe"], default="c2",
                     help="c2 (default): the headline solve; powerlaw: 8M x 4M Zipf "
                          "bipartite, k=64, dim=160, tol=1e-8, FP32 basis + FP64 polish vs "
@@ -346,18 +357,20 @@ def main():
     u0 = synth.start_vector(M, SEED + 1)
     log(f"[bench] matrix {A.m}x{A.n} nnz={A.nnz} generated in {time.perf_counter() - t0:.1f}s")
     t0 = time.perf_counter()
-    panels, lanes, tuned = None, None, None
+    panels, lanes, sort, tuned = None, None, None, None
     if args.panels:
         panels = tuple(int(x) for x in args.panels.split(","))
         lanes = tuple(int(x) for x in args.lanes.split(",")) if args.lanes else None
+        sort = tuple(int(x) for x in args.sort.split(",")) if args.sort else None
     elif not args.no_tune:
         tuned = api.autotune(A)
-        panels, lanes = tuned.panels, tuned.lanes
+        panels, lanes, sort = tuned.panels, tuned.lanes, tuned.sort
         log(f"[bench] SpMV autotune: panels A={panels[0]} A^T={panels[1]}, lane shifts "
-            f"A={lanes[0]} A^T={lanes[1]} ({tuned.us[0]:.1f} / {tuned.us[1]:.1f} us per "
-            f"application) in {time.perf_counter() - t0:.1f}s")
+            f"A={lanes[0]} A^T={lanes[1]}, sorted-layout caps A={sort[0]} A^T={sort[1]} "
+            f"({tuned.us[0]:.1f} / {tuned.us[1]:.1f} us per application) in "
+            f"{time.perf_counter() - t0:.1f}s")
     t0 = time.perf_counter()
-    op = api.CsrOperator(A, panels=panels, lanes=lanes)
+    op = api.CsrOperator(A, panels=panels, lanes=lanes, sort=sort)
     eng = api.Engine(op, kmax=DIM)
     log(f"[bench] device operator + A^T + engine in {time.perf_counter() - t0:.1f}s")
 
@@ -434,7 +447,7 @@ def main():
         A32 = A.astype(np.float32)
         t32 = None if args.no_tune else api.autotune(A32, prec=api.PK_F32)
         op32 = api.CsrOperator(A32, prec=api.PK_F32, panels=t32.panels if t32 else None,
-                               lanes=t32.lanes if t32 else None)
+                               lanes=t32.lanes if t32 else None, sort=t32.sort if t32 else None)
         eng32 = api.Engine(op32, kmax=DIM)
         u32 = torch.from_numpy(u0.astype(np.float32).astype(np.float64)).to("cuda")
 
@@ -508,11 +521,12 @@ def main():
                    "nopx": r.counters["nopx"], "nreorth": r.counters["nreorth"],
                    "nrestart": r.counters["nrestart"], "host_syncs": stats["nsync"],
                    "spmv_panels": list(op.panels()), "spmv_lane_shifts": list(op.lanes()),
+                   "spmv_sort_caps": list(op.sort()),
                    "autotuned": tuned is not None,
                    "max_rel_residual": resid},
         "roofline": {"kernel": dom, "bound": "hbm", "achieved": round(achieved, 1),
                      "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": measured_traffic(dom, op.panels(), op.lanes()),
+                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": measured_traffic(dom, op.panels(), op.lanes(), op.sort()),
                      "avg_launch_ms": per_launch_ms, "bytes_per_launch": per_launch_bytes,
                      "timed_in": "the timed solves" if prof_timed[dom]["launches"] else
                                  "an extra solve with every class timed"},

@@ -65,21 +65,35 @@ int pk_csr_spmv(pk_csr_t op, char trans, const void* x, void* y, int prec, int w
  * SpMV's row reduction, as the built-in choice (the power of two nearest a
  * panel's mean row length) times 2^shift, shift in [-2, 2], clamped to
  * [2, 64].
+ * pk_csr_set_sort / pk_csr_sort: the column-sorted layout of a direction
+ * (cap > 0: row blocks of at most cap nonzeros, 1024 <= cap <= 16384, each
+ * block's entries stored in increasing column order so that the gathers of
+ * nearby columns share cache lines; built on the device, about 14 bytes per
+ * nonzero on top of the row-block layout) or the row-block layout (cap = 0).
+ * Row sums are bit-identical to the row-block layout's for the same lanes;
+ * only the per-block partial sums of the norms are grouped differently.
+ * Not while a solve that uses the operator runs (a batch, pk_batch_create,
+ * takes the layout its operators have when it is created).
  * pk_csr_autotune: builds the operator with 1, 2, 4 and 8 panels per
- * direction and, when lanes_a and lanes_t are not NULL, lane shifts -1, 0
- * and 1 on each; times each direction on the device and returns the
- * fastest panel counts (and lane shifts) and their microseconds per
- * application (inputs as pk_csr_create; nothing is kept).  Settings differ
- * only in how row sums are grouped. */
+ * direction (up to 16 when sort_a and sort_t are not NULL) and, when
+ * lanes_a and lanes_t are not NULL, lane shifts -1, 0 and 1 on each, and,
+ * when sort_a and sort_t are not NULL, each in the row-block layout and in
+ * the sorted layout with caps 8192 and 16384; times each direction on the
+ * device and returns the fastest panel counts (and lane shifts, block caps)
+ * and their microseconds per application (inputs as pk_csr_create; nothing
+ * is kept).  Settings differ only in how sums are grouped. */
 int pk_csr_create_panels(int m, int n, long long nnz, const int* rowptr, const int* colind,
                          const void* val, int val_prec, int where, int panels_a, int panels_t,
                          pk_csr_t* op);
 int pk_csr_panels(pk_csr_t op, int* panels_a, int* panels_t);
 int pk_csr_set_lanes(pk_csr_t op, int shift_a, int shift_t);
 int pk_csr_lanes(pk_csr_t op, int* shift_a, int* shift_t);
+int pk_csr_set_sort(pk_csr_t op, int cap_a, int cap_t);
+int pk_csr_sort(pk_csr_t op, int* cap_a, int* cap_t);
 int pk_csr_autotune(int m, int n, long long nnz, const int* rowptr, const int* colind,
                     const void* val, int val_prec, int where, int* panels_a, int* panels_t,
-                    double* us_a, double* us_t, int* lanes_a, int* lanes_t);
+                    double* us_a, double* us_t, int* lanes_a, int* lanes_t, int* sort_a,
+                    int* sort_t);
 
 /* Matrix ingest (no reference counterpart in the library; the reference's
  * example drivers read Harwell-Boeing, coordinate and dense files,

@@ -40,6 +40,7 @@ EXPORTS = [
     "pk_batch_destroy", "pk_batch_lanbpro", "pk_lansvd_irl_batch", "pk_matrix_read",
     "pk_matrix_free", "pk_coo_to_csr", "pk_csr_create_coo", "pk_csr_read",
     "pk_csr_create_panels", "pk_csr_panels", "pk_csr_set_lanes", "pk_csr_lanes", "pk_csr_autotune",
+    "pk_csr_set_sort", "pk_csr_sort",
 ]
 
 _lib = None
@@ -78,9 +79,11 @@ def _dt(prec):
 class CsrOperator:
     """Device CSR operator built from a host ``Csr`` (A^T built on device)."""
 
-    def __init__(self, A, prec: int = PK_F64, panels=None, lanes=None):
+    def __init__(self, A, prec: int = PK_F64, panels=None, lanes=None, sort=None):
         """``panels``: (A, A^T) column-panel counts, ``lanes``: (A, A^T)
-        lanes-per-row shifts; default the built-in choices (see autotune)."""
+        lanes-per-row shifts, ``sort``: (A, A^T) block caps of the
+        column-sorted layout (0: row-block layout); default the built-in
+        choices (see autotune)."""
         L = lib()
         self.m, self.n, self.prec = A.m, A.n, prec
         rp = np.ascontiguousarray(A.rowptr, np.int32)
@@ -100,6 +103,8 @@ class CsrOperator:
         self.handle = h
         if lanes is not None:
             _check(L.pk_csr_set_lanes(h, C.c_int(lanes[0]), C.c_int(lanes[1])), "pk_csr_set_lanes")
+        if sort is not None:
+            self.set_sort(*sort)
         self.iparm = np.zeros(4, np.int32)
         _check(L.pk_csr_to_iparm(h, _p(self.iparm)), "pk_csr_to_iparm")
 
@@ -127,6 +132,17 @@ class CsrOperator:
         sa, st = C.c_int(), C.c_int()
         _check(lib().pk_csr_lanes(self.handle, C.byref(sa), C.byref(st)), "pk_csr_lanes")
         return sa.value, st.value
+
+    def set_sort(self, cap_a: int, cap_t: int):
+        """Column-sorted layout with blocks of at most cap nonzeros per
+        direction (0: row-block layout), pk_csr_set_sort."""
+        _check(lib().pk_csr_set_sort(self.handle, C.c_int(cap_a), C.c_int(cap_t)),
+               "pk_csr_set_sort")
+
+    def sort(self):
+        ca, ct = C.c_int(), C.c_int()
+        _check(lib().pk_csr_sort(self.handle, C.byref(ca), C.byref(ct)), "pk_csr_sort")
+        return ca.value, ct.value
 
     def spmv(self, x, trans="n"):
         x = np.ascontiguousarray(x, _dt(self.prec))
@@ -400,27 +416,32 @@ def lansvd_irl_dev(eng: Engine, neig, dim, p=None, tol=1e-10, which="L", jobu="y
 
 @dataclass
 class Tuning:
-    """pk_csr_autotune's choice: column panels and lane shifts of (A, A^T)
-    and their microseconds per application."""
+    """pk_csr_autotune's choice for (A, A^T): column panels, lane shifts,
+    block caps of the column-sorted layout (0: row-block layout), and their
+    microseconds per application."""
     panels: tuple
     lanes: tuple
     us: tuple
+    sort: tuple = (0, 0)
 
 
-def autotune(A, prec: int = PK_F64, lanes: bool = True) -> Tuning:
+def autotune(A, prec: int = PK_F64, lanes: bool = True, sort: bool = True) -> Tuning:
     """Fastest (A, A^T) column-panel counts (and, with ``lanes``, lanes-per-row
-    shifts) for ``A`` on this GPU (pk_csr_autotune)."""
+    shifts; with ``sort``, the column-sorted layout's block caps) for ``A``
+    on this GPU (pk_csr_autotune)."""
     rp = np.ascontiguousarray(A.rowptr, np.int32)
     ci = np.ascontiguousarray(A.col, np.int32)
     va = np.ascontiguousarray(A.val, _dt(prec))
     pa, pt, ua, ut = C.c_int(), C.c_int(), C.c_double(), C.c_double()
-    sa, st = C.c_int(), C.c_int()
+    sa, st, ca, ct = C.c_int(), C.c_int(), C.c_int(), C.c_int()
     _check(lib().pk_csr_autotune(C.c_int(A.m), C.c_int(A.n), C.c_longlong(int(rp[-1])), _p(rp),
                                  _p(ci), _p(va), C.c_int(prec), C.c_int(PK_HOST), C.byref(pa),
                                  C.byref(pt), C.byref(ua), C.byref(ut),
-                                 C.byref(sa) if lanes else None, C.byref(st) if lanes else None),
+                                 C.byref(sa) if lanes else None, C.byref(st) if lanes else None,
+                                 C.byref(ca) if sort else None, C.byref(ct) if sort else None),
            "pk_csr_autotune")
-    return Tuning((pa.value, pt.value), (sa.value, st.value), (ua.value, ut.value))
+    return Tuning((pa.value, pt.value), (sa.value, st.value), (ua.value, ut.value),
+                  (ca.value, ct.value))
 
 
 def read_matrix(path):

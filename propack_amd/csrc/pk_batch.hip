@@ -46,11 +46,14 @@ struct BProb {
   double* yacc_t;
 };
 
+// A panel of one problem's direction in the layout its operator uses: the
+// row-block arrays (spos null) or the column-sorted ones.
 struct BPanel {
   const int* rp;
   const int* ci;
   const void* va;
   double* lsum;
+  const unsigned short* spos;
 };
 
 // Work item of a batched SpMV launch: descriptor of problem pid, local
@@ -102,6 +105,34 @@ __global__ __launch_bounds__(kBlock) void k_bspmv(const BProb* __restrict__ prob
   spmv_rb_body<VT, T, G>(desc[blockIdx.x], pan.rp, pan.ci, static_cast<const VT*>(pan.va), x, y,
                          yprev, c, part + q, part + Q + q, pan.lsum, p == 0 ? nullptr : yacc,
                          last ? nullptr : yacc, last, prod, lds);
+}
+
+// The same over the column-sorted layout (spmv_sorted_body): one
+// kSortThreads workgroup per descriptor, the launch's largest block cap of
+// dynamic LDS.
+template <typename VT, typename T, int G>
+__global__ __launch_bounds__(kSortThreads) void k_bspmv_sorted(
+    const BProb* __restrict__ probs, const BPanel* __restrict__ panels,
+    const int4* __restrict__ desc, const BWork* __restrict__ work, int tr, int p, int J,
+    int phase, double* __restrict__ part, int Q) {
+  const BWork w = work[blockIdx.x];
+  const BProb P = probs[w.pid];
+  if (Gate{P.ctl, phase, J, 0}.off()) return;
+  extern __shared__ double prod_dyn[];
+  __shared__ double lds[(kSortThreads / kWave) * 2];
+  const BPanel pan = panels[(w.pid * 2 + tr) * kPanelsMax + p];
+  const int np = tr ? P.npt : P.npa;
+  const int last = p == np - 1;
+  double* yacc = tr ? P.yacc_t : P.yacc_a;
+  const T* x = tr ? bcol<T>(P, false, J) : bcol<T>(P, true, J);
+  T* y = tr ? bcol<T>(P, true, J) : bcol<T>(P, false, J + 1);
+  const T* yprev = tr ? (J > 1 ? bcol<T>(P, true, J - 1) : nullptr) : bcol<T>(P, false, J);
+  const double c = tr ? P.ctl->beta : P.ctl->alpha;
+  const int q = (tr ? P.qt0 : P.qa0) + w.loc;
+  spmv_sorted_body<VT, T, G>(desc[blockIdx.x], pan.rp, pan.ci, static_cast<const VT*>(pan.va),
+                             pan.spos, x, y, yprev, c, part + q, part + Q + q, pan.lsum,
+                             p == 0 ? nullptr : yacc, last ? nullptr : yacc, last, prod_dyn,
+                             lds);
 }
 
 template <typename T>
@@ -240,7 +271,7 @@ struct Batch {
   int* cslice_pid_n = nullptr;
   BPanel* panels = nullptr;    // (pid * 2 + tr) * kPanelsMax + p
   struct Launch {              // one batched SpMV launch
-    int tr, p, g, count;
+    int tr, p, g, cap, count;  // cap > 0: column-sorted blocks of at most cap nonzeros
     int4* desc;
     BWork* work;
   };
@@ -283,16 +314,39 @@ void enqueue_batch_step(Batch& B, int J, int elr, int tries) {
       hipLaunchKernelGGL((k_bspmv<float, T, G>), dim3(L.count), dim3(kBlock), 0, s, B.dprob,    \
                          B.panels, L.desc, L.work, tr, p, J, phase, B.part, Q);                 \
     break;
-        switch (L.g) {
-          PK_BS_CASE(2)
-          PK_BS_CASE(4)
-          PK_BS_CASE(8)
-          PK_BS_CASE(16)
-          PK_BS_CASE(32)
-          PK_BS_CASE(64)
-          default: break;
+#define PK_BSS_CASE(G)                                                                          \
+  case G:                                                                                       \
+    if (B.val_prec == PK_F64)                                                                   \
+      hipLaunchKernelGGL((k_bspmv_sorted<double, T, G>), dim3(L.count), dim3(kSortThreads), shm, \
+                         s, B.dprob, B.panels, L.desc, L.work, tr, p, J, phase, B.part, Q);     \
+    else                                                                                        \
+      hipLaunchKernelGGL((k_bspmv_sorted<float, T, G>), dim3(L.count), dim3(kSortThreads), shm,  \
+                         s, B.dprob, B.panels, L.desc, L.work, tr, p, J, phase, B.part, Q);     \
+    break;
+        if (L.cap > 0) {
+          const size_t shm = sizeof(double) * static_cast<size_t>(L.cap);
+          switch (L.g) {
+            PK_BSS_CASE(2)
+            PK_BSS_CASE(4)
+            PK_BSS_CASE(8)
+            PK_BSS_CASE(16)
+            PK_BSS_CASE(32)
+            PK_BSS_CASE(64)
+            default: break;
+          }
+        } else {
+          switch (L.g) {
+            PK_BS_CASE(2)
+            PK_BS_CASE(4)
+            PK_BS_CASE(8)
+            PK_BS_CASE(16)
+            PK_BS_CASE(32)
+            PK_BS_CASE(64)
+            default: break;
+          }
         }
 #undef PK_BS_CASE
+#undef PK_BSS_CASE
       }
       for (const auto& L : B.lng) {
         if (L.tr != tr || L.p != p || L.count == 0) continue;
@@ -390,9 +444,9 @@ int batch_create(int nprob, const pk_lbd_t* handles, Batch** out) {
   int rc = PK_OK;
   std::vector<int> spm, spn, cpm, cpn;
   std::vector<BPanel> hpan(static_cast<size_t>(nprob) * 2 * kPanelsMax, BPanel{});
-  // SpMV work lists keyed by (tr, panel, lanes per row)
+  // SpMV work lists keyed by (tr, panel, lanes per row, sorted-layout cap)
   struct Key {
-    int tr, p, g;
+    int tr, p, g, cap;
     std::vector<int4> desc;
     std::vector<BWork> work;
   };
@@ -455,23 +509,28 @@ int batch_create(int nprob, const pk_lbd_t* handles, Batch** out) {
       const CsrOp::Dir& D = tr ? e->op->dt : e->op->da;
       (tr ? P.npt : P.npa) = D.np;
       (tr ? P.yacc_t : P.yacc_a) = D.yacc;
+      // the layout the operator uses for this direction (row-block or
+      // column-sorted), exactly as its own SpMV runs it
+      const int cap = D.sort_cap;
       for (int p = 0; p < D.np; ++p) {
         const CsrOp::Panel& pan = D.pan[p];
+        const CsrOp::Part& part = cap > 0 ? pan.spart : pan.part;
         hpan[(static_cast<size_t>(i) * 2 + tr) * kPanelsMax + p] =
-            BPanel{pan.rp, pan.ci, pan.va, pan.part.lsum};
-        std::vector<int4> hd(pan.part.ndesc), hl(pan.part.nlong);
-        if ((!hd.empty() && hipMemcpy(hd.data(), pan.part.desc, sizeof(int4) * hd.size(),
+            cap > 0 ? BPanel{pan.rp, pan.sci, pan.sva, part.lsum, pan.spos}
+                    : BPanel{pan.rp, pan.ci, pan.va, part.lsum, nullptr};
+        std::vector<int4> hd(part.ndesc), hl(part.nlong);
+        if ((!hd.empty() && hipMemcpy(hd.data(), part.desc, sizeof(int4) * hd.size(),
                                       hipMemcpyDeviceToHost) != hipSuccess) ||
-            (!hl.empty() && hipMemcpy(hl.data(), pan.part.ldesc, sizeof(int4) * hl.size(),
+            (!hl.empty() && hipMemcpy(hl.data(), part.ldesc, sizeof(int4) * hl.size(),
                                       hipMemcpyDeviceToHost) != hipSuccess)) {
           rc = PK_ERR_HIP;
           break;
         }
         auto kit = std::find_if(keys.begin(), keys.end(), [&](const Key& k) {
-          return k.tr == tr && k.p == p && k.g == pan.g;
+          return k.tr == tr && k.p == p && k.g == pan.g && k.cap == cap;
         });
         if (kit == keys.end()) {
-          keys.push_back(Key{tr, p, pan.g, {}, {}});
+          keys.push_back(Key{tr, p, pan.g, cap, {}, {}});
           kit = keys.end() - 1;
         }
         for (int b = 0; b < static_cast<int>(hd.size()); ++b) {
@@ -486,10 +545,10 @@ int batch_create(int nprob, const pk_lbd_t* handles, Batch** out) {
         }
         for (int l = 0; l < static_cast<int>(hl.size()); ++l) {
           lit->ld.push_back(hl[l]);
-          lit->work.push_back(BWork{i, pan.part.ndesc + l});
+          lit->work.push_back(BWork{i, part.ndesc + l});
         }
         if (p == D.np - 1) {
-          const int slots = pan.part.slots();
+          const int slots = part.slots();
           if (tr) {
             P.qt0 = B->Qt;
             P.nqt = slots;
@@ -510,7 +569,7 @@ int batch_create(int nprob, const pk_lbd_t* handles, Batch** out) {
               upload_vec(cpm, &B->cslice_pid_m) && upload_vec(cpn, &B->cslice_pid_n) &&
               upload_vec(hpan, &B->panels);
     for (auto& k : keys) {
-      Batch::Launch L{k.tr, k.p, k.g, static_cast<int>(k.desc.size()), nullptr, nullptr};
+      Batch::Launch L{k.tr, k.p, k.g, k.cap, static_cast<int>(k.desc.size()), nullptr, nullptr};
       ok = ok && upload_vec(k.desc, &L.desc) && upload_vec(k.work, &L.work);
       B->spmv.push_back(L);
     }

@@ -20,73 +20,42 @@ constexpr int kCgsRows = kBlock * kCgsRPT;
 constexpr int kCT = 4;
 
 // ------------------------------------------------------ row-block SpMV ----
-// Descriptor d = {x, y, z, w}:
-//   * y >= 0: a row block, rows [x, y) holding at most kSpmvCap nonzeros
-//     and kSpmvRows rows.  All 256 threads turn its nonzeros into products
-//     val * x[col] in LDS (prod) -- index and value streams read fully
-//     coalesced, kUnroll independent gathers in flight per thread -- then
-//     groups of G lanes sum each row from LDS (consecutive, conflict-free
-//     reads; shuffle reduction) in a fixed order and apply the epilogue.
-//   * y < 0: chunk -(y+1) of the long row x, nonzeros [z, w): its partial
-//     dot product goes to lsum[-(y+1)]; the long-row fix-up finishes the row.
-// Epilogue (last = 1): y = T(sum - c * yprev), *q_s2 = sum y^2,
-// *q_sd = sum yprev.y over the block (zero for chunks).  Column panels
-// (last = 0) store the FP64 running row sums in yacc_out instead; a panel
-// after the first adds yacc_in.
-template <typename VT, typename T, int G>
-__device__ __forceinline__ void spmv_rb_body(const int4 d, const int* __restrict__ rp,
-                                             const int* __restrict__ ci,
-                                             const VT* __restrict__ va, const T* __restrict__ x,
-                                             T* __restrict__ y, const T* __restrict__ yprev,
-                                             double c, double* q_s2, double* q_sd,
-                                             double* __restrict__ lsum,
-                                             const double* __restrict__ yacc_in,
-                                             double* __restrict__ yacc_out, int last,
-                                             double* prod, double* lds) {
-  if (d.y < 0) {
-    double acc[1] = {0.0};
-    for (int p = d.z + static_cast<int>(threadIdx.x); p < d.w; p += kBlock)
-      acc[0] += static_cast<double>(va[p]) * static_cast<double>(x[ci[p]]);
-    block_sum<1>(acc, lds);
-    if (threadIdx.x == 0) {
-      lsum[-(d.y + 1)] = acc[0];
-      if (last) {
-        *q_s2 = 0.0;
-        *q_sd = 0.0;
-      }
+// Chunk -(d.y+1) of a long row, nonzeros [d.z, d.w), summed by an NT-thread
+// workgroup (thread-strided, then the block reduction); the sum goes to
+// lsum, the fix-up kernel finishes the row.
+template <typename VT, typename T, int NT>
+__device__ __forceinline__ void spmv_chunk(const int4 d, const int* __restrict__ ci,
+                                           const VT* __restrict__ va, const T* __restrict__ x,
+                                           double* q_s2, double* q_sd,
+                                           double* __restrict__ lsum, int last, double* lds) {
+  double acc[1] = {0.0};
+  for (int p = d.z + static_cast<int>(threadIdx.x); p < d.w; p += NT)
+    acc[0] += static_cast<double>(va[p]) * static_cast<double>(x[ci[p]]);
+  block_sum<1, NT>(acc, lds);
+  if (threadIdx.x == 0) {
+    lsum[-(d.y + 1)] = acc[0];
+    if (last) {
+      *q_s2 = 0.0;
+      *q_sd = 0.0;
     }
-    return;
   }
-  const int r0 = d.x, r1 = d.y;
-  const int p0 = rp[r0];
-  const int nz = rp[r1] - p0;
+}
+
+// Row sums of rows [r0, r1) from their products in LDS (prod[q] for the
+// block's q-th nonzero in row-major order): groups of G lanes, lane l of a
+// row adds prod[a + l], prod[a + l + G], ... then a shuffle tree; then the
+// epilogue.  The order depends only on G and the row, not on the block.
+template <typename T, int G, int NT>
+__device__ __forceinline__ void spmv_rows(int r0, int r1, int p0, const int* __restrict__ rp,
+                                          const double* prod, T* __restrict__ y,
+                                          const T* __restrict__ yprev, double c, double* q_s2,
+                                          double* q_sd, const double* __restrict__ yacc_in,
+                                          double* __restrict__ yacc_out, int last,
+                                          double* lds) {
   double s2 = 0.0, sd = 0.0;
-  // Eight independent loads per thread; with kSpmvCap = 1024 the upper
-  // half is masked, which measured faster than a 4-deep unroll.
-  constexpr int kUnroll = 8;
-  for (int base = 0; base < nz; base += kBlock * kUnroll) {
-    int idx[kUnroll];
-    double v[kUnroll], xv[kUnroll];
-#pragma unroll
-    for (int u = 0; u < kUnroll; ++u) {
-      const int t = base + u * kBlock + static_cast<int>(threadIdx.x);
-      // index and value streams are read once: non-temporal, so that they
-      // do not evict the gathered vector slice from L2
-      idx[u] = t < nz ? __builtin_nontemporal_load(ci + p0 + t) : -1;
-      v[u] = t < nz ? static_cast<double>(__builtin_nontemporal_load(va + p0 + t)) : 0.0;
-    }
-#pragma unroll
-    for (int u = 0; u < kUnroll; ++u) xv[u] = idx[u] >= 0 ? static_cast<double>(x[idx[u]]) : 0.0;
-#pragma unroll
-    for (int u = 0; u < kUnroll; ++u) {
-      const int t = base + u * kBlock + static_cast<int>(threadIdx.x);
-      if (t < nz) prod[t] = v[u] * xv[u];
-    }
-  }
-  __syncthreads();
   const int lane = threadIdx.x & (G - 1);
   const int group = threadIdx.x / G;
-  constexpr int ngroups = kBlock / G;
+  constexpr int ngroups = NT / G;
   const int nrows = r1 - r0;
   const int iters = (nrows + ngroups - 1) / ngroups;  // uniform across the block
   for (int it = 0; it < iters; ++it) {
@@ -115,11 +84,116 @@ __device__ __forceinline__ void spmv_rb_body(const int4 d, const int* __restrict
   }
   if (!last) return;
   double v2[2] = {s2, sd};
-  block_sum<2>(v2, lds);
+  block_sum<2, NT>(v2, lds);
   if (threadIdx.x == 0) {
     *q_s2 = v2[0];
     *q_sd = v2[1];
   }
+}
+
+// Descriptor d = {x, y, z, w}:
+//   * y >= 0: a row block, rows [x, y) holding at most kSpmvCap nonzeros
+//     and kSpmvRows rows.  All 256 threads turn its nonzeros into products
+//     val * x[col] in LDS (prod) -- index and value streams read fully
+//     coalesced, kUnroll independent gathers in flight per thread -- then
+//     groups of G lanes sum each row from LDS (consecutive, conflict-free
+//     reads; shuffle reduction) in a fixed order and apply the epilogue.
+//   * y < 0: chunk -(y+1) of the long row x, nonzeros [z, w): its partial
+//     dot product goes to lsum[-(y+1)]; the long-row fix-up finishes the row.
+// Epilogue (last = 1): y = T(sum - c * yprev), *q_s2 = sum y^2,
+// *q_sd = sum yprev.y over the block (zero for chunks).  Column panels
+// (last = 0) store the FP64 running row sums in yacc_out instead; a panel
+// after the first adds yacc_in.
+template <typename VT, typename T, int G>
+__device__ __forceinline__ void spmv_rb_body(const int4 d, const int* __restrict__ rp,
+                                             const int* __restrict__ ci,
+                                             const VT* __restrict__ va, const T* __restrict__ x,
+                                             T* __restrict__ y, const T* __restrict__ yprev,
+                                             double c, double* q_s2, double* q_sd,
+                                             double* __restrict__ lsum,
+                                             const double* __restrict__ yacc_in,
+                                             double* __restrict__ yacc_out, int last,
+                                             double* prod, double* lds) {
+  if (d.y < 0) {
+    spmv_chunk<VT, T, kBlock>(d, ci, va, x, q_s2, q_sd, lsum, last, lds);
+    return;
+  }
+  const int r0 = d.x, r1 = d.y;
+  const int p0 = rp[r0];
+  const int nz = rp[r1] - p0;
+  // Eight independent loads per thread; with kSpmvCap = 1024 the upper
+  // half is masked, which measured faster than a 4-deep unroll.
+  constexpr int kUnroll = 8;
+  for (int base = 0; base < nz; base += kBlock * kUnroll) {
+    int idx[kUnroll];
+    double v[kUnroll], xv[kUnroll];
+#pragma unroll
+    for (int u = 0; u < kUnroll; ++u) {
+      const int t = base + u * kBlock + static_cast<int>(threadIdx.x);
+      // index and value streams are read once: non-temporal, so that they
+      // do not evict the gathered vector slice from L2
+      idx[u] = t < nz ? __builtin_nontemporal_load(ci + p0 + t) : -1;
+      v[u] = t < nz ? static_cast<double>(__builtin_nontemporal_load(va + p0 + t)) : 0.0;
+    }
+#pragma unroll
+    for (int u = 0; u < kUnroll; ++u) xv[u] = idx[u] >= 0 ? static_cast<double>(x[idx[u]]) : 0.0;
+#pragma unroll
+    for (int u = 0; u < kUnroll; ++u) {
+      const int t = base + u * kBlock + static_cast<int>(threadIdx.x);
+      if (t < nz) prod[t] = v[u] * xv[u];
+    }
+  }
+  __syncthreads();
+  spmv_rows<T, G, kBlock>(r0, r1, p0, rp, prod, y, yprev, c, q_s2, q_sd, yacc_in, yacc_out,
+                          last, lds);
+}
+
+// ------------------------------------------------- column-sorted SpMV ----
+// Sorted layout of a panel: row blocks of at most cap nonzeros (a
+// per-operator choice, <= kSortCapMax) and kSortRows rows, rows longer than
+// kSpmvCap split into long-row chunks as in the row-block layout.  Within a
+// block the nonzeros are stored in increasing column order (sci, sva; ties
+// in row-major order) with spos[q] = the entry's offset in row-major order
+// within the block.  Consecutive lanes read consecutive sorted entries, so
+// the gathers of nearby columns fall into the same cache lines and merge;
+// each product is written to LDS at its row-major offset and the rows are
+// summed from there by spmv_rows -- for the same G every row sum is bit for
+// bit the row-block layout's.  Long-row chunks keep the sorted order too
+// (the chunk sum is a fixed-order block reduction either way).
+template <typename VT, typename T, int G>
+__device__ __forceinline__ void spmv_sorted_body(
+    const int4 d, const int* __restrict__ rp, const int* __restrict__ sci,
+    const VT* __restrict__ sva, const unsigned short* __restrict__ spos,
+    const T* __restrict__ x, T* __restrict__ y, const T* __restrict__ yprev, double c,
+    double* q_s2, double* q_sd, double* __restrict__ lsum, const double* __restrict__ yacc_in,
+    double* __restrict__ yacc_out, int last, double* prod, double* lds) {
+  if (d.y < 0) {
+    spmv_chunk<VT, T, kSortThreads>(d, sci, sva, x, q_s2, q_sd, lsum, last, lds);
+    return;
+  }
+  const int r0 = d.x, r1 = d.y;
+  const int p0 = rp[r0];
+  const int nz = rp[r1] - p0;
+  constexpr int kUnroll = 8;
+  for (int base = 0; base < nz; base += kSortThreads * kUnroll) {
+    int idx[kUnroll], pos[kUnroll];
+    double v[kUnroll], xv[kUnroll];
+#pragma unroll
+    for (int u = 0; u < kUnroll; ++u) {
+      const int t = base + u * kSortThreads + static_cast<int>(threadIdx.x);
+      idx[u] = t < nz ? __builtin_nontemporal_load(sci + p0 + t) : -1;
+      v[u] = t < nz ? static_cast<double>(__builtin_nontemporal_load(sva + p0 + t)) : 0.0;
+      pos[u] = t < nz ? static_cast<int>(__builtin_nontemporal_load(spos + p0 + t)) : 0;
+    }
+#pragma unroll
+    for (int u = 0; u < kUnroll; ++u) xv[u] = idx[u] >= 0 ? static_cast<double>(x[idx[u]]) : 0.0;
+#pragma unroll
+    for (int u = 0; u < kUnroll; ++u)
+      if (idx[u] >= 0) prod[pos[u]] = v[u] * xv[u];
+  }
+  __syncthreads();
+  spmv_rows<T, G, kSortThreads>(r0, r1, p0, rp, prod, y, yprev, c, q_s2, q_sd, yacc_in,
+                                yacc_out, last, lds);
 }
 
 // Long row ld = {row, first chunk, number of chunks, -}: one wave sums the

@@ -49,12 +49,13 @@ __device__ __forceinline__ double group_sum(double v) {
   return v;
 }
 
-// Deterministic block reduction of NV values held by every thread; thread 0
-// receives the sums.  lds must hold (kBlock / kWave) * NV doubles.
-template <int NV>
+// Deterministic block reduction of NV values held by every thread of an
+// NT-thread workgroup; thread 0 receives the sums.  lds must hold
+// (NT / kWave) * NV doubles.
+template <int NV, int NT = kBlock>
 __device__ __forceinline__ void block_sum(double (&v)[NV], double* lds) {
   const int lane = threadIdx.x % kWave, wid = threadIdx.x / kWave;
-  constexpr int nw = kBlock / kWave;
+  constexpr int nw = NT / kWave;
 #pragma unroll
   for (int i = 0; i < NV; ++i) v[i] = wave_sum(v[i]);
   if (lane == 0) {

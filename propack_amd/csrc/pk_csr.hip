@@ -6,6 +6,7 @@
 // entries in increasing original-row order: the build is deterministic and
 // the A^T SpMV sums in a fixed order.
 #include <rocprim/device/device_radix_sort.hpp>
+#include <rocprim/device/device_segmented_radix_sort.hpp>
 
 #include <algorithm>
 #include <cstdio>
@@ -87,15 +88,17 @@ int choose_lpr(long long nnz, int rows, int shift = 0) {
 }
 
 // Workgroup descriptors of the row-block SpMV: greedy blocks of at most
-// kSpmvRows rows and kSpmvCap nonzeros; a row longer than kSpmvCap becomes
-// ceil(len / kLongChunk) chunk descriptors plus an entry of the long-row
-// table {row, first chunk, chunks, 0}.
+// max_rows rows and cap nonzeros (kSpmvRows and kSpmvCap for the row-block
+// layout, larger for the column-sorted one); a row longer than kSpmvCap
+// becomes ceil(len / kLongChunk) chunk descriptors plus an entry of the
+// long-row table {row, first chunk, chunks, 0}.
 struct Partition {
   std::vector<int4> desc, ldesc;
   int nchunks = 0;
 };
 
-Partition partition(const std::vector<int>& rp, int rows) {
+Partition partition(const std::vector<int>& rp, int rows, int cap = kSpmvCap,
+                    int max_rows = kSpmvRows) {
   Partition P;
   P.desc.reserve(static_cast<size_t>(rows) / 64 + 2);
   int r = 0;
@@ -115,9 +118,9 @@ Partition partition(const std::vector<int>& rp, int rows) {
     }
     const int start = r;
     long long nz = 0;
-    while (r < rows && r - start < kSpmvRows) {
+    while (r < rows && r - start < max_rows) {
       const long long len = static_cast<long long>(rp[r + 1]) - rp[r];
-      if (len > kSpmvCap || nz + len > kSpmvCap) break;
+      if (len > kSpmvCap || nz + len > cap) break;
       nz += len;
       ++r;
     }
@@ -275,7 +278,123 @@ bool finish_panel(CsrOp::Panel& P, int rows) {
   return upload(partition(hrp, rows), P.part);
 }
 
+// Column-sorted copy of a panel, one workgroup per segment [sb, se) (a
+// descriptor's nonzeros): perm holds the panel positions in sorted order.
+template <typename VT>
+__global__ void k_sorted_finish(const int* __restrict__ sb, const int* __restrict__ se,
+                                const int* __restrict__ perm, const VT* __restrict__ va,
+                                VT* __restrict__ sva, unsigned short* __restrict__ spos) {
+  const int b = sb[blockIdx.x], e = se[blockIdx.x];
+  for (int q = b + static_cast<int>(threadIdx.x); q < e; q += static_cast<int>(blockDim.x)) {
+    const int p = perm[q];
+    sva[q] = va[p];
+    spos[q] = static_cast<unsigned short>(p - b);
+  }
+}
+
+void free_sorted(CsrOp::Panel& P) {
+  (void)hipFree(P.sci);
+  (void)hipFree(P.sva);
+  (void)hipFree(P.spos);
+  (void)hipFree(P.spart.desc);
+  (void)hipFree(P.spart.ldesc);
+  (void)hipFree(P.spart.lsum);
+  P.sci = nullptr;
+  P.sva = nullptr;
+  P.spos = nullptr;
+  P.spart = CsrOp::Part{};
+}
+
+// Sorted layout of one panel (rows x cols): the descriptors of partition()
+// with blocks of at most cap nonzeros and kSortRows rows; inside each
+// descriptor's range a stable segmented radix sort of the positions keyed
+// by column (rocPRIM), so ties keep their row-major order.
+template <typename VT>
+int build_sorted_panel(CsrOp::Panel& P, int rows, int cols, int cap, int max_slots,
+                       hipStream_t s) {
+  std::vector<int> hrp(static_cast<size_t>(rows) + 1);
+  if (hipMemcpy(hrp.data(), P.rp, sizeof(int) * hrp.size(), hipMemcpyDeviceToHost) !=
+      hipSuccess)
+    return PK_ERR_HIP;
+  const Partition part = partition(hrp, rows, cap, kSortRows);
+  if (static_cast<int>(part.desc.size() + part.ldesc.size()) > max_slots) return PK_ERR_NOTSUP;
+  const int nseg = static_cast<int>(part.desc.size());
+  std::vector<int> sb(std::max(nseg, 1)), se(std::max(nseg, 1));
+  for (int i = 0; i < nseg; ++i) {
+    const int4 d = part.desc[i];
+    sb[i] = d.y < 0 ? d.z : hrp[d.x];
+    se[i] = d.y < 0 ? d.w : hrp[d.y];
+  }
+  const long long nnz = P.nnz;
+  const size_t nz = static_cast<size_t>(std::max<long long>(nnz, 1));
+  int *dsb = nullptr, *dse = nullptr, *perm_in = nullptr, *perm_out = nullptr;
+  void* tmp = nullptr;
+  size_t tmp_bytes = 0;
+  int rc = PK_OK;
+#define PKT(x)             \
+  if ((x) != hipSuccess) { \
+    rc = PK_ERR_HIP;       \
+    goto out;              \
+  }
+  PKT(hipMalloc(&P.sci, sizeof(int) * nz));
+  PKT(hipMalloc(&P.sva, sizeof(VT) * nz));
+  PKT(hipMalloc(&P.spos, sizeof(unsigned short) * nz));
+  PKT(hipMalloc(&dsb, sizeof(int) * sb.size()));
+  PKT(hipMalloc(&dse, sizeof(int) * se.size()));
+  PKT(hipMalloc(&perm_in, sizeof(int) * nz));
+  PKT(hipMalloc(&perm_out, sizeof(int) * nz));
+  PKT(hipMemcpyAsync(dsb, sb.data(), sizeof(int) * sb.size(), hipMemcpyHostToDevice, s));
+  PKT(hipMemcpyAsync(dse, se.data(), sizeof(int) * se.size(), hipMemcpyHostToDevice, s));
+  if (nnz > 0 && nseg > 0) {
+    hipLaunchKernelGGL(k_iota, dim3(2048), dim3(256), 0, s, nnz, perm_in);
+    PKT(rocprim::segmented_radix_sort_pairs(nullptr, tmp_bytes, P.ci, P.sci, perm_in, perm_out,
+                                            static_cast<unsigned>(nnz),
+                                            static_cast<unsigned>(nseg), dsb, dse, 0,
+                                            bits_for(cols), s));
+    PKT(hipMalloc(&tmp, std::max<size_t>(tmp_bytes, 1)));
+    PKT(rocprim::segmented_radix_sort_pairs(tmp, tmp_bytes, P.ci, P.sci, perm_in, perm_out,
+                                            static_cast<unsigned>(nnz),
+                                            static_cast<unsigned>(nseg), dsb, dse, 0,
+                                            bits_for(cols), s));
+    hipLaunchKernelGGL((k_sorted_finish<VT>), dim3(nseg), dim3(256), 0, s, dsb, dse, perm_out,
+                       static_cast<const VT*>(P.va), static_cast<VT*>(P.sva), P.spos);
+  }
+  PKT(hipGetLastError());
+  PKT(hipStreamSynchronize(s));
+  if (!upload(part, P.spart)) rc = PK_ERR_HIP;
+out:
+#undef PKT
+  (void)hipFree(dsb);
+  (void)hipFree(dse);
+  (void)hipFree(perm_in);
+  (void)hipFree(perm_out);
+  (void)hipFree(tmp);
+  if (rc != PK_OK) free_sorted(P);
+  return rc;
+}
+
 }  // namespace
+
+int csr_set_sort(const CsrOp* op, CsrOp::Dir& D, int cap) {
+  if (cap != 0 && (cap < kSpmvCap || cap > kSortCapMax)) return PK_ERR_ARG;
+  for (int p = 0; p < D.np; ++p) free_sorted(D.pan[p]);
+  D.sort_cap = 0;
+  if (cap == 0) return PK_OK;
+  hipStream_t s = nullptr;
+  if (hipStreamCreateWithFlags(&s, hipStreamNonBlocking) != hipSuccess) return PK_ERR_HIP;
+  int rc = PK_OK;
+  for (int p = 0; p < D.np && rc == PK_OK; ++p)
+    rc = op->val_prec == PK_F64
+             ? build_sorted_panel<double>(D.pan[p], D.rows, D.cols, cap, D.slots, s)
+             : build_sorted_panel<float>(D.pan[p], D.rows, D.cols, cap, D.slots, s);
+  (void)hipStreamDestroy(s);
+  if (rc != PK_OK) {
+    for (int p = 0; p < D.np; ++p) free_sorted(D.pan[p]);
+    return rc;
+  }
+  D.sort_cap = cap;
+  return PK_OK;
+}
 
 // Lays one direction out as np column panels.  Takes ownership of the
 // plain CSR arrays (rp, ci, va): kept as the single panel when np = 1,
@@ -373,6 +492,7 @@ out:
 static void free_dir(CsrOp::Dir& D) {
   for (int p = 0; p < CsrOp::kMaxPanels; ++p) {
     CsrOp::Panel& P = D.pan[p];
+    free_sorted(P);
     (void)hipFree(P.rp);
     (void)hipFree(P.ci);
     (void)hipFree(P.va);
@@ -481,10 +601,25 @@ int csr_apply(const CsrOp* op, hipStream_t s, char trans, const T* x, T* y, cons
   int nb = -1;
   for (int p = 0; p < D.np; ++p) {
     const CsrOp::Panel& P = D.pan[p];
-    const CsrOp::Part& Q = P.part;
     const double* yin = p == 0 ? nullptr : D.yacc;
     const int last = p == D.np - 1;
     double* yout = last ? nullptr : D.yacc;
+    if (D.sort_cap > 0) {
+      const CsrOp::Part& Q = P.spart;
+      if (op->val_prec == PK_F64)
+        nb = launch_spmv_sorted<double, T>(s, Q.ndesc, Q.desc, Q.nlong, Q.ldesc, Q.lsum, P.rp,
+                                           P.sci, static_cast<const double*>(P.sva), P.spos,
+                                           D.sort_cap, x, y, yprev, coef, part, P.g, yin, yout,
+                                           last, gate);
+      else
+        nb = launch_spmv_sorted<float, T>(s, Q.ndesc, Q.desc, Q.nlong, Q.ldesc, Q.lsum, P.rp,
+                                          P.sci, static_cast<const float*>(P.sva), P.spos,
+                                          D.sort_cap, x, y, yprev, coef, part, P.g, yin, yout,
+                                          last, gate);
+      if (nb < 0) return nb;
+      continue;
+    }
+    const CsrOp::Part& Q = P.part;
     if (op->val_prec == PK_F64)
       nb = launch_spmv_rb<double, T>(s, Q.ndesc, Q.desc, Q.nlong, Q.ldesc, Q.lsum, P.rp, P.ci,
                                      static_cast<const double*>(P.va), x, y, yprev, coef, part,
@@ -647,45 +782,66 @@ int pk_csr_panels(pk_csr_t h, int* panels_a, int* panels_t) {
 }
 
 // SpMV autotuner: builds the operator with 1, 2, 4 and 8 column panels per
-// direction and, when lanes_a / lanes_t are given, tries lanes per row of
-// half, equal to and twice the built-in choice on each; times each
-// direction's application and returns the fastest setting of each direction
-// and its time.  The choice changes how the row sums are grouped (not what
-// is summed), so an operator built with it gives results equal to the
-// built-in layout's up to rounding.
+// direction (up to 16 when sorted layouts are tried) and, when lanes_a /
+// lanes_t are given, tries lanes per row of half, equal to and twice the
+// built-in choice on each; when sort_a / sort_t are given, each of these in
+// the row-block layout and in the column-sorted layout with blocks of 8192
+// and 16384 nonzeros.  Times each direction's application and returns the
+// fastest setting of each direction and its time.  The choice changes how
+// the products are grouped (not what is summed), so an operator built with
+// it gives results equal to the built-in layout's up to rounding.
 int pk_csr_autotune(int m, int n, long long nnz, const int* rowptr, const int* colind,
                     const void* val, int val_prec, int where, int* panels_a, int* panels_t,
-                    double* us_a, double* us_t, int* lanes_a, int* lanes_t) {
+                    double* us_a, double* us_t, int* lanes_a, int* lanes_t, int* sort_a,
+                    int* sort_t) {
   if (!panels_a || !panels_t) return PK_ERR_ARG;
   const bool lanes = lanes_a && lanes_t;
+  const bool sorts = sort_a && sort_t;
+  static const int kCaps[3] = {0, 8192, 16384};
+  const int ncaps = sorts ? 3 : 1;
+  // PK_AUTOTUNE_LOG=1: one line per timed setting on stderr
+  const char* lg = std::getenv("PK_AUTOTUNE_LOG");
+  const bool verbose = lg && *lg && *lg != '0';
+  const int pmax = sorts ? CsrOp::kMaxPanels : 8;
   double best_a = 1e300, best_t = 1e300;
-  int pa = 1, pt = 1, sa = 0, st = 0;
-  for (int P = 1; P <= 8; P *= 2) {
+  int pa = 1, pt = 1, sa = 0, st = 0, ca = 0, ct = 0;
+  for (int P = 1; P <= pmax; P *= 2) {
     CsrOp* o = nullptr;
     int rc = csr_create(m, n, nnz, rowptr, colind, val, val_prec, where, &o, P, P);
     if (rc != PK_OK) return rc;
     const int npa = o->da.np, npt = o->dt.np;
-    for (int sh = lanes ? -1 : 0; sh <= (lanes ? 1 : 0) && rc == PK_OK; ++sh) {
-      csr_set_lanes(o->da, sh);
-      csr_set_lanes(o->dt, sh);
-      double ta = 0, tt = 0;
-      if (val_prec == PK_F64) {
-        rc = time_apply<double>(o, 'n', 5, &ta);
-        if (rc == PK_OK) rc = time_apply<double>(o, 't', 5, &tt);
-      } else {
-        rc = time_apply<float>(o, 'n', 5, &ta);
-        if (rc == PK_OK) rc = time_apply<float>(o, 't', 5, &tt);
-      }
-      if (rc != PK_OK) break;
-      if (npa == P && ta < best_a) {
-        best_a = ta;
-        pa = P;
-        sa = sh;
-      }
-      if (npt == P && tt < best_t) {
-        best_t = tt;
-        pt = P;
-        st = sh;
+    for (int ic = 0; ic < ncaps && rc == PK_OK; ++ic) {
+      const int cap = kCaps[ic];
+      const int rca = csr_set_sort(o, o->da, cap), rct = csr_set_sort(o, o->dt, cap);
+      if (rca == PK_ERR_NOTSUP || rct == PK_ERR_NOTSUP) continue;  // more slots than built
+      rc = rca != PK_OK ? rca : rct;
+      for (int sh = lanes ? -1 : 0; sh <= (lanes ? 1 : 0) && rc == PK_OK; ++sh) {
+        csr_set_lanes(o->da, sh);
+        csr_set_lanes(o->dt, sh);
+        double ta = 0, tt = 0;
+        if (val_prec == PK_F64) {
+          rc = time_apply<double>(o, 'n', 5, &ta);
+          if (rc == PK_OK) rc = time_apply<double>(o, 't', 5, &tt);
+        } else {
+          rc = time_apply<float>(o, 'n', 5, &ta);
+          if (rc == PK_OK) rc = time_apply<float>(o, 't', 5, &tt);
+        }
+        if (rc != PK_OK) break;
+        if (verbose)
+          std::fprintf(stderr, "pk_csr_autotune: panels %d/%d cap %d shift %d: A %.1f us, A^T %.1f us\n",
+                       npa, npt, cap, sh, ta, tt);
+        if (npa == P && ta < best_a) {
+          best_a = ta;
+          pa = P;
+          sa = sh;
+          ca = cap;
+        }
+        if (npt == P && tt < best_t) {
+          best_t = tt;
+          pt = P;
+          st = sh;
+          ct = cap;
+        }
       }
     }
     csr_destroy(o);
@@ -697,8 +853,40 @@ int pk_csr_autotune(int m, int n, long long nnz, const int* rowptr, const int* c
     *lanes_a = sa;
     *lanes_t = st;
   }
+  if (sorts) {
+    *sort_a = ca;
+    *sort_t = ct;
+  }
   if (us_a) *us_a = best_a;
   if (us_t) *us_t = best_t;
+  return PK_OK;
+}
+
+int pk_csr_set_sort(pk_csr_t h, int cap_a, int cap_t) {
+  CsrOp* op = as_csr(h);
+  if (!op) return PK_ERR_HANDLE;
+  for (const int c : {cap_a, cap_t})
+    if (c != 0 && (c < kSpmvCap || c > kSortCapMax)) {
+      set_error("pk_csr_set_sort: a block cap must be 0 or lie in [%d, %d]", kSpmvCap,
+                kSortCapMax);
+      return PK_ERR_ARG;
+    }
+  int rc = csr_set_sort(op, op->da, cap_a);
+  if (rc == PK_OK) rc = csr_set_sort(op, op->dt, cap_t);
+  if (rc != PK_OK) {
+    (void)csr_set_sort(op, op->da, 0);
+    (void)csr_set_sort(op, op->dt, 0);
+    set_error("pk_csr_set_sort failed (code %d); the operator is back in the row-block layout",
+              rc);
+  }
+  return rc;
+}
+
+int pk_csr_sort(pk_csr_t h, int* cap_a, int* cap_t) {
+  CsrOp* op = as_csr(h);
+  if (!op) return PK_ERR_HANDLE;
+  if (cap_a) *cap_a = op->da.sort_cap;
+  if (cap_t) *cap_t = op->dt.sort_cap;
   return PK_OK;
 }
 

@@ -39,6 +39,12 @@ struct CsrOp {
     long long nnz = 0;
     int g = 8;  // lanes per row in the LDS row reduction
     Part part;
+    // column-sorted layout (built on request): per block, the entries in
+    // increasing column order and their row-major offsets; own descriptors
+    int* sci = nullptr;
+    void* sva = nullptr;
+    unsigned short* spos = nullptr;
+    Part spart;
   };
   struct Dir {
     int rows = 0, cols = 0;
@@ -47,6 +53,7 @@ struct CsrOp {
     double* yacc = nullptr;  // FP64 running row sums between panels (np > 1)
     int slots = 0;           // partial slots of the widest panel
     int lane_shift = 0;      // lanes per row = built-in choice * 2^lane_shift
+    int sort_cap = 0;        // > 0: column-sorted layout, blocks of <= sort_cap nonzeros
   } da, dt;
   int nrb = 0, nrbt = 0;  // partial slots per direction
 };
@@ -67,6 +74,10 @@ void csr_destroy(CsrOp* op);
 // Lanes per row of every panel of D: the built-in choice times 2^shift,
 // clamped to [2, 64]; shift in [-2, 2].
 void csr_set_lanes(CsrOp::Dir& D, int shift);
+// Switches direction D of op to the column-sorted layout with blocks of at
+// most cap nonzeros (kSpmvCap <= cap <= kSortCapMax; built on the device),
+// or back to the row-block layout (cap = 0, the sorted copy freed).
+int csr_set_sort(const CsrOp* op, CsrOp::Dir& D, int cap);
 template <typename T>
 int csr_apply(const CsrOp* op, hipStream_t s, char trans, const T* x, T* y, const T* yprev,
               Scal coef, double* part, Gate gate);

@@ -94,6 +94,60 @@ int launch_spmv_rb(hipStream_t s, int nblk, const int4* desc, int nlong, const i
   return hipGetLastError() == hipSuccess ? nbt : -1;
 }
 
+// ------------------------------------------------- column-sorted SpMV ----
+// One kSortThreads workgroup per descriptor of the sorted layout
+// (spmv_sorted_body); the products of a block take cap doubles of dynamic
+// LDS.  Partial slots and the long-row fix-up as in k_spmv_rb.
+template <typename VT, typename T, int G>
+__global__ __launch_bounds__(kSortThreads) void k_spmv_sorted(
+    const int4* __restrict__ desc, const int* __restrict__ rp, const int* __restrict__ sci,
+    const VT* __restrict__ sva, const unsigned short* __restrict__ spos,
+    const T* __restrict__ x, T* __restrict__ y, const T* __restrict__ yprev, Scal coef,
+    double* __restrict__ part, int nbt, double* __restrict__ lsum,
+    const double* __restrict__ yacc_in, double* __restrict__ yacc_out, int last, Gate gate) {
+  if (gate.off()) return;
+  extern __shared__ double prod_dyn[];
+  __shared__ double lds[(kSortThreads / kWave) * 2];
+  spmv_sorted_body<VT, T, G>(desc[blockIdx.x], rp, sci, sva, spos, x, y, yprev, coef.get(),
+                             part + blockIdx.x, part + nbt + blockIdx.x, lsum, yacc_in,
+                             yacc_out, last, prod_dyn, lds);
+}
+
+template <typename VT, typename T>
+int launch_spmv_sorted(hipStream_t s, int nblk, const int4* desc, int nlong, const int4* ldesc,
+                       double* lsum, const int* rp, const int* sci, const VT* sva,
+                       const unsigned short* spos, int cap, const T* x, T* y, const T* yprev,
+                       Scal coef, double* part, int g, const double* yacc_in, double* yacc_out,
+                       int last, Gate gate) {
+  if (cap <= 0 || cap > kSortCapMax) return -1;
+  const int nbt = nblk + nlong;
+  const size_t shm = sizeof(double) * static_cast<size_t>(cap);
+  if (nblk > 0) {
+#define PK_SO_CASE(G)                                                                          \
+  case G:                                                                                      \
+    hipLaunchKernelGGL((k_spmv_sorted<VT, T, G>), dim3(nblk), dim3(kSortThreads), shm, s, desc, \
+                       rp, sci, sva, spos, x, y, yprev, coef, part, nbt, lsum, yacc_in,         \
+                       yacc_out, last, gate);                                                  \
+    break;
+    switch (g) {
+      PK_SO_CASE(2)
+      PK_SO_CASE(4)
+      PK_SO_CASE(8)
+      PK_SO_CASE(16)
+      PK_SO_CASE(32)
+      PK_SO_CASE(64)
+      default:
+        return -1;
+    }
+#undef PK_SO_CASE
+  }
+  if (nlong > 0)
+    hipLaunchKernelGGL(k_spmv_longfix<T>, dim3(cdiv(nlong, kBlock / kWave)), dim3(kBlock), 0,
+                       s, nlong, ldesc, lsum, y, yprev, coef, part, nbt, nblk, yacc_in, yacc_out,
+                       last, gate);
+  return hipGetLastError() == hipSuccess ? nbt : -1;
+}
+
 // -------------------------------------------------------------- reduce ----
 __global__ __launch_bounds__(kBlock) void k_reduce(int nblk, const double* __restrict__ part,
                                                    double* __restrict__ out,
@@ -427,6 +481,18 @@ PK_INST_RB(float, float)
 PK_INST_RB(double, float)
 PK_INST_RB(float, double)
 #undef PK_INST_RB
+
+#define PK_INST_SO(VT, T)                                                                    \
+  template int launch_spmv_sorted<VT, T>(hipStream_t, int, const int4*, int, const int4*,    \
+                                         double*, const int*, const int*, const VT*,         \
+                                         const unsigned short*, int, const T*, T*, const T*, \
+                                         Scal, double*, int, const double*, double*, int,    \
+                                         Gate);
+PK_INST_SO(double, double)
+PK_INST_SO(float, float)
+PK_INST_SO(double, float)
+PK_INST_SO(float, double)
+#undef PK_INST_SO
 
 template int launch_convert<double, float>(hipStream_t, long long, const double*, float*);
 template int launch_convert<float, double>(hipStream_t, long long, const float*, double*);

@@ -261,18 +261,109 @@ def test_autotune_and_explicit_panels():
     gives the plain layout's products up to rounding."""
     A = synth.poisson_decay(200_000, 100_000, 16.0, 0.6, seed=4)
     t = api.autotune(A)
-    (pa, pt), (sa, st), (ua, ut) = t.panels, t.lanes, t.us
-    assert pa in (1, 2, 4, 8) and pt in (1, 2, 4, 8) and ua > 0 and ut > 0
+    (pa, pt), (sa, st), (ua, ut), (ca, ct) = t.panels, t.lanes, t.us, t.sort
+    assert pa in (1, 2, 4, 8, 16) and pt in (1, 2, 4, 8, 16) and ua > 0 and ut > 0
     assert sa in (-1, 0, 1) and st in (-1, 0, 1)
+    assert ca in (0, 8192, 16384) and ct in (0, 8192, 16384)
+    t0 = api.autotune(A, sort=False)
+    assert t0.sort == (0, 0) and t0.panels[0] in (1, 2, 4, 8) and t0.panels[1] in (1, 2, 4, 8)
     x = synth.start_vector(A.n, 2)
     y = synth.start_vector(A.m, 3)
     want_n, want_t = A.matvec(x), A.transpose().matvec(y)
-    for panels, lanes in (((pa, pt), (sa, st)), ((1, 1), None), ((8, 2), (2, -2)),
-                          ((16, 16), (-1, 1))):
-        op = api.CsrOperator(A, panels=panels, lanes=lanes)
+    for panels, lanes, sort in (((pa, pt), (sa, st), (ca, ct)), ((1, 1), None, None),
+                                ((8, 2), (2, -2), (1024, 16384)), ((16, 16), (-1, 1), None)):
+        op = api.CsrOperator(A, panels=panels, lanes=lanes, sort=sort)
         assert op.panels() == tuple(min(p, 16) for p in panels)
         assert op.lanes() == (lanes or (0, 0))
+        assert op.sort() == (sort or (0, 0))
         np.testing.assert_allclose(op.spmv(x, "n"), want_n, rtol=0,
                                    atol=1e-12 * np.abs(want_n).max())
         np.testing.assert_allclose(op.spmv(y, "t"), want_t, rtol=0,
                                    atol=1e-12 * np.abs(want_t).max())
+
+
+def _edge_matrix(rng, m=300, n=9500):
+    D = np.zeros((m, n))
+    for i in range(0, m, 3):
+        cols = rng.choice(n, rng.integers(1, 20), replace=False)
+        D[i, cols] = rng.standard_normal(cols.size)
+    D[7, :9000] = rng.standard_normal(9000)
+    D[:, 42] = rng.standard_normal(m)
+    return _csr_from_dense(D)
+
+
+@pytest.mark.parametrize("spec", [None, "4"])
+@pytest.mark.parametrize("cap", [1024, 8192, 16384])
+def test_sorted_spmv_matches_rowblock(panels, spec, cap):
+    """Column-sorted layout: each block's products are gathered in column
+    order and summed per row from LDS by the same lanes, so every row of at
+    most 1024 nonzeros is bit-identical to the row-block layout's; longer
+    rows (chunk sums grouped differently) agree to rounding with the float64
+    host product.  Edge shapes (empty rows, a 9000-nonzero row, a dense
+    column), the power-law matrix (rows of A^T with tens of thousands of
+    nonzeros, millions of empty ones), the uniform matrix; FP32 operator."""
+    if spec:
+        panels(spec)
+    rng = np.random.default_rng(cap + 7)
+    for A in (_edge_matrix(rng), synth.powerlaw_bipartite(400_000, 200_000, seed=3),
+              synth.poisson_decay(200_000, 100_000, 16.0, 0.6, seed=4)):
+        AT = A.transpose()
+        op0 = api.CsrOperator(A)
+        op1 = api.CsrOperator(A, sort=(cap, cap))
+        assert op1.sort() == (cap, cap)
+        x, y = rng.standard_normal(A.n), rng.standard_normal(A.m)
+        for trans, v, B in (("n", x, A), ("t", y, AT)):
+            short = np.diff(B.rowptr) <= 1024
+            got, ref = op1.spmv(v, trans), op0.spmv(v, trans)
+            assert np.array_equal(got[short], ref[short])
+            want = B.matvec(v)
+            np.testing.assert_allclose(got, want, rtol=0, atol=1e-10 * np.abs(want).max())
+            assert np.array_equal(got, op1.spmv(v, trans))
+        op32 = api.CsrOperator(A.astype(np.float32), prec=api.PK_F32)
+        op32s = api.CsrOperator(A.astype(np.float32), prec=api.PK_F32, sort=(cap, cap))
+        x32 = x.astype(np.float32)
+        short = np.diff(A.rowptr) <= 1024
+        assert np.array_equal(op32s.spmv(x32, "n")[short], op32.spmv(x32, "n")[short])
+        # back to the row-block layout: the row-block results, bit for bit
+        op1.set_sort(0, 0)
+        assert op1.sort() == (0, 0)
+        assert np.array_equal(op1.spmv(y, "t"), op0.spmv(y, "t"))
+
+
+def test_sorted_layout_rejects_bad_caps():
+    A = synth.random_uniform(3000, 1000, 20000, seed=5)
+    op = api.CsrOperator(A)
+    for caps in ((100, 0), (0, 16385), (-1, 1024)):
+        with pytest.raises(api.PropackError):
+            op.set_sort(*caps)
+        assert op.sort() == (0, 0)
+
+
+def test_sorted_same_triplets(small_case):
+    """DLANSVD and DLANSVD_IRL through operators in the column-sorted layout
+    give the row-block layout's singular values to rounding and the dense
+    SVD's; the restarted solve on a 200k x 100k matrix keeps its operator
+    application count."""
+    A, u0, Dm, s = small_case
+    r0 = api.lansvd(api.CsrOperator(A), A.m, A.n, 10, 300, u0=u0, tol=1e-12)
+    r1 = api.lansvd(api.CsrOperator(A, sort=(16384, 8192)), A.m, A.n, 10, 300, u0=u0,
+                    tol=1e-12)
+    assert r0.k == r1.k == 10
+    np.testing.assert_allclose(r1.sigma, r0.sigma, rtol=1e-12)
+    np.testing.assert_allclose(r1.sigma, s[:10], rtol=1e-12)
+    B = synth.poisson_decay(200_000, 100_000, 16.0, 0.6, seed=4)
+    u1 = synth.start_vector(B.m, 1)
+    res = []
+    for sort in (None, (16384, 16384)):
+        op = api.CsrOperator(B, panels=(4, 4), sort=sort)
+        eng = api.Engine(op, kmax=60)
+        eng.put_u1(u1)
+        res.append(api.lansvd_irl_dev(eng, 20, 60, tol=1e-10))
+        eng.close()
+    a, b = res
+    assert a.k == b.k == 20 and a.info == b.info == 0
+    np.testing.assert_allclose(b.sigma, a.sigma, rtol=1e-12)
+    assert abs(a.counters["nopx"] - b.counters["nopx"]) <= 2
+    for i in range(5):
+        r = B.matvec(b.V[:, i]) - b.sigma[i] * b.U[:, i]
+        assert np.linalg.norm(r) / b.sigma[0] < 1e-11
