@@ -16,19 +16,25 @@ def _profile_lines():
     with open(bench.TRAFFIC_PROFILE) as f:
         for line in f:
             w = line.split()
-            if len(w) == 7 and w[0] == "traffic_bytes_per_application":
-                out[(w[1], tuple(int(x) for x in w[3:]))] = float(w[2])
+            if len(w) in (7, 9) and w[0] == "traffic_bytes_per_application":
+                layout = tuple(int(x) for x in w[3:]) + (0, 0)[: 9 - len(w)]
+                out[(w[1], layout)] = float(w[2])
     return out
 
 
 def test_traffic_lookup_matches_profile():
     lines = _profile_lines()
     assert lines, "the traffic summary holds machine-readable lines"
+    assert any(layout[4:] == (0, 0) for _, layout in lines)      # row-block layout
+    assert any(layout[4:] != (0, 0) for _, layout in lines)      # column-sorted layout
     for (kclass, layout), value in lines.items():
-        assert bench.measured_traffic(kclass, layout[:2], layout[2:]) == value
+        assert bench.measured_traffic(kclass, layout[:2], layout[2:4], layout[4:]) == value
+        if layout[4:] == (0, 0):
+            assert bench.measured_traffic(kclass, layout[:2], layout[2:4]) == value
         # more than algorithmic bytes (about 0.42 GB per application), less than 3x
         assert 4.0e8 < value < 1.3e9
     assert bench.measured_traffic("spmv_t", (3, 5), (0, 0)) is None
+    assert bench.measured_traffic("spmv_t", (2, 4), (-1, -1), (16384, 16384)) is None
     assert bench.measured_traffic("gemvt", (4, 4), (0, -1)) is None
 
 
