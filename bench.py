@@ -262,10 +262,12 @@ def workload_batch(args):
     import torch
     t0 = time.perf_counter()
     probs = []
-    lanes = tuple(int(x) for x in args.lanes.split(",")) if args.lanes else None
-    # column-sorted SpMV blocks of 8192 nonzeros for every problem: the
-    # fastest cap on this workload (measured: 417, 462, 532 and 435
-    # problems/s for the row-block layout and caps 4096, 8192, 16384)
+    # column-sorted SpMV blocks of 8192 nonzeros for every problem, a quarter
+    # of the built-in lanes per row: the fastest fixed setting on this
+    # workload (measured problems/s: row-block layout 417; caps 4096, 8192,
+    # 16384 at the built-in lanes 462, 532, 435; cap 8192 with lane shifts
+    # 1, 0, -1, -2: 474, 529, 546-552, 558)
+    lanes = tuple(int(x) for x in args.lanes.split(",")) if args.lanes else (-2, -2)
     sort = tuple(int(x) for x in args.sort.split(",")) if args.sort else (8192, 8192)
     for prm in synth.batch_params(256)[: args.count]:
         A = synth.batch_problem(prm, dtype=np.float32)
@@ -301,7 +303,7 @@ def workload_batch(args):
            "vs_baseline": None, "dtype": "f32 storage / f64 accumulation",
            "data": "synthetic (seeded generator, matrices resident in HBM)",
            "config": {"workload": f"{len(probs)} problems of batch_params(256)", "nnz": nnz,
-                      "spmv_sort_caps": list(sort)},
+                      "spmv_sort_caps": list(sort), "spmv_lane_shifts": list(lanes)},
            "sequential_problems_per_s": len(probs) / ts, "speedup_vs_sequential": ts / tb,
            "converged": sum(r.k for r in rb), "requested": 16 * len(probs),
            "bit_identical_to_sequential": same}
@@ -321,11 +323,12 @@ def main():
     ap.add_argument("--panels", default=None,
                     help="explicit SpMV column panels 'A,At' (no autotuning)")
     ap.add_argument("--lanes", default=None,
-                    help="lanes-per-row shifts 'A,At' with --panels, or of every problem "
-                         "of --workload batch (default 0,0)")
+                    help="lanes-per-row shifts 'A,At' with --panels (default 0,0), or of "
+                         "every problem of --workload batch (default -2,-2)")
     ap.add_argument("--sort", default=None,
-                    help="column-sorted SpMV layout block caps 'A,At' with --panels, or of "
-                         "every problem of --workload batch (0: row-block layout; default 0,0)")
+                    help="column-sorted SpMV layout block caps 'A,At' with --panels "
+                         "(default 0,0: row-block layout), or of every problem of --workload "
+                         "batch (default 8192,8192)")
     ap.add_argument("--workload", choices=["c2", "powerlaw", "batch", "spmv-probe"], default="c2",
                     help="c2 (default): the headline solve; powerlaw: 8M x 4M Zipf "
                          "bipartite, k=64, dim=160, tol=1e-8, FP32 basis + FP64 polish vs "

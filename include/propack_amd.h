@@ -76,7 +76,8 @@ int pk_csr_spmv(pk_csr_t op, char trans, const void* x, void* y, int prec, int w
  * takes the layout its operators have when it is created).
  * pk_csr_autotune: builds the operator with 1, 2, 4 and 8 panels per
  * direction (up to 16 when sort_a and sort_t are not NULL) and, when
- * lanes_a and lanes_t are not NULL, lane shifts -1, 0 and 1 on each, and,
+ * lanes_a and lanes_t are not NULL, lane shifts -1, 0 and 1 on each (-2
+ * too when sort_a and sort_t are not NULL), and,
  * when sort_a and sort_t are not NULL, each in the row-block layout and in
  * the sorted layout with caps 8192 and 16384; times each direction on the
  * device and returns the fastest panel counts (and lane shifts, block caps)

@@ -784,7 +784,9 @@ int pk_csr_panels(pk_csr_t h, int* panels_a, int* panels_t) {
 // SpMV autotuner: builds the operator with 1, 2, 4 and 8 column panels per
 // direction (up to 16 when sorted layouts are tried) and, when lanes_a /
 // lanes_t are given, tries lanes per row of half, equal to and twice the
-// built-in choice on each; when sort_a / sort_t are given, each of these in
+// built-in choice on each (and a quarter when sorted layouts are tried: the
+// sorted kernel favours fewer lanes per row); when sort_a / sort_t are
+// given, each of these in
 // the row-block layout and in the column-sorted layout with blocks of 8192
 // and 16384 nonzeros.  Times each direction's application and returns the
 // fastest setting of each direction and its time.  The choice changes how
@@ -815,7 +817,7 @@ int pk_csr_autotune(int m, int n, long long nnz, const int* rowptr, const int* c
       const int rca = csr_set_sort(o, o->da, cap), rct = csr_set_sort(o, o->dt, cap);
       if (rca == PK_ERR_NOTSUP || rct == PK_ERR_NOTSUP) continue;  // more slots than built
       rc = rca != PK_OK ? rca : rct;
-      for (int sh = lanes ? -1 : 0; sh <= (lanes ? 1 : 0) && rc == PK_OK; ++sh) {
+      for (int sh = lanes ? (sorts ? -2 : -1) : 0; sh <= (lanes ? 1 : 0) && rc == PK_OK; ++sh) {
         csr_set_lanes(o->da, sh);
         csr_set_lanes(o->dt, sh);
         double ta = 0, tt = 0;

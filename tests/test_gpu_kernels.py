@@ -263,10 +263,11 @@ def test_autotune_and_explicit_panels():
     t = api.autotune(A)
     (pa, pt), (sa, st), (ua, ut), (ca, ct) = t.panels, t.lanes, t.us, t.sort
     assert pa in (1, 2, 4, 8, 16) and pt in (1, 2, 4, 8, 16) and ua > 0 and ut > 0
-    assert sa in (-1, 0, 1) and st in (-1, 0, 1)
+    assert sa in (-2, -1, 0, 1) and st in (-2, -1, 0, 1)
     assert ca in (0, 8192, 16384) and ct in (0, 8192, 16384)
     t0 = api.autotune(A, sort=False)
     assert t0.sort == (0, 0) and t0.panels[0] in (1, 2, 4, 8) and t0.panels[1] in (1, 2, 4, 8)
+    assert t0.lanes[0] in (-1, 0, 1) and t0.lanes[1] in (-1, 0, 1)
     x = synth.start_vector(A.n, 2)
     y = synth.start_vector(A.m, 3)
     want_n, want_t = A.matvec(x), A.transpose().matvec(y)
