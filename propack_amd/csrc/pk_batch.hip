@@ -54,6 +54,7 @@ struct BPanel {
   const void* va;
   double* lsum;
   const unsigned short* spos;
+  int col0, pbits;  // packed sorted layout (CsrOp::Panel::scol0, spbits)
 };
 
 // Work item of a batched SpMV launch: descriptor of problem pid, local
@@ -130,7 +131,8 @@ __global__ __launch_bounds__(kSortThreads) void k_bspmv_sorted(
   const double c = tr ? P.ctl->beta : P.ctl->alpha;
   const int q = (tr ? P.qt0 : P.qa0) + w.loc;
   spmv_sorted_body<VT, T, G>(desc[blockIdx.x], pan.rp, pan.ci, static_cast<const VT*>(pan.va),
-                             pan.spos, x, y, yprev, c, part + q, part + Q + q, pan.lsum,
+                             pan.spos, pan.col0, pan.pbits, x, y, yprev, c, part + q,
+                             part + Q + q, pan.lsum,
                              p == 0 ? nullptr : yacc, last ? nullptr : yacc, last, prod_dyn,
                              lds);
 }
@@ -516,8 +518,8 @@ int batch_create(int nprob, const pk_lbd_t* handles, Batch** out) {
         const CsrOp::Panel& pan = D.pan[p];
         const CsrOp::Part& part = cap > 0 ? pan.spart : pan.part;
         hpan[(static_cast<size_t>(i) * 2 + tr) * kPanelsMax + p] =
-            cap > 0 ? BPanel{pan.rp, pan.sci, pan.sva, part.lsum, pan.spos}
-                    : BPanel{pan.rp, pan.ci, pan.va, part.lsum, nullptr};
+            cap > 0 ? BPanel{pan.rp, pan.sci, pan.sva, part.lsum, pan.spos, pan.scol0, pan.spbits}
+                    : BPanel{pan.rp, pan.ci, pan.va, part.lsum, nullptr, 0, 0};
         std::vector<int4> hd(part.ndesc), hl(part.nlong);
         if ((!hd.empty() && hipMemcpy(hd.data(), part.desc, sizeof(int4) * hd.size(),
                                       hipMemcpyDeviceToHost) != hipSuccess) ||

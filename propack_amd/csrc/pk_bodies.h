@@ -22,15 +22,19 @@ constexpr int kCT = 4;
 // ------------------------------------------------------ row-block SpMV ----
 // Chunk -(d.y+1) of a long row, nonzeros [d.z, d.w), summed by an NT-thread
 // workgroup (thread-strided, then the block reduction); the sum goes to
-// lsum, the fix-up kernel finishes the row.
+// lsum, the fix-up kernel finishes the row.  Column of entry p:
+// col0 + (ci[p] >> pbits) (packed sorted layout; col0 = pbits = 0: ci[p]).
 template <typename VT, typename T, int NT>
 __device__ __forceinline__ void spmv_chunk(const int4 d, const int* __restrict__ ci,
                                            const VT* __restrict__ va, const T* __restrict__ x,
                                            double* q_s2, double* q_sd,
-                                           double* __restrict__ lsum, int last, double* lds) {
+                                           double* __restrict__ lsum, int last, double* lds,
+                                           int col0 = 0, int pbits = 0) {
   double acc[1] = {0.0};
-  for (int p = d.z + static_cast<int>(threadIdx.x); p < d.w; p += NT)
-    acc[0] += static_cast<double>(va[p]) * static_cast<double>(x[ci[p]]);
+  for (int p = d.z + static_cast<int>(threadIdx.x); p < d.w; p += NT) {
+    const int col = col0 + static_cast<int>(static_cast<unsigned>(ci[p]) >> pbits);
+    acc[0] += static_cast<double>(va[p]) * static_cast<double>(x[col]);
+  }
   block_sum<1, NT>(acc, lds);
   if (threadIdx.x == 0) {
     lsum[-(d.y + 1)] = acc[0];
@@ -159,18 +163,21 @@ __device__ __forceinline__ void spmv_rb_body(const int4 d, const int* __restrict
 // each product is written to LDS at its row-major offset and the rows are
 // summed from there by spmv_rows -- for the same G every row sum is bit for
 // bit the row-block layout's.  Long-row chunks keep the sorted order too
-// (the chunk sum is a fixed-order block reduction either way).
+// (the chunk sum is a fixed-order block reduction either way).  Packed form
+// (spos null): sci[q] = (column - col0) << pbits | offset, one 32-bit word
+// per entry, used when the panel's width and the cap fit in 32 bits.
 template <typename VT, typename T, int G>
 __device__ __forceinline__ void spmv_sorted_body(
     const int4 d, const int* __restrict__ rp, const int* __restrict__ sci,
-    const VT* __restrict__ sva, const unsigned short* __restrict__ spos,
+    const VT* __restrict__ sva, const unsigned short* __restrict__ spos, int col0, int pbits,
     const T* __restrict__ x, T* __restrict__ y, const T* __restrict__ yprev, double c,
     double* q_s2, double* q_sd, double* __restrict__ lsum, const double* __restrict__ yacc_in,
     double* __restrict__ yacc_out, int last, double* prod, double* lds) {
   if (d.y < 0) {
-    spmv_chunk<VT, T, kSortThreads>(d, sci, sva, x, q_s2, q_sd, lsum, last, lds);
+    spmv_chunk<VT, T, kSortThreads>(d, sci, sva, x, q_s2, q_sd, lsum, last, lds, col0, pbits);
     return;
   }
+  const unsigned pmask = (1u << pbits) - 1u;
   const int r0 = d.x, r1 = d.y;
   const int p0 = rp[r0];
   const int nz = rp[r1] - p0;
@@ -181,9 +188,14 @@ __device__ __forceinline__ void spmv_sorted_body(
 #pragma unroll
     for (int u = 0; u < kUnroll; ++u) {
       const int t = base + u * kSortThreads + static_cast<int>(threadIdx.x);
-      idx[u] = t < nz ? __builtin_nontemporal_load(sci + p0 + t) : -1;
+      const unsigned wv =
+          t < nz ? static_cast<unsigned>(__builtin_nontemporal_load(sci + p0 + t)) : 0u;
+      idx[u] = t < nz ? col0 + static_cast<int>(wv >> pbits) : -1;
       v[u] = t < nz ? static_cast<double>(__builtin_nontemporal_load(sva + p0 + t)) : 0.0;
-      pos[u] = t < nz ? static_cast<int>(__builtin_nontemporal_load(spos + p0 + t)) : 0;
+      if (spos)
+        pos[u] = t < nz ? static_cast<int>(__builtin_nontemporal_load(spos + p0 + t)) : 0;
+      else
+        pos[u] = static_cast<int>(wv & pmask);
     }
 #pragma unroll
     for (int u = 0; u < kUnroll; ++u) xv[u] = idx[u] >= 0 ? static_cast<double>(x[idx[u]]) : 0.0;

@@ -279,16 +279,27 @@ bool finish_panel(CsrOp::Panel& P, int rows) {
 }
 
 // Column-sorted copy of a panel, one workgroup per segment [sb, se) (a
-// descriptor's nonzeros): perm holds the panel positions in sorted order.
+// descriptor's nonzeros): perm holds the panel positions in sorted order,
+// sci the sorted columns.  pbits > 0: sci[q] becomes the packed word
+// (column - col0) << pbits | offset (offset bits past pbits dropped: only
+// long-row chunks have such offsets, and their offsets are not used);
+// otherwise the offset goes to spos.
 template <typename VT>
 __global__ void k_sorted_finish(const int* __restrict__ sb, const int* __restrict__ se,
                                 const int* __restrict__ perm, const VT* __restrict__ va,
-                                VT* __restrict__ sva, unsigned short* __restrict__ spos) {
+                                VT* __restrict__ sva, int* __restrict__ sci,
+                                unsigned short* __restrict__ spos, int col0, int pbits) {
   const int b = sb[blockIdx.x], e = se[blockIdx.x];
   for (int q = b + static_cast<int>(threadIdx.x); q < e; q += static_cast<int>(blockDim.x)) {
     const int p = perm[q];
     sva[q] = va[p];
-    spos[q] = static_cast<unsigned short>(p - b);
+    if (pbits > 0) {
+      const unsigned off = static_cast<unsigned>(sci[q] - col0);
+      const unsigned pos = static_cast<unsigned>(p - b) & ((1u << pbits) - 1u);
+      sci[q] = static_cast<int>((off << pbits) | pos);
+    } else {
+      spos[q] = static_cast<unsigned short>(p - b);
+    }
   }
 }
 
@@ -302,6 +313,7 @@ void free_sorted(CsrOp::Panel& P) {
   P.sci = nullptr;
   P.sva = nullptr;
   P.spos = nullptr;
+  P.scol0 = P.spbits = 0;
   P.spart = CsrOp::Part{};
 }
 
@@ -309,9 +321,11 @@ void free_sorted(CsrOp::Panel& P) {
 // with blocks of at most cap nonzeros and kSortRows rows; inside each
 // descriptor's range a stable segmented radix sort of the positions keyed
 // by column (rocPRIM), so ties keep their row-major order.
+// The offsets of a block (< cap) and the columns of a panel relative to
+// its first column (< width) share one 32-bit word when their bits fit.
 template <typename VT>
-int build_sorted_panel(CsrOp::Panel& P, int rows, int cols, int cap, int max_slots,
-                       hipStream_t s) {
+int build_sorted_panel(CsrOp::Panel& P, int rows, int cols, int col0, int width, int cap,
+                       int max_slots, hipStream_t s) {
   std::vector<int> hrp(static_cast<size_t>(rows) + 1);
   if (hipMemcpy(hrp.data(), P.rp, sizeof(int) * hrp.size(), hipMemcpyDeviceToHost) !=
       hipSuccess)
@@ -336,9 +350,15 @@ int build_sorted_panel(CsrOp::Panel& P, int rows, int cols, int cap, int max_slo
     rc = PK_ERR_HIP;       \
     goto out;              \
   }
+  {
+    // unpacked: sci holds global columns (base 0, no shift)
+    const int pbits = bits_for(cap), wbits = bits_for(std::max(width, 2));
+    P.spbits = pbits + wbits <= 32 ? pbits : 0;
+    P.scol0 = P.spbits > 0 ? col0 : 0;
+  }
   PKT(hipMalloc(&P.sci, sizeof(int) * nz));
   PKT(hipMalloc(&P.sva, sizeof(VT) * nz));
-  PKT(hipMalloc(&P.spos, sizeof(unsigned short) * nz));
+  if (P.spbits == 0) PKT(hipMalloc(&P.spos, sizeof(unsigned short) * nz));
   PKT(hipMalloc(&dsb, sizeof(int) * sb.size()));
   PKT(hipMalloc(&dse, sizeof(int) * se.size()));
   PKT(hipMalloc(&perm_in, sizeof(int) * nz));
@@ -357,7 +377,8 @@ int build_sorted_panel(CsrOp::Panel& P, int rows, int cols, int cap, int max_slo
                                             static_cast<unsigned>(nseg), dsb, dse, 0,
                                             bits_for(cols), s));
     hipLaunchKernelGGL((k_sorted_finish<VT>), dim3(nseg), dim3(256), 0, s, dsb, dse, perm_out,
-                       static_cast<const VT*>(P.va), static_cast<VT*>(P.sva), P.spos);
+                       static_cast<const VT*>(P.va), static_cast<VT*>(P.sva), P.sci, P.spos,
+                       P.scol0, P.spbits);
   }
   PKT(hipGetLastError());
   PKT(hipStreamSynchronize(s));
@@ -383,10 +404,14 @@ int csr_set_sort(const CsrOp* op, CsrOp::Dir& D, int cap) {
   hipStream_t s = nullptr;
   if (hipStreamCreateWithFlags(&s, hipStreamNonBlocking) != hipSuccess) return PK_ERR_HIP;
   int rc = PK_OK;
-  for (int p = 0; p < D.np && rc == PK_OK; ++p)
+  for (int p = 0; p < D.np && rc == PK_OK; ++p) {
+    // panel p holds columns [p * width, min((p + 1) * width, cols))
+    const int col0 = D.np > 1 ? p * D.width : 0;
+    const int width = D.np > 1 ? std::min(D.width, D.cols - col0) : D.cols;
     rc = op->val_prec == PK_F64
-             ? build_sorted_panel<double>(D.pan[p], D.rows, D.cols, cap, D.slots, s)
-             : build_sorted_panel<float>(D.pan[p], D.rows, D.cols, cap, D.slots, s);
+             ? build_sorted_panel<double>(D.pan[p], D.rows, D.cols, col0, width, cap, D.slots, s)
+             : build_sorted_panel<float>(D.pan[p], D.rows, D.cols, col0, width, cap, D.slots, s);
+  }
   (void)hipStreamDestroy(s);
   if (rc != PK_OK) {
     for (int p = 0; p < D.np; ++p) free_sorted(D.pan[p]);
@@ -609,13 +634,13 @@ int csr_apply(const CsrOp* op, hipStream_t s, char trans, const T* x, T* y, cons
       if (op->val_prec == PK_F64)
         nb = launch_spmv_sorted<double, T>(s, Q.ndesc, Q.desc, Q.nlong, Q.ldesc, Q.lsum, P.rp,
                                            P.sci, static_cast<const double*>(P.sva), P.spos,
-                                           D.sort_cap, x, y, yprev, coef, part, P.g, yin, yout,
-                                           last, gate);
+                                           P.scol0, P.spbits, D.sort_cap, x, y, yprev, coef,
+                                           part, P.g, yin, yout, last, gate);
       else
         nb = launch_spmv_sorted<float, T>(s, Q.ndesc, Q.desc, Q.nlong, Q.ldesc, Q.lsum, P.rp,
                                           P.sci, static_cast<const float*>(P.sva), P.spos,
-                                          D.sort_cap, x, y, yprev, coef, part, P.g, yin, yout,
-                                          last, gate);
+                                          P.scol0, P.spbits, D.sort_cap, x, y, yprev, coef,
+                                          part, P.g, yin, yout, last, gate);
       if (nb < 0) return nb;
       continue;
     }

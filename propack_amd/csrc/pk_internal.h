@@ -40,10 +40,13 @@ struct CsrOp {
     int g = 8;  // lanes per row in the LDS row reduction
     Part part;
     // column-sorted layout (built on request): per block, the entries in
-    // increasing column order and their row-major offsets; own descriptors
+    // increasing column order and their row-major offsets (spos, or packed
+    // into sci when spbits > 0: column scol0 + (sci >> spbits)); own
+    // descriptors
     int* sci = nullptr;
     void* sva = nullptr;
     unsigned short* spos = nullptr;
+    int scol0 = 0, spbits = 0;
     Part spart;
   };
   struct Dir {

@@ -36,13 +36,15 @@ int launch_spmv_rb(hipStream_t s, int nblk, const int4* desc, int nlong, const i
 
 // The same over the column-sorted layout (pk_bodies.h): descriptors of
 // blocks of at most cap (<= kSortCapMax) nonzeros, sorted column indices
-// sci, values sva and row-major offsets spos; same outputs and partial slots.
+// sci, values sva and row-major offsets spos (or, spos null, sci packed:
+// column col0 + (sci >> pbits), offset sci & (2^pbits - 1)); same outputs
+// and partial slots.
 template <typename VT, typename T>
 int launch_spmv_sorted(hipStream_t s, int nblk, const int4* desc, int nlong, const int4* ldesc,
                        double* lsum, const int* rp, const int* sci, const VT* sva,
-                       const unsigned short* spos, int cap, const T* x, T* y, const T* yprev,
-                       Scal coef, double* part, int g, const double* yacc_in, double* yacc_out,
-                       int last, Gate gate);
+                       const unsigned short* spos, int col0, int pbits, int cap, const T* x,
+                       T* y, const T* yprev, Scal coef, double* part, int g,
+                       const double* yacc_in, double* yacc_out, int last, Gate gate);
 
 // part (nval x nblk) -> out[v] = sum_b part[v*nblk+b] in index order.
 int launch_reduce(hipStream_t s, int nval, int nblk, const double* part, double* out,

@@ -101,14 +101,15 @@ int launch_spmv_rb(hipStream_t s, int nblk, const int4* desc, int nlong, const i
 template <typename VT, typename T, int G>
 __global__ __launch_bounds__(kSortThreads) void k_spmv_sorted(
     const int4* __restrict__ desc, const int* __restrict__ rp, const int* __restrict__ sci,
-    const VT* __restrict__ sva, const unsigned short* __restrict__ spos,
+    const VT* __restrict__ sva, const unsigned short* __restrict__ spos, int col0, int pbits,
     const T* __restrict__ x, T* __restrict__ y, const T* __restrict__ yprev, Scal coef,
     double* __restrict__ part, int nbt, double* __restrict__ lsum,
     const double* __restrict__ yacc_in, double* __restrict__ yacc_out, int last, Gate gate) {
   if (gate.off()) return;
   extern __shared__ double prod_dyn[];
   __shared__ double lds[(kSortThreads / kWave) * 2];
-  spmv_sorted_body<VT, T, G>(desc[blockIdx.x], rp, sci, sva, spos, x, y, yprev, coef.get(),
+  spmv_sorted_body<VT, T, G>(desc[blockIdx.x], rp, sci, sva, spos, col0, pbits, x, y, yprev,
+                             coef.get(),
                              part + blockIdx.x, part + nbt + blockIdx.x, lsum, yacc_in,
                              yacc_out, last, prod_dyn, lds);
 }
@@ -116,18 +117,21 @@ __global__ __launch_bounds__(kSortThreads) void k_spmv_sorted(
 template <typename VT, typename T>
 int launch_spmv_sorted(hipStream_t s, int nblk, const int4* desc, int nlong, const int4* ldesc,
                        double* lsum, const int* rp, const int* sci, const VT* sva,
-                       const unsigned short* spos, int cap, const T* x, T* y, const T* yprev,
-                       Scal coef, double* part, int g, const double* yacc_in, double* yacc_out,
-                       int last, Gate gate) {
-  if (cap <= 0 || cap > kSortCapMax) return -1;
+                       const unsigned short* spos, int col0, int pbits, int cap, const T* x,
+                       T* y, const T* yprev, Scal coef, double* part, int g,
+                       const double* yacc_in, double* yacc_out, int last, Gate gate) {
+  // packed (spos null): 0 < pbits <= 16; unpacked: global columns, no base or shift
+  if (cap <= 0 || cap > kSortCapMax || pbits < 0 || pbits > 16 || (!spos && pbits == 0) ||
+      (spos && (pbits != 0 || col0 != 0)))
+    return -1;
   const int nbt = nblk + nlong;
   const size_t shm = sizeof(double) * static_cast<size_t>(cap);
   if (nblk > 0) {
 #define PK_SO_CASE(G)                                                                          \
   case G:                                                                                      \
     hipLaunchKernelGGL((k_spmv_sorted<VT, T, G>), dim3(nblk), dim3(kSortThreads), shm, s, desc, \
-                       rp, sci, sva, spos, x, y, yprev, coef, part, nbt, lsum, yacc_in,         \
-                       yacc_out, last, gate);                                                  \
+                       rp, sci, sva, spos, col0, pbits, x, y, yprev, coef, part, nbt, lsum,    \
+                       yacc_in, yacc_out, last, gate);                                         \
     break;
     switch (g) {
       PK_SO_CASE(2)
@@ -485,7 +489,8 @@ PK_INST_RB(float, double)
 #define PK_INST_SO(VT, T)                                                                    \
   template int launch_spmv_sorted<VT, T>(hipStream_t, int, const int4*, int, const int4*,    \
                                          double*, const int*, const int*, const VT*,         \
-                                         const unsigned short*, int, const T*, T*, const T*, \
+                                         const unsigned short*, int, int, int, const T*, T*, \
+                                         const T*,                                           \
                                          Scal, double*, int, const double*, double*, int,    \
                                          Gate);
 PK_INST_SO(double, double)

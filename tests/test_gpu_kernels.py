@@ -302,12 +302,16 @@ def test_sorted_spmv_matches_rowblock(panels, spec, cap):
     rows (chunk sums grouped differently) agree to rounding with the float64
     host product.  Edge shapes (empty rows, a 9000-nonzero row, a dense
     column), the power-law matrix (rows of A^T with tens of thousands of
-    nonzeros, millions of empty ones), the uniform matrix; FP32 operator."""
+    nonzeros, millions of empty ones), the uniform matrix, a matrix of
+    600,000 columns (one panel of it is too wide to pack column and offset
+    into one word at caps 8192 and 16384, so the separate offset array is
+    used; every other case packs); FP32 operator."""
     if spec:
         panels(spec)
     rng = np.random.default_rng(cap + 7)
     for A in (_edge_matrix(rng), synth.powerlaw_bipartite(400_000, 200_000, seed=3),
-              synth.poisson_decay(200_000, 100_000, 16.0, 0.6, seed=4)):
+              synth.poisson_decay(200_000, 100_000, 16.0, 0.6, seed=4),
+              synth.random_uniform(2000, 600_000, 200_000, seed=9)):
         AT = A.transpose()
         op0 = api.CsrOperator(A)
         op1 = api.CsrOperator(A, sort=(cap, cap))
@@ -368,3 +372,33 @@ def test_sorted_same_triplets(small_case):
     for i in range(5):
         r = B.matvec(b.V[:, i]) - b.sigma[i] * b.U[:, i]
         assert np.linalg.norm(r) / b.sigma[0] < 1e-11
+
+
+@pytest.mark.parametrize("m,n", [(1, 1), (1, 50), (50, 1), (5000, 3), (3, 5000)])
+@pytest.mark.parametrize("cap", [1024, 16384])
+def test_sorted_spmv_tiny(m, n, cap):
+    """Column-sorted layout on degenerate shapes (one row, one column, a
+    block holding every row, rows of 5000 nonzeros split into chunks):
+    the float64 host product, and bit for bit the row-block layout's on
+    rows of at most 1024 nonzeros."""
+    rng = np.random.default_rng(m * 31 + n + cap)
+    D = rng.standard_normal((m, n)) * (rng.random((m, n)) < 0.5)
+    if not D.any():
+        D[0, 0] = 1.0
+    A = _csr_from_dense(D)
+    op0, op1 = api.CsrOperator(A), api.CsrOperator(A, sort=(cap, cap))
+    x, y = rng.standard_normal(n), rng.standard_normal(m)
+    for trans, v, want, lens in (("n", x, D @ x, np.count_nonzero(D, axis=1)),
+                                 ("t", y, D.T @ y, np.count_nonzero(D, axis=0))):
+        got = op1.spmv(v, trans)
+        np.testing.assert_allclose(got, want, rtol=0, atol=1e-12 * max(1.0, np.abs(want).max()))
+        short = lens <= 1024
+        assert np.array_equal(got[short], op0.spmv(v, trans)[short])
+
+
+def test_sorted_spmv_empty_matrix():
+    """An operator without nonzeros in the sorted layout returns zeros."""
+    A = synth.from_coo(40, 30, np.zeros(0, np.int64), np.zeros(0, np.int64), np.zeros(0))
+    op = api.CsrOperator(A, sort=(8192, 8192))
+    assert op.sort() == (8192, 8192)
+    assert not op.spmv(np.ones(30), "n").any() and not op.spmv(np.ones(40), "t").any()
