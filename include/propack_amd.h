@@ -68,8 +68,9 @@ int pk_csr_spmv(pk_csr_t op, char trans, const void* x, void* y, int prec, int w
  * pk_csr_set_sort / pk_csr_sort: the column-sorted layout of a direction
  * (cap > 0: row blocks of at most cap nonzeros, 1024 <= cap <= 16384, each
  * block's entries stored in increasing column order so that the gathers of
- * nearby columns share cache lines; built on the device, about 14 bytes per
- * nonzero on top of the row-block layout) or the row-block layout (cap = 0).
+ * nearby columns share cache lines; built on the device, 12 bytes per
+ * nonzero on top of the row-block layout when a panel's width and the cap
+ * fit one 32-bit word, 14 otherwise) or the row-block layout (cap = 0).
  * Row sums are bit-identical to the row-block layout's for the same lanes;
  * only the per-block partial sums of the norms are grouped differently.
  * Not while a solve that uses the operator runs (a batch, pk_batch_create,
