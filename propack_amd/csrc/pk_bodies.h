@@ -513,12 +513,15 @@ __device__ __forceinline__ void ctl_stage_body(LbdCtl* __restrict__ gc, int kind
   int4* dst = reinterpret_cast<int4*>(&c);
   // Word indices past the end are clamped to the last word, so every load is
   // unconditional and all of them are in flight at once; the clamped lanes
-  // store that word's own value again.
-#pragma unroll
-  for (int u = 0; u < per; ++u) {
-    const int i = min(static_cast<int>(threadIdx.x) + u * kBlock, nv - 1);
-    dst[i] = src[i];
-  }
+  // store that word's own value again.  The words wait in registers while
+  // the partials are reduced, so the state and the partials arrive in one
+  // memory round trip; they reach LDS after the reduction.
+  // (six named words: an array here was placed in scratch memory)
+  static_assert(per == 6, "the staging holds six 16-byte words per thread");
+  const int t = static_cast<int>(threadIdx.x);
+  const int i0 = t, i1 = t + kBlock, i2 = t + 2 * kBlock, i3 = t + 3 * kBlock,
+            i4 = t + 4 * kBlock, i5 = min(t + 5 * kBlock, nv - 1);
+  const int4 w0 = src[i0], w1 = src[i1], w2 = src[i2], w3 = src[i3], w4 = src[i4], w5 = src[i5];
   double a, b = 0.0;
   if (pb && nb == na) {
     const double* const q[2] = {pa, pb};
@@ -530,7 +533,12 @@ __device__ __forceinline__ void ctl_stage_body(LbdCtl* __restrict__ gc, int kind
     a = reduce_partials(pa, na, lds);
     if (pb) b = reduce_partials(pb, nb, lds);
   }
-
+  dst[i0] = w0;
+  dst[i1] = w1;
+  dst[i2] = w2;
+  dst[i3] = w3;
+  dst[i4] = w4;
+  dst[i5] = w5;
   __syncthreads();
   if (threadIdx.x == 0) {
     switch (kind) {

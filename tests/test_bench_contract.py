@@ -36,6 +36,13 @@ def test_traffic_lookup_matches_profile():
     assert bench.measured_traffic("spmv_t", (3, 5), (0, 0)) is None
     assert bench.measured_traffic("spmv_t", (2, 4), (-1, -1), (16384, 16384)) is None
     assert bench.measured_traffic("gemvt", (4, 4), (0, -1)) is None
+    # a lane shift pair that was not measured: the same panels and caps answer
+    for (kclass, layout), value in lines.items():
+        other = bench.measured_traffic(kclass, layout[:2], (-2, -2), layout[4:])
+        assert other is not None
+        same = [v for (k, l), v in lines.items() if k == kclass and l[:2] == layout[:2]
+                and l[4:] == layout[4:]]
+        assert other in same and abs(other - value) <= 0.02 * value
 
 
 def test_multi_gpu_request_exits():
