@@ -402,3 +402,17 @@ def test_sorted_spmv_empty_matrix():
     op = api.CsrOperator(A, sort=(8192, 8192))
     assert op.sort() == (8192, 8192)
     assert not op.spmv(np.ones(30), "n").any() and not op.spmv(np.ones(40), "t").any()
+
+
+def test_sorted_slansvd_dropin(small_case):
+    """SLANSVD through the Fortran drop-in on an FP32 operator in the
+    column-sorted layout: the row-block operator's singular values to FP32
+    rounding, and the dense FP64 SVD's to FP32-storage accuracy."""
+    A, u0, Dm, s = small_case
+    r0 = api.lansvd(api.CsrOperator(A, prec=api.PK_F32), A.m, A.n, 10, 300, u0=u0, tol=1e-6,
+                    prec=api.PK_F32)
+    r1 = api.lansvd(api.CsrOperator(A, prec=api.PK_F32, sort=(8192, 1024)), A.m, A.n, 10, 300,
+                    u0=u0, tol=1e-6, prec=api.PK_F32)
+    assert r0.k == r1.k == 10 and r0.info == r1.info == 0
+    np.testing.assert_allclose(r1.sigma, r0.sigma, rtol=2e-6)
+    np.testing.assert_allclose(r1.sigma, s[:10], rtol=2e-6)
