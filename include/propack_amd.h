@@ -73,8 +73,10 @@ int pk_csr_spmv(pk_csr_t op, char trans, const void* x, void* y, int prec, int w
  * fit one 32-bit word, 14 otherwise) or the row-block layout (cap = 0).
  * Row sums are bit-identical to the row-block layout's for the same lanes;
  * only the per-block partial sums of the norms are grouped differently.
- * Not while a solve that uses the operator runs (a batch, pk_batch_create,
- * takes the layout its operators have when it is created).
+ * Not while a solve that uses the operator runs.  A batch (pk_batch_create)
+ * takes the layout its operators have when it is created; once an
+ * operator's lanes or sorted layout change, the batch's calls fail with
+ * PK_ERR_ARG until a new batch is created.
  * pk_csr_autotune: builds the operator with 1, 2, 4 and 8 panels per
  * direction (up to 16 when sort_a and sort_t are not NULL) and, when
  * lanes_a and lanes_t are not NULL, lane shifts -1, 0 and 1 on each (-2

@@ -102,7 +102,7 @@ class CsrOperator:
                    "pk_csr_create_panels")
         self.handle = h
         if lanes is not None:
-            _check(L.pk_csr_set_lanes(h, C.c_int(lanes[0]), C.c_int(lanes[1])), "pk_csr_set_lanes")
+            self.set_lanes(*lanes)
         if sort is not None:
             self.set_sort(*sort)
         self.iparm = np.zeros(4, np.int32)
@@ -127,6 +127,12 @@ class CsrOperator:
         pa, pt = C.c_int(), C.c_int()
         _check(lib().pk_csr_panels(self.handle, C.byref(pa), C.byref(pt)), "pk_csr_panels")
         return pa.value, pt.value
+
+    def set_lanes(self, shift_a: int, shift_t: int):
+        """Lanes per row of (A, A^T): the built-in choice times 2^shift,
+        pk_csr_set_lanes."""
+        _check(lib().pk_csr_set_lanes(self.handle, C.c_int(shift_a), C.c_int(shift_t)),
+               "pk_csr_set_lanes")
 
     def lanes(self):
         sa, st = C.c_int(), C.c_int()

@@ -263,6 +263,7 @@ struct Batch {
   unsigned magic = 0;
   int nprob = 0, prec = PK_F64, val_prec = PK_F64;
   std::vector<Engine*> eng;
+  std::vector<unsigned> epoch;  // each operator's layout_epoch at creation
   hipStream_t stream = nullptr;
   // static tables (operators do not change)
   int Sm = 0, Sn = 0, Qa = 0, Qt = 0, maxk = 0;
@@ -486,6 +487,7 @@ int batch_create(int nprob, const pk_lbd_t* handles, Batch** out) {
       break;
     }
     B->eng.push_back(e);
+    B->epoch.push_back(e->op->layout_epoch);
     BProb& P = B->hprob[i];
     std::memset(&P, 0, sizeof(BProb));
     P.m = e->m;
@@ -603,6 +605,15 @@ int batch_lanbpro(Batch& B, const int* active, const int* k0, int* k, double* co
                   double* const* beta, double* rnorm, double* doption, const int* ioption,
                   int* ierr) {
   const int np = B.nprob;
+  // the batch's descriptor tables and panel pointers follow each operator's
+  // layout at creation; a layout changed since then would be read stale
+  for (int i = 0; i < np; ++i)
+    if (B.eng[i]->op->layout_epoch != B.epoch[i]) {
+      set_error("pk_batch_lanbpro: the SpMV layout of problem %d's operator changed after "
+                "pk_batch_create; create the batch again",
+                i);
+      return PK_ERR_ARG;
+    }
   std::vector<char> run(np, 0), fin(np, 0);
   std::vector<int> rcs(np, PK_OK);
   int Jmin = 1 << 30, Jmax = 0, elr = 0, tries = 1;

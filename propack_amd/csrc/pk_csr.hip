@@ -396,7 +396,8 @@ out:
 
 }  // namespace
 
-int csr_set_sort(const CsrOp* op, CsrOp::Dir& D, int cap) {
+int csr_set_sort(CsrOp* op, CsrOp::Dir& D, int cap) {
+  ++op->layout_epoch;
   if (cap != 0 && (cap < kSpmvCap || cap > kSortCapMax)) return PK_ERR_ARG;
   for (int p = 0; p < D.np; ++p) free_sorted(D.pan[p]);
   D.sort_cap = 0;
@@ -926,6 +927,7 @@ int pk_csr_set_lanes(pk_csr_t h, int shift_a, int shift_t) {
   }
   csr_set_lanes(op->da, shift_a);
   csr_set_lanes(op->dt, shift_t);
+  ++op->layout_epoch;
   return PK_OK;
 }
 

@@ -59,6 +59,9 @@ struct CsrOp {
     int sort_cap = 0;        // > 0: column-sorted layout, blocks of <= sort_cap nonzeros
   } da, dt;
   int nrb = 0, nrbt = 0;  // partial slots per direction
+  // bumped by every layout change (lanes per row, sorted layout): a batch
+  // built on the earlier layout refuses to run
+  unsigned layout_epoch = 0;
 };
 
 // Runs f(i) for i in [0, n) on up to host_threads() threads (the calling
@@ -80,7 +83,7 @@ void csr_set_lanes(CsrOp::Dir& D, int shift);
 // Switches direction D of op to the column-sorted layout with blocks of at
 // most cap nonzeros (kSpmvCap <= cap <= kSortCapMax; built on the device),
 // or back to the row-block layout (cap = 0, the sorted copy freed).
-int csr_set_sort(const CsrOp* op, CsrOp::Dir& D, int cap);
+int csr_set_sort(CsrOp* op, CsrOp::Dir& D, int cap);
 template <typename T>
 int csr_apply(const CsrOp* op, hipStream_t s, char trans, const T* x, T* y, const T* yprev,
               Scal coef, double* part, Gate gate);

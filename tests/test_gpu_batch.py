@@ -136,6 +136,27 @@ def test_batch_sorted_layout_matches_sequential():
             np.testing.assert_allclose(r.sigma, r0.sigma, rtol=1e-7 if prec == api.PK_F32 else 1e-10)
 
 
+def test_batch_refuses_changed_layout():
+    """A batch holds its operators' SpMV tables as they were at creation: a
+    layout changed afterwards (sorted blocks freed, lanes regrouped) makes
+    the batched call fail cleanly instead of reading freed arrays; a new
+    batch on the new layout runs."""
+    probs = _problems(2, 2_000, 20_000, seed=41)
+    probs[0][2].set_sort(8192, 8192)
+    for prm, A, op, eng, u0 in probs:
+        eng.put_u1(u0)
+    b = api.Batch([p[3] for p in probs])
+    probs[0][2].set_sort(0, 0)
+    with pytest.raises(api.PropackError, match="changed after"):
+        api.lansvd_irl_batch(b, NEIG, DIM, tol=TOL)
+    b.close()
+    probs[1][2].set_lanes(-1, -1)
+    bat = _batched(probs)
+    seq = _sequential(probs)
+    for (r, nopx), rb in zip(seq, bat):
+        assert r.k == rb.k == NEIG and np.array_equal(r.sigma, rb.sigma)
+
+
 def test_batch_host_threads_same_results(monkeypatch):
     """The per-problem host work on one thread or on several gives the same
     bits (each problem touches only its own engine)."""
