@@ -20,9 +20,10 @@ from . import api, synth
 _which = {"LM": "L", "SM": "S"}
 
 
-def _as_operator(A, prec):
-    """(aprod, m, n, keepalive): a CsrOperator for explicit matrices, a
-    Python callable for LinearOperators."""
+def _as_operator(A, prec, tune=False):
+    """(aprod, m, n, keepalive): a CsrOperator for explicit matrices (with
+    ``tune``, in the SpMV layout pk_csr_autotune measures fastest), a Python
+    callable for LinearOperators."""
     try:
         import scipy.sparse as sp
         from scipy.sparse.linalg import LinearOperator
@@ -47,17 +48,25 @@ def _as_operator(A, prec):
             raise ValueError("A must be two-dimensional")
         r, c = np.nonzero(D)
         csr = synth.from_coo(D.shape[0], D.shape[1], r, c, D[r, c].astype(dt))
-    op = api.CsrOperator(csr, prec=prec)
+    if tune:
+        t = api.autotune(csr, prec=prec)
+        op = api.CsrOperator(csr, prec=prec, panels=t.panels, lanes=t.lanes, sort=t.sort)
+    else:
+        op = api.CsrOperator(csr, prec=prec)
     return op, csr.m, csr.n, op
 
 
 def svdp(A, k, which="LM", irl_mode=True, kmax=None, compute_u=True, compute_v=True,
          v0=None, full_output=False, tol=0, delta=None, eta=None, anorm=0, cgs=False,
-         elr=True, min_relgap=0.002, shifts=None, maxiter=None, rng=None):
+         elr=True, min_relgap=0.002, shifts=None, maxiter=None, rng=None, *, tune=False):
     """Truncated SVD of A through the library's DLANSVD(_IRL) drop-ins.
 
     Returns ``(u, sigma, vt)`` or ``(u, sigma, vt, bnd)`` with
-    ``full_output`` -- the layout of scipy's ``_svdp``.
+    ``full_output`` -- the layout of scipy's ``_svdp``.  ``tune`` (not in
+    scipy's signature): time the SpMV layouts of an explicit matrix on the
+    device first and solve with the fastest (about a second for 32M
+    nonzeros; the singular values agree with the untuned solve to
+    rounding).
     """
     which = which.upper()
     if which not in _which:
@@ -68,7 +77,7 @@ def svdp(A, k, which="LM", irl_mode=True, kmax=None, compute_u=True, compute_v=T
     if np.issubdtype(np.dtype(dtype), np.complexfloating):
         raise NotImplementedError("complex matrices are not supported")
     prec = api.PK_F32 if np.dtype(dtype) == np.float32 else api.PK_F64
-    aprod, m, n, keep = _as_operator(A, prec)
+    aprod, m, n, keep = _as_operator(A, prec, tune)
     if k < 1 or k > min(m, n):
         raise ValueError("k must be positive and not greater than m or n")
     if kmax is None:

@@ -61,6 +61,22 @@ def test_linear_operator_callback():
     assert np.all(bnd >= 0)
 
 
+def test_tuned_layout_same_triplets():
+    """tune=True solves on the SpMV layout the autotuner picks (on this
+    matrix a column-panel and column-sorted layout is among the candidates):
+    the untuned solve's singular values to rounding, and triplets with small
+    residuals."""
+    from propack_amd import synth
+    A = synth.poisson_decay(200_000, 100_000, 16.0, 0.6, seed=4)
+    S = csr_array((A.val, A.col, A.rowptr), shape=(A.m, A.n))
+    v0 = np.random.default_rng(3).standard_normal(A.m)
+    u0, s0, vt0 = svdp(S, k=8, kmax=40, tol=1e-10, v0=v0)
+    u1, s1, vt1 = svdp(S, k=8, kmax=40, tol=1e-10, v0=v0, tune=True)
+    assert_allclose(s1, s0, rtol=1e-12)
+    R = S @ vt1.T - u1 * s1
+    assert np.linalg.norm(R, axis=0).max() <= 1e-8 * s1[0]
+
+
 def test_argument_errors():
     M = generate_matrix(10, 20, 0.8, np.float64)
     with pytest.raises(ValueError):
