@@ -137,10 +137,72 @@ def poisson_decay(m: int = 2_000_000, n: int = 1_000_000, lam: float = 16.0,
     return Csr(m, n, rp.astype(np.int32), cols, vals.astype(dtype))
 
 
+def _zipf_distinct(g: np.random.Generator, deg: np.ndarray, rcdf: np.ndarray) -> np.ndarray:
+    """0-based Zipf ranks, deg[i] distinct ones for row i, in row order.
+
+    Successive sampling: i.i.d. draws from the law whose CDF is rcdf, a draw
+    kept when its rank is new to its row, until the row holds deg[i] ranks.
+    The draws of a round are taken together; a later copy of a rank inside
+    a row is redrawn in the next round.  A redraw is conditioned on a rank
+    above the row's leading run of taken ranks 0..K-1: a draw inside the
+    run would be rejected anyway, so the law of the set is unchanged, and
+    rows of 10^5 entries (whose taken ranks cover a long prefix) finish in
+    a few dozen rounds instead of ~10^9 draws."""
+    n = rcdf.size
+    m = deg.size
+    nnz = int(deg.sum())
+    rp = np.zeros(m + 1, dtype=np.int64)
+    np.cumsum(deg, out=rp[1:])
+    ranks = np.minimum(np.searchsorted(rcdf, g.random(nnz), side="right"), n - 1)
+    # entries of the rows still holding duplicates, as (flat index, row)
+    idx = np.arange(nnz, dtype=np.int64)
+    rows = np.repeat(np.arange(m, dtype=np.int64), deg)
+    multi = deg[rows] > 1
+    idx, rows = idx[multi], rows[multi]
+    for _ in range(100_000):
+        if idx.size == 0:
+            return ranks
+        key = rows * n + ranks[idx]
+        order = np.argsort(key, kind="stable")
+        ks = key[order]
+        dup = np.zeros(ks.size, dtype=bool)
+        dup[1:] = ks[1:] == ks[:-1]
+        if not dup.any():
+            return ranks
+        drows = np.unique(rows[order[dup]])
+        # keep only the rows with duplicates (sorted order is by row)
+        keep = np.isin(rows[order], drows)
+        order, ks, dup = order[keep], ks[keep], dup[keep]
+        srows = rows[order]
+        sr = ks - srows * n                     # ranks, sorted within each row
+        # leading run K of each row: distinct ranks 0..K-1 at its first K
+        # distinct positions
+        first = np.ones(srows.size, dtype=bool)
+        first[1:] = srows[1:] != srows[:-1]
+        start = np.flatnonzero(first)
+        uniq = ~dup
+        cum = np.cumsum(uniq) - 1               # distinct position so far
+        pos = cum - cum[start][np.cumsum(first) - 1]
+        inrun = uniq & (sr == pos)
+        brk = np.where(uniq & ~inrun, pos, np.iinfo(np.int64).max)
+        kk = np.minimum.reduceat(brk, start)
+        ndist = np.add.reduceat(uniq.astype(np.int64), start)
+        K = np.minimum(kk, ndist)               # per row of drows
+        ridx = np.searchsorted(drows, srows[dup])
+        lo = np.where(K[ridx] > 0, rcdf[np.maximum(K[ridx] - 1, 0)], 0.0)
+        u = lo + (1.0 - lo) * g.random(lo.size)
+        ranks[idx[order[dup]]] = np.minimum(np.searchsorted(rcdf, u, side="right"), n - 1)
+        idx, rows = idx[order], srows
+    raise RuntimeError("could not make columns distinct")
+
+
 def powerlaw_bipartite(m: int = 8_000_000, n: int = 4_000_000,
                        gamma: float = 2.1, dmax: int = 100_000,
                        zipf_s: float = 1.8, seed: int = 8000,
                        dtype=np.float32) -> Csr:
+    """Row degrees d ~ d^-gamma on [1, dmax]; each row holds d distinct
+    columns drawn from a Zipf(zipf_s) law over a seeded permutation of the
+    column ids (successive sampling, _zipf_distinct); every value is 1.0."""
     g = _rng(seed)
     # Row degrees by inverse-CDF sampling of the discrete law d^-gamma.
     d = np.arange(1, dmax + 1, dtype=np.float64)
@@ -152,13 +214,17 @@ def powerlaw_bipartite(m: int = 8_000_000, n: int = 4_000_000,
     perm = g.permutation(n).astype(np.int64)
     rcdf = np.cumsum(np.arange(1, n + 1, dtype=np.float64) ** -zipf_s)
     rcdf /= rcdf[-1]
-    nnz = int(deg.sum())
+    ranks = _zipf_distinct(g, deg, rcdf)
+    rp = np.zeros(m + 1, dtype=np.int64)
+    np.cumsum(deg, out=rp[1:])
+    if rp[-1] >= 2**31:
+        raise ValueError("nnz exceeds int32 row pointers")
     rows = np.repeat(np.arange(m, dtype=np.int64), deg)
-    ranks = np.minimum(np.searchsorted(rcdf, g.random(nnz), side="right"), n - 1)
     cols = perm[ranks]
-    vals = np.ones(nnz, dtype=np.float64)
-    # Duplicate (row, col) pairs merge into one entry (value = multiplicity).
-    return from_coo(m, n, rows, cols, vals).astype(dtype)
+    key = rows * n + cols
+    order = np.argsort(key, kind="stable")
+    return Csr(m, n, rp.astype(np.int32), cols[order].astype(np.int32),
+               np.ones(int(rp[-1]), dtype=dtype))
 
 
 def batch_params(count: int = 256, m_lo: int = 5_000, m_hi: int = 500_000,

@@ -40,3 +40,54 @@ def test_poisson_decay_small():
 def test_start_vector_seeded():
     assert np.array_equal(synth.start_vector(10, 1), synth.start_vector(10, 1))
     assert not np.array_equal(synth.start_vector(10, 1), synth.start_vector(10, 2))
+
+
+def _naive_distinct(g, deg, rcdf):
+    """Successive sampling by plain rejection, one row at a time."""
+    out = []
+    for d in deg:
+        taken = []
+        while len(taken) < d:
+            r = int(min(np.searchsorted(rcdf, g.random(), side="right"), rcdf.size - 1))
+            if r not in taken:
+                taken.append(r)
+        out.append(sorted(taken))
+    return out
+
+
+def test_zipf_distinct_law_matches_rejection():
+    # inclusion frequency of every rank, generator vs plain rejection, on
+    # rows whose degree forces long runs of taken ranks (the conditioned
+    # redraws); a wrong run length K biases the low ranks far beyond noise
+    n, rows = 24, 6000
+    rcdf = np.cumsum(np.arange(1, n + 1, dtype=np.float64) ** -1.8)
+    rcdf /= rcdf[-1]
+    deg = np.fromiter(([1, 3, 8, 15][i % 4] for i in range(rows)), dtype=np.int64)
+    r = synth._zipf_distinct(synth._rng(5), deg, rcdf)
+    rp = np.concatenate([[0], np.cumsum(deg)])
+    for d in (3, 8, 15):
+        sel = [i for i in range(rows) if deg[i] == d]
+        fast = np.zeros(n)
+        for i in sel:
+            seg = r[rp[i]:rp[i + 1]]
+            assert len(set(seg.tolist())) == d
+            fast[seg] += 1
+        slow = np.zeros(n)
+        for s in _naive_distinct(synth._rng(6), deg[sel], rcdf):
+            slow[s] += 1
+        p = (fast + slow) / (2 * len(sel))
+        sd = np.sqrt(np.maximum(p * (1 - p), 1e-12) * 2 / len(sel))
+        z = np.abs(fast - slow) / len(sel) / sd
+        assert z.max() < 5.0, (d, z.max())
+
+
+def test_powerlaw_distinct_unit_values():
+    A = synth.powerlaw_bipartite(60000, 30000, seed=11)
+    deg = np.diff(A.rowptr)
+    assert np.all(A.val == 1.0)
+    for i in np.argsort(deg)[-20:]:
+        c = A.col[A.rowptr[i]:A.rowptr[i + 1]]
+        assert np.all(np.diff(c) > 0)          # distinct, sorted
+    # degree law d^-2.1 on [1, 1e5] (clipped at n): mean about 4.6
+    assert 3.5 < deg.mean() < 6.0
+    assert A.nnz == int(deg.sum())
