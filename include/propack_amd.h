@@ -42,7 +42,9 @@ enum { PK_HOST = 0, PK_DEVICE = 1 }; /* where caller arrays live */
 typedef struct pk_csr_opaque* pk_csr_t;
 typedef struct pk_lbd_opaque* pk_lbd_t;
 
-/* Last error message of the calling process (static storage). */
+/* Last error message raised in the process, by any thread (the batched
+ * drivers raise theirs on host worker threads); the returned copy stays
+ * valid until the calling thread's next call. */
 const char* pk_last_error(void);
 
 /* ---------------------------------------------------------------------- */
@@ -55,7 +57,11 @@ int pk_csr_create(int m, int n, long long nnz, const int* rowptr, const int* col
                   const void* val, int val_prec, int where, pk_csr_t* op);
 int pk_csr_destroy(pk_csr_t op);
 int pk_csr_dims(pk_csr_t op, int* m, int* n, long long* nnz);
-/* y = op(A) x ; x, y in precision prec, on the host or the device. */
+/* y = op(A) x ; x, y in precision prec, on the host or the device.  Direct
+ * applications (this call and pk_dcsr_aprod_/pk_scsr_aprod_) use the
+ * operator's own scratch for multi-panel and long-row sums, so they must not
+ * overlap each other on one operator; engines (pk_lbd_create) carry their
+ * own scratch and may share an operator. */
 int pk_csr_spmv(pk_csr_t op, char trans, const void* x, void* y, int prec, int where);
 /* Layout of the SpMV (no reference counterpart).
  * pk_csr_create_panels: pk_csr_create with the column panels of A and A^T
@@ -187,6 +193,15 @@ int pk_lbd_update(pk_lbd_t eng, char which, char transb, int kdim, int ncols, co
  * reorthogonalization, [4] host synchronizations, [5] CGS passes beyond the
  * first of a reorthogonalization. */
 int pk_lbd_stats(pk_lbd_t eng, long long* out);
+/* How often the rare paths of the reference's DLANBPRO ran (no reference
+ * counterpart; the reference only prints warnings): out holds 8 values,
+ * out[0..2] = alpha below anorm*epsn (new random direction,
+ * double/dlanbpro.F:372-409), |alpha| below the safe minimum (DSAFESCAL's
+ * stepwise scaling, double/dsafescal.F:49-53), five failed CGS passes
+ * (v_j = 0, double/dreorth.F:96-98); out[3..5] the same for beta
+ * (:503-539); out[6] near-invariance warnings (IERR = j > 0, :406-409,
+ * :536-539); out[7] = 0.  Reset with pk_lbd_reset_stats. */
+int pk_lbd_path_stats(pk_lbd_t eng, long long* out);
 int pk_lbd_reset_stats(pk_lbd_t eng);
 int pk_lbd_sync(pk_lbd_t eng);
 int pk_lbd_shape(pk_lbd_t eng, int* m, int* n, int* kmax, int* prec);

@@ -40,7 +40,7 @@ EXPORTS = [
     "pk_batch_destroy", "pk_batch_lanbpro", "pk_lansvd_irl_batch", "pk_matrix_read",
     "pk_matrix_free", "pk_coo_to_csr", "pk_csr_create_coo", "pk_csr_read",
     "pk_csr_create_panels", "pk_csr_panels", "pk_csr_set_lanes", "pk_csr_lanes", "pk_csr_autotune",
-    "pk_csr_set_sort", "pk_csr_sort",
+    "pk_csr_set_sort", "pk_csr_sort", "pk_lbd_path_stats",
 ]
 
 _lib = None
@@ -264,6 +264,8 @@ def lansvd(aprod, m, n, k, kmax, u0=None, tol=1e-12, jobu="y", jobv="y", prec=PK
       ii(lwork), _p(iwork), ii(8 * kmax), _p(dopt), _p(iopt), C.byref(info), _p(dparm),
       _p(iparm), C.c_size_t(1), C.c_size_t(1))
     del keep
+    if info.value <= -100:
+        _check(info.value, "dlansvd" if prec == PK_F64 else "slansvd")
     kout = kk.value
     return SvdResult(sigma[:kout].copy(), bnd[:kout].copy(),
                      U[:kout].T.copy() if jobu == "y" else None,
@@ -304,6 +306,8 @@ def lansvd_irl(aprod, m, n, neig, dim, p=None, u0=None, tol=1e-10, which="L", jo
       _p(iopt), C.byref(info), _p(dparm), _p(iparm), C.c_size_t(1), C.c_size_t(1),
       C.c_size_t(1))
     del keep
+    if info.value <= -100:
+        _check(info.value, "dlansvd_irl" if prec == PK_F64 else "slansvd_irl")
     k = ne.value
     return SvdResult(sigma[:k].copy(), bnd[:k].copy(),
                      U[:k].T.copy() if jobu == "y" else None,
@@ -349,6 +353,13 @@ class Engine:
         out = np.zeros(6, np.int64)
         _check(lib().pk_lbd_stats(self.handle, _p(out)), "pk_lbd_stats")
         return dict(zip(["nopx", "nreorth", "ndot", "ncols", "nsync", "nretry"], out.tolist()))
+
+    def path_stats(self) -> dict:
+        """How often the reference's rare DLANBPRO paths ran on this engine."""
+        out = np.zeros(8, np.int64)
+        _check(lib().pk_lbd_path_stats(self.handle, _p(out)), "pk_lbd_path_stats")
+        return dict(zip(["v_invariant", "v_tiny", "v_zero", "u_invariant", "u_tiny", "u_zero",
+                         "nwarn"], out[:7].tolist()))
 
     def set_mode(self, mode: str = "device", tries: int = 2):
         """'device': device-resident step chains; 'host': host executor."""

@@ -31,6 +31,8 @@ constexpr int kPanelsMax = CsrOp::kMaxPanels;
 // Per-problem view, rebuilt for every batched call (restarts swap bases).
 struct BProb {
   LbdCtl* ctl;
+  char* arr;     // its arrays (CtlArrays layout, capacity cap)
+  int cap;
   void* U;
   void* V;
   long long ldu, ldv;
@@ -168,7 +170,8 @@ __global__ __launch_bounds__(kBlock) void k_bctl(const BProb* __restrict__ probs
                                                  int phase, int J, const double* __restrict__ part,
                                                  int Qa, int Qt, int Sm, int Sn) {
   const BProb P = probs[blockIdx.x];
-  __shared__ LbdCtl c;
+  __shared__ LbdCtlLds c;
+  __shared__ LbdCtlRef cg;
   __shared__ double lds[(kBlock / kWave) * 2];
   if (P.ctl->phase != phase || P.ctl->j != J) return;
   const double* pa;
@@ -182,7 +185,7 @@ __global__ __launch_bounds__(kBlock) void k_bctl(const BProb* __restrict__ probs
     case CTL_V_REO: pa = part + P.cn0; na = P.ncn; break;
     default: pa = part + P.cm0; na = P.ncm; break;  // CTL_U_REO
   }
-  ctl_stage_body(P.ctl, kind, pa, na, pb, na, c, lds);
+  ctl_stage_body(P.ctl, P.arr, P.cap, J, kind, pa, na, pb, na, c, cg, lds);
 }
 
 // ELR update: v side w = v_J, v = v_{J-1}; u side w = u_{J+1}, v = u_J.
@@ -211,7 +214,7 @@ __global__ __launch_bounds__(kBlock) void k_bgemvt(const BProb* __restrict__ pro
   if (Gate{P.ctl, phase, J, 0}.off()) return;
   __shared__ double lds[2][(kBlock / kWave) * kCT];
   const int ncols = P.ctl->reo_ncols;
-  const int* scol = P.ctl->cols;
+  const int* scol = CtlArrays::cols(P.arr, P.cap);
   const long long b = static_cast<long long>(blockIdx.x) - (vside ? P.cn0 : P.cm0);
   const T* basis = static_cast<const T*>(vside ? P.V : P.U);
   const T* w = vside ? bcol<T>(P, true, J) : bcol<T>(P, false, J + 1);
@@ -219,12 +222,12 @@ __global__ __launch_bounds__(kBlock) void k_bgemvt(const BProb* __restrict__ pro
                 blockIdx.x, lds);
 }
 
-// red[pid * kMaxCols + c] = sum over the problem's slices of the projection
+// red[pid * rs + c] = sum over the problem's slices of the projection
 // partials of column c (grid: problems x columns).
 __global__ __launch_bounds__(kBlock) void k_breduce(const BProb* __restrict__ probs, int vside,
                                                     int J, int phase,
                                                     const double* __restrict__ part, int S,
-                                                    double* __restrict__ red) {
+                                                    double* __restrict__ red, int rs) {
   const BProb P = probs[blockIdx.x];
   if (Gate{P.ctl, phase, J, 0}.off()) return;
   const int c = blockIdx.y;
@@ -232,22 +235,22 @@ __global__ __launch_bounds__(kBlock) void k_breduce(const BProb* __restrict__ pr
   __shared__ double lds[kBlock / kWave];
   const int s0 = vside ? P.cn0 : P.cm0, ns = vside ? P.ncn : P.ncm;
   const double v = reduce_partials(part + static_cast<long long>(c) * S + s0, ns, lds);
-  if (threadIdx.x == 0) red[static_cast<long long>(blockIdx.x) * kMaxCols + c] = v;
+  if (threadIdx.x == 0) red[static_cast<long long>(blockIdx.x) * rs + c] = v;
 }
 
 template <typename T>
 __global__ __launch_bounds__(kBlock) void k_bgemv(const BProb* __restrict__ probs,
                                                   const int* __restrict__ slice_pid, int vside,
                                                   int J, int phase,
-                                                  const double* __restrict__ red,
+                                                  const double* __restrict__ red, int rs,
                                                   double* __restrict__ part) {
   const int pid = slice_pid[blockIdx.x];
   const BProb P = probs[pid];
   if (Gate{P.ctl, phase, J, 0}.off()) return;
   __shared__ double lds[kBlock / kWave];
   const int ncols = P.ctl->reo_ncols;
-  const int* scol = P.ctl->cols;
-  const double* sh = red + static_cast<long long>(pid) * kMaxCols;
+  const int* scol = CtlArrays::cols(P.arr, P.cap);
+  const double* sh = red + static_cast<long long>(pid) * rs;
   const long long b = static_cast<long long>(blockIdx.x) - (vside ? P.cn0 : P.cm0);
   const T* basis = static_cast<const T*>(vside ? P.V : P.U);
   T* w = vside ? bcol<T>(P, true, J) : bcol<T>(P, false, J + 1);
@@ -267,6 +270,7 @@ struct Batch {
   hipStream_t stream = nullptr;
   // static tables (operators do not change)
   int Sm = 0, Sn = 0, Qa = 0, Qt = 0, maxk = 0;
+  int rstride = 0;             // per-problem stride of red: the largest column capacity
   int Cm = 0, Cn = 0;          // Gram-Schmidt slices over m, n rows
   int* slice_pid_m = nullptr;  // global slice over m -> problem
   int* slice_pid_n = nullptr;
@@ -376,9 +380,9 @@ void enqueue_batch_step(Batch& B, int J, int elr, int tries) {
       hipLaunchKernelGGL(k_bgemvt<T>, dim3(B.Cn), dim3(kBlock), 0, s, B.dprob, B.cslice_pid_n, 1,
                          J, static_cast<int>(PH_V_REO), B.part, B.Cn);
       hipLaunchKernelGGL(k_breduce, dim3(np, J - 1), dim3(kBlock), 0, s, B.dprob, 1, J,
-                         static_cast<int>(PH_V_REO), B.part, B.Cn, B.red);
+                         static_cast<int>(PH_V_REO), B.part, B.Cn, B.red, B.rstride);
       hipLaunchKernelGGL(k_bgemv<T>, dim3(B.Cn), dim3(kBlock), 0, s, B.dprob, B.cslice_pid_n, 1,
-                         J, static_cast<int>(PH_V_REO), B.red, B.part);
+                         J, static_cast<int>(PH_V_REO), B.red, B.rstride, B.part);
       hipLaunchKernelGGL(k_bctl, dim3(np), dim3(kBlock), 0, s, B.dprob,
                          static_cast<int>(CTL_V_REO), static_cast<int>(PH_V_REO), J, B.part, B.Qa,
                          B.Qt, B.Sm, B.Sn);
@@ -400,9 +404,9 @@ void enqueue_batch_step(Batch& B, int J, int elr, int tries) {
     hipLaunchKernelGGL(k_bgemvt<T>, dim3(B.Cm), dim3(kBlock), 0, s, B.dprob, B.cslice_pid_m, 0, J,
                        static_cast<int>(PH_U_REO), B.part, B.Cm);
     hipLaunchKernelGGL(k_breduce, dim3(np, J), dim3(kBlock), 0, s, B.dprob, 0, J,
-                       static_cast<int>(PH_U_REO), B.part, B.Cm, B.red);
+                       static_cast<int>(PH_U_REO), B.part, B.Cm, B.red, B.rstride);
     hipLaunchKernelGGL(k_bgemv<T>, dim3(B.Cm), dim3(kBlock), 0, s, B.dprob, B.cslice_pid_m, 0, J,
-                       static_cast<int>(PH_U_REO), B.red, B.part);
+                       static_cast<int>(PH_U_REO), B.red, B.rstride, B.part);
     hipLaunchKernelGGL(k_bctl, dim3(np), dim3(kBlock), 0, s, B.dprob, static_cast<int>(CTL_U_REO),
                        static_cast<int>(PH_U_REO), J, B.part, B.Qa, B.Qt, B.Sm, B.Sn);
   }
@@ -509,10 +513,13 @@ int batch_create(int nprob, const pk_lbd_t* handles, Batch** out) {
     for (int s = 0; s < P.ncm; ++s) cpm.push_back(i);
     for (int s = 0; s < P.ncn; ++s) cpn.push_back(i);
     B->maxk = std::max(B->maxk, e->kmax);
+    B->rstride = std::max(B->rstride, e->ccap);
     for (int tr = 0; tr < 2 && rc == PK_OK; ++tr) {
       const CsrOp::Dir& D = tr ? e->op->dt : e->op->da;
       (tr ? P.npt : P.npa) = D.np;
-      (tr ? P.yacc_t : P.yacc_a) = D.yacc;
+      // the engine's own scratch: the SpMVs of both directions run one after
+      // the other on the batch's stream
+      (tr ? P.yacc_t : P.yacc_a) = e->sw_y;
       // the layout the operator uses for this direction (row-block or
       // column-sorted), exactly as its own SpMV runs it
       const int cap = D.sort_cap;
@@ -520,8 +527,8 @@ int batch_create(int nprob, const pk_lbd_t* handles, Batch** out) {
         const CsrOp::Panel& pan = D.pan[p];
         const CsrOp::Part& part = cap > 0 ? pan.spart : pan.part;
         hpan[(static_cast<size_t>(i) * 2 + tr) * kPanelsMax + p] =
-            cap > 0 ? BPanel{pan.rp, pan.sci, pan.sva, part.lsum, pan.spos, pan.scol0, pan.spbits}
-                    : BPanel{pan.rp, pan.ci, pan.va, part.lsum, nullptr, 0, 0};
+            cap > 0 ? BPanel{pan.rp, pan.sci, pan.sva, e->sw_l, pan.spos, pan.scol0, pan.spbits}
+                    : BPanel{pan.rp, pan.ci, pan.va, e->sw_l, nullptr, 0, 0};
         std::vector<int4> hd(part.ndesc), hl(part.nlong);
         if ((!hd.empty() && hipMemcpy(hd.data(), part.desc, sizeof(int4) * hd.size(),
                                       hipMemcpyDeviceToHost) != hipSuccess) ||
@@ -586,7 +593,7 @@ int batch_create(int nprob, const pk_lbd_t* handles, Batch** out) {
     const size_t Q = static_cast<size_t>(std::max(B->Qa, B->Qt));
     B->part_len = std::max(2 * Q, static_cast<size_t>(std::max(B->maxk + 1, 2)) * S);
     ok = ok && hipMalloc(&B->part, sizeof(double) * B->part_len) == hipSuccess &&
-         hipMalloc(&B->red, sizeof(double) * static_cast<size_t>(nprob) * kMaxCols) ==
+         hipMalloc(&B->red, sizeof(double) * static_cast<size_t>(nprob) * B->rstride) ==
              hipSuccess &&
          hipMalloc(&B->dprob, sizeof(BProb) * nprob) == hipSuccess;
     if (!ok) rc = PK_ERR_HIP;
@@ -649,14 +656,18 @@ int batch_lanbpro(Batch& B, const int* active, const int* k0, int* k, double* co
     BProb& P = B.hprob[i];
     Engine* e = B.eng[i];
     P.ctl = e->dctl;
+    P.arr = static_cast<char*>(e->darr);
+    P.cap = e->kmax;
     P.U = e->u();
     P.V = e->v();
     P.ldu = e->ldu;
     P.ldv = e->ldv;
     if (run[i]) {
       any = true;
-      if (hipMemcpyAsync(e->dctl, e->hctl, sizeof(LbdCtl), hipMemcpyHostToDevice, B.stream) !=
-          hipSuccess)
+      if (hipMemcpyAsync(e->dctl, static_cast<LbdCtl*>(e->hctl), sizeof(LbdCtl),
+                         hipMemcpyHostToDevice, B.stream) != hipSuccess ||
+          hipMemcpyAsync(e->darr, e->harr, static_cast<size_t>(CtlArrays::bytes(e->kmax)),
+                         hipMemcpyHostToDevice, B.stream) != hipSuccess)
         return PK_ERR_HIP;
     } else if (hipMemsetAsync(&e->dctl->phase, 0, sizeof(int), B.stream) != hipSuccess) {
       return PK_ERR_HIP;  // phase 0 gates the problem off
@@ -674,8 +685,12 @@ int batch_lanbpro(Batch& B, const int* active, const int* k0, int* k, double* co
     }
     if (hipGetLastError() != hipSuccess) return PK_ERR_HIP;
     for (int i = 0; i < np; ++i)
-      if (run[i] && hipMemcpyAsync(B.eng[i]->hctl, B.eng[i]->dctl, sizeof(LbdCtl),
-                                   hipMemcpyDeviceToHost, B.stream) != hipSuccess)
+      if (run[i] && (hipMemcpyAsync(static_cast<LbdCtl*>(B.eng[i]->hctl), B.eng[i]->dctl,
+                                    sizeof(LbdCtl), hipMemcpyDeviceToHost, B.stream) !=
+                         hipSuccess ||
+                     hipMemcpyAsync(B.eng[i]->harr, B.eng[i]->darr,
+                                    static_cast<size_t>(CtlArrays::bytes(B.eng[i]->kmax)),
+                                    hipMemcpyDeviceToHost, B.stream) != hipSuccess))
         return PK_ERR_HIP;
   }
   if (hipStreamSynchronize(B.stream) != hipSuccess) return PK_ERR_HIP;

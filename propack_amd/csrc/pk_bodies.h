@@ -435,7 +435,8 @@ __device__ __forceinline__ double block_max(double v, double* red) {
 // hd_update_nu (dlanbpro.F:684-723) with its elements spread over the
 // workgroup: every element is computed by the serial form's expression and
 // the maximum is exact, so the results are bit-identical to it.
-__device__ inline void par_update_nu(LbdCtl& c, double* red) {
+template <class C>
+__device__ inline void par_update_nu(C& c, double* red) {
 #pragma clang fp contract(off)
   const int j = c.j;
   if (j <= 1) return;
@@ -461,7 +462,8 @@ __device__ inline void par_update_nu(LbdCtl& c, double* red) {
 }
 
 // hd_update_mu (dlanbpro.F:628-679), spread over the workgroup likewise.
-__device__ inline void par_update_mu(LbdCtl& c, double* red) {
+template <class C>
+__device__ inline void par_update_mu(C& c, double* red) {
 #pragma clang fp contract(off)
   const int j = c.j;
   if (j == 1) {
@@ -497,49 +499,27 @@ __device__ inline void par_update_mu(LbdCtl& c, double* red) {
   }
 }
 
-// Control stage of one step for the state gc (global): stage it in LDS (c)
-// with 16-byte loads issued before the partial reductions, reduce
-// a = sum pa[0:na) (and b = sum pb[0:nb) when pb is set) in index order, let
-// thread 0 run the stage, spread a pending omega update over the workgroup,
-// finish the stage, write the state back.  lds holds (kBlock / kWave) * 2
-// doubles.
-__device__ __forceinline__ void ctl_stage_body(LbdCtl* __restrict__ gc, int kind,
-                                               const double* pa, int na, const double* pb,
-                                               int nb, LbdCtl& c, double* lds) {
-  constexpr int nv = sizeof(LbdCtl) / sizeof(int4);
-  constexpr int per = (nv + kBlock - 1) / kBlock;
-  static_assert(sizeof(LbdCtl) % sizeof(int4) == 0, "state must be whole 16-byte words");
-  const int4* src = reinterpret_cast<const int4*>(gc);
-  int4* dst = reinterpret_cast<int4*>(&c);
-  // Word indices past the end are clamped to the last word, so every load is
-  // unconditional and all of them are in flight at once; the clamped lanes
-  // store that word's own value again.  The words wait in registers while
-  // the partials are reduced, so the state and the partials arrive in one
-  // memory round trip; they reach LDS after the reduction.
-  // (six named words: an array here was placed in scratch memory)
-  static_assert(per == 6, "the staging holds six 16-byte words per thread");
-  const int t = static_cast<int>(threadIdx.x);
-  const int i0 = t, i1 = t + kBlock, i2 = t + 2 * kBlock, i3 = t + 3 * kBlock,
-            i4 = t + 4 * kBlock, i5 = min(t + 5 * kBlock, nv - 1);
-  const int4 w0 = src[i0], w1 = src[i1], w2 = src[i2], w3 = src[i3], w4 = src[i4], w5 = src[i5];
-  double a, b = 0.0;
-  if (pb && nb == na) {
-    const double* const q[2] = {pa, pb};
-    double v[2];
-    reduce_partials_n<2>(q, na, v, lds);
-    a = v[0];
-    b = v[1];
-  } else {
-    a = reduce_partials(pa, na, lds);
-    if (pb) b = reduce_partials(pb, nb, lds);
-  }
-  dst[i0] = w0;
-  dst[i1] = w1;
-  dst[i2] = w2;
-  dst[i3] = w3;
-  dst[i4] = w4;
-  dst[i5] = w5;
-  __syncthreads();
+// LDS copy of a control state at a step j <= kCtlLdsJ: the scalars and the
+// windows of the arrays the step reads and writes (mu, nu, B1, B2 over
+// 0..j+2, INDEX over 0..2j+4); the CGS column list stays in device memory.
+constexpr int kCtlWinD = (kCtlLdsJ + 4) & ~1;
+constexpr int kCtlWinI = (2 * kCtlLdsJ + 8) & ~3;
+struct LbdCtlLds : LbdCtl {
+  double mu[kCtlWinD], nu[kCtlWinD], B1[kCtlWinD], B2[kCtlWinD];
+  int index[kCtlWinI];
+  int* cols;
+};
+constexpr int kCtlHdrWords = static_cast<int>(sizeof(LbdCtl) / sizeof(int4));
+static_assert(sizeof(LbdCtl) % sizeof(int4) == 0, "state must be whole 16-byte words");
+// staged 16-byte words per thread at the largest LDS step
+constexpr int kCtlPer = 6;
+static_assert(kCtlHdrWords + 2 * kCtlWinD + kCtlWinI / 4 <= kCtlPer * kBlock,
+              "the staging holds six 16-byte words per thread");
+
+// The control stage itself on state c (LDS or device arrays): thread 0 runs
+// the stage, the workgroup a pending omega update, thread 0 the remainder.
+template <class C>
+__device__ __forceinline__ void ctl_run(C& c, int kind, double a, double b, double* lds) {
   if (threadIdx.x == 0) {
     switch (kind) {
       case CTL_V_SPMV: ctl_v_spmv_s(c, a, b); break;
@@ -568,12 +548,97 @@ __device__ __forceinline__ void ctl_stage_body(LbdCtl* __restrict__ gc, int kind
     }
   }
   __syncthreads();
-  int4* back = reinterpret_cast<int4*>(gc);
-#pragma unroll
-  for (int u = 0; u < per; ++u) {
-    const int i = min(static_cast<int>(threadIdx.x) + u * kBlock, nv - 1);
-    back[i] = dst[i];
+}
+
+// Word w of the staging of step J: its device address and its LDS address.
+// Words [0, kCtlHdrWords) are the scalars, then the four double windows of
+// wd doubles each, then the INDEX window.
+__device__ __forceinline__ void ctl_word(int w, LbdCtl* gc, char* garr, int cap, int wd,
+                                        LbdCtlLds& c, int4** src, int4** dst) {
+  if (w < kCtlHdrWords) {
+    *src = reinterpret_cast<int4*>(gc) + w;
+    *dst = reinterpret_cast<int4*>(static_cast<LbdCtl*>(&c)) + w;
+    return;
   }
+  w -= kCtlHdrWords;
+  const int hw = wd / 2;
+  if (w < 4 * hw) {
+    const int a = w / hw, o = w - a * hw;
+    *src = reinterpret_cast<int4*>(CtlArrays::mu(garr, cap) + a * CtlArrays::dlen(cap)) + o;
+    *dst = reinterpret_cast<int4*>(c.mu + a * kCtlWinD) + o;
+    return;
+  }
+  w -= 4 * hw;
+  *src = reinterpret_cast<int4*>(CtlArrays::index(garr, cap)) + w;
+  *dst = reinterpret_cast<int4*>(c.index) + w;
+}
+
+// Control stage of step J for the state gc (scalars) / garr (arrays of
+// capacity cap), both in device memory: stage the state in LDS with 16-byte
+// loads issued before the partial reductions, reduce a = sum pa[0:na) (and
+// b = sum pb[0:nb) when pb is set) in index order, run the stage
+// (ctl_run), write the state back.  Steps past kCtlLdsJ stage the scalars
+// only and work on the arrays in device memory.  lds holds
+// (kBlock / kWave) * 2 doubles.
+__device__ __forceinline__ void ctl_stage_body(LbdCtl* __restrict__ gc, char* garr, int cap,
+                                               int J, int kind, const double* pa, int na,
+                                               const double* pb, int nb, LbdCtlLds& c,
+                                               LbdCtlRef& cg, double* lds) {
+  const bool in_lds = J <= kCtlLdsJ;
+  const int wd = (J + 4) & ~1, wi = (2 * J + 8) & ~3;
+  const int nw = in_lds ? kCtlHdrWords + 2 * wd + wi / 4 : kCtlHdrWords;
+  const int t = static_cast<int>(threadIdx.x);
+  // Word indices past the end are clamped to the last word, so every load
+  // is unconditional and all of them are in flight at once; the clamped
+  // lanes store that word's own value again.  The words wait in registers
+  // while the partials are reduced, so the state and the partials arrive in
+  // one memory round trip.  (six named words: an array here was placed in
+  // scratch memory)
+  int4 *s0, *s1, *s2, *s3, *s4, *s5, *d0, *d1, *d2, *d3, *d4, *d5;
+  ctl_word(min(t, nw - 1), gc, garr, cap, wd, c, &s0, &d0);
+  ctl_word(min(t + kBlock, nw - 1), gc, garr, cap, wd, c, &s1, &d1);
+  ctl_word(min(t + 2 * kBlock, nw - 1), gc, garr, cap, wd, c, &s2, &d2);
+  ctl_word(min(t + 3 * kBlock, nw - 1), gc, garr, cap, wd, c, &s3, &d3);
+  ctl_word(min(t + 4 * kBlock, nw - 1), gc, garr, cap, wd, c, &s4, &d4);
+  ctl_word(min(t + 5 * kBlock, nw - 1), gc, garr, cap, wd, c, &s5, &d5);
+  const int4 w0 = *s0, w1 = *s1, w2 = *s2, w3 = *s3, w4 = *s4, w5 = *s5;
+  double a, b = 0.0;
+  if (pb && nb == na) {
+    const double* const q[2] = {pa, pb};
+    double v[2];
+    reduce_partials_n<2>(q, na, v, lds);
+    a = v[0];
+    b = v[1];
+  } else {
+    a = reduce_partials(pa, na, lds);
+    if (pb) b = reduce_partials(pb, nb, lds);
+  }
+  *d0 = w0;
+  *d1 = w1;
+  *d2 = w2;
+  *d3 = w3;
+  *d4 = w4;
+  *d5 = w5;
+  __syncthreads();
+  if (in_lds) {
+    if (threadIdx.x == 0) c.cols = CtlArrays::cols(garr, cap);
+    ctl_run(c, kind, a, b, lds);
+  } else {
+    if (threadIdx.x == 0) {
+      *static_cast<LbdCtl*>(&cg) = *static_cast<LbdCtl*>(&c);
+      cg.bind(garr);
+    }
+    __syncthreads();
+    ctl_run(cg, kind, a, b, lds);
+    if (threadIdx.x == 0) *static_cast<LbdCtl*>(&c) = *static_cast<LbdCtl*>(&cg);
+    __syncthreads();
+  }
+  *s0 = *d0;
+  *s1 = *d1;
+  *s2 = *d2;
+  *s3 = *d3;
+  *s4 = *d4;
+  *s5 = *d5;
 }
 
 }  // namespace pk

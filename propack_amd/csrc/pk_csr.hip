@@ -621,24 +621,26 @@ void csr_destroy(CsrOp* op) {
 // sums stay in FP64 between them).  Returns the partial count or < 0.
 template <typename T>
 int csr_apply(const CsrOp* op, hipStream_t s, char trans, const T* x, T* y, const T* yprev,
-              Scal coef, double* part, Gate gate) {
+              Scal coef, double* part, Gate gate, double* ywork, double* lwork) {
   const bool tr = (trans == 't' || trans == 'T');
   const CsrOp::Dir& D = tr ? op->dt : op->da;
+  double* yacc = ywork ? ywork : D.yacc;
   int nb = -1;
   for (int p = 0; p < D.np; ++p) {
     const CsrOp::Panel& P = D.pan[p];
-    const double* yin = p == 0 ? nullptr : D.yacc;
+    const double* yin = p == 0 ? nullptr : yacc;
     const int last = p == D.np - 1;
-    double* yout = last ? nullptr : D.yacc;
+    double* yout = last ? nullptr : yacc;
     if (D.sort_cap > 0) {
       const CsrOp::Part& Q = P.spart;
+      double* lsum = lwork ? lwork : Q.lsum;
       if (op->val_prec == PK_F64)
-        nb = launch_spmv_sorted<double, T>(s, Q.ndesc, Q.desc, Q.nlong, Q.ldesc, Q.lsum, P.rp,
+        nb = launch_spmv_sorted<double, T>(s, Q.ndesc, Q.desc, Q.nlong, Q.ldesc, lsum, P.rp,
                                            P.sci, static_cast<const double*>(P.sva), P.spos,
                                            P.scol0, P.spbits, D.sort_cap, x, y, yprev, coef,
                                            part, P.g, yin, yout, last, gate);
       else
-        nb = launch_spmv_sorted<float, T>(s, Q.ndesc, Q.desc, Q.nlong, Q.ldesc, Q.lsum, P.rp,
+        nb = launch_spmv_sorted<float, T>(s, Q.ndesc, Q.desc, Q.nlong, Q.ldesc, lsum, P.rp,
                                           P.sci, static_cast<const float*>(P.sva), P.spos,
                                           P.scol0, P.spbits, D.sort_cap, x, y, yprev, coef,
                                           part, P.g, yin, yout, last, gate);
@@ -646,12 +648,13 @@ int csr_apply(const CsrOp* op, hipStream_t s, char trans, const T* x, T* y, cons
       continue;
     }
     const CsrOp::Part& Q = P.part;
+    double* lsum = lwork ? lwork : Q.lsum;
     if (op->val_prec == PK_F64)
-      nb = launch_spmv_rb<double, T>(s, Q.ndesc, Q.desc, Q.nlong, Q.ldesc, Q.lsum, P.rp, P.ci,
+      nb = launch_spmv_rb<double, T>(s, Q.ndesc, Q.desc, Q.nlong, Q.ldesc, lsum, P.rp, P.ci,
                                      static_cast<const double*>(P.va), x, y, yprev, coef, part,
                                      P.g, yin, yout, last, gate);
     else
-      nb = launch_spmv_rb<float, T>(s, Q.ndesc, Q.desc, Q.nlong, Q.ldesc, Q.lsum, P.rp, P.ci,
+      nb = launch_spmv_rb<float, T>(s, Q.ndesc, Q.desc, Q.nlong, Q.ldesc, lsum, P.rp, P.ci,
                                     static_cast<const float*>(P.va), x, y, yprev, coef, part,
                                     P.g, yin, yout, last, gate);
     if (nb < 0) return nb;
@@ -660,9 +663,9 @@ int csr_apply(const CsrOp* op, hipStream_t s, char trans, const T* x, T* y, cons
 }
 
 template int csr_apply<double>(const CsrOp*, hipStream_t, char, const double*, double*,
-                               const double*, Scal, double*, Gate);
+                               const double*, Scal, double*, Gate, double*, double*);
 template int csr_apply<float>(const CsrOp*, hipStream_t, char, const float*, float*,
-                              const float*, Scal, double*, Gate);
+                              const float*, Scal, double*, Gate, double*, double*);
 
 }  // namespace pk
 

@@ -21,21 +21,29 @@
 
 namespace pk {
 
-static thread_local char g_err[512] = "";
-
-// Batched drivers run per-problem host work on several threads: the message
-// buffer is written under a lock.
+// One message buffer for the process: batched drivers raise errors on
+// their host worker threads and the caller reads them on its own thread.
+// Writers and readers take the lock; a reader gets a copy in its thread's
+// own buffer, which stays valid until that thread's next call.
+static char g_err[512] = "";
 static std::mutex g_err_lock;
 
 void set_error(const char* fmt, ...) {
-  std::lock_guard<std::mutex> lk(g_err_lock);
+  char msg[sizeof g_err];
   va_list ap;
   va_start(ap, fmt);
-  vsnprintf(g_err, sizeof g_err, fmt, ap);
+  vsnprintf(msg, sizeof msg, fmt, ap);
   va_end(ap);
+  std::lock_guard<std::mutex> lk(g_err_lock);
+  std::memcpy(g_err, msg, sizeof g_err);
 }
 
-const char* last_error() { return g_err; }
+const char* last_error() {
+  static thread_local char copy[sizeof g_err];
+  std::lock_guard<std::mutex> lk(g_err_lock);
+  std::memcpy(copy, g_err, sizeof copy);
+  return copy;
+}
 
 #define PK_HIP(expr)                                                                     \
   do {                                                                                   \
@@ -209,7 +217,8 @@ struct Impl {
     int nb;
     if (e->op) {
       hipEvent_t t0 = tstart(tr ? KC_SPMV_T : KC_SPMV_N);
-      nb = csr_apply<T>(e->op, strm, trans, x, y, yprev, sval(c), e->part, Gate{});
+      nb = csr_apply<T>(e->op, strm, trans, x, y, yprev, sval(c), e->part, Gate{}, e->sw_y,
+                        e->sw_l);
       PK_LAUNCH(nb);
       tstop(t0, tr ? KC_SPMV_T : KC_SPMV_N, spmv_bytes(tr, yprev != nullptr),
             2.0 * static_cast<double>(e->op->nnz));
@@ -323,8 +332,8 @@ struct Impl {
       const int p = index[i], q = index[i + 1];
       nd += q - p + 1;
       for (int c = p; c <= q; ++c) {
-        if (ncols >= kMaxCols) {
-          set_error("reorthogonalization touches more than %d columns", kMaxCols);
+        if (ncols >= e->ccap) {
+          set_error("reorthogonalization touches more than %d columns", e->ccap);
           return PK_ERR_NOTSUP;
         }
         e->hcols[ncols++] = c - 1;
@@ -351,9 +360,9 @@ struct Impl {
                                   e->part, Gate{});
         PK_LAUNCH(nb2);
         tstop(t1, KC_GEMV, (ncols + 2.0) * L * sb, 2.0 * ncols * L);
-        PK_LAUNCH(launch_reduce(strm, 1, nb2, e->part, e->red + kMaxCols, Gate{}));
-        PK_TRY(read(kMaxCols, 1));
-        *normvnew = std::sqrt(e->hred[kMaxCols]);
+        PK_LAUNCH(launch_reduce(strm, 1, nb2, e->part, e->red + e->ccap, Gate{}));
+        PK_TRY(read(e->ccap, 1));
+        *normvnew = std::sqrt(e->hred[e->ccap]);
       } else {
         PK_TRY(nrm2(vnew, L, normvnew));
       }
@@ -381,7 +390,9 @@ struct Impl {
     const double kap = static_cast<double>(0.717f);
     *ierr = -1;
     for (int itry = 1; itry <= ntry; ++itry) {
-      PK_LAUNCH(launch_larnv_pm1<T>(strm, rsize, x));
+      // DLARNV advances the seed DGETU0 set once per call (double/dgetu0.F:
+      // 47-50,67): try t continues the sequence at element (t-1)*rsize
+      PK_LAUNCH(launch_larnv_pm1<T>(strm, rsize, x, (itry - 1) * rsize));
       double nrm;
       PK_TRY(nrm2(x, rsize, &nrm));
       double s2, sd;
@@ -403,7 +414,8 @@ struct Impl {
 
   // One CGS pass of vnew against basis(:, c.cols) on the host executor:
   // returns ||vnew'||^2.
-  int reo_pass(long long L, const T* basis, long long ld, T* vnew, const LbdCtl& c, double* s2) {
+  int reo_pass(long long L, const T* basis, long long ld, T* vnew, const LbdCtlRef& c,
+               double* s2) {
     const int ncols = c.reo_ncols;
     if (ncols > 0) {
       std::memcpy(e->hcols, c.cols, sizeof(int) * ncols);
@@ -420,9 +432,9 @@ struct Impl {
                                             vnew, e->part, Gate{});
       PK_LAUNCH(nb2);
       tstop(t1, KC_GEMV, (ncols + 2.0) * L * sb, 2.0 * ncols * L);
-      PK_LAUNCH(launch_reduce(strm, 1, nb2, e->part, e->red + kMaxCols, Gate{}));
-      PK_TRY(read(kMaxCols, 1));
-      *s2 = e->hred[kMaxCols];
+      PK_LAUNCH(launch_reduce(strm, 1, nb2, e->part, e->red + e->ccap, Gate{}));
+      PK_TRY(read(e->ccap, 1));
+      *s2 = e->hred[e->ccap];
     } else {
       double nrm;
       PK_TRY(nrm2(vnew, L, &nrm));
@@ -443,10 +455,11 @@ struct Impl {
 
   // Execute the current phase of the step on the host (same control
   // functions as the device chain, plus the rare paths).
-  int host_phase(LbdCtl& c) {
+  int host_phase(LbdCtlRef& c) {
     const int j = c.j, m = e->m, n = e->n;
     double s2 = 0, sd = 0, anormest = 0;
     int e2 = 0;
+    if (c.phase >= PH_V_INVARIANT && c.phase <= PH_U_ZERO) e->cnt.rare[c.phase - PH_V_INVARIANT]++;
     switch (c.phase) {
       case PH_V_SPMV:
         if (c.uscale) {
@@ -557,67 +570,69 @@ struct Impl {
   // state.  Nothing here waits for the device.
   int enqueue_step(int J, int elr, int tries) {
     LbdCtl* dc = e->dctl;
+    const int* dcl = CtlArrays::cols(e->darr, e->kmax);  // CGS column list (device)
     const int m = e->m, n = e->n;
     const double sb = static_cast<double>(sizeof(T));
     // ---- v_J = A^T u_J - beta v_{J-1} ----
     PK_LAUNCH(launch_scale<T>(strm, m, ucol(J), sptr(&dc->uscale_f), Gate{dc, PH_V_SPMV, J, 2}));
     hipEvent_t t0 = tstart(KC_SPMV_T);
     int nb = csr_apply<T>(e->op, strm, 't', ucol(J), vcol(J), J > 1 ? vcol(J - 1) : nullptr,
-                          sptr(&dc->beta), e->part, Gate{dc, PH_V_SPMV, J, 0});
+                          sptr(&dc->beta), e->part, Gate{dc, PH_V_SPMV, J, 0}, e->sw_y,
+                          e->sw_l);
     PK_LAUNCH(nb);
     tstop(t0, KC_SPMV_T, spmv_bytes(true, J > 1), 2.0 * static_cast<double>(e->op->nnz));
-    PK_LAUNCH(launch_ctl(strm, dc, CTL_V_SPMV, PH_V_SPMV, J, e->part, nb));
+    PK_LAUNCH(launch_ctl(strm, dc, e->darr, e->kmax, CTL_V_SPMV, PH_V_SPMV, J, e->part, nb));
     if (J > 1) {
       for (int i = 0; i < elr; ++i) {
         nb = launch_axpy_norm_dot<T>(strm, n, vcol(J), vcol(J - 1), sptr(&dc->elr_s), e->part,
                                      Gate{dc, PH_V_ELR, J, 0});
         PK_LAUNCH(nb);
-        PK_LAUNCH(launch_ctl(strm, dc, CTL_V_ELR, PH_V_ELR, J, e->part, nb));
+        PK_LAUNCH(launch_ctl(strm, dc, e->darr, e->kmax, CTL_V_ELR, PH_V_ELR, J, e->part, nb));
       }
       for (int t = 0; t < tries; ++t) {
         const Gate g{dc, PH_V_REO, J, 0};
         hipEvent_t t1 = tstart(KC_GEMVT);
-        nb = launch_gemvt<T>(strm, n, V(), e->ldv, dc->cols, J - 1, &dc->reo_ncols, vcol(J),
+        nb = launch_gemvt<T>(strm, n, V(), e->ldv, dcl, J - 1, &dc->reo_ncols, vcol(J),
                              e->part, g);
         PK_LAUNCH(nb);
         tstop(t1, KC_GEMVT, 0.0);
         PK_LAUNCH(launch_reduce(strm, J - 1, nb, e->part, e->red, g, &dc->reo_ncols));
         hipEvent_t t2 = tstart(KC_GEMV);
-        const int nb2 = launch_gemv_update<T>(strm, n, V(), e->ldv, dc->cols, J - 1,
+        const int nb2 = launch_gemv_update<T>(strm, n, V(), e->ldv, dcl, J - 1,
                                               &dc->reo_ncols, e->red, vcol(J), e->part, g);
         PK_LAUNCH(nb2);
         tstop(t2, KC_GEMV, 0.0);
-        PK_LAUNCH(launch_ctl(strm, dc, CTL_V_REO, PH_V_REO, J, e->part, nb2));
+        PK_LAUNCH(launch_ctl(strm, dc, e->darr, e->kmax, CTL_V_REO, PH_V_REO, J, e->part, nb2));
       }
     }
     // ---- u_{J+1} = A v_J - alpha u_J ----
     PK_LAUNCH(launch_scale<T>(strm, n, vcol(J), sptr(&dc->vscale_f), Gate{dc, PH_U_SPMV, J, 1}));
     hipEvent_t t3 = tstart(KC_SPMV_N);
     nb = csr_apply<T>(e->op, strm, 'n', vcol(J), ucol(J + 1), ucol(J), sptr(&dc->alpha), e->part,
-                      Gate{dc, PH_U_SPMV, J, 0});
+                      Gate{dc, PH_U_SPMV, J, 0}, e->sw_y, e->sw_l);
     PK_LAUNCH(nb);
     tstop(t3, KC_SPMV_N, spmv_bytes(false, true), 2.0 * static_cast<double>(e->op->nnz));
-    PK_LAUNCH(launch_ctl(strm, dc, CTL_U_SPMV, PH_U_SPMV, J, e->part, nb));
+    PK_LAUNCH(launch_ctl(strm, dc, e->darr, e->kmax, CTL_U_SPMV, PH_U_SPMV, J, e->part, nb));
     for (int i = 0; i < elr; ++i) {
       nb = launch_axpy_norm_dot<T>(strm, m, ucol(J + 1), ucol(J), sptr(&dc->elr_s), e->part,
                                    Gate{dc, PH_U_ELR, J, 0});
       PK_LAUNCH(nb);
-      PK_LAUNCH(launch_ctl(strm, dc, CTL_U_ELR, PH_U_ELR, J, e->part, nb));
+      PK_LAUNCH(launch_ctl(strm, dc, e->darr, e->kmax, CTL_U_ELR, PH_U_ELR, J, e->part, nb));
     }
     for (int t = 0; t < tries; ++t) {
       const Gate g{dc, PH_U_REO, J, 0};
       hipEvent_t t1 = tstart(KC_GEMVT);
-      nb = launch_gemvt<T>(strm, m, U(), e->ldu, dc->cols, J, &dc->reo_ncols, ucol(J + 1),
+      nb = launch_gemvt<T>(strm, m, U(), e->ldu, dcl, J, &dc->reo_ncols, ucol(J + 1),
                            e->part, g);
       PK_LAUNCH(nb);
       tstop(t1, KC_GEMVT, 0.0);
       PK_LAUNCH(launch_reduce(strm, J, nb, e->part, e->red, g, &dc->reo_ncols));
       hipEvent_t t2 = tstart(KC_GEMV);
-      const int nb2 = launch_gemv_update<T>(strm, m, U(), e->ldu, dc->cols, J, &dc->reo_ncols,
+      const int nb2 = launch_gemv_update<T>(strm, m, U(), e->ldu, dcl, J, &dc->reo_ncols,
                                             e->red, ucol(J + 1), e->part, g);
       PK_LAUNCH(nb2);
       tstop(t2, KC_GEMV, 0.0);
-      PK_LAUNCH(launch_ctl(strm, dc, CTL_U_REO, PH_U_REO, J, e->part, nb2));
+      PK_LAUNCH(launch_ctl(strm, dc, e->darr, e->kmax, CTL_U_REO, PH_U_REO, J, e->part, nb2));
     }
     (void)sb;
     return PK_OK;
@@ -625,11 +640,16 @@ struct Impl {
 
   // Run steps c.j..c.k on the device until done or until the state needs
   // the host (a rare path, or more CGS passes than the chain holds).
-  int run_device(LbdCtl& c, int elr) {
+  int run_device(LbdCtlRef& c, int elr) {
     const long long pv0 = c.npass_v, pu0 = c.npass_u, cv0 = c.ncols_v, cu0 = c.ncols_u;
-    PK_HIP(hipMemcpyAsync(e->dctl, &c, sizeof(LbdCtl), hipMemcpyHostToDevice, strm));
+    const size_t ab = static_cast<size_t>(CtlArrays::bytes(e->kmax));
+    PK_HIP(hipMemcpyAsync(e->dctl, static_cast<LbdCtl*>(&c), sizeof(LbdCtl),
+                          hipMemcpyHostToDevice, strm));
+    PK_HIP(hipMemcpyAsync(e->darr, e->harr, ab, hipMemcpyHostToDevice, strm));
     for (int J = c.j; J <= c.k; ++J) PK_TRY(enqueue_step(J, elr, e->dev_tries));
-    PK_HIP(hipMemcpyAsync(&c, e->dctl, sizeof(LbdCtl), hipMemcpyDeviceToHost, strm));
+    PK_HIP(hipMemcpyAsync(static_cast<LbdCtl*>(&c), e->dctl, sizeof(LbdCtl),
+                          hipMemcpyDeviceToHost, strm));
+    PK_HIP(hipMemcpyAsync(e->harr, e->darr, ab, hipMemcpyDeviceToHost, strm));
     PK_HIP(stream_wait(strm));
     e->cnt.nsync++;
     // algorithmic bytes of the CGS passes the chain actually ran
@@ -679,8 +699,10 @@ struct Impl {
       set_error("lanbpro: k=%d k0=%d outside the engine's kmax=%d", k, k0, e->kmax);
       return PK_ERR_ARG;
     }
-    LbdCtl& c = *e->hctl;
-    std::memset(&c, 0, sizeof(LbdCtl));
+    LbdCtlRef& c = *e->hctl;
+    std::memset(static_cast<LbdCtl*>(&c), 0, sizeof(LbdCtl));
+    c.cap = e->kmax;
+    std::memset(e->harr, 0, static_cast<size_t>(CtlArrays::bytes(e->kmax)));
     double* mu = c.mu;
     double* nu = c.nu;
     int* index = c.index;
@@ -799,7 +821,7 @@ struct Impl {
   // Steps of the call from the state in e->hctl until it is done: device
   // chains from PH_V_SPMV, host execution of every other phase.
   int lanbpro_loop() {
-    LbdCtl& c = *e->hctl;
+    LbdCtlRef& c = *e->hctl;
     const bool device = e->mode == 0 && e->op != nullptr;
     while (c.phase != PH_DONE) {
       if (device && c.phase == PH_V_SPMV) {
@@ -822,7 +844,7 @@ struct Impl {
   // End of the call: outputs in the reference's places (:540-548).
   int lanbpro_end(int k0, int* kp, double* alpha_, double* beta_, double* rnorm, double* doption,
                   int* ierr_out) {
-    LbdCtl& c = *e->hctl;
+    LbdCtlRef& c = *e->hctl;
     const double* B1 = c.B1;
     const double* B2 = c.B2;
     // The 1/beta scaling of u_{k+1} is deferred to the start of the next
@@ -846,6 +868,7 @@ struct Impl {
     e->cnt.ndot += c.ndot;
     e->cnt.ncols_touched += c.ncols_touched;
     e->cnt.nretry += c.nretry;
+    e->cnt.nwarn += c.nwarn;
     PK_HIP(stream_wait(strm));
     return resolve_timers();
   }
@@ -950,8 +973,12 @@ static void engine_free(Engine* e) {
   (void)hipFree(e->dcols);
   (void)hipFree(e->dB);
   (void)hipFree(e->vec);
+  (void)hipFree(e->sw_y);
+  (void)hipFree(e->sw_l);
   (void)hipFree(e->dctl);
+  (void)hipFree(e->darr);
   (void)hipHostFree(e->hctl);
+  (void)hipHostFree(e->harr);
   (void)hipHostFree(e->hred);
   (void)hipHostFree(e->hcols);
   (void)hipHostFree(e->hB);
@@ -978,7 +1005,7 @@ extern "C" {
 const char* pk_last_error(void) { return pk::last_error(); }
 
 int pk_lbd_create(int prec, int m, int n, int kmax, pk_csr_t op, pk_lbd_t* out) {
-  if (!out || m < 1 || n < 1 || kmax < 1 || kmax > kMaxK - 2 ||
+  if (!out || m < 1 || n < 1 || kmax < 1 ||
       (prec != PK_F64 && prec != PK_F32)) {
     set_error("pk_lbd_create: bad arguments m=%d n=%d kmax=%d prec=%d", m, n, kmax, prec);
     return PK_ERR_ARG;
@@ -1006,12 +1033,13 @@ int pk_lbd_create(int prec, int m, int n, int kmax, pk_csr_t op, pk_lbd_t* out) 
   e->ldu = round_up(m, 64);
   e->ldv = round_up(n, 64);
   e->op = cop;
+  e->ccap = std::max(kMaxCols, kmax + 2);
   const long long mx = std::max(m, n);
   size_t nblk = static_cast<size_t>(std::max(stream_blocks(mx), kMaxBlocks));
   if (cop) {
     // the SpMV writes two partials per row block
     const size_t need = 2 * static_cast<size_t>(std::max(cop->nrb, cop->nrbt));
-    nblk = std::max(nblk, (need + kMaxCols - 1) / kMaxCols);
+    nblk = std::max(nblk, (need + e->ccap - 1) / e->ccap);
   }
   const size_t nb = static_cast<size_t>(kmax + 1) * (kmax + 1);
   hipError_t err = hipSuccess;
@@ -1021,16 +1049,28 @@ int pk_lbd_create(int prec, int m, int n, int kmax, pk_csr_t op, pk_lbd_t* out) 
   chk(hipStreamCreateWithFlags(&e->stream, hipStreamNonBlocking));
   chk(hipMalloc(&e->U[0], e->tbytes * e->ldu * (kmax + 1)));
   chk(hipMalloc(&e->V[0], e->tbytes * e->ldv * kmax));
-  chk(hipMalloc(&e->part, sizeof(double) * nblk * kMaxCols));
-  chk(hipMalloc(&e->red, sizeof(double) * (kMaxCols + 16)));
-  chk(hipMalloc(&e->dcols, sizeof(int) * kMaxCols));
+  chk(hipMalloc(&e->part, sizeof(double) * nblk * e->ccap));
+  chk(hipMalloc(&e->red, sizeof(double) * (e->ccap + 16)));
+  chk(hipMalloc(&e->dcols, sizeof(int) * e->ccap));
   chk(hipMalloc(&e->dB, sizeof(double) * nb));
   chk(hipMalloc(&e->vec, e->tbytes * mx * 2));
-  chk(hipHostMalloc(&e->hred, sizeof(double) * (kMaxCols + 16), hipHostMallocDefault));
-  chk(hipHostMalloc(&e->hcols, sizeof(int) * kMaxCols, hipHostMallocDefault));
+  if (cop) {
+    chk(hipMalloc(&e->sw_y, sizeof(double) * mx));
+    chk(hipMalloc(&e->sw_l, sizeof(double) * csr_lwork(cop)));
+  }
+  chk(hipHostMalloc(&e->hred, sizeof(double) * (e->ccap + 16), hipHostMallocDefault));
+  chk(hipHostMalloc(&e->hcols, sizeof(int) * e->ccap, hipHostMallocDefault));
   chk(hipHostMalloc(&e->hB, sizeof(double) * nb, hipHostMallocDefault));
+  const size_t ab = static_cast<size_t>(CtlArrays::bytes(kmax));
   chk(hipMalloc(&e->dctl, sizeof(LbdCtl)));
-  chk(hipHostMalloc(&e->hctl, sizeof(LbdCtl), hipHostMallocDefault));
+  chk(hipMalloc(&e->darr, ab));
+  chk(hipHostMalloc(&e->hctl, sizeof(LbdCtlRef), hipHostMallocDefault));
+  chk(hipHostMalloc(&e->harr, ab, hipHostMallocDefault));
+  if (e->hctl && e->harr) {
+    std::memset(static_cast<void*>(e->hctl), 0, sizeof(LbdCtlRef));
+    e->hctl->cap = kmax;
+    e->hctl->bind(e->harr);
+  }
   if (const char* md = std::getenv("PK_LBD_MODE")) e->mode = (md[0] == 'h' || md[0] == 'H') ? 1 : 0;
   if (err == hipSuccess) {
     // The bases start zeroed so that unwritten columns never hold NaN garbage.
@@ -1190,6 +1230,15 @@ int pk_lbd_stats(pk_lbd_t h, long long* out) {
   out[3] = e->cnt.ncols_touched;
   out[4] = e->cnt.nsync;
   out[5] = e->cnt.nretry;
+  return PK_OK;
+}
+
+int pk_lbd_path_stats(pk_lbd_t h, long long* out) {
+  PK_ENGINE(h);
+  if (!out) return PK_ERR_ARG;
+  for (int i = 0; i < 6; ++i) out[i] = e->cnt.rare[i];
+  out[6] = e->cnt.nwarn;
+  out[7] = 0;
   return PK_OK;
 }
 

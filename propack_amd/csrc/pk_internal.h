@@ -84,9 +84,17 @@ void csr_set_lanes(CsrOp::Dir& D, int shift);
 // most cap nonzeros (kSpmvCap <= cap <= kSortCapMax; built on the device),
 // or back to the row-block layout (cap = 0, the sorted copy freed).
 int csr_set_sort(CsrOp* op, CsrOp::Dir& D, int cap);
+// ywork (rows of the direction, doubles) and lwork (csr_lwork(op) doubles),
+// when given, replace the operator's own running row sums and long-row
+// partial sums, so that callers with their own scratch (every engine) may
+// apply one operator at the same time on different streams.
 template <typename T>
 int csr_apply(const CsrOp* op, hipStream_t s, char trans, const T* x, T* y, const T* yprev,
-              Scal coef, double* part, Gate gate);
+              Scal coef, double* part, Gate gate, double* ywork = nullptr,
+              double* lwork = nullptr);
+// Long-row partial sums one panel can need, for any layout of op: a row of
+// len > kSpmvCap nonzeros has ceil(len / kLongChunk) <= len / kSpmvCap chunks.
+inline long long csr_lwork(const CsrOp* op) { return op->nnz / 1024 + 2; }
 
 // Fortran APROD callback: (TRANSA, M, N, X, Y, DPARM, IPARM, len(TRANSA)).
 typedef void (*aprod_d_fn)(const char*, const int*, const int*, const double*, double*,
@@ -112,6 +120,10 @@ struct Profile {
 
 struct Counters {
   long long nopx = 0, nreorth = 0, ndot = 0, ncols_touched = 0, nsync = 0, nretry = 0;
+  // host-executed rare phases (PH_V_INVARIANT, PH_V_TINY, PH_V_ZERO,
+  // PH_U_INVARIANT, PH_U_TINY, PH_U_ZERO) and near-invariance warnings
+  long long rare[6] = {0, 0, 0, 0, 0, 0};
+  long long nwarn = 0;
 };
 
 struct Engine {
@@ -134,8 +146,9 @@ struct Engine {
   void* hy = nullptr;
   // device work
   hipStream_t stream = nullptr;
-  double* part = nullptr;            // partial sums (max_blocks * kMaxCols)
-  double* red = nullptr;             // reduced sums (kMaxCols + 16)
+  int ccap = 0;                      // column capacity of one Gram-Schmidt pass: max(kMaxCols, kmax + 2)
+  double* part = nullptr;            // partial sums (max_blocks * ccap)
+  double* red = nullptr;             // reduced sums (ccap + 16)
   double* hred = nullptr;            // pinned host mirror of red
   int* dcols = nullptr;              // reorthogonalization column list
   int* hcols = nullptr;              // pinned host staging of the list
@@ -143,8 +156,12 @@ struct Engine {
   double* hB = nullptr;              // pinned host staging
   hipEvent_t evB = nullptr;          // the last copy out of hB (reuse guard)
   void* vec = nullptr;               // scratch vector, max(m, n)
-  LbdCtl* dctl = nullptr;            // step control state (device)
-  LbdCtl* hctl = nullptr;            // its pinned host copy
+  double* sw_y = nullptr;            // SpMV running row sums (max(m, n)), the engine's own
+  double* sw_l = nullptr;            // SpMV long-row partial sums (csr_lwork(op))
+  LbdCtl* dctl = nullptr;            // step control state: scalars (device)
+  void* darr = nullptr;              // its arrays, capacity kmax (CtlArrays layout)
+  LbdCtlRef* hctl = nullptr;         // pinned host copy: scalars, pointers into harr
+  void* harr = nullptr;              // pinned host copy of the arrays
   int mode = 0;                      // 0: device-resident steps, 1: host executor
   int dev_tries = 1;                 // CGS passes per side held by a device chain
   // dev_tries adapts: a chain that ran out of passes (a second CGS pass was

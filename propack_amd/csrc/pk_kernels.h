@@ -75,10 +75,12 @@ int launch_axpy_norm_dot(hipStream_t s, long long L, T* w, const T* v, Scal c, d
 template <typename T>
 int launch_norm(hipStream_t s, long long L, const T* x, double* part);
 
-// x[0:n) = xLARNV(IDIST=2) from the fixed seed (1,3,5,7), generated on the
+// x[0:n) = elements first .. first+n-1 of the xLARNV(IDIST=2) sequence from
+// the fixed seed (1,3,5,7) (a DGETU0 call's try t draws elements
+// (t-1)*n .. t*n-1: xLARNV advances its seed by n), generated on the
 // device; bit-identical to the host restatement for T = double and float.
 template <typename T>
-int launch_larnv_pm1(hipStream_t s, long long n, T* x);
+int launch_larnv_pm1(hipStream_t s, long long n, T* x, long long first = 0);
 
 // dst = src (typed copy with conversion), L elements
 template <typename S, typename D>
@@ -92,8 +94,8 @@ int launch_gemm_mfma(hipStream_t s, long long L, const T* A, long long lda, int 
 
 // Control kernel of the device-resident step (see pk_lbd.h).
 enum CtlKind { CTL_V_SPMV = 0, CTL_V_ELR, CTL_V_REO, CTL_U_SPMV, CTL_U_ELR, CTL_U_REO };
-int launch_ctl(hipStream_t s, LbdCtl* c, int kind, int want_phase, int want_j,
-               const double* part, int nb);
+int launch_ctl(hipStream_t s, LbdCtl* c, void* arrays, int cap, int kind, int want_phase,
+               int want_j, const double* part, int nb);
 
 // dst = sum of the first ncols columns of src (double accumulation).
 template <typename S, typename D>

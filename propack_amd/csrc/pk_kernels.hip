@@ -186,7 +186,6 @@ __global__ __launch_bounds__(kBlock) void k_gemvt(long long L, const T* __restri
 template <typename T>
 int launch_gemvt(hipStream_t s, long long L, const T* V, long long ldv, const int* cols,
                  int ncols, const int* ncols_p, const T* w, double* part, Gate gate) {
-  if (ncols > kMaxCols) return -1;
   const int nblk = cgs_blocks(L);
   hipLaunchKernelGGL(k_gemvt<T>, dim3(nblk), dim3(kBlock), 0, s, L, V, ldv, cols, ncols,
                      ncols_p, w, part, gate);
@@ -213,7 +212,6 @@ template <typename T>
 int launch_gemv_update(hipStream_t s, long long L, const T* V, long long ldv, const int* cols,
                        int ncols, const int* ncols_p, const double* h, T* w, double* part,
                        Gate gate) {
-  if (ncols > kMaxCols) return -1;
   const int nblk = cgs_blocks(L);
   hipLaunchKernelGGL(k_gemv_update<T>, dim3(nblk), dim3(kBlock), 0, s, L, V, ldv, cols, ncols,
                      ncols_p, h, w, part, gate);
@@ -313,20 +311,23 @@ int launch_fill_zero_bytes(hipStream_t s, void* p, size_t bytes) {
 // order as k_reduce), then run the step's control stage (ctl_stage_body).
 // Runs only while the state is in (want_phase, want_j); otherwise the whole
 // chain for this step is idle.
-__global__ __launch_bounds__(kBlock) void k_ctl(LbdCtl* __restrict__ gc, int kind, int want_phase,
-                                                int want_j, const double* __restrict__ part,
-                                                int nb) {
-  __shared__ LbdCtl c;
+__global__ __launch_bounds__(kBlock) void k_ctl(LbdCtl* __restrict__ gc, char* garr, int cap,
+                                                int kind, int want_phase, int want_j,
+                                                const double* __restrict__ part, int nb) {
+  __shared__ LbdCtlLds c;
+  __shared__ LbdCtlRef cg;
   __shared__ double lds[(kBlock / kWave) * 2];
   if (gc->phase != want_phase || gc->j != want_j) return;
   const bool two = (kind == CTL_V_SPMV || kind == CTL_V_ELR || kind == CTL_U_SPMV ||
                     kind == CTL_U_ELR);
-  ctl_stage_body(gc, kind, part, nb, two ? part + nb : nullptr, nb, c, lds);
+  ctl_stage_body(gc, garr, cap, want_j, kind, part, nb, two ? part + nb : nullptr, nb, c, cg,
+                 lds);
 }
 
-int launch_ctl(hipStream_t s, LbdCtl* c, int kind, int want_phase, int want_j,
-               const double* part, int nb) {
-  hipLaunchKernelGGL(k_ctl, dim3(1), dim3(kBlock), 0, s, c, kind, want_phase, want_j, part, nb);
+int launch_ctl(hipStream_t s, LbdCtl* c, void* arrays, int cap, int kind, int want_phase,
+               int want_j, const double* part, int nb) {
+  hipLaunchKernelGGL(k_ctl, dim3(1), dim3(kBlock), 0, s, c, static_cast<char*>(arrays), cap,
+                     kind, want_phase, want_j, part, nb);
   return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 
@@ -407,12 +408,14 @@ int launch_gemm_mfma(hipStream_t s, long long L, const T* A, long long lda, int 
 // every store instruction is coalesced); element g uses the state
 // s0 a^(g+1), and stepping by S multiplies the state by a^S.
 template <typename T>
-__global__ __launch_bounds__(kBlock) void k_larnv_pm1(long long n, unsigned long long step,
+__global__ __launch_bounds__(kBlock) void k_larnv_pm1(long long n, unsigned long long first,
+                                                      unsigned long long step,
                                                       T* __restrict__ x) {
   const long long t = static_cast<long long>(blockIdx.x) * kBlock + threadIdx.x;
   const long long S = static_cast<long long>(gridDim.x) * kBlock;
   if (t >= n) return;
-  unsigned long long s = lcg_mulmod(kLcgSeed, lcg_pow(static_cast<unsigned long long>(t) + 1));
+  unsigned long long s =
+      lcg_mulmod(kLcgSeed, lcg_pow(first + static_cast<unsigned long long>(t) + 1));
   for (long long g = t; g < n; g += S) {
     x[g] = lcg_pm1<T>(s);
     s = lcg_mulmod(s, step);
@@ -420,15 +423,16 @@ __global__ __launch_bounds__(kBlock) void k_larnv_pm1(long long n, unsigned long
 }
 
 template <typename T>
-int launch_larnv_pm1(hipStream_t s, long long n, T* x) {
+int launch_larnv_pm1(hipStream_t s, long long n, T* x, long long first) {
   if (n <= 0) return 0;
   const int nblk = static_cast<int>(std::min<long long>(cdiv(n, kBlock), 2048));
   const unsigned long long step = lcg_pow(static_cast<unsigned long long>(nblk) * kBlock);
-  hipLaunchKernelGGL(k_larnv_pm1<T>, dim3(nblk), dim3(kBlock), 0, s, n, step, x);
+  hipLaunchKernelGGL(k_larnv_pm1<T>, dim3(nblk), dim3(kBlock), 0, s, n,
+                     static_cast<unsigned long long>(first), step, x);
   return hipGetLastError() == hipSuccess ? 0 : -1;
 }
-template int launch_larnv_pm1<double>(hipStream_t, long long, double*);
-template int launch_larnv_pm1<float>(hipStream_t, long long, float*);
+template int launch_larnv_pm1<double>(hipStream_t, long long, double*, long long);
+template int launch_larnv_pm1<float>(hipStream_t, long long, float*, long long);
 
 // ------------------------------------------------------------ seeding ----
 // dst = sum_{c < ncols} src(:, c), accumulated in double (start vector of an

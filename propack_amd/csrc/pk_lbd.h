@@ -24,7 +24,10 @@ constexpr int kMaxCols = 512;
 
 namespace pk {
 
-constexpr int kMaxK = kMaxCols;  // largest Lanczos dimension of the engine
+// Largest step j whose arrays the control kernels stage in LDS; later steps
+// (Lanczos dimensions above it) run the same control code on the arrays in
+// device memory.
+constexpr int kCtlLdsJ = 500;
 
 // An omega-recurrence update a control stage left to run before its
 // remainder (the control kernels spread it over the workgroup).
@@ -48,8 +51,10 @@ enum Phase : int {
   PH_DONE = 100
 };
 
-// 16-byte aligned, size a multiple of 16: control kernels stage it in LDS
-// with 16-byte loads.
+// Scalar state of a DLANBPRO call.  16-byte aligned, size a multiple of 16:
+// control kernels stage it in LDS with 16-byte loads.  The arrays of the
+// call (mu, nu, B, INDEX, the CGS column list) live in a separate buffer
+// sized for the engine's Lanczos dimension (CtlArrays).
 struct alignas(16) LbdCtl {
   // parameters of the call
   int m, n, k, elr;
@@ -65,11 +70,51 @@ struct alignas(16) LbdCtl {
   double vscale_f, uscale_f;
   long long nopx, nreorth, ndot, ncols_touched, nretry;
   long long npass_v, npass_u, ncols_v, ncols_u;  // CGS passes and columns streamed
+  long long nwarn;    // steps that set IERR = j > 0 (near invariance, :406-409,536-539)
   int pending;        // an omega update the step still has to run (Pending)
-  // 1-based arrays
-  double mu[kMaxK + 3], nu[kMaxK + 3], B1[kMaxK + 2], B2[kMaxK + 2];
-  int index[2 * kMaxK + 4];
-  int cols[kMaxCols];  // 0-based column list of the pending CGS
+  int cap;            // capacity of the arrays: the largest k of the engine
+};
+
+// Layout of the array buffer of capacity cap (1-based arrays, element 0
+// unused): mu, nu, B1 = B(:,1), B2 = B(:,2) as doubles, then INDEX and the
+// 0-based CGS column list as ints; every segment starts on 16 bytes.
+struct CtlArrays {
+  __host__ __device__ static int dlen(int cap) { return (cap + 4) & ~1; }      // >= cap + 3
+  __host__ __device__ static int ilen(int cap) { return (2 * cap + 8) & ~3; }  // >= 2 cap + 5
+  __host__ __device__ static int clen(int cap) { return (cap + 4) & ~3; }      // >= cap + 1
+  __host__ __device__ static long long bytes(int cap) {
+    return 32LL * dlen(cap) + 4LL * (ilen(cap) + clen(cap));
+  }
+  __host__ __device__ static double* mu(void* b, int cap) { return static_cast<double*>(b); }
+  __host__ __device__ static double* nu(void* b, int cap) {
+    return static_cast<double*>(b) + dlen(cap);
+  }
+  __host__ __device__ static double* B1(void* b, int cap) {
+    return static_cast<double*>(b) + 2 * dlen(cap);
+  }
+  __host__ __device__ static double* B2(void* b, int cap) {
+    return static_cast<double*>(b) + 3 * dlen(cap);
+  }
+  __host__ __device__ static int* index(void* b, int cap) {
+    return reinterpret_cast<int*>(static_cast<double*>(b) + 4 * dlen(cap));
+  }
+  __host__ __device__ static int* cols(void* b, int cap) { return index(b, cap) + ilen(cap); }
+};
+
+// The state the control functions work on: the scalars plus pointers to
+// the arrays (host memory for the host executor, device memory for the
+// control kernels past the LDS window).
+struct LbdCtlRef : LbdCtl {
+  double *mu, *nu, *B1, *B2;
+  int *index, *cols;
+  __host__ __device__ void bind(void* arrays) {
+    mu = CtlArrays::mu(arrays, cap);
+    nu = CtlArrays::nu(arrays, cap);
+    B1 = CtlArrays::B1(arrays, cap);
+    B2 = CtlArrays::B2(arrays, cap);
+    index = CtlArrays::index(arrays, cap);
+    cols = CtlArrays::cols(arrays, cap);
+  }
 };
 
 // ---- scalar helpers (double/dlanbpro.F:555-723, LAPACK DLAPY2) ----
@@ -161,27 +206,25 @@ __host__ __device__ inline void hd_update_nu(double* numax, const double* mu, do
 }
 
 // Column list of INDEX's intervals for a CGS against columns 1..k
-// (dcgs, dreorth.F:158-206); nd = sum of interval lengths (ndot).
-// Returns false when the list would exceed kMaxCols.
-__host__ __device__ inline bool hd_build_cols(LbdCtl& c, int k) {
+// (dcgs, dreorth.F:158-206); nd = sum of interval lengths (ndot).  The list
+// holds at most k <= cap columns.
+template <class C>
+__host__ __device__ inline void hd_build_cols(C& c, int k) {
   int nc = 0, nd = 0;
   for (int i = 1; c.index[i] <= k && c.index[i] > 0; i += 2) {
     const int p = c.index[i], q = c.index[i + 1];
     nd += q - p + 1;
-    for (int col = p; col <= q; ++col) {
-      if (nc >= kMaxCols) return false;
-      c.cols[nc++] = col - 1;
-    }
+    for (int col = p; col <= q; ++col) c.cols[nc++] = col - 1;
   }
   c.reo_k = k;
   c.reo_ncols = nc;
   c.reo_nd = nd;
   c.reo_try = 0;
-  return true;
 }
 
 // ---------------------------------------------------------------- V side --
-__host__ __device__ inline void ctl_v_post(LbdCtl& c) {
+template <class C>
+__host__ __device__ inline void ctl_v_post(C& c) {
 #pragma clang fp contract(off)
   const int j = c.j;
   if (c.alpha < c.anorm * c.epsn && j < c.k) {
@@ -189,6 +232,7 @@ __host__ __device__ inline void ctl_v_post(LbdCtl& c) {
     return;
   } else if (j > 1 && !c.full_reorth && j < c.k && (c.delta * c.alpha < c.anorm * c.eps)) {
     c.ierr = j;
+    c.nwarn++;
   }
   c.B1[j] = c.alpha;
   c.vscale = 0;
@@ -204,7 +248,8 @@ __host__ __device__ inline void ctl_v_post(LbdCtl& c) {
   c.phase = PH_U_SPMV;
 }
 
-__host__ __device__ inline void ctl_v_after_reorth(LbdCtl& c) {
+template <class C>
+__host__ __device__ inline void ctl_v_after_reorth(C& c) {
   hd_set_mu(c.j - 1, c.nu, c.index, c.eps);
   c.numax = c.eta;
   c.force_reorth = !c.force_reorth;
@@ -212,7 +257,8 @@ __host__ __device__ inline void ctl_v_after_reorth(LbdCtl& c) {
 }
 
 // Remainder of the nu step after the omega update (dlanbpro.F:321-367).
-__host__ __device__ inline void ctl_v_nu_post(LbdCtl& c) {
+template <class C>
+__host__ __device__ inline void ctl_v_nu_post(C& c) {
   const int j = c.j;
   if ((c.full_reorth || c.numax > c.delta || c.force_reorth) && c.alpha != 0.0) {
     if (c.full_reorth || c.eta == 0.0) {
@@ -235,7 +281,8 @@ __host__ __device__ inline void ctl_v_nu_post(LbdCtl& c) {
 }
 
 // nu step (dlanbpro.F:318-320): the omega update is left pending.
-__host__ __device__ inline void ctl_v_nu(LbdCtl& c) {
+template <class C>
+__host__ __device__ inline void ctl_v_nu(C& c) {
   if (!c.full_reorth && c.alpha != 0.0) {
     c.pending = PEND_NU;
     return;
@@ -243,7 +290,8 @@ __host__ __device__ inline void ctl_v_nu(LbdCtl& c) {
   ctl_v_nu_post(c);
 }
 
-__host__ __device__ inline void ctl_v_fin(LbdCtl& c) {
+template <class C>
+__host__ __device__ inline void ctl_v_fin(C& c) {
 #pragma clang fp contract(off)
   const int j = c.j;
   const double* B1 = c.B1;
@@ -264,7 +312,8 @@ __host__ __device__ inline void ctl_v_fin(LbdCtl& c) {
 }
 
 // After the fused SpMV A^T u_j - beta v_{j-1}: s2 = ||v_j||^2, sd = v_{j-1}.v_j.
-__host__ __device__ inline void ctl_v_spmv_s(LbdCtl& c, double s2, double sd) {
+template <class C>
+__host__ __device__ inline void ctl_v_spmv_s(C& c, double s2, double sd) {
 #pragma clang fp contract(off)
   c.nopx++;
   c.uscale = 0;
@@ -284,7 +333,8 @@ __host__ __device__ inline void ctl_v_spmv_s(LbdCtl& c, double s2, double sd) {
 }
 
 // After v_j -= elr_s v_{j-1}: ns2 = ||v_j||^2, nsd = v_{j-1}.v_j.
-__host__ __device__ inline void ctl_v_elr_s(LbdCtl& c, double ns2, double nsd) {
+template <class C>
+__host__ __device__ inline void ctl_v_elr_s(C& c, double ns2, double nsd) {
 #pragma clang fp contract(off)
   const double s = c.elr_s;
   if (c.beta != 0.0) {
@@ -304,7 +354,8 @@ __host__ __device__ inline void ctl_v_elr_s(LbdCtl& c, double ns2, double nsd) {
 }
 
 // After one CGS pass of v_j: s2 = ||v_j||^2 (dreorth.F:85-99).
-__host__ __device__ inline void ctl_v_reo_s(LbdCtl& c, double s2) {
+template <class C>
+__host__ __device__ inline void ctl_v_reo_s(C& c, double s2) {
 #pragma clang fp contract(off)
   c.ndot += c.reo_nd + c.reo_k;
   c.ncols_touched += c.reo_ncols;
@@ -325,7 +376,8 @@ __host__ __device__ inline void ctl_v_reo_s(LbdCtl& c, double s2) {
 }
 
 // ---------------------------------------------------------------- U side --
-__host__ __device__ inline void ctl_u_post(LbdCtl& c) {
+template <class C>
+__host__ __device__ inline void ctl_u_post(C& c) {
 #pragma clang fp contract(off)
   const int j = c.j;
   if (c.beta < c.anorm * c.epsn && j < c.k) {
@@ -333,6 +385,7 @@ __host__ __device__ inline void ctl_u_post(LbdCtl& c) {
     return;
   } else if (!c.full_reorth && j < c.k && (c.delta * c.beta < c.anorm * c.eps)) {
     c.ierr = j;
+    c.nwarn++;
   }
   c.B2[j] = c.beta;
   c.uscale = 0;
@@ -350,7 +403,8 @@ __host__ __device__ inline void ctl_u_post(LbdCtl& c) {
   c.phase = c.j > c.k ? PH_DONE : PH_V_SPMV;
 }
 
-__host__ __device__ inline void ctl_u_after_reorth(LbdCtl& c) {
+template <class C>
+__host__ __device__ inline void ctl_u_after_reorth(C& c) {
   hd_set_mu(c.j, c.mu, c.index, c.eps);
   c.mumax = c.eta;
   c.force_reorth = !c.force_reorth;
@@ -358,7 +412,8 @@ __host__ __device__ inline void ctl_u_after_reorth(LbdCtl& c) {
 }
 
 // Remainder of the mu step after the omega update (dlanbpro.F:449-498).
-__host__ __device__ inline void ctl_u_mu_post(LbdCtl& c) {
+template <class C>
+__host__ __device__ inline void ctl_u_mu_post(C& c) {
   const int j = c.j;
   if ((c.full_reorth || c.mumax > c.delta || c.force_reorth) && c.beta != 0.0) {
     if (c.full_reorth || c.eta == 0.0) {
@@ -383,7 +438,8 @@ __host__ __device__ inline void ctl_u_mu_post(LbdCtl& c) {
 }
 
 // mu step (dlanbpro.F:446-448): the omega update is left pending.
-__host__ __device__ inline void ctl_u_mu(LbdCtl& c) {
+template <class C>
+__host__ __device__ inline void ctl_u_mu(C& c) {
   if (!c.full_reorth && c.beta != 0.0) {
     c.pending = PEND_MU;
     return;
@@ -391,7 +447,8 @@ __host__ __device__ inline void ctl_u_mu(LbdCtl& c) {
   ctl_u_mu_post(c);
 }
 
-__host__ __device__ inline void ctl_u_fin(LbdCtl& c) {
+template <class C>
+__host__ __device__ inline void ctl_u_fin(C& c) {
 #pragma clang fp contract(off)
   const int j = c.j;
   c.B2[j] = c.beta;
@@ -407,7 +464,8 @@ __host__ __device__ inline void ctl_u_fin(LbdCtl& c) {
   ctl_u_mu(c);
 }
 
-__host__ __device__ inline void ctl_u_spmv_s(LbdCtl& c, double s2, double sd) {
+template <class C>
+__host__ __device__ inline void ctl_u_spmv_s(C& c, double s2, double sd) {
 #pragma clang fp contract(off)
   c.nopx++;
   c.vscale = 0;
@@ -421,7 +479,8 @@ __host__ __device__ inline void ctl_u_spmv_s(LbdCtl& c, double s2, double sd) {
   ctl_u_fin(c);
 }
 
-__host__ __device__ inline void ctl_u_elr_s(LbdCtl& c, double ns2, double nsd) {
+template <class C>
+__host__ __device__ inline void ctl_u_elr_s(C& c, double ns2, double nsd) {
 #pragma clang fp contract(off)
   const double s = c.elr_s;
   if (c.alpha != 0.0) {
@@ -440,7 +499,8 @@ __host__ __device__ inline void ctl_u_elr_s(LbdCtl& c, double ns2, double nsd) {
   }
 }
 
-__host__ __device__ inline void ctl_u_reo_s(LbdCtl& c, double s2) {
+template <class C>
+__host__ __device__ inline void ctl_u_reo_s(C& c, double s2) {
 #pragma clang fp contract(off)
   c.ndot += c.reo_nd + c.reo_k;
   c.ncols_touched += c.reo_ncols;
@@ -461,7 +521,8 @@ __host__ __device__ inline void ctl_u_reo_s(LbdCtl& c, double s2) {
 }
 
 // Runs a pending omega update serially, then the rest of its step.
-__host__ __device__ inline void ctl_resume(LbdCtl& c) {
+template <class C>
+__host__ __device__ inline void ctl_resume(C& c) {
   if (c.pending == PEND_NU) {
     c.pending = PEND_NONE;
     hd_update_nu(&c.numax, c.mu, c.nu, c.j, c.B1, c.B2, c.anorm, c.epsn2);
@@ -476,29 +537,35 @@ __host__ __device__ inline void ctl_resume(LbdCtl& c) {
 // Control stages of one step.  The *_s forms stop at a pending omega update
 // (the control kernels run it in parallel); these complete the stage.
 // After the fused SpMV A^T u_j - beta v_{j-1}: s2 = ||v_j||^2, sd = v_{j-1}.v_j.
-__host__ __device__ inline void ctl_v_spmv(LbdCtl& c, double s2, double sd) {
+template <class C>
+__host__ __device__ inline void ctl_v_spmv(C& c, double s2, double sd) {
   ctl_v_spmv_s(c, s2, sd);
   ctl_resume(c);
 }
 // After v_j -= elr_s v_{j-1}: ns2 = ||v_j||^2, nsd = v_{j-1}.v_j.
-__host__ __device__ inline void ctl_v_elr(LbdCtl& c, double ns2, double nsd) {
+template <class C>
+__host__ __device__ inline void ctl_v_elr(C& c, double ns2, double nsd) {
   ctl_v_elr_s(c, ns2, nsd);
   ctl_resume(c);
 }
 // After one CGS pass of v_j: s2 = ||v_j||^2.
-__host__ __device__ inline void ctl_v_reo(LbdCtl& c, double s2) {
+template <class C>
+__host__ __device__ inline void ctl_v_reo(C& c, double s2) {
   ctl_v_reo_s(c, s2);
   ctl_resume(c);
 }
-__host__ __device__ inline void ctl_u_spmv(LbdCtl& c, double s2, double sd) {
+template <class C>
+__host__ __device__ inline void ctl_u_spmv(C& c, double s2, double sd) {
   ctl_u_spmv_s(c, s2, sd);
   ctl_resume(c);
 }
-__host__ __device__ inline void ctl_u_elr(LbdCtl& c, double ns2, double nsd) {
+template <class C>
+__host__ __device__ inline void ctl_u_elr(C& c, double ns2, double nsd) {
   ctl_u_elr_s(c, ns2, nsd);
   ctl_resume(c);
 }
-__host__ __device__ inline void ctl_u_reo(LbdCtl& c, double s2) {
+template <class C>
+__host__ __device__ inline void ctl_u_reo(C& c, double s2) {
   ctl_u_reo_s(c, s2);
   ctl_resume(c);
 }

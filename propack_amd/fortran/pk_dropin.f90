@@ -89,10 +89,14 @@ subroutine dlansvd(jobu, jobv, m, n, k, kmax, aprod, U, ldu, sigma, bnd, V, ldv,
     call lansvd_core(jobu, jobv, m, n, kk, lanmax, eng, 2d0**(-53), sigma, bnd, tolin, &
                      doption, ioption, info)
     k = kk
-    if (info > -100 .and. lower1(jobu) == 'y' .and. k > 0) &
+    if (info > -100 .and. lower1(jobu) == 'y' .and. k > 0) then
       rc = pk_lbd_get(eng, 'u', k, U, ldu, PK_F64)
-    if (info > -100 .and. lower1(jobv) == 'y' .and. k > 0) &
+      if (rc /= PK_OK) info = rc
+    end if
+    if (info > -100 .and. lower1(jobv) == 'y' .and. k > 0) then
       rc = pk_lbd_get(eng, 'v', k, V, ldv, PK_F64)
+      if (rc /= PK_OK) info = rc
+    end if
   end if
   rc = pk_lbd_destroy(eng)
 end subroutine
@@ -137,10 +141,14 @@ subroutine dlansvd_irl(which, jobu, jobv, m, n, dim, p, neig, maxiter, aprod, U,
     call lansvd_irl_core(which, jobu, jobv, m, n, dim, p, ne, maxiter, eng, 2d0**(-53), &
                          sigma, bnd, tolin, doption, ioption, info)
     neig = ne
-    if (info > -100 .and. lower1(jobu) == 'y' .and. neig > 0) &
+    if (info > -100 .and. lower1(jobu) == 'y' .and. neig > 0) then
       rc = pk_lbd_get(eng, 'u', neig, U, ldu, PK_F64)
-    if (info > -100 .and. lower1(jobv) == 'y' .and. neig > 0) &
+      if (rc /= PK_OK) info = rc
+    end if
+    if (info > -100 .and. lower1(jobv) == 'y' .and. neig > 0) then
       rc = pk_lbd_get(eng, 'v', neig, V, ldv, PK_F64)
+      if (rc /= PK_OK) info = rc
+    end if
   end if
   rc = pk_lbd_destroy(eng)
 end subroutine
@@ -195,10 +203,14 @@ subroutine slansvd(jobu, jobv, m, n, k, kmax, aprod, U, ldu, sigma, bnd, V, ldv,
     sigma(1:k0) = real(sd)
     bnd(1:k0) = real(bd)
     soption(3) = real(dopt(3))
-    if (info > -100 .and. lower1(jobu) == 'y' .and. k > 0) &
+    if (info > -100 .and. lower1(jobu) == 'y' .and. k > 0) then
       rc = pk_lbd_get(eng, 'u', k, U, ldu, PK_F32)
-    if (info > -100 .and. lower1(jobv) == 'y' .and. k > 0) &
+      if (rc /= PK_OK) info = rc
+    end if
+    if (info > -100 .and. lower1(jobv) == 'y' .and. k > 0) then
       rc = pk_lbd_get(eng, 'v', k, V, ldv, PK_F32)
+      if (rc /= PK_OK) info = rc
+    end if
   end if
   rc = pk_lbd_destroy(eng)
 end subroutine
@@ -253,10 +265,14 @@ subroutine slansvd_irl(which, jobu, jobv, m, n, dim, p, neig, maxiter, aprod, U,
     sigma(1:n0) = real(sd)
     bnd(1:n0) = real(bd)
     soption(3) = real(dopt(3))
-    if (info > -100 .and. lower1(jobu) == 'y' .and. neig > 0) &
+    if (info > -100 .and. lower1(jobu) == 'y' .and. neig > 0) then
       rc = pk_lbd_get(eng, 'u', neig, U, ldu, PK_F32)
-    if (info > -100 .and. lower1(jobv) == 'y' .and. neig > 0) &
+      if (rc /= PK_OK) info = rc
+    end if
+    if (info > -100 .and. lower1(jobv) == 'y' .and. neig > 0) then
       rc = pk_lbd_get(eng, 'v', neig, V, ldv, PK_F32)
+      if (rc /= PK_OK) info = rc
+    end if
   end if
   rc = pk_lbd_destroy(eng)
 end subroutine

@@ -88,8 +88,11 @@ extern "C" int lbd_cpu_lanbpro(int m, int n, const int* rp, const int* ci, const
                                double* V, int k, double* alpha, double* beta, double* rnorm,
                                double delta, double eta, int elr, long long* counters) {
   Csr A{m, n, rp, ci, rpt, cit, va, vat};
-  static LbdCtl c;
-  std::memset(&c, 0, sizeof c);
+  LbdCtlRef c;
+  std::memset(static_cast<void*>(&c), 0, sizeof c);
+  c.cap = k;
+  std::vector<double> arrays(static_cast<size_t>(CtlArrays::bytes(k) / 8 + 1), 0.0);
+  c.bind(arrays.data());
   const double eps = 0x1p-53;
   const int mx = m > n ? m : n;
   auto ucol = [&](int j) { return U + static_cast<long long>(j - 1) * m; };
