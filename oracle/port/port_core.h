@@ -98,11 +98,14 @@ void P(csr_aprod)(char trans, const R *x, R *y, void *ctx)
  * output (0-based) is 2*((s0 * a^(g+1) mod 2^48) / 2^48) - 1 with
  * s0 = 1*2^36 + 3*2^24 + 5*2^12 + 7.  The 48-bit value is converted the
  * way dlaruv.f does, from four 12-bit limbs in Horner form. */
-void P(larnv_pm1)(int n, R *x)
+static void larnv_pm1_from(long long first, int n, R *x)
 {
     const uint64_t mask = (1ULL << 48) - 1, a = 33952834046453ULL;
     uint64_t s = (1ULL << 36) + (3ULL << 24) + (5ULL << 12) + 7ULL;
     const R r = (R)1 / (R)4096;
+    /* skip the first `first` elements: xLARNV advances its seed by the
+     * count it generates (dlaruv.f sets ISEED to the last state) */
+    for (long long g = 0; g < first; ++g) s = (uint64_t)(((unsigned __int128)s * a) & mask);
     for (int g = 0; g < n; ++g) {
         s = (uint64_t)(((unsigned __int128)s * a) & mask);
         R i1 = (R)(s >> 36), i2 = (R)((s >> 24) & 4095);
@@ -111,6 +114,8 @@ void P(larnv_pm1)(int n, R *x)
         x[g] = (R)2 * u - (R)1;
     }
 }
+
+void P(larnv_pm1)(int n, R *x) { larnv_pm1_from(0, n, x); }
 
 /* DLAPY2 (double/Lapack_Util/dlapy2.f): sqrt(x^2+y^2) without overflow. */
 R P(lapy2)(R x, R y)
@@ -473,8 +478,10 @@ static int getu0(char transa, int m, int n, int j, int ntry, R *u0, R *u0norm,
     int ierr = -1;
     /* kappa here is the single-precision literal 0.717 (dgetu0.F:28-29). */
     const R kap = (R)0.717f;
+    /* the seed is set once per call (dgetu0.F:47-50); each try's DLARNV
+     * continues the sequence (dgetu0.F:67) */
     for (int itry = 1; itry <= ntry; ++itry) {
-        P(larnv_pm1)(rsize, work);
+        larnv_pm1_from((long long)(itry - 1) * rsize, rsize, work);
         R nrm = nrm2(rsize, work);
         aprod(tr ? 'n' : 't', work, u0, ctx);
         st.nopx++;

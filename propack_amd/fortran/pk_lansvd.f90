@@ -413,7 +413,14 @@ contains
             st%sig(st%nconv) = a1(i)
             st%bd(st%nconv) = wb(i)
             i = i + 1
+          else if (i <= k) then
+            ! the reference continues the count after the kept Ritz values
+            ! (i = k+1, double/dlansvd_irl.F:285-286); it only matters when
+            ! neig > dim - p
+            i = k + 1
           else
+            ! an unconverged value past k: the reference sets i = k+1 again
+            ! and never leaves the loop; the count stops here instead
             exit
           end if
         end do
