@@ -26,15 +26,18 @@ def _one_thread_reference():
     refpy.set_threads(8)
 
 
-def _same(r, ref, sig_rtol=1e-10):
+def _same(r, ref, sig_rtol=1e-10, ndot_rtol=0.0):
+    """Singular values within sig_rtol * sigma_1 (the absolute accuracy
+    Lanczos gives every singular value, noise-level ones included)."""
     print("gpu", r.info, r.k, r.anorm, r.counters)
     print("ref", ref.info, ref.k, ref.anorm, ref.counters)
     assert r.info == ref.info
     assert r.k == ref.k
     if ref.k:
-        np.testing.assert_allclose(r.sigma, ref.sigma, rtol=sig_rtol)
-    for key in ("nopx", "nreorth", "ndot"):
+        assert np.abs(r.sigma - ref.sigma).max() <= sig_rtol * ref.sigma.max()
+    for key in ("nopx", "nreorth"):
         assert r.counters[key] == ref.counters[key], key
+    assert abs(r.counters["ndot"] - ref.counters["ndot"]) <= ndot_rtol * ref.counters["ndot"]
     assert abs(r.anorm - ref.anorm) <= 1e-12 * abs(ref.anorm)
 
 
@@ -97,11 +100,16 @@ def test_five_failed_cgs_passes():
 
 
 def test_near_invariance_warning():
+    """Past the rank the Lanczos vectors are ~1e-9 ||A|| residues, and
+    whether a CGS pass keeps kappa = 0.717 of the norm is decided at the
+    rounding level: the reference projects each INDEX interval in turn, the
+    engine all intervals in one block (equal in exact arithmetic), so the
+    number of second passes, and ndot with it, may differ (10 %)."""
     A, D, u0 = _near_invariant()
     op = api.CsrOperator(A)
     r = api.lansvd(op, A.m, A.n, 20, 60, u0=u0, tol=1e-12)
     ref = refpy.lansvd(A, 20, 60, u0=u0, tol=1e-12, cgs=1)
-    _same(r, ref)
+    _same(r, ref, ndot_rtol=0.1)
     eng = api.Engine(op, kmax=60)
     eng.put_u1(u0)
     rd = api.lansvd_dev(eng, 20, 60, tol=1e-12)
