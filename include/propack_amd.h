@@ -133,6 +133,19 @@ int pk_csr_read(const char* path, int val_prec, pk_csr_t* op);
 /* Store the handle in IPARM(1:2) so that passing pk_dcsr_aprod_ (or
  * pk_scsr_aprod_) as APROD selects the device operator in the drivers. */
 int pk_csr_to_iparm(pk_csr_t op, int* iparm);
+/* y = op(A) x with no allocation of any kind (no reference counterpart: it
+ * is for callers that cannot allocate -- code under HIP graph capture,
+ * tight serving loops, harnesses that forbid hipMalloc).  x and y are
+ * device memory in the operator's precision; the work is enqueued on
+ * `stream` (a hipStream_t; NULL for the default stream) and the call
+ * returns without waiting.  Partial sums, multi-panel running sums and
+ * long-row sums go to a fixed 48 MiB device array per GPU, set up on the
+ * first call on each device; an operator needing more (about 6M rows in a
+ * multi-panel direction) is refused with PK_ERR_NOTSUP.  Results are
+ * bit-identical to pk_csr_spmv's.  Calls that overlap on one GPU share that
+ * array and race: order them on one stream. */
+int pk_csr_spmv_noalloc(pk_csr_t op, char trans, const void* x, void* y, int prec,
+                        void* stream);
 
 /* The library's own APROD with the reference callback signature
  * (double/dlanbpro.F:28-41; hidden CHARACTER length last).  Passed to

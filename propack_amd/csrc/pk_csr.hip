@@ -12,6 +12,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <mutex>
 #include <vector>
 
 #include "pk_internal.h"
@@ -774,6 +775,53 @@ int time_apply(const CsrOp* op, char trans, int reps, double* us) {
 
 }  // namespace
 
+// ------------------------------------------------------ no-allocation SpMV --
+// Scratch of pk_csr_spmv_noalloc: the SpMV's partial sums, its running row
+// sums between column panels and its long-row chunk sums, laid out as
+// [part | yacc | lsum] (doubles).  A file-scope device array: every GPU the
+// library runs on holds its own copy (48 MiB) until the process ends.
+constexpr size_t kNoAllocDoubles = size_t{6} << 20;
+static __device__ __align__(256) double scratch[kNoAllocDoubles];
+
+namespace {
+
+constexpr int kNoAllocDevices = 64;
+std::mutex g_noalloc_lock;
+double* g_noalloc_ptr[kNoAllocDevices] = {};
+
+// Device address of `scratch` on the current device, looked up on the
+// first call on that device (with it current) and kept per device.  In host
+// code the array's name is only a host-side stand-in: kernels get this
+// address, never the name.
+int noalloc_scratch(double** out) {
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= kNoAllocDevices) {
+    set_error("pk_csr_spmv_noalloc: no usable current device (%d)", dev);
+    return PK_ERR_HIP;
+  }
+  std::lock_guard<std::mutex> lk(g_noalloc_lock);
+  if (!g_noalloc_ptr[dev]) {
+    void* ptr = nullptr;
+    const hipError_t r = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+    if (r != hipSuccess || !ptr) {
+      set_error("pk_csr_spmv_noalloc: hipGetSymbolAddress of the scratch array failed on "
+                "device %d: %s", dev, hipGetErrorString(r));
+      return PK_ERR_HIP;
+    }
+    g_noalloc_ptr[dev] = static_cast<double*>(ptr);
+  }
+  *out = g_noalloc_ptr[dev];
+  return PK_OK;
+}
+
+bool dir_has_long_rows(const CsrOp::Dir& D) {
+  for (int p = 0; p < D.np; ++p)
+    if ((D.sort_cap > 0 ? D.pan[p].spart.nlong : D.pan[p].part.nlong) > 0) return true;
+  return false;
+}
+
+}  // namespace
+
 extern "C" {
 
 int pk_csr_create(int m, int n, long long nnz, const int* rowptr, const int* colind,
@@ -968,6 +1016,45 @@ int pk_csr_spmv(pk_csr_t h, char trans, const void* x, void* y, int prec, int wh
   if (prec == PK_F32)
     return spmv_any<float>(op, trans, static_cast<const float*>(x), static_cast<float*>(y), where);
   return PK_ERR_ARG;
+}
+
+int pk_csr_spmv_noalloc(pk_csr_t h, char trans, const void* x, void* y, int prec,
+                        void* stream) {
+  CsrOp* op = as_csr(h);
+  if (!op) return PK_ERR_HANDLE;
+  if (!x || !y || (prec != PK_F64 && prec != PK_F32)) return PK_ERR_ARG;
+  if (op->val_prec != prec) {
+    set_error("pk_csr_spmv_noalloc: vector precision differs from the operator's");
+    return PK_ERR_NOTSUP;
+  }
+  const bool tr = (trans == 't' || trans == 'T');
+  const CsrOp::Dir& D = tr ? op->dt : op->da;
+  const size_t npart =
+      2 * static_cast<size_t>(std::max(std::max(op->nrb, op->nrbt), kMaxBlocks));
+  const size_t nyacc = D.np > 1 ? static_cast<size_t>(D.rows) : 0;
+  const size_t nlong = dir_has_long_rows(D) ? static_cast<size_t>(csr_lwork(op)) : 0;
+  if (npart + nyacc + nlong > kNoAllocDoubles) {
+    set_error("pk_csr_spmv_noalloc: this operator needs %zu doubles of scratch (%zu partial "
+              "sums, %zu running row sums, %zu long-row sums); the fixed scratch holds %zu",
+              npart + nyacc + nlong, npart, nyacc, nlong, kNoAllocDoubles);
+    return PK_ERR_NOTSUP;
+  }
+  double* base = nullptr;
+  const int rc = noalloc_scratch(&base);
+  if (rc != PK_OK) return rc;
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  const int nb =
+      prec == PK_F64
+          ? csr_apply<double>(op, s, trans, static_cast<const double*>(x), static_cast<double*>(y),
+                              nullptr, sval(0.0), base, Gate{}, base + npart,
+                              base + npart + nyacc)
+          : csr_apply<float>(op, s, trans, static_cast<const float*>(x), static_cast<float*>(y),
+                             nullptr, sval(0.0), base, Gate{}, base + npart, base + npart + nyacc);
+  if (nb < 0) {
+    set_error("pk_csr_spmv_noalloc: kernel launch failed");
+    return PK_ERR_HIP;
+  }
+  return PK_OK;
 }
 
 int pk_csr_to_iparm(pk_csr_t h, int* iparm) {
