@@ -46,6 +46,7 @@ os.environ.setdefault("OMP_STACKSIZE", "512M")
 from propack_amd import api, synth  # noqa: E402
 
 HBM_PEAK_GBS = 8000.0      # MI355X HBM3E
+FP64_PEAK_TFLOPS = 78.6    # MI355X dense FP64 matrix (MFMA)
 FP64_PEAK_TFLOPS = 78.6    # MI355X dense FP64 MFMA
 
 M, N, LAM, DECAY, SEED = 2_000_000, 1_000_000, 16.0, 0.6, 2024
@@ -326,6 +327,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-check", action="store_true")
     ap.add_argument("--no-mixed", action="store_true")
+    ap.add_argument("--no-gemm", action="store_true",
+                    help="skip the restart-GEMM timing at the headline shape")
     ap.add_argument("--no-tune", action="store_true",
                     help="built-in SpMV column panels instead of the autotuner's")
     ap.add_argument("--panels", default=None,
@@ -434,6 +437,10 @@ def main():
                    "GBps": round(v["bytes"] / (v["ms"] * 1e-3) / 1e9, 1) if v["ms"] > 0 else None,
                    "bytes_per_launch": v["bytes"] / v["launches"] if v["launches"] else 0}
                for c, v in prof_all.items()}
+    g = prof_all["gemm"]
+    if g["ms"] > 0:
+        kernels["gemm"]["TFLOPs"] = round(g["flops"] / (g["ms"] * 1e-3) / 1e12, 2)
+        kernels["gemm"]["mfma_util"] = round(g["flops"] / (g["ms"] * 1e-3) / 1e12 / FP64_PEAK_TFLOPS, 4)
 
     # SpMV and reorthogonalization together (the two HBM-bound classes):
     # algorithmic bytes over device time
@@ -506,6 +513,33 @@ def main():
             f"max sigma err {mixed['fp32_phase_max_rel_sigma_err_vs_fp64']}; FP32 phase alone "
             f"{mixed['fp32_phase_s']:.3f}s (x{mixed['fp32_phase_speedup_vs_fp64']:.2f})")
 
+    # The restart GEMM at the headline shape (DGEMM_OVWR_LEFT,
+    # double/dlansvd_irl.F:387): U(:, 1:51) <- U(:, 1:121) Q, 2,000,000 x 121
+    # times 121 x 51, FP64 MFMA; the solve above converges without a restart,
+    # so it is timed here on its own (two untimed launches, then ten timed).
+    restart_gemm = None
+    if not args.no_gemm:
+        Q = np.asfortranarray(np.random.default_rng(5).standard_normal((DIM + 1, NEIG + 1)))
+        upd = lambda: api.lib().pk_lbd_update(  # noqa: E731
+            eng.handle, api.C.c_char(b"u"), api.C.c_char(b"n"), api.C.c_int(DIM + 1),
+            api.C.c_int(NEIG + 1), Q.ctypes.data_as(api.C.c_void_p), api.C.c_int(DIM + 1))
+        for _ in range(2):
+            upd()
+        api.lib().pk_lbd_sync(eng.handle)
+        eng.profile(True, classes=1 << 4)
+        for _ in range(10):
+            upd()
+        pg = eng.profile_get()["gemm"]
+        eng.profile(False)
+        if pg["launches"]:
+            ms = pg["ms"] / pg["launches"]
+            tf = pg["flops"] / (pg["ms"] * 1e-3) / 1e12
+            restart_gemm = {"shape": f"{A.m}x{DIM + 1} * {DIM + 1}x{NEIG + 1}",
+                            "ms_per_launch": round(ms, 4), "TFLOPs": round(tf, 2),
+                            "mfma_util": round(tf / FP64_PEAK_TFLOPS, 4),
+                            "GBps": round(pg["bytes"] / (pg["ms"] * 1e-3) / 1e9, 1)}
+            log(f"[bench] restart GEMM {restart_gemm}")
+
     cpu = None
     if not args.no_cpu:
         log("[bench] cpu baseline (reference PROPACK, bounded sample) ...")
@@ -545,6 +579,7 @@ def main():
         "kernels_timed_in": "one extra solve with every kernel class timed (after the timed solves)",
         "spmv_plus_reorth": combined,
         "fp32_basis_form": mixed,
+        "restart_gemm": restart_gemm,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out))

@@ -480,13 +480,13 @@ int batch_create(int nprob, const pk_lbd_t* handles, Batch** out) {
       rc = PK_ERR_ARG;
       break;
     }
-    // An operator's panel row sums and partial slots are its own scratch:
-    // two problems of one batch run their SpMVs at the same time, so they
-    // may not share an operator (or an engine).
+    // An engine holds one problem's state and scratch (its SpMV running row
+    // sums and long-row sums among them): it may appear once in a batch.
+    // Engines on one operator may: the operator's arrays are only read.
     bool shared = false;
-    for (const Engine* o : B->eng) shared = shared || o == e || o->op == e->op;
+    for (const Engine* o : B->eng) shared = shared || o == e;
     if (shared) {
-      set_error("pk_batch_create: problem %d shares its engine or operator with another", i);
+      set_error("pk_batch_create: problem %d shares its engine with another", i);
       rc = PK_ERR_ARG;
       break;
     }

@@ -20,19 +20,29 @@ def _p(a):
 def _engine(m, n, kmax, prec=api.PK_F64):
     # a tiny operator just to own an engine of the right shape
     A = synth.random_uniform(m, n, 3 * m, seed=5)
+    A = synth.Csr(A.m, A.n, A.rowptr, A.col,
+                  A.val.astype(np.float64 if prec == api.PK_F64 else np.float32))
     op = api.CsrOperator(A, prec=prec)
-    return op, api.Engine(op, kmax=kmax)
+    return op, api.Engine(op, kmax=kmax, prec=prec)
 
 
+# (m, kmax, ncols, transb): every column-tile count of the LDS-staged
+# kernel (ncols 3, 12, 20, 40, 64, 70 = 64 + 6), row counts around its
+# 256-row blocks and below one 16-row tile, k extents up to its LDS limit
+# (kdim 201 and 313) and past it (kdim 321: the unstaged kernel)
 @pytest.mark.parametrize("m,kmax,ncols,transb", [(1000, 37, 20, "n"), (2049, 80, 70, "n"),
-                                                  (777, 30, 12, "t"), (64, 5, 3, "n")])
-def test_basis_update_gemm(m, kmax, ncols, transb):
-    op, eng = _engine(m, 50, kmax)
+                                                  (777, 30, 12, "t"), (64, 5, 3, "n"),
+                                                  (10, 50, 33, "n"), (5000, 200, 64, "n"),
+                                                  (3001, 312, 40, "t"), (2000, 320, 40, "n")])
+@pytest.mark.parametrize("prec", [api.PK_F64, api.PK_F32])
+def test_basis_update_gemm(m, kmax, ncols, transb, prec):
+    op, eng = _engine(m, 50, kmax, prec)
     rng = np.random.default_rng(m)
-    U = rng.standard_normal((m, kmax + 1))
+    dt = np.float64 if prec == api.PK_F64 else np.float32
+    U = rng.standard_normal((m, kmax + 1)).astype(dt)
     for j in range(kmax + 1):
         assert api.lib().pk_lbd_put(eng.handle, C.c_char(b"u"), C.c_int(j + 1),
-                                    _p(np.ascontiguousarray(U[:, j])), C.c_int(0)) == 0
+                                    _p(np.ascontiguousarray(U[:, j])), C.c_int(prec)) == 0
     kdim = kmax + 1
     ncols = min(ncols, kdim)
     B = rng.standard_normal((kdim, ncols))
@@ -41,9 +51,11 @@ def test_basis_update_gemm(m, kmax, ncols, transb):
     rc = api.lib().pk_lbd_update(eng.handle, C.c_char(b"u"), C.c_char(transb.encode()),
                                  C.c_int(kdim), C.c_int(ncols), _p(Bh), C.c_int(ldb))
     assert rc == 0, api.lib().pk_last_error()
-    got = eng.get("u", ncols)
-    want = U @ B
-    assert np.abs(got - want).max() <= 1e-13 * np.abs(want).max() * np.sqrt(kdim)
+    got = eng.get("u", ncols, prec).astype(np.float64)
+    want = U.astype(np.float64) @ B
+    # FP64 accumulation; FP32 storage rounds each result once
+    tol = 1e-13 * np.sqrt(kdim) if prec == api.PK_F64 else 2 ** -23
+    assert np.abs(got - want).max() <= tol * np.abs(want).max()
 
 
 def test_basis_update_chained():
