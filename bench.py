@@ -278,15 +278,24 @@ def workload_batch(args):
     # 1, 0, -1, -2: 474, 529, 546-552, 558)
     lanes = tuple(int(x) for x in args.lanes.split(",")) if args.lanes else (-2, -2)
     sort = tuple(int(x) for x in args.sort.split(",")) if args.sort else (8192, 8192)
+    tuned = []
     for prm in synth.batch_params(256)[: args.count]:
         A = synth.batch_problem(prm, dtype=np.float32)
-        op = api.CsrOperator(A, prec=api.PK_F32, lanes=lanes, sort=sort)
+        if args.tune_each:
+            t = api.autotune(A, prec=api.PK_F32)
+            tuned.append((t.panels, t.lanes, t.sort))
+            op = api.CsrOperator(A, prec=api.PK_F32, panels=t.panels, lanes=t.lanes,
+                                 sort=t.sort)
+        else:
+            op = api.CsrOperator(A, prec=api.PK_F32, lanes=lanes, sort=sort)
         e = api.Engine(op, kmax=48, prec=api.PK_F32)
         e.set_rounding(1)
         # start vectors are inputs: resident in HBM before the timed region
         u0 = torch.from_numpy(synth.start_vector(A.m, prm["seed"] + 7)).to("cuda")
         probs.append((op, e, u0, A.nnz))
         del A
+        if len(probs) % 32 == 0:
+            log(f"[bench] {len(probs)} problems set up ({time.perf_counter() - t0:.1f}s)")
     nnz = sum(p[3] for p in probs)
     log(f"[bench] {len(probs)} problems, {nnz / 1e6:.1f}M nnz in {time.perf_counter() - t0:.1f}s")
     sync = lambda: api.lib().pk_lbd_sync(probs[0][1].handle)  # noqa: E731
@@ -298,6 +307,7 @@ def workload_batch(args):
             out.append(api.lansvd_irl_dev(e, 16, 48, tol=1e-8, jobu="n", jobv="n", fetch=False))
         return out
     b = api.Batch([p[1] for p in probs])
+    b.set_graph(not args.no_graph)
 
     def bat():
         for op, e, u0, _ in probs:
@@ -312,7 +322,9 @@ def workload_batch(args):
            "vs_baseline": None, "dtype": "f32 storage / f64 accumulation",
            "data": "synthetic (seeded generator, matrices resident in HBM)",
            "config": {"workload": f"{len(probs)} problems of batch_params(256)", "nnz": nnz,
-                      "spmv_sort_caps": list(sort), "spmv_lane_shifts": list(lanes)},
+                      "spmv_layout": ("autotuned per problem" if args.tune_each else
+                                      {"sort_caps": list(sort), "lane_shifts": list(lanes)}),
+                      "hip_graphs": not args.no_graph},
            "sequential_problems_per_s": len(probs) / ts, "speedup_vs_sequential": ts / tb,
            "converged": sum(r.k for r in rb), "requested": 16 * len(probs),
            "bit_identical_to_sequential": same}
@@ -345,6 +357,11 @@ def main():
                          "bipartite, k=64, dim=160, tol=1e-8, FP32 basis + FP64 polish vs "
                          "FP64; batch: the 256-problem batched workload vs a per-problem loop")
     ap.add_argument("--count", type=int, default=256, help="problems of --workload batch")
+    ap.add_argument("--no-graph", action="store_true",
+                    help="--workload batch: enqueue the batched step chains directly instead of "
+                         "replaying captured HIP graphs")
+    ap.add_argument("--tune-each", action="store_true",
+                    help="--workload batch: autotune every problem's SpMV layout")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a plain 1 GiB device copy 5 times (the reference "
                          "kernel for scaling FETCH_SIZE/WRITE_SIZE counters) and exit")

@@ -239,6 +239,11 @@ int pk_lbd_eps(pk_lbd_t eng, double* eps, double* leff);
 typedef struct pk_batch_opaque* pk_batch_t;
 int pk_batch_create(int nprob, const pk_lbd_t* engines, pk_batch_t* batch);
 int pk_batch_destroy(pk_batch_t batch);
+/* Step chains of batched calls as HIP graphs (on != 0, the default): each
+ * call's kernel chain is captured once per (step range, ELR and CGS pass
+ * counts) and replayed, up to 8 chains per batch; on = 0 enqueues the
+ * kernels directly.  Same results either way (no reference counterpart). */
+int pk_batch_set_graph(pk_batch_t batch, int on);
 /* pk_lbd_lanbpro for every problem i with active[i] != 0: k0[i] -> k[i]
  * steps; alpha[i], beta[i] its B(:,1), B(:,2); rnorm[i]; doption(4*i+1:
  * 4*i+3) its delta, eta, anorm; ioption shared; ierr[i]. */

@@ -40,7 +40,8 @@ EXPORTS = [
     "pk_batch_destroy", "pk_batch_lanbpro", "pk_lansvd_irl_batch", "pk_matrix_read",
     "pk_matrix_free", "pk_coo_to_csr", "pk_csr_create_coo", "pk_csr_read",
     "pk_csr_create_panels", "pk_csr_panels", "pk_csr_set_lanes", "pk_csr_lanes", "pk_csr_autotune",
-    "pk_csr_set_sort", "pk_csr_sort", "pk_lbd_path_stats",
+    "pk_csr_set_sort", "pk_csr_sort", "pk_lbd_path_stats", "pk_batch_set_graph",
+    "pk_csr_spmv_noalloc",
 ]
 
 _lib = None
@@ -511,6 +512,11 @@ class Batch:
         _check(lib().pk_batch_create(C.c_int(len(self.engines)), arr, C.byref(h)),
                "pk_batch_create")
         self.handle = h
+
+    def set_graph(self, on: bool = True):
+        """Replay each batched call's kernel chain as a captured HIP graph
+        (default) or enqueue it directly (pk_batch_set_graph)."""
+        _check(lib().pk_batch_set_graph(self.handle, C.c_int(int(on))), "pk_batch_set_graph")
 
     def close(self):
         if self.handle:

@@ -294,7 +294,19 @@ struct Batch {
   double* part = nullptr;
   double* red = nullptr;
   size_t part_len = 0;
+  // The kernel chain of a batched call depends only on its step range, the
+  // ELR and CGS pass counts and the tables above (per-problem data reach
+  // the kernels through dprob, rewritten before every launch), so it is
+  // captured once per key into a hipGraph and replayed (a restarted solve
+  // replays the same chain every restart).
+  struct Graph {
+    int j0, j1, elr, tries;
+    hipGraphExec_t exec;
+  };
+  std::vector<Graph> graphs;
+  bool use_graph = true;
 };
+constexpr size_t kMaxBatchGraphs = 8;
 
 namespace {
 
@@ -414,6 +426,7 @@ void enqueue_batch_step(Batch& B, int J, int elr, int tries) {
 
 void batch_free(Batch* b) {
   if (!b) return;
+  for (auto& g : b->graphs) (void)hipGraphExecDestroy(g.exec);
   for (auto& L : b->spmv) {
     (void)hipFree(L.desc);
     (void)hipFree(L.work);
@@ -677,11 +690,42 @@ int batch_lanbpro(Batch& B, const int* active, const int* k0, int* k, double* co
     if (hipMemcpyAsync(B.dprob, B.hprob.data(), sizeof(BProb) * np, hipMemcpyHostToDevice,
                        B.stream) != hipSuccess)
       return PK_ERR_HIP;
-    for (int J = Jmin; J <= Jmax; ++J) {
-      if (B.prec == PK_F64)
-        enqueue_batch_step<double>(B, J, elr, tries);
-      else
-        enqueue_batch_step<float>(B, J, elr, tries);
+    auto enqueue = [&] {
+      for (int J = Jmin; J <= Jmax; ++J) {
+        if (B.prec == PK_F64)
+          enqueue_batch_step<double>(B, J, elr, tries);
+        else
+          enqueue_batch_step<float>(B, J, elr, tries);
+      }
+    };
+    hipGraphExec_t exec = nullptr;
+    if (B.use_graph) {
+      for (const auto& g : B.graphs)
+        if (g.j0 == Jmin && g.j1 == Jmax && g.elr == elr && g.tries == tries) exec = g.exec;
+      if (!exec && B.graphs.size() < kMaxBatchGraphs) {
+        hipGraph_t graph = nullptr;
+        if (hipStreamBeginCapture(B.stream, hipStreamCaptureModeThreadLocal) != hipSuccess)
+          return PK_ERR_HIP;
+        enqueue();
+        const hipError_t ce = hipStreamEndCapture(B.stream, &graph);
+        if (ce != hipSuccess || hipGetLastError() != hipSuccess) {
+          if (graph) (void)hipGraphDestroy(graph);
+          set_error("pk_batch_lanbpro: capture of the batched steps failed");
+          return PK_ERR_HIP;
+        }
+        const hipError_t ie = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(graph);
+        if (ie != hipSuccess) {
+          set_error("pk_batch_lanbpro: hipGraphInstantiate: %s", hipGetErrorString(ie));
+          return PK_ERR_HIP;
+        }
+        B.graphs.push_back(Batch::Graph{Jmin, Jmax, elr, tries, exec});
+      }
+    }
+    if (exec) {
+      if (hipGraphLaunch(exec, B.stream) != hipSuccess) return PK_ERR_HIP;
+    } else {
+      enqueue();
     }
     if (hipGetLastError() != hipSuccess) return PK_ERR_HIP;
     for (int i = 0; i < np; ++i)
@@ -776,6 +820,13 @@ int pk_batch_destroy(pk_batch_t h) {
   if (!b) return PK_ERR_HANDLE;
   (void)hipStreamSynchronize(b->stream);
   batch_free(b);
+  return PK_OK;
+}
+
+int pk_batch_set_graph(pk_batch_t h, int on) {
+  Batch* b = as_batch(h);
+  if (!b) return PK_ERR_HANDLE;
+  b->use_graph = on != 0;
   return PK_OK;
 }
 
