@@ -110,6 +110,32 @@ __global__ __launch_bounds__(kBlock) void k_bspmv(const BProb* __restrict__ prob
                          last ? nullptr : yacc, last, prod, lds);
 }
 
+// The row-wise form over the row-block descriptors (spmv_vec_body).
+template <typename VT, typename T, int G>
+__global__ __launch_bounds__(kBlock) void k_bspmv_vec(const BProb* __restrict__ probs,
+                                                      const BPanel* __restrict__ panels,
+                                                      const int4* __restrict__ desc,
+                                                      const BWork* __restrict__ work, int tr,
+                                                      int p, int J, int phase,
+                                                      double* __restrict__ part, int Q) {
+  const BWork w = work[blockIdx.x];
+  const BProb P = probs[w.pid];
+  if (Gate{P.ctl, phase, J, 0}.off()) return;
+  __shared__ double lds[(kBlock / kWave) * 2];
+  const BPanel pan = panels[(w.pid * 2 + tr) * kPanelsMax + p];
+  const int np = tr ? P.npt : P.npa;
+  const int last = p == np - 1;
+  double* yacc = tr ? P.yacc_t : P.yacc_a;
+  const T* x = tr ? bcol<T>(P, false, J) : bcol<T>(P, true, J);
+  T* y = tr ? bcol<T>(P, true, J) : bcol<T>(P, false, J + 1);
+  const T* yprev = tr ? (J > 1 ? bcol<T>(P, true, J - 1) : nullptr) : bcol<T>(P, false, J);
+  const double c = tr ? P.ctl->beta : P.ctl->alpha;
+  const int q = (tr ? P.qt0 : P.qa0) + w.loc;
+  spmv_vec_body<VT, T, G>(desc[blockIdx.x], pan.rp, pan.ci, static_cast<const VT*>(pan.va), x,
+                          y, yprev, c, part + q, part + Q + q, pan.lsum,
+                          p == 0 ? nullptr : yacc, last ? nullptr : yacc, last, lds);
+}
+
 // The same over the column-sorted layout (spmv_sorted_body): one
 // kSortThreads workgroup per descriptor, the launch's largest block cap of
 // dynamic LDS.
@@ -278,7 +304,8 @@ struct Batch {
   int* cslice_pid_n = nullptr;
   BPanel* panels = nullptr;    // (pid * 2 + tr) * kPanelsMax + p
   struct Launch {              // one batched SpMV launch
-    int tr, p, g, cap, count;  // cap > 0: column-sorted blocks of at most cap nonzeros
+    int tr, p, g, cap, count;  // cap > 0: column-sorted blocks of at most cap nonzeros;
+                               // cap < 0: row-wise over the row-block arrays
     int4* desc;
     BWork* work;
   };
@@ -333,6 +360,15 @@ void enqueue_batch_step(Batch& B, int J, int elr, int tries) {
       hipLaunchKernelGGL((k_bspmv<float, T, G>), dim3(L.count), dim3(kBlock), 0, s, B.dprob,    \
                          B.panels, L.desc, L.work, tr, p, J, phase, B.part, Q);                 \
     break;
+#define PK_BV_CASE(G)                                                                           \
+  case G:                                                                                       \
+    if (B.val_prec == PK_F64)                                                                   \
+      hipLaunchKernelGGL((k_bspmv_vec<double, T, G>), dim3(L.count), dim3(kBlock), 0, s,        \
+                         B.dprob, B.panels, L.desc, L.work, tr, p, J, phase, B.part, Q);        \
+    else                                                                                        \
+      hipLaunchKernelGGL((k_bspmv_vec<float, T, G>), dim3(L.count), dim3(kBlock), 0, s,         \
+                         B.dprob, B.panels, L.desc, L.work, tr, p, J, phase, B.part, Q);        \
+    break;
 #define PK_BSS_CASE(G)                                                                          \
   case G:                                                                                       \
     if (B.val_prec == PK_F64)                                                                   \
@@ -353,6 +389,16 @@ void enqueue_batch_step(Batch& B, int J, int elr, int tries) {
             PK_BSS_CASE(64)
             default: break;
           }
+        } else if (L.cap < 0) {
+          switch (L.g) {
+            PK_BV_CASE(2)
+            PK_BV_CASE(4)
+            PK_BV_CASE(8)
+            PK_BV_CASE(16)
+            PK_BV_CASE(32)
+            PK_BV_CASE(64)
+            default: break;
+          }
         } else {
           switch (L.g) {
             PK_BS_CASE(2)
@@ -365,6 +411,7 @@ void enqueue_batch_step(Batch& B, int J, int elr, int tries) {
           }
         }
 #undef PK_BS_CASE
+#undef PK_BV_CASE
 #undef PK_BSS_CASE
       }
       for (const auto& L : B.lng) {
@@ -533,9 +580,10 @@ int batch_create(int nprob, const pk_lbd_t* handles, Batch** out) {
       // the engine's own scratch: the SpMVs of both directions run one after
       // the other on the batch's stream
       (tr ? P.yacc_t : P.yacc_a) = e->sw_y;
-      // the layout the operator uses for this direction (row-block or
-      // column-sorted), exactly as its own SpMV runs it
-      const int cap = D.sort_cap;
+      // the layout the operator uses for this direction (row-block,
+      // row-wise over the row-block arrays (cap -1), or column-sorted),
+      // exactly as its own SpMV runs it
+      const int cap = D.rowwise ? -1 : D.sort_cap;
       for (int p = 0; p < D.np; ++p) {
         const CsrOp::Panel& pan = D.pan[p];
         const CsrOp::Part& part = cap > 0 ? pan.spart : pan.part;

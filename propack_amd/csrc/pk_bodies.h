@@ -152,6 +152,93 @@ __device__ __forceinline__ void spmv_rb_body(const int4 d, const int* __restrict
                           last, lds);
 }
 
+// --------------------------------------------------- row-wise SpMV ----
+// Row-wise form of a row-block descriptor: the block's rows are taken by
+// groups of G lanes exactly as spmv_rows takes them (same rows per group,
+// same iterations), but each lane multiplies its entries straight from the
+// index and value streams -- entries a + l, a + l + G, ... of its row, four
+// of them in flight, added in that order -- so there is no LDS staging and
+// no barrier before the sums.  Products are rounded to double before they
+// are added (no contraction), as spmv_rb_body stores them: every row sum,
+// and the block's partial sums, are bit for bit the row-block layout's for
+// the same G.  Long-row chunks as in spmv_rb_body.
+template <typename VT, typename T, int G>
+__device__ __forceinline__ void spmv_vec_body(const int4 d, const int* __restrict__ rp,
+                                              const int* __restrict__ ci,
+                                              const VT* __restrict__ va, const T* __restrict__ x,
+                                              T* __restrict__ y, const T* __restrict__ yprev,
+                                              double c, double* q_s2, double* q_sd,
+                                              double* __restrict__ lsum,
+                                              const double* __restrict__ yacc_in,
+                                              double* __restrict__ yacc_out, int last,
+                                              double* lds) {
+  if (d.y < 0) {
+    spmv_chunk<VT, T, kBlock>(d, ci, va, x, q_s2, q_sd, lsum, last, lds);
+    return;
+  }
+  const int r0 = d.x, r1 = d.y;
+  double s2 = 0.0, sd = 0.0;
+  const int lane = threadIdx.x & (G - 1);
+  const int group = threadIdx.x / G;
+  constexpr int ngroups = kBlock / G;
+  const int iters = (r1 - r0 + ngroups - 1) / ngroups;  // uniform across the block
+  for (int it = 0; it < iters; ++it) {
+    const int r = r0 + group + it * ngroups;
+    const bool valid = r < r1;
+    int a = 0, b = 0;
+    if (valid) {
+      a = rp[r];
+      b = rp[r + 1];
+    }
+    double acc = 0.0;
+    int q = a + lane;
+    for (; q + 3 * G < b; q += 4 * G) {
+      int cc[4];
+      double vv[4], xx[4];
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        cc[u] = __builtin_nontemporal_load(ci + q + u * G);
+        vv[u] = static_cast<double>(__builtin_nontemporal_load(va + q + u * G));
+      }
+#pragma unroll
+      for (int u = 0; u < 4; ++u) xx[u] = static_cast<double>(x[cc[u]]);
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        // rounded product, then the sum (no FMA): as the staged products
+#pragma clang fp contract(off)
+        const double pr = vv[u] * xx[u];
+        acc += pr;
+      }
+    }
+    for (; q < b; q += G) {
+#pragma clang fp contract(off)
+      const double pr = static_cast<double>(__builtin_nontemporal_load(va + q)) *
+                        static_cast<double>(x[__builtin_nontemporal_load(ci + q)]);
+      acc += pr;
+    }
+    acc = group_sum<G>(acc);
+    if (valid && lane == 0) {
+      if (yacc_in) acc = yacc_in[r] + acc;
+      if (!last) {
+        yacc_out[r] = acc;
+      } else {
+        const double yp = yprev ? static_cast<double>(yprev[r]) : 0.0;
+        const T yv = static_cast<T>(acc - c * yp);
+        y[r] = yv;
+        s2 += static_cast<double>(yv) * static_cast<double>(yv);
+        sd += yp * static_cast<double>(yv);
+      }
+    }
+  }
+  if (!last) return;
+  double v2[2] = {s2, sd};
+  block_sum<2, kBlock>(v2, lds);
+  if (threadIdx.x == 0) {
+    *q_s2 = v2[0];
+    *q_sd = v2[1];
+  }
+}
+
 // ------------------------------------------------- column-sorted SpMV ----
 // Sorted layout of a panel: row blocks of at most cap nonzeros (a
 // per-operator choice, <= kSortCapMax) and kSortRows rows, rows longer than

@@ -399,10 +399,11 @@ out:
 
 int csr_set_sort(CsrOp* op, CsrOp::Dir& D, int cap) {
   ++op->layout_epoch;
-  if (cap != 0 && (cap < kSpmvCap || cap > kSortCapMax)) return PK_ERR_ARG;
+  if (cap != 0 && cap != -1 && (cap < kSpmvCap || cap > kSortCapMax)) return PK_ERR_ARG;
   for (int p = 0; p < D.np; ++p) free_sorted(D.pan[p]);
   D.sort_cap = 0;
-  if (cap == 0) return PK_OK;
+  D.rowwise = cap == -1;
+  if (cap <= 0) return PK_OK;
   hipStream_t s = nullptr;
   if (hipStreamCreateWithFlags(&s, hipStreamNonBlocking) != hipSuccess) return PK_ERR_HIP;
   int rc = PK_OK;
@@ -653,11 +654,11 @@ int csr_apply(const CsrOp* op, hipStream_t s, char trans, const T* x, T* y, cons
     if (op->val_prec == PK_F64)
       nb = launch_spmv_rb<double, T>(s, Q.ndesc, Q.desc, Q.nlong, Q.ldesc, lsum, P.rp, P.ci,
                                      static_cast<const double*>(P.va), x, y, yprev, coef, part,
-                                     P.g, yin, yout, last, gate);
+                                     P.g, yin, yout, last, gate, D.rowwise);
     else
       nb = launch_spmv_rb<float, T>(s, Q.ndesc, Q.desc, Q.nlong, Q.ldesc, lsum, P.rp, P.ci,
                                     static_cast<const float*>(P.va), x, y, yprev, coef, part,
-                                    P.g, yin, yout, last, gate);
+                                    P.g, yin, yout, last, gate, D.rowwise);
     if (nb < 0) return nb;
   }
   return nb;
@@ -876,8 +877,8 @@ int pk_csr_autotune(int m, int n, long long nnz, const int* rowptr, const int* c
   if (!panels_a || !panels_t) return PK_ERR_ARG;
   const bool lanes = lanes_a && lanes_t;
   const bool sorts = sort_a && sort_t;
-  static const int kCaps[3] = {0, 8192, 16384};
-  const int ncaps = sorts ? 3 : 1;
+  static const int kCaps[4] = {0, -1, 8192, 16384};
+  const int ncaps = sorts ? 4 : 1;
   // PK_AUTOTUNE_LOG=1: one line per timed setting on stderr
   const char* lg = std::getenv("PK_AUTOTUNE_LOG");
   const bool verbose = lg && *lg && *lg != '0';
@@ -945,8 +946,8 @@ int pk_csr_set_sort(pk_csr_t h, int cap_a, int cap_t) {
   CsrOp* op = as_csr(h);
   if (!op) return PK_ERR_HANDLE;
   for (const int c : {cap_a, cap_t})
-    if (c != 0 && (c < kSpmvCap || c > kSortCapMax)) {
-      set_error("pk_csr_set_sort: a block cap must be 0 or lie in [%d, %d]", kSpmvCap,
+    if (c != 0 && c != -1 && (c < kSpmvCap || c > kSortCapMax)) {
+      set_error("pk_csr_set_sort: a block cap must be 0, -1 or lie in [%d, %d]", kSpmvCap,
                 kSortCapMax);
       return PK_ERR_ARG;
     }
@@ -964,8 +965,8 @@ int pk_csr_set_sort(pk_csr_t h, int cap_a, int cap_t) {
 int pk_csr_sort(pk_csr_t h, int* cap_a, int* cap_t) {
   CsrOp* op = as_csr(h);
   if (!op) return PK_ERR_HANDLE;
-  if (cap_a) *cap_a = op->da.sort_cap;
-  if (cap_t) *cap_t = op->dt.sort_cap;
+  if (cap_a) *cap_a = op->da.rowwise ? -1 : op->da.sort_cap;
+  if (cap_t) *cap_t = op->dt.rowwise ? -1 : op->dt.sort_cap;
   return PK_OK;
 }
 

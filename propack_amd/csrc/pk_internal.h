@@ -57,6 +57,7 @@ struct CsrOp {
     int slots = 0;           // partial slots of the widest panel
     int lane_shift = 0;      // lanes per row = built-in choice * 2^lane_shift
     int sort_cap = 0;        // > 0: column-sorted layout, blocks of <= sort_cap nonzeros
+    int rowwise = 0;         // row-block arrays summed row-wise, no LDS staging (k_spmv_vec)
   } da, dt;
   int nrb = 0, nrbt = 0;  // partial slots per direction
   // bumped by every layout change (lanes per row, sorted layout): a batch
@@ -82,7 +83,8 @@ void csr_destroy(CsrOp* op);
 void csr_set_lanes(CsrOp::Dir& D, int shift);
 // Switches direction D of op to the column-sorted layout with blocks of at
 // most cap nonzeros (kSpmvCap <= cap <= kSortCapMax; built on the device),
-// or back to the row-block layout (cap = 0, the sorted copy freed).
+// or back to the row-block layout (cap = 0, the sorted copy freed), or to
+// the row-block arrays read row-wise (cap = -1).
 int csr_set_sort(CsrOp* op, CsrOp::Dir& D, int cap);
 // ywork (rows of the direction, doubles) and lwork (csr_lwork(op) doubles),
 // when given, replace the operator's own running row sums and long-row

@@ -28,11 +28,13 @@ constexpr int kLongChunk = 8192;
 constexpr int kSortThreads = 1024;
 constexpr int kSortCapMax = 16384;
 constexpr int kSortRows = 16384;
+// rowwise = 1: the same descriptors summed row by row without LDS staging
+// (k_spmv_vec; bit-identical row sums).
 template <typename VT, typename T>
 int launch_spmv_rb(hipStream_t s, int nblk, const int4* desc, int nlong, const int4* ldesc,
                    double* lsum, const int* rp, const int* ci, const VT* va, const T* x, T* y,
                    const T* yprev, Scal coef, double* part, int g, const double* yacc_in,
-                   double* yacc_out, int last, Gate gate);
+                   double* yacc_out, int last, Gate gate, int rowwise = 0);
 
 // The same over the column-sorted layout (pk_bodies.h): descriptors of
 // blocks of at most cap (<= kSortCapMax) nonzeros, sorted column indices

@@ -45,6 +45,26 @@ __global__ __launch_bounds__(kBlock) void k_spmv_rb(const int4* __restrict__ des
                          last, prod, lds);
 }
 
+// Row-wise form over the same descriptors (spmv_vec_body): no LDS staging.
+template <typename VT, typename T, int G>
+__global__ __launch_bounds__(kBlock) void k_spmv_vec(const int4* __restrict__ desc,
+                                                     const int* __restrict__ rp,
+                                                     const int* __restrict__ ci,
+                                                     const VT* __restrict__ va,
+                                                     const T* __restrict__ x, T* __restrict__ y,
+                                                     const T* __restrict__ yprev, Scal coef,
+                                                     double* __restrict__ part, int nbt,
+                                                     double* __restrict__ lsum,
+                                                     const double* __restrict__ yacc_in,
+                                                     double* __restrict__ yacc_out, int last,
+                                                     Gate gate) {
+  if (gate.off()) return;
+  __shared__ double lds[(kBlock / kWave) * 2];
+  spmv_vec_body<VT, T, G>(desc[blockIdx.x], rp, ci, va, x, y, yprev, coef.get(),
+                          part + blockIdx.x, part + nbt + blockIdx.x, lsum, yacc_in, yacc_out,
+                          last, lds);
+}
+
 template <typename T>
 __global__ __launch_bounds__(kBlock) void k_spmv_longfix(int nlong, const int4* __restrict__ ld,
                                                          const double* __restrict__ lsum,
@@ -67,13 +87,17 @@ template <typename VT, typename T>
 int launch_spmv_rb(hipStream_t s, int nblk, const int4* desc, int nlong, const int4* ldesc,
                    double* lsum, const int* rp, const int* ci, const VT* va, const T* x, T* y,
                    const T* yprev, Scal coef, double* part, int g, const double* yacc_in,
-                   double* yacc_out, int last, Gate gate) {
+                   double* yacc_out, int last, Gate gate, int rowwise) {
   const int nbt = nblk + nlong;
   if (nblk > 0) {
-#define PK_RB_CASE(G)                                                                          \
-  case G:                                                                                      \
-    hipLaunchKernelGGL((k_spmv_rb<VT, T, G>), dim3(nblk), dim3(kBlock), 0, s, desc, rp, ci, va, \
-                       x, y, yprev, coef, part, nbt, lsum, yacc_in, yacc_out, last, gate);     \
+#define PK_RB_CASE(G)                                                                           \
+  case G:                                                                                       \
+    if (rowwise)                                                                                \
+      hipLaunchKernelGGL((k_spmv_vec<VT, T, G>), dim3(nblk), dim3(kBlock), 0, s, desc, rp, ci,  \
+                         va, x, y, yprev, coef, part, nbt, lsum, yacc_in, yacc_out, last, gate); \
+    else                                                                                        \
+      hipLaunchKernelGGL((k_spmv_rb<VT, T, G>), dim3(nblk), dim3(kBlock), 0, s, desc, rp, ci,   \
+                         va, x, y, yprev, coef, part, nbt, lsum, yacc_in, yacc_out, last, gate); \
     break;
     switch (g) {
       PK_RB_CASE(2)
@@ -595,7 +619,7 @@ PK_INST_T(float)
   template int launch_spmv_rb<VT, T>(hipStream_t, int, const int4*, int, const int4*, double*, \
                                      const int*, const int*, const VT*, const T*, T*,         \
                                      const T*, Scal, double*, int, const double*, double*,   \
-                                     int, Gate);
+                                     int, Gate, int);
 PK_INST_RB(double, double)
 PK_INST_RB(float, float)
 PK_INST_RB(double, float)

@@ -134,14 +134,15 @@ def test_batch_shared_operator_and_repeated_engine():
 
 
 def test_batch_sorted_layout_matches_sequential():
-    """Operators in the column-sorted SpMV layout (different block caps, one
-    row-block operator among them, FP64 and FP32): the batch runs each
+    """Operators in the column-sorted SpMV layout (different block caps), in
+    the row-wise one (cap -1) and one row-block operator, FP64 and FP32,
+    directions of one operator in different layouts: the batch runs each
     direction in its operator's own layout, so it reproduces the
     per-problem engine bit for bit; the sorted layout's singular values
     agree with the row-block layout's to the solve's tolerance."""
     for prec in (api.PK_F64, api.PK_F32):
         probs = _problems(5, 2_000, 60_000, seed=17, prec=prec)
-        caps = [(8192, 8192), (16384, 1024), (0, 0), (1024, 16384), (8192, 0)]
+        caps = [(8192, 8192), (16384, 1024), (0, 0), (-1, 16384), (8192, -1)]
         for p, c in zip(probs, caps):
             p[2].set_sort(*c)
             assert p[2].sort() == c

@@ -347,10 +347,54 @@ def test_sorted_spmv_matches_rowblock(panels, spec, cap):
         assert np.array_equal(op1.spmv(y, "t"), op0.spmv(y, "t"))
 
 
+@pytest.mark.parametrize("spec", [None, "4"])
+def test_rowwise_spmv_matches_rowblock(panels, spec):
+    """Row-wise layout (cap -1: the row-block arrays, each row summed by its
+    lanes straight from the index and value streams, no LDS staging): every
+    row, long rows included, and the norm partials are bit for bit the
+    row-block layout's for the same lanes; so a whole solve is too."""
+    if spec:
+        panels(spec)
+    rng = np.random.default_rng(11)
+    for A in (_edge_matrix(rng), synth.powerlaw_bipartite(400_000, 200_000, seed=3),
+              synth.poisson_decay(200_000, 100_000, 16.0, 0.6, seed=4),
+              synth.random_uniform(2000, 600_000, 200_000, seed=9)):
+        x, y = rng.standard_normal(A.n), rng.standard_normal(A.m)
+        for lanes in ((0, 0), (-2, 1)):
+            op0 = api.CsrOperator(A, lanes=lanes)
+            op1 = api.CsrOperator(A, lanes=lanes, sort=(-1, -1))
+            assert op1.sort() == (-1, -1)
+            for trans, v in (("n", x), ("t", y)):
+                assert np.array_equal(op1.spmv(v, trans), op0.spmv(v, trans))
+        op32 = api.CsrOperator(A.astype(np.float32), prec=api.PK_F32)
+        op32r = api.CsrOperator(A.astype(np.float32), prec=api.PK_F32, sort=(-1, 0))
+        x32 = x.astype(np.float32)
+        assert np.array_equal(op32r.spmv(x32, "n"), op32.spmv(x32, "n"))
+        want = A.matvec(x)
+        np.testing.assert_allclose(op1.spmv(x, "n"), want, rtol=0,
+                                   atol=1e-10 * np.abs(want).max())
+
+
+def test_rowwise_same_solve(small_case):
+    A, u0, Dm, s = small_case
+    r0 = api.lansvd(api.CsrOperator(A), A.m, A.n, 10, 300, u0=u0, tol=1e-12)
+    r1 = api.lansvd(api.CsrOperator(A, sort=(-1, -1)), A.m, A.n, 10, 300, u0=u0, tol=1e-12)
+    assert r0.k == r1.k == 10 and np.array_equal(r0.sigma, r1.sigma)
+    assert r0.counters == r1.counters
+    for m, n in ((1, 1), (1, 50), (50, 1), (5000, 3), (3, 5000)):
+        rng = np.random.default_rng(m + n)
+        D = rng.standard_normal((m, n)) * (rng.random((m, n)) < 0.5)
+        D[0, 0] = 1.0
+        B = _csr_from_dense(D)
+        op0, op1 = api.CsrOperator(B), api.CsrOperator(B, sort=(-1, -1))
+        for trans, v in (("n", rng.standard_normal(n)), ("t", rng.standard_normal(m))):
+            assert np.array_equal(op1.spmv(v, trans), op0.spmv(v, trans))
+
+
 def test_sorted_layout_rejects_bad_caps():
     A = synth.random_uniform(3000, 1000, 20000, seed=5)
     op = api.CsrOperator(A)
-    for caps in ((100, 0), (0, 16385), (-1, 1024)):
+    for caps in ((100, 0), (0, 16385), (-2, 1024)):
         with pytest.raises(api.PropackError):
             op.set_sort(*caps)
         assert op.sort() == (0, 0)
