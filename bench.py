@@ -550,6 +550,9 @@ def main():
             t32.append(time.perf_counter() - t0)
         k32 = min(f32["k"], r.k)
         err32 = np.abs(f32["sigma"][:k32] - r.sigma[:k32]) / r.sigma[:k32]
+        # the FP32 phase's triplets checked with the FP64 operator on the
+        # device (pk_lbd_residuals): ||A v - s u|| / s_1 per triplet
+        res32 = eng.residuals(eng32, r32.sigma[:r32.k])[:, 0] / r.sigma[0]
         errp = np.abs(rm.sigma[:r.k] - r.sigma[:r.k]) / r.sigma[:r.k] if rm.k >= r.k else None
         mixed = {"s": float(np.mean(tm)), "speedup_vs_fp64": float(ms_per_step / 1000.0 / np.mean(tm)),
                  "fp32_phase_converged": f32["k"], "fp32_phase_info": f32["info"],
@@ -558,6 +561,10 @@ def main():
                  "polished_converged": rm.k,
                  "polished_max_rel_sigma_err_vs_fp64": float(errp.max()) if errp is not None else None,
                  "nopx_total": rm.counters["nopx"],
+                 "polish_nopx": rm.counters["nopx"] - r32.counters["nopx"],
+                 "fp32_phase_fp64_rel_residual_per_triplet": [float(x) for x in res32],
+                 "fp32_phase_fp64_max_rel_residual": float(res32.max()),
+                 "triplets_missing_tol_before_polish": int((res32 > TOL).sum()),
                  "fp32_phase_s": float(np.mean(t32)), "fp32_phase_nopx": r32.counters["nopx"],
                  "fp32_phase_speedup_vs_fp64": float(ms_per_step / 1000.0 / np.mean(t32))}
         log(f"[bench] FP32-basis + FP64 polish: {mixed['s']:.3f}s "

@@ -264,6 +264,18 @@ int pk_lbd_set_mode(pk_lbd_t eng, int mode, int tries);
  * src's U (any precision): the start vector of the FP64 polish that follows
  * an FP32-basis solve (no reference counterpart). */
 int pk_lbd_seed(pk_lbd_t dst, pk_lbd_t src, int ncols);
+/* Residuals of k triplets on the device (no reference counterpart; the
+ * reference only estimates them from B_k): the singular vectors are
+ * columns 1..k of src's bases (src an engine of eng's shape, any
+ * precision), sigma[i] their values, and the operator is eng's (a device
+ * CSR):  res[2i] = ||A v_{i+1} - sigma[i] u_{i+1}||, res[2i+1] =
+ * ||A^T u_{i+1} - sigma[i] v_{i+1}||, the vectors converted to eng's
+ * precision and every sum accumulated in double.  With a double eng and an
+ * FP32-basis src this is the FP64 check of FP32-basis triplets.
+ * pk_lbd_promote: columns 1..ncols of src's bases copied (converted) into
+ * dst's. */
+int pk_lbd_residuals(pk_lbd_t eng, pk_lbd_t src, int k, const double* sigma, double* res);
+int pk_lbd_promote(pk_lbd_t dst, pk_lbd_t src, int ncols);
 /* Live kernel timing (no reference counterpart).  classes: bit mask of the
  * kernel classes to time (0 turns timing off, PK_PROF_ALL times all); each
  * launch of a selected class is bracketed by HIP events on the engine's
@@ -294,8 +306,12 @@ int pk_lansvd_irl_run(pk_lbd_t eng, char which, char jobu, char jobv, int* dim, 
                       int* ioption);
 /* FP32-basis form with FP64 polish: the restarted solve runs first on
  * eng32 (FP32 operator values and bases, FP64 reductions, omega recurrence,
- * B and its SVD) to tolerance max(tolin, 16*eps32); the sum of its left Ritz
- * vectors seeds the FP64 restarted solve on eng64, which meets tolin.
+ * B and its SVD) to tolerance max(tolin, 16*eps32).  When tolin is at least
+ * 16*eps32 the FP32 triplets are checked with eng64's operator
+ * (pk_lbd_residuals); if every ||A v - s u|| meets tolin * s_1 they are the
+ * result (promoted into eng64's bases) and no polish runs.  Otherwise the
+ * sum of the FP32 left Ritz vectors seeds the FP64 restarted solve on
+ * eng64, which meets tolin.
  * sigma32/bnd32 receive the FP32-phase values, info32 = {INFO, converged}
  * of that phase; the polished vectors are left in eng64's bases. */
 int pk_lansvd_irl_mixed(pk_lbd_t eng64, pk_lbd_t eng32, char which, char jobu, char jobv,

@@ -41,7 +41,7 @@ EXPORTS = [
     "pk_matrix_free", "pk_coo_to_csr", "pk_csr_create_coo", "pk_csr_read",
     "pk_csr_create_panels", "pk_csr_panels", "pk_csr_set_lanes", "pk_csr_lanes", "pk_csr_autotune",
     "pk_csr_set_sort", "pk_csr_sort", "pk_lbd_path_stats", "pk_batch_set_graph",
-    "pk_csr_spmv_noalloc",
+    "pk_csr_spmv_noalloc", "pk_lbd_residuals", "pk_lbd_promote",
 ]
 
 _lib = None
@@ -354,6 +354,16 @@ class Engine:
         out = np.zeros(6, np.int64)
         _check(lib().pk_lbd_stats(self.handle, _p(out)), "pk_lbd_stats")
         return dict(zip(["nopx", "nreorth", "ndot", "ncols", "nsync", "nretry"], out.tolist()))
+
+    def residuals(self, src: "Engine", sigma) -> np.ndarray:
+        """Device residuals of the triplets in src's bases with this engine's
+        operator: an array of (||A v - s u||, ||A^T u - s v||) per triplet
+        (pk_lbd_residuals)."""
+        sig = np.ascontiguousarray(sigma, np.float64)
+        out = np.zeros(2 * sig.size)
+        _check(lib().pk_lbd_residuals(self.handle, src.handle, C.c_int(sig.size), _p(sig),
+                                      _p(out)), "pk_lbd_residuals")
+        return out.reshape(-1, 2)
 
     def path_stats(self) -> dict:
         """How often the reference's rare DLANBPRO paths ran on this engine."""

@@ -873,6 +873,40 @@ struct Impl {
     return resolve_timers();
   }
 
+  // Residuals of the triplets (sigma[i], u_{i+1}, v_{i+1}) held in src's
+  // bases (src has the engine's shape, any precision), with this engine's
+  // operator: res[2i] = ||A v - s u||, res[2i+1] = ||A^T u - s v||.  The
+  // vectors are converted to T, and each residual is the fused epilogue of
+  // one SpMV (y = op(A) x - s yprev, ||y||^2 in FP64).
+  int residuals(const Engine* src, int k, const double* sigma, double* res) {
+    const long long mx = std::max(e->m, e->n);
+    if (!e->kres) PK_HIP(hipMalloc(&e->kres, e->tbytes * mx * 3));
+    T* x = static_cast<T*>(e->kres);
+    T* yp = x + mx;
+    T* y = yp + mx;
+    PK_HIP(hipStreamSynchronize(src->stream));
+    auto load = [&](char which, int col, T* dst) -> int {
+      const bool isu = which == 'u';
+      const long long L = isu ? e->m : e->n;
+      const char* base = static_cast<const char*>(isu ? src->u() : src->v()) +
+                         src->tbytes * static_cast<size_t>(col - 1) * (isu ? src->ldu : src->ldv);
+      const int r = src->prec == PK_F64
+                        ? launch_convert<double, T>(strm, L, reinterpret_cast<const double*>(base), dst)
+                        : launch_convert<float, T>(strm, L, reinterpret_cast<const float*>(base), dst);
+      return r < 0 ? PK_ERR_HIP : PK_OK;
+    };
+    for (int i = 0; i < k; ++i) {
+      double s2 = 0, sd = 0;
+      PK_TRY(load('v', i + 1, x));
+      PK_TRY(load('u', i + 1, yp));
+      PK_TRY(apply('n', x, y, yp, sigma[i], &s2, &sd));
+      res[2 * i] = std::sqrt(s2);
+      PK_TRY(apply('t', yp, y, x, sigma[i], &s2, &sd));
+      res[2 * i + 1] = std::sqrt(s2);
+    }
+    return PK_OK;
+  }
+
   // X(:, 1:ncols) = X(:, 1:kdim) * op(B) into the spare buffer, then swap.
   int update(char which, char transb, int kdim, int ncols, const double* B, int ldb) {
     const bool isu = (which == 'u' || which == 'U');
@@ -975,6 +1009,7 @@ static void engine_free(Engine* e) {
   (void)hipFree(e->vec);
   (void)hipFree(e->sw_y);
   (void)hipFree(e->sw_l);
+  (void)hipFree(e->kres);
   (void)hipFree(e->dctl);
   (void)hipFree(e->darr);
   (void)hipHostFree(e->hctl);
@@ -1245,6 +1280,56 @@ int pk_lbd_path_stats(pk_lbd_t h, long long* out) {
 int pk_lbd_reset_stats(pk_lbd_t h) {
   PK_ENGINE(h);
   e->cnt = Counters();
+  return PK_OK;
+}
+
+int pk_lbd_residuals(pk_lbd_t h, pk_lbd_t src_h, int k, const double* sigma, double* res) {
+  PK_ENGINE(h);
+  Engine* sr = as_engine(src_h);
+  if (!sr) return PK_ERR_HANDLE;
+  if (!e->op || sr->m != e->m || sr->n != e->n || k < 0 || k > sr->kmax ||
+      (k > 0 && (!sigma || !res))) {
+    set_error("pk_lbd_residuals: needs an engine with a device operator, a source engine of "
+              "its shape and 0 <= k <= the source's kmax");
+    return PK_ERR_ARG;
+  }
+  if (e->prec == PK_F64) return Impl<double>(e).residuals(sr, k, sigma, res);
+  return Impl<float>(e).residuals(sr, k, sigma, res);
+}
+
+int pk_lbd_promote(pk_lbd_t dst_h, pk_lbd_t src_h, int ncols) {
+  Engine* d = as_engine(dst_h);
+  Engine* sr = as_engine(src_h);
+  if (!d || !sr) return PK_ERR_HANDLE;
+  if (d->m != sr->m || d->n != sr->n || ncols < 0 || ncols > std::min(d->kmax, sr->kmax)) {
+    set_error("pk_lbd_promote: engines of one shape and 0 <= ncols <= both kmax");
+    return PK_ERR_ARG;
+  }
+  PK_HIP(hipStreamSynchronize(sr->stream));
+  for (int isu = 0; isu < 2; ++isu) {
+    const long long L = isu ? d->m : d->n;
+    for (int c = 0; c < ncols; ++c) {
+      const char* s = static_cast<const char*>(isu ? sr->u() : sr->v()) +
+                      sr->tbytes * static_cast<size_t>(c) * (isu ? sr->ldu : sr->ldv);
+      char* t = static_cast<char*>(isu ? d->u() : d->v()) +
+                d->tbytes * static_cast<size_t>(c) * (isu ? d->ldu : d->ldv);
+      int r;
+      if (sr->prec == PK_F32 && d->prec == PK_F64)
+        r = launch_convert<float, double>(d->stream, L, reinterpret_cast<const float*>(s),
+                                          reinterpret_cast<double*>(t));
+      else if (sr->prec == PK_F64 && d->prec == PK_F32)
+        r = launch_convert<double, float>(d->stream, L, reinterpret_cast<const double*>(s),
+                                          reinterpret_cast<float*>(t));
+      else if (sr->prec == PK_F64)
+        r = launch_convert<double, double>(d->stream, L, reinterpret_cast<const double*>(s),
+                                           reinterpret_cast<double*>(t));
+      else
+        r = launch_convert<float, float>(d->stream, L, reinterpret_cast<const float*>(s),
+                                         reinterpret_cast<float*>(t));
+      if (r < 0) return PK_ERR_HIP;
+    }
+  }
+  PK_HIP(hipStreamSynchronize(d->stream));
   return PK_OK;
 }
 

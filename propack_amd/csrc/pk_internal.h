@@ -160,6 +160,7 @@ struct Engine {
   void* vec = nullptr;               // scratch vector, max(m, n)
   double* sw_y = nullptr;            // SpMV running row sums (max(m, n)), the engine's own
   double* sw_l = nullptr;            // SpMV long-row partial sums (csr_lwork(op))
+  void* kres = nullptr;              // residual check: three vectors of max(m, n), on first use
   LbdCtl* dctl = nullptr;            // step control state: scalars (device)
   void* darr = nullptr;              // its arrays, capacity kmax (CtlArrays layout)
   LbdCtlRef* hctl = nullptr;         // pinned host copy: scalars, pointers into harr

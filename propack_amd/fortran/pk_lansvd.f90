@@ -775,10 +775,24 @@ integer(c_int) function pk_lansvd_irl_mixed(eng64, eng32, which, jobu, jobv, dim
       type(c_ptr), value :: e
       integer(c_int), value :: model
     end function
+    integer(c_int) function pk_lbd_residuals(e, src, k, sig, res) &
+        bind(C, name='pk_lbd_residuals')
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: e, src
+      integer(c_int), value :: k
+      real(c_double) :: sig(*), res(*)
+    end function
+    integer(c_int) function pk_lbd_promote(dst, src, ncols) bind(C, name='pk_lbd_promote')
+      import :: c_int, c_ptr
+      type(c_ptr), value :: dst, src
+      integer(c_int), value :: ncols
+    end function
   end interface
   integer :: m, n, info, dd, ne, ne32, m32, n32
   double precision :: eps, eps32, dopt32(4)
+  double precision, allocatable :: res(:)
   integer(c_int) :: rc
+  logical :: check
   call engine_shape(eng64, m, n, eps, info)
   if (info /= 0) then
     pk_lansvd_irl_mixed = info
@@ -801,13 +815,37 @@ integer(c_int) function pk_lansvd_irl_mixed(eng64, eng32, which, jobu, jobv, dim
   dopt32(2) = eps32**0.75d0
   dopt32(3) = 0d0
   dopt32(4) = doption(4)
-  call lansvd_irl_core(which, 'y', 'n', m, n, dd, p, ne32, maxiter, eng32, eps32, sigma32, &
-                       bnd32, max(tolin, 16d0 * eps32), dopt32, ioption, info)
+  ! tolin reachable by the FP32 phase's own tolerance: its triplets are
+  ! checked in FP64 and, when every one meets tolin, no polish runs
+  check = tolin >= 16d0 * eps32
+  call lansvd_irl_core(which, 'y', merge('y', 'n', check), m, n, dd, p, ne32, maxiter, eng32, &
+                       eps32, sigma32, bnd32, max(tolin, 16d0 * eps32), dopt32, ioption, info)
   info32(1) = info
   info32(2) = ne32
   if (info <= -100) then
     pk_lansvd_irl_mixed = info
     return
+  end if
+  if (check .and. info == 0 .and. ne32 >= neig .and. ne32 > 0) then
+    allocate(res(2 * ne32))
+    rc = pk_lbd_residuals(eng64, eng32, ne32, sigma32, res)
+    if (rc /= PK_OK) then
+      pk_lansvd_irl_mixed = rc
+      return
+    end if
+    if (all(res(1:2 * ne32:2) <= tolin * sigma32(1))) then
+      rc = pk_lbd_promote(eng64, eng32, ne32)
+      if (rc /= PK_OK) then
+        pk_lansvd_irl_mixed = rc
+        return
+      end if
+      sigma(1:ne32) = sigma32(1:ne32)
+      bnd(1:ne32) = bnd32(1:ne32)
+      dim = dd
+      neig = ne32
+      pk_lansvd_irl_mixed = 0
+      return
+    end if
   end if
   ! FP64 polish seeded with the sum of the FP32 left Ritz vectors
   if (ne32 > 0) then

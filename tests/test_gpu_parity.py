@@ -187,6 +187,49 @@ def test_fp32_basis_with_fp64_polish(small_case):
     _check_triplets(r, D, 1e-11, s[0])
 
 
+def test_device_residuals(small_case):
+    """pk_lbd_residuals: ||A v - s u|| and ||A^T u - s v|| formed on the
+    device against the same residuals formed on the host, for FP64
+    triplets and for FP32-basis triplets checked with the FP64 operator."""
+    A, u0, D, s = small_case
+    op64 = api.CsrOperator(A)
+    op32 = api.CsrOperator(A, prec=api.PK_F32)
+    e64 = api.Engine(op64, kmax=30)
+    e64.put_u1(u0)
+    r = api.lansvd_irl_dev(e64, 10, 30, tol=1e-12)
+    e32 = api.Engine(op32, kmax=30)
+    e32.set_rounding(1)
+    e32.put_u1(u0)
+    r32 = api.lansvd_irl_dev(e32, 10, 30, tol=1e-6)
+    for eng_src, rr in ((e64, r), (e32, r32)):
+        got = e64.residuals(eng_src, rr.sigma)
+        U, V = rr.U.astype(np.float64), rr.V.astype(np.float64)
+        want = np.stack([np.linalg.norm(D @ V - U * rr.sigma, axis=0),
+                         np.linalg.norm(D.T @ U - V * rr.sigma, axis=0)], axis=1)
+        assert np.abs(got - want).max() <= 1e-13 * s[0] + 1e-6 * want.max()
+    assert got[:, 0].max() > 1e-10 * s[0]  # the FP32 triplets: FP32-level residuals
+
+
+@pytest.mark.parametrize("tol,polished", [(1e-6, False), (1e-10, True)])
+def test_fp32_basis_skips_polish_when_triplets_meet_tol(small_case, tol, polished):
+    """When tol is within the FP32 phase's reach (>= 16 eps32) its triplets
+    are checked with the FP64 operator on the device and, when all meet
+    tol, returned without the FP64 polish (the FP64 engine runs no Lanczos
+    step); a tighter tol polishes.  Either way the result meets tol by the
+    host residual."""
+    A, u0, D, s = small_case
+    e64 = api.Engine(api.CsrOperator(A), kmax=30)
+    e32 = api.Engine(api.CsrOperator(A, prec=api.PK_F32), kmax=30)
+    e32.put_u1(u0)
+    e64.reset_stats()
+    r, f32 = api.lansvd_irl_mixed(e64, e32, 10, 30, tol=tol)
+    assert r.info == 0 and r.k == 10
+    assert (e64.stats()["nopx"] > 0) == polished
+    res = np.linalg.norm(D @ r.V - r.U * r.sigma, axis=0).max()
+    assert res <= 10 * tol * s[0]
+    np.testing.assert_allclose(r.sigma, s[:10], rtol=10 * tol)
+
+
 def test_single_precision_irl(small_case):
     A, u0, D, s = small_case
     op = api.CsrOperator(A, prec=api.PK_F32)
