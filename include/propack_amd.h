@@ -76,7 +76,9 @@ int pk_csr_spmv(pk_csr_t op, char trans, const void* x, void* y, int prec, int w
  * block's entries stored in increasing column order so that the gathers of
  * nearby columns share cache lines; built on the device, 12 bytes per
  * nonzero on top of the row-block layout when a panel's width and the cap
- * fit one 32-bit word, 14 otherwise) or the row-block layout (cap = 0).
+ * fit one 32-bit word, 14 otherwise), the row-block layout (cap = 0) or the
+ * row-wise layout (cap = -1: each row's lanes sum the row-block arrays
+ * straight from memory, no LDS staging).
  * Row sums are bit-identical to the row-block layout's for the same lanes;
  * only the per-block partial sums of the norms are grouped differently.
  * Not while a solve that uses the operator runs.  A batch (pk_batch_create)
@@ -87,8 +89,9 @@ int pk_csr_spmv(pk_csr_t op, char trans, const void* x, void* y, int prec, int w
  * direction (up to 16 when sort_a and sort_t are not NULL) and, when
  * lanes_a and lanes_t are not NULL, lane shifts -1, 0 and 1 on each (-2
  * too when sort_a and sort_t are not NULL), and,
- * when sort_a and sort_t are not NULL, each in the row-block layout and in
- * the sorted layout with caps 8192 and 16384; times each direction on the
+ * when sort_a and sort_t are not NULL, each in the row-block layout, the
+ * row-wise layout and the sorted layout with caps 8192 and 16384 (reported
+ * as 0, -1, 8192, 16384); times each direction on the
  * device and returns the fastest panel counts (and lane shifts, block caps)
  * and their microseconds per application (inputs as pk_csr_create; nothing
  * is kept).  Settings differ only in how sums are grouped. */
@@ -318,6 +321,13 @@ int pk_lansvd_irl_mixed(pk_lbd_t eng64, pk_lbd_t eng32, char which, char jobu, c
                         int* dim, int p, int* neig, int maxiter, double* sigma, double* bnd,
                         double* sigma32, double* bnd32, double tolin, double* doption,
                         int* ioption, int* info32);
+/* The same scheme around the non-restarted driver (DLANSVD): the FP32
+ * phase runs DLANSVD on eng32 to max(tolin, 16*eps32) with kmax; the check,
+ * promotion or seeded FP64 DLANSVD on eng64 follow as above.  k is in/out
+ * as in pk_lansvd_run; sigma/bnd/sigma32/bnd32 hold at least kmax values. */
+int pk_lansvd_mixed(pk_lbd_t eng64, pk_lbd_t eng32, char jobu, char jobv, int* k, int kmax,
+                    double* sigma, double* bnd, double* sigma32, double* bnd32, double tolin,
+                    double* doption, int* ioption, int* info32);
 /* DLANSVD_IRL on every engine of a batch (engs[i] in pk_batch_create's
  * order; start vectors their U(:,1)); no reference counterpart, it replaces
  * a loop of DLANSVD_IRL calls (double/dlansvd_irl.F:1-3) over independent

@@ -33,7 +33,7 @@ EXPORTS = [
     "pk_lbd_destroy", "pk_lbd_set_callback", "pk_lbd_put", "pk_lbd_get", "pk_lbd_basis",
     "pk_lbd_nrm2", "pk_lbd_getu0", "pk_lbd_lanbpro", "pk_lbd_update", "pk_lbd_stats",
     "pk_lbd_reset_stats", "pk_lbd_sync", "pk_lbd_shape", "pk_lansvd_run",
-    "pk_lansvd_irl_run", "pk_lansvd_irl_mixed", "pk_lbd_seed", "pk_lbd_set_mode", "pk_lbd_profile", "pk_lbd_profile_get", "pk_lbd_profile_reset",
+    "pk_lansvd_irl_run", "pk_lansvd_irl_mixed", "pk_lansvd_mixed", "pk_lbd_seed", "pk_lbd_set_mode", "pk_lbd_profile", "pk_lbd_profile_get", "pk_lbd_profile_reset",
     "pk_larnv_pm1", "pk_larnv_pm1_device", "pkh_update_mu", "pkh_update_nu", "pkh_compute_int", "pkh_set_mu",
     "pkh_lartg", "pkh_bdqr", "pkh_bsvdstep", "pkh_refinebounds", "dlansvd_", "dlansvd_irl_", "slansvd_", "slansvd_irl_",
     "clearstat_", "printstat_", "pk_lbd_set_rounding", "pk_lbd_eps", "pk_batch_create",
@@ -597,6 +597,37 @@ def lansvd_dev(eng: Engine, k, kmax, tol=1e-12, jobu="y", jobv="y", delta=None, 
     U = eng.get("u", k) if (fetch and jobu == "y") else None
     V = eng.get("v", k) if (fetch and jobv == "y") else None
     return SvdResult(sigma[:k].copy(), bnd[:k].copy(), U, V, info, k, timing(), anorm=float(dopt[2]))
+
+
+def lansvd_mixed(eng64: Engine, eng32: Engine, k, kmax, tol=1e-12, jobu="y", jobv="y",
+                 cgs=1, elr=1, fetch=True):
+    """Non-restarted FP32-basis solve (DLANSVD on ``eng32``, whose U(:,1) is
+    the start vector) with the FP64 check / polish on ``eng64``.  Returns
+    (polished SvdResult, dict with the FP32-phase sigma/bnd/info/k)."""
+    sigma = np.zeros(kmax + 1)
+    bnd = np.zeros(kmax + 1)
+    s32 = np.zeros(kmax + 1)
+    b32 = np.zeros(kmax + 1)
+    eps = 2.0 ** -52
+    dopt = np.array([np.sqrt(eps), eps ** 0.75, 0.0, 0.0])
+    iopt = np.array([cgs, elr], np.int32)
+    info32 = np.zeros(2, np.int32)
+    kk = C.c_int(k)
+    clearstat()
+    info = lib().pk_lansvd_mixed(eng64.handle, eng32.handle, C.c_char(jobu.encode()),
+                                 C.c_char(jobv.encode()), C.byref(kk), C.c_int(kmax), _p(sigma),
+                                 _p(bnd), _p(s32), _p(b32), C.c_double(tol), _p(dopt), _p(iopt),
+                                 _p(info32))
+    if info <= -100:
+        _check(info, "pk_lansvd_mixed")
+    k = kk.value
+    k32 = int(info32[1])
+    U = eng64.get("u", k) if (fetch and jobu == "y") else None
+    V = eng64.get("v", k) if (fetch and jobv == "y") else None
+    res = SvdResult(sigma[:k].copy(), bnd[:k].copy(), U, V, info, k, timing(),
+                    anorm=float(dopt[2]))
+    return res, {"sigma": s32[:k32].copy(), "bnd": b32[:k32].copy(), "info": int(info32[0]),
+                 "k": k32}
 
 
 def lansvd_irl_mixed(eng64: Engine, eng32: Engine, neig, dim, p=None, tol=1e-10, which="L",

@@ -742,6 +742,117 @@ subroutine engine_shape(eng, m, n, eps, info)
   eps = ee
 end subroutine
 
+! Non-restarted FP32-basis form with FP64 polish (no reference counterpart):
+! pk_lansvd_irl_mixed's scheme around the DLANSVD driver.  eng32 (FP32
+! values and bases, FP64 reductions and scalars, storage-rounding
+! thresholds) runs DLANSVD to max(tolin, 16*eps32); when tolin is reachable
+! there, its triplets' FP64 residuals are formed on the device with eng64's
+! operator and, if every one meets tolin*sigma(1), they are promoted to
+! eng64 and no polish runs.  Otherwise the sum of the FP32 left Ritz vectors
+! seeds an FP64 DLANSVD on eng64.  sigma32/bnd32 return the FP32 phase's
+! values, info32(1:2) its INFO and converged count.
+integer(c_int) function pk_lansvd_mixed(eng64, eng32, jobu, jobv, k, kmax, sigma, bnd, &
+                                        sigma32, bnd32, tolin, doption, ioption, info32) &
+    bind(C, name='pk_lansvd_mixed')
+  use iso_c_binding
+  use pk_api
+  use pk_drivers
+  implicit none
+  type(c_ptr), value :: eng64, eng32
+  character(kind=c_char), value :: jobu, jobv
+  integer(c_int) :: k, info32(2)
+  integer(c_int), value :: kmax
+  real(c_double) :: sigma(*), bnd(*), sigma32(*), bnd32(*), doption(*)
+  real(c_double), value :: tolin
+  integer(c_int) :: ioption(*)
+  interface
+    integer(c_int) function pk_lbd_seed(dst, src, ncols) bind(C, name='pk_lbd_seed')
+      import :: c_int, c_ptr
+      type(c_ptr), value :: dst, src
+      integer(c_int), value :: ncols
+    end function
+    integer(c_int) function pk_lbd_set_rounding(e, model) bind(C, name='pk_lbd_set_rounding')
+      import :: c_int, c_ptr
+      type(c_ptr), value :: e
+      integer(c_int), value :: model
+    end function
+    integer(c_int) function pk_lbd_residuals(e, src, kk, sig, res) &
+        bind(C, name='pk_lbd_residuals')
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: e, src
+      integer(c_int), value :: kk
+      real(c_double) :: sig(*), res(*)
+    end function
+    integer(c_int) function pk_lbd_promote(dst, src, ncols) bind(C, name='pk_lbd_promote')
+      import :: c_int, c_ptr
+      type(c_ptr), value :: dst, src
+      integer(c_int), value :: ncols
+    end function
+  end interface
+  integer :: m, n, info, kk, k32, m32, n32
+  double precision :: eps, eps32, dopt32(4)
+  double precision, allocatable :: res(:)
+  integer(c_int) :: rc
+  logical :: check
+  call engine_shape(eng64, m, n, eps, info)
+  if (info /= 0) then
+    pk_lansvd_mixed = info
+    return
+  end if
+  call engine_shape(eng32, m32, n32, eps32, info)
+  if (info /= 0 .or. m32 /= m .or. n32 /= n) then
+    pk_lansvd_mixed = -101
+    return
+  end if
+  rc = pk_lbd_set_rounding(eng32, 1)
+  dopt32(1) = sqrt(eps32)
+  dopt32(2) = eps32**0.75d0
+  dopt32(3) = 0d0
+  dopt32(4) = 0d0
+  check = tolin >= 16d0 * eps32
+  k32 = k
+  call lansvd_core('y', merge('y', 'n', check), m, n, k32, kmax, eng32, eps32, sigma32, &
+                   bnd32, max(tolin, 16d0 * eps32), dopt32, ioption, info)
+  info32(1) = info
+  info32(2) = k32
+  if (info <= -100) then
+    pk_lansvd_mixed = info
+    return
+  end if
+  if (check .and. info == 0 .and. k32 >= k .and. k32 > 0) then
+    allocate(res(2 * k32))
+    rc = pk_lbd_residuals(eng64, eng32, k32, sigma32, res)
+    if (rc /= PK_OK) then
+      pk_lansvd_mixed = rc
+      return
+    end if
+    if (all(res(1:2 * k32:2) <= tolin * sigma32(1))) then
+      rc = pk_lbd_promote(eng64, eng32, k32)
+      if (rc /= PK_OK) then
+        pk_lansvd_mixed = rc
+        return
+      end if
+      sigma(1:k32) = sigma32(1:k32)
+      bnd(1:k32) = bnd32(1:k32)
+      k = k32
+      pk_lansvd_mixed = 0
+      return
+    end if
+  end if
+  if (k32 > 0) then
+    rc = pk_lbd_seed(eng64, eng32, k32)
+    if (rc /= PK_OK) then
+      pk_lansvd_mixed = rc
+      return
+    end if
+  end if
+  kk = k
+  call lansvd_core(jobu, jobv, m, n, kk, kmax, eng64, eps, sigma, bnd, tolin, doption, &
+                   ioption, info)
+  k = kk
+  pk_lansvd_mixed = info
+end function
+
 ! FP32-basis form with FP64 polish (no reference counterpart).  eng32 holds
 ! FP32 values and bases (all reductions and scalars FP64); it runs the
 ! restarted driver to the FP32-attainable tolerance max(tolin, 16*eps32).

@@ -230,6 +230,44 @@ def test_fp32_basis_skips_polish_when_triplets_meet_tol(small_case, tol, polishe
     np.testing.assert_allclose(r.sigma, s[:10], rtol=10 * tol)
 
 
+def test_config1_fp32_basis_form(small_case):
+    """The FP32-basis form of the non-restarted solve (DLANSVD, k=10,
+    kmax=300, tol=1e-12 on the 1850 x 712 matrix): tol lies below 16 eps32,
+    so the FP64 DLANSVD polish runs.  Per triplet, the FP32 phase's sigma
+    against the FP64 reference result within 1e-5 relative (FP32 storage);
+    the polished sigma within 1e-10 of the reference, residuals <= 1e-11 s_1
+    by the host."""
+    A, u0, D, s = small_case
+    ref = refpy.lansvd(A, 10, 300, u0=u0, tol=1e-12, cgs=1)
+    e64 = api.Engine(api.CsrOperator(A), kmax=300)
+    e32 = api.Engine(api.CsrOperator(A, prec=api.PK_F32), kmax=300)
+    e32.put_u1(u0)
+    r, f32 = api.lansvd_mixed(e64, e32, 10, 300, tol=1e-12)
+    print("fp32 phase", f32["info"], f32["k"], np.abs(f32["sigma"] - ref.sigma) / ref.sigma)
+    assert f32["info"] == 0 and f32["k"] == 10
+    np.testing.assert_allclose(f32["sigma"], ref.sigma, rtol=1e-5)
+    assert r.info == 0 and r.k == 10
+    np.testing.assert_allclose(r.sigma, ref.sigma, rtol=1e-10)
+    _check_triplets(r, D, 1e-11, s[0])
+
+
+@pytest.mark.parametrize("tol,polished", [(1e-6, False), (1e-10, True)])
+def test_config1_fp32_basis_skips_polish(small_case, tol, polished):
+    """Non-restarted form: within the FP32 phase's reach its triplets are
+    checked with the FP64 operator and returned without a polish."""
+    A, u0, D, s = small_case
+    e64 = api.Engine(api.CsrOperator(A), kmax=300)
+    e32 = api.Engine(api.CsrOperator(A, prec=api.PK_F32), kmax=300)
+    e32.put_u1(u0)
+    e64.reset_stats()
+    r, f32 = api.lansvd_mixed(e64, e32, 10, 300, tol=tol)
+    assert r.info == 0 and r.k == 10
+    assert (e64.stats()["nopx"] > 0) == polished
+    res = np.linalg.norm(D @ r.V - r.U * r.sigma, axis=0).max()
+    assert res <= 10 * tol * s[0]
+    np.testing.assert_allclose(r.sigma, s[:10], rtol=10 * tol)
+
+
 def test_single_precision_irl(small_case):
     A, u0, D, s = small_case
     op = api.CsrOperator(A, prec=api.PK_F32)
