@@ -412,6 +412,16 @@ def workload_batch(args):
         rs, ts = _timed(seq, args.steps, sync)
         rb, tb = _timed(bat, args.steps, sync)
         same = all(np.array_equal(x.sigma, y.sigma) for x, y in zip(rs, rb))
+        # the same batched solve with its bidiagonal SVDs on the device
+        b.set_bsvd(True)
+        rd, td = _timed(bat, args.steps, sync)
+        b.set_bsvd(False)
+        dev = {"problems_per_s": len(probs) / td,
+               "converged": sum(r.k for r in rd),
+               "max_rel_sigma_diff_vs_host_dbdsqr": float(max(
+                   np.abs(x.sigma - y.sigma).max() / x.sigma[0] for x, y in zip(rb, rd)
+                   if x.k == y.k and x.k > 0)),
+               "same_converged_counts": all(x.k == y.k for x, y in zip(rb, rd))}
         worst = 0.0
         for r, (op, e, u0, A) in zip(bat(fetch=True), probs):
             M = sp.csr_array((A.val.astype(np.float64), A.col, A.rowptr), shape=(A.m, A.n))
@@ -423,7 +433,7 @@ def workload_batch(args):
                         "speedup_vs_sequential": ts / tb,
                         "converged": sum(r.k for r in rb), "requested": 16 * len(probs),
                         "bit_identical_to_sequential": same,
-                        "max_rel_residual": worst}
+                        "max_rel_residual": worst, "device_bsvd": dev}
         log(f"[bench] {basis}: {forms[basis]}")
         b.close()
         del probs, b, rs, rb
@@ -444,6 +454,7 @@ def workload_batch(args):
            "converged": f32["converged"], "requested": f32["requested"],
            "bit_identical_to_sequential": f32["bit_identical_to_sequential"],
            "max_rel_residual": f32["max_rel_residual"],
+           "device_bsvd": f32["device_bsvd"],
            "fp64_basis": forms["fp64"]}
     print(json.dumps(out))
 

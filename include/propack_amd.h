@@ -235,10 +235,10 @@ int pk_lbd_eps(pk_lbd_t eng, double* eps, double* leff);
  * problems): the engines of a batch (same precision, each with its own
  * device CSR operator) run their Lanczos steps together, one launch per
  * kernel per step for the whole batch, each problem gated by its own
- * control state; a problem gives the same results as alone.  Engines and
- * operators may not be shared between problems of one batch (PK_ERR_ARG):
- * their SpMVs run at the same time.  Per-problem host work runs on up to
- * 16 host threads (PK_HOST_THREADS overrides). */
+ * control state; a problem gives the same results as alone.  One engine
+ * may not appear twice in a batch (PK_ERR_ARG); problems may share an
+ * operator (each engine carries its own SpMV scratch).  Per-problem host
+ * work runs on up to 16 host threads (PK_HOST_THREADS overrides). */
 typedef struct pk_batch_opaque* pk_batch_t;
 int pk_batch_create(int nprob, const pk_lbd_t* engines, pk_batch_t* batch);
 int pk_batch_destroy(pk_batch_t batch);
@@ -247,6 +247,23 @@ int pk_batch_destroy(pk_batch_t batch);
  * counts) and replayed, up to 8 chains per batch; on = 0 enqueues the
  * kernels directly.  Same results either way (no reference counterpart). */
 int pk_batch_set_graph(pk_batch_t batch, int on);
+/* Bidiagonal SVDs of the batched restarted driver on the device (replaces
+ * the host DBDSQR of its error-bound step, double/dlansvd_irl.F:226-229,
+ * for every problem at once; on = 0, the default, keeps the host DBDSQR,
+ * which makes a batched problem bit-identical to the same problem alone).
+ * pk_batch_bsvd / pk_bsvd_batch: for nb upper bidiagonal matrices (order
+ * n[i] <= 128, diagonal d[i*ld + j], superdiagonal e[i*ld + j]), the
+ * singular values in decreasing order to sigma[i*ld + r] and the last
+ * component of each matching left singular vector to bottom[i*ld + r]
+ * (one-sided Jacobi on the device, one workgroup per matrix; host arrays;
+ * pk_bsvd_batch allocates its device buffers per call, pk_batch_bsvd keeps
+ * them in the batch and runs on its stream). */
+int pk_batch_set_bsvd(pk_batch_t batch, int on);
+int pk_batch_get_bsvd(pk_batch_t batch, int* on);
+int pk_batch_bsvd(pk_batch_t batch, int nb, const int* n, const double* d, const double* e,
+                  int ld, double* sigma, double* bottom);
+int pk_bsvd_batch(int nb, const int* n, const double* d, const double* e, int ld, double* sigma,
+                  double* bottom);
 /* pk_lbd_lanbpro for every problem i with active[i] != 0: k0[i] -> k[i]
  * steps; alpha[i], beta[i] its B(:,1), B(:,2); rnorm[i]; doption(4*i+1:
  * 4*i+3) its delta, eta, anorm; ioption shared; ierr[i]. */

@@ -37,7 +37,8 @@ EXPORTS = [
     "pk_larnv_pm1", "pk_larnv_pm1_device", "pkh_update_mu", "pkh_update_nu", "pkh_compute_int", "pkh_set_mu",
     "pkh_lartg", "pkh_bdqr", "pkh_bsvdstep", "pkh_refinebounds", "dlansvd_", "dlansvd_irl_", "slansvd_", "slansvd_irl_",
     "clearstat_", "printstat_", "pk_lbd_set_rounding", "pk_lbd_eps", "pk_batch_create",
-    "pk_batch_destroy", "pk_batch_lanbpro", "pk_lansvd_irl_batch", "pk_matrix_read",
+    "pk_batch_destroy", "pk_batch_lanbpro", "pk_lansvd_irl_batch", "pk_batch_set_bsvd",
+    "pk_batch_get_bsvd", "pk_batch_bsvd", "pk_bsvd_batch", "pk_matrix_read",
     "pk_matrix_free", "pk_coo_to_csr", "pk_csr_create_coo", "pk_csr_read",
     "pk_csr_create_panels", "pk_csr_panels", "pk_csr_set_lanes", "pk_csr_lanes", "pk_csr_autotune",
     "pk_csr_set_sort", "pk_csr_sort", "pk_lbd_path_stats", "pk_batch_set_graph",
@@ -528,6 +529,11 @@ class Batch:
         (default) or enqueue it directly (pk_batch_set_graph)."""
         _check(lib().pk_batch_set_graph(self.handle, C.c_int(int(on))), "pk_batch_set_graph")
 
+    def set_bsvd(self, on: bool = True):
+        """Run the restarted driver's bidiagonal SVDs on the device for the
+        whole batch (pk_batch_set_bsvd) instead of the host DBDSQR."""
+        _check(lib().pk_batch_set_bsvd(self.handle, C.c_int(int(on))), "pk_batch_set_bsvd")
+
     def close(self):
         if self.handle:
             lib().pk_batch_destroy(self.handle)
@@ -538,6 +544,25 @@ class Batch:
             self.close()
         except Exception:
             pass
+
+
+def bsvd_batch(ds, es):
+    """Singular values (decreasing) and the last components of the left
+    singular vectors of upper bidiagonal matrices (diagonals ds[i],
+    superdiagonals es[i]) on the device (pk_bsvd_batch)."""
+    nb = len(ds)
+    n = np.array([len(d) for d in ds], np.int32)
+    ld = max(1, int(n.max()) if nb else 1)
+    D = np.zeros((nb, ld))
+    E = np.zeros((nb, ld))
+    for i, (d, e) in enumerate(zip(ds, es)):
+        D[i, :len(d)] = d
+        E[i, :len(e)] = e
+    S = np.zeros((nb, ld))
+    X = np.zeros((nb, ld))
+    _check(lib().pk_bsvd_batch(C.c_int(nb), _p(n), _p(D), _p(E), C.c_int(ld), _p(S), _p(X)),
+           "pk_bsvd_batch")
+    return [S[i, :n[i]].copy() for i in range(nb)], [X[i, :n[i]].copy() for i in range(nb)]
 
 
 def lansvd_irl_batch(batch: Batch, neig, dim, p=None, tol=1e-10, which="L", jobu="n", jobv="n",

@@ -332,6 +332,9 @@ struct Batch {
   };
   std::vector<Graph> graphs;
   bool use_graph = true;
+  // the restarted driver's bidiagonal SVDs on the device (pk_bsvd.hip)
+  bool use_bsvd = false;
+  BsvdBuf bsvd;
 };
 constexpr size_t kMaxBatchGraphs = 8;
 
@@ -474,6 +477,7 @@ void enqueue_batch_step(Batch& B, int J, int elr, int tries) {
 void batch_free(Batch* b) {
   if (!b) return;
   for (auto& g : b->graphs) (void)hipGraphExecDestroy(g.exec);
+  bsvd_free(b->bsvd);
   for (auto& L : b->spmv) {
     (void)hipFree(L.desc);
     (void)hipFree(L.work);
@@ -876,6 +880,29 @@ int pk_batch_set_graph(pk_batch_t h, int on) {
   if (!b) return PK_ERR_HANDLE;
   b->use_graph = on != 0;
   return PK_OK;
+}
+
+int pk_batch_set_bsvd(pk_batch_t h, int on) {
+  Batch* b = as_batch(h);
+  if (!b) return PK_ERR_HANDLE;
+  b->use_bsvd = on != 0;
+  return PK_OK;
+}
+
+int pk_batch_get_bsvd(pk_batch_t h, int* on) {
+  Batch* b = as_batch(h);
+  if (!b) return PK_ERR_HANDLE;
+  if (!on) return PK_ERR_ARG;
+  *on = b->use_bsvd ? 1 : 0;
+  return PK_OK;
+}
+
+int pk_batch_bsvd(pk_batch_t h, int nb, const int* n, const double* d, const double* e, int ld,
+                  double* sigma, double* bottom) {
+  Batch* b = as_batch(h);
+  if (!b) return PK_ERR_HANDLE;
+  if (nb > 0 && (!n || !d || !e || !sigma || !bottom)) return PK_ERR_ARG;
+  return bsvd_host(b->bsvd, b->stream, nb, n, d, e, ld, sigma, bottom);
 }
 
 int pk_batch_lanbpro(pk_batch_t h, const int* active, const int* k0, int* k,

@@ -72,6 +72,18 @@ struct CsrOp {
 int host_threads();
 void host_parallel_for(int n, const std::function<void(int)>& f);
 
+// Batched bidiagonal SVD on the device (pk_bsvd.hip): orders up to
+// kBsvdMax; device buffers kept in a BsvdBuf between calls.
+constexpr int kBsvdMax = 128;
+struct BsvdBuf {
+  int* dn = nullptr;
+  double* dv = nullptr;
+  int nb = 0, ld = 0;
+};
+int bsvd_host(BsvdBuf& buf, hipStream_t s, int nb, const int* n, const double* d,
+              const double* e, int ld, double* sigma, double* bot);
+void bsvd_free(BsvdBuf& buf);
+
 // panels_a / panels_t: column panels of A and A^T (powers of two up to
 // CsrOp::kMaxPanels), or <= 0 for the built-in choice.
 int csr_create(int m, int n, long long nnz, const int* rowptr, const int* colind,

@@ -17,7 +17,8 @@ module pk_drivers
   use pk_api
   implicit none
   private
-  public :: lansvd_core, lansvd_irl_core, add_timing, irl_begin, irl_after, irl_end
+  public :: lansvd_core, lansvd_irl_core, add_timing, irl_begin, irl_after, irl_end, &
+            irl_pre_svd, irl_host_svd, irl_post_svd
 
   ! External LAPACK (host): bidiagonal SVD.
   interface
@@ -357,13 +358,24 @@ contains
   end subroutine
 
   ! One pass of the restart loop after its DLANBPRO call returned dim (kk)
-  ! and ierr (double/dlansvd_irl.F:213-414).
+  ! and ierr (double/dlansvd_irl.F:213-414): irl_pre_svd, the bidiagonal
+  ! SVD with one row of left vectors (host DBDSQR here; the batched driver
+  ! may run them on the device for every problem at once), irl_post_svd.
   subroutine irl_after(st, kk, ierr_c)
     type(irl_state), intent(inout) :: st
     integer(c_int), intent(in) :: kk, ierr_c
-    double precision :: relgap, dummy(1, 1)
-    integer :: i, nshft, lapinfo, dim, k, neig, m, n
-    integer(c_int) :: rc
+    call irl_pre_svd(st, kk, ierr_c)
+    call irl_host_svd(st)
+    call irl_post_svd(st)
+  end subroutine
+
+  ! B_dim to upper bidiagonal form in a1/b1 with the last row of the
+  ! rotations in wb(dim:dim+1) (double/dlansvd_irl.F:220-226).
+  subroutine irl_pre_svd(st, kk, ierr_c)
+    type(irl_state), intent(inout) :: st
+    integer(c_int), intent(in) :: kk, ierr_c
+    double precision :: dummy(1, 1)
+    integer :: dim
 
     st%ierr = ierr_c
     ! DLANBPRO lowers its K argument (DIM here) when it stops at an
@@ -371,18 +383,39 @@ contains
     st%dim = kk
     st%kold = st%k
     dim = st%dim
+    st%a1 = st%alpha
+    st%b1 = st%beta
+    st%wb = 0d0
+    call pk_bdqr(dim == min(st%m, st%n), 'N', dim, st%a1, st%b1, st%wb(dim), st%wb(dim + 1), &
+                 dummy, 1)
+  end subroutine
+
+  ! Singular values of the upper bidiagonal (a1, b1) into a1 and the row
+  ! wb(1:dim) times the left singular vectors into wb (double/dlansvd_irl.F:
+  ! 227-229).
+  subroutine irl_host_svd(st)
+    type(irl_state), intent(inout) :: st
+    double precision :: dummy(1, 1)
+    integer :: lapinfo
+    call dbdsqr('U', st%dim, 0, 1, 0, st%a1, st%b1, dummy, 1, st%wb, 1, dummy, 1, st%wrk, &
+                lapinfo)
+  end subroutine
+
+  ! Error bounds, convergence count, shifts and the implicit restart
+  ! (double/dlansvd_irl.F:230-394).
+  subroutine irl_post_svd(st)
+    type(irl_state), intent(inout) :: st
+    double precision :: relgap
+    integer :: i, nshft, dim, k, neig, m, n
+    integer(c_int) :: rc
+
+    dim = st%dim
     k = st%k
     neig = st%neig
     m = st%m
     n = st%n
-    associate (a1 => st%a1, b1 => st%b1, wb => st%wb, alpha => st%alpha, beta => st%beta, &
+    associate (a1 => st%a1, wb => st%wb, alpha => st%alpha, beta => st%beta, &
                shift => st%shift, pm => st%pm, qm => st%qm)
-      ! SVD of B_dim and error bounds (double/dlansvd_irl.F:220-251)
-      a1 = alpha
-      b1 = beta
-      wb = 0d0
-      call pk_bdqr(dim == min(m, n), 'N', dim, a1, b1, wb(dim), wb(dim + 1), dummy, 1)
-      call dbdsqr('U', dim, 0, 1, 0, a1, b1, dummy, 1, wb, 1, dummy, 1, st%wrk, lapinfo)
       st%nbsvd = st%nbsvd + 1
       if (dim > 5) then
         st%anorm = a1(1)
@@ -526,11 +559,12 @@ module pk_batch_par
   use pk_drivers
   implicit none
   private
-  public :: after_ctx, after_one, pkx_parallel_for
+  public :: after_ctx, after_one, pre_one, post_one, pkx_parallel_for
 
   type :: after_ctx
     type(irl_state), pointer :: st(:) => null()
     integer(c_int), pointer :: active(:) => null(), kk(:) => null(), ierr(:) => null()
+    integer(c_int), pointer :: dev(:) => null()  ! 1: the bidiagonal SVD ran on the device
     real(c_double), pointer :: rn(:) => null(), dop(:, :) => null()
   end type
 
@@ -556,6 +590,32 @@ contains
     c%st(i)%rnorm = c%rn(i)
     c%st(i)%dopt = c%dop(:, i)
     call irl_after(c%st(i), c%kk(i), c%ierr(i))
+  end subroutine
+
+  ! The same in two halves around the device bidiagonal SVDs.
+  subroutine pre_one(ctx, i0) bind(C)
+    type(c_ptr), value :: ctx
+    integer(c_int), value :: i0
+    type(after_ctx), pointer :: c
+    integer :: i
+    call c_f_pointer(ctx, c)
+    i = i0 + 1
+    if (c%active(i) == 0) return
+    c%st(i)%rnorm = c%rn(i)
+    c%st(i)%dopt = c%dop(:, i)
+    call irl_pre_svd(c%st(i), c%kk(i), c%ierr(i))
+  end subroutine
+
+  subroutine post_one(ctx, i0) bind(C)
+    type(c_ptr), value :: ctx
+    integer(c_int), value :: i0
+    type(after_ctx), pointer :: c
+    integer :: i
+    call c_f_pointer(ctx, c)
+    i = i0 + 1
+    if (c%active(i) == 0) return
+    if (c%dev(i) == 0) call irl_host_svd(c%st(i))
+    call irl_post_svd(c%st(i))
   end subroutine
 
 end module pk_batch_par
@@ -655,17 +715,43 @@ integer(c_int) function pk_lansvd_irl_batch(nb, engs, batch, which, jobu, jobv, 
       type(c_ptr) :: alpha(*), beta(*)
       real(c_double) :: rnorm(*), dopt(4, *)
     end function
+    integer(c_int) function pk_batch_get_bsvd(b, on) bind(C, name='pk_batch_get_bsvd')
+      import :: c_int, c_ptr
+      type(c_ptr), value :: b
+      integer(c_int) :: on
+    end function
+    integer(c_int) function pk_batch_bsvd(b, nd, nn, d, e, ld, sig, bot) &
+        bind(C, name='pk_batch_bsvd')
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: b
+      integer(c_int), value :: nd, ld
+      integer(c_int) :: nn(*)
+      real(c_double) :: d(*), e(*), sig(*), bot(*)
+    end function
   end interface
+  integer, parameter :: bsvd_max = 128  ! largest order of the device bidiagonal SVD
   type(irl_state), allocatable, target :: st(:)
-  integer(c_int), allocatable, target :: active(:), k0(:), kk(:), ierr(:)
+  integer(c_int), allocatable, target :: active(:), k0(:), kk(:), ierr(:), dev(:)
+  integer(c_int), allocatable :: bn(:), idx(:)
   type(c_ptr), allocatable :: ap(:), bp(:)
   real(c_double), allocatable, target :: rn(:), dop(:, :)
+  real(c_double), allocatable :: bd(:, :), be(:, :), bs(:, :), bb(:, :)
   type(after_ctx), target :: ctx
-  integer :: i, m, n, rc, dd, ne
-  double precision :: eps, dummy(4)
+  integer :: i, j, m, n, rc, dd, ne, nd, ldb
+  integer(c_int) :: usedev
+  double precision :: eps, dummy(4), c1
 
   pk_lansvd_irl_batch = PK_OK
-  allocate(st(nb), active(nb), k0(nb), kk(nb), ierr(nb), ap(nb), bp(nb), rn(nb), dop(4, nb))
+  allocate(st(nb), active(nb), k0(nb), kk(nb), ierr(nb), ap(nb), bp(nb), rn(nb), dop(4, nb), &
+           dev(nb), bn(nb), idx(nb))
+  usedev = 0
+  rc = pk_batch_get_bsvd(batch, usedev)
+  if (rc /= PK_OK) then
+    pk_lansvd_irl_batch = rc
+    return
+  end if
+  ldb = max(dim, 1)
+  if (usedev /= 0) allocate(bd(ldb, nb), be(ldb, nb), bs(ldb, nb), bb(ldb, nb))
   do i = 1, nb
     call engine_shape(engs(i), m, n, eps, rc)
     if (rc /= 0) then
@@ -702,9 +788,50 @@ integer(c_int) function pk_lansvd_irl_batch(nb, engs, batch, which, jobu, jobv, 
     ctx%active => active
     ctx%kk => kk
     ctx%ierr => ierr
+    ctx%dev => dev
     ctx%rn => rn
     ctx%dop => dop
-    call pkx_parallel_for(nb, c_funloc(after_one), c_loc(ctx))
+    if (usedev == 0) then
+      call pkx_parallel_for(nb, c_funloc(after_one), c_loc(ctx))
+      cycle
+    end if
+    ! bidiagonal SVDs of every problem on the device, between the two
+    ! halves of each problem's restart work
+    call pkx_parallel_for(nb, c_funloc(pre_one), c_loc(ctx))
+    nd = 0
+    do i = 1, nb
+      dev(i) = 0
+      dd = st(i)%dim
+      if (active(i) /= 0 .and. dd >= 1 .and. dd <= min(bsvd_max, ldb)) then
+        nd = nd + 1
+        idx(nd) = i
+        dev(i) = 1
+        bn(nd) = dd
+        bd(1:dd, nd) = st(i)%a1(1:dd)
+        be(:, nd) = 0d0
+        if (dd > 1) be(1:dd - 1, nd) = st(i)%b1(1:dd - 1)
+      end if
+    end do
+    rc = pk_batch_bsvd(batch, nd, bn, bd, be, ldb, bs, bb)
+    if (rc /= PK_OK) then
+      pk_lansvd_irl_batch = rc
+      do i = 1, nb
+        if (active(i) /= 0) then
+          st(i)%info = rc
+          st(i)%failed = .true.
+          st(i)%done = .true.
+        end if
+      end do
+      exit
+    end if
+    do j = 1, nd
+      i = idx(j)
+      dd = st(i)%dim
+      c1 = st(i)%wb(dd)
+      st(i)%a1(1:dd) = bs(1:dd, j)
+      st(i)%wb(1:dd) = c1 * bb(1:dd, j)
+    end do
+    call pkx_parallel_for(nb, c_funloc(post_one), c_loc(ctx))
   end do
   do i = 1, nb
     dd = dim
