@@ -76,10 +76,12 @@ def test_batch_matches_sequential_fp64():
         assert np.array_equal(r.sigma, rb.sigma)
         assert nopx == rb.counters["nopx"]
     # the per-problem engine against the reference on two of them
-    for (prm, A, op, eng, u0), (r, _) in list(zip(probs, seq))[:2]:
+    for (prm, A, op, eng, u0), (r, nopx) in list(zip(probs, seq))[:2]:
         ref = refpy.lansvd_irl(A, NEIG, DIM, u0=u0, tol=TOL, cgs=1, jobu="n", jobv="n")
+        print("ref", ref.counters, "engine nopx", nopx)
         assert ref.k == r.k
         np.testing.assert_allclose(r.sigma, ref.sigma, rtol=1e-10)
+        assert nopx == ref.counters["nopx"]
 
 
 def test_batch_fp32_basis():

@@ -51,7 +51,10 @@ def test_lansvd_device_operator(small_case, cgs):
     _check_triplets(r, D, 1e-11, s[0])
     assert _cos(r.U, ref.U).min() > 1 - 1e-8
     assert _cos(r.V, ref.V).min() > 1 - 1e-8
-    assert abs(r.counters["nopx"] - ref.counters["nopx"]) <= 0.1 * ref.counters["nopx"]
+    # the reference's counters exactly (ioption(1) = 0 runs block CGS here
+    # against the reference's MGS; the counts agree on this case too)
+    for key in ("nopx", "nreorth", "ndot", "nbsvd", "nlandim", "nsing"):
+        assert r.counters[key] == ref.counters[key], key
 
 
 def test_lansvd_irl_device_operator(small_case):
@@ -65,7 +68,8 @@ def test_lansvd_irl_device_operator(small_case):
     np.testing.assert_allclose(r.sigma, ref.sigma, rtol=1e-10)
     _check_triplets(r, D, 1e-11, s[0])
     assert _cos(r.U, ref.U).min() > 1 - 1e-8
-    assert abs(r.counters["nrestart"] - ref.counters["nrestart"]) <= 2
+    for key in ("nopx", "nreorth", "ndot", "nrestart", "nbsvd", "nlandim"):
+        assert r.counters[key] == ref.counters[key], key
 
 
 def test_lansvd_host_callback(small_case):
@@ -298,6 +302,12 @@ def test_irl_smallest_triplets():
     print("ref", ref.info, ref.k, ref.sigma, ref.counters)
     assert r.k == ref.k and r.k > 0
     np.testing.assert_allclose(np.sort(r.sigma), np.sort(ref.sigma), rtol=1e-9)
+    # counters: exact, but ndot within 1 % (a second CGS pass is decided at
+    # the rounding level when the reference projects INDEX interval by
+    # interval and the engine all intervals in one block)
+    for key in ("nopx", "nreorth", "nrestart", "nbsvd"):
+        assert r.counters[key] == ref.counters[key], key
+    assert abs(r.counters["ndot"] - ref.counters["ndot"]) <= 0.01 * ref.counters["ndot"]
     smallest = np.sort(s)[: r.k]
     np.testing.assert_allclose(np.sort(r.sigma), smallest, rtol=1e-8)
 
