@@ -76,9 +76,18 @@ int pk_csr_spmv(pk_csr_t op, char trans, const void* x, void* y, int prec, int w
  * block's entries stored in increasing column order so that the gathers of
  * nearby columns share cache lines; built on the device, 12 bytes per
  * nonzero on top of the row-block layout when a panel's width and the cap
- * fit one 32-bit word, 14 otherwise), the row-block layout (cap = 0) or the
+ * fit one 32-bit word, 14 otherwise), the row-block layout (cap = 0), the
  * row-wise layout (cap = -1: each row's lanes sum the row-block arrays
- * straight from memory, no LDS staging).
+ * straight from memory, no LDS staging), or the column-sorted layout with
+ * every column panel in one launch (cap in [-16384, -1024], blocks of at
+ * most -cap nonzeros: each panel's FP64 row sums go to an array of their
+ * own, added in panel order by a second kernel -- the same sums as panel
+ * by panel -- and the workgroups are ordered so that each XCD works on one
+ * panel at a time; np * rows doubles more per operator and per engine.  A
+ * direction with one panel, rows longer than 1024 nonzeros, or panels with
+ * different lanes per row runs its panels one launch each; pk_csr_sort
+ * reports -cap only when the panels run together; an engine created before
+ * the switch keeps the panel-by-panel path).
  * Row sums are bit-identical to the row-block layout's for the same lanes;
  * only the per-block partial sums of the norms are grouped differently.
  * Not while a solve that uses the operator runs.  A batch (pk_batch_create)
@@ -90,8 +99,9 @@ int pk_csr_spmv(pk_csr_t op, char trans, const void* x, void* y, int prec, int w
  * lanes_a and lanes_t are not NULL, lane shifts -1, 0 and 1 on each (-2
  * too when sort_a and sort_t are not NULL), and,
  * when sort_a and sort_t are not NULL, each in the row-block layout, the
- * row-wise layout and the sorted layout with caps 8192 and 16384 (reported
- * as 0, -1, 8192, 16384); times each direction on the
+ * row-wise layout and the sorted layout with caps 8192 and 16384, panel by
+ * panel and all panels in one launch (reported as 0, -1, 8192, 16384,
+ * -8192, -16384); times each direction on the
  * device and returns the fastest panel counts (and lane shifts, block caps)
  * and their microseconds per application (inputs as pk_csr_create; nothing
  * is kept).  Settings differ only in how sums are grouped. */

@@ -142,8 +142,10 @@ class CsrOperator:
         return sa.value, st.value
 
     def set_sort(self, cap_a: int, cap_t: int):
-        """Column-sorted layout with blocks of at most cap nonzeros per
-        direction (0: row-block layout), pk_csr_set_sort."""
+        """SpMV layout per direction (pk_csr_set_sort): cap > 0 the
+        column-sorted layout with blocks of at most cap nonzeros, -cap the
+        same with every column panel in one launch, 0 the row-block layout,
+        -1 the row-wise layout."""
         _check(lib().pk_csr_set_sort(self.handle, C.c_int(cap_a), C.c_int(cap_t)),
                "pk_csr_set_sort")
 

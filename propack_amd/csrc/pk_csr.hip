@@ -397,9 +397,96 @@ out:
 
 }  // namespace
 
+namespace {
+
+void free_concurrent(CsrOp::Dir& D) {
+  (void)hipFree(D.ctable);
+  (void)hipFree(D.cmap);
+  (void)hipFree(D.ypart);
+  D.ctable = nullptr;
+  D.cmap = nullptr;
+  D.ypart = nullptr;
+  D.cmap_n = 0;
+  D.concurrent = 0;
+}
+
+// The concurrent launch's workgroup order.  Workgroups are dispatched to the
+// GPU's eight XCDs in turn (workgroup w to XCD w mod 8), each XCD with its
+// own L2: XCD x is given whole panels -- panel x / (8 / np) with every
+// (8 / np)-th of its blocks when np <= 8, panels x, x + 8, ... one after the
+// other when np = 16 -- so that each L2 holds the slice of x of one panel at
+// a time.  The order decides only where blocks run, never what they sum.
+int build_concurrent(CsrOp::Dir& D) {
+  constexpr int kX = 8;
+  std::vector<std::vector<int2>> lists(kX);
+  for (int x = 0; x < kX; ++x) {
+    if (D.np <= kX) {
+      const int g = kX / D.np, p = x / g, j = x % g;
+      for (int i = j; i < D.pan[p].spart.ndesc; i += g) lists[x].push_back(int2{p, i});
+    } else {
+      for (int p = x; p < D.np; p += kX)
+        for (int i = 0; i < D.pan[p].spart.ndesc; ++i) lists[x].push_back(int2{p, i});
+    }
+  }
+  size_t w = 0;
+  for (const auto& l : lists) w = std::max(w, l.size());
+  std::vector<int2> map(w * kX, int2{-1, -1});
+  for (size_t k = 0; k < w; ++k)
+    for (int x = 0; x < kX; ++x)
+      if (k < lists[x].size()) map[k * kX + x] = lists[x][k];
+  SortPanels t{};
+  for (int p = 0; p < D.np; ++p) {
+    const CsrOp::Panel& P = D.pan[p];
+    t.desc[p] = P.spart.desc;
+    t.rp[p] = P.rp;
+    t.sci[p] = P.sci;
+    t.sva[p] = P.sva;
+    t.spos[p] = P.spos;
+    t.col0[p] = P.scol0;
+    t.pbits[p] = P.spbits;
+  }
+  const size_t ny = static_cast<size_t>(D.np) * std::max(D.rows, 1);
+  if (hipMalloc(&D.ctable, sizeof(SortPanels)) != hipSuccess ||
+      hipMalloc(&D.cmap, sizeof(int2) * std::max<size_t>(map.size(), 1)) != hipSuccess ||
+      hipMalloc(&D.ypart, sizeof(double) * ny) != hipSuccess ||
+      hipMemcpy(D.ctable, &t, sizeof t, hipMemcpyHostToDevice) != hipSuccess ||
+      hipMemcpy(D.cmap, map.data(), sizeof(int2) * map.size(), hipMemcpyHostToDevice) !=
+          hipSuccess) {
+    free_concurrent(D);
+    return PK_ERR_HIP;
+  }
+  D.cmap_n = static_cast<int>(map.size());
+  D.concurrent = 1;
+  return PK_OK;
+}
+
+}  // namespace
+
+int csr_sort_code(const CsrOp::Dir& D) {
+  if (D.rowwise) return -1;
+  return D.concurrent ? -D.sort_cap : D.sort_cap;
+}
+
+bool csr_concurrent_ok(const CsrOp::Dir& D) {
+  if (!D.concurrent || D.np <= 1 || D.sort_cap <= 0 || D.cmap_n <= 0) return false;
+  for (int p = 1; p < D.np; ++p)
+    if (D.pan[p].g != D.pan[0].g) return false;
+  return true;
+}
+
+long long csr_ywork(const CsrOp* op) {
+  long long w = std::max<long long>(std::max(op->m, op->n), 1);
+  for (const CsrOp::Dir* D : {&op->da, &op->dt})
+    if (D->concurrent) w = std::max(w, static_cast<long long>(D->np) * D->rows);
+  return w;
+}
+
 int csr_set_sort(CsrOp* op, CsrOp::Dir& D, int cap) {
   ++op->layout_epoch;
+  const bool conc = cap <= -kSpmvCap;
+  if (conc) cap = -cap;
   if (cap != 0 && cap != -1 && (cap < kSpmvCap || cap > kSortCapMax)) return PK_ERR_ARG;
+  free_concurrent(D);
   for (int p = 0; p < D.np; ++p) free_sorted(D.pan[p]);
   D.sort_cap = 0;
   D.rowwise = cap == -1;
@@ -421,6 +508,18 @@ int csr_set_sort(CsrOp* op, CsrOp::Dir& D, int cap) {
     return rc;
   }
   D.sort_cap = cap;
+  if (conc && D.np > 1) {
+    bool chunks = false;
+    for (int p = 0; p < D.np; ++p) chunks = chunks || D.pan[p].spart.nlong > 0;
+    if (!chunks) {
+      rc = build_concurrent(D);
+      if (rc != PK_OK) {
+        for (int p = 0; p < D.np; ++p) free_sorted(D.pan[p]);
+        D.sort_cap = 0;
+        return rc;
+      }
+    }
+  }
   return PK_OK;
 }
 
@@ -531,6 +630,7 @@ static void free_dir(CsrOp::Dir& D) {
   }
   (void)hipFree(D.yacc);
   D.yacc = nullptr;
+  free_concurrent(D);
 }
 
 int csr_create(int m, int n, long long nnz, const int* rowptr, const int* colind,
@@ -623,9 +723,25 @@ void csr_destroy(CsrOp* op) {
 // sums stay in FP64 between them).  Returns the partial count or < 0.
 template <typename T>
 int csr_apply(const CsrOp* op, hipStream_t s, char trans, const T* x, T* y, const T* yprev,
-              Scal coef, double* part, Gate gate, double* ywork, double* lwork) {
+              Scal coef, double* part, Gate gate, double* ywork, double* lwork,
+              long long ywork_len) {
   const bool tr = (trans == 't' || trans == 'T');
   const CsrOp::Dir& D = tr ? op->dt : op->da;
+  if (csr_concurrent_ok(D)) {
+    const long long need = static_cast<long long>(D.np) * D.rows;
+    double* yp = ywork ? (std::max<long long>(ywork_len, D.rows) >= need ? ywork : nullptr)
+                       : D.ypart;
+    if (yp) {
+      const int rc =
+          op->val_prec == PK_F64
+              ? launch_spmv_sorted_cp<double, T>(s, D.ctable, D.cmap, D.cmap_n, D.sort_cap,
+                                                 D.pan[0].g, D.rows, x, yp, gate)
+              : launch_spmv_sorted_cp<float, T>(s, D.ctable, D.cmap, D.cmap_n, D.sort_cap,
+                                                D.pan[0].g, D.rows, x, yp, gate);
+      if (rc < 0) return rc;
+      return launch_spmv_combine<T>(s, D.rows, D.np, yp, yprev, coef, y, part, gate);
+    }
+  }
   double* yacc = ywork ? ywork : D.yacc;
   int nb = -1;
   for (int p = 0; p < D.np; ++p) {
@@ -665,9 +781,9 @@ int csr_apply(const CsrOp* op, hipStream_t s, char trans, const T* x, T* y, cons
 }
 
 template int csr_apply<double>(const CsrOp*, hipStream_t, char, const double*, double*,
-                               const double*, Scal, double*, Gate, double*, double*);
+                               const double*, Scal, double*, Gate, double*, double*, long long);
 template int csr_apply<float>(const CsrOp*, hipStream_t, char, const float*, float*,
-                              const float*, Scal, double*, Gate, double*, double*);
+                              const float*, Scal, double*, Gate, double*, double*, long long);
 
 }  // namespace pk
 
@@ -877,8 +993,8 @@ int pk_csr_autotune(int m, int n, long long nnz, const int* rowptr, const int* c
   if (!panels_a || !panels_t) return PK_ERR_ARG;
   const bool lanes = lanes_a && lanes_t;
   const bool sorts = sort_a && sort_t;
-  static const int kCaps[4] = {0, -1, 8192, 16384};
-  const int ncaps = sorts ? 4 : 1;
+  static const int kCaps[6] = {0, -1, 8192, 16384, -8192, -16384};
+  const int ncaps = sorts ? 6 : 1;
   // PK_AUTOTUNE_LOG=1: one line per timed setting on stderr
   const char* lg = std::getenv("PK_AUTOTUNE_LOG");
   const bool verbose = lg && *lg && *lg != '0';
@@ -892,6 +1008,7 @@ int pk_csr_autotune(int m, int n, long long nnz, const int* rowptr, const int* c
     const int npa = o->da.np, npt = o->dt.np;
     for (int ic = 0; ic < ncaps && rc == PK_OK; ++ic) {
       const int cap = kCaps[ic];
+      if (cap < -1 && npa == 1 && npt == 1) continue;  // one panel: the sorted layout itself
       const int rca = csr_set_sort(o, o->da, cap), rct = csr_set_sort(o, o->dt, cap);
       if (rca == PK_ERR_NOTSUP || rct == PK_ERR_NOTSUP) continue;  // more slots than built
       rc = rca != PK_OK ? rca : rct;
@@ -914,13 +1031,13 @@ int pk_csr_autotune(int m, int n, long long nnz, const int* rowptr, const int* c
           best_a = ta;
           pa = P;
           sa = sh;
-          ca = cap;
+          ca = csr_sort_code(o->da);
         }
         if (npt == P && tt < best_t) {
           best_t = tt;
           pt = P;
           st = sh;
-          ct = cap;
+          ct = csr_sort_code(o->dt);
         }
       }
     }
@@ -945,12 +1062,14 @@ int pk_csr_autotune(int m, int n, long long nnz, const int* rowptr, const int* c
 int pk_csr_set_sort(pk_csr_t h, int cap_a, int cap_t) {
   CsrOp* op = as_csr(h);
   if (!op) return PK_ERR_HANDLE;
-  for (const int c : {cap_a, cap_t})
-    if (c != 0 && c != -1 && (c < kSpmvCap || c > kSortCapMax)) {
-      set_error("pk_csr_set_sort: a block cap must be 0, -1 or lie in [%d, %d]", kSpmvCap,
-                kSortCapMax);
+  for (const int c : {cap_a, cap_t}) {
+    const int a = c < -1 ? -c : c;
+    if (c != 0 && c != -1 && (a < kSpmvCap || a > kSortCapMax)) {
+      set_error("pk_csr_set_sort: a block cap must be 0, -1 or lie in [%d, %d] (negated: "
+                "every column panel in one launch)", kSpmvCap, kSortCapMax);
       return PK_ERR_ARG;
     }
+  }
   int rc = csr_set_sort(op, op->da, cap_a);
   if (rc == PK_OK) rc = csr_set_sort(op, op->dt, cap_t);
   if (rc != PK_OK) {
@@ -965,8 +1084,8 @@ int pk_csr_set_sort(pk_csr_t h, int cap_a, int cap_t) {
 int pk_csr_sort(pk_csr_t h, int* cap_a, int* cap_t) {
   CsrOp* op = as_csr(h);
   if (!op) return PK_ERR_HANDLE;
-  if (cap_a) *cap_a = op->da.rowwise ? -1 : op->da.sort_cap;
-  if (cap_t) *cap_t = op->dt.rowwise ? -1 : op->dt.sort_cap;
+  if (cap_a) *cap_a = csr_sort_code(op->da);
+  if (cap_t) *cap_t = csr_sort_code(op->dt);
   return PK_OK;
 }
 
@@ -1048,9 +1167,10 @@ int pk_csr_spmv_noalloc(pk_csr_t h, char trans, const void* x, void* y, int prec
       prec == PK_F64
           ? csr_apply<double>(op, s, trans, static_cast<const double*>(x), static_cast<double*>(y),
                               nullptr, sval(0.0), base, Gate{}, base + npart,
-                              base + npart + nyacc)
+                              base + npart + nyacc, static_cast<long long>(nyacc))
           : csr_apply<float>(op, s, trans, static_cast<const float*>(x), static_cast<float*>(y),
-                             nullptr, sval(0.0), base, Gate{}, base + npart, base + npart + nyacc);
+                             nullptr, sval(0.0), base, Gate{}, base + npart, base + npart + nyacc,
+                             static_cast<long long>(nyacc));
   if (nb < 0) {
     set_error("pk_csr_spmv_noalloc: kernel launch failed");
     return PK_ERR_HIP;

@@ -218,7 +218,7 @@ struct Impl {
     if (e->op) {
       hipEvent_t t0 = tstart(tr ? KC_SPMV_T : KC_SPMV_N);
       nb = csr_apply<T>(e->op, strm, trans, x, y, yprev, sval(c), e->part, Gate{}, e->sw_y,
-                        e->sw_l);
+                        e->sw_l, e->sw_y_len);
       PK_LAUNCH(nb);
       tstop(t0, tr ? KC_SPMV_T : KC_SPMV_N, spmv_bytes(tr, yprev != nullptr),
             2.0 * static_cast<double>(e->op->nnz));
@@ -578,7 +578,7 @@ struct Impl {
     hipEvent_t t0 = tstart(KC_SPMV_T);
     int nb = csr_apply<T>(e->op, strm, 't', ucol(J), vcol(J), J > 1 ? vcol(J - 1) : nullptr,
                           sptr(&dc->beta), e->part, Gate{dc, PH_V_SPMV, J, 0}, e->sw_y,
-                          e->sw_l);
+                          e->sw_l, e->sw_y_len);
     PK_LAUNCH(nb);
     tstop(t0, KC_SPMV_T, spmv_bytes(true, J > 1), 2.0 * static_cast<double>(e->op->nnz));
     PK_LAUNCH(launch_ctl(strm, dc, e->darr, e->kmax, CTL_V_SPMV, PH_V_SPMV, J, e->part, nb));
@@ -609,7 +609,7 @@ struct Impl {
     PK_LAUNCH(launch_scale<T>(strm, n, vcol(J), sptr(&dc->vscale_f), Gate{dc, PH_U_SPMV, J, 1}));
     hipEvent_t t3 = tstart(KC_SPMV_N);
     nb = csr_apply<T>(e->op, strm, 'n', vcol(J), ucol(J + 1), ucol(J), sptr(&dc->alpha), e->part,
-                      Gate{dc, PH_U_SPMV, J, 0}, e->sw_y, e->sw_l);
+                      Gate{dc, PH_U_SPMV, J, 0}, e->sw_y, e->sw_l, e->sw_y_len);
     PK_LAUNCH(nb);
     tstop(t3, KC_SPMV_N, spmv_bytes(false, true), 2.0 * static_cast<double>(e->op->nnz));
     PK_LAUNCH(launch_ctl(strm, dc, e->darr, e->kmax, CTL_U_SPMV, PH_U_SPMV, J, e->part, nb));
@@ -1090,7 +1090,8 @@ int pk_lbd_create(int prec, int m, int n, int kmax, pk_csr_t op, pk_lbd_t* out) 
   chk(hipMalloc(&e->dB, sizeof(double) * nb));
   chk(hipMalloc(&e->vec, e->tbytes * mx * 2));
   if (cop) {
-    chk(hipMalloc(&e->sw_y, sizeof(double) * mx));
+    e->sw_y_len = csr_ywork(cop);
+    chk(hipMalloc(&e->sw_y, sizeof(double) * e->sw_y_len));
     chk(hipMalloc(&e->sw_l, sizeof(double) * csr_lwork(cop)));
   }
   chk(hipHostMalloc(&e->hred, sizeof(double) * (e->ccap + 16), hipHostMallocDefault));

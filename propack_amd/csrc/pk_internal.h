@@ -58,6 +58,13 @@ struct CsrOp {
     int lane_shift = 0;      // lanes per row = built-in choice * 2^lane_shift
     int sort_cap = 0;        // > 0: column-sorted layout, blocks of <= sort_cap nonzeros
     int rowwise = 0;         // row-block arrays summed row-wise, no LDS staging (k_spmv_vec)
+    // column-sorted layout with every panel in one launch (np > 1): per-panel
+    // FP64 row sums (np * rows doubles) combined in panel order afterwards
+    int concurrent = 0;
+    SortPanels* ctable = nullptr;  // device copy of the panels' sorted arrays
+    int2* cmap = nullptr;          // (panel, descriptor) of each workgroup
+    int cmap_n = 0;
+    double* ypart = nullptr;       // the operator's own per-panel row sums
   } da, dt;
   int nrb = 0, nrbt = 0;  // partial slots per direction
   // bumped by every layout change (lanes per row, sorted layout): a batch
@@ -97,15 +104,28 @@ void csr_set_lanes(CsrOp::Dir& D, int shift);
 // most cap nonzeros (kSpmvCap <= cap <= kSortCapMax; built on the device),
 // or back to the row-block layout (cap = 0, the sorted copy freed), or to
 // the row-block arrays read row-wise (cap = -1).
+// cap <= -kSpmvCap: the column-sorted layout with blocks of at most -cap
+// nonzeros and every column panel in one launch (concurrent panels; kept
+// sequential when the direction has one panel or long-row chunks).
 int csr_set_sort(CsrOp* op, CsrOp::Dir& D, int cap);
+// The layout code pk_csr_sort reports for a direction: -1 row-wise, -cap
+// concurrent panels, cap sorted, 0 row-block.
+int csr_sort_code(const CsrOp::Dir& D);
+// Whether a direction runs its panels in one launch (concurrent layout, one
+// lanes-per-row count across its panels), and the running-sum scratch an
+// application needs (doubles): np * rows then, rows otherwise.
+bool csr_concurrent_ok(const CsrOp::Dir& D);
+long long csr_ywork(const CsrOp* op);
 // ywork (rows of the direction, doubles) and lwork (csr_lwork(op) doubles),
 // when given, replace the operator's own running row sums and long-row
 // partial sums, so that callers with their own scratch (every engine) may
 // apply one operator at the same time on different streams.
+// ywork_len (doubles; 0: rows) bounds ywork: a concurrent direction whose
+// np * rows does not fit runs its panels one launch each instead.
 template <typename T>
 int csr_apply(const CsrOp* op, hipStream_t s, char trans, const T* x, T* y, const T* yprev,
               Scal coef, double* part, Gate gate, double* ywork = nullptr,
-              double* lwork = nullptr);
+              double* lwork = nullptr, long long ywork_len = 0);
 // Long-row partial sums one panel can need, for any layout of op: a row of
 // len > kSpmvCap nonzeros has ceil(len / kLongChunk) <= len / kSpmvCap chunks.
 inline long long csr_lwork(const CsrOp* op) { return op->nnz / 1024 + 2; }
@@ -170,7 +190,8 @@ struct Engine {
   double* hB = nullptr;              // pinned host staging
   hipEvent_t evB = nullptr;          // the last copy out of hB (reuse guard)
   void* vec = nullptr;               // scratch vector, max(m, n)
-  double* sw_y = nullptr;            // SpMV running row sums (max(m, n)), the engine's own
+  double* sw_y = nullptr;            // SpMV running row sums (csr_ywork(op) at creation)
+  long long sw_y_len = 0;
   double* sw_l = nullptr;            // SpMV long-row partial sums (csr_lwork(op))
   void* kres = nullptr;              // residual check: three vectors of max(m, n), on first use
   LbdCtl* dctl = nullptr;            // step control state: scalars (device)

@@ -48,6 +48,33 @@ int launch_spmv_sorted(hipStream_t s, int nblk, const int4* desc, int nlong, con
                        T* y, const T* yprev, Scal coef, double* part, int g,
                        const double* yacc_in, double* yacc_out, int last, Gate gate);
 
+// Column-sorted layout with every column panel in one launch: the panels'
+// arrays (device table), workgroup w runs descriptor map[w].y of panel
+// map[w].x (x < 0: nothing) and stores that panel's FP64 row sums in
+// ypart[panel * rows + row] (the panels' blocks must be free of long-row
+// chunks and share one lanes-per-row count g).  The combine then forms, per
+// row, ((ypart[0] + ypart[1]) + ...) in panel order -- the sums the
+// one-launch-per-panel path forms in its running sums -- and the epilogue
+// y = T(sum - coef * yprev) with partials of ||y||^2 and yprev.y over the
+// stream_blocks(rows) grid (returned).
+constexpr int kMaxSortPanels = 16;
+struct SortPanels {
+  const int4* desc[kMaxSortPanels];
+  const int* rp[kMaxSortPanels];
+  const int* sci[kMaxSortPanels];
+  const void* sva[kMaxSortPanels];
+  const unsigned short* spos[kMaxSortPanels];
+  int col0[kMaxSortPanels];
+  int pbits[kMaxSortPanels];
+};
+template <typename VT, typename T>
+int launch_spmv_sorted_cp(hipStream_t s, const SortPanels* table, const int2* map, int nmap,
+                          int cap, int g, long long rows, const T* x, double* ypart,
+                          Gate gate);
+template <typename T>
+int launch_spmv_combine(hipStream_t s, long long rows, int np, const double* ypart,
+                        const T* yprev, Scal coef, T* y, double* part, Gate gate);
+
 // part (nval x nblk) -> out[v] = sum_b part[v*nblk+b] in index order.
 int launch_reduce(hipStream_t s, int nval, int nblk, const double* part, double* out,
                   Gate gate, const int* nval_p = nullptr);

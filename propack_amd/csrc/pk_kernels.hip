@@ -176,6 +176,90 @@ int launch_spmv_sorted(hipStream_t s, int nblk, const int4* desc, int nlong, con
   return hipGetLastError() == hipSuccess ? nbt : -1;
 }
 
+// -------------------------------------------- sorted SpMV, panels at once --
+template <typename VT, typename T, int G>
+__global__ __launch_bounds__(kSortThreads) void k_spmv_sorted_cp(
+    const SortPanels* __restrict__ table, const int2* __restrict__ map, long long rows,
+    const T* __restrict__ x, double* __restrict__ ypart, Gate gate) {
+  if (gate.off()) return;
+  const int2 mp = map[blockIdx.x];
+  if (mp.x < 0) return;
+  extern __shared__ double prod_dyn[];
+  __shared__ double lds[(kSortThreads / kWave) * 2];
+  const int p = mp.x;
+  spmv_sorted_body<VT, T, G>(table->desc[p][mp.y], table->rp[p], table->sci[p],
+                             static_cast<const VT*>(table->sva[p]), table->spos[p],
+                             table->col0[p], table->pbits[p], x, nullptr, nullptr, 0.0,
+                             nullptr, nullptr, nullptr, nullptr,
+                             ypart + static_cast<long long>(p) * rows, 0, prod_dyn, lds);
+}
+
+template <typename VT, typename T>
+int launch_spmv_sorted_cp(hipStream_t s, const SortPanels* table, const int2* map, int nmap,
+                          int cap, int g, long long rows, const T* x, double* ypart,
+                          Gate gate) {
+  if (cap <= 0 || cap > kSortCapMax || nmap <= 0) return -1;
+  const size_t shm = sizeof(double) * static_cast<size_t>(cap);
+  switch (g) {
+#define PK_CP_CASE(G)                                                                       \
+  case G:                                                                                   \
+    hipLaunchKernelGGL((k_spmv_sorted_cp<VT, T, G>), dim3(nmap), dim3(kSortThreads), shm, s, \
+                       table, map, rows, x, ypart, gate);                                   \
+    break;
+    PK_CP_CASE(2)
+    PK_CP_CASE(4)
+    PK_CP_CASE(8)
+    PK_CP_CASE(16)
+    PK_CP_CASE(32)
+    PK_CP_CASE(64)
+#undef PK_CP_CASE
+    default:
+      return -1;
+  }
+  return hipGetLastError() == hipSuccess ? 0 : -1;
+}
+
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_spmv_combine(long long L, int np,
+                                                         const double* __restrict__ ypart,
+                                                         const T* __restrict__ yprev,
+                                                         Scal coef, T* __restrict__ y,
+                                                         double* __restrict__ part,
+                                                         Gate gate) {
+  if (gate.off()) return;
+  __shared__ double lds[(kBlock / kWave) * 2];
+  const double c = coef.get();
+  const long long base = static_cast<long long>(blockIdx.x) * kRowsPerBlock + threadIdx.x;
+  double v[2] = {0.0, 0.0};
+#pragma unroll
+  for (int r = 0; r < kRPT; ++r) {
+    const long long row = base + static_cast<long long>(r) * kBlock;
+    if (row < L) {
+      double acc = ypart[row];
+      for (int p = 1; p < np; ++p) acc = acc + ypart[static_cast<long long>(p) * L + row];
+      const double yp = yprev ? static_cast<double>(yprev[row]) : 0.0;
+      const T yv = static_cast<T>(acc - c * yp);
+      y[row] = yv;
+      v[0] += static_cast<double>(yv) * static_cast<double>(yv);
+      v[1] += yp * static_cast<double>(yv);
+    }
+  }
+  block_sum<2>(v, lds);
+  if (threadIdx.x == 0) {
+    part[blockIdx.x] = v[0];
+    part[gridDim.x + blockIdx.x] = v[1];
+  }
+}
+
+template <typename T>
+int launch_spmv_combine(hipStream_t s, long long rows, int np, const double* ypart,
+                        const T* yprev, Scal coef, T* y, double* part, Gate gate) {
+  const int nblk = stream_blocks(rows);
+  hipLaunchKernelGGL(k_spmv_combine<T>, dim3(nblk), dim3(kBlock), 0, s, rows, np, ypart, yprev,
+                     coef, y, part, gate);
+  return hipGetLastError() == hipSuccess ? nblk : -1;
+}
+
 // -------------------------------------------------------------- reduce ----
 __global__ __launch_bounds__(kBlock) void k_reduce(int nblk, const double* __restrict__ part,
                                                    double* __restrict__ out,
@@ -638,6 +722,18 @@ PK_INST_SO(float, float)
 PK_INST_SO(double, float)
 PK_INST_SO(float, double)
 #undef PK_INST_SO
+#define PK_INST_CP(VT, T)                                                                    \
+  template int launch_spmv_sorted_cp<VT, T>(hipStream_t, const SortPanels*, const int2*, int, \
+                                            int, int, long long, const T*, double*, Gate);
+PK_INST_CP(double, double)
+PK_INST_CP(float, float)
+PK_INST_CP(double, float)
+PK_INST_CP(float, double)
+#undef PK_INST_CP
+template int launch_spmv_combine<double>(hipStream_t, long long, int, const double*,
+                                         const double*, Scal, double*, double*, Gate);
+template int launch_spmv_combine<float>(hipStream_t, long long, int, const double*,
+                                        const float*, Scal, float*, double*, Gate);
 
 template int launch_convert<double, float>(hipStream_t, long long, const double*, float*);
 template int launch_convert<float, double>(hipStream_t, long long, const float*, double*);

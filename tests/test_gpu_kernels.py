@@ -391,10 +391,76 @@ def test_rowwise_same_solve(small_case):
             assert np.array_equal(op1.spmv(v, trans), op0.spmv(v, trans))
 
 
+@pytest.mark.parametrize("cap", [8192, 16384])
+def test_concurrent_panels_match_sequential(cap):
+    """Column-sorted layout with every column panel in one launch (the cap
+    negated): each panel's row sums go to an FP64 array of their own and
+    are added in panel order afterwards -- the sums the one-launch-per-panel
+    path forms in its running sums -- so every output element is bit for
+    bit that path's (only the norm partials are grouped differently).  2, 4,
+    8 and 16 panels (16: two panels per XCD in turn); FP64 and FP32.  One
+    panel, or long-row chunks in a panel (the power-law matrix), keep the
+    panels sequential, and the operator reports the plain cap."""
+    rng = np.random.default_rng(cap)
+    A = synth.poisson_decay(200_000, 100_000, 16.0, 0.6, seed=4)
+    x, y = rng.standard_normal(A.n), rng.standard_normal(A.m)
+    for panels in ((2, 4), (4, 8), (16, 16)):
+        op0 = api.CsrOperator(A, panels=panels, sort=(cap, cap))
+        op1 = api.CsrOperator(A, panels=panels, sort=(-cap, -cap))
+        assert op1.sort() == (-cap, -cap)
+        for trans, v in (("n", x), ("t", y)):
+            got = op1.spmv(v, trans)
+            assert np.array_equal(got, op0.spmv(v, trans)), (panels, trans)
+            assert np.array_equal(got, op1.spmv(v, trans))
+    A32 = A.astype(np.float32)
+    op0 = api.CsrOperator(A32, prec=api.PK_F32, panels=(4, 8), sort=(cap, cap))
+    op1 = api.CsrOperator(A32, prec=api.PK_F32, panels=(4, 8), sort=(-cap, -cap))
+    x32, y32 = x.astype(np.float32), y.astype(np.float32)
+    assert np.array_equal(op1.spmv(x32, "n"), op0.spmv(x32, "n"))
+    assert np.array_equal(op1.spmv(y32, "t"), op0.spmv(y32, "t"))
+    assert api.CsrOperator(A, panels=(1, 1), sort=(-cap, -cap)).sort() == (cap, cap)
+    P = synth.powerlaw_bipartite(400_000, 200_000, seed=3)
+    opp = api.CsrOperator(P, panels=(4, 4), sort=(-cap, -cap))
+    assert opp.sort()[1] == cap  # A^T has rows longer than 1024: sequential panels
+    xp = rng.standard_normal(P.m)
+    assert np.array_equal(opp.spmv(xp, "t"),
+                          api.CsrOperator(P, panels=(4, 4), sort=(cap, cap)).spmv(xp, "t"))
+
+
+def test_concurrent_panels_solve():
+    """A restarted solve on concurrent panels: the singular values of the
+    sequential-panel solve to 1e-12 and the same operator count.  An engine
+    created before its operator switched to concurrent panels keeps the
+    sequential path (its scratch holds one running-sum vector) and gives
+    that path's singular values bit for bit."""
+    B = synth.poisson_decay(200_000, 100_000, 16.0, 0.6, seed=4)
+    u1 = synth.start_vector(B.m, 1)
+    res = []
+    for sort in ((8192, 8192), (-8192, -8192)):
+        op = api.CsrOperator(B, panels=(4, 8), sort=sort)
+        eng = api.Engine(op, kmax=60)
+        eng.put_u1(u1)
+        res.append(api.lansvd_irl_dev(eng, 20, 60, tol=1e-10))
+        eng.close()
+    a, b = res
+    assert a.k == b.k == 20 and a.info == b.info == 0
+    np.testing.assert_allclose(b.sigma, a.sigma, rtol=1e-12)
+    assert a.counters["nopx"] == b.counters["nopx"]
+    for i in range(5):
+        r = B.matvec(b.V[:, i]) - b.sigma[i] * b.U[:, i]
+        assert np.linalg.norm(r) / b.sigma[0] < 1e-11
+    op = api.CsrOperator(B, panels=(4, 8), sort=(8192, 8192))
+    eng = api.Engine(op, kmax=60)
+    op.set_sort(-8192, -8192)
+    eng.put_u1(u1)
+    c = api.lansvd_irl_dev(eng, 20, 60, tol=1e-10)
+    assert np.array_equal(c.sigma, a.sigma)
+
+
 def test_sorted_layout_rejects_bad_caps():
     A = synth.random_uniform(3000, 1000, 20000, seed=5)
     op = api.CsrOperator(A)
-    for caps in ((100, 0), (0, 16385), (-2, 1024)):
+    for caps in ((100, 0), (0, 16385), (-2, 1024), (-100, 0), (0, -16385)):
         with pytest.raises(api.PropackError):
             op.set_sort(*caps)
         assert op.sort() == (0, 0)
