@@ -27,13 +27,15 @@ def _engine(m, n, kmax, prec=api.PK_F64):
 
 
 # (m, kmax, ncols, transb): every column-tile count of the LDS-staged
-# kernel (ncols 3, 12, 20, 40, 64, 70 = 64 + 6), row counts around its
-# 256-row blocks and below one 16-row tile, k extents up to its LDS limit
-# (kdim 201 and 313) and past it (kdim 321: the unstaged kernel)
+# kernel (ncols 3, 12, 20, 40, 64, 70 = 64 + 6), the restart and Ritz
+# shapes' column counts (51, 50), row counts around its 256-row blocks and
+# below one 16-row tile, k extents up to its LDS limit (kdim 201 and 313)
+# and past it (kdim 321: the unstaged kernel)
 @pytest.mark.parametrize("m,kmax,ncols,transb", [(1000, 37, 20, "n"), (2049, 80, 70, "n"),
                                                   (777, 30, 12, "t"), (64, 5, 3, "n"),
                                                   (10, 50, 33, "n"), (5000, 200, 64, "n"),
-                                                  (3001, 312, 40, "t"), (2000, 320, 40, "n")])
+                                                  (3001, 312, 40, "t"), (2000, 320, 40, "n"),
+                                                  (4097, 120, 51, "n"), (1500, 60, 50, "t")])
 @pytest.mark.parametrize("prec", [api.PK_F64, api.PK_F32])
 def test_basis_update_gemm(m, kmax, ncols, transb, prec):
     op, eng = _engine(m, 50, kmax, prec)
