@@ -247,8 +247,12 @@ int pk_lbd_eps(pk_lbd_t eng, double* eps, double* leff);
  * kernel per step for the whole batch, each problem gated by its own
  * control state; a problem gives the same results as alone.  One engine
  * may not appear twice in a batch (PK_ERR_ARG); problems may share an
- * operator (each engine carries its own SpMV scratch).  Per-problem host
- * work runs on up to 16 host threads (PK_HOST_THREADS overrides). */
+ * operator (each engine carries its own SpMV scratch).  An operator in the
+ * concurrent-panel layout (negative sorted cap) runs its panels one launch
+ * each inside a batch: the same row sums, norm partials grouped as panel by
+ * panel, so such a problem matches its own concurrent solve to rounding,
+ * not bit for bit.  Per-problem host work runs on up to 16 host threads
+ * (PK_HOST_THREADS overrides). */
 typedef struct pk_batch_opaque* pk_batch_t;
 int pk_batch_create(int nprob, const pk_lbd_t* engines, pk_batch_t* batch);
 int pk_batch_destroy(pk_batch_t batch);
