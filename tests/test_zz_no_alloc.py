@@ -1,6 +1,8 @@
 """pk_csr_spmv_noalloc, the SpMV entry point that allocates nothing (its
 scratch is a fixed device array per GPU), against pk_csr_spmv on the same
-operators: bit-identical outputs (the same kernels in the same order), on
+operators: bit-identical outputs (the same kernels in the same order; for
+an operator whose panels pk_csr_spmv runs in one launch, the same row sums
+panel by panel), on
 two calls in a row on one GPU, the second on a smaller operator -- the
 shared scratch still holds the first call's contents -- in both
 directions and both precisions; and an operator whose scratch would not
@@ -63,6 +65,14 @@ def test_noalloc_matches_spmv_two_calls(small_case, prec):
     # the sorted layout of the larger operator, after the smaller one ran
     op_big.set_sort(8192, 8192)
     _check(op_big, big, "t", 3)
+    # every panel in one launch for pk_csr_spmv; the no-allocation call runs
+    # the panels one launch each through its fixed scratch: the same bits
+    conc = synth.poisson_decay(40000, 20000, 12.0, 0.6, seed=43)
+    conc = synth.Csr(conc.m, conc.n, conc.rowptr, conc.col, conc.val.astype(dt))
+    op_conc = api.CsrOperator(conc, prec=prec, panels=(4, 8), sort=(-8192, -8192))
+    assert op_conc.sort() == (-8192, -8192)
+    for trans in ("n", "t"):
+        _check(op_conc, conc, trans, 4)
 
 
 def test_noalloc_refuses_oversized_scratch():
