@@ -401,21 +401,43 @@ def workload_batch(args):
                 out.append(api.lansvd_irl_dev(e, 16, 48, tol=1e-8, jobu="n", jobv="n",
                                               fetch=False))
             return out
-        b = api.Batch([p[1] for p in probs])
-        b.set_graph(not args.no_graph)
+        # --split G: G batches over interleaved problems (largest first, dealt
+        # round-robin), each run from a host thread of its own, so that one
+        # batch's host work between restarts overlaps the others' GPU work
+        order = sorted(range(len(probs)), key=lambda i: -probs[i][3].nnz)
+        groups = [order[g::args.split] for g in range(args.split)]
+        bs = [api.Batch([probs[i][1] for i in grp]) for grp in groups]
+        for b in bs:
+            b.set_graph(not args.no_graph)
 
         def bat(fetch=False):
             for op, e, u0, _ in probs:
                 e.put_u1(u0)
-            return api.lansvd_irl_batch(b, 16, 48, tol=1e-8, jobu="y" if fetch else "n",
-                                        jobv="y" if fetch else "n", fetch=fetch)
+            out = [None] * len(probs)
+
+            def run(g):
+                res = api.lansvd_irl_batch(bs[g], 16, 48, tol=1e-8, jobu="y" if fetch else "n",
+                                           jobv="y" if fetch else "n", fetch=fetch)
+                for i, r in zip(groups[g], res):
+                    out[i] = r
+            if len(bs) == 1:
+                run(0)
+            else:
+                ths = [threading.Thread(target=run, args=(g,)) for g in range(len(bs))]
+                for t in ths:
+                    t.start()
+                for t in ths:
+                    t.join()
+            return out
         rs, ts = _timed(seq, args.steps, sync)
         rb, tb = _timed(bat, args.steps, sync)
         same = all(np.array_equal(x.sigma, y.sigma) for x, y in zip(rs, rb))
         # the same batched solve with its bidiagonal SVDs on the device
-        b.set_bsvd(True)
+        for b in bs:
+            b.set_bsvd(True)
         rd, td = _timed(bat, args.steps, sync)
-        b.set_bsvd(False)
+        for b in bs:
+            b.set_bsvd(False)
         dev = {"problems_per_s": len(probs) / td,
                "converged": sum(r.k for r in rd),
                "max_rel_sigma_diff_vs_host_dbdsqr": float(max(
@@ -435,8 +457,9 @@ def workload_batch(args):
                         "bit_identical_to_sequential": same,
                         "max_rel_residual": worst, "device_bsvd": dev}
         log(f"[bench] {basis}: {forms[basis]}")
-        b.close()
-        del probs, b, rs, rb
+        for b in bs:
+            b.close()
+        del probs, bs, rs, rb
         gc.collect()
     f32 = forms["fp32"]
     out = {"metric": "batched truncated SVDs per second (k=16, dim=48, tol=1e-8, FP32 basis)",
@@ -448,7 +471,7 @@ def workload_batch(args):
            "config": {"workload": f"{len(mats)} problems of batch_params(256)", "nnz": nnz,
                       "spmv_layout": ("autotuned per problem" if args.tune_each else
                                       {"sort_caps": list(sort), "lane_shifts": list(lanes)}),
-                      "hip_graphs": not args.no_graph},
+                      "hip_graphs": not args.no_graph, "batches": args.split},
            "sequential_problems_per_s": f32["sequential_problems_per_s"],
            "speedup_vs_sequential": f32["speedup_vs_sequential"],
            "converged": f32["converged"], "requested": f32["requested"],
@@ -490,6 +513,9 @@ def main():
     ap.add_argument("--no-graph", action="store_true",
                     help="--workload batch: enqueue the batched step chains directly instead of "
                          "replaying captured HIP graphs")
+    ap.add_argument("--split", type=int, default=1,
+                    help="--workload batch: run the problems as this many batches, each from a "
+                         "host thread of its own")
     ap.add_argument("--tune-each", action="store_true",
                     help="--workload batch: autotune every problem's SpMV layout")
     ap.add_argument("--copy-calibration", action="store_true",
