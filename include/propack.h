@@ -8,10 +8,16 @@
  *   dlansvd_irl_  replaces DLANSVD_IRL  (double/dlansvd_irl.F:1-3)
  *   slansvd_      replaces SLANSVD      (single/slansvd.F:1-3)
  *   slansvd_irl_  replaces SLANSVD_IRL  (single/slansvd_irl.F:1-3)
+ *   zlansvd_      replaces ZLANSVD      (complex16/zlansvd.F:1-3)
+ *   zlansvd_irl_  replaces ZLANSVD_IRL  (complex16/zlansvd_irl.F:1-3)
+ *   clansvd_      replaces CLANSVD      (complex8/clansvd.F:1-3)
+ *   clansvd_irl_  replaces CLANSVD_IRL  (complex8/clansvd_irl.F:1-3)
  *   clearstat_ / printstat_ over COMMON /timing/ (double/printstat.F:4-75)
  *
  * APROD has the reference callback signature (double/dlanbpro.F:28-41).
- * Passing pk_dcsr_aprod_ / pk_scsr_aprod_ (propack_amd.h) with IPARM filled
+ * Complex arrays (COMPLEX*16 / COMPLEX) are (re, im) pairs, passed as void*.
+ * Passing pk_dcsr_aprod_ / pk_scsr_aprod_ / pk_zcsr_aprod_ / pk_ccsr_aprod_
+ * (propack_amd.h) with IPARM filled
  * by pk_csr_to_iparm keeps the operator on the GPU; any other APROD is
  * called on the host.  WORK/IWORK are accepted and unused; INFO <= -100
  * reports a device-layer error (PK_ERR_* of propack_amd.h).
@@ -57,6 +63,41 @@ void slansvd_irl_(const char* which, const char* jobu, const char* jobv, const i
                   int* iwork, const int* liwork, float* soption, int* ioption, int* info,
                   float* sparm, int* iparm, size_t len_which, size_t len_jobu,
                   size_t len_jobv);
+
+/* APROD of the complex trees (complex16/zlanbpro.F:28-41): TRANSA 'n' or
+ * 'c' (A^H). */
+typedef void (*pk_zaprod_t)(const char* transa, const int* m, const int* n, const void* x,
+                            void* y, void* zparm, int* iparm, size_t len_transa);
+
+void zlansvd_(const char* jobu, const char* jobv, const int* m, const int* n, int* k,
+              const int* kmax, pk_zaprod_t aprod, void* U, const int* ldu, double* sigma,
+              double* bnd, void* V, const int* ldv, const double* tolin, double* work,
+              const int* lwork, void* zwork, const int* lzwrk, int* iwork, const int* liwork,
+              double* doption, int* ioption, int* info, void* zparm, int* iparm,
+              size_t len_jobu, size_t len_jobv);
+
+void zlansvd_irl_(const char* which, const char* jobu, const char* jobv, const int* m,
+                  const int* n, int* dim, const int* p, int* neig, const int* maxiter,
+                  pk_zaprod_t aprod, void* U, const int* ldu, double* sigma, double* bnd,
+                  void* V, const int* ldv, const double* tolin, double* work, const int* lwork,
+                  void* zwork, const int* lzwrk, int* iwork, const int* liwork,
+                  double* doption, int* ioption, int* info, void* zparm, int* iparm,
+                  size_t len_which, size_t len_jobu, size_t len_jobv);
+
+void clansvd_(const char* jobu, const char* jobv, const int* m, const int* n, int* k,
+              const int* kmax, pk_zaprod_t aprod, void* U, const int* ldu, float* sigma,
+              float* bnd, void* V, const int* ldv, const float* tolin, float* work,
+              const int* lwork, void* cwork, const int* lcwrk, int* iwork, const int* liwork,
+              float* soption, int* ioption, int* info, void* cparm, int* iparm,
+              size_t len_jobu, size_t len_jobv);
+
+void clansvd_irl_(const char* which, const char* jobu, const char* jobv, const int* m,
+                  const int* n, int* dim, const int* p, int* neig, const int* maxiter,
+                  pk_zaprod_t aprod, void* U, const int* ldu, float* sigma, float* bnd,
+                  void* V, const int* ldv, const float* tolin, float* work, const int* lwork,
+                  void* cwork, const int* lcwrk, int* iwork, const int* liwork,
+                  float* soption, int* ioption, int* info, void* cparm, int* iparm,
+                  size_t len_which, size_t len_jobu, size_t len_jobv);
 
 void clearstat_(void);
 void printstat_(void);

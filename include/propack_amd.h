@@ -36,7 +36,9 @@ enum {
   PK_ERR_HANDLE = -104, /* not a live handle */
   PK_ERR_NOTSUP = -105  /* combination not supported */
 };
-enum { PK_F64 = 0, PK_F32 = 1 };   /* element precision of arrays */
+/* Element precision of arrays: real double / float, or complex double /
+ * float stored as (re, im) pairs (Fortran COMPLEX*16 / COMPLEX). */
+enum { PK_F64 = 0, PK_F32 = 1, PK_C128 = 2, PK_C64 = 3 };
 enum { PK_HOST = 0, PK_DEVICE = 1 }; /* where caller arrays live */
 
 typedef struct pk_csr_opaque* pk_csr_t;
@@ -51,8 +53,12 @@ const char* pk_last_error(void);
 /* Device CSR operator.  Replaces the user-supplied APROD of
  * double/dlansvd.F:20-33 and double/dlanbpro.F:28-41 (Y = A*X for 'n',
  * Y = A^T*X for 't').  rowptr (m+1) and colind (nnz) are 0-based int32;
- * values are double (PK_F64) or float (PK_F32).  A^T is built on the
- * device at creation.                                                     */
+ * values are double (PK_F64) or float (PK_F32), or complex (PK_C128 /
+ * PK_C64, (re, im) pairs) for the complex drivers, whose APROD applies
+ * A^H for 'c' (complex16/zlanbpro.F:297).  A^T is built on the device at
+ * creation.  Complex operators have one layout: the calls that choose
+ * panels, lanes or sorted layouts, pk_csr_autotune and
+ * pk_csr_spmv_noalloc return PK_ERR_NOTSUP for them.                      */
 int pk_csr_create(int m, int n, long long nnz, const int* rowptr, const int* colind,
                   const void* val, int val_prec, int where, pk_csr_t* op);
 int pk_csr_destroy(pk_csr_t op);
@@ -169,12 +175,24 @@ void pk_dcsr_aprod_(const char* transa, const int* m, const int* n, const double
                     double* dparm, int* iparm, size_t len_transa);
 void pk_scsr_aprod_(const char* transa, const int* m, const int* n, const float* x, float* y,
                     float* dparm, int* iparm, size_t len_transa);
+/* The same for complex operators (complex16/zlanbpro.F:28-41: TRANSA 'n'
+ * for Y = A*X, 'c' for Y = A^H*X; X and Y COMPLEX*16 / COMPLEX as (re, im)
+ * pairs), for ZLANSVD/ZLANSVD_IRL and CLANSVD/CLANSVD_IRL. */
+void pk_zcsr_aprod_(const char* transa, const int* m, const int* n, const void* x, void* y,
+                    void* zparm, int* iparm, size_t len_transa);
+void pk_ccsr_aprod_(const char* transa, const int* m, const int* n, const void* x, void* y,
+                    void* cparm, int* iparm, size_t len_transa);
 
 /* ---------------------------------------------------------------------- */
 /* Lanczos bidiagonalization engine.  prec = storage type of the bases
  * U (m x (kmax+1)) and V (n x kmax) kept on the device; all reductions and
  * the omega-recurrence run in double.  op may be NULL when a host callback
- * is registered with pk_lbd_set_callback.                                   */
+ * is registered with pk_lbd_set_callback.  A complex prec (PK_C128 /
+ * PK_C64) makes the engine of ZLANBPRO / CLANBPRO (complex16/zlanbpro.F):
+ * complex bases, the same real B, its operator complex of the same
+ * precision and a callback with the complex signature of that precision;
+ * its steps run on the host executor.  pk_lbd_residuals, pk_lbd_promote,
+ * pk_lbd_seed and the batched calls take real engines only.                */
 int pk_lbd_create(int prec, int m, int n, int kmax, pk_csr_t op, pk_lbd_t* eng);
 int pk_lbd_destroy(pk_lbd_t eng);
 /* Host APROD callback (Fortran ABI, arrays in aprod_prec) used for

@@ -1,9 +1,11 @@
 """Test infrastructure only: ctypes access to the reference PROPACK oracle.
 
-Loads oracle/_ref/libpropack_ref_{d,s}.so (built by ``make -C oracle ref``
-from the reference's own Fortran sources) and runs its drivers
-DLANSVD (double/dlansvd.F:1) and DLANSVD_IRL (double/dlansvd_irl.F:1), or
-the S twins, on a CSR matrix through the harness APROD in ref_harness.c.
+Loads oracle/_ref/libpropack_ref_{d,s,z,c}.so (built by ``make -C oracle
+ref`` from the reference's own Fortran sources) and runs its drivers
+DLANSVD (double/dlansvd.F:1) and DLANSVD_IRL (double/dlansvd_irl.F:1), the
+S twins, or the complex ZLANSVD / ZLANSVD_IRL (complex16/zlansvd.F:1,
+complex16/zlansvd_irl.F:1) and C twins, on a CSR matrix through the harness
+APROD in ref_harness.c.
 Only tests/, the smoke check and bench.py's cpu_baseline leg use this.
 """
 from __future__ import annotations
@@ -64,9 +66,13 @@ class _Op:
         getattr(L, f"ref_{prec}_set_operator")(C.c_int(A.m), C.c_int(A.n), *ptrs)
 
 
+_DT = {"d": np.float64, "s": np.float32, "z": np.complex128, "c": np.complex64}
+_RT = {"d": np.float64, "s": np.float32, "z": np.float64, "c": np.float32}
+
+
 def _prep(prec, A, AT):
     L = lib(prec)
-    dtype = np.float64 if prec == "d" else np.float32
+    dtype = _DT[prec]
     if AT is None:
         AT = A.transpose()
     return L, dtype, _Op(L, prec, A, AT, dtype)
@@ -94,23 +100,25 @@ def aprod(A, x, trans="n", prec="d", AT=None):
 
 def lansvd(A, k, kmax, u0=None, tol=1e-12, jobu="y", jobv="y", prec="d",
            delta=None, eta=None, anorm=0.0, cgs=0, elr=1, AT=None) -> RefResult:
-    """DLANSVD / SLANSVD on CSR ``A`` (double/dlansvd.F:1-291)."""
+    """DLANSVD / SLANSVD / ZLANSVD / CLANSVD on CSR ``A``
+    (double/dlansvd.F:1-291, complex16/zlansvd.F:1-301)."""
     L, dtype, op = _prep(prec, A, AT)
+    rdt = _RT[prec]
     m, n = A.m, A.n
-    eps = float(np.finfo(dtype).eps)
+    eps = float(np.finfo(rdt).eps)
     U = np.zeros((kmax + 1, m), dtype)   # column-major U(m, kmax+1)
     V = np.zeros((kmax, n), dtype)
     if u0 is not None:
         U[0, :] = u0
-    sigma = np.zeros(kmax, dtype)
-    bnd = np.zeros(kmax, dtype)
+    sigma = np.zeros(kmax, rdt)
+    bnd = np.zeros(kmax, rdt)
     dopt = np.array([np.sqrt(eps) if delta is None else delta,
-                     eps ** 0.75 if eta is None else eta, anorm, 0.0], dtype)
+                     eps ** 0.75 if eta is None else eta, anorm, 0.0], rdt)
     iopt = np.array([cgs, elr], np.int32)
     kk = C.c_int(k)
     getattr(L, f"ref_{prec}_clear_counters")()
     fn = getattr(L, f"ref_{prec}_lansvd")
-    R = C.c_double if prec == "d" else C.c_float
+    R = C.c_double if rdt == np.float64 else C.c_float
     p = lambda a: a.ctypes.data_as(C.c_void_p)
     info = fn(C.c_char(jobu.encode()), C.c_char(jobv.encode()), C.c_int(m),
               C.c_int(n), C.byref(kk), C.c_int(kmax), p(U), C.c_int(m),
@@ -125,27 +133,29 @@ def lansvd(A, k, kmax, u0=None, tol=1e-12, jobu="y", jobv="y", prec="d",
 def lansvd_irl(A, neig, dim, p=None, u0=None, tol=1e-10, which="L", jobu="y",
                jobv="y", maxiter=1000, prec="d", delta=None, eta=None,
                anorm=0.0, min_relgap=0.002, cgs=0, elr=1, AT=None) -> RefResult:
-    """DLANSVD_IRL / SLANSVD_IRL on CSR ``A`` (double/dlansvd_irl.F:1-419)."""
+    """DLANSVD_IRL / SLANSVD_IRL / ZLANSVD_IRL / CLANSVD_IRL on CSR ``A``
+    (double/dlansvd_irl.F:1-419, complex16/zlansvd_irl.F:1-421)."""
     L, dtype, op = _prep(prec, A, AT)
+    rdt = _RT[prec]
     m, n = A.m, A.n
-    eps = float(np.finfo(dtype).eps)
+    eps = float(np.finfo(rdt).eps)
     if p is None:
         p = dim - neig
     U = np.zeros((dim + 1, m), dtype)
     V = np.zeros((dim, n), dtype)
     if u0 is not None:
         U[0, :] = u0
-    sigma = np.zeros(dim, dtype)
-    bnd = np.zeros(dim, dtype)
+    sigma = np.zeros(dim, rdt)
+    bnd = np.zeros(dim, rdt)
     dopt = np.array([np.sqrt(eps) if delta is None else delta,
                      eps ** 0.75 if eta is None else eta, anorm, min_relgap],
-                    dtype)
+                    rdt)
     iopt = np.array([cgs, elr], np.int32)
     dd = C.c_int(dim)
     ne = C.c_int(neig)
     getattr(L, f"ref_{prec}_clear_counters")()
     fn = getattr(L, f"ref_{prec}_lansvd_irl")
-    R = C.c_double if prec == "d" else C.c_float
+    R = C.c_double if rdt == np.float64 else C.c_float
     ptr = lambda a: a.ctypes.data_as(C.c_void_p)
     info = fn(C.c_char(which.encode()), C.c_char(jobu.encode()),
               C.c_char(jobv.encode()), C.c_int(m), C.c_int(n), C.byref(dd),

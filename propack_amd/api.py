@@ -24,7 +24,7 @@ import numpy as np
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "_lib", "libpropack_amd.so")
 
-PK_F64, PK_F32 = 0, 1
+PK_F64, PK_F32, PK_C128, PK_C64 = 0, 1, 2, 3
 PK_HOST, PK_DEVICE = 0, 1
 
 EXPORTS = [
@@ -43,6 +43,7 @@ EXPORTS = [
     "pk_csr_create_panels", "pk_csr_panels", "pk_csr_set_lanes", "pk_csr_lanes", "pk_csr_autotune",
     "pk_csr_set_sort", "pk_csr_sort", "pk_lbd_path_stats", "pk_batch_set_graph",
     "pk_csr_spmv_noalloc", "pk_lbd_residuals", "pk_lbd_promote",
+    "zlansvd_", "zlansvd_irl_", "clansvd_", "clansvd_irl_", "pk_zcsr_aprod_", "pk_ccsr_aprod_",
 ]
 
 _lib = None
@@ -75,11 +76,34 @@ def _p(a):
 
 
 def _dt(prec):
-    return np.float64 if prec == PK_F64 else np.float32
+    """Element type of arrays of precision ``prec``."""
+    return {PK_F64: np.float64, PK_F32: np.float32, PK_C128: np.complex128,
+            PK_C64: np.complex64}[prec]
+
+
+def _rt(prec):
+    """Real type of precision ``prec`` (singular values, options)."""
+    return np.float64 if prec in (PK_F64, PK_C128) else np.float32
+
+
+def is_complex(prec) -> bool:
+    return prec in (PK_C128, PK_C64)
+
+
+def prec_of(dtype) -> int:
+    """Precision code of a numpy dtype."""
+    dt = np.dtype(dtype)
+    if dt == np.complex128:
+        return PK_C128
+    if dt == np.complex64:
+        return PK_C64
+    return PK_F32 if dt == np.float32 else PK_F64
 
 
 class CsrOperator:
-    """Device CSR operator built from a host ``Csr`` (A^T built on device)."""
+    """Device CSR operator built from a host ``Csr`` (A^T built on device).
+    Complex precisions (PK_C128 / PK_C64) apply A, A^T or A^H ('c') and
+    have one layout (no panels, lanes or sorted layouts)."""
 
     def __init__(self, A, prec: int = PK_F64, panels=None, lanes=None, sort=None):
         """``panels``: (A, A^T) column-panel counts, ``lanes``: (A, A^T)
@@ -156,7 +180,7 @@ class CsrOperator:
 
     def spmv(self, x, trans="n"):
         x = np.ascontiguousarray(x, _dt(self.prec))
-        y = np.zeros(self.n if trans == "t" else self.m, _dt(self.prec))
+        y = np.zeros(self.m if trans == "n" else self.n, _dt(self.prec))
         _check(lib().pk_csr_spmv(self.handle, C.c_char(trans.encode()), _p(x), _p(y),
                                  C.c_int(self.prec), C.c_int(PK_HOST)), "pk_csr_spmv")
         return y
@@ -207,69 +231,78 @@ def clearstat() -> None:
     lib().clearstat_()
 
 
-_APROD_D = C.CFUNCTYPE(None, C.c_char_p, C.POINTER(C.c_int), C.POINTER(C.c_int),
-                       C.POINTER(C.c_double), C.POINTER(C.c_double), C.c_void_p, C.c_void_p,
-                       C.c_size_t)
-_APROD_S = C.CFUNCTYPE(None, C.c_char_p, C.POINTER(C.c_int), C.POINTER(C.c_int),
-                       C.POINTER(C.c_float), C.POINTER(C.c_float), C.c_void_p, C.c_void_p,
-                       C.c_size_t)
+_APRODS = {PK_F64: "pk_dcsr_aprod_", PK_F32: "pk_scsr_aprod_", PK_C128: "pk_zcsr_aprod_",
+           PK_C64: "pk_ccsr_aprod_"}
 
 
 def _make_aprod(aprod, m, n, prec):
-    """(function pointer, iparm, keepalive) for an APROD argument."""
+    """(function pointer, iparm, keepalive) for an APROD argument: a
+    CsrOperator (its device path) or a Python callable aprod(t, x) with
+    t = 'n' or 't' ('c' = A^H for complex precisions)."""
     if isinstance(aprod, CsrOperator):
         if aprod.prec != prec:
             raise PropackError("operator precision differs from the driver's")
-        name = "pk_dcsr_aprod_" if prec == PK_F64 else "pk_scsr_aprod_"
-        fn = C.cast(getattr(lib(), name), C.c_void_p)
+        fn = C.cast(getattr(lib(), _APRODS[prec]), C.c_void_p)
         return fn, aprod.iparm, None
     dt = _dt(prec)
-    ct = C.c_double if prec == PK_F64 else C.c_float
+    ct = C.c_double if _rt(prec) == np.float64 else C.c_float
+    w = 2 if is_complex(prec) else 1
+    wide = np.complex128 if is_complex(prec) else np.float64
 
     def cb(transa, mp, np_, x, y, dparm, iparm, ln):
         t = transa[0:1].decode().lower()
         nin, nout = (n, m) if t == "n" else (m, n)
-        xa = np.ctypeslib.as_array(C.cast(x, C.POINTER(ct)), shape=(nin,))
-        ya = np.ctypeslib.as_array(C.cast(y, C.POINTER(ct)), shape=(nout,))
-        ya[:] = np.asarray(aprod(t, xa.astype(np.float64)), dtype=dt)
+        xa = np.ctypeslib.as_array(C.cast(x, C.POINTER(ct)), shape=(w * nin,)).view(dt)
+        ya = np.ctypeslib.as_array(C.cast(y, C.POINTER(ct)), shape=(w * nout,)).view(dt)
+        ya[:] = np.asarray(aprod(t, xa.astype(wide)), dtype=dt)
 
-    f = (_APROD_D if prec == PK_F64 else _APROD_S)(cb)
+    proto = C.CFUNCTYPE(None, C.c_char_p, C.POINTER(C.c_int), C.POINTER(C.c_int), C.c_void_p,
+                        C.c_void_p, C.c_void_p, C.c_void_p, C.c_size_t)
+    f = proto(cb)
     return C.cast(f, C.c_void_p), np.zeros(4, np.int32), f
+
+
+_DRV = {PK_F64: ("dlansvd_", "dlansvd_irl_"), PK_F32: ("slansvd_", "slansvd_irl_"),
+        PK_C128: ("zlansvd_", "zlansvd_irl_"), PK_C64: ("clansvd_", "clansvd_irl_")}
 
 
 def lansvd(aprod, m, n, k, kmax, u0=None, tol=1e-12, jobu="y", jobv="y", prec=PK_F64,
            delta=None, eta=None, anorm=0.0, cgs=1, elr=1) -> SvdResult:
-    """DLANSVD / SLANSVD through the library's Fortran drop-in."""
+    """DLANSVD / SLANSVD / ZLANSVD / CLANSVD through the library's Fortran
+    drop-in."""
     L = lib()
-    dt = _dt(prec)
-    eps = float(np.finfo(dt).eps)
+    dt, rt = _dt(prec), _rt(prec)
+    eps = float(np.finfo(rt).eps)
     U = np.zeros((kmax + 1, m), dt)
     V = np.zeros((kmax, n), dt)
     if u0 is not None:
         U[0] = u0
-    sigma = np.zeros(kmax, dt)
-    bnd = np.zeros(kmax, dt)
+    sigma = np.zeros(kmax, rt)
+    bnd = np.zeros(kmax, rt)
     dopt = np.array([np.sqrt(eps) if delta is None else delta,
-                     eps ** 0.75 if eta is None else eta, anorm, 0.0], dt)
+                     eps ** 0.75 if eta is None else eta, anorm, 0.0], rt)
     iopt = np.array([cgs, elr], np.int32)
     lwork = m + n + 9 * kmax + 5 * kmax * kmax + 4 + max(3 * kmax * kmax + 4 * kmax + 4,
                                                          16 * max(m, n))
-    work = np.zeros(1, dt)
+    work = np.zeros(1, rt)
     iwork = np.zeros(1, np.int32)
     fn, iparm, keep = _make_aprod(aprod, m, n, prec)
     dparm = np.zeros(1, dt)
-    R = C.c_double if prec == PK_F64 else C.c_float
+    R = C.c_double if rt == np.float64 else C.c_float
     ii = lambda v: C.byref(C.c_int(v))
     kk, info = C.c_int(k), C.c_int(0)
     clearstat()
-    f = L.dlansvd_ if prec == PK_F64 else L.slansvd_
-    f(C.c_char_p(jobu.encode()), C.c_char_p(jobv.encode()), ii(m), ii(n), C.byref(kk),
-      ii(kmax), fn, _p(U), ii(m), _p(sigma), _p(bnd), _p(V), ii(n), C.byref(R(tol)), _p(work),
-      ii(lwork), _p(iwork), ii(8 * kmax), _p(dopt), _p(iopt), C.byref(info), _p(dparm),
-      _p(iparm), C.c_size_t(1), C.c_size_t(1))
+    name = _DRV[prec][0]
+    # the complex drivers take ZWORK(LZWRK) after WORK(LWORK)
+    zw = (_p(np.zeros(1, dt)), ii(1)) if is_complex(prec) else ()
+    getattr(L, name)(C.c_char_p(jobu.encode()), C.c_char_p(jobv.encode()), ii(m), ii(n),
+                     C.byref(kk), ii(kmax), fn, _p(U), ii(m), _p(sigma), _p(bnd), _p(V), ii(n),
+                     C.byref(R(tol)), _p(work), ii(lwork), *zw, _p(iwork), ii(8 * kmax),
+                     _p(dopt), _p(iopt), C.byref(info), _p(dparm), _p(iparm), C.c_size_t(1),
+                     C.c_size_t(1))
     del keep
     if info.value <= -100:
-        _check(info.value, "dlansvd" if prec == PK_F64 else "slansvd")
+        _check(info.value, name.rstrip("_"))
     kout = kk.value
     return SvdResult(sigma[:kout].copy(), bnd[:kout].copy(),
                      U[:kout].T.copy() if jobu == "y" else None,
@@ -280,38 +313,41 @@ def lansvd(aprod, m, n, k, kmax, u0=None, tol=1e-12, jobu="y", jobv="y", prec=PK
 def lansvd_irl(aprod, m, n, neig, dim, p=None, u0=None, tol=1e-10, which="L", jobu="y",
                jobv="y", maxiter=1000, prec=PK_F64, delta=None, eta=None, anorm=0.0,
                min_relgap=0.002, cgs=1, elr=1) -> SvdResult:
-    """DLANSVD_IRL / SLANSVD_IRL through the library's Fortran drop-in."""
+    """DLANSVD_IRL / SLANSVD_IRL / ZLANSVD_IRL / CLANSVD_IRL through the
+    library's Fortran drop-in."""
     L = lib()
-    dt = _dt(prec)
-    eps = float(np.finfo(dt).eps)
+    dt, rt = _dt(prec), _rt(prec)
+    eps = float(np.finfo(rt).eps)
     if p is None:
         p = dim - neig
     U = np.zeros((dim + 1, m), dt)
     V = np.zeros((dim, n), dt)
     if u0 is not None:
         U[0] = u0
-    sigma = np.zeros(dim, dt)
-    bnd = np.zeros(dim, dt)
+    sigma = np.zeros(dim, rt)
+    bnd = np.zeros(dim, rt)
     dopt = np.array([np.sqrt(eps) if delta is None else delta,
-                     eps ** 0.75 if eta is None else eta, anorm, min_relgap], dt)
+                     eps ** 0.75 if eta is None else eta, anorm, min_relgap], rt)
     iopt = np.array([cgs, elr], np.int32)
-    work = np.zeros(1, dt)
+    work = np.zeros(1, rt)
     iwork = np.zeros(1, np.int32)
     fn, iparm, keep = _make_aprod(aprod, m, n, prec)
     dparm = np.zeros(1, dt)
-    R = C.c_double if prec == PK_F64 else C.c_float
+    R = C.c_double if rt == np.float64 else C.c_float
     ii = lambda v: C.byref(C.c_int(v))
     dd, ne, info = C.c_int(dim), C.c_int(neig), C.c_int(0)
     clearstat()
-    f = L.dlansvd_irl_ if prec == PK_F64 else L.slansvd_irl_
-    f(C.c_char_p(which.encode()), C.c_char_p(jobu.encode()), C.c_char_p(jobv.encode()), ii(m),
-      ii(n), C.byref(dd), ii(p), C.byref(ne), ii(maxiter), fn, _p(U), ii(m), _p(sigma),
-      _p(bnd), _p(V), ii(n), C.byref(R(tol)), _p(work), ii(1), _p(iwork), ii(1), _p(dopt),
-      _p(iopt), C.byref(info), _p(dparm), _p(iparm), C.c_size_t(1), C.c_size_t(1),
-      C.c_size_t(1))
+    name = _DRV[prec][1]
+    zw = (_p(np.zeros(1, dt)), ii(1)) if is_complex(prec) else ()
+    getattr(L, name)(C.c_char_p(which.encode()), C.c_char_p(jobu.encode()),
+                     C.c_char_p(jobv.encode()), ii(m), ii(n), C.byref(dd), ii(p), C.byref(ne),
+                     ii(maxiter), fn, _p(U), ii(m), _p(sigma), _p(bnd), _p(V), ii(n),
+                     C.byref(R(tol)), _p(work), ii(1), *zw, _p(iwork), ii(1), _p(dopt),
+                     _p(iopt), C.byref(info), _p(dparm), _p(iparm), C.c_size_t(1),
+                     C.c_size_t(1), C.c_size_t(1))
     del keep
     if info.value <= -100:
-        _check(info.value, "dlansvd_irl" if prec == PK_F64 else "slansvd_irl")
+        _check(info.value, name.rstrip("_"))
     k = ne.value
     return SvdResult(sigma[:k].copy(), bnd[:k].copy(),
                      U[:k].T.copy() if jobu == "y" else None,

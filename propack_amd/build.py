@@ -42,6 +42,7 @@ HIP_FLAGS = [
 F_FLAGS = ["-O2", "-fPIC"]
 
 HIP_SOURCES = ["pk_kernels.hip", "pk_csr.hip", "pk_engine.hip", "pk_batch.hip", "pk_ingest.hip",
+               "pk_cplx.hip",
                "pk_bsvd.hip"]
 F_SOURCES = ["pk_api.f90", "pk_lansvd.f90", "pk_dropin.f90"]  # module order
 

@@ -536,6 +536,11 @@ int batch_create(int nprob, const pk_lbd_t* handles, Batch** out) {
       rc = PK_ERR_ARG;
       break;
     }
+    if (prec_is_complex(e->prec)) {
+      set_error("pk_batch_create: problem %d is complex; batches take real engines", i);
+      rc = PK_ERR_NOTSUP;
+      break;
+    }
     if (i == 0) {
       B->prec = e->prec;
       B->val_prec = e->op->val_prec;

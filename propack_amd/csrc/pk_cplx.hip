@@ -1,0 +1,350 @@
+// Complex-valued kernels of the complex precisions (ZLANSVD / ZLANSVD_IRL,
+// complex16/, and the CLANSVD twins, complex8/).
+//
+// PROPACK's complex drivers keep the bidiagonal B real
+// (complex16/zlanbpro.F:111-112); only the vector work is complex: APROD
+// applies A or A^H (TRANSA = 'c', complex16/zlanbpro.F:297), dot products
+// conjugate their first argument (pzdotc, complex16/zblasext.F:167-194) and
+// the Gram-Schmidt projections are V^H w (complex16/zreorth.F:169-170).
+// Bases and operator values are stored as Fortran COMPLEX: (re, im) pairs.
+// What is real-linear in the pairs (norms, scaling by a real factor, the
+// restart / Ritz GEMMs with a real right factor) runs the real kernels on
+// the 2L-long real view; the kernels here are the ones that need complex
+// arithmetic.  Sums are formed in FP64 in a fixed order (deterministic), the
+// results rounded once to the storage precision:
+//   k_zspmv          y = op(A) x - c yprev, op = A or A^H (CSR of A, and of
+//                    the explicit transpose A^T read with conjugated values),
+//                    one group of G lanes per row; partials ||y||^2,
+//                    Re / Im of yprev^H y
+//   k_zaxpy_norm_dot w -= s v for complex s; partials ||w||^2, Re / Im v^H w
+//   k_zgemvt         h_j = V(:, cols[j])^H w, two partials per column
+//   k_zgemv_update   w -= V(:, cols) h; partial ||w||^2
+// Partial sums follow the real kernels' layout: value v of block b at
+// part[v * nblk + b], reduced by launch_reduce.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+
+#include "pk_internal.h"
+
+namespace pk {
+
+void set_error(const char* fmt, ...);
+
+namespace {
+
+constexpr int kZRPT = 4;                   // rows per thread of the streaming kernels
+constexpr int kZRows = kBlock * kZRPT;     // rows per workgroup slice
+constexpr int kZCols = 4;                  // basis columns per pass of the projection
+
+template <typename R>
+struct V2T;
+template <>
+struct V2T<double> {
+  using type = double2;
+};
+template <>
+struct V2T<float> {
+  using type = float2;
+};
+
+// ----------------------------------------------------------------- SpMV ----
+// Block b owns rows [b * kZRows, (b + 1) * kZRows); its kBlock / G groups
+// take the rows in turn (every lane of a wave runs the same trip count, so
+// the group shuffles see whole groups).  Lane l of a group adds the row's
+// nonzeros l, l + G, ...; the group sum and the epilogue follow.
+template <typename R, int G>
+__global__ __launch_bounds__(kBlock) void k_zspmv(long long rows, const int* __restrict__ rp,
+                                                  const int* __restrict__ ci,
+                                                  const typename V2T<R>::type* __restrict__ va,
+                                                  int conj,
+                                                  const typename V2T<R>::type* __restrict__ x,
+                                                  typename V2T<R>::type* __restrict__ y,
+                                                  const typename V2T<R>::type* __restrict__ yprev,
+                                                  double c, double* __restrict__ part) {
+  using C2 = typename V2T<R>::type;
+  __shared__ double lds[(kBlock / kWave) * 3];
+  constexpr int ngrp = kBlock / G;
+  const int lane = threadIdx.x % G, grp = threadIdx.x / G;
+  const long long r0 = static_cast<long long>(blockIdx.x) * kZRows;
+  const long long r1 = r0 + kZRows < rows ? r0 + kZRows : rows;
+  const double sg = conj ? -1.0 : 1.0;
+  double acc[3] = {0.0, 0.0, 0.0};
+  for (long long base = r0; base < r1; base += ngrp) {
+    const long long row = base + grp;
+    double sr = 0.0, si = 0.0;
+    if (row < r1) {
+      const int p1 = rp[row + 1];
+      for (int p = rp[row] + lane; p < p1; p += G) {
+        const C2 a = va[p];
+        const C2 xv = x[ci[p]];
+        const double ar = static_cast<double>(a.x), ai = sg * static_cast<double>(a.y);
+        const double xr = static_cast<double>(xv.x), xi = static_cast<double>(xv.y);
+        sr += ar * xr - ai * xi;
+        si += ar * xi + ai * xr;
+      }
+    }
+    sr = group_sum<G>(sr);
+    si = group_sum<G>(si);
+    if (lane == 0 && row < r1) {
+      double yr = sr, yi = si, pr = 0.0, pi = 0.0;
+      if (yprev) {
+        const C2 pv = yprev[row];
+        pr = static_cast<double>(pv.x);
+        pi = static_cast<double>(pv.y);
+        yr = sr - c * pr;
+        yi = si - c * pi;
+      }
+      C2 out;
+      out.x = static_cast<R>(yr);
+      out.y = static_cast<R>(yi);
+      y[row] = out;
+      const double qr = static_cast<double>(out.x), qi = static_cast<double>(out.y);
+      acc[0] += qr * qr + qi * qi;
+      acc[1] += pr * qr + pi * qi;  // conj(yprev) * y
+      acc[2] += pr * qi - pi * qr;
+    }
+  }
+  block_sum<3>(acc, lds);
+  if (threadIdx.x == 0) {
+    part[blockIdx.x] = acc[0];
+    part[gridDim.x + blockIdx.x] = acc[1];
+    part[2 * gridDim.x + blockIdx.x] = acc[2];
+  }
+}
+
+// ------------------------------------------------------------- level 1 ----
+template <typename R>
+__global__ __launch_bounds__(kBlock) void k_zaxpy_norm_dot(long long L,
+                                                           typename V2T<R>::type* __restrict__ w,
+                                                           const typename V2T<R>::type* __restrict__ v,
+                                                           double sr, double si,
+                                                           double* __restrict__ part) {
+  using C2 = typename V2T<R>::type;
+  __shared__ double lds[(kBlock / kWave) * 3];
+  const long long base = static_cast<long long>(blockIdx.x) * kZRows + threadIdx.x;
+  double acc[3] = {0.0, 0.0, 0.0};
+#pragma unroll
+  for (int r = 0; r < kZRPT; ++r) {
+    const long long row = base + static_cast<long long>(r) * kBlock;
+    if (row < L) {
+      const C2 vv = v[row];
+      const C2 wv = w[row];
+      const double vr = static_cast<double>(vv.x), vi = static_cast<double>(vv.y);
+      C2 out;
+      out.x = static_cast<R>(static_cast<double>(wv.x) - (sr * vr - si * vi));
+      out.y = static_cast<R>(static_cast<double>(wv.y) - (sr * vi + si * vr));
+      w[row] = out;
+      const double qr = static_cast<double>(out.x), qi = static_cast<double>(out.y);
+      acc[0] += qr * qr + qi * qi;
+      acc[1] += vr * qr + vi * qi;
+      acc[2] += vr * qi - vi * qr;
+    }
+  }
+  block_sum<3>(acc, lds);
+  if (threadIdx.x == 0) {
+    part[blockIdx.x] = acc[0];
+    part[gridDim.x + blockIdx.x] = acc[1];
+    part[2 * gridDim.x + blockIdx.x] = acc[2];
+  }
+}
+
+// ------------------------------------------------------- Gram-Schmidt ----
+// Slice b of h = V(:, cols)^H w: the slice of w stays in registers while
+// the columns stream past in groups of kZCols; partials of column j at
+// part[(2j) * nblk + b] (real) and part[(2j + 1) * nblk + b] (imaginary).
+template <typename R>
+__global__ __launch_bounds__(kBlock) void k_zgemvt(long long L,
+                                                   const typename V2T<R>::type* __restrict__ V,
+                                                   long long ldv, const int* __restrict__ cols,
+                                                   int ncols,
+                                                   const typename V2T<R>::type* __restrict__ w,
+                                                   double* __restrict__ part) {
+  using C2 = typename V2T<R>::type;
+  __shared__ double lds[(kBlock / kWave) * 2 * kZCols];
+  const long long base = static_cast<long long>(blockIdx.x) * kZRows + threadIdx.x;
+  double wr[kZRPT], wi[kZRPT];
+#pragma unroll
+  for (int r = 0; r < kZRPT; ++r) {
+    const long long row = base + static_cast<long long>(r) * kBlock;
+    wr[r] = wi[r] = 0.0;
+    if (row < L) {
+      const C2 wv = w[row];
+      wr[r] = static_cast<double>(wv.x);
+      wi[r] = static_cast<double>(wv.y);
+    }
+  }
+  const size_t nb = gridDim.x;
+  for (int j0 = 0; j0 < ncols; j0 += kZCols) {
+    double acc[2 * kZCols];
+#pragma unroll
+    for (int q = 0; q < kZCols; ++q) {
+      acc[2 * q] = acc[2 * q + 1] = 0.0;
+      if (j0 + q < ncols) {
+        const C2* col = V + static_cast<long long>(cols[j0 + q]) * ldv;
+#pragma unroll
+        for (int r = 0; r < kZRPT; ++r) {
+          const long long row = base + static_cast<long long>(r) * kBlock;
+          if (row < L) {
+            const C2 a = col[row];
+            const double ar = static_cast<double>(a.x), ai = static_cast<double>(a.y);
+            acc[2 * q] += ar * wr[r] + ai * wi[r];      // conj(a) * w
+            acc[2 * q + 1] += ar * wi[r] - ai * wr[r];
+          }
+        }
+      }
+    }
+    block_sum<2 * kZCols>(acc, lds);
+    if (threadIdx.x == 0) {
+#pragma unroll
+      for (int q = 0; q < kZCols; ++q)
+        if (j0 + q < ncols) {
+          part[static_cast<size_t>(2 * (j0 + q)) * nb + blockIdx.x] = acc[2 * q];
+          part[static_cast<size_t>(2 * (j0 + q) + 1) * nb + blockIdx.x] = acc[2 * q + 1];
+        }
+    }
+  }
+}
+
+// w -= V(:, cols) h with h[2j], h[2j+1] the real and imaginary parts of h_j;
+// the update of a row is summed in FP64 over the columns in order, then
+// subtracted and rounded once.  Partial: ||w||^2.
+template <typename R>
+__global__ __launch_bounds__(kBlock) void k_zgemv_update(long long L,
+                                                         const typename V2T<R>::type* __restrict__ V,
+                                                         long long ldv,
+                                                         const int* __restrict__ cols, int ncols,
+                                                         const double* __restrict__ h,
+                                                         typename V2T<R>::type* __restrict__ w,
+                                                         double* __restrict__ part) {
+  using C2 = typename V2T<R>::type;
+  __shared__ double lds[kBlock / kWave];
+  const long long base = static_cast<long long>(blockIdx.x) * kZRows + threadIdx.x;
+  double ur[kZRPT], ui[kZRPT];
+#pragma unroll
+  for (int r = 0; r < kZRPT; ++r) ur[r] = ui[r] = 0.0;
+  for (int j = 0; j < ncols; ++j) {
+    const C2* col = V + static_cast<long long>(cols[j]) * ldv;
+    const double hr = h[2 * j], hi = h[2 * j + 1];
+#pragma unroll
+    for (int r = 0; r < kZRPT; ++r) {
+      const long long row = base + static_cast<long long>(r) * kBlock;
+      if (row < L) {
+        const C2 a = col[row];
+        const double ar = static_cast<double>(a.x), ai = static_cast<double>(a.y);
+        ur[r] += ar * hr - ai * hi;
+        ui[r] += ar * hi + ai * hr;
+      }
+    }
+  }
+  double acc[1] = {0.0};
+#pragma unroll
+  for (int r = 0; r < kZRPT; ++r) {
+    const long long row = base + static_cast<long long>(r) * kBlock;
+    if (row < L) {
+      const C2 wv = w[row];
+      C2 out;
+      out.x = static_cast<R>(static_cast<double>(wv.x) - ur[r]);
+      out.y = static_cast<R>(static_cast<double>(wv.y) - ui[r]);
+      w[row] = out;
+      const double qr = static_cast<double>(out.x), qi = static_cast<double>(out.y);
+      acc[0] += qr * qr + qi * qi;
+    }
+  }
+  block_sum<1>(acc, lds);
+  if (threadIdx.x == 0) part[blockIdx.x] = acc[0];
+}
+
+}  // namespace
+
+int zstream_blocks(long long L) { return L > 0 ? cdiv(L, kZRows) : 1; }
+
+// Lanes per row of the complex SpMV: the power of two nearest the mean row
+// length (one nonzero per lane), clamped to [2, 64].
+int zspmv_lanes(long long nnz, long long rows) {
+  const double avg = rows > 0 ? static_cast<double>(nnz) / static_cast<double>(rows) : 1.0;
+  int g = 2;
+  while (g < 64 && g * 1.5 < avg) g *= 2;
+  return g;
+}
+
+template <typename R>
+int launch_zspmv(hipStream_t s, long long rows, const int* rp, const int* ci, const void* va,
+                 int conj, int g, const void* x, void* y, const void* yprev, double c,
+                 double* part) {
+  using C2 = typename V2T<R>::type;
+  const int nblk = zstream_blocks(rows);
+#define PK_ZSPMV(G)                                                                            \
+  hipLaunchKernelGGL((k_zspmv<R, G>), dim3(nblk), dim3(kBlock), 0, s, rows, rp, ci,             \
+                     static_cast<const C2*>(va), conj, static_cast<const C2*>(x),               \
+                     static_cast<C2*>(y), static_cast<const C2*>(yprev), c, part)
+  switch (g) {
+    case 2: PK_ZSPMV(2); break;
+    case 4: PK_ZSPMV(4); break;
+    case 8: PK_ZSPMV(8); break;
+    case 16: PK_ZSPMV(16); break;
+    case 32: PK_ZSPMV(32); break;
+    default: PK_ZSPMV(64); break;
+  }
+#undef PK_ZSPMV
+  return hipGetLastError() == hipSuccess ? nblk : -1;
+}
+
+template <typename R>
+int launch_zaxpy_norm_dot(hipStream_t s, long long L, void* w, const void* v, double sr,
+                          double si, double* part) {
+  using C2 = typename V2T<R>::type;
+  const int nblk = zstream_blocks(L);
+  hipLaunchKernelGGL(k_zaxpy_norm_dot<R>, dim3(nblk), dim3(kBlock), 0, s, L, static_cast<C2*>(w),
+                     static_cast<const C2*>(v), sr, si, part);
+  return hipGetLastError() == hipSuccess ? nblk : -1;
+}
+
+template <typename R>
+int launch_zgemvt(hipStream_t s, long long L, const void* V, long long ldv, const int* cols,
+                  int ncols, const void* w, double* part) {
+  using C2 = typename V2T<R>::type;
+  const int nblk = zstream_blocks(L);
+  hipLaunchKernelGGL(k_zgemvt<R>, dim3(nblk), dim3(kBlock), 0, s, L, static_cast<const C2*>(V),
+                     ldv, cols, ncols, static_cast<const C2*>(w), part);
+  return hipGetLastError() == hipSuccess ? nblk : -1;
+}
+
+template <typename R>
+int launch_zgemv_update(hipStream_t s, long long L, const void* V, long long ldv,
+                        const int* cols, int ncols, const double* h, void* w, double* part) {
+  using C2 = typename V2T<R>::type;
+  const int nblk = zstream_blocks(L);
+  hipLaunchKernelGGL(k_zgemv_update<R>, dim3(nblk), dim3(kBlock), 0, s, L,
+                     static_cast<const C2*>(V), ldv, cols, ncols, h, static_cast<C2*>(w), part);
+  return hipGetLastError() == hipSuccess ? nblk : -1;
+}
+
+// y = op(A) x - c * yprev for a complex operator (trans 'n': A; 't': A^T;
+// 'c': A^H); returns the partial-slot count (three values per slot).
+template <typename R>
+int zcsr_apply(const CsrOp* op, hipStream_t s, char trans, const void* x, void* y,
+               const void* yprev, double c, double* part) {
+  const bool tr = trans != 'n' && trans != 'N';
+  const bool cj = trans == 'c' || trans == 'C';
+  const CsrOp::Panel& P = tr ? op->dt.pan[0] : op->da.pan[0];
+  const long long rows = tr ? op->n : op->m;
+  return launch_zspmv<R>(s, rows, P.rp, P.ci, P.va, cj ? 1 : 0, P.g, x, y, yprev, c, part);
+}
+
+#define PK_INST_Z(R)                                                                            \
+  template int launch_zspmv<R>(hipStream_t, long long, const int*, const int*, const void*, int, \
+                               int, const void*, void*, const void*, double, double*);          \
+  template int launch_zaxpy_norm_dot<R>(hipStream_t, long long, void*, const void*, double,     \
+                                        double, double*);                                       \
+  template int launch_zgemvt<R>(hipStream_t, long long, const void*, long long, const int*, int, \
+                                const void*, double*);                                          \
+  template int launch_zgemv_update<R>(hipStream_t, long long, const void*, long long,           \
+                                      const int*, int, const double*, void*, double*);          \
+  template int zcsr_apply<R>(const CsrOp*, hipStream_t, char, const void*, void*, const void*,  \
+                             double, double*);
+PK_INST_Z(double)
+PK_INST_Z(float)
+#undef PK_INST_Z
+
+}  // namespace pk

@@ -637,7 +637,7 @@ int csr_create(int m, int n, long long nnz, const int* rowptr, const int* colind
                const void* val, int val_prec, int where, CsrOp** out, int panels_a,
                int panels_t) {
   if (!out || m < 0 || n < 0 || nnz < 0 || nnz >= (1LL << 31)) return PK_ERR_ARG;
-  if (val_prec != PK_F64 && val_prec != PK_F32) return PK_ERR_ARG;
+  if (val_prec != PK_F64 && val_prec != PK_F32 && !prec_is_complex(val_prec)) return PK_ERR_ARG;
   if (where != PK_HOST && where != PK_DEVICE) return PK_ERR_ARG;
   if (!rowptr || (nnz > 0 && (!colind || !val))) return PK_ERR_ARG;
   if (where == PK_HOST) {
@@ -652,7 +652,7 @@ int csr_create(int m, int n, long long nnz, const int* rowptr, const int* colind
   op->n = n;
   op->nnz = nnz;
   op->val_prec = val_prec;
-  op->val_bytes = val_prec == PK_F64 ? 8 : 4;
+  op->val_bytes = prec_bytes(val_prec);
   op->magic = CsrOp::kMagic;
   const hipMemcpyKind kind = where == PK_HOST ? hipMemcpyHostToDevice : hipMemcpyDeviceToDevice;
   hipStream_t s = nullptr;
@@ -674,11 +674,38 @@ int csr_create(int m, int n, long long nnz, const int* rowptr, const int* colind
       hipStreamSynchronize(s) != hipSuccess) {
     rc = PK_ERR_HIP;
   }
-  if (rc == PK_OK)
+  if (rc == PK_OK && prec_is_complex(val_prec)) {
+    // complex values: A and the explicit A^T as one plain CSR panel each
+    // (the transpose keeps the values; A^H conjugates them as it reads)
+    rc = val_prec == PK_C128
+             ? build_transpose<double2>(m, n, nnz, rp, ci, va, &rpt, &cit, &vat, s)
+             : build_transpose<float2>(m, n, nnz, rp, ci, va, &rpt, &cit, &vat, s);
+    if (rc == PK_OK) {
+      CsrOp::Dir* dirs[2] = {&op->da, &op->dt};
+      int* rps[2] = {rp, rpt};
+      int* cis[2] = {ci, cit};
+      void* vas[2] = {va, vat};
+      for (int d = 0; d < 2; ++d) {
+        CsrOp::Dir& D = *dirs[d];
+        D.rows = d == 0 ? m : n;
+        D.cols = d == 0 ? n : m;
+        D.np = 1;
+        D.width = D.cols;
+        D.pan[0].rp = rps[d];
+        D.pan[0].ci = cis[d];
+        D.pan[0].va = vas[d];
+        D.pan[0].nnz = nnz;
+        D.pan[0].g = zspmv_lanes(nnz, D.rows);
+      }
+      rp = ci = rpt = cit = nullptr;
+      va = vat = nullptr;
+    }
+  } else if (rc == PK_OK) {
     rc = val_prec == PK_F64
              ? build_transpose<double>(m, n, nnz, rp, ci, va, &rpt, &cit, &vat, s)
              : build_transpose<float>(m, n, nnz, rp, ci, va, &rpt, &cit, &vat, s);
-  if (rc == PK_OK) {
+  }
+  if (rc == PK_OK && !prec_is_complex(val_prec)) {
     // build_dir owns the arrays from here on
     const int pa = panels_a > 0 ? pow2_floor(panels_a) : panels_for(false, m, n, nnz, op->val_bytes);
     const int pt = panels_t > 0 ? pow2_floor(panels_t) : panels_for(true, n, m, nnz, op->val_bytes);
@@ -829,6 +856,47 @@ int spmv_any(CsrOp* op, char trans, const T* x, T* y, int where) {
   }
   if (rc == PK_OK && where == PK_HOST &&
       hipMemcpyAsync(y, dy, sizeof(T) * rout, hipMemcpyDeviceToHost, s) != hipSuccess)
+    rc = PK_ERR_HIP;
+  if (hipStreamSynchronize(s) != hipSuccess && rc == PK_OK) rc = PK_ERR_HIP;
+  if (where == PK_HOST) {
+    (void)hipFree(dx);
+    (void)hipFree(dy);
+  }
+  (void)hipFree(part);
+  (void)hipStreamDestroy(s);
+  return rc;
+}
+
+// y = op(A) x for a complex operator (trans 'n', 't' or 'c'), vectors in
+// its precision, on the host or the device.
+int zspmv_any(CsrOp* op, char trans, const void* x, void* y, int where) {
+  const bool tr = trans != 'n' && trans != 'N';
+  const long long rin = tr ? op->m : op->n, rout = tr ? op->n : op->m;
+  const size_t eb = op->val_bytes;
+  hipStream_t s;
+  if (hipStreamCreateWithFlags(&s, hipStreamNonBlocking) != hipSuccess) return PK_ERR_HIP;
+  void *dx = nullptr, *dy = nullptr;
+  double* part = nullptr;
+  int rc = PK_OK;
+  const size_t npart = 3 * static_cast<size_t>(zstream_blocks(std::max(op->m, op->n)));
+  if (hipMalloc(&part, sizeof(double) * npart) != hipSuccess) rc = PK_ERR_OOM;
+  if (rc == PK_OK && where == PK_HOST) {
+    if (hipMalloc(&dx, eb * std::max<long long>(rin, 1)) != hipSuccess ||
+        hipMalloc(&dy, eb * std::max<long long>(rout, 1)) != hipSuccess ||
+        hipMemcpyAsync(dx, x, eb * rin, hipMemcpyHostToDevice, s) != hipSuccess)
+      rc = PK_ERR_HIP;
+  } else if (rc == PK_OK) {
+    dx = const_cast<void*>(x);
+    dy = y;
+  }
+  if (rc == PK_OK) {
+    const int nb = op->val_prec == PK_C128
+                       ? zcsr_apply<double>(op, s, trans, dx, dy, nullptr, 0.0, part)
+                       : zcsr_apply<float>(op, s, trans, dx, dy, nullptr, 0.0, part);
+    if (nb < 0) rc = PK_ERR_HIP;
+  }
+  if (rc == PK_OK && where == PK_HOST &&
+      hipMemcpyAsync(y, dy, eb * rout, hipMemcpyDeviceToHost, s) != hipSuccess)
     rc = PK_ERR_HIP;
   if (hipStreamSynchronize(s) != hipSuccess && rc == PK_OK) rc = PK_ERR_HIP;
   if (where == PK_HOST) {
@@ -991,6 +1059,10 @@ int pk_csr_autotune(int m, int n, long long nnz, const int* rowptr, const int* c
                     double* us_a, double* us_t, int* lanes_a, int* lanes_t, int* sort_a,
                     int* sort_t) {
   if (!panels_a || !panels_t) return PK_ERR_ARG;
+  if (prec_is_complex(val_prec)) {
+    set_error("pk_csr_autotune: complex operators have one layout");
+    return PK_ERR_NOTSUP;
+  }
   const bool lanes = lanes_a && lanes_t;
   const bool sorts = sort_a && sort_t;
   static const int kCaps[6] = {0, -1, 8192, 16384, -8192, -16384};
@@ -1062,6 +1134,10 @@ int pk_csr_autotune(int m, int n, long long nnz, const int* rowptr, const int* c
 int pk_csr_set_sort(pk_csr_t h, int cap_a, int cap_t) {
   CsrOp* op = as_csr(h);
   if (!op) return PK_ERR_HANDLE;
+  if (prec_is_complex(op->val_prec)) {
+    set_error("pk_csr_set_sort: complex operators have one layout");
+    return PK_ERR_NOTSUP;
+  }
   for (const int c : {cap_a, cap_t}) {
     const int a = c < -1 ? -c : c;
     if (c != 0 && c != -1 && (a < kSpmvCap || a > kSortCapMax)) {
@@ -1092,6 +1168,10 @@ int pk_csr_sort(pk_csr_t h, int* cap_a, int* cap_t) {
 int pk_csr_set_lanes(pk_csr_t h, int shift_a, int shift_t) {
   CsrOp* op = as_csr(h);
   if (!op) return PK_ERR_HANDLE;
+  if (prec_is_complex(op->val_prec)) {
+    set_error("pk_csr_set_lanes: complex operators have one layout");
+    return PK_ERR_NOTSUP;
+  }
   if (shift_a < -2 || shift_a > 2 || shift_t < -2 || shift_t > 2) {
     set_error("pk_csr_set_lanes: shifts must lie in [-2, 2]");
     return PK_ERR_ARG;
@@ -1130,6 +1210,13 @@ int pk_csr_spmv(pk_csr_t h, char trans, const void* x, void* y, int prec, int wh
   CsrOp* op = as_csr(h);
   if (!op) return PK_ERR_HANDLE;
   if (!x || !y || (where != PK_HOST && where != PK_DEVICE)) return PK_ERR_ARG;
+  if (prec_is_complex(prec) || prec_is_complex(op->val_prec)) {
+    if (prec != op->val_prec) {
+      set_error("pk_csr_spmv: vector precision differs from the operator's");
+      return PK_ERR_NOTSUP;
+    }
+    return zspmv_any(op, trans, x, y, where);
+  }
   if (prec == PK_F64)
     return spmv_any<double>(op, trans, static_cast<const double*>(x), static_cast<double*>(y),
                             where);
@@ -1143,6 +1230,10 @@ int pk_csr_spmv_noalloc(pk_csr_t h, char trans, const void* x, void* y, int prec
   CsrOp* op = as_csr(h);
   if (!op) return PK_ERR_HANDLE;
   if (!x || !y || (prec != PK_F64 && prec != PK_F32)) return PK_ERR_ARG;
+  if (prec_is_complex(op->val_prec)) {
+    set_error("pk_csr_spmv_noalloc: not available for complex operators");
+    return PK_ERR_NOTSUP;
+  }
   if (op->val_prec != prec) {
     set_error("pk_csr_spmv_noalloc: vector precision differs from the operator's");
     return PK_ERR_NOTSUP;
@@ -1200,19 +1291,36 @@ void pk_scsr_aprod_(const char* transa, const int* m, const int* n, const float*
   if (op) spmv_any<float>(op, *transa, x, y, PK_HOST);
 }
 
+void pk_zcsr_aprod_(const char* transa, const int* m, const int* n, const void* x, void* y,
+                    void* zparm, int* iparm, size_t len_transa) {
+  (void)m; (void)n; (void)zparm; (void)len_transa;
+  CsrOp* op = csr_from_iparm(iparm);
+  if (op && op->val_prec == PK_C128) zspmv_any(op, *transa, x, y, PK_HOST);
+}
+
+void pk_ccsr_aprod_(const char* transa, const int* m, const int* n, const void* x, void* y,
+                    void* cparm, int* iparm, size_t len_transa) {
+  (void)m; (void)n; (void)cparm; (void)len_transa;
+  CsrOp* op = csr_from_iparm(iparm);
+  if (op && op->val_prec == PK_C64) zspmv_any(op, *transa, x, y, PK_HOST);
+}
+
 // Fortran-side helper (implicit interface, all arguments by reference):
 // kind = 1 when APROD is the library's pk_dcsr_aprod_, 2 for pk_scsr_aprod_,
-// with a live operator handle in IPARM(1:2) returned in *handle; kind = 0
-// for any other (user) APROD.
+// 3 for pk_zcsr_aprod_, 4 for pk_ccsr_aprod_, with a live operator handle
+// in IPARM(1:2) returned in *handle; kind = 0 for any other (user) APROD.
 void pk_f_classify_(void* aprod, const int* iparm, int* kind, long long* handle) {
   *kind = 0;
   *handle = 0;
-  const bool dk = aprod == reinterpret_cast<void*>(&pk_dcsr_aprod_);
-  const bool sk = aprod == reinterpret_cast<void*>(&pk_scsr_aprod_);
-  if (!dk && !sk) return;
+  int k = 0;
+  if (aprod == reinterpret_cast<void*>(&pk_dcsr_aprod_)) k = 1;
+  if (aprod == reinterpret_cast<void*>(&pk_scsr_aprod_)) k = 2;
+  if (aprod == reinterpret_cast<void*>(&pk_zcsr_aprod_)) k = 3;
+  if (aprod == reinterpret_cast<void*>(&pk_ccsr_aprod_)) k = 4;
+  if (k == 0) return;
   CsrOp* op = csr_from_iparm(iparm);
   if (!op) return;
-  *kind = dk ? 1 : 2;
+  *kind = k;
   *handle = static_cast<long long>(reinterpret_cast<uintptr_t>(op));
 }
 
@@ -1227,6 +1335,16 @@ void pk_f_set_callback_s_(const long long* eng, void* aprod, float* sparm, int* 
                           int* info) {
   *info = pk_lbd_set_callback(reinterpret_cast<pk_lbd_t>(static_cast<uintptr_t>(*eng)), aprod,
                               PK_F32, sparm, iparm);
+}
+void pk_f_set_callback_z_(const long long* eng, void* aprod, void* zparm, int* iparm,
+                          int* info) {
+  *info = pk_lbd_set_callback(reinterpret_cast<pk_lbd_t>(static_cast<uintptr_t>(*eng)), aprod,
+                              PK_C128, zparm, iparm);
+}
+void pk_f_set_callback_c_(const long long* eng, void* aprod, void* cparm, int* iparm,
+                          int* info) {
+  *info = pk_lbd_set_callback(reinterpret_cast<pk_lbd_t>(static_cast<uintptr_t>(*eng)), aprod,
+                              PK_C64, cparm, iparm);
 }
 
 }  // extern "C"

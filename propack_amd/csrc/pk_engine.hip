@@ -127,8 +127,15 @@ void profile_resolve(Engine* e) {
   e->prof.pending.clear();
 }
 
+}  // namespace pk
+
+#include "pk_lbd_host.h"
+
+namespace pk {
+
 template <typename T>
 struct Impl {
+  using Real = T;
   Engine* e;
   hipStream_t strm;
   explicit Impl(Engine* eng) : e(eng), strm(eng->stream) {}
@@ -295,30 +302,8 @@ struct Impl {
     return PK_OK;
   }
 
-  // dsafescal (double/dsafescal.F:4-55): x /= alpha; below the safe minimum
-  // the factor is applied in the stepwise way of dlascl.
-  int safescal(long long L, double alpha, T* x) {
-    if (std::fabs(alpha) >= Mach<T>::sfmin) return scale(x, L, 1.0 / alpha);
-    const double smlnum = Mach<T>::sfmin, bignum = 1.0 / smlnum;
-    double cfromc = alpha, ctoc = 1.0;
-    bool done = false;
-    while (!done) {
-      const double cfrom1 = cfromc * smlnum, cto1 = ctoc / bignum;
-      double mul;
-      if (std::fabs(cfrom1) > std::fabs(ctoc) && ctoc != 0.0) {
-        mul = smlnum;
-        cfromc = cfrom1;
-      } else if (std::fabs(cto1) > std::fabs(cfromc)) {
-        mul = bignum;
-        ctoc = cto1;
-      } else {
-        mul = ctoc / cfromc;
-        done = true;
-      }
-      PK_TRY(scale(x, L, mul));
-    }
-    return PK_OK;
-  }
+  // dsafescal (double/dsafescal.F:4-55)
+  int safescal(long long L, double alpha, T* x) { return lbd_safescal(*this, L, alpha, x); }
 
   // dreorth + dcgs (double/dreorth.F:5-210): iterated block classical
   // Gram-Schmidt of vnew against basis(:, INDEX intervals), up to 5 passes
@@ -453,114 +438,49 @@ struct Impl {
     return PK_OK;
   }
 
-  // Execute the current phase of the step on the host (same control
-  // functions as the device chain, plus the rare paths).
-  int host_phase(LbdCtlRef& c) {
-    const int j = c.j, m = e->m, n = e->n;
-    double s2 = 0, sd = 0, anormest = 0;
-    int e2 = 0;
-    if (c.phase >= PH_V_INVARIANT && c.phase <= PH_U_ZERO) e->cnt.rare[c.phase - PH_V_INVARIANT]++;
-    switch (c.phase) {
-      case PH_V_SPMV:
-        if (c.uscale) {
-          PK_TRY(scale(ucol(j), m, c.uscale_f));
-          c.uscale = 0;
-        }
-        PK_TRY(apply('t', ucol(j), vcol(j), j > 1 ? vcol(j - 1) : nullptr, j > 1 ? c.beta : 0.0,
-                     &s2, &sd));
-        ctl_v_spmv(c, s2, sd);
-        break;
-      case PH_V_ELR:
-        PK_TRY(axpy_read(n, vcol(j), vcol(j - 1), c.elr_s, &s2, &sd));
-        ctl_v_elr(c, s2, sd);
-        break;
-      case PH_V_REO:
-        PK_TRY(reo_pass(n, V(), e->ldv, vcol(j), c, &s2));
-        ctl_v_reo(c, s2);
-        break;
-      case PH_V_INVARIANT:  // dlanbpro.F:372-409
-        c.rnorm = c.alpha;
-        c.alpha = 0.0;
-        PK_TRY(getu0('t', j - 1, 3, vcol(j), &c.alpha, V(), e->ldv, &anormest, &e2));
-        c.ierr = e2;
-        if (c.alpha == 0.0) {
-          c.k = j - 1;
-          c.ierr = -j;
-          c.phase = PH_DONE;
-        } else {
-          PK_TRY(safescal(n, c.alpha, vcol(j)));
-          c.alpha = 0.0;
-          c.force_reorth = 1;
-          if (c.delta > 0.0) c.full_reorth = 0;
-          c.B1[j] = 0.0;
-          c.vscale = 0;
-          c.phase = PH_U_SPMV;
-        }
-        break;
-      case PH_V_TINY:
-        PK_TRY(safescal(n, c.alpha, vcol(j)));
-        c.vscale = 0;
-        c.phase = PH_U_SPMV;
-        break;
-      case PH_V_ZERO:  // dreorth.F:96-99
-        PK_LAUNCH(launch_fill_zero_bytes(strm, vcol(j), sizeof(T) * n));
-        c.alpha = 0.0;
-        c.nreorth++;
-        ctl_v_after_reorth(c);
-        break;
-      case PH_U_SPMV:
-        if (c.vscale) {
-          PK_TRY(scale(vcol(j), n, c.vscale_f));
-          c.vscale = 0;
-        }
-        PK_TRY(apply('n', vcol(j), ucol(j + 1), ucol(j), c.alpha, &s2, &sd));
-        ctl_u_spmv(c, s2, sd);
-        break;
-      case PH_U_ELR:
-        PK_TRY(axpy_read(m, ucol(j + 1), ucol(j), c.elr_s, &s2, &sd));
-        ctl_u_elr(c, s2, sd);
-        break;
-      case PH_U_REO:
-        PK_TRY(reo_pass(m, U(), e->ldu, ucol(j + 1), c, &s2));
-        ctl_u_reo(c, s2);
-        break;
-      case PH_U_INVARIANT:  // dlanbpro.F:503-539
-        c.rnorm = c.beta;
-        c.beta = 0.0;
-        PK_TRY(getu0('n', j, 3, ucol(j + 1), &c.beta, U(), e->ldu, &anormest, &e2));
-        c.ierr = e2;
-        if (c.beta == 0.0) {
-          c.k = j;
-          c.ierr = -j;
-          c.phase = PH_DONE;
-        } else {
-          PK_TRY(safescal(m, c.beta, ucol(j + 1)));
-          c.beta = 0.0;
-          c.force_reorth = 1;
-          if (c.delta > 0.0) c.full_reorth = 0;
-          c.B2[j] = 0.0;
-          c.uscale = 0;
-          c.rnorm = 0.0;
-          c.j = j + 1;
-          c.phase = c.j > c.k ? PH_DONE : PH_V_SPMV;
-        }
-        break;
-      case PH_U_TINY:
-        PK_TRY(safescal(m, c.beta, ucol(j + 1)));
+  // Execute the current phase of the step on the host (lbd_host_phase).
+  int host_phase(LbdCtlRef& c) { return lbd_host_phase(*this, c); }
+
+  int zero(T* x, long long L) {
+    PK_LAUNCH(launch_fill_zero_bytes(strm, x, sizeof(T) * L));
+    return PK_OK;
+  }
+
+  // The operator application of one side of step c.j with its control stage
+  // (double/dlanbpro.F:288-296 for v, :420-424 for u).
+  int spmv_step(LbdCtlRef& c, bool vside) {
+    const int j = c.j;
+    double s2 = 0, sd = 0;
+    if (vside) {
+      if (c.uscale) {
+        PK_TRY(scale(ucol(j), e->m, c.uscale_f));
         c.uscale = 0;
-        c.rnorm = c.beta;
-        c.j = j + 1;
-        c.phase = c.j > c.k ? PH_DONE : PH_V_SPMV;
-        break;
-      case PH_U_ZERO:
-        PK_LAUNCH(launch_fill_zero_bytes(strm, ucol(j + 1), sizeof(T) * m));
-        c.beta = 0.0;
-        c.nreorth++;
-        ctl_u_after_reorth(c);
-        break;
-      default:
-        set_error("lanbpro: unknown phase %d", c.phase);
-        return PK_ERR_ARG;
+      }
+      PK_TRY(apply('t', ucol(j), vcol(j), j > 1 ? vcol(j - 1) : nullptr, j > 1 ? c.beta : 0.0,
+                   &s2, &sd));
+      ctl_v_spmv(c, s2, sd);
+    } else {
+      if (c.vscale) {
+        PK_TRY(scale(vcol(j), e->n, c.vscale_f));
+        c.vscale = 0;
+      }
+      PK_TRY(apply('n', vcol(j), ucol(j + 1), ucol(j), c.alpha, &s2, &sd));
+      ctl_u_spmv(c, s2, sd);
+    }
+    return PK_OK;
+  }
+
+  // One extended local reorthogonalization of a side (dlanbpro.F:301-316,
+  // :429-443).
+  int elr_step(LbdCtlRef& c, bool vside) {
+    const int j = c.j;
+    double s2 = 0, sd = 0;
+    if (vside) {
+      PK_TRY(axpy_read(e->n, vcol(j), vcol(j - 1), c.elr_s, &s2, &sd));
+      ctl_v_elr(c, s2, sd);
+    } else {
+      PK_TRY(axpy_read(e->m, ucol(j + 1), ucol(j), c.elr_s, &s2, &sd));
+      ctl_u_elr(c, s2, sd);
     }
     return PK_OK;
   }
@@ -679,143 +599,10 @@ struct Impl {
     return lanbpro_end(k0, kp, alpha_, beta_, rnorm, doption, ierr_out);
   }
 
-  // Setup of a DLANBPRO call (double/dlanbpro.F:134-282): the state of the
-  // first step goes to e->hctl.  *finished: the call already returned (as
-  // the reference does for a zero B at k0 > 0).
+  // Setup of a DLANBPRO call (lbd_begin).
   int lanbpro_begin(int k0, int k_in, double* alpha_, double* beta_, double* rnorm,
                     double* doption, const int* ioption, int* ierr_out, bool* finished) {
-    *finished = false;
-    const double one = 1.0, zero = 0.0, FUDGE = 1.01, kappa = 0.717;
-    const int m = e->m, n = e->n;
-    int k = k_in, ierr = 0;
-    const double eps = Mach<T>::eps;
-    const double leff = eff_length();
-    const double epsn = leff * eps;
-    const double epsn2 = std::sqrt(leff) * eps;
-    double delta, eta, anorm, anormest = 0, s = 0, a1, b1, amax = 0, alpha, beta;
-    bool full_reorth, force_reorth;
-    int j0;
-    if (k > e->kmax || k0 < 0 || k0 > k) {
-      set_error("lanbpro: k=%d k0=%d outside the engine's kmax=%d", k, k0, e->kmax);
-      return PK_ERR_ARG;
-    }
-    LbdCtlRef& c = *e->hctl;
-    std::memset(static_cast<LbdCtl*>(&c), 0, sizeof(LbdCtl));
-    c.cap = e->kmax;
-    std::memset(e->harr, 0, static_cast<size_t>(CtlArrays::bytes(e->kmax)));
-    double* mu = c.mu;
-    double* nu = c.nu;
-    int* index = c.index;
-    double* B1 = c.B1;  // B(j,1) = B1[j]
-    double* B2 = c.B2;  // B(j,2) = B2[j]
-    for (int i = 1; i <= k; ++i) {
-      B1[i] = alpha_[i - 1];
-      B2[i] = beta_[i - 1];
-    }
-
-    delta = doption[0] < zero ? std::sqrt(eps / k) : doption[0];
-    eta = doption[1] < zero ? std::pow(eps, 0.75) / std::sqrt(static_cast<double>(k)) : doption[1];
-    full_reorth = (delta <= eta || delta == zero);
-    if (doption[2] > zero) {
-      anorm = doption[2];
-    } else if (k0 > 0) {
-      anorm = lapy2(B1[1], B2[1]);
-      if (anorm <= zero) {
-        *ierr_out = -1;
-        doption[2] = anorm;
-        *finished = true;
-        return PK_OK;
-      }
-    } else {
-      anorm = zero;
-    }
-    ierr = 0;
-
-    if (*rnorm == zero) {
-      int e2;
-      PK_TRY(getu0('n', k0, 3, ucol(k0 + 1), rnorm, U(), e->ldu, &anormest, &e2));
-      ierr = e2;
-      anorm = std::max(anorm, anormest);
-    }
-
-    if (k0 == 0) {
-      amax = zero;
-      alpha = zero;
-      beta = *rnorm;
-      force_reorth = false;
-      int e2;
-      if (n > m)
-        PK_TRY(getu0('n', 0, 1, scratch(1), &s, U(), e->ldu, &anormest, &e2));
-      else
-        PK_TRY(getu0('t', 0, 1, scratch(1), &s, V(), e->ldv, &anormest, &e2));
-      ierr = e2;
-      anorm = std::max(anorm, FUDGE * anormest);
-      j0 = 1;
-      if (beta != zero) PK_TRY(safescal(m, beta, ucol(1)));
-      mu[1] = one;
-      nu[1] = one;
-    } else {
-      force_reorth = true;
-      alpha = B1[k0];
-      beta = *rnorm;
-      if (k0 < k && beta * delta < anorm * eps) {
-        full_reorth = true;
-        ierr = k0;
-      }
-      index[1] = 1;
-      index[2] = k0;
-      index[3] = k0 + 1;
-      PK_TRY(scale(ucol(k0 + 1), m, *rnorm));
-      PK_TRY(reorth(m, k0, U(), e->ldu, ucol(k0 + 1), rnorm, index, kappa));
-      PK_TRY(safescal(m, *rnorm, ucol(k0 + 1)));
-      set_mu(k0, mu, index, epsn2);
-      set_mu(k0, nu, index, epsn2);
-      beta = *rnorm;
-      B2[k0] = beta;
-      amax = zero;
-      for (int j = 1; j <= k0; ++j) {
-        amax = std::max(amax, std::max(B1[j], B2[j]));
-        if (j == 1) {
-          anorm = std::max(anorm, FUDGE * alpha);
-        } else if (j == 2) {
-          a1 = B2[1] / amax;
-          a1 = FUDGE * amax * std::sqrt((B1[1] / amax) * (B1[1] / amax) + a1 * a1 + B1[2] / amax * a1);
-          anorm = std::max(anorm, a1);
-        } else {
-          a1 = B1[j - 1] / amax;
-          b1 = B2[j - 1] / amax;
-          a1 = FUDGE * amax * std::sqrt(a1 * a1 + b1 * b1 + a1 * B2[j - 2] / amax + B1[j] / amax * b1);
-          anorm = std::max(anorm, a1);
-        }
-      }
-      j0 = k0 + 1;
-    }
-
-    c.m = m;
-    c.n = n;
-    c.k = k;
-    c.elr = ioption[1];
-    c.delta = delta;
-    c.eta = eta;
-    c.eps = eps;
-    c.epsn = epsn;
-    c.epsn2 = epsn2;
-    c.sfmin = Mach<T>::sfmin;
-    c.kappa = kappa;
-    c.fudge = FUDGE;
-    c.j = j0;
-    c.phase = j0 <= k ? PH_V_SPMV : PH_DONE;
-    c.full_reorth = full_reorth;
-    c.force_reorth = force_reorth;
-    c.ierr = ierr;
-    c.alpha = alpha;
-    c.beta = beta;
-    c.anorm = anorm;
-    c.amax = amax;
-    c.numax = zero;
-    c.mumax = zero;
-    c.rnorm = *rnorm;
-    return PK_OK;
+    return lbd_begin(*this, k0, k_in, alpha_, beta_, rnorm, doption, ioption, ierr_out, finished);
   }
 
   // Steps of the call from the state in e->hctl until it is done: device
@@ -841,36 +628,10 @@ struct Impl {
     return PK_OK;
   }
 
-  // End of the call: outputs in the reference's places (:540-548).
+  // End of the call (lbd_end).
   int lanbpro_end(int k0, int* kp, double* alpha_, double* beta_, double* rnorm, double* doption,
                   int* ierr_out) {
-    LbdCtlRef& c = *e->hctl;
-    const double* B1 = c.B1;
-    const double* B2 = c.B2;
-    // The 1/beta scaling of u_{k+1} is deferred to the start of the next
-    // step (see the chains); the last step of the call applies it here.
-    if (c.uscale) {
-      PK_TRY(scale(ucol(c.j), e->m, c.uscale_f));
-      c.uscale = 0;
-    }
-
-    for (int i = 1; i <= c.k; ++i) {
-      alpha_[i - 1] = B1[i];
-      beta_[i - 1] = B2[i];
-    }
-    if (k0 > 0 && k0 <= c.k) beta_[k0 - 1] = B2[k0];
-    doption[2] = c.anorm;
-    *kp = c.k;
-    *ierr_out = c.ierr;
-    *rnorm = c.rnorm;
-    e->cnt.nopx += c.nopx;
-    e->cnt.nreorth += c.nreorth;
-    e->cnt.ndot += c.ndot;
-    e->cnt.ncols_touched += c.ncols_touched;
-    e->cnt.nretry += c.nretry;
-    e->cnt.nwarn += c.nwarn;
-    PK_HIP(stream_wait(strm));
-    return resolve_timers();
+    return lbd_end(*this, k0, kp, alpha_, beta_, rnorm, doption, ierr_out);
   }
 
   // Residuals of the triplets (sigma[i], u_{i+1}, v_{i+1}) held in src's
@@ -909,42 +670,349 @@ struct Impl {
 
   // X(:, 1:ncols) = X(:, 1:kdim) * op(B) into the spare buffer, then swap.
   int update(char which, char transb, int kdim, int ncols, const double* B, int ldb) {
-    const bool isu = (which == 'u' || which == 'U');
-    const bool tb = (transb == 't' || transb == 'T');
-    const int maxk = isu ? e->kmax + 1 : e->kmax;
-    if (kdim < 1 || ncols < 1 || kdim > maxk || ncols > maxk) {
-      set_error("update: kdim=%d ncols=%d outside [1, %d]", kdim, ncols, maxk);
-      return PK_ERR_ARG;
-    }
-    if (ldb < (tb ? ncols : kdim)) {
-      set_error("update: ldb=%d too small", ldb);
-      return PK_ERR_ARG;
-    }
-    void** buf = isu ? e->U : e->V;
-    int& cur = isu ? e->ucur : e->vcur;
-    const long long ld = isu ? e->ldu : e->ldv;
-    const long long L = isu ? e->m : e->n;
-    if (!buf[1 - cur]) {
-      PK_HIP(hipMalloc(&buf[1 - cur], e->tbytes * ld * maxk));
-    }
-    // The call returns with the GEMM in flight: the host refills hB only
-    // after the stream has read its previous contents, and dB is rewritten by
-    // a copy that follows the previous GEMM on the same stream.
-    if (e->evB) PK_HIP(hipEventSynchronize(e->evB));
-    for (int j = 0; j < ncols; ++j)
-      for (int l = 0; l < kdim; ++l)
-        e->hB[static_cast<size_t>(j) * kdim + l] =
-            tb ? B[static_cast<size_t>(l) * ldb + j] : B[static_cast<size_t>(j) * ldb + l];
-    PK_HIP(hipMemcpyAsync(e->dB, e->hB, sizeof(double) * kdim * ncols, hipMemcpyHostToDevice, strm));
-    if (!e->evB) PK_HIP(hipEventCreateWithFlags(&e->evB, hipEventDisableTiming));
-    PK_HIP(hipEventRecord(e->evB, strm));
+    return lbd_update(*this, which, transb, kdim, ncols, B, ldb);
+  }
+
+  // C = A(:, 1:kdim) * dB (kdim x ncols) on FP64 matrix cores, L rows.
+  int gemm(long long L, const void* A, long long ld, int kdim, int ncols, void* C) {
     hipEvent_t t0 = tstart(KC_GEMM);
-    PK_LAUNCH(launch_gemm_mfma<T>(strm, L, static_cast<const T*>(buf[cur]), ld, kdim, e->dB, kdim,
-                                  ncols, static_cast<T*>(buf[1 - cur]), ld));
+    PK_LAUNCH(launch_gemm_mfma<T>(strm, L, static_cast<const T*>(A), ld, kdim, e->dB, kdim,
+                                  ncols, static_cast<T*>(C), ld));
     tstop(t0, KC_GEMM, static_cast<double>(sizeof(T)) * L * (kdim + ncols),
           2.0 * static_cast<double>(L) * kdim * ncols);
-    cur = 1 - cur;
     return PK_OK;
+  }
+};
+
+// ---- complex bases: ZLANBPRO / CLANBPRO ----
+// Extended local reorthogonalization of the complex tree
+// (complex16/zlanbpro.F:316-335 for v, :449-466 for u): the coefficient
+// s = v_{j-1}^H v_j is complex, so B keeps its entry (the real tree adds s
+// to it); otherwise the stage is the real one.  nsd is the real part of the
+// next coefficient.
+static void zctl_v_elr(LbdCtlRef& c, double ns2, double nsd) {
+  const double sn = std::sqrt(ns2);
+  c.elr_i++;
+  if (sn >= c.kappa * c.alpha || c.elr_i >= c.elr) {
+    c.nu[c.j - 1] = c.eps;
+    c.alpha = sn;
+    ctl_v_fin(c);
+  } else {
+    c.alpha = sn;
+    c.elr_s = nsd;
+  }
+  ctl_resume(c);
+}
+
+static void zctl_u_elr(LbdCtlRef& c, double ns2, double nsd) {
+  const double sn = std::sqrt(ns2);
+  c.elr_i++;
+  if (sn >= c.kappa * c.beta || c.elr_i >= c.elr) {
+    c.mu[c.j] = c.eps;
+    c.beta = sn;
+    ctl_u_fin(c);
+  } else {
+    c.beta = sn;
+    c.elr_s = nsd;
+  }
+  ctl_resume(c);
+}
+
+template <typename R>
+struct CxOf;
+template <>
+struct CxOf<double> {
+  using type = double2;
+};
+template <>
+struct CxOf<float> {
+  using type = float2;
+};
+
+// The engine over complex bases (PK_C128: R = double, PK_C64: R = float).
+// Every step runs on the host executor (the phase machine of pk_lbd.h), one
+// synchronization per reduction; norms, real scalings, the random start
+// vector and the GEMMs run the real kernels on the 2L-long real view of a
+// column, the operator, dot products and Gram-Schmidt the complex kernels
+// of pk_cplx.hip.
+template <typename R>
+struct ZImpl {
+  using Real = R;
+  using T = typename CxOf<R>::type;
+  Engine* e;
+  hipStream_t strm;
+  explicit ZImpl(Engine* eng) : e(eng), strm(eng->stream) {}
+
+  T* U() const { return static_cast<T*>(e->u()); }
+  T* V() const { return static_cast<T*>(e->v()); }
+  T* ucol(int j) const { return U() + static_cast<long long>(j - 1) * e->ldu; }
+  T* vcol(int j) const { return V() + static_cast<long long>(j - 1) * e->ldv; }
+  T* scratch(int i) const {
+    return static_cast<T*>(e->vec) + static_cast<long long>(i) * std::max(e->m, e->n);
+  }
+  static R* re(T* p) { return reinterpret_cast<R*>(p); }
+  static const R* re(const T* p) { return reinterpret_cast<const R*>(p); }
+  double* hsum() const { return e->red + 2 * e->ccap; }  // slot of the CGS norm
+
+  int read(int offset, int nvals) {
+    PK_HIP(hipMemcpyAsync(e->hred + offset, e->red + offset, sizeof(double) * nvals,
+                          hipMemcpyDeviceToHost, strm));
+    PK_HIP(stream_wait(strm));
+    e->cnt.nsync++;
+    return resolve_timers();
+  }
+  int resolve_timers() {
+    profile_resolve(e);
+    return PK_OK;
+  }
+  double eff_length() const {
+    return e->rounding ? 4.0 : static_cast<double>(std::max(e->m, e->n));
+  }
+
+  // ||x|| (pdznrm2, complex16/zblasext.F:6-30)
+  int nrm2(const T* x, long long L, double* out) {
+    if (L <= 0) {
+      *out = 0.0;
+      return PK_OK;
+    }
+    const int nb = launch_norm<R>(strm, 2 * L, re(x), e->part);
+    PK_LAUNCH(nb);
+    PK_LAUNCH(launch_reduce(strm, 1, nb, e->part, e->red, Gate{}));
+    PK_TRY(read(0, 1));
+    *out = std::sqrt(e->hred[0]);
+    return PK_OK;
+  }
+
+  int scale(T* x, long long L, double a) {
+    PK_LAUNCH(launch_scale<R>(strm, 2 * L, re(x), sval(a), Gate{}));
+    return PK_OK;
+  }
+  int zero(T* x, long long L) {
+    PK_LAUNCH(launch_fill_zero_bytes(strm, x, sizeof(T) * L));
+    return PK_OK;
+  }
+
+  // y = op(A) x - c * yprev (trans 'n' or 'c' = A^H); *s2 = ||y||^2,
+  // (*sdr, *sdi) = yprev^H y (0 without yprev).
+  int apply(char trans, const T* x, T* y, const T* yprev, double c, double* s2, double* sdr,
+            double* sdi) {
+    const bool tr = trans != 'n' && trans != 'N';
+    const long long rows_out = tr ? e->n : e->m, rows_in = tr ? e->m : e->n;
+    int nb;
+    if (e->op) {
+      nb = zcsr_apply<R>(e->op, strm, tr ? 'c' : 'n', x, y, yprev, c, e->part);
+      PK_LAUNCH(nb);
+    } else if (e->aprod) {
+      PK_TRY(host_apply(tr ? 'c' : 'n', x, y, rows_in, rows_out));
+      if (!yprev) {
+        double nrm;
+        PK_TRY(nrm2(y, rows_out, &nrm));
+        *s2 = nrm * nrm;
+        *sdr = *sdi = 0.0;
+        return PK_OK;
+      }
+      nb = launch_zaxpy_norm_dot<R>(strm, rows_out, y, yprev, c, 0.0, e->part);
+      PK_LAUNCH(nb);
+    } else {
+      set_error("engine has no operator");
+      return PK_ERR_ARG;
+    }
+    PK_LAUNCH(launch_reduce(strm, 3, nb, e->part, e->red, Gate{}));
+    PK_TRY(read(0, 3));
+    *s2 = e->hred[0];
+    *sdr = yprev ? e->hred[1] : 0.0;
+    *sdi = yprev ? e->hred[2] : 0.0;
+    return PK_OK;
+  }
+
+  // Host callback (complex16/zlanbpro.F:28-41 with 'n' / 'c'): X staged
+  // device -> pinned host, Y back; the callback's precision is the engine's.
+  int host_apply(char trans, const T* x, T* y, long long rows_in, long long rows_out) {
+    const int m = e->m, n = e->n;
+    PK_HIP(hipMemcpyAsync(e->hx, x, sizeof(T) * rows_in, hipMemcpyDeviceToHost, strm));
+    PK_HIP(hipStreamSynchronize(strm));
+    e->cnt.nsync++;
+    typedef void (*zaprod_fn)(const char*, const int*, const int*, const void*, void*, void*,
+                              int*, size_t);
+    reinterpret_cast<zaprod_fn>(e->aprod)(&trans, &m, &n, e->hx, e->hy, e->dparm, e->iparm, 1);
+    PK_HIP(hipMemcpyAsync(y, e->hy, sizeof(T) * rows_out, hipMemcpyHostToDevice, strm));
+    return PK_OK;
+  }
+
+  // One block classical Gram-Schmidt pass of vnew against basis(:, cols)
+  // (zcgs, complex16/zreorth.F:120-205): h = basis^H vnew, vnew -= basis h;
+  // returns ||vnew'||^2.
+  int cgs_pass(long long L, const T* basis, long long ld, T* vnew, int ncols, double* s2) {
+    const int nb = launch_zgemvt<R>(strm, L, basis, ld, e->dcols, ncols, vnew, e->part);
+    PK_LAUNCH(nb);
+    PK_LAUNCH(launch_reduce(strm, 2 * ncols, nb, e->part, e->red, Gate{}));
+    const int nb2 = launch_zgemv_update<R>(strm, L, basis, ld, e->dcols, ncols, e->red, vnew,
+                                           e->part);
+    PK_LAUNCH(nb2);
+    PK_LAUNCH(launch_reduce(strm, 1, nb2, e->part, hsum(), Gate{}));
+    PK_TRY(read(2 * e->ccap, 1));
+    *s2 = e->hred[2 * e->ccap];
+    return PK_OK;
+  }
+
+  // zreorth (complex16/zreorth.F:5-110): up to 5 passes until
+  // ||vnew'|| > alpha ||vnew||; otherwise vnew = 0.  INDEX is 1-based.
+  int reorth(long long L, int k, T* basis, long long ld, T* vnew, double* normvnew,
+             const int* index, double alpha) {
+    if (k <= 0 || L <= 0) return PK_OK;
+    int ncols = 0;
+    long long nd = 0;
+    for (int i = 1; index[i] <= k && index[i] > 0; i += 2) {
+      const int p = index[i], q = index[i + 1];
+      nd += q - p + 1;
+      for (int c = p; c <= q; ++c) {
+        if (ncols >= e->ccap) {
+          set_error("reorthogonalization touches more than %d columns", e->ccap);
+          return PK_ERR_NOTSUP;
+        }
+        e->hcols[ncols++] = c - 1;
+      }
+    }
+    if (ncols > 0)
+      PK_HIP(hipMemcpyAsync(e->dcols, e->hcols, sizeof(int) * ncols, hipMemcpyHostToDevice, strm));
+    bool ok = false;
+    for (int itry = 1; itry <= 5; ++itry) {
+      const double n0 = *normvnew;
+      e->cnt.ndot += nd + k;
+      e->cnt.ncols_touched += ncols;
+      if (ncols > 0) {
+        double s2;
+        PK_TRY(cgs_pass(L, basis, ld, vnew, ncols, &s2));
+        *normvnew = std::sqrt(s2);
+      } else {
+        PK_TRY(nrm2(vnew, L, normvnew));
+      }
+      if (*normvnew > alpha * n0) {
+        ok = true;
+        break;
+      }
+    }
+    if (!ok) {
+      *normvnew = 0.0;
+      PK_TRY(zero(vnew, L));
+    }
+    e->cnt.nreorth++;
+    return PK_OK;
+  }
+
+  // zgetu0 (complex16/zgetu0.F:11-90): a ZLARNV(IDIST=2) vector -- real and
+  // imaginary parts drawn in turn from the xLARUV sequence, so the complex
+  // vector of length r is the real sequence of length 2r -- through op(A),
+  // then orthogonalized against basis(:, 1:j).
+  int getu0(char transa, int j, int ntry, T* u0, double* u0norm, T* basis, long long ld,
+            double* anormest, int* ierr) {
+    const bool tn = (transa == 'n' || transa == 'N');
+    const long long rsize = tn ? e->n : e->m, usize = tn ? e->m : e->n;
+    T* x = scratch(0);
+    const double kap = static_cast<double>(0.717f);  // single-precision literal, zgetu0.F:28-29
+    *ierr = -1;
+    for (int itry = 1; itry <= ntry; ++itry) {
+      PK_LAUNCH(launch_larnv_pm1<R>(strm, 2 * rsize, re(x), (itry - 1) * 2 * rsize));
+      double nrm;
+      PK_TRY(nrm2(x, rsize, &nrm));
+      double s2, sr, si;
+      PK_TRY(apply(tn ? 'n' : 'c', x, u0, nullptr, 0.0, &s2, &sr, &si));
+      e->cnt.nopx++;
+      *u0norm = std::sqrt(s2);
+      *anormest = *u0norm / nrm;
+      if (j >= 1) {
+        int index[4] = {0, 1, j, j + 1};
+        PK_TRY(reorth(usize, j, basis, ld, u0, u0norm, index, kap));
+      }
+      if (*u0norm > 0) {
+        *ierr = 0;
+        break;
+      }
+    }
+    return PK_OK;
+  }
+
+  // One CGS pass of the step (host executor): returns ||vnew'||^2.
+  int reo_pass(long long L, const T* basis, long long ld, T* vnew, const LbdCtlRef& c,
+               double* s2) {
+    const int ncols = c.reo_ncols;
+    if (ncols > 0) {
+      std::memcpy(e->hcols, c.cols, sizeof(int) * ncols);
+      PK_HIP(hipMemcpyAsync(e->dcols, e->hcols, sizeof(int) * ncols, hipMemcpyHostToDevice, strm));
+      return cgs_pass(L, basis, ld, vnew, ncols, s2);
+    }
+    double nrm;
+    PK_TRY(nrm2(vnew, L, &nrm));
+    *s2 = nrm * nrm;
+    return PK_OK;
+  }
+
+  // The operator application of one side of step c.j with its control stage
+  // (complex16/zlanbpro.F:297-311 for v, :444-452 for u).
+  int spmv_step(LbdCtlRef& c, bool vside) {
+    const int j = c.j;
+    double s2 = 0, sr = 0, si = 0;
+    if (vside) {
+      if (c.uscale) {
+        PK_TRY(scale(ucol(j), e->m, c.uscale_f));
+        c.uscale = 0;
+      }
+      PK_TRY(apply('c', ucol(j), vcol(j), j > 1 ? vcol(j - 1) : nullptr, j > 1 ? c.beta : 0.0,
+                   &s2, &sr, &si));
+      e->elr_im = si;
+      ctl_v_spmv(c, s2, sr);
+    } else {
+      if (c.vscale) {
+        PK_TRY(scale(vcol(j), e->n, c.vscale_f));
+        c.vscale = 0;
+      }
+      PK_TRY(apply('n', vcol(j), ucol(j + 1), ucol(j), c.alpha, &s2, &sr, &si));
+      e->elr_im = si;
+      ctl_u_spmv(c, s2, sr);
+    }
+    return PK_OK;
+  }
+
+  // v_j -= s v_{j-1} with s = v_{j-1}^H v_j (pzaxpy with -s,
+  // complex16/zlanbpro.F:321-323; u side :454-456).
+  int elr_step(LbdCtlRef& c, bool vside) {
+    const int j = c.j;
+    const long long L = vside ? e->n : e->m;
+    T* w = vside ? vcol(j) : ucol(j + 1);
+    const T* v = vside ? vcol(j - 1) : ucol(j);
+    const int nb = launch_zaxpy_norm_dot<R>(strm, L, w, v, c.elr_s, e->elr_im, e->part);
+    PK_LAUNCH(nb);
+    PK_LAUNCH(launch_reduce(strm, 3, nb, e->part, e->red, Gate{}));
+    PK_TRY(read(0, 3));
+    e->elr_im = e->hred[2];
+    if (vside)
+      zctl_v_elr(c, e->hred[0], e->hred[1]);
+    else
+      zctl_u_elr(c, e->hred[0], e->hred[1]);
+    return PK_OK;
+  }
+
+  // zlanbpro (complex16/zlanbpro.F:1-553): every phase on the host executor.
+  int lanbpro(int k0, int* kp, double* alpha_, double* beta_, double* rnorm, double* doption,
+              const int* ioption, int* ierr_out) {
+    bool finished = false;
+    PK_TRY(lbd_begin(*this, k0, *kp, alpha_, beta_, rnorm, doption, ioption, ierr_out,
+                     &finished));
+    if (finished) return PK_OK;
+    LbdCtlRef& c = *e->hctl;
+    while (c.phase != PH_DONE) PK_TRY(lbd_host_phase(*this, c));
+    return lbd_end(*this, k0, kp, alpha_, beta_, rnorm, doption, ierr_out);
+  }
+
+  // Restart / Ritz GEMM on the real view: a complex column of length L is a
+  // real column of length 2L and B is real (complex16/zgemm_ovwr.F:6-39).
+  int gemm(long long L, const void* A, long long ld, int kdim, int ncols, void* C) {
+    PK_LAUNCH(launch_gemm_mfma<R>(strm, 2 * L, static_cast<const R*>(A), 2 * ld, kdim, e->dB,
+                                  kdim, ncols, static_cast<R*>(C), 2 * ld));
+    return PK_OK;
+  }
+  int update(char which, char transb, int kdim, int ncols, const double* B, int ldb) {
+    return lbd_update(*this, which, transb, kdim, ncols, B, ldb);
   }
 };
 
@@ -954,6 +1022,10 @@ int engine_lanbpro_begin(Engine* e, int k0, int k, double* alpha, double* beta, 
   if (k > e->kmax || k0 < 0 || k0 > k) {
     set_error("lanbpro: k=%d k0=%d outside the engine's kmax=%d", k, k0, e->kmax);
     return PK_ERR_ARG;
+  }
+  if (prec_is_complex(e->prec)) {
+    set_error("batched steps take real engines only");
+    return PK_ERR_NOTSUP;
   }
   if (e->prec == PK_F64)
     return Impl<double>(e).lanbpro_begin(k0, k, alpha, beta, rnorm, doption, ioption, ierr,
@@ -1041,7 +1113,7 @@ const char* pk_last_error(void) { return pk::last_error(); }
 
 int pk_lbd_create(int prec, int m, int n, int kmax, pk_csr_t op, pk_lbd_t* out) {
   if (!out || m < 1 || n < 1 || kmax < 1 ||
-      (prec != PK_F64 && prec != PK_F32)) {
+      (prec != PK_F64 && prec != PK_F32 && !prec_is_complex(prec))) {
     set_error("pk_lbd_create: bad arguments m=%d n=%d kmax=%d prec=%d", m, n, kmax, prec);
     return PK_ERR_ARG;
   }
@@ -1064,7 +1136,7 @@ int pk_lbd_create(int prec, int m, int n, int kmax, pk_csr_t op, pk_lbd_t* out) 
   e->m = m;
   e->n = n;
   e->kmax = kmax;
-  e->tbytes = prec == PK_F64 ? 8 : 4;
+  e->tbytes = prec_bytes(prec);
   e->ldu = round_up(m, 64);
   e->ldv = round_up(n, 64);
   e->op = cop;
@@ -1076,6 +1148,9 @@ int pk_lbd_create(int prec, int m, int n, int kmax, pk_csr_t op, pk_lbd_t* out) 
     const size_t need = 2 * static_cast<size_t>(std::max(cop->nrb, cop->nrbt));
     nblk = std::max(nblk, (need + e->ccap - 1) / e->ccap);
   }
+  // complex bases: three partials per SpMV slot and two per projected
+  // column, the projection (2 ccap values) and the CGS norm in red
+  const size_t cx = prec_is_complex(prec) ? 2 : 1;
   const size_t nb = static_cast<size_t>(kmax + 1) * (kmax + 1);
   hipError_t err = hipSuccess;
   auto chk = [&](hipError_t r) {
@@ -1084,17 +1159,17 @@ int pk_lbd_create(int prec, int m, int n, int kmax, pk_csr_t op, pk_lbd_t* out) 
   chk(hipStreamCreateWithFlags(&e->stream, hipStreamNonBlocking));
   chk(hipMalloc(&e->U[0], e->tbytes * e->ldu * (kmax + 1)));
   chk(hipMalloc(&e->V[0], e->tbytes * e->ldv * kmax));
-  chk(hipMalloc(&e->part, sizeof(double) * nblk * e->ccap));
-  chk(hipMalloc(&e->red, sizeof(double) * (e->ccap + 16)));
+  chk(hipMalloc(&e->part, sizeof(double) * nblk * e->ccap * cx));
+  chk(hipMalloc(&e->red, sizeof(double) * (e->ccap * cx + 16)));
   chk(hipMalloc(&e->dcols, sizeof(int) * e->ccap));
   chk(hipMalloc(&e->dB, sizeof(double) * nb));
   chk(hipMalloc(&e->vec, e->tbytes * mx * 2));
-  if (cop) {
+  if (cop && !prec_is_complex(prec)) {
     e->sw_y_len = csr_ywork(cop);
     chk(hipMalloc(&e->sw_y, sizeof(double) * e->sw_y_len));
     chk(hipMalloc(&e->sw_l, sizeof(double) * csr_lwork(cop)));
   }
-  chk(hipHostMalloc(&e->hred, sizeof(double) * (e->ccap + 16), hipHostMallocDefault));
+  chk(hipHostMalloc(&e->hred, sizeof(double) * (e->ccap * cx + 16), hipHostMallocDefault));
   chk(hipHostMalloc(&e->hcols, sizeof(int) * e->ccap, hipHostMallocDefault));
   chk(hipHostMalloc(&e->hB, sizeof(double) * nb, hipHostMallocDefault));
   const size_t ab = static_cast<size_t>(CtlArrays::bytes(kmax));
@@ -1132,13 +1207,18 @@ int pk_lbd_destroy(pk_lbd_t h) {
 
 int pk_lbd_set_callback(pk_lbd_t h, void* aprod, int aprod_prec, void* dparm, int* iparm) {
   PK_ENGINE(h);
-  if (!aprod || (aprod_prec != PK_F64 && aprod_prec != PK_F32)) return PK_ERR_ARG;
+  if (!aprod || (prec_is_complex(e->prec) ? aprod_prec != e->prec
+                                          : (aprod_prec != PK_F64 && aprod_prec != PK_F32))) {
+    set_error("pk_lbd_set_callback: a complex engine takes a callback of its own precision, a "
+              "real engine one of PK_F64 or PK_F32");
+    return PK_ERR_ARG;
+  }
   e->aprod = aprod;
   e->aprod_prec = aprod_prec;
   e->dparm = dparm;
   e->iparm = iparm;
   if (!e->hx) {
-    const size_t b = sizeof(double) * std::max(e->m, e->n);
+    const size_t b = std::max<size_t>(sizeof(double), e->tbytes) * std::max(e->m, e->n);
     if (hipHostMalloc(&e->hx, b, hipHostMallocDefault) != hipSuccess ||
         hipHostMalloc(&e->hy, b, hipHostMallocDefault) != hipSuccess) {
       set_error("pk_lbd_set_callback: pinned allocation failed");
@@ -1156,17 +1236,23 @@ int pk_lbd_put(pk_lbd_t h, char which, int col, const void* x, int prec) {
   const long long L = isu ? e->m : e->n;
   char* dst = static_cast<char*>(isu ? e->u() : e->v()) +
               e->tbytes * static_cast<size_t>(col - 1) * (isu ? e->ldu : e->ldv);
-  const size_t sb = prec == PK_F64 ? 8 : 4;
+  if (prec_is_complex(prec) != prec_is_complex(e->prec)) {
+    set_error("pk_lbd_put: real and complex arrays do not mix");
+    return PK_ERR_ARG;
+  }
+  const size_t sb = prec_bytes(prec);
+  // a complex column converts as its real view of 2L values
+  const long long LC = prec_is_complex(prec) ? 2 * L : L;
   // hipMemcpyDefault: x may be host or device memory (the runtime reads the
   // direction from the pointer); a device-resident x costs a device copy.
   if (sb == e->tbytes) {
     PK_HIP(hipMemcpyAsync(dst, x, sb * L, hipMemcpyDefault, e->stream));
   } else {
     PK_HIP(hipMemcpyAsync(e->vec, x, sb * L, hipMemcpyDefault, e->stream));
-    int r = e->prec == PK_F64
-                ? launch_convert<float, double>(e->stream, L, static_cast<const float*>(e->vec),
+    int r = (e->prec == PK_F64 || e->prec == PK_C128)
+                ? launch_convert<float, double>(e->stream, LC, static_cast<const float*>(e->vec),
                                                 reinterpret_cast<double*>(dst))
-                : launch_convert<double, float>(e->stream, L, static_cast<const double*>(e->vec),
+                : launch_convert<double, float>(e->stream, LC, static_cast<const double*>(e->vec),
                                                 reinterpret_cast<float*>(dst));
     if (r < 0) return PK_ERR_HIP;
   }
@@ -1180,7 +1266,12 @@ int pk_lbd_get(pk_lbd_t h, char which, int ncols, void* X, int ldx, int prec) {
   const int maxc = isu ? e->kmax + 1 : e->kmax;
   const long long L = isu ? e->m : e->n;
   if (ncols < 0 || ncols > maxc || (ncols > 0 && !X) || ldx < L) return PK_ERR_ARG;
-  const size_t sb = prec == PK_F64 ? 8 : 4;
+  if (prec_is_complex(prec) != prec_is_complex(e->prec)) {
+    set_error("pk_lbd_get: real and complex arrays do not mix");
+    return PK_ERR_ARG;
+  }
+  const size_t sb = prec_bytes(prec);
+  const long long LC = prec_is_complex(prec) ? 2 * L : L;
   const char* src = static_cast<const char*>(isu ? e->u() : e->v());
   const long long ld = isu ? e->ldu : e->ldv;
   for (int j = 0; j < ncols; ++j) {
@@ -1189,10 +1280,10 @@ int pk_lbd_get(pk_lbd_t h, char which, int ncols, void* X, int ldx, int prec) {
     if (sb == e->tbytes) {
       PK_HIP(hipMemcpyAsync(cd, cs, sb * L, hipMemcpyDeviceToHost, e->stream));
     } else {
-      int r = e->prec == PK_F64
-                  ? launch_convert<double, float>(e->stream, L, reinterpret_cast<const double*>(cs),
+      int r = (e->prec == PK_F64 || e->prec == PK_C128)
+                  ? launch_convert<double, float>(e->stream, LC, reinterpret_cast<const double*>(cs),
                                                   static_cast<float*>(e->vec))
-                  : launch_convert<float, double>(e->stream, L, reinterpret_cast<const float*>(cs),
+                  : launch_convert<float, double>(e->stream, LC, reinterpret_cast<const float*>(cs),
                                                   static_cast<double*>(e->vec));
       if (r < 0) return PK_ERR_HIP;
       PK_HIP(hipMemcpyAsync(cd, e->vec, sb * L, hipMemcpyDeviceToHost, e->stream));
@@ -1216,6 +1307,14 @@ int pk_lbd_nrm2(pk_lbd_t h, char which, int col, double* nrm) {
   const bool isu = (which == 'u' || which == 'U');
   const int maxc = isu ? e->kmax + 1 : e->kmax;
   if (col < 1 || col > maxc || !nrm) return PK_ERR_ARG;
+  if (prec_is_complex(e->prec)) {
+    if (e->prec == PK_C128) {
+      ZImpl<double> im(e);
+      return im.nrm2(isu ? im.ucol(col) : im.vcol(col), isu ? e->m : e->n, nrm);
+    }
+    ZImpl<float> im(e);
+    return im.nrm2(isu ? im.ucol(col) : im.vcol(col), isu ? e->m : e->n, nrm);
+  }
   if (e->prec == PK_F64) {
     Impl<double> im(e);
     return im.nrm2(isu ? im.ucol(col) : im.vcol(col), isu ? e->m : e->n, nrm);
@@ -1231,6 +1330,16 @@ int pk_lbd_getu0(pk_lbd_t h, char transa, int j, int ntry, double* u0norm, int i
   const bool tn = (transa == 'n' || transa == 'N');
   const int maxc = tn ? e->kmax + 1 : e->kmax;
   if (j < 0 || j + 1 > maxc || ntry < 1 || !u0norm || !anormest || !ierr) return PK_ERR_ARG;
+  if (e->prec == PK_C128) {
+    ZImpl<double> im(e);
+    return im.getu0(transa, j, ntry, tn ? im.ucol(j + 1) : im.vcol(j + 1), u0norm,
+                    tn ? im.U() : im.V(), tn ? e->ldu : e->ldv, anormest, ierr);
+  }
+  if (e->prec == PK_C64) {
+    ZImpl<float> im(e);
+    return im.getu0(transa, j, ntry, tn ? im.ucol(j + 1) : im.vcol(j + 1), u0norm,
+                    tn ? im.U() : im.V(), tn ? e->ldu : e->ldv, anormest, ierr);
+  }
   if (e->prec == PK_F64) {
     Impl<double> im(e);
     return im.getu0(transa, j, ntry, tn ? im.ucol(j + 1) : im.vcol(j + 1), u0norm,
@@ -1245,6 +1354,10 @@ int pk_lbd_lanbpro(pk_lbd_t h, int k0, int* k, double* alpha, double* beta, doub
                    double* doption, const int* ioption, int* ierr) {
   PK_ENGINE(h);
   if (!k || !alpha || !beta || !rnorm || !doption || !ioption || !ierr) return PK_ERR_ARG;
+  if (e->prec == PK_C128)
+    return ZImpl<double>(e).lanbpro(k0, k, alpha, beta, rnorm, doption, ioption, ierr);
+  if (e->prec == PK_C64)
+    return ZImpl<float>(e).lanbpro(k0, k, alpha, beta, rnorm, doption, ioption, ierr);
   if (e->prec == PK_F64) return Impl<double>(e).lanbpro(k0, k, alpha, beta, rnorm, doption, ioption, ierr);
   return Impl<float>(e).lanbpro(k0, k, alpha, beta, rnorm, doption, ioption, ierr);
 }
@@ -1253,6 +1366,8 @@ int pk_lbd_update(pk_lbd_t h, char which, char transb, int kdim, int ncols, cons
                   int ldb) {
   PK_ENGINE(h);
   if (!B) return PK_ERR_ARG;
+  if (e->prec == PK_C128) return ZImpl<double>(e).update(which, transb, kdim, ncols, B, ldb);
+  if (e->prec == PK_C64) return ZImpl<float>(e).update(which, transb, kdim, ncols, B, ldb);
   if (e->prec == PK_F64) return Impl<double>(e).update(which, transb, kdim, ncols, B, ldb);
   return Impl<float>(e).update(which, transb, kdim, ncols, B, ldb);
 }
@@ -1288,6 +1403,10 @@ int pk_lbd_residuals(pk_lbd_t h, pk_lbd_t src_h, int k, const double* sigma, dou
   PK_ENGINE(h);
   Engine* sr = as_engine(src_h);
   if (!sr) return PK_ERR_HANDLE;
+  if (prec_is_complex(e->prec) || prec_is_complex(sr->prec)) {
+    set_error("pk_lbd_residuals: real engines only");
+    return PK_ERR_NOTSUP;
+  }
   if (!e->op || sr->m != e->m || sr->n != e->n || k < 0 || k > sr->kmax ||
       (k > 0 && (!sigma || !res))) {
     set_error("pk_lbd_residuals: needs an engine with a device operator, a source engine of "
@@ -1302,6 +1421,10 @@ int pk_lbd_promote(pk_lbd_t dst_h, pk_lbd_t src_h, int ncols) {
   Engine* d = as_engine(dst_h);
   Engine* sr = as_engine(src_h);
   if (!d || !sr) return PK_ERR_HANDLE;
+  if (prec_is_complex(d->prec) || prec_is_complex(sr->prec)) {
+    set_error("pk_lbd_promote: real engines only");
+    return PK_ERR_NOTSUP;
+  }
   if (d->m != sr->m || d->n != sr->n || ncols < 0 || ncols > std::min(d->kmax, sr->kmax)) {
     set_error("pk_lbd_promote: engines of one shape and 0 <= ncols <= both kmax");
     return PK_ERR_ARG;
@@ -1338,7 +1461,8 @@ int pk_lbd_seed(pk_lbd_t dst_h, pk_lbd_t src_h, int ncols) {
   Engine* d = as_engine(dst_h);
   Engine* sr = as_engine(src_h);
   if (!d || !sr) return PK_ERR_HANDLE;
-  if (d->prec != PK_F64 || d->m != sr->m || ncols < 1 || ncols > sr->kmax + 1) {
+  if (d->prec != PK_F64 || prec_is_complex(sr->prec) || d->m != sr->m || ncols < 1 ||
+      ncols > sr->kmax + 1) {
     set_error("pk_lbd_seed: needs a double engine of the same shape and 1 <= ncols <= kmax+1");
     return PK_ERR_ARG;
   }
@@ -1411,7 +1535,7 @@ int pk_lbd_set_rounding(pk_lbd_t h, int model) {
 
 int pk_lbd_eps(pk_lbd_t h, double* eps, double* leff) {
   PK_ENGINE(h);
-  if (eps) *eps = e->prec == PK_F64 ? 0x1p-53 : 0x1p-24;
+  if (eps) *eps = (e->prec == PK_F64 || e->prec == PK_C128) ? 0x1p-53 : 0x1p-24;
   if (leff) *leff = e->rounding ? 4.0 : static_cast<double>(std::max(e->m, e->n));
   return PK_OK;
 }

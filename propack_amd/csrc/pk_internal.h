@@ -130,6 +130,37 @@ int csr_apply(const CsrOp* op, hipStream_t s, char trans, const T* x, T* y, cons
 // len > kSpmvCap nonzeros has ceil(len / kLongChunk) <= len / kSpmvCap chunks.
 inline long long csr_lwork(const CsrOp* op) { return op->nnz / 1024 + 2; }
 
+// ---- complex precisions (pk_cplx.hip) ----
+// A complex operator (val_prec PK_C128 / PK_C64) keeps A and the explicit
+// A^T as one plain CSR panel each (da.pan[0], dt.pan[0]; values as (re, im)
+// pairs, lanes per row in pan[0].g); the layout options of the real
+// operators do not apply to it.
+inline bool prec_is_complex(int prec) { return prec == PK_C128 || prec == PK_C64; }
+inline size_t prec_bytes(int prec) {
+  return prec == PK_F64 ? 8 : prec == PK_F32 ? 4 : prec == PK_C128 ? 16 : 8;
+}
+int zstream_blocks(long long L);
+int zspmv_lanes(long long nnz, long long rows);
+// y = op(A) x - c yprev (trans 'n', 't' or 'c' = A^H) with partials
+// ||y||^2, Re and Im of yprev^H y (three values per slot); returns the slot
+// count.  R is the real type of the storage precision.
+template <typename R>
+int zcsr_apply(const CsrOp* op, hipStream_t s, char trans, const void* x, void* y,
+               const void* yprev, double c, double* part);
+// w -= s v (complex s = sr + i si); partials ||w||^2, Re and Im of v^H w.
+template <typename R>
+int launch_zaxpy_norm_dot(hipStream_t s, long long L, void* w, const void* v, double sr,
+                          double si, double* part);
+// Partials of h = V(:, cols)^H w: real and imaginary parts of h_j at
+// part[(2j) * nblk + b] and part[(2j+1) * nblk + b].
+template <typename R>
+int launch_zgemvt(hipStream_t s, long long L, const void* V, long long ldv, const int* cols,
+                  int ncols, const void* w, double* part);
+// w -= V(:, cols) h (h as 2 ncols doubles, re / im pairs); partial ||w||^2.
+template <typename R>
+int launch_zgemv_update(hipStream_t s, long long L, const void* V, long long ldv,
+                        const int* cols, int ncols, const double* h, void* w, double* part);
+
 // Fortran APROD callback: (TRANSA, M, N, X, Y, DPARM, IPARM, len(TRANSA)).
 typedef void (*aprod_d_fn)(const char*, const int*, const int*, const double*, double*,
                            double*, int*, size_t);
@@ -163,7 +194,8 @@ struct Counters {
 struct Engine {
   static constexpr unsigned kMagic = 0x504b4c42u;  // "PKLB"
   unsigned magic = 0;
-  int prec = PK_F64;                 // storage type of bases (and of the operator values)
+  int prec = PK_F64;                 // storage type of bases (and of the operator values):
+                                     // PK_F64, PK_F32, or complex PK_C128 / PK_C64
   int m = 0, n = 0, kmax = 0;
   long long ldu = 0, ldv = 0;
   void* U[2] = {nullptr, nullptr};   // current / spare left basis  (ldu x (kmax+1))
@@ -210,6 +242,9 @@ struct Engine {
   // accumulation, where max(m, n) * 2^-24 would exceed the Lanczos residuals
   // of large problems and flag false invariant subspaces).
   int rounding = 0;
+  // complex bases (prec PK_C128 / PK_C64): the imaginary part of the
+  // pending ELR coefficient (its real part is LbdCtl::elr_s)
+  double elr_im = 0.0;
   Counters cnt;
   Profile prof;
   int last_error = 0;

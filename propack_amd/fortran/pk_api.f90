@@ -5,7 +5,7 @@ module pk_api
   use iso_c_binding
   implicit none
   private
-  integer(c_int), parameter, public :: PK_F64 = 0, PK_F32 = 1
+  integer(c_int), parameter, public :: PK_F64 = 0, PK_F32 = 1, PK_C128 = 2, PK_C64 = 3
   integer(c_int), parameter, public :: PK_OK = 0
 
   public :: pk_lbd_create, pk_lbd_destroy, pk_lbd_put, pk_lbd_get, pk_lbd_nrm2, &

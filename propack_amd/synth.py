@@ -30,7 +30,7 @@ class Csr:
     n: int
     rowptr: np.ndarray  # int32, length m + 1
     col: np.ndarray     # int32, length nnz
-    val: np.ndarray     # float64 or float32, length nnz
+    val: np.ndarray     # float64, float32, complex128 or complex64, length nnz
 
     @property
     def nnz(self) -> int:
@@ -51,17 +51,26 @@ class Csr:
         return Csr(self.n, self.m, rp.astype(np.int32),
                    rows[order].astype(np.int32), self.val[order].copy())
 
+    @property
+    def is_complex(self) -> bool:
+        return np.iscomplexobj(self.val)
+
+    def _wide(self):
+        return np.complex128 if self.is_complex else np.float64
+
     def todense(self) -> np.ndarray:
-        a = np.zeros((self.m, self.n), dtype=np.float64)
+        a = np.zeros((self.m, self.n), dtype=self._wide())
         rows = np.repeat(np.arange(self.m), np.diff(self.rowptr))
-        np.add.at(a, (rows, self.col), self.val.astype(np.float64))
+        np.add.at(a, (rows, self.col), self.val.astype(self._wide()))
         return a
 
     def matvec(self, x: np.ndarray) -> np.ndarray:
-        """y = A x in float64 (host helper for checks, not a product path)."""
+        """y = A x in float64 / complex128 (host helper for checks, not a
+        product path)."""
         rows = np.repeat(np.arange(self.m), np.diff(self.rowptr))
-        y = np.zeros(self.m, dtype=np.float64)
-        np.add.at(y, rows, self.val.astype(np.float64) * x[self.col])
+        wide = np.result_type(self._wide(), np.asarray(x).dtype)
+        y = np.zeros(self.m, dtype=wide)
+        np.add.at(y, rows, self.val.astype(wide) * x[self.col])
         return y
 
 
@@ -83,8 +92,9 @@ def from_coo(m: int, n: int, rows: np.ndarray, cols: np.ndarray,
         first = np.ones(key.size, dtype=bool)
         first[1:] = key[1:] != key[:-1]
         seg = np.cumsum(first) - 1
-        summed = np.zeros(int(first.sum()), dtype=np.float64)
-        np.add.at(summed, seg, vals.astype(np.float64))
+        wide = np.complex128 if np.iscomplexobj(vals) else np.float64
+        summed = np.zeros(int(first.sum()), dtype=wide)
+        np.add.at(summed, seg, vals.astype(wide))
         key = key[first]
         vals = summed.astype(vals.dtype)
     r = (key // n).astype(np.int64)
@@ -104,6 +114,26 @@ def random_uniform(m: int = 1850, n: int = 712, nnz_draws: int = 8636,
     cols = g.integers(0, n, nnz_draws)
     vals = g.standard_normal(nnz_draws)
     return from_coo(m, n, rows, cols, vals).astype(dtype)
+
+
+def complex_uniform(m: int = 1850, n: int = 712, nnz_draws: int = 8636,
+                    seed: int = 1850, dtype=np.complex128) -> Csr:
+    """random_uniform's sparsity pattern with complex N(0,1) + i N(0,1)
+    values (the imaginary parts drawn after the real ones)."""
+    g = _rng(seed)
+    rows = g.integers(0, m, nnz_draws)
+    cols = g.integers(0, n, nnz_draws)
+    vals = g.standard_normal(nnz_draws) + 1j * g.standard_normal(nnz_draws)
+    return from_coo(m, n, rows, cols, vals).astype(dtype)
+
+
+def complex_poisson_decay(m: int = 1_000_000, n: int = 500_000, lam: float = 16.0,
+                          p: float = 0.6, seed: int = 2025, dtype=np.complex128) -> Csr:
+    """poisson_decay's pattern and decay with each value multiplied by a
+    seeded unit phase exp(i theta), theta uniform on [0, 2 pi)."""
+    A = poisson_decay(m, n, lam, p, seed=seed, dtype=np.float64)
+    theta = _rng(seed + 1).uniform(0.0, 2.0 * np.pi, A.nnz)
+    return Csr(A.m, A.n, A.rowptr, A.col, (A.val * np.exp(1j * theta)).astype(dtype))
 
 
 def poisson_decay(m: int = 2_000_000, n: int = 1_000_000, lam: float = 16.0,
@@ -249,5 +279,10 @@ def batch_problem(prm: dict, dtype=np.float64) -> Csr:
 
 
 def start_vector(m: int, seed: int, dtype=np.float64) -> np.ndarray:
-    """Seeded N(0,1) start vector of length m."""
-    return _rng(seed).standard_normal(m).astype(dtype)
+    """Seeded N(0,1) start vector of length m (complex dtypes: N(0,1) real
+    parts, then N(0,1) imaginary parts)."""
+    g = _rng(seed)
+    if np.issubdtype(np.dtype(dtype), np.complexfloating):
+        re = g.standard_normal(m)
+        return (re + 1j * g.standard_normal(m)).astype(dtype)
+    return g.standard_normal(m).astype(dtype)

@@ -19,8 +19,8 @@ def declared_functions():
         for m in re.finditer(r"\b([A-Za-z_]\w*)\s*\(", txt):
             name = m.group(1)
             if name.startswith(("pk_", "pkh_")) or name in (
-                    "dlansvd_", "dlansvd_irl_", "slansvd_", "slansvd_irl_", "clearstat_",
-                    "printstat_"):
+                    "dlansvd_", "dlansvd_irl_", "slansvd_", "slansvd_irl_", "zlansvd_",
+                    "zlansvd_irl_", "clansvd_", "clansvd_irl_", "clearstat_", "printstat_"):
                 names.add(name)
     # typedef'd function-pointer names are not exports
     return sorted(n for n in names if not n.endswith("_t"))
