@@ -21,7 +21,8 @@ bounded sample of the same solve.
 
 Other workloads: --workload powerlaw (8M x 4M Zipf bipartite, FP32 basis +
 FP64 polish), --workload batch (256 uneven problems, batched vs per-problem
-loop), --workload spmv-probe (SpMV time by column pattern).
+loop), --workload complex (ZLANSVD_IRL on a 1M x 500k complex matrix),
+--workload spmv-probe (SpMV time by column pattern).
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu]
 """
@@ -46,7 +47,6 @@ os.environ.setdefault("OMP_STACKSIZE", "512M")
 from propack_amd import api, synth  # noqa: E402
 
 HBM_PEAK_GBS = 8000.0      # MI355X HBM3E
-FP64_PEAK_TFLOPS = 78.6    # MI355X dense FP64 matrix (MFMA)
 FP64_PEAK_TFLOPS = 78.6    # MI355X dense FP64 MFMA
 
 M, N, LAM, DECAY, SEED = 2_000_000, 1_000_000, 16.0, 0.6, 2024
@@ -312,6 +312,124 @@ def workload_c3(args):
     print(json.dumps(out))
 
 
+def workload_complex(args):
+    """ZLANSVD_IRL (complex16/zlansvd_irl.F) on a complex 1,000,000 x 500,000
+    CSR matrix: the poisson_decay pattern (Poisson(16) distinct columns per
+    row, |a_ij| = |N(0,1)| (1 + j)^-0.6) with a seeded unit phase per value;
+    k=50, dim=120, tol=1e-10, complex FP64 bases, start vector resident in
+    HBM.  Live timing of every kernel class in one more solve; the complex
+    SpMV's achieved bandwidth against HBM peak; residuals of three triplets
+    on the host; the reference ZLANSVD_IRL (oracle/_ref) on the host cores on
+    one Lanczos pass (maxiter = 1), scaled by operator applications."""
+    k, dim, tol = 50, 120, 1e-10
+    m, n, seed = 1_000_000, 500_000, 2025
+    t0 = time.perf_counter()
+    A = synth.complex_poisson_decay(m, n, seed=seed)
+    u0h = synth.start_vector(m, seed + 2, np.complex128)
+    import torch
+    u0 = torch.from_numpy(u0h).to("cuda")
+    log(f"[bench] complex matrix {A.m}x{A.n} nnz={A.nnz} in {time.perf_counter() - t0:.1f}s")
+    op = api.CsrOperator(A, api.PK_C128)
+    eng = api.Engine(op, kmax=dim)
+    sync = lambda: api.lib().pk_lbd_sync(eng.handle)  # noqa: E731
+
+    def solve():
+        eng.put_u1(u0)
+        return api.lansvd_irl_dev(eng, k, dim, tol=tol, jobu="y", jobv="y", fetch=False)
+    r, t = _timed(solve, args.steps, sync)
+    log(f"[bench] complex solve {t:.4f} s, k={r.k} info={r.info} {r.counters}")
+    eng.profile(True)
+    eng.put_u1(u0)
+    rp = api.lansvd_irl_dev(eng, k, dim, tol=tol, jobu="y", jobv="y", fetch=False)
+    sync()
+    prof = eng.profile_get()
+    eng.profile(False)
+    kern = {c: dict(v, GBps=(v["bytes"] / (v["ms"] * 1e6) if v["ms"] else None))
+            for c, v in prof.items()}
+    # the dominant kernel: the SpMV class with more time
+    dom = max(("spmv_n", "spmv_t"), key=lambda c: prof[c]["ms"])
+    pd = prof[dom]
+    achieved = pd["bytes"] / (pd["ms"] * 1e6) if pd["ms"] else 0.0
+    # residuals of three triplets on the host
+    nres = min(3, r.k)
+    resid = None
+    if nres and not args.no_check:
+        U = eng.get("u", nres)
+        V = eng.get("v", nres)
+        res = [np.linalg.norm(A.matvec(V[:, i]) - r.sigma[i] * U[:, i]) for i in range(nres)]
+        resid = float(max(res) / r.sigma[0])
+    cpu = None
+    if not args.no_cpu:
+        cpu = cpu_baseline_complex(A, u0h, r.counters["nopx"], k, dim, tol)
+    out = {"metric": "ZLANSVD_IRL time to 50 triplets at resid<=1e-10 (complex FP64 basis)",
+           "value": t, "unit": "s", "n_gpus": 1, "steps": args.steps, "warmup": 1,
+           "ms_per_step": t * 1e3, "higher_is_better": False, "scaling": "strong",
+           "vs_baseline": None, "dtype": "c128",
+           "data": "synthetic (seeded generator, matrix resident in HBM)",
+           "config": {"workload": f"ZLANSVD_IRL k={k} dim={dim} tol={tol} on "
+                                  f"complex_poisson_decay {m}x{n} (Poisson(16) per row, "
+                                  "decay 0.6, unit phases)",
+                      "m": A.m, "n": A.n, "nnz": A.nnz, "converged": r.k, "info": r.info,
+                      "nopx": r.counters["nopx"], "nreorth": r.counters["nreorth"],
+                      "nrestart": r.counters["nrestart"], "max_rel_residual_3": resid},
+           "roofline": {"kernel": dom, "bound": "hbm", "achieved": round(achieved, 1),
+                        "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                        "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                        "avg_launch_ms": pd["ms"] / max(pd["launches"], 1),
+                        "bytes_per_launch": pd["bytes"] / max(pd["launches"], 1),
+                        "timed_in": "one profiled solve after the timed ones"},
+           "kernels": kern,
+           "profiled_solve_nopx": rp.counters["nopx"],
+           "cpu_baseline": cpu}
+    print(json.dumps(out))
+
+
+def cpu_baseline_complex(A, u0, gpu_nopx, k, dim, tol):
+    """Reference ZLANSVD_IRL (oracle/_ref) on one Lanczos pass (maxiter = 1),
+    scaled by the device run's operator applications; median of 3 runs on
+    all host cores (a one-thread run of this sample takes over a minute and
+    is left out).  The reference runs its
+    modified Gram-Schmidt (ioption(1) = 0): its complex classical
+    Gram-Schmidt races on many threads (complex16/zreorth.F:140-201)."""
+    try:
+        from oracle import refpy
+    except Exception as exc:  # pragma: no cover
+        return {"error": f"oracle unavailable: {exc}"}
+    if not refpy.available("z"):
+        return {"error": "oracle/_ref not built"}
+    cores = os.cpu_count() or 1
+    AT = A.transpose()
+    out = {}
+
+    def run(threads):
+        refpy.set_threads(threads, "z")
+        t0 = time.perf_counter()
+        out["r"] = refpy.lansvd_irl(A, k, dim, u0=u0, tol=tol, maxiter=1, cgs=0, elr=1, AT=AT,
+                                    jobu="y", jobv="y", prec="z")
+        out["t"] = time.perf_counter() - t0
+
+    threading.stack_size(1 << 30)
+
+    def timed(threads):
+        th = threading.Thread(target=run, args=(threads,))
+        th.start()
+        th.join()
+        return out["r"], out["t"]
+
+    runs = [timed(cores) for _ in range(3)]
+    ts = sorted(t for _, t in runs)
+    r, t = runs[0][0], ts[1]
+    nopx = r.counters["nopx"]
+    keep = ("nopx", "nreorth", "ndot", "nrestart", "nbsvd")
+    return {"value": t * gpu_nopx / nopx, "unit": "s", "cores": cores, "kind": "reference",
+            "sample": f"ZLANSVD_IRL maxiter=1 (MGS, singular vectors on) on the same matrix, "
+                      f"{nopx} operator applications, median of 3 runs on {cores} threads "
+                      f"{t:.2f} s wall; scaled by the device run's {gpu_nopx} applications",
+            "sample_s": t, "sample_s_runs": ts, "sample_nopx": nopx, "info": r.info,
+            "counters": {c: r.counters[c] for c in keep},
+            "cpu_model": _cpu_model()}
+
+
 def workload_spmv_probe(args):
     """Where the SpMV time goes: A x on the bench matrix's rows (same row
     pointers, values and nonzero count) with three column patterns: every
@@ -503,12 +621,14 @@ def main():
                     help="column-sorted SpMV layout block caps 'A,At' with --panels "
                          "(default 0,0: row-block layout), or of every problem of --workload "
                          "batch (default 8192,8192)")
-    ap.add_argument("--workload", choices=["c2", "c3", "powerlaw", "batch", "spmv-probe"],
+    ap.add_argument("--workload", choices=["c2", "c3", "powerlaw", "batch", "complex",
+                                           "spmv-probe"],
                     default="c2",
                     help="c2 (default): the headline solve; powerlaw: 8M x 4M Zipf "
                          "bipartite, k=64, dim=160, tol=1e-8, FP32 basis + FP64 polish vs "
                          "FP64; batch: the 256-problem batched workload vs a per-problem loop; "
-                         "c3: the FP32-basis form of the 1850x712 DLANSVD solve")
+                         "c3: the FP32-basis form of the 1850x712 DLANSVD solve; complex: "
+                         "ZLANSVD_IRL on a 1M x 500k complex matrix")
     ap.add_argument("--count", type=int, default=256, help="problems of --workload batch")
     ap.add_argument("--no-graph", action="store_true",
                     help="--workload batch: enqueue the batched step chains directly instead of "
@@ -540,6 +660,9 @@ def main():
         return
     if args.workload == "spmv-probe":
         workload_spmv_probe(args)
+        return
+    if args.workload == "complex":
+        workload_complex(args)
         return
 
     t0 = time.perf_counter()

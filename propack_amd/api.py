@@ -368,20 +368,28 @@ class Engine:
         self.handle = h
 
     def put_u1(self, u0):
-        """U(:,1) <- u0: a host array, or a contiguous float64 torch tensor on
-        the GPU (then a device-to-device copy)."""
+        """U(:,1) <- u0: a host array, or a contiguous torch tensor on the GPU
+        (float64, or complex128 for a complex engine; then a device-to-device
+        copy)."""
+        wide = PK_C128 if is_complex(self.prec) else PK_F64
         if hasattr(u0, "is_cuda") and u0.is_cuda:
-            if str(u0.dtype) != "torch.float64" or not u0.is_contiguous() or u0.numel() != self.m:
-                raise ValueError("put_u1: device start vector must be contiguous float64 of length m")
+            want = "torch.complex128" if wide == PK_C128 else "torch.float64"
+            if str(u0.dtype) != want or not u0.is_contiguous() or u0.numel() != self.m:
+                raise ValueError(f"put_u1: device start vector must be contiguous {want} of "
+                                 "length m")
             ptr = C.c_void_p(u0.data_ptr())
             _check(lib().pk_lbd_put(self.handle, C.c_char(b"u"), C.c_int(1), ptr,
-                                    C.c_int(PK_F64)), "pk_lbd_put")
+                                    C.c_int(wide)), "pk_lbd_put")
             return
-        u0 = np.ascontiguousarray(u0, np.float64)
+        u0 = np.ascontiguousarray(u0, _dt(wide))
         _check(lib().pk_lbd_put(self.handle, C.c_char(b"u"), C.c_int(1), _p(u0),
-                                C.c_int(PK_F64)), "pk_lbd_put")
+                                C.c_int(wide)), "pk_lbd_put")
 
-    def get(self, which: str, ncols: int, prec=PK_F64):
+    def get(self, which: str, ncols: int, prec=None):
+        """The first ncols columns of U or V as an (L x ncols) host array (in
+        double or complex double unless prec says otherwise)."""
+        if prec is None:
+            prec = PK_C128 if is_complex(self.prec) else PK_F64
         L = self.m if which == "u" else self.n
         X = np.zeros((ncols, L), _dt(prec))
         if ncols:
@@ -457,8 +465,9 @@ class Engine:
 def lansvd_irl_dev(eng: Engine, neig, dim, p=None, tol=1e-10, which="L", jobu="y", jobv="y",
                    maxiter=1000, delta=None, eta=None, anorm=0.0, min_relgap=0.002, cgs=1,
                    elr=1, fetch=True) -> SvdResult:
-    """Device-resident DLANSVD_IRL on ``eng`` (start vector = eng U(:,1))."""
-    eps = 2.0 ** -53 if eng.prec == PK_F64 else 2.0 ** -24
+    """Device-resident DLANSVD_IRL (ZLANSVD_IRL for a complex engine) on
+    ``eng`` (start vector = eng U(:,1))."""
+    eps = 2.0 ** -53 if _rt(eng.prec) == np.float64 else 2.0 ** -24
     if p is None:
         p = dim - neig
     sigma = np.zeros(dim + 1)
@@ -642,8 +651,9 @@ def lansvd_irl_batch(batch: Batch, neig, dim, p=None, tol=1e-10, which="L", jobu
 
 def lansvd_dev(eng: Engine, k, kmax, tol=1e-12, jobu="y", jobv="y", delta=None, eta=None,
                anorm=0.0, cgs=1, elr=1, fetch=True) -> SvdResult:
-    """Device-resident DLANSVD on ``eng`` (start vector = eng U(:,1))."""
-    eps = 2.0 ** -53 if eng.prec == PK_F64 else 2.0 ** -24
+    """Device-resident DLANSVD (ZLANSVD for a complex engine) on ``eng``
+    (start vector = eng U(:,1))."""
+    eps = 2.0 ** -53 if _rt(eng.prec) == np.float64 else 2.0 ** -24
     sigma = np.zeros(kmax + 1)
     bnd = np.zeros(kmax + 1)
     dopt = np.array([np.sqrt(2 * eps) if delta is None else delta,

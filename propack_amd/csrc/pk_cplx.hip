@@ -14,8 +14,8 @@
 // results rounded once to the storage precision:
 //   k_zspmv          y = op(A) x - c yprev, op = A or A^H (CSR of A, and of
 //                    the explicit transpose A^T read with conjugated values),
-//                    one group of G lanes per row; partials ||y||^2,
-//                    Re / Im of yprev^H y
+//                    one group of G lanes per row, grid-stride over row
+//                    chunks; partials ||y||^2, Re / Im of yprev^H y
 //   k_zaxpy_norm_dot w -= s v for complex s; partials ||w||^2, Re / Im v^H w
 //   k_zgemvt         h_j = V(:, cols[j])^H w, two partials per column
 //   k_zgemv_update   w -= V(:, cols) h; partial ||w||^2
@@ -35,7 +35,7 @@ namespace {
 
 constexpr int kZRPT = 4;                   // rows per thread of the streaming kernels
 constexpr int kZRows = kBlock * kZRPT;     // rows per workgroup slice
-constexpr int kZCols = 4;                  // basis columns per pass of the projection
+constexpr int kZCols = 8;                  // basis columns per pass of the projection
 
 template <typename R>
 struct V2T;
@@ -49,10 +49,12 @@ struct V2T<float> {
 };
 
 // ----------------------------------------------------------------- SpMV ----
-// Block b owns rows [b * kZRows, (b + 1) * kZRows); its kBlock / G groups
-// take the rows in turn (every lane of a wave runs the same trip count, so
-// the group shuffles see whole groups).  Lane l of a group adds the row's
-// nonzeros l, l + G, ...; the group sum and the epilogue follow.
+// The rows come in chunks of kBlock / G, one row per group of G lanes;
+// workgroup b takes chunks b, b + nblk, ... (a fixed assignment, so the
+// partials are deterministic), and every lane of a wave runs the same trip
+// count, so the group shuffles see whole groups.  Lane l of a group adds the
+// row's nonzeros l, l + G, ..., two at a time so that two gathers of x are
+// in flight; the group sum and the epilogue follow.
 template <typename R, int G>
 __global__ __launch_bounds__(kBlock) void k_zspmv(long long rows, const int* __restrict__ rp,
                                                   const int* __restrict__ ci,
@@ -66,27 +68,36 @@ __global__ __launch_bounds__(kBlock) void k_zspmv(long long rows, const int* __r
   __shared__ double lds[(kBlock / kWave) * 3];
   constexpr int ngrp = kBlock / G;
   const int lane = threadIdx.x % G, grp = threadIdx.x / G;
-  const long long r0 = static_cast<long long>(blockIdx.x) * kZRows;
-  const long long r1 = r0 + kZRows < rows ? r0 + kZRows : rows;
   const double sg = conj ? -1.0 : 1.0;
   double acc[3] = {0.0, 0.0, 0.0};
-  for (long long base = r0; base < r1; base += ngrp) {
+  const long long stride = static_cast<long long>(gridDim.x) * ngrp;
+  for (long long base = static_cast<long long>(blockIdx.x) * ngrp; base < rows; base += stride) {
     const long long row = base + grp;
     double sr = 0.0, si = 0.0;
-    if (row < r1) {
+    if (row < rows) {
       const int p1 = rp[row + 1];
-      for (int p = rp[row] + lane; p < p1; p += G) {
+      int p = rp[row] + lane;
+      for (; p + G < p1; p += 2 * G) {
+        const C2 a0 = va[p], a1 = va[p + G];
+        const C2 x0 = x[ci[p]], x1 = x[ci[p + G]];
+        const double ar0 = static_cast<double>(a0.x), ai0 = sg * static_cast<double>(a0.y);
+        const double ar1 = static_cast<double>(a1.x), ai1 = sg * static_cast<double>(a1.y);
+        sr += ar0 * static_cast<double>(x0.x) - ai0 * static_cast<double>(x0.y);
+        si += ar0 * static_cast<double>(x0.y) + ai0 * static_cast<double>(x0.x);
+        sr += ar1 * static_cast<double>(x1.x) - ai1 * static_cast<double>(x1.y);
+        si += ar1 * static_cast<double>(x1.y) + ai1 * static_cast<double>(x1.x);
+      }
+      if (p < p1) {
         const C2 a = va[p];
         const C2 xv = x[ci[p]];
         const double ar = static_cast<double>(a.x), ai = sg * static_cast<double>(a.y);
-        const double xr = static_cast<double>(xv.x), xi = static_cast<double>(xv.y);
-        sr += ar * xr - ai * xi;
-        si += ar * xi + ai * xr;
+        sr += ar * static_cast<double>(xv.x) - ai * static_cast<double>(xv.y);
+        si += ar * static_cast<double>(xv.y) + ai * static_cast<double>(xv.x);
       }
     }
     sr = group_sum<G>(sr);
     si = group_sum<G>(si);
-    if (lane == 0 && row < r1) {
+    if (lane == 0 && row < rows) {
       double yr = sr, yi = si, pr = 0.0, pi = 0.0;
       if (yprev) {
         const C2 pv = yprev[row];
@@ -259,12 +270,20 @@ __global__ __launch_bounds__(kBlock) void k_zgemv_update(long long L,
 
 int zstream_blocks(long long L) { return L > 0 ? cdiv(L, kZRows) : 1; }
 
-// Lanes per row of the complex SpMV: the power of two nearest the mean row
-// length (one nonzero per lane), clamped to [2, 64].
+// Workgroups of the complex SpMV: one per chunk of kBlock / g rows, at most
+// kMaxBlocks (eight per CU: a full wave complement), the chunks handed out
+// grid-stride.
+int zspmv_blocks(long long rows, int g) {
+  const long long chunks = rows > 0 ? (rows + kBlock / g - 1) / (kBlock / g) : 1;
+  return static_cast<int>(std::min<long long>(chunks, kMaxBlocks));
+}
+
+// Lanes per row of the complex SpMV: the power of two nearest half the mean
+// row length (two nonzeros per lane), clamped to [2, 64].
 int zspmv_lanes(long long nnz, long long rows) {
   const double avg = rows > 0 ? static_cast<double>(nnz) / static_cast<double>(rows) : 1.0;
   int g = 2;
-  while (g < 64 && g * 1.5 < avg) g *= 2;
+  while (g < 64 && g * 3.0 < avg) g *= 2;
   return g;
 }
 
@@ -273,7 +292,7 @@ int launch_zspmv(hipStream_t s, long long rows, const int* rp, const int* ci, co
                  int conj, int g, const void* x, void* y, const void* yprev, double c,
                  double* part) {
   using C2 = typename V2T<R>::type;
-  const int nblk = zstream_blocks(rows);
+  const int nblk = zspmv_blocks(rows, g);
 #define PK_ZSPMV(G)                                                                            \
   hipLaunchKernelGGL((k_zspmv<R, G>), dim3(nblk), dim3(kBlock), 0, s, rows, rp, ci,             \
                      static_cast<const C2*>(va), conj, static_cast<const C2*>(x),               \

@@ -878,7 +878,7 @@ int zspmv_any(CsrOp* op, char trans, const void* x, void* y, int where) {
   void *dx = nullptr, *dy = nullptr;
   double* part = nullptr;
   int rc = PK_OK;
-  const size_t npart = 3 * static_cast<size_t>(zstream_blocks(std::max(op->m, op->n)));
+  const size_t npart = 3 * static_cast<size_t>(kMaxBlocks);  // zspmv_blocks <= kMaxBlocks
   if (hipMalloc(&part, sizeof(double) * npart) != hipSuccess) rc = PK_ERR_OOM;
   if (rc == PK_OK && where == PK_HOST) {
     if (hipMalloc(&dx, eb * std::max<long long>(rin, 1)) != hipSuccess ||

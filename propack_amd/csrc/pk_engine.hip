@@ -768,6 +768,12 @@ struct ZImpl {
   double eff_length() const {
     return e->rounding ? 4.0 : static_cast<double>(std::max(e->m, e->n));
   }
+  // live kernel timing: the real engine's event helpers (they touch only the
+  // engine's stream and event pool)
+  hipEvent_t tstart(int cls) { return Impl<R>(e).tstart(cls); }
+  void tstop(hipEvent_t a, int cls, double bytes, double flops) {
+    Impl<R>(e).tstop(a, cls, bytes, flops);
+  }
 
   // ||x|| (pdznrm2, complex16/zblasext.F:6-30)
   int nrm2(const T* x, long long L, double* out) {
@@ -800,8 +806,16 @@ struct ZImpl {
     const long long rows_out = tr ? e->n : e->m, rows_in = tr ? e->m : e->n;
     int nb;
     if (e->op) {
+      // algorithmic bytes: every nonzero and index once, row pointers, x and
+      // y once, yprev once
+      const double sb = static_cast<double>(sizeof(T)), nnz = static_cast<double>(e->op->nnz);
+      const double bytes = nnz * (static_cast<double>(e->op->val_bytes) + 4.0) +
+                           (rows_out + 1) * 4.0 + rows_in * sb + rows_out * sb +
+                           (yprev ? rows_out * sb : 0.0);
+      hipEvent_t t0 = tstart(tr ? KC_SPMV_T : KC_SPMV_N);
       nb = zcsr_apply<R>(e->op, strm, tr ? 'c' : 'n', x, y, yprev, c, e->part);
       PK_LAUNCH(nb);
+      tstop(t0, tr ? KC_SPMV_T : KC_SPMV_N, bytes, 8.0 * nnz);
     } else if (e->aprod) {
       PK_TRY(host_apply(tr ? 'c' : 'n', x, y, rows_in, rows_out));
       if (!yprev) {
@@ -843,12 +857,17 @@ struct ZImpl {
   // (zcgs, complex16/zreorth.F:120-205): h = basis^H vnew, vnew -= basis h;
   // returns ||vnew'||^2.
   int cgs_pass(long long L, const T* basis, long long ld, T* vnew, int ncols, double* s2) {
+    const double sb = static_cast<double>(sizeof(T));
+    hipEvent_t t0 = tstart(KC_GEMVT);
     const int nb = launch_zgemvt<R>(strm, L, basis, ld, e->dcols, ncols, vnew, e->part);
     PK_LAUNCH(nb);
+    tstop(t0, KC_GEMVT, (ncols + 1.0) * L * sb, 8.0 * ncols * L);
     PK_LAUNCH(launch_reduce(strm, 2 * ncols, nb, e->part, e->red, Gate{}));
+    hipEvent_t t1 = tstart(KC_GEMV);
     const int nb2 = launch_zgemv_update<R>(strm, L, basis, ld, e->dcols, ncols, e->red, vnew,
                                            e->part);
     PK_LAUNCH(nb2);
+    tstop(t1, KC_GEMV, (ncols + 2.0) * L * sb, 8.0 * ncols * L);
     PK_LAUNCH(launch_reduce(strm, 1, nb2, e->part, hsum(), Gate{}));
     PK_TRY(read(2 * e->ccap, 1));
     *s2 = e->hred[2 * e->ccap];
@@ -1007,8 +1026,11 @@ struct ZImpl {
   // Restart / Ritz GEMM on the real view: a complex column of length L is a
   // real column of length 2L and B is real (complex16/zgemm_ovwr.F:6-39).
   int gemm(long long L, const void* A, long long ld, int kdim, int ncols, void* C) {
+    hipEvent_t t0 = tstart(KC_GEMM);
     PK_LAUNCH(launch_gemm_mfma<R>(strm, 2 * L, static_cast<const R*>(A), 2 * ld, kdim, e->dB,
                                   kdim, ncols, static_cast<R*>(C), 2 * ld));
+    tstop(t0, KC_GEMM, static_cast<double>(sizeof(T)) * L * (kdim + ncols),
+          4.0 * static_cast<double>(L) * kdim * ncols);
     return PK_OK;
   }
   int update(char which, char transb, int kdim, int ncols, const double* B, int ldb) {

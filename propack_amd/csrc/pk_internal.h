@@ -141,6 +141,7 @@ inline size_t prec_bytes(int prec) {
 }
 int zstream_blocks(long long L);
 int zspmv_lanes(long long nnz, long long rows);
+int zspmv_blocks(long long rows, int g);  // <= kMaxBlocks
 // y = op(A) x - c yprev (trans 'n', 't' or 'c' = A^H) with partials
 // ||y||^2, Re and Im of yprev^H y (three values per slot); returns the slot
 // count.  R is the real type of the storage precision.
