@@ -7,8 +7,9 @@ A^T resident in HBM; the drivers recognise the library's own APROD and keep
 the whole solve on the GPU).  A ``scipy.sparse.linalg.LinearOperator`` is
 called from the host as the APROD callback (matvec / rmatvec), the rest of
 the Lanczos process still runs on the GPU.  float32 inputs use SLANSVD /
-SLANSVD_IRL, everything else DLANSVD / DLANSVD_IRL.  Complex inputs are
-not supported.
+SLANSVD_IRL, complex128 ZLANSVD / ZLANSVD_IRL, complex64 CLANSVD /
+CLANSVD_IRL, everything else DLANSVD / DLANSVD_IRL; for complex inputs the
+returned ``vt`` is V^H.
 """
 from __future__ import annotations
 
@@ -29,11 +30,11 @@ def _as_operator(A, prec, tune=False):
         from scipy.sparse.linalg import LinearOperator
     except Exception:  # pragma: no cover - scipy is optional for explicit matrices
         sp, LinearOperator = None, None
-    dt = np.float64 if prec == api.PK_F64 else np.float32
+    dt = api._dt(prec)
     if LinearOperator is not None and isinstance(A, LinearOperator):
         m, n = A.shape
 
-        def aprod(t, x):
+        def aprod(t, x):  # t = 't' (real) or 'c' (complex): rmatvec is A^H x
             return A.matvec(x) if t == "n" else A.rmatvec(x)
         return aprod, m, n, None
     if sp is not None and sp.issparse(A):
@@ -48,7 +49,7 @@ def _as_operator(A, prec, tune=False):
             raise ValueError("A must be two-dimensional")
         r, c = np.nonzero(D)
         csr = synth.from_coo(D.shape[0], D.shape[1], r, c, D[r, c].astype(dt))
-    if tune:
+    if tune and not api.is_complex(prec):
         t = api.autotune(csr, prec=prec)
         op = api.CsrOperator(csr, prec=prec, panels=t.panels, lanes=t.lanes, sort=t.sort)
     else:
@@ -59,7 +60,7 @@ def _as_operator(A, prec, tune=False):
 def svdp(A, k, which="LM", irl_mode=True, kmax=None, compute_u=True, compute_v=True,
          v0=None, full_output=False, tol=0, delta=None, eta=None, anorm=0, cgs=False,
          elr=True, min_relgap=0.002, shifts=None, maxiter=None, rng=None, *, tune=False):
-    """Truncated SVD of A through the library's DLANSVD(_IRL) drop-ins.
+    """Truncated SVD of A through the library's xLANSVD(_IRL) drop-ins.
 
     Returns ``(u, sigma, vt)`` or ``(u, sigma, vt, bnd)`` with
     ``full_output`` -- the layout of scipy's ``_svdp``.  ``tune`` (not in
@@ -73,10 +74,14 @@ def svdp(A, k, which="LM", irl_mode=True, kmax=None, compute_u=True, compute_v=T
         raise ValueError("`which` must be either 'LM' or 'SM'")
     if not irl_mode and which == "SM":
         raise ValueError("`which`='SM' requires irl_mode=True")
-    dtype = getattr(A, "dtype", np.float64)
-    if np.issubdtype(np.dtype(dtype), np.complexfloating):
-        raise NotImplementedError("complex matrices are not supported")
-    prec = api.PK_F32 if np.dtype(dtype) == np.float32 else api.PK_F64
+    dtype = np.dtype(getattr(A, "dtype", np.float64))
+    if dtype in (np.float32, np.complex64, np.complex128):
+        prec = api.prec_of(dtype)
+    elif np.issubdtype(dtype, np.complexfloating):
+        prec = api.PK_C128
+    else:
+        prec = api.PK_F64
+    cplx = api.is_complex(prec)
     aprod, m, n, keep = _as_operator(A, prec, tune)
     if k < 1 or k > min(m, n):
         raise ValueError("k must be positive and not greater than m or n")
@@ -91,11 +96,13 @@ def svdp(A, k, which="LM", irl_mode=True, kmax=None, compute_u=True, compute_v=T
         if rng is None:
             rng = np.random.default_rng()
         u0 = rng.uniform(size=m)
+        if cplx:
+            u0 = u0 + 1j * rng.uniform(size=m)
     else:
-        u0 = np.asarray(v0, dtype=np.float64)
+        u0 = np.asarray(v0, dtype=np.complex128 if cplx else np.float64)
         if u0.shape != (m,):
             raise ValueError(f"v0 must be of length {m}")
-    eps = float(np.finfo(np.float64 if prec == api.PK_F64 else np.float32).eps)
+    eps = float(np.finfo(api._rt(prec)).eps)
     delta = np.sqrt(eps) if delta is None else delta
     eta = eps ** 0.75 if eta is None else eta
     jobu = "y" if compute_u else "n"
@@ -122,7 +129,7 @@ def svdp(A, k, which="LM", irl_mode=True, kmax=None, compute_u=True, compute_v=T
         raise LinAlgError(f"k={k} singular triplets did not converge within kmax={kmax} "
                           "iterations")
     u = r.U if compute_u else None
-    vt = r.V.T.copy() if compute_v else None
+    vt = (r.V.conj().T.copy() if cplx else r.V.T.copy()) if compute_v else None
     if full_output:
         return u, r.sigma, vt, r.bnd
     return u, r.sigma, vt

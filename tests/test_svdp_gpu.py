@@ -13,23 +13,28 @@ from propack_amd.svdp import svdp
 
 pytestmark = pytest.mark.gpu
 
-TOLS = {np.float32: 1e-4, np.float64: 1e-8}
+TOLS = {np.float32: 1e-4, np.float64: 1e-8, np.complex64: 1e-4, np.complex128: 1e-8}
 
 
 def generate_matrix(n, m, f, dtype, rseed=0):
+    """Uniform entries in [-5, 5), a fraction 1 - f of them zeroed; complex
+    types add an imaginary part drawn the same way."""
     rng = np.random.RandomState(rseed)
-    M = (-5 + 10 * rng.rand(n, m)).astype(dtype)
-    M[M > 10 * f - 5] = 0
+    M = -5 + 10 * rng.rand(n, m)
+    if np.issubdtype(np.dtype(dtype), np.complexfloating):
+        M = M + 1j * (-5 + 10 * rng.rand(n, m))
+    M = M.astype(dtype)
+    M[M.real > 10 * f - 5] = 0
     return M
 
 
 def assert_orthogonal(u1, u2, rtol, atol):
-    A = abs(np.dot(u1.T, u2))
+    A = abs(np.dot(u1.conj().T, u2))
     assert_allclose(A, np.eye(u1.shape[1], u2.shape[1]), rtol=rtol, atol=atol)
 
 
 @pytest.mark.parametrize("ctor", (np.array, csr_array, csc_array))
-@pytest.mark.parametrize("dtype", (np.float32, np.float64))
+@pytest.mark.parametrize("dtype", (np.float32, np.float64, np.complex64, np.complex128))
 @pytest.mark.parametrize("irl", (True, False))
 @pytest.mark.parametrize("which", ("LM", "SM"))
 def test_svdp(ctor, dtype, irl, which):
@@ -40,7 +45,8 @@ def test_svdp(ctor, dtype, irl, which):
         with pytest.raises(ValueError, match="requires irl_mode=True"):
             svdp(ctor(M), k=k, which=which, irl_mode=irl, tol=tol)
         return
-    u1, s1, vt1 = np.linalg.svd(M.astype(np.float64), full_matrices=False)
+    wide = np.complex128 if np.iscomplexobj(M) else np.float64
+    u1, s1, vt1 = np.linalg.svd(M.astype(wide), full_matrices=False)
     u2, s2, vt2 = svdp(ctor(M), k=k, which=which, irl_mode=irl, tol=tol,
                        rng=np.random.default_rng(0))
     if which == "SM":
@@ -48,8 +54,9 @@ def test_svdp(ctor, dtype, irl, which):
         vt1 = np.roll(vt1, k, 0)
         s1 = np.roll(s1, k)
     assert_allclose(s1[:k], s2, rtol=tol, atol=tol)
-    assert_orthogonal(u1, u2.astype(np.float64), rtol=tol, atol=tol)
-    assert_orthogonal(vt1.T, vt2.T.astype(np.float64), rtol=tol, atol=tol)
+    assert_orthogonal(u1, u2.astype(wide), rtol=tol, atol=tol)
+    # vt = V^H for complex inputs, as numpy's
+    assert_orthogonal(vt1.conj().T, vt2.conj().T.astype(wide), rtol=tol, atol=tol)
 
 
 def test_linear_operator_callback():
@@ -85,5 +92,11 @@ def test_argument_errors():
         svdp(M, k=3, which="XX")
     with pytest.raises(ValueError):
         svdp(M, k=3, v0=np.ones(5))
-    with pytest.raises(NotImplementedError):
-        svdp(M.astype(complex), k=3)
+
+
+def test_complex_linear_operator_callback():
+    """A complex LinearOperator: rmatvec supplies A^H x for ZLANSVD's 'c'."""
+    M = generate_matrix(30, 50, 0.5, np.complex128, rseed=3)
+    u, s, vt = svdp(aslinearoperator(M), k=4, tol=1e-10, rng=np.random.default_rng(1))
+    assert_allclose(s, np.linalg.svd(M, compute_uv=False)[:4], rtol=1e-10)
+    assert_allclose(M @ vt.conj().T, u * s, atol=1e-8 * s[0])
