@@ -17,6 +17,8 @@ def build():
     cmds = [
         [FLANG, "-O2", os.path.join(HERE, "fortran", "dropin_example.f90"), "-o",
          os.path.join(OUT, "dropin_example_f"), "-module-dir", OUT, *link],
+        [FLANG, "-O2", os.path.join(HERE, "fortran", "zdropin_example.f90"), "-o",
+         os.path.join(OUT, "zdropin_example_f"), "-module-dir", OUT, *link],
         ["gcc", "-O2", "-std=c11", "-I" + os.path.join(ROOT, "include"),
          os.path.join(HERE, "c", "dropin_example.c"), "-o", os.path.join(OUT, "dropin_example_c"),
          *link, "-lm"],

@@ -1,5 +1,6 @@
 """Compiled callers of the drop-in drivers (tests/fortran/dropin_example.f90,
-tests/c/dropin_example.c, built by tests/build_programs.py): the
+tests/fortran/zdropin_example.f90, tests/c/dropin_example.c, built by
+tests/build_programs.py): the
 reference's calling sequences linked against libpropack_amd, with a host
 APROD and with the device operator.  Their INFO and converged counts equal
 the compiled reference's on the same matrix and start vector, and their
@@ -68,3 +69,28 @@ def test_c_caller():
     np.testing.assert_allclose(out["HOST_SIGMA"], s[:6], rtol=1e-11)
     assert out["DEV_INFO_K"][:2] == [0, 6]
     np.testing.assert_allclose(out["DEV_SIGMA"], s[:6], rtol=2e-5)
+
+
+def test_fortran_complex_caller():
+    """ZLANSVD with the caller's complex host APROD and ZLANSVD_IRL with the
+    device operator (pk_zcsr_aprod), against the dense complex SVD and the
+    compiled ZLANSVD."""
+    m, n, per = 260, 190, 5
+    D = np.zeros((m, n), np.complex128)
+    for i in range(1, m + 1):
+        for t in range(1, per + 1):
+            D[i - 1, (i * 13 + t * 37 + t * t) % n] += complex(
+                np.sin(0.29 * i + 0.9 * t) + 0.1 * t, np.cos(0.41 * i - 0.6 * t))
+    s = np.linalg.svd(D, compute_uv=False)
+    out = _run("zdropin_example_f")
+    r, c = np.nonzero(D)
+    A = synth.from_coo(m, n, r, c, D[r, c])
+    u0 = np.cos(0.11 * np.arange(1, m + 1)) + 1j * np.sin(0.07 * np.arange(1, m + 1))
+    refpy.set_threads(1, "z")
+    ref = refpy.lansvd(A, 6, 190, u0=u0, tol=1e-12, cgs=0, prec="z")
+    assert out["HOST_INFO_K"] == [ref.info, ref.k] == [0, 6]
+    np.testing.assert_allclose(out["HOST_SIGMA"], s[:6], rtol=1e-11)
+    np.testing.assert_allclose(out["HOST_SIGMA"], ref.sigma, rtol=1e-11)
+    assert out["DEV_INFO_K"][:2] == [0, 6]
+    np.testing.assert_allclose(out["DEV_SIGMA"], s[:6], rtol=1e-10)
+    assert out["DEV_RESID1"][0] < 1e-10
