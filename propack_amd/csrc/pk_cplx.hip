@@ -35,7 +35,7 @@ namespace {
 
 constexpr int kZRPT = 4;                   // rows per thread of the streaming kernels
 constexpr int kZRows = kBlock * kZRPT;     // rows per workgroup slice
-constexpr int kZCols = 8;                  // basis columns per pass of the projection
+constexpr int kZCols = 4;                  // basis columns per pass of the Gram-Schmidt kernels
 
 template <typename R>
 struct V2T;
@@ -164,6 +164,9 @@ __global__ __launch_bounds__(kBlock) void k_zaxpy_norm_dot(long long L,
 // Slice b of h = V(:, cols)^H w: the slice of w stays in registers while
 // the columns stream past in groups of kZCols; partials of column j at
 // part[(2j) * nblk + b] (real) and part[(2j + 1) * nblk + b] (imaginary).
+// Every load is unconditional: rows past L read row L-1 against a zero w,
+// columns past ncols read the last listed column and are not stored, so a
+// group's loads are all in flight before its sums.
 template <typename R>
 __global__ __launch_bounds__(kBlock) void k_zgemvt(long long L,
                                                    const typename V2T<R>::type* __restrict__ V,
@@ -173,36 +176,40 @@ __global__ __launch_bounds__(kBlock) void k_zgemvt(long long L,
                                                    double* __restrict__ part) {
   using C2 = typename V2T<R>::type;
   __shared__ double lds[(kBlock / kWave) * 2 * kZCols];
+  const size_t nb = gridDim.x;
+  if (L <= 0) {  // empty vectors: zero projections, no loads
+    for (int j = threadIdx.x; j < 2 * ncols; j += kBlock) part[j * nb + blockIdx.x] = 0.0;
+    return;
+  }
   const long long base = static_cast<long long>(blockIdx.x) * kZRows + threadIdx.x;
   double wr[kZRPT], wi[kZRPT];
+  long long rowc[kZRPT];
 #pragma unroll
   for (int r = 0; r < kZRPT; ++r) {
     const long long row = base + static_cast<long long>(r) * kBlock;
-    wr[r] = wi[r] = 0.0;
-    if (row < L) {
-      const C2 wv = w[row];
-      wr[r] = static_cast<double>(wv.x);
-      wi[r] = static_cast<double>(wv.y);
-    }
+    rowc[r] = row < L ? row : L - 1;
+    const C2 wv = w[rowc[r]];
+    wr[r] = row < L ? static_cast<double>(wv.x) : 0.0;
+    wi[r] = row < L ? static_cast<double>(wv.y) : 0.0;
   }
-  const size_t nb = gridDim.x;
   for (int j0 = 0; j0 < ncols; j0 += kZCols) {
+    C2 a[kZCols][kZRPT];
+#pragma unroll
+    for (int q = 0; q < kZCols; ++q) {
+      const int jq = j0 + q < ncols ? j0 + q : ncols - 1;
+      const C2* col = V + static_cast<long long>(cols[jq]) * ldv;
+#pragma unroll
+      for (int r = 0; r < kZRPT; ++r) a[q][r] = col[rowc[r]];
+    }
     double acc[2 * kZCols];
 #pragma unroll
     for (int q = 0; q < kZCols; ++q) {
       acc[2 * q] = acc[2 * q + 1] = 0.0;
-      if (j0 + q < ncols) {
-        const C2* col = V + static_cast<long long>(cols[j0 + q]) * ldv;
 #pragma unroll
-        for (int r = 0; r < kZRPT; ++r) {
-          const long long row = base + static_cast<long long>(r) * kBlock;
-          if (row < L) {
-            const C2 a = col[row];
-            const double ar = static_cast<double>(a.x), ai = static_cast<double>(a.y);
-            acc[2 * q] += ar * wr[r] + ai * wi[r];      // conj(a) * w
-            acc[2 * q + 1] += ar * wi[r] - ai * wr[r];
-          }
-        }
+      for (int r = 0; r < kZRPT; ++r) {
+        const double ar = static_cast<double>(a[q][r].x), ai = static_cast<double>(a[q][r].y);
+        acc[2 * q] += ar * wr[r] + ai * wi[r];  // conj(a) * w
+        acc[2 * q + 1] += ar * wi[r] - ai * wr[r];
       }
     }
     block_sum<2 * kZCols>(acc, lds);
@@ -219,7 +226,9 @@ __global__ __launch_bounds__(kBlock) void k_zgemvt(long long L,
 
 // w -= V(:, cols) h with h[2j], h[2j+1] the real and imaginary parts of h_j;
 // the update of a row is summed in FP64 over the columns in order, then
-// subtracted and rounded once.  Partial: ||w||^2.
+// subtracted and rounded once.  Columns go kZCols at a time with their loads
+// unconditional (past ncols: the last listed column with a zero weight;
+// past L: row L-1, not stored).  Partial: ||w||^2.
 template <typename R>
 __global__ __launch_bounds__(kBlock) void k_zgemv_update(long long L,
                                                          const typename V2T<R>::type* __restrict__ V,
@@ -230,23 +239,40 @@ __global__ __launch_bounds__(kBlock) void k_zgemv_update(long long L,
                                                          double* __restrict__ part) {
   using C2 = typename V2T<R>::type;
   __shared__ double lds[kBlock / kWave];
+  if (L <= 0) {
+    if (threadIdx.x == 0) part[blockIdx.x] = 0.0;
+    return;
+  }
   const long long base = static_cast<long long>(blockIdx.x) * kZRows + threadIdx.x;
+  long long rowc[kZRPT];
   double ur[kZRPT], ui[kZRPT];
 #pragma unroll
-  for (int r = 0; r < kZRPT; ++r) ur[r] = ui[r] = 0.0;
-  for (int j = 0; j < ncols; ++j) {
-    const C2* col = V + static_cast<long long>(cols[j]) * ldv;
-    const double hr = h[2 * j], hi = h[2 * j + 1];
+  for (int r = 0; r < kZRPT; ++r) {
+    const long long row = base + static_cast<long long>(r) * kBlock;
+    rowc[r] = row < L ? row : L - 1;
+    ur[r] = ui[r] = 0.0;
+  }
+  for (int j0 = 0; j0 < ncols; j0 += kZCols) {
+    C2 a[kZCols][kZRPT];
+    double hr[kZCols], hi[kZCols];
 #pragma unroll
-    for (int r = 0; r < kZRPT; ++r) {
-      const long long row = base + static_cast<long long>(r) * kBlock;
-      if (row < L) {
-        const C2 a = col[row];
-        const double ar = static_cast<double>(a.x), ai = static_cast<double>(a.y);
-        ur[r] += ar * hr - ai * hi;
-        ui[r] += ar * hi + ai * hr;
-      }
+    for (int q = 0; q < kZCols; ++q) {
+      const bool in = j0 + q < ncols;
+      const int jq = in ? j0 + q : ncols - 1;
+      const C2* col = V + static_cast<long long>(cols[jq]) * ldv;
+      hr[q] = in ? h[2 * jq] : 0.0;
+      hi[q] = in ? h[2 * jq + 1] : 0.0;
+#pragma unroll
+      for (int r = 0; r < kZRPT; ++r) a[q][r] = col[rowc[r]];
     }
+#pragma unroll
+    for (int q = 0; q < kZCols; ++q)
+#pragma unroll
+      for (int r = 0; r < kZRPT; ++r) {
+        const double ar = static_cast<double>(a[q][r].x), ai = static_cast<double>(a[q][r].y);
+        ur[r] += ar * hr[q] - ai * hi[q];
+        ui[r] += ar * hi[q] + ai * hr[q];
+      }
   }
   double acc[1] = {0.0};
 #pragma unroll
