@@ -56,8 +56,10 @@ const char* pk_last_error(void);
  * values are double (PK_F64) or float (PK_F32), or complex (PK_C128 /
  * PK_C64, (re, im) pairs) for the complex drivers, whose APROD applies
  * A^H for 'c' (complex16/zlanbpro.F:297).  A^T is built on the device at
- * creation.  Complex operators have one layout: the calls that choose
- * panels, lanes or sorted layouts, pk_csr_autotune and
+ * creation.  Complex operators take column panels (pk_csr_create_panels,
+ * at most 8 per direction; by default as many as keep a panel's slice of x
+ * within 2 MiB when rows hold 8 or more nonzeros on average), every panel
+ * in one launch; pk_csr_set_lanes, pk_csr_set_sort, pk_csr_autotune and
  * pk_csr_spmv_noalloc return PK_ERR_NOTSUP for them.                      */
 int pk_csr_create(int m, int n, long long nnz, const int* rowptr, const int* colind,
                   const void* val, int val_prec, int where, pk_csr_t* op);

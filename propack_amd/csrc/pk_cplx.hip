@@ -16,6 +16,11 @@
 //                    the explicit transpose A^T read with conjugated values),
 //                    one group of G lanes per row, grid-stride over row
 //                    chunks; partials ||y||^2, Re / Im of yprev^H y
+//   k_zspmv_cp       the same over column panels, every panel in one launch:
+//                    each panel's FP64 row sums to an array of its own, the
+//                    workgroups of a panel on the XCDs that hold its slice
+//                    of x; k_zcombine adds the panels' sums in panel order
+//                    and applies the epilogue
 //   k_zaxpy_norm_dot w -= s v for complex s; partials ||w||^2, Re / Im v^H w
 //   k_zgemvt         h_j = V(:, cols[j])^H w, two partials per column
 //   k_zgemv_update   w -= V(:, cols) h; partial ||w||^2
@@ -113,6 +118,112 @@ __global__ __launch_bounds__(kBlock) void k_zspmv(long long rows, const int* __r
       const double qr = static_cast<double>(out.x), qi = static_cast<double>(out.y);
       acc[0] += qr * qr + qi * qi;
       acc[1] += pr * qr + pi * qi;  // conj(yprev) * y
+      acc[2] += pr * qi - pi * qr;
+    }
+  }
+  block_sum<3>(acc, lds);
+  if (threadIdx.x == 0) {
+    part[blockIdx.x] = acc[0];
+    part[gridDim.x + blockIdx.x] = acc[1];
+    part[2 * gridDim.x + blockIdx.x] = acc[2];
+  }
+}
+
+// Every column panel in one launch.  Workgroup w runs panel w mod np (the
+// dispatcher sends workgroup w to XCD w mod 8, so with np = 8 each XCD's L2
+// holds one panel's slice of x, with np = 4 two XCDs share one), taking that
+// panel's row chunks w / np, w / np + nblk / np, ...; the panel's FP64 row
+// sums go to ypart[p * rows + row] as (re, im).
+struct ZPanels {
+  const int* rp[kMaxZPanels];
+  const int* ci[kMaxZPanels];
+  const void* va[kMaxZPanels];
+};
+
+template <typename R, int G>
+__global__ __launch_bounds__(kBlock) void k_zspmv_cp(long long rows, int np, ZPanels tab,
+                                                     int conj,
+                                                     const typename V2T<R>::type* __restrict__ x,
+                                                     double2* __restrict__ ypart) {
+  using C2 = typename V2T<R>::type;
+  constexpr int ngrp = kBlock / G;
+  const int lane = threadIdx.x % G, grp = threadIdx.x / G;
+  const int pn = static_cast<int>(blockIdx.x % np);
+  const int* __restrict__ rp = tab.rp[pn];
+  const int* __restrict__ ci = tab.ci[pn];
+  const C2* __restrict__ va = static_cast<const C2*>(tab.va[pn]);
+  double2* __restrict__ yp = ypart + static_cast<long long>(pn) * rows;
+  const double sg = conj ? -1.0 : 1.0;
+  const long long stride = static_cast<long long>(gridDim.x / np) * ngrp;
+  for (long long base = static_cast<long long>(blockIdx.x / np) * ngrp; base < rows;
+       base += stride) {
+    const long long row = base + grp;
+    double sr = 0.0, si = 0.0;
+    if (row < rows) {
+      const int p1 = rp[row + 1];
+      int p = rp[row] + lane;
+      for (; p + G < p1; p += 2 * G) {
+        const C2 a0 = va[p], a1 = va[p + G];
+        const C2 x0 = x[ci[p]], x1 = x[ci[p + G]];
+        const double ar0 = static_cast<double>(a0.x), ai0 = sg * static_cast<double>(a0.y);
+        const double ar1 = static_cast<double>(a1.x), ai1 = sg * static_cast<double>(a1.y);
+        sr += ar0 * static_cast<double>(x0.x) - ai0 * static_cast<double>(x0.y);
+        si += ar0 * static_cast<double>(x0.y) + ai0 * static_cast<double>(x0.x);
+        sr += ar1 * static_cast<double>(x1.x) - ai1 * static_cast<double>(x1.y);
+        si += ar1 * static_cast<double>(x1.y) + ai1 * static_cast<double>(x1.x);
+      }
+      if (p < p1) {
+        const C2 a = va[p];
+        const C2 xv = x[ci[p]];
+        const double ar = static_cast<double>(a.x), ai = sg * static_cast<double>(a.y);
+        sr += ar * static_cast<double>(xv.x) - ai * static_cast<double>(xv.y);
+        si += ar * static_cast<double>(xv.y) + ai * static_cast<double>(xv.x);
+      }
+    }
+    sr = group_sum<G>(sr);
+    si = group_sum<G>(si);
+    if (lane == 0 && row < rows) yp[row] = make_double2(sr, si);
+  }
+}
+
+// y = T(sum over panels, in panel order, of ypart - c yprev); partials
+// ||y||^2, Re / Im of yprev^H y.
+template <typename R>
+__global__ __launch_bounds__(kBlock) void k_zcombine(long long rows, int np,
+                                                     const double2* __restrict__ ypart,
+                                                     const typename V2T<R>::type* __restrict__ yprev,
+                                                     double c,
+                                                     typename V2T<R>::type* __restrict__ y,
+                                                     double* __restrict__ part) {
+  using C2 = typename V2T<R>::type;
+  __shared__ double lds[(kBlock / kWave) * 3];
+  const long long base = static_cast<long long>(blockIdx.x) * kZRows + threadIdx.x;
+  double acc[3] = {0.0, 0.0, 0.0};
+#pragma unroll
+  for (int r = 0; r < kZRPT; ++r) {
+    const long long row = base + static_cast<long long>(r) * kBlock;
+    if (row < rows) {
+      double2 sum = ypart[row];
+      for (int q = 1; q < np; ++q) {
+        const double2 t = ypart[static_cast<long long>(q) * rows + row];
+        sum.x += t.x;
+        sum.y += t.y;
+      }
+      double yr = sum.x, yi = sum.y, pr = 0.0, pi = 0.0;
+      if (yprev) {
+        const C2 pv = yprev[row];
+        pr = static_cast<double>(pv.x);
+        pi = static_cast<double>(pv.y);
+        yr = sum.x - c * pr;
+        yi = sum.y - c * pi;
+      }
+      C2 out;
+      out.x = static_cast<R>(yr);
+      out.y = static_cast<R>(yi);
+      y[row] = out;
+      const double qr = static_cast<double>(out.x), qi = static_cast<double>(out.y);
+      acc[0] += qr * qr + qi * qi;
+      acc[1] += pr * qr + pi * qi;
       acc[2] += pr * qi - pi * qr;
     }
   }
@@ -365,16 +476,63 @@ int launch_zgemv_update(hipStream_t s, long long L, const void* V, long long ldv
   return hipGetLastError() == hipSuccess ? nblk : -1;
 }
 
+long long zcsr_ywork(const CsrOp* op) {
+  long long w = 0;
+  for (const CsrOp::Dir* D : {&op->da, &op->dt})
+    if (D->np > 1) w = std::max(w, 2LL * D->np * D->rows);
+  return w;
+}
+
+template <typename R>
+static int launch_zspmv_cp(hipStream_t s, const CsrOp::Dir& D, int conj, const void* x,
+                           void* y, const void* yprev, double c, double* part, double* ywork) {
+  using C2 = typename V2T<R>::type;
+  ZPanels tab{};
+  for (int p = 0; p < D.np; ++p) {
+    tab.rp[p] = D.pan[p].rp;
+    tab.ci[p] = D.pan[p].ci;
+    tab.va[p] = D.pan[p].va;
+  }
+  const int g = D.pan[0].g, np = D.np;
+  const long long rows = D.rows;
+  // per panel as many workgroups as the one-panel kernel would use, the
+  // grid a multiple of np
+  const int per = std::max(1, std::min(zspmv_blocks(rows, g), kMaxBlocks / np));
+  const int grid = per * np;
+  double2* yp = reinterpret_cast<double2*>(ywork);
+#define PK_ZSPMV_CP(G)                                                                       \
+  hipLaunchKernelGGL((k_zspmv_cp<R, G>), dim3(grid), dim3(kBlock), 0, s, rows, np, tab, conj, \
+                     static_cast<const C2*>(x), yp)
+  switch (g) {
+    case 2: PK_ZSPMV_CP(2); break;
+    case 4: PK_ZSPMV_CP(4); break;
+    case 8: PK_ZSPMV_CP(8); break;
+    case 16: PK_ZSPMV_CP(16); break;
+    case 32: PK_ZSPMV_CP(32); break;
+    default: PK_ZSPMV_CP(64); break;
+  }
+#undef PK_ZSPMV_CP
+  if (hipGetLastError() != hipSuccess) return -1;
+  const int nblk = zstream_blocks(rows);
+  hipLaunchKernelGGL(k_zcombine<R>, dim3(nblk), dim3(kBlock), 0, s, rows, np, yp,
+                     static_cast<const C2*>(yprev), c, static_cast<C2*>(y), part);
+  return hipGetLastError() == hipSuccess ? nblk : -1;
+}
+
 // y = op(A) x - c * yprev for a complex operator (trans 'n': A; 't': A^T;
 // 'c': A^H); returns the partial-slot count (three values per slot).
 template <typename R>
 int zcsr_apply(const CsrOp* op, hipStream_t s, char trans, const void* x, void* y,
-               const void* yprev, double c, double* part) {
+               const void* yprev, double c, double* part, double* ywork) {
   const bool tr = trans != 'n' && trans != 'N';
   const bool cj = trans == 'c' || trans == 'C';
-  const CsrOp::Panel& P = tr ? op->dt.pan[0] : op->da.pan[0];
-  const long long rows = tr ? op->n : op->m;
-  return launch_zspmv<R>(s, rows, P.rp, P.ci, P.va, cj ? 1 : 0, P.g, x, y, yprev, c, part);
+  const CsrOp::Dir& D = tr ? op->dt : op->da;
+  if (D.np > 1) {
+    if (!ywork) return -1;
+    return launch_zspmv_cp<R>(s, D, cj ? 1 : 0, x, y, yprev, c, part, ywork);
+  }
+  const CsrOp::Panel& P = D.pan[0];
+  return launch_zspmv<R>(s, D.rows, P.rp, P.ci, P.va, cj ? 1 : 0, P.g, x, y, yprev, c, part);
 }
 
 #define PK_INST_Z(R)                                                                            \
@@ -387,7 +545,7 @@ int zcsr_apply(const CsrOp* op, hipStream_t s, char trans, const void* x, void* 
   template int launch_zgemv_update<R>(hipStream_t, long long, const void*, long long,           \
                                       const int*, int, const double*, void*, double*);          \
   template int zcsr_apply<R>(const CsrOp*, hipStream_t, char, const void*, void*, const void*,  \
-                             double, double*);
+                             double, double*, double*);
 PK_INST_Z(double)
 PK_INST_Z(float)
 #undef PK_INST_Z

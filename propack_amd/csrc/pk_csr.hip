@@ -681,24 +681,30 @@ int csr_create(int m, int n, long long nnz, const int* rowptr, const int* colind
              ? build_transpose<double2>(m, n, nnz, rp, ci, va, &rpt, &cit, &vat, s)
              : build_transpose<float2>(m, n, nnz, rp, ci, va, &rpt, &cit, &vat, s);
     if (rc == PK_OK) {
-      CsrOp::Dir* dirs[2] = {&op->da, &op->dt};
-      int* rps[2] = {rp, rpt};
-      int* cis[2] = {ci, cit};
-      void* vas[2] = {va, vat};
-      for (int d = 0; d < 2; ++d) {
-        CsrOp::Dir& D = *dirs[d];
-        D.rows = d == 0 ? m : n;
-        D.cols = d == 0 ? n : m;
-        D.np = 1;
-        D.width = D.cols;
-        D.pan[0].rp = rps[d];
-        D.pan[0].ci = cis[d];
-        D.pan[0].va = vas[d];
-        D.pan[0].nnz = nnz;
-        D.pan[0].g = zspmv_lanes(nnz, D.rows);
+      // column panels as for real values (each panel's slice of x sized for
+      // one XCD's L2), at most kMaxZPanels; build_dir owns the arrays
+      auto cap_np = [](int q) { return q > kMaxZPanels ? kMaxZPanels : q; };
+      const int pa = cap_np(panels_a > 0 ? pow2_floor(panels_a)
+                                         : panels_for(false, m, n, nnz, op->val_bytes));
+      const int pt = cap_np(panels_t > 0 ? pow2_floor(panels_t)
+                                         : panels_for(true, n, m, nnz, op->val_bytes));
+      rc = val_prec == PK_C128 ? build_dir<double2>(op->da, m, n, nnz, pa, rp, ci, va, s)
+                               : build_dir<float2>(op->da, m, n, nnz, pa, rp, ci, va, s);
+      rp = ci = nullptr;
+      va = nullptr;
+      const int rc2 = val_prec == PK_C128
+                          ? build_dir<double2>(op->dt, n, m, nnz, pt, rpt, cit, vat, s)
+                          : build_dir<float2>(op->dt, n, m, nnz, pt, rpt, cit, vat, s);
+      rpt = cit = nullptr;
+      vat = nullptr;
+      if (rc == PK_OK) rc = rc2;
+      if (rc == PK_OK) {
+        // one lanes-per-row count per direction (the kernel's template
+        // parameter), from the mean row length of a panel
+        for (CsrOp::Dir* D : {&op->da, &op->dt})
+          for (int p = 0; p < D->np; ++p)
+            D->pan[p].g = zspmv_lanes(nnz / D->np, D->rows);
       }
-      rp = ci = rpt = cit = nullptr;
-      va = vat = nullptr;
     }
   } else if (rc == PK_OK) {
     rc = val_prec == PK_F64
@@ -876,10 +882,14 @@ int zspmv_any(CsrOp* op, char trans, const void* x, void* y, int where) {
   hipStream_t s;
   if (hipStreamCreateWithFlags(&s, hipStreamNonBlocking) != hipSuccess) return PK_ERR_HIP;
   void *dx = nullptr, *dy = nullptr;
-  double* part = nullptr;
+  double *part = nullptr, *yw = nullptr;
   int rc = PK_OK;
-  const size_t npart = 3 * static_cast<size_t>(kMaxBlocks);  // zspmv_blocks <= kMaxBlocks
-  if (hipMalloc(&part, sizeof(double) * npart) != hipSuccess) rc = PK_ERR_OOM;
+  // zspmv_blocks and the combine's zstream_blocks stay within kMaxBlocks slots
+  const size_t npart = 3 * static_cast<size_t>(std::max(kMaxBlocks, zstream_blocks(std::max(op->m, op->n))));
+  const long long nyw = zcsr_ywork(op);
+  if (hipMalloc(&part, sizeof(double) * npart) != hipSuccess ||
+      (nyw > 0 && hipMalloc(&yw, sizeof(double) * nyw) != hipSuccess))
+    rc = PK_ERR_OOM;
   if (rc == PK_OK && where == PK_HOST) {
     if (hipMalloc(&dx, eb * std::max<long long>(rin, 1)) != hipSuccess ||
         hipMalloc(&dy, eb * std::max<long long>(rout, 1)) != hipSuccess ||
@@ -891,8 +901,8 @@ int zspmv_any(CsrOp* op, char trans, const void* x, void* y, int where) {
   }
   if (rc == PK_OK) {
     const int nb = op->val_prec == PK_C128
-                       ? zcsr_apply<double>(op, s, trans, dx, dy, nullptr, 0.0, part)
-                       : zcsr_apply<float>(op, s, trans, dx, dy, nullptr, 0.0, part);
+                       ? zcsr_apply<double>(op, s, trans, dx, dy, nullptr, 0.0, part, yw)
+                       : zcsr_apply<float>(op, s, trans, dx, dy, nullptr, 0.0, part, yw);
     if (nb < 0) rc = PK_ERR_HIP;
   }
   if (rc == PK_OK && where == PK_HOST &&
@@ -904,6 +914,7 @@ int zspmv_any(CsrOp* op, char trans, const void* x, void* y, int where) {
     (void)hipFree(dy);
   }
   (void)hipFree(part);
+  (void)hipFree(yw);
   (void)hipStreamDestroy(s);
   return rc;
 }
@@ -1060,7 +1071,7 @@ int pk_csr_autotune(int m, int n, long long nnz, const int* rowptr, const int* c
                     int* sort_t) {
   if (!panels_a || !panels_t) return PK_ERR_ARG;
   if (prec_is_complex(val_prec)) {
-    set_error("pk_csr_autotune: complex operators have one layout");
+    set_error("pk_csr_autotune: complex operators take column panels only (pk_csr_create_panels)");
     return PK_ERR_NOTSUP;
   }
   const bool lanes = lanes_a && lanes_t;
@@ -1135,7 +1146,7 @@ int pk_csr_set_sort(pk_csr_t h, int cap_a, int cap_t) {
   CsrOp* op = as_csr(h);
   if (!op) return PK_ERR_HANDLE;
   if (prec_is_complex(op->val_prec)) {
-    set_error("pk_csr_set_sort: complex operators have one layout");
+    set_error("pk_csr_set_sort: complex operators take column panels only (pk_csr_create_panels)");
     return PK_ERR_NOTSUP;
   }
   for (const int c : {cap_a, cap_t}) {
@@ -1169,7 +1180,7 @@ int pk_csr_set_lanes(pk_csr_t h, int shift_a, int shift_t) {
   CsrOp* op = as_csr(h);
   if (!op) return PK_ERR_HANDLE;
   if (prec_is_complex(op->val_prec)) {
-    set_error("pk_csr_set_lanes: complex operators have one layout");
+    set_error("pk_csr_set_lanes: complex operators take column panels only (pk_csr_create_panels)");
     return PK_ERR_NOTSUP;
   }
   if (shift_a < -2 || shift_a > 2 || shift_t < -2 || shift_t > 2) {

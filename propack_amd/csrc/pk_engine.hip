@@ -813,7 +813,7 @@ struct ZImpl {
                            (rows_out + 1) * 4.0 + rows_in * sb + rows_out * sb +
                            (yprev ? rows_out * sb : 0.0);
       hipEvent_t t0 = tstart(tr ? KC_SPMV_T : KC_SPMV_N);
-      nb = zcsr_apply<R>(e->op, strm, tr ? 'c' : 'n', x, y, yprev, c, e->part);
+      nb = zcsr_apply<R>(e->op, strm, tr ? 'c' : 'n', x, y, yprev, c, e->part, e->sw_y);
       PK_LAUNCH(nb);
       tstop(t0, tr ? KC_SPMV_T : KC_SPMV_N, bytes, 8.0 * nnz);
     } else if (e->aprod) {
@@ -1186,7 +1186,11 @@ int pk_lbd_create(int prec, int m, int n, int kmax, pk_csr_t op, pk_lbd_t* out) 
   chk(hipMalloc(&e->dcols, sizeof(int) * e->ccap));
   chk(hipMalloc(&e->dB, sizeof(double) * nb));
   chk(hipMalloc(&e->vec, e->tbytes * mx * 2));
-  if (cop && !prec_is_complex(prec)) {
+  if (cop && prec_is_complex(prec)) {
+    // the panels' FP64 row sums of a multi-panel complex direction
+    e->sw_y_len = zcsr_ywork(cop);
+    if (e->sw_y_len > 0) chk(hipMalloc(&e->sw_y, sizeof(double) * e->sw_y_len));
+  } else if (cop) {
     e->sw_y_len = csr_ywork(cop);
     chk(hipMalloc(&e->sw_y, sizeof(double) * e->sw_y_len));
     chk(hipMalloc(&e->sw_l, sizeof(double) * csr_lwork(cop)));

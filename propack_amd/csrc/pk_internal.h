@@ -132,9 +132,11 @@ inline long long csr_lwork(const CsrOp* op) { return op->nnz / 1024 + 2; }
 
 // ---- complex precisions (pk_cplx.hip) ----
 // A complex operator (val_prec PK_C128 / PK_C64) keeps A and the explicit
-// A^T as one plain CSR panel each (da.pan[0], dt.pan[0]; values as (re, im)
-// pairs, lanes per row in pan[0].g); the layout options of the real
-// operators do not apply to it.
+// A^T as column panels of plain CSR (values as (re, im) pairs, one
+// lanes-per-row count per direction in pan[p].g), at most kMaxZPanels per
+// direction; the sorted layouts and lane shifts of the real operators do not
+// apply to it.
+constexpr int kMaxZPanels = 8;
 inline bool prec_is_complex(int prec) { return prec == PK_C128 || prec == PK_C64; }
 inline size_t prec_bytes(int prec) {
   return prec == PK_F64 ? 8 : prec == PK_F32 ? 4 : prec == PK_C128 ? 16 : 8;
@@ -144,10 +146,13 @@ int zspmv_lanes(long long nnz, long long rows);
 int zspmv_blocks(long long rows, int g);  // <= kMaxBlocks
 // y = op(A) x - c yprev (trans 'n', 't' or 'c' = A^H) with partials
 // ||y||^2, Re and Im of yprev^H y (three values per slot); returns the slot
-// count.  R is the real type of the storage precision.
+// count.  R is the real type of the storage precision.  A direction with
+// several panels runs them in one launch and needs ywork of
+// zcsr_ywork(op) doubles (the panels' FP64 row sums).
 template <typename R>
 int zcsr_apply(const CsrOp* op, hipStream_t s, char trans, const void* x, void* y,
-               const void* yprev, double c, double* part);
+               const void* yprev, double c, double* part, double* ywork);
+long long zcsr_ywork(const CsrOp* op);
 // w -= s v (complex s = sr + i si); partials ||w||^2, Re and Im of v^H w.
 template <typename R>
 int launch_zaxpy_norm_dot(hipStream_t s, long long L, void* w, const void* v, double sr,
