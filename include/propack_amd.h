@@ -304,6 +304,16 @@ int pk_bsvd_batch(int nb, const int* n, const double* d, const double* e, int ld
 int pk_batch_lanbpro(pk_batch_t batch, const int* active, const int* k0, int* k,
                      double* const* alpha, double* const* beta, double* rnorm, double* doption,
                      const int* ioption, int* ierr);
+/* pk_lbd_update (DGEMM_OVWR_LEFT, double/dgemm_ovwr.F:56-89) for problems
+ * sel[0..nsel) (0-based) of the batch in one grouped launch: problem
+ * sel[q]'s basis `which` becomes X(:, 1:ncols[q]) = X(:, 1:kdim[q]) op(B_q),
+ * B_q = bm[q] (host, leading dimension ldb[q]); the products are bit for bit
+ * the per-engine update's.  Lanczos dimensions past 315 fall back to one
+ * update per engine.  Returns when the products are done.  The batched
+ * restarted driver issues its restart GEMMs this way. */
+int pk_batch_update(pk_batch_t batch, int nsel, const int* sel, char which, char transb,
+                    const int* kdim, const int* ncols, const double* const* bm,
+                    const int* ldb);
 /* Step execution (no reference counterpart): mode 0 runs the Lanczos steps
  * as device-resident kernel chains whose scalar control (the omega
  * recurrence, ELR and reorthogonalization decisions) runs in control

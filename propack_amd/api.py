@@ -44,6 +44,7 @@ EXPORTS = [
     "pk_csr_set_sort", "pk_csr_sort", "pk_lbd_path_stats", "pk_batch_set_graph",
     "pk_csr_spmv_noalloc", "pk_lbd_residuals", "pk_lbd_promote",
     "zlansvd_", "zlansvd_irl_", "clansvd_", "clansvd_irl_", "pk_zcsr_aprod_", "pk_ccsr_aprod_",
+    "pk_batch_update",
 ]
 
 _lib = None

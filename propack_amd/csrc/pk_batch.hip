@@ -335,6 +335,9 @@ struct Batch {
   // the restarted driver's bidiagonal SVDs on the device (pk_bsvd.hip)
   bool use_bsvd = false;
   BsvdBuf bsvd;
+  // the restart GEMMs of all problems in one launch (launch_gemm_group)
+  GemmItem* gitems = nullptr;
+  size_t gcap = 0;
 };
 constexpr size_t kMaxBatchGraphs = 8;
 
@@ -494,6 +497,7 @@ void batch_free(Batch* b) {
   (void)hipFree(b->dprob);
   (void)hipFree(b->part);
   (void)hipFree(b->red);
+  (void)hipFree(b->gitems);
   if (b->stream) (void)hipStreamDestroy(b->stream);
   b->magic = 0;
   delete b;
@@ -908,6 +912,85 @@ int pk_batch_bsvd(pk_batch_t h, int nb, const int* n, const double* d, const dou
   if (!b) return PK_ERR_HANDLE;
   if (nb > 0 && (!n || !d || !e || !sigma || !bottom)) return PK_ERR_ARG;
   return bsvd_host(b->bsvd, b->stream, nb, n, d, e, ld, sigma, bottom);
+}
+
+// DGEMM_OVWR_LEFT (double/dgemm_ovwr.F:56-89) of problems sel[0..nsel) in
+// one grouped launch: X_i(:, 1:ncols[q]) = X_i(:, 1:kdim[q]) op(B_q) into
+// each engine's spare buffer, then the buffers swap, as pk_lbd_update does
+// for one engine (and bit for bit the same products).
+int pk_batch_update(pk_batch_t h, int nsel, const int* sel, char which, char transb,
+                    const int* kdim, const int* ncols, const double* const* bm, const int* ldb) {
+  Batch* b = as_batch(h);
+  if (!b) return PK_ERR_HANDLE;
+  if (nsel < 0 || (nsel > 0 && (!sel || !kdim || !ncols || !bm || !ldb))) return PK_ERR_ARG;
+  const bool isu = which == 'u' || which == 'U';
+  const bool tb = transb == 't' || transb == 'T';
+  // the grouped kernel stages B in LDS (kdim <= 316); past that, one update
+  // per engine (the long-kdim GEMM)
+  for (int q = 0; q < nsel; ++q)
+    if (kdim[q] > 316) {
+      for (int r = 0; r < nsel; ++r) {
+        if (sel[r] < 0 || sel[r] >= b->nprob) return PK_ERR_ARG;
+        const int rc = pk_lbd_update(reinterpret_cast<pk_lbd_t>(b->eng[sel[r]]), which, transb,
+                                     kdim[r], ncols[r], bm[r], ldb[r]);
+        if (rc != PK_OK) return rc;
+      }
+      return PK_OK;
+    }
+  std::vector<GemmJob> jobs;
+  jobs.reserve(nsel);
+  for (int q = 0; q < nsel; ++q) {
+    const int i = sel[q];
+    if (i < 0 || i >= b->nprob) {
+      set_error("pk_batch_update: problem %d outside the batch", i);
+      return PK_ERR_ARG;
+    }
+    Engine* e = b->eng[i];
+    const int maxk = isu ? e->kmax + 1 : e->kmax;
+    if (kdim[q] < 1 || ncols[q] < 1 || kdim[q] > maxk || ncols[q] > maxk ||
+        ldb[q] < (tb ? ncols[q] : kdim[q]) || !bm[q]) {
+      set_error("pk_batch_update: problem %d: kdim=%d ncols=%d ldb=%d outside [1, %d]", i,
+                kdim[q], ncols[q], ldb[q], maxk);
+      return PK_ERR_ARG;
+    }
+    void** buf = isu ? e->U : e->V;
+    const int cur = isu ? e->ucur : e->vcur;
+    const long long ld = isu ? e->ldu : e->ldv;
+    if (!buf[1 - cur] && hipMalloc(&buf[1 - cur], e->tbytes * ld * maxk) != hipSuccess) {
+      set_error("pk_batch_update: spare basis of problem %d", i);
+      return PK_ERR_OOM;
+    }
+    // the engine's earlier work (a copy out of hB among it) is done first
+    if (hipStreamSynchronize(e->stream) != hipSuccess) return PK_ERR_HIP;
+    const double* B = bm[q];
+    for (int j = 0; j < ncols[q]; ++j)
+      for (int l = 0; l < kdim[q]; ++l)
+        e->hB[static_cast<size_t>(j) * kdim[q] + l] =
+            tb ? B[static_cast<size_t>(l) * ldb[q] + j] : B[static_cast<size_t>(j) * ldb[q] + l];
+    if (hipMemcpyAsync(e->dB, e->hB, sizeof(double) * kdim[q] * ncols[q], hipMemcpyHostToDevice,
+                       b->stream) != hipSuccess)
+      return PK_ERR_HIP;
+    jobs.push_back(GemmJob{buf[cur], buf[1 - cur], e->dB, ld, ld, isu ? e->m : e->n, kdim[q],
+                           kdim[q], ncols[q]});
+  }
+  if (jobs.empty()) return PK_OK;
+  const int rc = b->prec == PK_F64
+                     ? launch_gemm_group<double>(b->stream, jobs.data(),
+                                                 static_cast<int>(jobs.size()), &b->gitems,
+                                                 &b->gcap)
+                     : launch_gemm_group<float>(b->stream, jobs.data(),
+                                                static_cast<int>(jobs.size()), &b->gitems,
+                                                &b->gcap);
+  if (rc != 0) {
+    set_error("pk_batch_update: grouped GEMM launch failed");
+    return PK_ERR_HIP;
+  }
+  for (int q = 0; q < nsel; ++q) {
+    Engine* e = b->eng[sel[q]];
+    int& cur = isu ? e->ucur : e->vcur;
+    cur = 1 - cur;
+  }
+  return PK_OK;
 }
 
 int pk_batch_lanbpro(pk_batch_t h, const int* active, const int* k0, int* k,

@@ -121,6 +121,31 @@ template <typename T>
 int launch_gemm_mfma(hipStream_t s, long long L, const T* A, long long lda, int kdim,
                      const double* B, int ldb, int ncols, T* C, long long ldc);
 
+// Many products C_q(:, 0:ncols_q) = A_q(:, 0:kdim_q) * B_q in one launch per
+// column-tile count (the restart GEMMs of a batch): every row block computed
+// as launch_gemm_mfma computes it (bit for bit), kdim_q <= 316 (B staged in
+// LDS).  ditems / dcap: a device buffer of items the call grows as needed.
+// Returns 0, -1 on a HIP error, -2 for a kdim past the LDS limit.  Waits for
+// its stream before returning.
+struct GemmJob {
+  const void* A;
+  void* C;
+  const double* B;  // device, kdim x ncols, leading dimension ldb
+  long long lda, ldc, L;
+  int kdim, ldb, ncols;
+};
+struct GemmItem {
+  const void* A;
+  void* C;
+  const double* B;
+  long long lda, ldc, L;
+  int kdim, kpad, ldb, ncols;
+  long long blk0, blk1;
+};
+template <typename T>
+int launch_gemm_group(hipStream_t s, const GemmJob* jobs, int njobs, GemmItem** ditems,
+                      size_t* dcap);
+
 // Control kernel of the device-resident step (see pk_lbd.h).
 enum CtlKind { CTL_V_SPMV = 0, CTL_V_ELR, CTL_V_REO, CTL_U_SPMV, CTL_U_ELR, CTL_U_REO };
 int launch_ctl(hipStream_t s, LbdCtl* c, void* arrays, int cap, int kind, int want_phase,

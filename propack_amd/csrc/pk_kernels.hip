@@ -14,6 +14,9 @@
 //   k_reduce        second reduction stage (fixed order => deterministic).
 //   k_gemm_mfma     C(:,1:n) = A(:,1:k) B on FP64 matrix cores for restarts /
 //                   Ritz vectors (out of place into the spare basis buffer).
+#include <algorithm>
+#include <vector>
+
 #include "pk_bodies.h"
 
 namespace pk {
@@ -581,6 +584,161 @@ __global__ __launch_bounds__(kWave * kGemmWaves) void k_gemm_lds(
   }
 }
 
+// Many products in one launch (the restart GEMMs of a batch): workgroup w
+// runs item w, a range of row blocks of one product's 64-column chunk, with
+// that chunk's B staged.  The body is k_gemm_lds's, statement for statement,
+// so every row block comes out bit for bit as that kernel computes it (kept
+// as its own text: sharing one device function between the two kernels
+// changed k_gemm_lds's register allocation and slowed it by a fifth,
+// measured on the 2M x 121 * 121 x 51 shape).
+template <typename T, int NT>
+__global__ __launch_bounds__(kWave * kGemmWaves) void k_gemm_lds_group(
+    const GemmItem* __restrict__ items) {
+  const GemmItem it = items[blockIdx.x];
+  const long long L = it.L, lda = it.lda, ldc = it.ldc;
+  const int kdim = it.kdim, kpad = it.kpad, ldb = it.ldb, ncols = it.ncols;
+  const T* __restrict__ A = static_cast<const T*>(it.A);
+  const double* __restrict__ B = it.B;
+  T* __restrict__ C = static_cast<T*>(it.C);
+  extern __shared__ double Bs[];  // Bs[j * kpad + k] = B(k, j), zero outside kdim x ncols
+  const int lane = threadIdx.x % kWave, wid = threadIdx.x / kWave;
+  const int li = lane % 16, lk = lane / 16;
+  for (int e = threadIdx.x; e < NT * 16 * kpad; e += blockDim.x) {
+    const int j = e / kpad, k = e - j * kpad;
+    Bs[e] = (j < ncols && k < kdim) ? B[static_cast<long long>(j) * ldb + k] : 0.0;
+  }
+  __syncthreads();
+  const int nks = (kdim + 3) / 4;
+  for (long long blk = it.blk0; blk < it.blk1; ++blk) {
+    const long long i0 = blk * kGemmRows + static_cast<long long>(wid) * (16 * kGemmRowTiles);
+    if (i0 >= L) continue;  // wave-uniform
+    long long rowc[kGemmRowTiles];
+#pragma unroll
+    for (int rt = 0; rt < kGemmRowTiles; ++rt) {
+      const long long row = i0 + rt * 16 + li;
+      rowc[rt] = row < L ? row : L - 1;
+    }
+    v4d acc[kGemmRowTiles][NT];
+#pragma unroll
+    for (int rt = 0; rt < kGemmRowTiles; ++rt)
+#pragma unroll
+      for (int ct = 0; ct < NT; ++ct) acc[rt][ct] = v4d{0.0, 0.0, 0.0, 0.0};
+    double a0[kGemmRowTiles], a1[kGemmRowTiles], a2[kGemmRowTiles];
+    auto load = [&](int ks, double (&a)[kGemmRowTiles]) {
+      // clamped: the loads stay in bounds, and a step past the end is never used
+      const long long k = min(ks * 4 + lk, kdim - 1);
+#pragma unroll
+      for (int rt = 0; rt < kGemmRowTiles; ++rt) a[rt] = static_cast<double>(A[k * lda + rowc[rt]]);
+    };
+    load(0, a0);
+    load(1, a1);
+    for (int ks = 0; ks < nks; ++ks) {
+      load(ks + 2, a2);
+      const int k = ks * 4 + lk;
+      double b[NT];
+#pragma unroll
+      for (int ct = 0; ct < NT; ++ct) b[ct] = Bs[(ct * 16 + li) * kpad + k];
+#pragma unroll
+      for (int rt = 0; rt < kGemmRowTiles; ++rt)
+#pragma unroll
+        for (int ct = 0; ct < NT; ++ct)
+          acc[rt][ct] = __builtin_amdgcn_mfma_f64_16x16x4f64(b[ct], a0[rt], acc[rt][ct], 0, 0, 0);
+#pragma unroll
+      for (int rt = 0; rt < kGemmRowTiles; ++rt) {
+        a0[rt] = a1[rt];
+        a1[rt] = a2[rt];
+      }
+    }
+#pragma unroll
+    for (int rt = 0; rt < kGemmRowTiles; ++rt) {
+      const long long row = i0 + rt * 16 + li;
+      if (row >= L) continue;
+#pragma unroll
+      for (int ct = 0; ct < NT; ++ct)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          const int j = ct * 16 + lk + 4 * r;
+          if (j < ncols) C[static_cast<long long>(j) * ldc + row] = static_cast<T>(acc[rt][ct][r]);
+        }
+    }
+  }
+}
+
+template <typename T>
+int launch_gemm_group(hipStream_t s, const GemmJob* jobs, int njobs, GemmItem** ditems,
+                      size_t* dcap) {
+  // items per column-tile count (NT = 1..4), row blocks cut into ranges of
+  // at most kGroupBlocks
+  constexpr long long kGroupBlocks = 8;
+  std::vector<GemmItem> items[4];
+  int kmax_pad[4] = {0, 0, 0, 0};
+  for (int q = 0; q < njobs; ++q) {
+    const GemmJob& jb = jobs[q];
+    if (jb.L <= 0 || jb.ncols <= 0 || jb.kdim <= 0) continue;
+    const int kpad = (jb.kdim + 3) / 4 * 4 + 1;
+    if (kpad > kGemmMaxKpad) return -2;  // the grouped path stages B in LDS only
+    const long long nblocks = (jb.L + kGemmRows - 1) / kGemmRows;
+    for (int j0 = 0; j0 < jb.ncols; j0 += 64) {
+      const int nc = std::min(64, jb.ncols - j0);
+      const int nt = (nc + 15) / 16;
+      kmax_pad[nt - 1] = std::max(kmax_pad[nt - 1], kpad);
+      for (long long b0 = 0; b0 < nblocks; b0 += kGroupBlocks) {
+        GemmItem it;
+        it.A = jb.A;
+        it.C = static_cast<char*>(jb.C) + sizeof(T) * static_cast<size_t>(j0) * jb.ldc;
+        it.B = jb.B + static_cast<long long>(j0) * jb.ldb;
+        it.lda = jb.lda;
+        it.ldc = jb.ldc;
+        it.L = jb.L;
+        it.kdim = jb.kdim;
+        it.kpad = kpad;
+        it.ldb = jb.ldb;
+        it.ncols = nc;
+        it.blk0 = b0;
+        it.blk1 = std::min(nblocks, b0 + kGroupBlocks);
+        items[nt - 1].push_back(it);
+      }
+    }
+  }
+  size_t total = 0;
+  for (auto& v : items) total += v.size();
+  if (total == 0) return 0;
+  if (*dcap < total) {
+    (void)hipFree(*ditems);
+    *ditems = nullptr;
+    *dcap = 0;
+    if (hipMalloc(ditems, sizeof(GemmItem) * total) != hipSuccess) return -1;
+    *dcap = total;
+  }
+  std::vector<GemmItem> all;
+  all.reserve(total);
+  for (auto& v : items) all.insert(all.end(), v.begin(), v.end());
+  if (hipMemcpyAsync(*ditems, all.data(), sizeof(GemmItem) * total, hipMemcpyHostToDevice, s) !=
+      hipSuccess)
+    return -1;
+  size_t off = 0;
+  for (int nt = 1; nt <= 4; ++nt) {
+    const size_t cnt = items[nt - 1].size();
+    if (cnt == 0) continue;
+    const size_t shm = sizeof(double) * 16 * nt * kmax_pad[nt - 1];
+    const GemmItem* d = *ditems + off;
+#define PK_GEMM_GRP(NTV)                                                                     \
+  hipLaunchKernelGGL((k_gemm_lds_group<T, NTV>), dim3(static_cast<unsigned>(cnt)),            \
+                     dim3(kWave * kGemmWaves), shm, s, d)
+    switch (nt) {
+      case 1: PK_GEMM_GRP(1); break;
+      case 2: PK_GEMM_GRP(2); break;
+      case 3: PK_GEMM_GRP(3); break;
+      default: PK_GEMM_GRP(4); break;
+    }
+#undef PK_GEMM_GRP
+    if (hipGetLastError() != hipSuccess) return -1;
+    off += cnt;
+  }
+  // the host vector of items is read by the copy above: wait for it
+  return hipStreamSynchronize(s) == hipSuccess ? 0 : -1;
+}
+
 template <typename T>
 int launch_gemm_mfma(hipStream_t s, long long L, const T* A, long long lda, int kdim,
                      const double* B, int ldb, int ncols, T* C, long long ldc) {
@@ -694,7 +852,8 @@ template int launch_colsum<double, double>(hipStream_t, long long, const double*
                                        Gate);                                                 \
   template int launch_norm<T>(hipStream_t, long long, const T*, double*);                     \
   template int launch_gemm_mfma<T>(hipStream_t, long long, const T*, long long, int,         \
-                                   const double*, int, int, T*, long long);
+                                   const double*, int, int, T*, long long);                   \
+  template int launch_gemm_group<T>(hipStream_t, const GemmJob*, int, GemmItem**, size_t*);
 PK_INST_T(double)
 PK_INST_T(float)
 #undef PK_INST_T

@@ -44,6 +44,9 @@ module pk_drivers
     double precision :: eps = 0d0, eps34 = 0d0, epsn = 0d0, tol = 0d0, anorm = 0d0
     double precision :: rnorm = 0d0, dopt(4) = 0d0
     logical :: small = .false., done = .true., failed = .false.
+    ! defer: the restart GEMMs are left to the caller (the batched driver
+    ! runs every problem's in one launch); pending: this pass left them
+    logical :: defer = .false., pending = .false.
     integer(c_long_long) :: st0(8) = 0
     double precision, allocatable :: alpha(:), beta(:), a1(:), b1(:), wb(:), shift(:), &
                                      pm(:, :), qm(:, :), wrk(:), sig(:), bd(:)
@@ -504,13 +507,17 @@ contains
                            size(pm, 1), qm, size(qm, 1))
         end do
         ! U(:,1:k+1) <- U(:,1:dim+1) P(:,1:k+1); V(:,1:k) <- V(:,1:dim) Q(:,1:k)
-        rc = pk_lbd_update(st%eng, 'u', 'n', dim + 1, k + 1, pm, size(pm, 1))
-        if (rc == PK_OK) rc = pk_lbd_update(st%eng, 'v', 'n', dim, k, qm, size(qm, 1))
-        if (rc /= PK_OK) then
-          st%info = rc
-          st%done = .true.
-          st%failed = .true.
-          return
+        if (st%defer) then
+          st%pending = .true.
+        else
+          rc = pk_lbd_update(st%eng, 'u', 'n', dim + 1, k + 1, pm, size(pm, 1))
+          if (rc == PK_OK) rc = pk_lbd_update(st%eng, 'v', 'n', dim, k, qm, size(qm, 1))
+          if (rc /= PK_OK) then
+            st%info = rc
+            st%done = .true.
+            st%failed = .true.
+            return
+          end if
         end if
         st%rnorm = beta(k)
         st%nrestart = st%nrestart + 1
@@ -728,12 +735,22 @@ integer(c_int) function pk_lansvd_irl_batch(nb, engs, batch, which, jobu, jobv, 
       integer(c_int) :: nn(*)
       real(c_double) :: d(*), e(*), sig(*), bot(*)
     end function
+    integer(c_int) function pk_batch_update(b, nsel, sel, which, transb, kdim, ncols, bm, ldb) &
+        bind(C, name='pk_batch_update')
+      import :: c_int, c_ptr, c_char
+      type(c_ptr), value :: b
+      integer(c_int), value :: nsel
+      character(kind=c_char), value :: which, transb
+      integer(c_int) :: sel(*), kdim(*), ncols(*), ldb(*)
+      type(c_ptr) :: bm(*)
+    end function
   end interface
   integer, parameter :: bsvd_max = 128  ! largest order of the device bidiagonal SVD
   type(irl_state), allocatable, target :: st(:)
   integer(c_int), allocatable, target :: active(:), k0(:), kk(:), ierr(:), dev(:)
-  integer(c_int), allocatable :: bn(:), idx(:)
-  type(c_ptr), allocatable :: ap(:), bp(:)
+  integer(c_int), allocatable :: bn(:), idx(:), usel(:), ukd(:), unc(:), uld(:), vkd(:), &
+                                 vnc(:), vld(:)
+  type(c_ptr), allocatable :: ap(:), bp(:), upm(:), vqm(:)
   real(c_double), allocatable, target :: rn(:), dop(:, :)
   real(c_double), allocatable :: bd(:, :), be(:, :), bs(:, :), bb(:, :)
   type(after_ctx), target :: ctx
@@ -743,7 +760,8 @@ integer(c_int) function pk_lansvd_irl_batch(nb, engs, batch, which, jobu, jobv, 
 
   pk_lansvd_irl_batch = PK_OK
   allocate(st(nb), active(nb), k0(nb), kk(nb), ierr(nb), ap(nb), bp(nb), rn(nb), dop(4, nb), &
-           dev(nb), bn(nb), idx(nb))
+           dev(nb), bn(nb), idx(nb), usel(nb), ukd(nb), unc(nb), uld(nb), vkd(nb), vnc(nb), &
+           vld(nb), upm(nb), vqm(nb))
   usedev = 0
   rc = pk_batch_get_bsvd(batch, usedev)
   if (rc /= PK_OK) then
@@ -760,6 +778,7 @@ integer(c_int) function pk_lansvd_irl_batch(nb, engs, batch, which, jobu, jobv, 
     end if
     call irl_begin(st(i), which, jobu, jobv, m, n, dim, p, neig, maxiter, engs(i), eps, tolin, &
                    doption, ioption)
+    st(i)%defer = .true.
   end do
   do while (any(.not. st(:)%done))
     do i = 1, nb
@@ -793,6 +812,7 @@ integer(c_int) function pk_lansvd_irl_batch(nb, engs, batch, which, jobu, jobv, 
     ctx%dop => dop
     if (usedev == 0) then
       call pkx_parallel_for(nb, c_funloc(after_one), c_loc(ctx))
+      call restart_gemms()
       cycle
     end if
     ! bidiagonal SVDs of every problem on the device, between the two
@@ -832,6 +852,7 @@ integer(c_int) function pk_lansvd_irl_batch(nb, engs, batch, which, jobu, jobv, 
       st(i)%wb(1:dd) = c1 * bb(1:dd, j)
     end do
     call pkx_parallel_for(nb, c_funloc(post_one), c_loc(ctx))
+    call restart_gemms()
   end do
   do i = 1, nb
     dd = dim
@@ -840,6 +861,41 @@ integer(c_int) function pk_lansvd_irl_batch(nb, engs, batch, which, jobu, jobv, 
     call irl_end(st(i), sigma(:, i), bnd(:, i), dd, ne, info(i), dummy)
     nconv(i) = ne
   end do
+contains
+  ! The restart GEMMs this pass left (double/dlansvd_irl.F:387-395) for every
+  ! problem at once: U(:,1:k+1) <- U(:,1:dim+1) P and V(:,1:k) <- V(:,1:dim) Q,
+  ! one grouped launch each.
+  subroutine restart_gemms()
+    integer :: nu, q
+    integer(c_int) :: rcu
+    nu = 0
+    do q = 1, nb
+      if (st(q)%pending) then
+        nu = nu + 1
+        usel(nu) = q - 1
+        ukd(nu) = st(q)%dim + 1
+        unc(nu) = st(q)%k + 1
+        uld(nu) = size(st(q)%pm, 1)
+        upm(nu) = c_loc(st(q)%pm)
+        vkd(nu) = st(q)%dim
+        vnc(nu) = st(q)%k
+        vld(nu) = size(st(q)%qm, 1)
+        vqm(nu) = c_loc(st(q)%qm)
+        st(q)%pending = .false.
+      end if
+    end do
+    if (nu == 0) return
+    rcu = pk_batch_update(batch, nu, usel, 'u', 'n', ukd, unc, upm, uld)
+    if (rcu == PK_OK) rcu = pk_batch_update(batch, nu, usel, 'v', 'n', vkd, vnc, vqm, vld)
+    if (rcu /= PK_OK) then
+      pk_lansvd_irl_batch = rcu
+      do q = 1, nu
+        st(usel(q) + 1)%info = rcu
+        st(usel(q) + 1)%failed = .true.
+        st(usel(q) + 1)%done = .true.
+      end do
+    end if
+  end subroutine
 end function
 
 subroutine engine_shape(eng, m, n, eps, info)

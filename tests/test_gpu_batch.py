@@ -75,6 +75,8 @@ def test_batch_matches_sequential_fp64():
         assert r.info == rb.info == 0 and r.k == rb.k == NEIG
         assert np.array_equal(r.sigma, rb.sigma)
         assert nopx == rb.counters["nopx"]
+    # restarted solves: the batch's restart GEMMs ran grouped (pk_batch_update)
+    assert any(r.counters["nrestart"] > 0 for r, _ in seq)
     # the per-problem engine against the reference on two of them
     for (prm, A, op, eng, u0), (r, nopx) in list(zip(probs, seq))[:2]:
         ref = refpy.lansvd_irl(A, NEIG, DIM, u0=u0, tol=TOL, cgs=1, jobu="n", jobv="n")
@@ -262,3 +264,53 @@ def test_full_batch_workload():
                 assert ref.k == NEIG
                 np.testing.assert_allclose(bat[i].sigma, ref.sigma, rtol=1e-10)
         del probs, bat
+
+
+@pytest.mark.parametrize("prec", [api.PK_F64, api.PK_F32])
+def test_batch_update_matches_engine_update(prec):
+    """pk_batch_update (the restart GEMMs of several problems in one grouped
+    launch) against pk_lbd_update engine by engine, bit for bit, on uneven
+    problems, a selection that skips one, both bases, and a column count
+    past one 64-column chunk."""
+    import ctypes as C
+    L = api.lib()
+    dt = np.float64 if prec == api.PK_F64 else np.float32
+    sizes = [(3000, 1700), (700, 2100), (5000, 900)]
+    kmax = 90
+    rng = np.random.default_rng(11)
+    sets = []
+    for _ in range(2):
+        engs = []
+        for m, n in sizes:
+            A = synth.random_uniform(m, n, 8 * m, seed=m + n, dtype=dt)
+            engs.append(api.Engine(api.CsrOperator(A, prec=prec), kmax=kmax, prec=prec))
+        sets.append(engs)
+    for i, (m, n) in enumerate(sizes):
+        for which, L_, cols in (("u", m, kmax + 1), ("v", n, kmax)):
+            X = rng.standard_normal((cols, L_))
+            for e in (sets[0][i], sets[1][i]):
+                for c in range(cols):
+                    x = np.ascontiguousarray(X[c])
+                    assert L.pk_lbd_put(e.handle, C.c_char(which.encode()), C.c_int(c + 1),
+                                        api._p(x), C.c_int(api.PK_F64)) == 0
+    batch = api.Batch(sets[1])
+    sel = np.array([0, 2], np.int32)
+    for which, kd, nc in (("u", 81, 71), ("v", 80, 70)):
+        Bs = [np.asfortranarray(rng.standard_normal((kd + 3, nc))) for _ in sel]
+        for q, i in enumerate(sel):
+            assert L.pk_lbd_update(sets[0][i].handle, C.c_char(which.encode()), C.c_char(b"n"),
+                                   C.c_int(kd), C.c_int(nc), api._p(Bs[q]),
+                                   C.c_int(kd + 3)) == 0
+        ptrs = (C.c_void_p * len(sel))(*[b.ctypes.data for b in Bs])
+        kdv = np.full(len(sel), kd, np.int32)
+        ncv = np.full(len(sel), nc, np.int32)
+        ldv = np.full(len(sel), kd + 3, np.int32)
+        rc = L.pk_batch_update(batch.handle, C.c_int(len(sel)), api._p(sel),
+                               C.c_char(which.encode()), C.c_char(b"n"), api._p(kdv),
+                               api._p(ncv), ptrs, api._p(ldv))
+        assert rc == 0, api.lib().pk_last_error()
+        for i in range(len(sizes)):
+            a = sets[0][i].get(which, nc)
+            b = sets[1][i].get(which, nc)
+            assert np.array_equal(a, b), (which, i)
+    batch.close()
