@@ -211,3 +211,130 @@ def test_complex_engine_refuses_real_only_calls(zcase):
     eng = api.Engine(op, 20)
     with pytest.raises(api.PropackError):
         api.Batch([eng])
+
+
+# ---- the reference's rare and failure paths, complex tree (one reference
+# thread; INFO, K, nopx and nreorth exactly, sigma to 1e-10 of sigma_1) ----
+
+def _zsame(r, ref, sig_rtol=1e-10, ndot_rtol=0.0):
+    print("gpu", r.info, r.k, r.anorm, r.counters)
+    print("ref", ref.info, ref.k, ref.anorm, ref.counters)
+    assert r.info == ref.info and r.k == ref.k
+    if ref.k:
+        assert np.abs(r.sigma - ref.sigma).max() <= sig_rtol * ref.sigma.max()
+    for key in ("nopx", "nreorth"):
+        assert r.counters[key] == ref.counters[key], key
+    assert abs(r.counters["ndot"] - ref.counters["ndot"]) <= ndot_rtol * ref.counters["ndot"]
+
+
+def _dense_csr(D):
+    rows, cols = np.nonzero(np.ones(D.shape))
+    return synth.from_coo(D.shape[0], D.shape[1], rows, cols, D.ravel())
+
+
+def test_zlansvd_kmax_exhausted(zcase):
+    """kmax too small for k=10 at tol 1e-12: INFO = -1 with K converged
+    (complex16/zlansvd.F:250-258)."""
+    A, D, s, u0 = zcase
+    r = api.lansvd(api.CsrOperator(A, api.PK_C128), A.m, A.n, 10, 20, u0=u0, tol=1e-12,
+                   prec=api.PK_C128)
+    ref = refpy.lansvd(A, 10, 20, u0=u0, tol=1e-12, cgs=1, prec="z")
+    assert ref.info == -1
+    _zsame(r, ref)
+
+
+def test_zlansvd_wide_matrix(zcase):
+    """m < n (A^H of the test matrix): DLANBPRO's norm estimate draws its
+    random vector through A instead of A^H (complex16/zlanbpro.F:224-228)."""
+    A, D, s, u0 = zcase
+    T = A.transpose()
+    AH = synth.Csr(T.m, T.n, T.rowptr, T.col, T.val.conj())
+    v0 = synth.start_vector(AH.m, 9, np.complex128)
+    r = api.lansvd(api.CsrOperator(AH, api.PK_C128), AH.m, AH.n, 8, 300, u0=v0, tol=1e-12,
+                   prec=api.PK_C128)
+    ref = refpy.lansvd(AH, 8, 300, u0=v0, tol=1e-12, cgs=1, prec="z")
+    _zsame(r, ref)
+    np.testing.assert_allclose(r.sigma, s[:8], rtol=1e-12)
+
+
+def test_zlansvd_five_failed_cgs_passes():
+    """40 x 20 complex with k = 20, kmax = 21: at step 21 the right Lanczos
+    vectors span C^20 and after five CGS passes v_21 = 0
+    (complex16/zreorth.F:85-99)."""
+    B = synth.complex_uniform(40, 20, 400, seed=9)
+    v0 = synth.start_vector(40, 2, np.complex128)
+    s = np.linalg.svd(B.todense(), compute_uv=False)
+    op = api.CsrOperator(B, api.PK_C128)
+    r = api.lansvd(op, B.m, B.n, 20, 21, u0=v0, tol=1e-12, prec=api.PK_C128)
+    ref = refpy.lansvd(B, 20, 21, u0=v0, tol=1e-12, cgs=1, prec="z")
+    _zsame(r, ref)
+    np.testing.assert_allclose(r.sigma, s, rtol=1e-12)
+    eng = api.Engine(op, kmax=21)
+    eng.put_u1(v0)
+    api.lansvd_dev(eng, 20, 21, tol=1e-12)
+    assert eng.path_stats()["v_zero"] >= 1
+
+
+def test_zlansvd_invariant_subspace_new_direction():
+    """Complex rank 15 exactly, k = 20: alpha falls below anorm*epsn and the
+    step draws a new random direction (complex16/zlanbpro.F:390-427); the 15
+    nonzero singular values match the reference, K within one (how many of
+    the zero values count as converged is decided at the rounding level)."""
+    g = np.random.Generator(np.random.Philox(4))
+    m, n, rk = 200, 80, 15
+    D = (g.standard_normal((m, rk)) + 1j * g.standard_normal((m, rk))) @ \
+        (g.standard_normal((rk, n)) + 1j * g.standard_normal((rk, n)))
+    A = _dense_csr(D)
+    u0 = synth.start_vector(m, 6, np.complex128)
+    op = api.CsrOperator(A, api.PK_C128)
+    r = api.lansvd(op, m, n, 20, 40, u0=u0, tol=1e-12, prec=api.PK_C128)
+    ref = refpy.lansvd(A, 20, 40, u0=u0, tol=1e-12, cgs=1, prec="z")
+    print("gpu", r.info, r.k, r.counters)
+    print("ref", ref.info, ref.k, ref.counters)
+    assert r.info == ref.info and abs(r.k - ref.k) <= 1 and r.k >= rk
+    np.testing.assert_allclose(r.sigma[:rk], ref.sigma[:rk], rtol=1e-10)
+    eng = api.Engine(op, kmax=40)
+    eng.put_u1(u0)
+    api.lansvd_dev(eng, 20, 40, tol=1e-12)
+    ps = eng.path_stats()
+    assert ps["v_invariant"] + ps["u_invariant"] >= 1
+
+
+def test_zlansvd_nan_operator(zcase):
+    """A NaN value in the complex operator: INFO = 0 with K = 0 and a NaN
+    ||A|| estimate, as the reference's control flow on NaN comparisons
+    gives; the engine works afterwards."""
+    A, D, s, u0 = zcase
+    An = synth.Csr(A.m, A.n, A.rowptr, A.col, A.val.copy())
+    An.val[100] = complex(np.nan, 0.0)
+    ref = refpy.lansvd(An, 10, 300, u0=u0, tol=1e-12, cgs=1, prec="z")
+    r = api.lansvd(api.CsrOperator(An, api.PK_C128), A.m, A.n, 10, 300, u0=u0, tol=1e-12,
+                   prec=api.PK_C128)
+    print("gpu", r.info, r.k, r.anorm, r.counters)
+    print("ref", ref.info, ref.k, ref.anorm, ref.counters)
+    assert r.info == ref.info and r.k == ref.k == 0
+    assert np.isnan(r.anorm) and np.isnan(ref.anorm)
+    assert r.counters["nopx"] == ref.counters["nopx"]
+    r2 = api.lansvd(api.CsrOperator(A, api.PK_C128), A.m, A.n, 10, 300, u0=u0, tol=1e-12,
+                    prec=api.PK_C128)
+    np.testing.assert_allclose(r2.sigma, s[:10], rtol=1e-12)
+
+
+def test_clansvd_tiny_alpha(zcase):
+    """CLANSVD on the test matrix scaled by 1e-39 (complex64 values): every
+    alpha and beta lies below FP32's safe minimum, so each scaling takes
+    the stepwise path of CSAFESCAL (complex8/csafescal.F); the engine
+    accumulates in FP64, and the values match the dense SVD of the stored
+    matrix to FP32 storage precision."""
+    A, D, s, u0 = zcase
+    As = synth.Csr(A.m, A.n, A.rowptr, A.col, (A.val * 1e-39).astype(np.complex64))
+    sd = np.linalg.svd(As.todense(), compute_uv=False)
+    op = api.CsrOperator(As, api.PK_C64)
+    eng = api.Engine(op, kmax=300)
+    eng.put_u1(u0)
+    r = api.lansvd_dev(eng, 10, 300, tol=1e-6)
+    ps = eng.path_stats()
+    print("paths", ps, "info", r.info, "k", r.k)
+    assert ps["v_tiny"] >= 1 and ps["u_tiny"] >= 1
+    assert r.info == 0 and r.k == 10
+    np.testing.assert_allclose(r.sigma, sd[:10], rtol=1e-5)
