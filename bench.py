@@ -89,6 +89,25 @@ def measured_traffic(kclass, panels, lanes, sort=(0, 0)):
     return same_bytes
 
 
+ZTRAFFIC_PROFILE = os.path.join(ROOT, "profiles", "hbm_traffic_zspmv_complex_poisson_decay_1Mx500k.md")
+
+
+def measured_ztraffic(kclass, panels):
+    """HBM bytes per application of the complex SpMV class kclass for the
+    operator's column panels (A, A^H), from the FETCH_SIZE / WRITE_SIZE runs
+    summarised in profiles/; None if that layout was not measured."""
+    try:
+        with open(ZTRAFFIC_PROFILE) as f:
+            for line in f:
+                w = line.split()
+                if len(w) == 5 and w[0] == "ztraffic_bytes_per_application" and \
+                        w[1] == kclass and (int(w[3]), int(w[4])) == tuple(panels):
+                    return float(w[2])
+    except OSError:
+        pass
+    return None
+
+
 def residuals(op, r):
     """max_i ||A v_i - s_i u_i|| / s_1 over the returned triplets."""
     worst = 0.0
@@ -369,12 +388,14 @@ def workload_complex(args):
            "config": {"workload": f"ZLANSVD_IRL k={k} dim={dim} tol={tol} on "
                                   f"complex_poisson_decay {m}x{n} (Poisson(16) per row, "
                                   "decay 0.6, unit phases)",
-                      "m": A.m, "n": A.n, "nnz": A.nnz, "converged": r.k, "info": r.info,
+                      "m": A.m, "n": A.n, "nnz": A.nnz, "spmv_panels": list(op.panels()),
+                      "converged": r.k, "info": r.info,
                       "nopx": r.counters["nopx"], "nreorth": r.counters["nreorth"],
                       "nrestart": r.counters["nrestart"], "max_rel_residual_3": resid},
            "roofline": {"kernel": dom, "bound": "hbm", "achieved": round(achieved, 1),
                         "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                        "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                        "frac": round(achieved / HBM_PEAK_GBS, 4),
+                        "traffic": measured_ztraffic(dom, op.panels()),
                         "avg_launch_ms": pd["ms"] / max(pd["launches"], 1),
                         "bytes_per_launch": pd["bytes"] / max(pd["launches"], 1),
                         "timed_in": "one profiled solve after the timed ones"},
