@@ -140,8 +140,14 @@ int pk_csr_autotune(int m, int n, long long nnz, const int* rowptr, const int* c
  * summed in input order, built on the device; inputs and outputs in host
  * or device memory (`where`); colind/val need nnz entries of room,
  * *nnz_out receives the distinct count.
+ * Complex values (val_prec PK_C128 / PK_C64, (re, im) pairs) are summed
+ * part by part.
  * pk_csr_create_coo: the same straight into a device operator.
- * pk_csr_read: a file (formats above) straight into a device operator. */
+ * pk_csr_read: a file (formats above) straight into a device operator;
+ * with a complex precision it also reads Matrix Market complex files
+ * (general / symmetric / skew-symmetric / Hermitian; a real file reads with
+ * zero imaginary parts).  pk_matrix_read and real precisions refuse complex
+ * files with PK_ERR_NOTSUP. */
 int pk_matrix_read(const char* path, int* m, int* n, long long* nnz, int** rows, int** cols,
                    double** vals);
 void pk_matrix_free(void* p);

@@ -543,9 +543,11 @@ def read_matrix(path):
 
 def coo_to_csr(m, n, rows, cols, vals, base=0):
     """CSR (synth.Csr) of coordinate entries, built on the device: sorted by
-    (row, column), duplicates summed in input order."""
+    (row, column), duplicates summed in input order (complex values: real
+    and imaginary parts each)."""
     vals = np.ascontiguousarray(vals)
-    prec = PK_F32 if vals.dtype == np.float32 else PK_F64
+    prec = prec_of(vals.dtype) if vals.dtype in (np.float32, np.complex64, np.complex128) \
+        else PK_F64
     vals = vals.astype(_dt(prec))
     rows = np.ascontiguousarray(rows, np.int32)
     cols = np.ascontiguousarray(cols, np.int32)

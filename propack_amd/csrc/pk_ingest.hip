@@ -29,6 +29,8 @@ struct Coo {
   int m = 0, n = 0;
   std::vector<int> r, c;  // 0-based
   std::vector<double> v;
+  bool cplx = false;      // a complex Matrix Market file: imaginary parts in vi
+  std::vector<double> vi;
 };
 
 std::string lower(std::string s) {
@@ -65,7 +67,8 @@ double parse_real(std::string s) {
   return std::strtod(s.c_str(), nullptr);
 }
 
-// Symmetric storage: add the mirrored entries (skew: negated).
+// Symmetric storage: add the mirrored entries (kind 1 symmetric, 2 skew:
+// negated, 3 Hermitian: conjugated).
 void expand_symmetry(Coo& A, int kind) {
   if (kind == 0) return;
   const size_t nz = A.r.size();
@@ -74,27 +77,24 @@ void expand_symmetry(Coo& A, int kind) {
     A.r.push_back(A.c[q]);
     A.c.push_back(A.r[q]);
     A.v.push_back(kind == 2 ? -A.v[q] : A.v[q]);
+    if (A.cplx) A.vi.push_back(kind == 2 ? -A.vi[q] : kind == 3 ? -A.vi[q] : A.vi[q]);
   }
 }
 
-// Matrix Market coordinate: real / integer / pattern, general / symmetric /
-// skew-symmetric.
+// Matrix Market coordinate: real / integer / pattern / complex, general /
+// symmetric / skew-symmetric / Hermitian.
 int parse_mm(const std::vector<std::string>& L, Coo& A) {
   const std::string h = lower(L[0]);
   if (h.find("coordinate") == std::string::npos) {
     set_error("Matrix Market: only the coordinate format is read");
     return PK_ERR_NOTSUP;
   }
-  if (h.find("complex") != std::string::npos) {
-    set_error("Matrix Market: complex matrices are not supported");
-    return PK_ERR_NOTSUP;
-  }
+  A.cplx = h.find("complex") != std::string::npos;
   const bool pattern = h.find("pattern") != std::string::npos;
   const int sym = h.find("skew-symmetric") != std::string::npos ? 2
-                  : (h.find("symmetric") != std::string::npos ||
-                     h.find("hermitian") != std::string::npos)
-                      ? 1
-                      : 0;
+                  : h.find("hermitian") != std::string::npos  ? (A.cplx ? 3 : 1)
+                  : h.find("symmetric") != std::string::npos  ? 1
+                                                              : 0;
   size_t i = 1;
   while (i < L.size() && (L[i].empty() || L[i][0] == '%')) ++i;
   if (i >= L.size()) return PK_ERR_ARG;
@@ -107,13 +107,30 @@ int parse_mm(const std::vector<std::string>& L, Coo& A) {
   A.r.reserve(nz);
   A.c.reserve(nz);
   A.v.reserve(nz);
+  if (A.cplx) A.vi.reserve(nz);
   for (++i; i < L.size() && static_cast<long long>(A.r.size()) < nz; ++i) {
     if (L[i].empty() || L[i][0] == '%') continue;
     const char* p = L[i].c_str();
     char* e;
+    char* s;
+    bool ok = true;
     const long ri = std::strtol(p, &e, 10);
-    const long ci = std::strtol(e, &e, 10);
-    const double v = pattern ? 1.0 : std::strtod(e, nullptr);
+    ok = ok && e != p;
+    const long ci = std::strtol(s = e, &e, 10);
+    ok = ok && e != s;
+    double v = 1.0;
+    if (!pattern) {
+      v = std::strtod(s = e, &e);
+      ok = ok && e != s;
+    }
+    if (A.cplx) {
+      A.vi.push_back(std::strtod(s = e, &e));
+      ok = ok && e != s;
+    }
+    if (!ok) {
+      set_error("Matrix Market: unreadable entry on line %zu", i + 1);
+      return PK_ERR_ARG;
+    }
     if (ri < 1 || ri > A.m || ci < 1 || ci > A.n) {
       set_error("Matrix Market: entry (%ld, %ld) outside %d x %d", ri, ci, A.m, A.n);
       return PK_ERR_ARG;
@@ -527,6 +544,11 @@ int pk_matrix_read(const char* path, int* m, int* n, long long* nnz, int** rows,
   Coo A;
   const int rc = read_any(path, A);
   if (rc != PK_OK) return rc;
+  if (A.cplx) {
+    set_error("pk_matrix_read: %s holds a complex matrix (pk_csr_read with PK_C128 or PK_C64 "
+              "reads it)", path);
+    return PK_ERR_NOTSUP;
+  }
   const size_t nz = A.r.size();
   *rows = static_cast<int*>(std::malloc(sizeof(int) * std::max<size_t>(nz, 1)));
   *cols = static_cast<int*>(std::malloc(sizeof(int) * std::max<size_t>(nz, 1)));
@@ -563,6 +585,12 @@ int pk_coo_to_csr(int m, int n, long long nnz, const int* rows, const int* cols,
   if (val_prec == PK_F32)
     return coo_to_csr_any<float>(m, n, nnz, rows, cols, vals, where, base, rowptr, colind,
                                  val_out, nnz_out);
+  if (val_prec == PK_C128)
+    return coo_to_csr_any<double2>(m, n, nnz, rows, cols, vals, where, base, rowptr, colind,
+                                   val_out, nnz_out);
+  if (val_prec == PK_C64)
+    return coo_to_csr_any<float2>(m, n, nnz, rows, cols, vals, where, base, rowptr, colind,
+                                  val_out, nnz_out);
   return PK_ERR_ARG;
 }
 
@@ -577,6 +605,10 @@ int pk_csr_create_coo(int m, int n, long long nnz, const int* rows, const int* c
     rc = create_from_coo<double>(m, n, nnz, rows, cols, vals, where, base, val_prec, &o);
   else if (val_prec == PK_F32)
     rc = create_from_coo<float>(m, n, nnz, rows, cols, vals, where, base, val_prec, &o);
+  else if (val_prec == PK_C128)
+    rc = create_from_coo<double2>(m, n, nnz, rows, cols, vals, where, base, val_prec, &o);
+  else if (val_prec == PK_C64)
+    rc = create_from_coo<float2>(m, n, nnz, rows, cols, vals, where, base, val_prec, &o);
   else
     return PK_ERR_ARG;
   if (rc == PK_OK) *op = reinterpret_cast<pk_csr_t>(o);
@@ -584,11 +616,30 @@ int pk_csr_create_coo(int m, int n, long long nnz, const int* rows, const int* c
 }
 
 int pk_csr_read(const char* path, int val_prec, pk_csr_t* op) {
-  if (!op || (val_prec != PK_F64 && val_prec != PK_F32)) return PK_ERR_ARG;
+  if (!op || (val_prec != PK_F64 && val_prec != PK_F32 && !prec_is_complex(val_prec)))
+    return PK_ERR_ARG;
   Coo A;
   int rc = read_any(path, A);
   if (rc != PK_OK) return rc;
   const long long nz = static_cast<long long>(A.r.size());
+  if (prec_is_complex(val_prec)) {
+    // (re, im) pairs; a real file reads with zero imaginary parts
+    std::vector<double> z(2 * static_cast<size_t>(nz));
+    for (long long q = 0; q < nz; ++q) {
+      z[2 * q] = A.v[q];
+      z[2 * q + 1] = A.cplx ? A.vi[q] : 0.0;
+    }
+    if (val_prec == PK_C128)
+      return pk_csr_create_coo(A.m, A.n, nz, A.r.data(), A.c.data(), z.data(), PK_C128, PK_HOST,
+                               0, op);
+    std::vector<float> z32(z.begin(), z.end());
+    return pk_csr_create_coo(A.m, A.n, nz, A.r.data(), A.c.data(), z32.data(), PK_C64, PK_HOST,
+                             0, op);
+  }
+  if (A.cplx) {
+    set_error("pk_csr_read: %s holds a complex matrix (read it with PK_C128 or PK_C64)", path);
+    return PK_ERR_NOTSUP;
+  }
   if (val_prec == PK_F64)
     return pk_csr_create_coo(A.m, A.n, nz, A.r.data(), A.c.data(), A.v.data(), PK_F64, PK_HOST,
                              0, op);
