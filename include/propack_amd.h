@@ -314,7 +314,8 @@ int pk_batch_lanbpro(pk_batch_t batch, const int* active, const int* k0, int* k,
  * sel[0..nsel) (0-based) of the batch in one grouped launch: problem
  * sel[q]'s basis `which` becomes X(:, 1:ncols[q]) = X(:, 1:kdim[q]) op(B_q),
  * B_q = bm[q] (host, leading dimension ldb[q]); the products are bit for bit
- * the per-engine update's.  Lanczos dimensions past 315 fall back to one
+ * the per-engine update's.  A problem may appear once in sel (PK_ERR_ARG
+ * otherwise, before anything runs).  kdim past 316 falls back to one
  * update per engine.  Returns when the products are done.  The batched
  * restarted driver issues its restart GEMMs this way. */
 int pk_batch_update(pk_batch_t batch, int nsel, const int* sel, char which, char transb,

@@ -925,12 +925,26 @@ int pk_batch_update(pk_batch_t h, int nsel, const int* sel, char which, char tra
   if (nsel < 0 || (nsel > 0 && (!sel || !kdim || !ncols || !bm || !ldb))) return PK_ERR_ARG;
   const bool isu = which == 'u' || which == 'U';
   const bool tb = transb == 't' || transb == 'T';
+  // each problem at most once: one problem's staging buffers (hB, dB) and
+  // spare basis hold one product at a time
+  std::vector<char> seen(b->nprob, 0);
+  for (int q = 0; q < nsel; ++q) {
+    const int i = sel[q];
+    if (i < 0 || i >= b->nprob) {
+      set_error("pk_batch_update: problem %d outside the batch", i);
+      return PK_ERR_ARG;
+    }
+    if (seen[i]) {
+      set_error("pk_batch_update: problem %d selected twice", i);
+      return PK_ERR_ARG;
+    }
+    seen[i] = 1;
+  }
   // the grouped kernel stages B in LDS (kdim <= 316); past that, one update
   // per engine (the long-kdim GEMM)
   for (int q = 0; q < nsel; ++q)
     if (kdim[q] > 316) {
       for (int r = 0; r < nsel; ++r) {
-        if (sel[r] < 0 || sel[r] >= b->nprob) return PK_ERR_ARG;
         const int rc = pk_lbd_update(reinterpret_cast<pk_lbd_t>(b->eng[sel[r]]), which, transb,
                                      kdim[r], ncols[r], bm[r], ldb[r]);
         if (rc != PK_OK) return rc;
@@ -941,10 +955,6 @@ int pk_batch_update(pk_batch_t h, int nsel, const int* sel, char which, char tra
   jobs.reserve(nsel);
   for (int q = 0; q < nsel; ++q) {
     const int i = sel[q];
-    if (i < 0 || i >= b->nprob) {
-      set_error("pk_batch_update: problem %d outside the batch", i);
-      return PK_ERR_ARG;
-    }
     Engine* e = b->eng[i];
     const int maxk = isu ? e->kmax + 1 : e->kmax;
     if (kdim[q] < 1 || ncols[q] < 1 || kdim[q] > maxk || ncols[q] > maxk ||

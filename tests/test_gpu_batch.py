@@ -313,4 +313,14 @@ def test_batch_update_matches_engine_update(prec):
             a = sets[0][i].get(which, nc)
             b = sets[1][i].get(which, nc)
             assert np.array_equal(a, b), (which, i)
+    # a problem selected twice is refused before anything runs
+    dup = np.array([2, 2], np.int32)
+    B = np.asfortranarray(rng.standard_normal((10, 10)))
+    ptrs = (C.c_void_p * 2)(B.ctypes.data, B.ctypes.data)
+    ten = np.full(2, 10, np.int32)
+    before = sets[1][2].get("v", 10)
+    rc = L.pk_batch_update(batch.handle, C.c_int(2), api._p(dup), C.c_char(b"v"), C.c_char(b"n"),
+                           api._p(ten), api._p(ten), ptrs, api._p(ten))
+    assert rc == -101 and b"selected twice" in L.pk_last_error()  # PK_ERR_ARG
+    assert np.array_equal(sets[1][2].get("v", 10), before)
     batch.close()
