@@ -187,8 +187,11 @@ __global__ __launch_bounds__(kBlock) void k_zspmv_cp(long long rows, int np, ZPa
 }
 
 // y = T(sum over panels, in panel order, of ypart - c yprev); partials
-// ||y||^2, Re / Im of yprev^H y.
-template <typename R>
+// ||y||^2, Re / Im of yprev^H y.  NP > 0: the panel count is known at
+// compile time and every thread's kZRPT x NP loads (clamped to row rows - 1;
+// a row past the end is loaded but neither stored nor counted) are issued
+// before the first add; NP = 0 loops over np.  Both add in the same order.
+template <typename R, int NP>
 __global__ __launch_bounds__(kBlock) void k_zcombine(long long rows, int np,
                                                      const double2* __restrict__ ypart,
                                                      const typename V2T<R>::type* __restrict__ yprev,
@@ -199,23 +202,49 @@ __global__ __launch_bounds__(kBlock) void k_zcombine(long long rows, int np,
   __shared__ double lds[(kBlock / kWave) * 3];
   const long long base = static_cast<long long>(blockIdx.x) * kZRows + threadIdx.x;
   double acc[3] = {0.0, 0.0, 0.0};
+  double2 sum[kZRPT] = {};
+  if (rows <= 0) {
+    // no rows: zero partials below
+  } else if constexpr (NP > 0) {
+    double2 t[kZRPT][NP];
+#pragma unroll
+    for (int r = 0; r < kZRPT; ++r) {
+      const long long row = min(base + static_cast<long long>(r) * kBlock, rows - 1);
+#pragma unroll
+      for (int q = 0; q < NP; ++q) t[r][q] = ypart[static_cast<long long>(q) * rows + row];
+    }
+#pragma unroll
+    for (int r = 0; r < kZRPT; ++r) {
+      sum[r] = t[r][0];
+#pragma unroll
+      for (int q = 1; q < NP; ++q) {
+        sum[r].x += t[r][q].x;
+        sum[r].y += t[r][q].y;
+      }
+    }
+  } else {
+#pragma unroll
+    for (int r = 0; r < kZRPT; ++r) {
+      const long long row = min(base + static_cast<long long>(r) * kBlock, rows - 1);
+      sum[r] = ypart[row];
+      for (int q = 1; q < np; ++q) {
+        const double2 t = ypart[static_cast<long long>(q) * rows + row];
+        sum[r].x += t.x;
+        sum[r].y += t.y;
+      }
+    }
+  }
 #pragma unroll
   for (int r = 0; r < kZRPT; ++r) {
     const long long row = base + static_cast<long long>(r) * kBlock;
     if (row < rows) {
-      double2 sum = ypart[row];
-      for (int q = 1; q < np; ++q) {
-        const double2 t = ypart[static_cast<long long>(q) * rows + row];
-        sum.x += t.x;
-        sum.y += t.y;
-      }
-      double yr = sum.x, yi = sum.y, pr = 0.0, pi = 0.0;
+      double yr = sum[r].x, yi = sum[r].y, pr = 0.0, pi = 0.0;
       if (yprev) {
         const C2 pv = yprev[row];
         pr = static_cast<double>(pv.x);
         pi = static_cast<double>(pv.y);
-        yr = sum.x - c * pr;
-        yi = sum.y - c * pi;
+        yr = sum[r].x - c * pr;
+        yi = sum[r].y - c * pi;
       }
       C2 out;
       out.x = static_cast<R>(yr);
@@ -514,8 +543,16 @@ static int launch_zspmv_cp(hipStream_t s, const CsrOp::Dir& D, int conj, const v
 #undef PK_ZSPMV_CP
   if (hipGetLastError() != hipSuccess) return -1;
   const int nblk = zstream_blocks(rows);
-  hipLaunchKernelGGL(k_zcombine<R>, dim3(nblk), dim3(kBlock), 0, s, rows, np, yp,
-                     static_cast<const C2*>(yprev), c, static_cast<C2*>(y), part);
+#define PK_ZCB(NP)                                                                         \
+  hipLaunchKernelGGL((k_zcombine<R, NP>), dim3(nblk), dim3(kBlock), 0, s, rows, np, yp,      \
+                     static_cast<const C2*>(yprev), c, static_cast<C2*>(y), part)
+  switch (np) {
+    case 2: PK_ZCB(2); break;
+    case 4: PK_ZCB(4); break;
+    case 8: PK_ZCB(8); break;
+    default: PK_ZCB(0); break;
+  }
+#undef PK_ZCB
   return hipGetLastError() == hipSuccess ? nblk : -1;
 }
 

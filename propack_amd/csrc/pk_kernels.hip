@@ -222,7 +222,12 @@ int launch_spmv_sorted_cp(hipStream_t s, const SortPanels* table, const int2* ma
   return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 
-template <typename T>
+// y = (sum of the np panels' row sums, added in panel order) - c yprev and
+// the partials of ||y||^2 and yprev.y.  NP > 0: the panel count is known at
+// compile time, so every thread's kRPT x NP loads (clamped to row L - 1; a
+// row past L is loaded but neither stored nor counted) are issued before the
+// first add; NP = 0 loops over np.  Both forms add in the same order.
+template <typename T, int NP>
 __global__ __launch_bounds__(kBlock) void k_spmv_combine(long long L, int np,
                                                          const double* __restrict__ ypart,
                                                          const T* __restrict__ yprev,
@@ -234,14 +239,37 @@ __global__ __launch_bounds__(kBlock) void k_spmv_combine(long long L, int np,
   const double c = coef.get();
   const long long base = static_cast<long long>(blockIdx.x) * kRowsPerBlock + threadIdx.x;
   double v[2] = {0.0, 0.0};
+  double acc[kRPT] = {};
+  if (L <= 0) {
+    // no rows: zero partials below
+  } else if constexpr (NP > 0) {
+    double t[kRPT][NP];
+#pragma unroll
+    for (int r = 0; r < kRPT; ++r) {
+      const long long row = min(base + static_cast<long long>(r) * kBlock, L - 1);
+#pragma unroll
+      for (int p = 0; p < NP; ++p) t[r][p] = ypart[static_cast<long long>(p) * L + row];
+    }
+#pragma unroll
+    for (int r = 0; r < kRPT; ++r) {
+      acc[r] = t[r][0];
+#pragma unroll
+      for (int p = 1; p < NP; ++p) acc[r] = acc[r] + t[r][p];
+    }
+  } else {
+#pragma unroll
+    for (int r = 0; r < kRPT; ++r) {
+      const long long row = min(base + static_cast<long long>(r) * kBlock, L - 1);
+      acc[r] = ypart[row];
+      for (int p = 1; p < np; ++p) acc[r] = acc[r] + ypart[static_cast<long long>(p) * L + row];
+    }
+  }
 #pragma unroll
   for (int r = 0; r < kRPT; ++r) {
     const long long row = base + static_cast<long long>(r) * kBlock;
     if (row < L) {
-      double acc = ypart[row];
-      for (int p = 1; p < np; ++p) acc = acc + ypart[static_cast<long long>(p) * L + row];
       const double yp = yprev ? static_cast<double>(yprev[row]) : 0.0;
-      const T yv = static_cast<T>(acc - c * yp);
+      const T yv = static_cast<T>(acc[r] - c * yp);
       y[row] = yv;
       v[0] += static_cast<double>(yv) * static_cast<double>(yv);
       v[1] += yp * static_cast<double>(yv);
@@ -257,9 +285,18 @@ __global__ __launch_bounds__(kBlock) void k_spmv_combine(long long L, int np,
 template <typename T>
 int launch_spmv_combine(hipStream_t s, long long rows, int np, const double* ypart,
                         const T* yprev, Scal coef, T* y, double* part, Gate gate) {
+  if (np < 1) return -1;
   const int nblk = stream_blocks(rows);
-  hipLaunchKernelGGL(k_spmv_combine<T>, dim3(nblk), dim3(kBlock), 0, s, rows, np, ypart, yprev,
-                     coef, y, part, gate);
+#define PK_CB_CASE(NP)                                                                     \
+  hipLaunchKernelGGL((k_spmv_combine<T, NP>), dim3(nblk), dim3(kBlock), 0, s, rows, np, ypart, \
+                     yprev, coef, y, part, gate)
+  switch (np) {
+    case 2: PK_CB_CASE(2); break;
+    case 4: PK_CB_CASE(4); break;
+    case 8: PK_CB_CASE(8); break;
+    default: PK_CB_CASE(0); break;
+  }
+#undef PK_CB_CASE
   return hipGetLastError() == hipSuccess ? nblk : -1;
 }
 

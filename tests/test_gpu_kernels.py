@@ -278,7 +278,10 @@ def test_autotune_and_explicit_panels():
     (pa, pt), (sa, st), (ua, ut), (ca, ct) = t.panels, t.lanes, t.us, t.sort
     assert pa in (1, 2, 4, 8, 16) and pt in (1, 2, 4, 8, 16) and ua > 0 and ut > 0
     assert sa in (-2, -1, 0, 1) and st in (-2, -1, 0, 1)
-    assert ca in (0, 8192, 16384) and ct in (0, 8192, 16384)
+    # row-block 0, row-wise -1, sorted 8192 / 16384 panel by panel, or negated:
+    # every panel in one launch
+    caps = (0, -1, 8192, 16384, -8192, -16384)
+    assert ca in caps and ct in caps
     t0 = api.autotune(A, sort=False)
     assert t0.sort == (0, 0) and t0.panels[0] in (1, 2, 4, 8) and t0.panels[1] in (1, 2, 4, 8)
     assert t0.lanes[0] in (-1, 0, 1) and t0.lanes[1] in (-1, 0, 1)
@@ -290,7 +293,10 @@ def test_autotune_and_explicit_panels():
         op = api.CsrOperator(A, panels=panels, lanes=lanes, sort=sort)
         assert op.panels() == tuple(min(p, 16) for p in panels)
         assert op.lanes() == (lanes or (0, 0))
-        assert op.sort() == (sort or (0, 0))
+        # a one-panel direction runs a negated cap panel by panel and reports it plain
+        want = tuple(-c if c < -1 and p == 1 else c
+                     for c, p in zip(sort or (0, 0), op.panels()))
+        assert op.sort() == want
         np.testing.assert_allclose(op.spmv(x, "n"), want_n, rtol=0,
                                    atol=1e-12 * np.abs(want_n).max())
         np.testing.assert_allclose(op.spmv(y, "t"), want_t, rtol=0,
