@@ -55,8 +55,8 @@ def test_headline_workload_properties():
 
 
 def test_powerlaw_workload_properties():
-    """The 8,000,000 x 4,000,000 power-law workload (14.4M nonzeros after
-    merging duplicate draws), k=64, dim=160, tol=1e-8: the FP64 solve and the
+    """The 8,000,000 x 4,000,000 power-law workload (37.8M nonzeros: each
+    row's columns distinct, values 1.0), k=64, dim=160, tol=1e-8: the FP64 solve and the
     FP32-basis form with its FP64 polish.  The polished triplets' residuals
     within tol * sigma_1 (A v and A^T u formed on the host, the second within
     10 tol sigma_1), U and V orthonormal to 5 sqrt(2 eps), the polished sigma
