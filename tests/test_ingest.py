@@ -175,18 +175,23 @@ def test_complex_coo_to_csr_matches_scipy():
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("kind", ["general", "hermitian"])
+@pytest.mark.parametrize("kind", ["general", "hermitian", "skew-symmetric"])
 def test_complex_matrix_market_to_operator_solves(tmp_path, kind):
-    """A complex Matrix Market file (general, or Hermitian with only the
-    lower triangle stored) read straight into a complex operator; ZLANSVD's
-    singular values against the dense SVD.  Real precisions and
-    pk_matrix_read refuse the file."""
+    """A complex Matrix Market file (general; Hermitian or skew-symmetric
+    with only the lower triangle stored, the upper one mirrored conjugated
+    or negated) read straight into a complex operator; ZLANSVD's singular
+    values against the dense SVD.  Real precisions and pk_matrix_read
+    refuse the file."""
     rng = np.random.default_rng(12)
     D = (rng.standard_normal((150, 150)) + 1j * rng.standard_normal((150, 150)))
     D[rng.random((150, 150)) > 0.1] = 0
     if kind == "hermitian":
         D = np.tril(D, -1) + np.diag(rng.standard_normal(150))
         D = D + np.tril(D, -1).conj().T
+        stored = sp.coo_matrix(np.tril(D))
+    elif kind == "skew-symmetric":
+        D = np.tril(D, -1)
+        D = D - D.T
         stored = sp.coo_matrix(np.tril(D))
     else:
         stored = sp.coo_matrix(D)
