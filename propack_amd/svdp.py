@@ -9,7 +9,8 @@ called from the host as the APROD callback (matvec / rmatvec), the rest of
 the Lanczos process still runs on the GPU.  float32 inputs use SLANSVD /
 SLANSVD_IRL, complex128 ZLANSVD / ZLANSVD_IRL, complex64 CLANSVD /
 CLANSVD_IRL, everything else DLANSVD / DLANSVD_IRL; for complex inputs the
-returned ``vt`` is V^H.
+returned ``vt`` is V^H.  ``svds`` is ``scipy.sparse.linalg.svds`` with
+``solver='propack'`` over the same drivers.
 """
 from __future__ import annotations
 
@@ -133,3 +134,62 @@ def svdp(A, k, which="LM", irl_mode=True, kmax=None, compute_u=True, compute_v=T
     if full_output:
         return u, r.sigma, vt, r.bnd
     return u, r.sigma, vt
+
+
+def svds(A, k=6, ncv=None, tol=0, which="LM", v0=None, maxiter=None,
+         return_singular_vectors=True, solver="propack", rng=None, options=None):
+    """``scipy.sparse.linalg.svds`` with ``solver='propack'`` on the MI355X
+    library: the arguments and results of scipy's propack branch
+    (scipy/sparse/linalg/_eigen/_svds.py, ``svds`` and ``_iv``) -- ``tol``
+    squared for the Lanczos solve, the restarted driver only for
+    ``which='SM'``, ``maxiter`` as the Lanczos dimension, singular values in
+    ascending order (u's columns and vh's rows with them).  Explicit
+    matrices stay on the device (scipy wraps them in a LinearOperator); a
+    LinearOperator is called from the host.  ``ncv`` and ``options`` are
+    accepted and ignored, as scipy's propack branch does."""
+    if str(solver).lower() != "propack":
+        raise ValueError("solver must be 'propack'")
+    shape = getattr(A, "shape", None)
+    if shape is None:
+        A = np.asarray(A)
+        shape = A.shape
+    if len(shape) != 2:
+        raise ValueError("A must be two-dimensional")
+    dtype = np.dtype(getattr(A, "dtype", np.float64))
+    if not np.issubdtype(dtype, np.number):
+        raise ValueError("`A` must be of numeric data type.")
+    if shape[0] * shape[1] == 0:
+        raise ValueError("`A` must not be empty.")
+    if int(k) != k or not (0 < k <= min(shape)):
+        raise ValueError("`k` must be an integer satisfying `0 < k < min(A.shape)`.")
+    k = int(k)
+    if tol < 0 or not np.isfinite(tol):
+        raise ValueError("`tol` must be a non-negative floating point value.")
+    which = str(which).upper()
+    if which not in ("LM", "SM"):
+        raise ValueError("`which` must be in {'LM', 'SM'}.")
+    if v0 is not None:
+        v0 = np.atleast_1d(v0)
+        if not (np.issubdtype(v0.dtype, np.complexfloating)
+                or np.issubdtype(v0.dtype, np.floating)):
+            raise ValueError("`v0` must be of floating or complex floating data type.")
+        if v0.shape != (shape[0],):
+            raise ValueError(f"`v0` must have shape {(shape[0],)}.")
+    if maxiter is not None and (int(maxiter) != maxiter or maxiter <= 0):
+        raise ValueError("`maxiter` must be a positive integer.")
+    if return_singular_vectors not in (True, False, "vh", "u"):
+        raise ValueError("`return_singular_vectors` must be in {True, False, 'vh', 'u'}.")
+    jobu = return_singular_vectors in (True, "u")
+    jobv = return_singular_vectors in (True, "vh")
+    if rng is not None and not isinstance(rng, np.random.Generator):
+        rng = np.random.default_rng(rng)
+    u, s, vh, _ = svdp(A, k=k, tol=float(tol) ** 2, which=which, maxiter=None,
+                       compute_u=jobu, compute_v=jobv, irl_mode=(which == "SM"),
+                       kmax=None if maxiter is None else int(maxiter), v0=v0,
+                       full_output=True, rng=rng)
+    s = s[::-1]
+    u = u[:, ::-1] if jobu else None
+    vh = vh[::-1] if jobv else None
+    if return_singular_vectors:
+        return u, s, vh
+    return s

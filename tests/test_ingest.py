@@ -208,6 +208,10 @@ def test_complex_matrix_market_to_operator_solves(tmp_path, kind):
     u0 = rng.standard_normal(150) + 1j * rng.standard_normal(150)
     r = api.lansvd(op, 150, 150, 5, 100, u0=u0, tol=1e-12, prec=api.PK_C128)
     s = np.linalg.svd(D, compute_uv=False)
+    if kind == "skew-symmetric":
+        # a skew-symmetric matrix's singular values come in equal pairs; a
+        # Lanczos process from one start vector meets each distinct value once
+        s = s[np.r_[True, np.diff(s) < -1e-9 * s[0]]]
     np.testing.assert_allclose(r.sigma, s[:5], rtol=1e-12)
     with pytest.raises(api.PropackError, match="complex"):
         api.CsrOperator.from_file(f, prec=api.PK_F64)
