@@ -542,9 +542,13 @@ __global__ __launch_bounds__(kWave * kMfmaWaves) void k_gemm_mfma(
 // than a contiguous range of blocks per workgroup).  Each wave owns
 // kGemmRowTiles 16-row tiles and all NT column tiles, so one B operand read
 // from LDS feeds kGemmRowTiles MFMAs; the A operands of the next two k-steps
-// are in flight while the current one multiplies (measured on 2,000,000 x
-// 121 times 121 x 51: 0.68 ms; three steps in flight 0.75 ms, two row tiles
-// per wave with six waves 0.98 ms, 16-byte loads of row pairs 0.87 ms).  Rows past L load row
+// are in flight while the current one multiplies, in three register stages
+// of a loop unrolled by three (measured on 2,000,000 x 121 times 121 x 51:
+// 0.63 ms; with the stages rotated by register copies, which made every
+// k-step wait for all its loads, 0.67 ms; four stages 0.62 ms, within the
+// spread between boxes; two row tiles per wave with six waves 0.98 ms, 16-byte
+// loads of row pairs 0.87 ms).  The launch bound of two waves per SIMD keeps
+// the accumulators and stages within 256 registers.  Rows past L load row
 // L - 1 and are not stored; k past kdim loads column kdim - 1, which meets
 // zero rows of the staged B.
 constexpr int kGemmRowTiles = 4;
@@ -553,7 +557,7 @@ constexpr int kGemmRows = 16 * kGemmRowTiles * kGemmWaves;
 constexpr int kGemmMaxKpad = 317;  // 64 columns x 317 x 8 bytes fit the 160 KiB of LDS
 
 template <typename T, int NT>
-__global__ __launch_bounds__(kWave * kGemmWaves) void k_gemm_lds(
+__global__ __launch_bounds__(kWave * kGemmWaves, 2) void k_gemm_lds(
     long long L, const T* __restrict__ A, long long lda, int kdim, int kpad,
     const double* __restrict__ B, int ldb, int ncols, T* __restrict__ C, long long ldc,
     long long nblocks) {
@@ -587,10 +591,7 @@ __global__ __launch_bounds__(kWave * kGemmWaves) void k_gemm_lds(
 #pragma unroll
       for (int rt = 0; rt < kGemmRowTiles; ++rt) a[rt] = static_cast<double>(A[k * lda + rowc[rt]]);
     };
-    load(0, a0);
-    load(1, a1);
-    for (int ks = 0; ks < nks; ++ks) {
-      load(ks + 2, a2);
+    auto step = [&](int ks, const double (&a)[kGemmRowTiles]) {
       const int k = ks * 4 + lk;
       double b[NT];
 #pragma unroll
@@ -599,13 +600,25 @@ __global__ __launch_bounds__(kWave * kGemmWaves) void k_gemm_lds(
       for (int rt = 0; rt < kGemmRowTiles; ++rt)
 #pragma unroll
         for (int ct = 0; ct < NT; ++ct)
-          acc[rt][ct] = __builtin_amdgcn_mfma_f64_16x16x4f64(b[ct], a0[rt], acc[rt][ct], 0, 0, 0);
-#pragma unroll
-      for (int rt = 0; rt < kGemmRowTiles; ++rt) {
-        a0[rt] = a1[rt];
-        a1[rt] = a2[rt];
-      }
+          acc[rt][ct] = __builtin_amdgcn_mfma_f64_16x16x4f64(b[ct], a[rt], acc[rt][ct], 0, 0, 0);
+    };
+    // Three register stages, the loop unrolled by three so that each stage
+    // keeps its registers: rotating the operands by copies made the compiler
+    // wait for every load in flight at the end of each k-step.  Loop entry:
+    // a0 holds step ks, a1 step ks + 1.
+    load(0, a0);
+    load(1, a1);
+    int ks = 0;
+    for (; ks + 3 <= nks; ks += 3) {
+      load(ks + 2, a2);
+      step(ks, a0);
+      load(ks + 3, a0);
+      step(ks + 1, a1);
+      load(ks + 4, a1);
+      step(ks + 2, a2);
     }
+    if (ks < nks) step(ks, a0);
+    if (ks + 1 < nks) step(ks + 1, a1);
 #pragma unroll
     for (int rt = 0; rt < kGemmRowTiles; ++rt) {
       const long long row = i0 + rt * 16 + li;
@@ -629,7 +642,7 @@ __global__ __launch_bounds__(kWave * kGemmWaves) void k_gemm_lds(
 // changed k_gemm_lds's register allocation and slowed it by a fifth,
 // measured on the 2M x 121 * 121 x 51 shape).
 template <typename T, int NT>
-__global__ __launch_bounds__(kWave * kGemmWaves) void k_gemm_lds_group(
+__global__ __launch_bounds__(kWave * kGemmWaves, 2) void k_gemm_lds_group(
     const GemmItem* __restrict__ items) {
   const GemmItem it = items[blockIdx.x];
   const long long L = it.L, lda = it.lda, ldc = it.ldc;
@@ -667,10 +680,7 @@ __global__ __launch_bounds__(kWave * kGemmWaves) void k_gemm_lds_group(
 #pragma unroll
       for (int rt = 0; rt < kGemmRowTiles; ++rt) a[rt] = static_cast<double>(A[k * lda + rowc[rt]]);
     };
-    load(0, a0);
-    load(1, a1);
-    for (int ks = 0; ks < nks; ++ks) {
-      load(ks + 2, a2);
+    auto step = [&](int ks, const double (&a)[kGemmRowTiles]) {
       const int k = ks * 4 + lk;
       double b[NT];
 #pragma unroll
@@ -679,13 +689,25 @@ __global__ __launch_bounds__(kWave * kGemmWaves) void k_gemm_lds_group(
       for (int rt = 0; rt < kGemmRowTiles; ++rt)
 #pragma unroll
         for (int ct = 0; ct < NT; ++ct)
-          acc[rt][ct] = __builtin_amdgcn_mfma_f64_16x16x4f64(b[ct], a0[rt], acc[rt][ct], 0, 0, 0);
-#pragma unroll
-      for (int rt = 0; rt < kGemmRowTiles; ++rt) {
-        a0[rt] = a1[rt];
-        a1[rt] = a2[rt];
-      }
+          acc[rt][ct] = __builtin_amdgcn_mfma_f64_16x16x4f64(b[ct], a[rt], acc[rt][ct], 0, 0, 0);
+    };
+    // Three register stages, the loop unrolled by three so that each stage
+    // keeps its registers: rotating the operands by copies made the compiler
+    // wait for every load in flight at the end of each k-step.  Loop entry:
+    // a0 holds step ks, a1 step ks + 1.
+    load(0, a0);
+    load(1, a1);
+    int ks = 0;
+    for (; ks + 3 <= nks; ks += 3) {
+      load(ks + 2, a2);
+      step(ks, a0);
+      load(ks + 3, a0);
+      step(ks + 1, a1);
+      load(ks + 4, a1);
+      step(ks + 2, a2);
     }
+    if (ks < nks) step(ks, a0);
+    if (ks + 1 < nks) step(ks + 1, a1);
 #pragma unroll
     for (int rt = 0; rt < kGemmRowTiles; ++rt) {
       const long long row = i0 + rt * 16 + li;
