@@ -543,10 +543,11 @@ __global__ __launch_bounds__(kWave * kMfmaWaves) void k_gemm_mfma(
 // kGemmRowTiles 16-row tiles and all NT column tiles, so one B operand read
 // from LDS feeds kGemmRowTiles MFMAs; the A operands of the next two k-steps
 // are in flight while the current one multiplies, in three register stages
-// of a loop unrolled by three (measured on 2,000,000 x 121 times 121 x 51:
-// 0.63 ms; with the stages rotated by register copies, which made every
-// k-step wait for all its loads, 0.67 ms; four stages 0.62 ms, within the
-// spread between boxes; two row tiles per wave with six waves 0.98 ms, 16-byte
+// of a loop unrolled by three, and the next row block's first two k-steps
+// load while a block's C is stored (measured on 2,000,000 x 121 times 121 x
+// 51: 0.62 ms; without the next-block loads 0.63 ms; with the stages rotated
+// by register copies, which made every k-step wait for all its loads, 0.67
+// ms; four stages 0.62 ms, within the spread between boxes; two row tiles per wave with six waves 0.98 ms, 16-byte
 // loads of row pairs 0.87 ms).  The launch bound of two waves per SIMD keeps
 // the accumulators and stages within 256 registers.  Rows past L load row
 // L - 1 and are not stored; k past kdim loads column kdim - 1, which meets
@@ -570,27 +571,35 @@ __global__ __launch_bounds__(kWave * kGemmWaves, 2) void k_gemm_lds(
   }
   __syncthreads();
   const int nks = (kdim + 3) / 4;
-  for (long long blk = blockIdx.x; blk < nblocks; blk += gridDim.x) {
-    const long long i0 = blk * kGemmRows + static_cast<long long>(wid) * (16 * kGemmRowTiles);
-    if (i0 >= L) continue;  // wave-uniform
-    long long rowc[kGemmRowTiles];
+  const long long wrow = static_cast<long long>(wid) * (16 * kGemmRowTiles);
+  // this wave's blocks: static_cast<long long>(blockIdx.x), + static_cast<long long>(gridDim.x), ... while blk < nblocks and the wave's
+  // first row is inside L (the first rows grow with blk: a prefix)
+  long long blk = static_cast<long long>(blockIdx.x);
+  if (blk >= nblocks || blk * kGemmRows + wrow >= L) return;
+  auto rows = [&](long long bk, long long (&rc)[kGemmRowTiles]) {
 #pragma unroll
     for (int rt = 0; rt < kGemmRowTiles; ++rt) {
-      const long long row = i0 + rt * 16 + li;
-      rowc[rt] = row < L ? row : L - 1;
+      const long long row = bk * kGemmRows + wrow + rt * 16 + li;
+      rc[rt] = row < L ? row : L - 1;
     }
+  };
+  long long rowc[kGemmRowTiles];
+  rows(blk, rowc);
+  double a0[kGemmRowTiles], a1[kGemmRowTiles], a2[kGemmRowTiles];
+  auto load = [&](int ks, const long long (&rc)[kGemmRowTiles], double (&a)[kGemmRowTiles]) {
+    // clamped: the loads stay in bounds, and a step past the end is never used
+    const long long k = min(ks * 4 + lk, kdim - 1);
+#pragma unroll
+    for (int rt = 0; rt < kGemmRowTiles; ++rt) a[rt] = static_cast<double>(A[k * lda + rc[rt]]);
+  };
+  load(0, rowc, a0);
+  load(1, rowc, a1);
+  for (;;) {
     v4d acc[kGemmRowTiles][NT];
 #pragma unroll
     for (int rt = 0; rt < kGemmRowTiles; ++rt)
 #pragma unroll
       for (int ct = 0; ct < NT; ++ct) acc[rt][ct] = v4d{0.0, 0.0, 0.0, 0.0};
-    double a0[kGemmRowTiles], a1[kGemmRowTiles], a2[kGemmRowTiles];
-    auto load = [&](int ks, double (&a)[kGemmRowTiles]) {
-      // clamped: the loads stay in bounds, and a step past the end is never used
-      const long long k = min(ks * 4 + lk, kdim - 1);
-#pragma unroll
-      for (int rt = 0; rt < kGemmRowTiles; ++rt) a[rt] = static_cast<double>(A[k * lda + rowc[rt]]);
-    };
     auto step = [&](int ks, const double (&a)[kGemmRowTiles]) {
       const int k = ks * 4 + lk;
       double b[NT];
@@ -606,19 +615,27 @@ __global__ __launch_bounds__(kWave * kGemmWaves, 2) void k_gemm_lds(
     // keeps its registers: rotating the operands by copies made the compiler
     // wait for every load in flight at the end of each k-step.  Loop entry:
     // a0 holds step ks, a1 step ks + 1.
-    load(0, a0);
-    load(1, a1);
     int ks = 0;
     for (; ks + 3 <= nks; ks += 3) {
-      load(ks + 2, a2);
+      load(ks + 2, rowc, a2);
       step(ks, a0);
-      load(ks + 3, a0);
+      load(ks + 3, rowc, a0);
       step(ks + 1, a1);
-      load(ks + 4, a1);
+      load(ks + 4, rowc, a1);
       step(ks + 2, a2);
     }
     if (ks < nks) step(ks, a0);
     if (ks + 1 < nks) step(ks + 1, a1);
+    // the next block's first two steps load while this block's C is stored
+    const long long nb = blk + static_cast<long long>(gridDim.x);
+    const bool more = nb < nblocks && nb * kGemmRows + wrow < L;
+    long long rown[kGemmRowTiles];
+    if (more) {
+      rows(nb, rown);
+      load(0, rown, a0);
+      load(1, rown, a1);
+    }
+    const long long i0 = blk * kGemmRows + wrow;
 #pragma unroll
     for (int rt = 0; rt < kGemmRowTiles; ++rt) {
       const long long row = i0 + rt * 16 + li;
@@ -631,6 +648,10 @@ __global__ __launch_bounds__(kWave * kGemmWaves, 2) void k_gemm_lds(
           if (j < ncols) C[static_cast<long long>(j) * ldc + row] = static_cast<T>(acc[rt][ct][r]);
         }
     }
+    if (!more) break;
+    blk = nb;
+#pragma unroll
+    for (int rt = 0; rt < kGemmRowTiles; ++rt) rowc[rt] = rown[rt];
   }
 }
 
@@ -659,27 +680,35 @@ __global__ __launch_bounds__(kWave * kGemmWaves, 2) void k_gemm_lds_group(
   }
   __syncthreads();
   const int nks = (kdim + 3) / 4;
-  for (long long blk = it.blk0; blk < it.blk1; ++blk) {
-    const long long i0 = blk * kGemmRows + static_cast<long long>(wid) * (16 * kGemmRowTiles);
-    if (i0 >= L) continue;  // wave-uniform
-    long long rowc[kGemmRowTiles];
+  const long long wrow = static_cast<long long>(wid) * (16 * kGemmRowTiles);
+  // this wave's blocks: it.blk0, + 1, ... while blk < it.blk1 and the wave's
+  // first row is inside L (the first rows grow with blk: a prefix)
+  long long blk = it.blk0;
+  if (blk >= it.blk1 || blk * kGemmRows + wrow >= L) return;
+  auto rows = [&](long long bk, long long (&rc)[kGemmRowTiles]) {
 #pragma unroll
     for (int rt = 0; rt < kGemmRowTiles; ++rt) {
-      const long long row = i0 + rt * 16 + li;
-      rowc[rt] = row < L ? row : L - 1;
+      const long long row = bk * kGemmRows + wrow + rt * 16 + li;
+      rc[rt] = row < L ? row : L - 1;
     }
+  };
+  long long rowc[kGemmRowTiles];
+  rows(blk, rowc);
+  double a0[kGemmRowTiles], a1[kGemmRowTiles], a2[kGemmRowTiles];
+  auto load = [&](int ks, const long long (&rc)[kGemmRowTiles], double (&a)[kGemmRowTiles]) {
+    // clamped: the loads stay in bounds, and a step past the end is never used
+    const long long k = min(ks * 4 + lk, kdim - 1);
+#pragma unroll
+    for (int rt = 0; rt < kGemmRowTiles; ++rt) a[rt] = static_cast<double>(A[k * lda + rc[rt]]);
+  };
+  load(0, rowc, a0);
+  load(1, rowc, a1);
+  for (;;) {
     v4d acc[kGemmRowTiles][NT];
 #pragma unroll
     for (int rt = 0; rt < kGemmRowTiles; ++rt)
 #pragma unroll
       for (int ct = 0; ct < NT; ++ct) acc[rt][ct] = v4d{0.0, 0.0, 0.0, 0.0};
-    double a0[kGemmRowTiles], a1[kGemmRowTiles], a2[kGemmRowTiles];
-    auto load = [&](int ks, double (&a)[kGemmRowTiles]) {
-      // clamped: the loads stay in bounds, and a step past the end is never used
-      const long long k = min(ks * 4 + lk, kdim - 1);
-#pragma unroll
-      for (int rt = 0; rt < kGemmRowTiles; ++rt) a[rt] = static_cast<double>(A[k * lda + rowc[rt]]);
-    };
     auto step = [&](int ks, const double (&a)[kGemmRowTiles]) {
       const int k = ks * 4 + lk;
       double b[NT];
@@ -695,19 +724,27 @@ __global__ __launch_bounds__(kWave * kGemmWaves, 2) void k_gemm_lds_group(
     // keeps its registers: rotating the operands by copies made the compiler
     // wait for every load in flight at the end of each k-step.  Loop entry:
     // a0 holds step ks, a1 step ks + 1.
-    load(0, a0);
-    load(1, a1);
     int ks = 0;
     for (; ks + 3 <= nks; ks += 3) {
-      load(ks + 2, a2);
+      load(ks + 2, rowc, a2);
       step(ks, a0);
-      load(ks + 3, a0);
+      load(ks + 3, rowc, a0);
       step(ks + 1, a1);
-      load(ks + 4, a1);
+      load(ks + 4, rowc, a1);
       step(ks + 2, a2);
     }
     if (ks < nks) step(ks, a0);
     if (ks + 1 < nks) step(ks + 1, a1);
+    // the next block's first two steps load while this block's C is stored
+    const long long nb = blk + 1;
+    const bool more = nb < it.blk1 && nb * kGemmRows + wrow < L;
+    long long rown[kGemmRowTiles];
+    if (more) {
+      rows(nb, rown);
+      load(0, rown, a0);
+      load(1, rown, a1);
+    }
+    const long long i0 = blk * kGemmRows + wrow;
 #pragma unroll
     for (int rt = 0; rt < kGemmRowTiles; ++rt) {
       const long long row = i0 + rt * 16 + li;
@@ -720,6 +757,10 @@ __global__ __launch_bounds__(kWave * kGemmWaves, 2) void k_gemm_lds_group(
           if (j < ncols) C[static_cast<long long>(j) * ldc + row] = static_cast<T>(acc[rt][ct][r]);
         }
     }
+    if (!more) break;
+    blk = nb;
+#pragma unroll
+    for (int rt = 0; rt < kGemmRowTiles; ++rt) rowc[rt] = rown[rt];
   }
 }
 
