@@ -390,31 +390,27 @@ __device__ __forceinline__ void gemvt_body(long long b, long long L, const T* __
   }
   const int lane = threadIdx.x % kWave, wid = threadIdx.x / kWave;
   int buf = 0;
-  // software pipeline: the loads of the next column tile are in flight
-  // while the current tile is reduced
-  T cur[kCT][kCgsRPT], nxt[kCT][kCgsRPT];
+  // Software pipeline over column tiles in two register buffers, t0 and t1,
+  // with no copy between them (a rotation by copies made the compiler wait
+  // for the next tile's loads before reducing the current one): the loads of
+  // the next tile are in flight while the current tile is reduced.  Columns
+  // past ncols load the last column and are not written.
+  T t0[kCT][kCgsRPT], t1[kCT][kCgsRPT];
+  auto load = [&](int c0, T (&t)[kCT][kCgsRPT]) {
 #pragma unroll
-  for (int c = 0; c < kCT; ++c) {
-    const T* vc = V + static_cast<long long>(scol[c < ncols ? c : ncols - 1]) * ldv;
+    for (int c = 0; c < kCT; ++c) {
+      const T* vc = V + static_cast<long long>(scol[c0 + c < ncols ? c0 + c : ncols - 1]) * ldv;
 #pragma unroll
-    for (int r = 0; r < kCgsRPT; ++r) cur[c][r] = vc[rowc[r]];
-  }
-  for (int c0 = 0; c0 < ncols; c0 += kCT) {
-    const int n0 = c0 + kCT;
-    if (n0 < ncols) {
-#pragma unroll
-      for (int c = 0; c < kCT; ++c) {
-        const T* vc = V + static_cast<long long>(scol[n0 + c < ncols ? n0 + c : ncols - 1]) * ldv;
-#pragma unroll
-        for (int r = 0; r < kCgsRPT; ++r) nxt[c][r] = vc[rowc[r]];
-      }
+      for (int r = 0; r < kCgsRPT; ++r) t[c][r] = vc[rowc[r]];
     }
+  };
+  auto reduce = [&](int c0, const T (&t)[kCT][kCgsRPT]) {
     double acc[kCT];
 #pragma unroll
     for (int c = 0; c < kCT; ++c) {
       acc[c] = 0.0;
 #pragma unroll
-      for (int r = 0; r < kCgsRPT; ++r) acc[c] += static_cast<double>(cur[c][r]) * wr[r];
+      for (int r = 0; r < kCgsRPT; ++r) acc[c] += static_cast<double>(t[c][r]) * wr[r];
     }
     const double colsum = wave_sum4(acc, lane);
     if ((lane & 15) == 0) lds[buf][wid * kCT + (lane >> 4)] = colsum;
@@ -425,11 +421,23 @@ __device__ __forceinline__ void gemvt_body(long long b, long long L, const T* __
       part[static_cast<long long>(c0 + threadIdx.x) * stride + slot] = sum;
     }
     buf ^= 1;
-#pragma unroll
-    for (int c = 0; c < kCT; ++c)
-#pragma unroll
-      for (int r = 0; r < kCgsRPT; ++r) cur[c][r] = nxt[c][r];
+  };
+  // loop entry: t0 holds tile c0 (c0 < ncols)
+  load(0, t0);
+  int c0 = 0;
+  while (c0 + kCT < ncols) {
+    load(c0 + kCT, t1);
+    reduce(c0, t0);
+    if (c0 + 2 * kCT >= ncols) {
+      reduce(c0 + kCT, t1);
+      c0 = ncols;
+      break;
+    }
+    load(c0 + 2 * kCT, t0);
+    reduce(c0 + kCT, t1);
+    c0 += 2 * kCT;
   }
+  if (c0 < ncols) reduce(c0, t0);
 }
 
 // Slice b (kCgsRows rows) of w -= V(:, scol[0:ncols]) sh (scol, sh global,
